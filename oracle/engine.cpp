@@ -1,0 +1,1057 @@
+// CPU ORACLE (test infrastructure): the search pipeline.
+// Restates rmblast-lib/src/search/engine.rs:383-572 (improve_seed),
+// :734-790 (search_one_subject), :863-1017 (mask level, output order),
+// :1063-1309 (seeding + ungapped + sort), :1317-1762 (Phase 2a/2b/2c),
+// :1850-1895 (pivots), :1969-2083 (prelim purge), :2092-2823 (cut, reevaluate,
+// common-endpoint purge) and rmblastn/src/main.rs:1133-1234 (single-chunk driver).
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <optional>
+#include <thread>
+
+#include "itree.hpp"
+#include "oracle.hpp"
+
+namespace orc {
+
+static const int64_t MIN_DIAG_SEP = 50;  // engine.rs:173
+
+// ------------------------------------------------------------ Phase 1 --------
+static void collect_ungapped(const QueryLookup& ql, const std::vector<uint8_t>& query,
+                             const std::vector<uint8_t>& query_rc, const std::vector<uint8_t>& ext_query,
+                             const PreparedSubject& sp, const Params& p, const Matrix& m,
+                             std::vector<UngappedRecord>& out, Taps* taps) {
+  // engine.rs:1063-1277
+  const std::vector<uint8_t>& subject_plus = sp.plus;
+  const std::vector<uint8_t>& subj_rc = sp.rc;
+  const Lookup& L = ql.lut;
+  const size_t s_real = subject_plus.size() >= 2 ? subject_plus.size() - 2 : 0;
+  const int64_t lw = L.lut_width;
+  const int32_t end_range = (int32_t)(s_real >= (size_t)lw ? s_real - lw : 0);
+  const int64_t ext_needed = (int64_t)p.word_size - lw;
+  const int64_t q_real_len = (int64_t)query.size() - 2;
+  const int64_t s_real_len = (int64_t)s_real;
+  const uint32_t q_len = (uint32_t)q_real_len, s_len = (uint32_t)s_real;
+  const int32_t cutoff = p.ungapped_keep();
+  const int64_t c1 = ql.c1_start, fn = ql.fwd_n;
+  const uint32_t ws = (uint32_t)p.word_size;
+  const int32_t window = 1 - p.word_size;
+  DiagHash hash;
+  uint64_t evals = 0, steps = 0;
+  (void)steps;
+  const uint32_t cmask = (uint32_t)((1ull << (2 * lw)) - 1);
+  for (int64_t s_fwd = 0; s_fwd <= end_range; s_fwd += L.scan_step) {
+    uint32_t idx = 0;
+    for (int64_t k = 0; k < lw; ++k) idx = ((idx << 2) | (subject_plus[1 + s_fwd + k] & 3)) & cmask;
+    for (uint32_t e = L.cell_start[idx]; e < L.cell_start[idx + 1]; ++e) {
+      const int64_t q0c = L.entries[e];
+      if (q0c < c1) {
+        uint32_t q_off, s_off;
+        if (ext_needed > 0) {
+          int64_t el = 0;
+          while (el < ext_needed) {
+            int64_t qi = q0c - el - 1, si = s_fwd - el - 1;
+            if (qi < 0 || si < 0) break;
+            uint8_t qb = ext_query[1 + qi], sb = subject_plus[1 + si];
+            if (qb >= 4 || sb >= 4 || qb != sb) break;
+            ++el;
+          }
+          int64_t er = 0;
+          while (el + er < ext_needed) {
+            int64_t qi = q0c + lw + er, si = s_fwd + lw + er;
+            if (qi >= q_real_len || si >= s_real_len) break;
+            uint8_t qb = ext_query[1 + qi], sb = subject_plus[1 + si];
+            if (qb >= 4 || sb >= 4 || qb != sb) break;
+            ++er;
+          }
+          if (el + er < ext_needed) continue;
+          q_off = (uint32_t)(q0c - el);
+          s_off = (uint32_t)(s_fwd - el);
+        } else {
+          q_off = (uint32_t)q0c;
+          s_off = (uint32_t)s_fwd;
+        }
+        const int32_t diag = (int32_t)(s_off - q_off);
+        if (s_off < hash.get(diag)) continue;
+        ++evals;
+        if (taps && taps->seeds) taps->seeds->push_back({q_off, s_off, PLUS});
+        Ungapped u = extend_ungapped(query.data(), query.size(), subject_plus.data(), subject_plus.size(), q_off,
+                                     s_off, m, p.xdrop_ungap);
+        uint32_t store = u.score >= cutoff ? u.s_end : s_off + ws;
+        hash.insert(diag, store, s_off, window);
+        if (u.score < cutoff) continue;
+        out.push_back({q_off, s_off, u.q_start, u.q_end, u.s_start, u.s_end, u.score, PLUS});
+      } else {
+        const int64_t rc_q_off = q0c - c1;
+        const int64_t q0 = fn - rc_q_off - lw;
+        const int64_t s0 = s_real_len - s_fwd - lw;
+        if (q0 < 0 || s0 < 0) continue;
+        uint32_t q_off, s_off;
+        if (ext_needed > 0) {
+          int64_t er = 0;
+          while (er < ext_needed) {
+            int64_t qi = q0 + lw + er, si = s0 + lw + er;
+            if (qi >= q_real_len || si >= s_real_len) break;
+            uint8_t qb = ext_query[1 + qi], sb = subj_rc[1 + si];
+            if (qb >= 4 || sb >= 4 || qb != sb) break;
+            ++er;
+          }
+          int64_t el = 0;
+          while (el + er < ext_needed) {
+            int64_t qi = q0 - el - 1, si = s0 - el - 1;
+            if (qi < 0 || si < 0) break;
+            uint8_t qb = ext_query[1 + qi], sb = subj_rc[1 + si];
+            if (qb >= 4 || sb >= 4 || qb != sb) break;
+            ++el;
+          }
+          if (el + er < ext_needed) continue;
+          q_off = (uint32_t)(q0 + er);
+          s_off = (uint32_t)(s0 + er);
+        } else {
+          q_off = (uint32_t)q0;
+          s_off = (uint32_t)s0;
+        }
+        const int32_t diag = (int32_t)((uint32_t)s_fwd - (uint32_t)q0c);
+        const uint32_t s_fwd_eff = s_len - s_off - (uint32_t)lw;
+        const uint32_t fwd_s_end = s_fwd_eff + ws;
+        if (s_fwd_eff < hash.get(diag)) continue;
+        ++evals;
+        if (taps && taps->seeds) {
+          uint32_t q_seed = q_off >= (uint32_t)std::max<int64_t>(ext_needed, 0) ? q_off - (uint32_t)ext_needed : 0;
+          uint32_t s_seed = s_len >= s_off + (uint32_t)lw ? s_len - (s_off + (uint32_t)lw) : 0;
+          taps->seeds->push_back({q_seed, s_seed, MINUS});
+        }
+        const uint32_t qn = q_len - 1 - q_off, sn = s_len - 1 - s_off;
+        Ungapped r = extend_ungapped(query_rc.data(), query_rc.size(), subject_plus.data(), subject_plus.size(), qn,
+                                     sn, m, p.xdrop_ungap);
+        uint32_t store = r.score >= cutoff ? r.s_end : fwd_s_end;
+        hash.insert(diag, store, s_fwd_eff, window);
+        if (r.score < cutoff) continue;
+        out.push_back({q_off, s_off, q_len - r.q_end, q_len - r.q_start, s_len - r.s_end, s_len - r.s_start, r.score,
+                       MINUS});
+      }
+    }
+  }
+  if (taps && taps->n_seed_evals) *taps->n_seed_evals += evals;
+}
+
+static void sort_ungapped_like_ncbi(std::vector<UngappedRecord>& v, uint32_t q_len, uint32_t s_len) {
+  // engine.rs:1280-1309
+  auto key = [&](const UngappedRecord& a) {
+    uint32_t ss = a.strand == PLUS ? a.s_start : s_len - a.s_end;
+    uint64_t qk = a.strand == PLUS ? (uint64_t)a.q_start
+                                   : ((uint64_t)q_len + 1) + ((uint64_t)q_len > a.q_end ? (uint64_t)q_len - a.q_end : 0);
+    uint32_t sk = a.strand == PLUS ? a.s_seed : s_len - a.s_seed;
+    return std::make_tuple(ss, qk, sk);
+  };
+  std::sort(v.begin(), v.end(), [&](const UngappedRecord& a, const UngappedRecord& b) {
+    if (a.score != b.score) return a.score > b.score;
+    auto ka = key(a), kb = key(b);
+    if (std::get<0>(ka) != std::get<0>(kb)) return std::get<0>(ka) < std::get<0>(kb);
+    uint32_t la = a.s_end - a.s_start, lb = b.s_end - b.s_start;
+    if (la != lb) return la > lb;
+    if (std::get<1>(ka) != std::get<1>(kb)) return std::get<1>(ka) < std::get<1>(kb);
+    return std::get<2>(ka) < std::get<2>(kb);
+  });
+}
+
+// ------------------------------------------------------------ Phase 2a -------
+static void prelim_pivot(const UngappedRecord& u, uint32_t qa_len, uint32_t sa_len, uint32_t lw, uint32_t& pq,
+                         uint32_t& ps, uint32_t& tq, uint32_t& ts) {
+  // engine.rs:1850-1895
+  if (u.strand == PLUS) {
+    if (u.s_end >= u.s_seed + 8) { tq = u.q_seed + 3; ts = u.s_seed + 3; }
+    else { tq = u.q_seed; ts = u.s_seed; }
+    uint32_t adj = 4 - (ts % 4);
+    ps = ts + adj;
+    pq = tq + adj;
+    if (ps >= sa_len || pq >= qa_len) { ps -= 4; pq -= 4; }
+  } else {
+    uint32_t s_off_saved_fwd = sa_len - u.s_seed - lw;
+    uint32_t s_end_fwd = sa_len - u.s_start;
+    bool extends = s_end_fwd >= s_off_saved_fwd + 8;
+    if (extends) { tq = u.q_seed + lw - 4; ts = u.s_seed + lw - 4; }
+    else { tq = u.q_seed; ts = u.s_seed; }
+    uint32_t shift = extends ? 3 : 0;
+    uint32_t q_off_n = (qa_len - u.q_seed - lw) + shift;
+    uint32_t s_off_n = (sa_len - u.s_seed - lw) + shift;
+    uint32_t oa = 4 - (s_off_n % 4);
+    pq = q_off_n + oa;
+    ps = s_off_n + oa;
+    if (pq > qa_len || ps > sa_len) { pq -= 4; ps -= 4; }
+  }
+}
+
+void purge_prelim_common_endpoints(std::vector<PrelimHsp>& v, uint32_t sa_len) {  // engine.rs:1969-2083
+  const size_t n = v.size();
+  if (n < 2) return;
+  std::vector<char> keep(n, 1);
+  auto fss = [&](const PrelimHsp& h) { return h.strand == PLUS ? h.s_start : sa_len - h.s_end; };
+  auto fse = [&](const PrelimHsp& h) { return h.strand == PLUS ? h.s_end : sa_len - h.s_start; };
+  auto qoff = [](const PrelimHsp& h) { return h.strand == PLUS ? h.q_start : h.q_end; };
+  auto qend = [](const PrelimHsp& h) { return h.strand == PLUS ? h.q_end : h.q_start; };
+  auto ctx = [](const PrelimHsp& h) { return h.strand == PLUS ? 0 : 1; };
+  std::vector<size_t> idx(n);
+  for (size_t i = 0; i < n; ++i) idx[i] = i;
+  std::sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+    const PrelimHsp &ha = v[a], &hb = v[b];
+    if (ctx(ha) != ctx(hb)) return ctx(ha) < ctx(hb);
+    if (qoff(ha) != qoff(hb)) return qoff(ha) < qoff(hb);
+    if (fss(ha) != fss(hb)) return fss(ha) < fss(hb);
+    if (ha.score != hb.score) return ha.score > hb.score;
+    if (qend(ha) != qend(hb)) return ha.strand == PLUS ? qend(ha) > qend(hb) : qend(ha) < qend(hb);
+    if (fse(ha) != fse(hb)) return fse(ha) > fse(hb);
+    return a < b;
+  });
+  for (size_t i = 0; i < n;) {
+    if (!keep[idx[i]]) { ++i; continue; }
+    const PrelimHsp& h0 = v[idx[i]];
+    size_t j = i + 1;
+    while (j < n) {
+      const PrelimHsp& hj = v[idx[j]];
+      if (ctx(hj) != ctx(h0) || qoff(hj) != qoff(h0) || fss(hj) != fss(h0)) break;
+      keep[idx[j]] = 0;
+      ++j;
+    }
+    i = j;
+  }
+  for (size_t i = 0; i < n; ++i) idx[i] = i;
+  std::sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+    const PrelimHsp &ha = v[a], &hb = v[b];
+    if (ctx(ha) != ctx(hb)) return ctx(ha) < ctx(hb);
+    if (qend(ha) != qend(hb)) return qend(ha) < qend(hb);
+    if (fse(ha) != fse(hb)) return fse(ha) < fse(hb);
+    if (ha.score != hb.score) return ha.score > hb.score;
+    if (qoff(ha) != qoff(hb)) return ha.strand == PLUS ? qoff(ha) > qoff(hb) : qoff(ha) < qoff(hb);
+    if (fss(ha) != fss(hb)) return fss(ha) > fss(hb);
+    return a < b;
+  });
+  for (size_t i = 0; i < n;) {
+    if (!keep[idx[i]]) { ++i; continue; }
+    const PrelimHsp& h0 = v[idx[i]];
+    size_t j = i + 1;
+    while (j < n) {
+      const PrelimHsp& hj = v[idx[j]];
+      if (ctx(hj) != ctx(h0) || qend(hj) != qend(h0) || fse(hj) != fse(h0)) break;
+      keep[idx[j]] = 0;
+      ++j;
+    }
+    i = j;
+  }
+  size_t w = 0;
+  for (size_t i = 0; i < n; ++i)
+    if (keep[i]) std::swap(v[i], v[w++]);
+  v.resize(w);
+}
+
+static std::vector<PrelimHsp> run_phase2a(const std::vector<uint8_t>& query, const std::vector<uint8_t>& query_rc,
+                                          const std::vector<uint8_t>& subject_plus, uint32_t qa_len, uint32_t sa_len,
+                                          const std::vector<UngappedRecord>& ung, const Params& p, const Matrix& m,
+                                          uint32_t lw, DpCounters* ctr) {
+  // engine.rs:1317-1411
+  IntervalTree tp(0, qa_len + 1, 0, sa_len + 1), tm(0, qa_len + 1, 0, sa_len + 1);
+  std::vector<PrelimHsp> out;
+  for (const auto& u : ung) {
+    const bool plus = u.strand == PLUS;
+    uint32_t css = plus ? u.s_start : sa_len - u.s_end;
+    uint32_t cse = plus ? u.s_end : sa_len - u.s_start;
+    if ((plus ? tp : tm).contains(u.q_start, u.q_end, css, cse, u.score, plus, MIN_DIAG_SEP)) continue;
+    uint32_t pq, ps, tq, ts;
+    prelim_pivot(u, qa_len, sa_len, lw, pq, ps, tq, ts);
+    int xd = std::min(p.xdrop_gap, u.score);
+    int32_t sc;
+    uint32_t q0, q1, s0, s1;
+    if (!gapped_score_only(plus ? query : query_rc, subject_plus, pq, ps, p.gap_open, p.gap_extend, xd, m, sc, q0, q1,
+                           s0, s1, ctr))
+      continue;
+    uint32_t qs, qe, ss, se;
+    if (!plus) { qs = qa_len - q1; qe = qa_len - q0; ss = sa_len - s1; se = sa_len - s0; }
+    else { qs = q0; qe = q1; ss = s0; se = s1; }
+    if (sc < p.gapped_cutoff()) continue;
+    uint32_t nss = plus ? ss : sa_len - se, nse = plus ? se : sa_len - ss;
+    (plus ? tp : tm).add(ITreeHsp{qs, qe, nss, nse, sc}, plus);
+    out.push_back({tq, ts, qs, qe, ss, se, sc, u.strand});
+  }
+  purge_prelim_common_endpoints(out, sa_len);
+  return out;
+}
+
+// ------------------------------------------------------------ improve_seed ----
+static void improve_seed(const uint8_t* query, size_t qlen, const uint8_t* subject, size_t slen, uint32_t q_seed32,
+                         uint32_t s_seed32, uint32_t pqs, uint32_t pqe, uint32_t pss, uint32_t pse, bool minus,
+                         uint32_t& oq, uint32_t& os) {
+  // engine.rs:383-572
+  const int32_t RUN = 10;
+  const int32_t MAXRUN = (int32_t)(10.0 * 1.5);
+  const size_t q_seed = q_seed32, s_seed = s_seed32;
+  const size_t prelim_q_start = pqs, prelim_q_end = pqe, prelim_s_start = pss, prelim_s_end = pse;
+  auto sat = [](size_t a, size_t b) { return a > b ? a - b : 0; };
+  oq = q_seed32;
+  os = s_seed32;
+  if (!minus) {
+    int32_t score = -1;
+    size_t qf = q_seed, sf = s_seed;
+    while (qf < prelim_q_end && qf < qlen && sf < slen && query[qf] == subject[sf]) {
+      if (++score > RUN) return;
+      ++qf; ++sf;
+    }
+    size_t qb = q_seed, sb = s_seed;
+    for (;;) {
+      if (query[qb] != subject[sb]) break;
+      if (++score > RUN) return;
+      if (qb == 0 || sb == 0) break;
+      --qb; --sb;
+    }
+    size_t offset = std::min(sat(q_seed, prelim_q_start), sat(s_seed, prelim_s_start));
+    size_t sqs = q_seed - offset, sss = s_seed - offset;
+    size_t q_len = std::min(sat(prelim_s_end, sss), sat(prelim_q_end, sqs));
+    int32_t max_score = 0;
+    size_t max_offset = sqs;
+    score = 0;
+    bool is_match = false, prev = false;
+    for (size_t i = 0; i < q_len; ++i) {
+      size_t qi = sqs + i, si = sss + i;
+      if (qi >= qlen || si >= slen) break;
+      is_match = query[qi] == subject[si];
+      if (is_match != prev) {
+        prev = is_match;
+        if (is_match) score = 1;
+        else if (score > max_score) { max_score = score; max_offset = qi - (size_t)(score / 2); }
+      } else if (is_match) {
+        ++score;
+        if (score > MAXRUN) {
+          size_t nq = qi - (size_t)(MAXRUN / 2);
+          oq = (uint32_t)nq;
+          os = (uint32_t)(sss + (nq - sqs));
+          return;
+        }
+      }
+    }
+    if (is_match && score > max_score) {
+      max_score = score;
+      size_t qi = sqs + q_len;
+      max_offset = qi - (size_t)(score / 2);
+    }
+    if (max_score > 0) {
+      oq = (uint32_t)max_offset;
+      os = (uint32_t)(sss + (max_offset - sqs));
+    }
+    return;
+  }
+  int32_t score = -1;
+  size_t qf = q_seed, sf = s_seed;
+  while (qf < qlen && sf < slen && query[qf] == subject[sf]) {
+    if (++score > RUN) return;
+    ++qf; ++sf;
+  }
+  size_t qb = q_seed, sb = s_seed;
+  for (;;) {
+    if (query[qb] != subject[sb]) break;
+    if (++score > RUN) return;
+    if (qb <= prelim_q_start || sb == 0) break;
+    --qb; --sb;
+  }
+  size_t offset = std::min(sat(sat(prelim_q_end, 1), q_seed), sat(sat(prelim_s_end, 1), s_seed));
+  size_t top_q = q_seed + offset, top_s = s_seed + offset;
+  size_t q_len = std::min(sat(top_q, prelim_q_start) + 1, sat(top_s, prelim_s_start) + 1);
+  int32_t max_score = 0;
+  int64_t max_offset = (int64_t)q_seed;
+  score = 0;
+  bool is_match = false, prev = false;
+  for (size_t i = 0; i < q_len; ++i) {
+    size_t qi = top_q - i, si = top_s - i;
+    if (qi >= qlen || si >= slen) continue;
+    is_match = query[qi] == subject[si];
+    if (is_match != prev) {
+      prev = is_match;
+      if (is_match) score = 1;
+      else if (score > max_score) { max_score = score; max_offset = (int64_t)qi + score / 2; }
+    } else if (is_match) {
+      ++score;
+      if (score > MAXRUN) {
+        size_t nq = qi + (size_t)(MAXRUN / 2);
+        oq = (uint32_t)nq;
+        os = (uint32_t)(top_s - (top_q - nq));
+        return;
+      }
+    }
+  }
+  if (is_match && score > max_score) {
+    max_score = score;
+    int64_t qi = (int64_t)top_q - (int64_t)q_len;
+    max_offset = qi + score / 2;
+  }
+  if (max_score > 0) {
+    if (max_offset < 0) return;
+    int64_t delta = (int64_t)top_q - max_offset;
+    int64_t ns = (int64_t)top_s - delta;
+    oq = (uint32_t)max_offset;
+    os = (uint32_t)std::max<int64_t>(ns, 0);
+  }
+}
+
+// --------------------------------------------------------- cut + purge --------
+static bool cut_scan_front(const EditOps& ops, uint32_t nq, uint32_t ns, size_t& ci, uint32_t& part, uint32_t& gq,
+                           uint32_t& gs) {
+  gq = gs = 0;
+  for (size_t i = 0; i < ops.size(); ++i) {
+    auto [op, cnt] = ops[i];
+    if (op == OP_SUB) {
+      for (uint32_t j = 0; j < cnt; ++j) {
+        ++gq; ++gs;
+        if (gq >= nq && gs >= ns) { ci = i; part = j + 1; return true; }
+      }
+    } else if (op == OP_GAP_IN_QUERY) {
+      gs += cnt;
+      if (gq >= nq && gs >= ns) { ci = i; part = cnt; return true; }
+    } else {
+      gq += cnt;
+      if (gq >= nq && gs >= ns) { ci = i; part = cnt; return true; }
+    }
+  }
+  return false;
+}
+static bool cut_scan_back(const EditOps& ops, uint32_t nq, uint32_t ns, size_t& ci, uint32_t& part, uint32_t& gq,
+                          uint32_t& gs) {
+  gq = gs = 0;
+  for (size_t k = ops.size(); k-- > 0;) {
+    auto [op, cnt] = ops[k];
+    if (op == OP_SUB) {
+      for (uint32_t j = 0; j < cnt; ++j) {
+        ++gq; ++gs;
+        if (gq >= nq && gs >= ns) { ci = k; part = j + 1; return true; }
+      }
+    } else if (op == OP_GAP_IN_QUERY) {
+      gs += cnt;
+      if (gq >= nq && gs >= ns) { ci = k; part = cnt; return true; }
+    } else {
+      gq += cnt;
+      if (gq >= nq && gs >= ns) { ci = k; part = cnt; return true; }
+    }
+  }
+  return false;
+}
+// engine.rs:2092-2126 cut_ops_keep_front
+static bool cut_ops_keep_front(EditOps& ops, uint32_t nq, uint32_t ns, uint32_t& gq, uint32_t& gs) {
+  if (nq == 0 && ns == 0) { gq = gs = 0; return true; }
+  size_t ci; uint32_t part;
+  if (!cut_scan_front(ops, nq, ns, ci, part, gq, gs)) return false;
+  if (part < ops[ci].second) ops[ci].second = part;
+  ops.resize(ci + 1);
+  return true;
+}
+// engine.rs:2131-2167 cut_ops_front
+static bool cut_ops_front(EditOps& ops, uint32_t nq, uint32_t ns, uint32_t& gq, uint32_t& gs) {
+  if (nq == 0 && ns == 0) { gq = gs = 0; return true; }
+  size_t ci; uint32_t part;
+  if (!cut_scan_front(ops, nq, ns, ci, part, gq, gs)) return false;
+  auto [op, cnt] = ops[ci];
+  uint32_t rem = cnt - part;
+  ops.erase(ops.begin(), ops.begin() + ci + 1);
+  if (rem > 0) ops.insert(ops.begin(), {op, rem});
+  return true;
+}
+// engine.rs:2174-2209 cut_ops_back
+static bool cut_ops_back(EditOps& ops, uint32_t nq, uint32_t ns, uint32_t& gq, uint32_t& gs) {
+  if (nq == 0 && ns == 0) { gq = gs = 0; return true; }
+  size_t ci; uint32_t part;
+  if (!cut_scan_back(ops, nq, ns, ci, part, gq, gs)) return false;
+  auto [op, cnt] = ops[ci];
+  uint32_t keep = cnt - part;
+  ops.resize(ci);
+  if (keep > 0) ops.push_back({op, keep});
+  return true;
+}
+// engine.rs:2220-2255 keep_ops_back
+static bool keep_ops_back(EditOps& ops, uint32_t nq, uint32_t ns, uint32_t& gq, uint32_t& gs) {
+  if (nq == 0 && ns == 0) { gq = gs = 0; return true; }
+  size_t ci; uint32_t part;
+  if (!cut_scan_back(ops, nq, ns, ci, part, gq, gs)) return false;
+  auto [op, cnt] = ops[ci];
+  ops.erase(ops.begin(), ops.begin() + ci);
+  if (part < cnt) ops[0] = {op, part};
+  return true;
+}
+
+static uint32_t sat32(uint32_t a, uint32_t b) { return a > b ? a - b : 0; }
+
+static bool cut_start_secondary(const AlignResult& pr, AlignResult& sec) {  // engine.rs:2645-2677
+  const Hsp& p = pr.hsp;
+  Hsp& s = sec.hsp;
+  bool ext = p.strand == PLUS ? s.q_end > p.q_end : s.q_start < p.q_start;
+  if (!ext) return false;
+  uint32_t gq, gs;
+  if (p.strand == PLUS) {
+    uint32_t dq = sat32(p.q_end, s.q_start), ds = sat32(p.s_end, s.s_start);
+    uint32_t oqs = s.q_start, oss = s.s_start;
+    if (cut_ops_front(s.ops, dq, ds, gq, gs)) { s.q_start = oqs + gq; s.s_start = oss + gs; }
+  } else {
+    uint32_t dq = sat32(s.q_end, p.q_start), ds = sat32(p.s_end, s.s_start);
+    uint32_t oqe = s.q_end, oss = s.s_start;
+    if (cut_ops_back(s.ops, dq, ds, gq, gs)) { s.q_end = oqe - gq; s.s_start = oss + gs; }
+  }
+  return !s.ops.empty();
+}
+static bool cut_end_secondary(const AlignResult& pr, AlignResult& sec) {  // engine.rs:2681-2708
+  const Hsp& p = pr.hsp;
+  Hsp& s = sec.hsp;
+  bool ext = p.strand == PLUS ? s.q_start < p.q_start : s.q_end > p.q_end;
+  if (!ext) return false;
+  uint32_t gq, gs;
+  if (p.strand == PLUS) {
+    uint32_t nq = sat32(p.q_start, s.q_start), ns = sat32(p.s_start, s.s_start);
+    if (cut_ops_keep_front(s.ops, nq, ns, gq, gs)) { s.q_end = s.q_start + gq; s.s_end = s.s_start + gs; }
+  } else {
+    uint32_t nq = sat32(s.q_end, p.q_end), ns = sat32(p.s_start, s.s_start);
+    if (keep_ops_back(s.ops, nq, ns, gq, gs)) { s.q_start = sat32(s.q_end, gq); s.s_end = s.s_start + gs; }
+  }
+  return !s.ops.empty();
+}
+
+static bool cmp_query_offset(const AlignResult& A, const AlignResult& B) {  // engine.rs:2589-2605, as "less"
+  const Hsp &a = A.hsp, &b = B.hsp;
+  if (a.strand != b.strand) return a.strand < b.strand;
+  uint32_t ka = a.strand == PLUS ? a.q_start : a.q_end, kb = b.strand == PLUS ? b.q_start : b.q_end;
+  if (ka != kb) return ka < kb;
+  if (a.s_start != b.s_start) return a.s_start < b.s_start;
+  if (a.score != b.score) return a.score > b.score;
+  if (a.strand == PLUS) { if (a.q_end != b.q_end) return a.q_end > b.q_end; }
+  else if (a.q_start != b.q_start) return a.q_start < b.q_start;
+  return a.s_end > b.s_end;
+}
+static bool cmp_query_end(const AlignResult& A, const AlignResult& B) {  // engine.rs:2609-2625
+  const Hsp &a = A.hsp, &b = B.hsp;
+  if (a.strand != b.strand) return a.strand < b.strand;
+  uint32_t ka = a.strand == PLUS ? a.q_end : a.q_start, kb = b.strand == PLUS ? b.q_end : b.q_start;
+  if (ka != kb) return ka < kb;
+  if (a.s_end != b.s_end) return a.s_end < b.s_end;
+  if (a.score != b.score) return a.score > b.score;
+  if (a.strand == PLUS) { if (a.q_start != b.q_start) return a.q_start > b.q_start; }
+  else if (a.q_end != b.q_end) return a.q_end < b.q_end;
+  return a.s_start > b.s_start;
+}
+static bool same_start(const AlignResult& A, const AlignResult& B) {
+  if (A.hsp.strand != B.hsp.strand) return false;
+  uint32_t ka = A.hsp.strand == PLUS ? A.hsp.q_start : A.hsp.q_end;
+  uint32_t kb = B.hsp.strand == PLUS ? B.hsp.q_start : B.hsp.q_end;
+  return ka == kb && A.hsp.s_start == B.hsp.s_start;
+}
+static bool same_end(const AlignResult& A, const AlignResult& B) {
+  if (A.hsp.strand != B.hsp.strand) return false;
+  uint32_t ka = A.hsp.strand == PLUS ? A.hsp.q_end : A.hsp.q_start;
+  uint32_t kb = B.hsp.strand == PLUS ? B.hsp.q_end : B.hsp.q_start;
+  return ka == kb && A.hsp.s_end == B.hsp.s_end;
+}
+
+using Slots = std::vector<std::optional<AlignResult>>;
+static size_t collapse(Slots& arr, size_t count, bool purge, bool (*same)(const AlignResult&, const AlignResult&),
+                       bool (*cut)(const AlignResult&, AlignResult&)) {  // engine.rs:2745-2769
+  std::vector<std::optional<AlignResult>> removed;
+  size_t surv = 0;
+  for (size_t r = 0; r < count; ++r) {
+    bool joins = surv > 0 && same(*arr[surv - 1], *arr[r]);
+    if (joins) {
+      AlignResult sec = std::move(*arr[r]);
+      arr[r].reset();
+      bool keep = !purge && cut(*arr[surv - 1], sec);
+      if (keep) removed.emplace_back(std::move(sec)); else removed.emplace_back(std::nullopt);
+    } else {
+      std::swap(arr[surv], arr[r]);
+      ++surv;
+    }
+  }
+  size_t k = surv;
+  for (auto it = removed.rbegin(); it != removed.rend() && k < count; ++it, ++k) arr[k] = std::move(*it);
+  return surv;
+}
+static size_t purge_common_endpoints(Slots& arr, bool purge) {  // engine.rs:2721-2738
+  size_t n = arr.size();
+  if (n < 2) return n;
+  std::stable_sort(arr.begin(), arr.begin() + n,
+                   [](const std::optional<AlignResult>& a, const std::optional<AlignResult>& b) {
+                     return cmp_query_offset(*a, *b);
+                   });
+  n = collapse(arr, n, purge, same_start, cut_start_secondary);
+  std::stable_sort(arr.begin(), arr.begin() + n,
+                   [](const std::optional<AlignResult>& a, const std::optional<AlignResult>& b) {
+                     return cmp_query_end(*a, *b);
+                   });
+  return collapse(arr, n, purge, same_end, cut_end_secondary);
+}
+
+int purge_hsps_with_common_endpoints(std::vector<AlignResult*>& v, bool purge) {
+  Slots arr;
+  for (auto* p : v) arr.emplace_back(*p);
+  size_t c = purge_common_endpoints(arr, purge);
+  for (size_t i = 0; i < v.size(); ++i) {
+    if (arr[i]) *v[i] = *arr[i];
+    else v[i]->hsp.score = INT32_MIN;  // marks a freed slot for the caller
+  }
+  return (int)c;
+}
+
+// engine.rs:2268-2567 Blast_HSPReevaluateWithAmbiguitiesGapped
+static bool reevaluate_gapped(Hsp& hsp, const std::vector<uint8_t>& q_seq, const std::vector<uint8_t>& s_seq,
+                              const Matrix& m, int go, int ge, int min_score, bool minus, const uint8_t* raw_q,
+                              size_t raw_q_len, size_t q_raw_start, const uint8_t* raw_s, size_t raw_s_len,
+                              size_t s_raw_start) {
+  if (hsp.ops.empty()) return true;
+  int32_t sum = 0, score = 0;
+  size_t seq_i = 0;
+  uint32_t q_pos = 0, s_pos = 0;
+  uint32_t bqs = 0, bss = 0, bqe = 0, bse = 0, cqs = 0, css = 0;
+  size_t best_start_i = 0, best_end_i = 0, cur_start_i = 0;
+  uint32_t best_end_num = 0, cur_start_skip = 0, best_start_skip = 0;
+  const EditOps snap = hsp.ops;
+  const size_t esp_len = snap.size();
+  EditOps sops = snap;
+  std::vector<uint8_t> sq = q_seq, ssq = s_seq;
+  if (minus) {
+    std::reverse(sops.begin(), sops.end());
+    std::reverse(sq.begin(), sq.end());
+    std::reverse(ssq.begin(), ssq.end());
+  }
+  for (size_t ei = 0; ei < sops.size(); ++ei) {
+    auto [op, cnt] = sops[ei];
+    if (op == OP_SUB) {
+      for (uint32_t k = 0; k < cnt; ++k) {
+        uint8_t qb0 = (seq_i < sq.size() ? sq[seq_i] : 14) & 0xf;
+        uint8_t sb0 = (seq_i < ssq.size() ? ssq[seq_i] : 14) & 0xf;
+        uint8_t qb = minus ? (kComplement[qb0] & 0xf) : qb0;
+        uint8_t sb = minus ? (kComplement[sb0] & 0xf) : sb0;
+        sum += m.s[qb][sb];
+        ++seq_i; ++q_pos; ++s_pos;
+        uint32_t op_idx = k + 1;
+        if (sum < 0) {
+          if (op_idx < cnt) { cur_start_i = ei; cur_start_skip = op_idx; }
+          else { cur_start_i = ei + 1; cur_start_skip = 0; }
+          sum = 0;
+          cqs = q_pos; css = s_pos;
+          if (score < min_score) {
+            bqs = q_pos; bss = s_pos; score = 0;
+            best_start_i = cur_start_i; best_start_skip = cur_start_skip;
+            best_end_i = cur_start_i; best_end_num = 0;
+          }
+        } else if (sum > score) {
+          score = sum;
+          bqs = cqs; bss = css; bqe = q_pos; bse = s_pos;
+          best_start_i = cur_start_i; best_start_skip = cur_start_skip;
+          best_end_i = ei; best_end_num = op_idx;
+        }
+      }
+    } else {
+      sum -= go + ge * (int32_t)cnt;
+      if (op == OP_GAP_IN_QUERY) { seq_i += cnt; s_pos += cnt; }
+      else { seq_i += cnt; q_pos += cnt; }
+      if (sum < 0) {
+        cur_start_i = ei + 1; cur_start_skip = 0;
+        sum = 0; cqs = q_pos; css = s_pos;
+        if (score < min_score) {
+          bqs = q_pos; bss = s_pos; score = 0;
+          best_start_i = cur_start_i; best_start_skip = 0;
+          best_end_i = cur_start_i; best_end_num = 0;
+        }
+      } else if (sum > score) {
+        score = sum;
+        bqs = cqs; bss = css; bqe = q_pos; bse = s_pos;
+        best_start_i = cur_start_i; best_start_skip = cur_start_skip;
+        best_end_i = ei; best_end_num = cnt;
+      }
+    }
+  }
+  if (score < min_score) return true;
+  if (minus) {
+    uint32_t qtot = 0, stot = 0;
+    for (auto& [op, c] : snap) {
+      if (op == OP_SUB) { qtot += c; stot += c; }
+      else if (op == OP_GAP_IN_SUBJECT) qtot += c;
+      else stot += c;
+    }
+    size_t f_start_i = esp_len - 1 - best_end_i;
+    size_t f_end_i = esp_len - 1 - best_start_i;
+    uint32_t f_end_num = snap[f_end_i].second - best_start_skip;
+    uint32_t f_start_skip = snap[f_start_i].second - best_end_num;
+    uint32_t fqs = qtot - bqe, fqe = qtot - bqs, fss = stot - bse, fse = stot - bss;
+    best_start_i = f_start_i; best_end_i = f_end_i; best_start_skip = f_start_skip; best_end_num = f_end_num;
+    bqs = fqs; bqe = fqe; bss = fss; bse = fse;
+  }
+  {
+    EditOps& ops = hsp.ops;
+    if (best_end_i < (esp_len > 0 ? esp_len - 1 : 0) || best_end_num < snap[best_end_i].second) {
+      ops.resize(best_end_i + 1);
+      if (best_end_num > 0) ops[best_end_i].second = best_end_num;
+      else ops.resize(best_end_i);
+    }
+    if (best_start_i > 0) ops.erase(ops.begin(), ops.begin() + std::min(best_start_i, ops.size()));
+    if (best_start_skip > 0 && !ops.empty()) {
+      auto [op0, c0] = ops[0];
+      if (best_start_skip < c0) ops[0] = {op0, c0 - best_start_skip};
+      else ops.erase(ops.begin());
+    }
+  }
+  int64_t ebqs = bqs, ebss = bss, ebqe = bqe, ebse = bse;
+  for (;;) {
+    int64_t qi = (int64_t)q_raw_start + ebqs, si = (int64_t)s_raw_start + ebss;
+    if (qi <= 0 || si <= 0) break;
+    uint8_t qb = raw_q[qi - 1], sb = raw_s[si - 1];
+    if (qb < 4 && qb == sb) {
+      --ebqs; --ebss;
+      if (!hsp.ops.empty() && hsp.ops.front().first == OP_SUB) hsp.ops.front().second++;
+      else hsp.ops.insert(hsp.ops.begin(), {OP_SUB, 1});
+    } else break;
+  }
+  for (;;) {
+    int64_t qi = (int64_t)q_raw_start + ebqe, si = (int64_t)s_raw_start + ebse;
+    if (qi >= (int64_t)raw_q_len || si >= (int64_t)raw_s_len) break;
+    uint8_t qb = raw_q[qi], sb = raw_s[si];
+    if (qb < 4 && qb == sb) {
+      ++ebqe; ++ebse;
+      if (!hsp.ops.empty() && hsp.ops.back().first == OP_SUB) hsp.ops.back().second++;
+      else hsp.ops.push_back({OP_SUB, 1});
+    } else break;
+  }
+  hsp.score = score;
+  if (minus) {
+    int64_t oqs = hsp.q_start, ose = hsp.s_end;
+    hsp.q_start = (uint32_t)std::max<int64_t>(oqs + ebqs, 0);
+    hsp.q_end = (uint32_t)std::max<int64_t>(oqs + ebqe, 0);
+    hsp.s_end = (uint32_t)std::max<int64_t>(ose - ebss, 0);
+    hsp.s_start = (uint32_t)std::max<int64_t>(ose - ebse, 0);
+  } else {
+    int64_t oqs = hsp.q_start, oss = hsp.s_start;
+    uint32_t nqs = (uint32_t)std::max<int64_t>(oqs + ebqs, 0);
+    uint32_t nss = (uint32_t)std::max<int64_t>(oss + ebss, 0);
+    hsp.q_start = nqs;
+    hsp.q_end = nqs + (uint32_t)(ebqe - ebqs);
+    hsp.s_start = nss;
+    hsp.s_end = nss + (uint32_t)(ebse - ebss);
+  }
+  return false;
+}
+
+// ------------------------------------------------------------ Phase 2b -------
+static std::vector<AlignResult> traceback_phase(const std::vector<uint8_t>& query,
+                                                const std::vector<uint8_t>& query_rc, const std::string& query_id,
+                                                const PreparedSubject& subj, const std::string& subject_id,
+                                                std::vector<PrelimHsp> prelims, const Params& p, const Matrix& m,
+                                                DpCounters* ctr) {
+  // engine.rs:1438-1762
+  const uint32_t qa_len = (uint32_t)query.size() - 2;
+  const uint32_t s_len = (uint32_t)subj.align.size() - 2, sa_len = s_len;
+  const std::vector<uint8_t>& subject_plus = subj.align;
+  const std::vector<uint8_t>& subj_rc = subj.align_rc;
+  const std::vector<uint8_t>& n_mask = subj.n_mask;
+  std::stable_sort(prelims.begin(), prelims.end(), [&](const PrelimHsp& a, const PrelimHsp& b) {
+    uint32_t fsa = a.strand == PLUS ? a.s_start : s_len - a.s_end, fsb = b.strand == PLUS ? b.s_start : s_len - b.s_end;
+    uint32_t fea = a.strand == PLUS ? a.s_end : s_len - a.s_start, feb = b.strand == PLUS ? b.s_end : s_len - b.s_start;
+    uint32_t qoa = a.strand == PLUS ? a.q_start : qa_len - a.q_end, qob = b.strand == PLUS ? b.q_start : qa_len - b.q_end;
+    uint32_t qea = a.strand == PLUS ? a.q_end : qa_len - a.q_start, qeb = b.strand == PLUS ? b.q_end : qa_len - b.q_start;
+    if (a.score != b.score) return a.score > b.score;
+    if (fsa != fsb) return fsa < fsb;
+    if (fea != feb) return fea > feb;
+    if (qoa != qob) return qoa < qob;
+    return qea > qeb;
+  });
+  IntervalTree tbp(0, qa_len + 1, 0, s_len + 1), tbm(0, qa_len + 1, 0, s_len + 1, true);
+  std::vector<AlignResult> results;
+  for (const auto& ph : prelims) {
+    const bool plus = ph.strand == PLUS;
+    uint32_t pss = plus ? ph.s_start : s_len - ph.s_end, pse = plus ? ph.s_end : s_len - ph.s_start;
+    uint32_t cqs = plus ? ph.q_start : qa_len - ph.q_end, cqe = plus ? ph.q_end : qa_len - ph.q_start;
+    if ((plus ? tbp : tbm).contains(cqs, cqe, pss, pse, ph.score, plus, MIN_DIAG_SEP)) continue;
+    const std::vector<uint8_t>& sfs = plus ? subject_plus : subj_rc;
+    uint32_t nq, ns;
+    improve_seed(query.data() + 1, qa_len, sfs.data() + 1, sfs.size() - 2, ph.q_seed, ph.s_seed, ph.q_start, ph.q_end,
+                 ph.s_start, ph.s_end, !plus, nq, ns);
+    uint32_t tq = plus ? nq : qa_len - 1 - nq, ts = plus ? ns : sa_len - 1 - ns;
+    int32_t score;
+    uint32_t q0, q1, s0, s1;
+    EditOps ops;
+    if (!gapped_extend_bidirectional(plus ? query : query_rc, subject_plus, tq, ts, p.gap_open, p.gap_extend,
+                                     p.xdrop_gap_final, m, score, q0, q1, s0, s1, ops, ctr))
+      continue;
+    uint32_t qs, qe, ss, se;
+    std::vector<uint8_t> qb, sb;
+    if (!plus) {
+      std::vector<uint8_t> qx, sx;
+      extract_aligned(query_rc.data() + 1, query_rc.size() - 2, subject_plus.data() + 1, subject_plus.size() - 2, q0,
+                      s0, ops, n_mask, qx, sx);
+      qs = qa_len - q1; qe = qa_len - q0; ss = sa_len - s1; se = sa_len - s0;
+      qb = revcomp(qx);
+      sb = revcomp(sx);
+      std::reverse(ops.begin(), ops.end());
+    } else {
+      extract_aligned(query.data() + 1, query.size() - 2, subject_plus.data() + 1, subject_plus.size() - 2, q0, s0, ops,
+                      n_mask, qb, sb);
+      qs = q0; qe = q1; ss = s0; se = s1;
+    }
+    if (p.complexity_adjust) {
+      int32_t adj;
+      if (!apply_complexity_adjust(score, query, qs, ops, m, p.gapped_cutoff(), !plus, adj)) continue;
+      score = adj;
+    }
+    uint32_t rqs = qs, rqe = qe, rss = plus ? ss : s_len - se, rse = plus ? se : s_len - ss;
+    {
+      uint32_t tqs = plus ? rqs : qa_len - rqe, tqe = plus ? rqe : qa_len - rqs;
+      (plus ? tbp : tbm).add(ITreeHsp{tqs, tqe, rss, rse, score}, true);
+    }
+    AlignResult r;
+    r.hsp = Hsp{score, rqs, rqe, qa_len, rss, rse, s_len, ph.strand, std::move(ops), std::move(qb), std::move(sb)};
+    r.query_id = query_id;
+    r.subject_id = subject_id;
+    r.stats = compute_align_stats(to_iupac(r.hsp.q_seq), to_iupac(r.hsp.s_seq));
+    results.push_back(std::move(r));
+  }
+  // engine.rs:1637-1726 purge(FALSE) cut pass, reevaluate remainders, purge(TRUE)
+  {
+    Slots arr;
+    for (auto& r : results) arr.emplace_back(std::move(r));
+    results.clear();
+    size_t extra_start = purge_common_endpoints(arr, false);
+    const uint8_t* q_core = query.data() + 1;
+    const size_t q_core_len = query.size() - 2;
+    const uint8_t* src_core = subject_plus.data() + 1;
+    const uint8_t* rc_core = subj_rc.data() + 1;
+    const size_t s_core_len = subject_plus.size() - 2;
+    for (size_t i = extra_start; i < arr.size(); ++i) {
+      if (!arr[i]) continue;
+      AlignResult r = std::move(*arr[i]);
+      arr[i].reset();
+      bool minus = r.hsp.strand == MINUS;
+      size_t a_start = r.hsp.q_start;
+      size_t b_start = minus ? (sa_len - r.hsp.s_end) : r.hsp.s_start;
+      const uint8_t* b_core = minus ? rc_core : src_core;
+      const std::vector<uint8_t>& bmask = minus ? subj.n_mask_rc : n_mask;
+      std::vector<uint8_t> qsq, ssq;
+      extract_aligned(q_core, q_core_len, b_core, s_core_len, a_start, b_start, r.hsp.ops, bmask, qsq, ssq);
+      if (reevaluate_gapped(r.hsp, qsq, ssq, m, p.gap_open, p.gap_extend, p.gapped_cutoff(), minus, q_core, q_core_len,
+                            a_start, b_core, s_core_len, b_start))
+        continue;
+      size_t a2 = r.hsp.q_start;
+      size_t b2 = minus ? (sa_len - r.hsp.s_end) : r.hsp.s_start;
+      std::vector<uint8_t> q2, s2;
+      extract_aligned(q_core, q_core_len, b_core, s_core_len, a2, b2, r.hsp.ops, bmask, q2, s2);
+      r.stats = compute_align_stats(to_iupac(q2), to_iupac(s2));
+      r.hsp.q_seq = std::move(q2);
+      r.hsp.s_seq = std::move(s2);
+      arr[i] = std::move(r);
+    }
+    Slots arr2;
+    for (auto& s : arr)
+      if (s) arr2.emplace_back(std::move(s));
+    purge_common_endpoints(arr2, true);
+    for (auto& s : arr2)
+      if (s) results.push_back(std::move(*s));
+  }
+  // engine.rs:1731-1759 Phase 2c
+  std::stable_sort(results.begin(), results.end(), [&](const AlignResult& A, const AlignResult& B) {
+    const Hsp &a = A.hsp, &b = B.hsp;
+    uint32_t qoa = a.strand == PLUS ? a.q_start : qa_len - a.q_end, qob = b.strand == PLUS ? b.q_start : qa_len - b.q_end;
+    uint32_t qea = a.strand == PLUS ? a.q_end : qa_len - a.q_start, qeb = b.strand == PLUS ? b.q_end : qa_len - b.q_start;
+    if (a.score != b.score) return a.score > b.score;
+    if (a.s_start != b.s_start) return a.s_start < b.s_start;
+    if (a.s_end != b.s_end) return a.s_end > b.s_end;
+    if (qoa != qob) return qoa < qob;
+    return qea > qeb;
+  });
+  IntervalTree fp(0, qa_len + 1, 0, s_len + 1), fm(0, qa_len + 1, 0, s_len + 1);
+  std::vector<AlignResult> out;
+  for (auto& r : results) {
+    bool plus = r.hsp.strand == PLUS;
+    IntervalTree& t = plus ? fp : fm;
+    if (t.contains(r.hsp.q_start, r.hsp.q_end, r.hsp.s_start, r.hsp.s_end, r.hsp.score, plus, MIN_DIAG_SEP)) continue;
+    t.add_simple(ITreeHsp{r.hsp.q_start, r.hsp.q_end, r.hsp.s_start, r.hsp.s_end, r.hsp.score});
+    out.push_back(std::move(r));
+  }
+  return out;
+}
+
+// ------------------------------------------------------------ drivers --------
+std::vector<AlignResult> search_one_subject(const QueryLookup& ql, const std::vector<uint8_t>& query,
+                                            const std::vector<uint8_t>& query_rc, const std::string& query_id,
+                                            const PreparedSubject& subj, const std::string& subject_id,
+                                            const Params& p, const Matrix& m, Taps* taps) {
+  // engine.rs:734-790, run_gapped_phase :1905-1957
+  const uint32_t q_len = query.size() >= 2 ? (uint32_t)query.size() - 2 : 0;
+  const uint32_t s_len = subj.plus.size() >= 2 ? (uint32_t)subj.plus.size() - 2 : 0;
+  if (q_len < (uint32_t)p.word_size || s_len < (uint32_t)p.word_size) return {};
+  std::vector<UngappedRecord> ung;
+  collect_ungapped(ql, query, query_rc, query, subj, p, m, ung, taps);
+  sort_ungapped_like_ncbi(ung, q_len, s_len);
+  if (taps && taps->ungapped) *taps->ungapped = ung;
+  if (ung.empty()) return {};
+  DpCounters* ctr = taps ? taps->counters : nullptr;
+  std::vector<PrelimHsp> pre =
+      run_phase2a(query, query_rc, subj.plus, q_len, s_len, ung, p, m, (uint32_t)ql.lut.lut_width, ctr);
+  if (taps && taps->prelims) *taps->prelims = pre;
+  return traceback_phase(query, query_rc, query_id, subj, subject_id, std::move(pre), p, m, ctr);
+}
+
+void apply_mask_level(std::vector<AlignResult>& r, int mask_level, const std::vector<std::string>& names) {
+  // engine.rs:863-974
+  if (mask_level >= 100) return;
+  if (names.empty()) {
+    uint32_t qa = r.empty() ? 0 : r[0].hsp.q_len;
+    std::sort(r.begin(), r.end(), [&](const AlignResult& A, const AlignResult& B) {
+      const Hsp &a = A.hsp, &b = B.hsp;
+      uint32_t qoa = a.strand == PLUS ? a.q_start : qa - a.q_end, qob = b.strand == PLUS ? b.q_start : qa - b.q_end;
+      uint32_t qea = a.strand == PLUS ? a.q_end : qa - a.q_start, qeb = b.strand == PLUS ? b.q_end : qa - b.q_start;
+      if (a.score != b.score) return a.score > b.score;
+      if (a.s_start != b.s_start) return a.s_start < b.s_start;
+      if (a.s_end != b.s_end) return a.s_end > b.s_end;
+      if (qoa != qob) return qoa < qob;
+      return qea > qeb;
+    });
+  } else {
+    std::map<std::string, size_t> oid;
+    for (size_t i = 0; i < names.size(); ++i) oid.emplace(names[i], i);
+    auto getoid = [&](const std::string& s) { auto it = oid.find(s); return it == oid.end() ? (size_t)0 : it->second; };
+    std::sort(r.begin(), r.end(), [&](const AlignResult& A, const AlignResult& B) {
+      const Hsp &a = A.hsp, &b = B.hsp;
+      if (a.score != b.score) return a.score > b.score;
+      size_t oa = getoid(A.subject_id), ob = getoid(B.subject_id);
+      if (oa != ob) return oa > ob;
+      if (a.s_start != b.s_start) return a.s_start < b.s_start;
+      if (a.s_end != b.s_end) return a.s_end > b.s_end;
+      uint32_t qoa = a.strand == PLUS ? a.q_start : a.q_len - a.q_end, qob = b.strand == PLUS ? b.q_start : b.q_len - b.q_end;
+      if (qoa != qob) return qoa < qob;
+      uint32_t qea = a.strand == PLUS ? a.q_end : a.q_len - a.q_start, qeb = b.strand == PLUS ? b.q_end : b.q_len - b.q_start;
+      return qea > qeb;
+    });
+  }
+  struct Acc { uint32_t qs, qe; int32_t sc; };
+  std::vector<Acc> acc;  // sorted by qs; equivalent to engine.rs:211-246 SpanIndex lookups
+  uint32_t max_span = 0;
+  std::vector<char> keep(r.size(), 1);
+  const bool touching = mask_level == 0;
+  for (size_t i = 0; i < r.size(); ++i) {
+    uint32_t qs = r[i].hsp.q_start, qe = r[i].hsp.q_end;
+    if (r[i].hsp.strand == MINUS) { qs += 1; qe += 1; }
+    int32_t sc = r[i].hsp.score;
+    int64_t span = (int64_t)qe - (int64_t)qs;
+    if (span == 0) { keep[i] = 0; continue; }
+    uint32_t hi = touching ? qe : qe - 1;
+    uint32_t lo = qs > max_span ? qs - max_span : 0;
+    auto it = std::lower_bound(acc.begin(), acc.end(), lo, [](const Acc& a, uint32_t v) { return a.qs < v; });
+    bool masked = false;
+    for (; it != acc.end() && it->qs <= hi; ++it) {
+      if (it->qe < qs || (it->qe == qs && !touching)) continue;
+      if (it->sc < sc) continue;
+      int64_t ovlp = (int64_t)std::min(it->qe, qe) - (int64_t)std::max(it->qs, qs);
+      int64_t perc = (int64_t)(100.0 * ((double)ovlp / (double)span));
+      if (perc >= mask_level) { masked = true; break; }
+    }
+    if (masked) { keep[i] = 0; continue; }
+    auto pos = std::upper_bound(acc.begin(), acc.end(), qs, [](uint32_t v, const Acc& a) { return v < a.qs; });
+    acc.insert(pos, Acc{qs, qe, sc});
+    max_span = std::max(max_span, qe - qs);
+  }
+  std::vector<AlignResult> out;
+  for (size_t i = 0; i < r.size(); ++i)
+    if (keep[i]) out.push_back(std::move(r[i]));
+  r.swap(out);
+}
+
+static int score_compare(const Hsp& a, const Hsp& b) {  // hits.rs:208-216, <0 if a first
+  uint32_t qoa = a.strand == PLUS ? a.q_start : a.q_len - a.q_end, qob = b.strand == PLUS ? b.q_start : b.q_len - b.q_end;
+  uint32_t qea = a.strand == PLUS ? a.q_end : a.q_len - a.q_start, qeb = b.strand == PLUS ? b.q_end : b.q_len - b.q_start;
+  if (a.score != b.score) return a.score > b.score ? -1 : 1;
+  if (a.s_start != b.s_start) return a.s_start < b.s_start ? -1 : 1;
+  if (a.s_end != b.s_end) return a.s_end > b.s_end ? -1 : 1;
+  if (qoa != qob) return qoa < qob ? -1 : 1;
+  if (qea != qeb) return qea > qeb ? -1 : 1;
+  return 0;
+}
+
+void sort_hit_list_order(std::vector<AlignResult>& r, const std::vector<std::string>& names) {  // engine.rs:998-1017
+  std::map<std::string, size_t> oid;
+  for (size_t i = 0; i < names.size(); ++i) oid.emplace(names[i], i);
+  std::map<std::string, std::pair<int32_t, size_t>> rank;
+  for (auto& x : r) {
+    auto it = oid.find(x.subject_id);
+    size_t o = it == oid.end() ? 0 : it->second;
+    auto ins = rank.emplace(x.subject_id, std::make_pair(INT32_MIN, o));
+    ins.first->second.first = std::max(ins.first->second.first, x.hsp.score);
+  }
+  std::stable_sort(r.begin(), r.end(), [&](const AlignResult& a, const AlignResult& b) {
+    auto ra = rank[a.subject_id], rb = rank[b.subject_id];
+    if (ra.first != rb.first) return ra.first > rb.first;
+    if (ra.second != rb.second) return ra.second > rb.second;
+    return score_compare(a.hsp, b.hsp) < 0;
+  });
+}
+
+std::vector<AlignResult> search_query(const std::vector<uint8_t>& query, const std::string& query_id,
+                                      const std::vector<PreparedSubject>& subjects,
+                                      const std::vector<std::string>& names, const Params& p, const Matrix& m,
+                                      int threads) {
+  // rmblastn/src/main.rs:1133-1234 (single chunk)
+  QueryLookup ql = build_query_lookup(query, p);
+  std::vector<uint8_t> qrc = revcomp(query);
+  std::vector<std::vector<AlignResult>> per(subjects.size());
+  if (threads <= 1) {
+    for (size_t i = 0; i < subjects.size(); ++i)
+      per[i] = search_one_subject(ql, query, qrc, query_id, subjects[i], names[i], p, m, nullptr);
+  } else {
+    std::vector<std::thread> pool;
+    std::atomic<size_t> next{0};
+    for (int t = 0; t < threads; ++t)
+      pool.emplace_back([&]() {
+        for (size_t i; (i = next.fetch_add(1)) < subjects.size();)
+          per[i] = search_one_subject(ql, query, qrc, query_id, subjects[i], names[i], p, m, nullptr);
+      });
+    for (auto& th : pool) th.join();
+  }
+  std::vector<AlignResult> res;
+  for (auto& v : per)
+    for (auto& x : v) res.push_back(std::move(x));
+  apply_mask_level(res, p.mask_level, names);
+  std::sort(res.begin(), res.end(), [](const AlignResult& a, const AlignResult& b) {
+    if (a.subject_id != b.subject_id) return a.subject_id < b.subject_id;
+    if (a.hsp.strand != b.hsp.strand) return a.hsp.strand < b.hsp.strand;  // "+" < "-"
+    return a.hsp.q_start < b.hsp.q_start;
+  });
+  return res;
+}
+
+std::string format_tabular(const AlignResult& r, const std::vector<std::string>& fields) {  // output.rs:294-363
+  const Hsp& h = r.hsp;
+  const AlignStats& st = r.stats;
+  uint32_t ss = h.strand == PLUS ? h.s_start + 1 : h.s_end;
+  uint32_t se = h.strand == PLUS ? h.s_end : h.s_start + 1;
+  std::string out;
+  char buf[64];
+  for (size_t i = 0; i < fields.size(); ++i) {
+    if (i) out += '\t';
+    const std::string& f = fields[i];
+    if (f == "score") out += std::to_string(h.score);
+    else if (f == "perc_sub") { snprintf(buf, sizeof buf, "%.2f", (double)st.perc_sub); out += buf; }
+    else if (f == "perc_query_gap") { snprintf(buf, sizeof buf, "%.2f", (double)st.perc_query_gap); out += buf; }
+    else if (f == "perc_db_gap") { snprintf(buf, sizeof buf, "%.2f", (double)st.perc_db_gap); out += buf; }
+    else if (f == "qseqid") out += r.query_id;
+    else if (f == "qstart") out += std::to_string(h.q_start + 1);
+    else if (f == "qend") out += std::to_string(h.q_end);
+    else if (f == "qlen") out += std::to_string(h.q_len);
+    else if (f == "sstrand") out += h.strand == PLUS ? "plus" : "minus";
+    else if (f == "sseqid") out += r.subject_id;
+    else if (f == "sstart") out += std::to_string(ss);
+    else if (f == "send") out += std::to_string(se);
+    else if (f == "slen") out += std::to_string(h.s_len);
+    else if (f == "kdiv") { snprintf(buf, sizeof buf, "%.2f", st.kdiv); out += buf; }
+    else if (f == "cpg_kdiv") { snprintf(buf, sizeof buf, "%.2f", st.cpg_kdiv); out += buf; }
+    else if (f == "transi") out += std::to_string(st.transversions);   // output.rs:338 quirk
+    else if (f == "transv") out += std::to_string(st.cpg_sites);       // output.rs:339 quirk
+    else if (f == "cpg_sites") out += std::to_string(st.transitions);  // output.rs:340 quirk
+    else if (f == "qseq") out += to_iupac(h.q_seq);
+    else if (f == "sseq") out += to_iupac(h.s_seq);
+    else if (f == "evalue") out += "1.0";
+    else if (f == "bitscore") out += " 0.0";
+  }
+  return out;
+}
+
+}  // namespace orc
