@@ -1,0 +1,149 @@
+/*
+ * rmblast_hip.h — C ABI of librmblast_hip.so, the MI355X (gfx950) search path
+ * of rmblastn.
+ *
+ * The library replaces the per-(query, subject) search of the reference's
+ * rmblast-lib search API (rmblast-lib/src/search/mod.rs:9-21):
+ *   build_query_lookup       (rmblast-lib/src/search/engine.rs:606)
+ *   search_with_query_lookup (rmblast-lib/src/search/engine.rs:715-729)
+ * driven over every subject by rmblastn's search_db_parallel
+ * (rmblastn/src/main.rs:1133-1234).  One call searches a batch of query
+ * records against the loaded library (both strands) and returns, per
+ * (query, subject) pair, the final HSPs of the reference's traceback phase
+ * (engine.rs:1438-1762: after complexity adjustment, the common-endpoint purge
+ * and the final containment pass) with their edit scripts.  Mask level,
+ * output ordering, alignment statistics and formatting stay with the caller
+ * (engine.rs:863-1017, stats.rs, output.rs).
+ *
+ * Conventions
+ *  - Sequences are BLASTNA codes (A0 C1 G2 T3 R4 Y5 M6 K7 W8 S9 B10 D11 H12
+ *    V13 N14; encoding.rs:19-54) WITHOUT the sentinel bytes; the library adds
+ *    them.  Library subjects are given with their original ambiguity codes;
+ *    the library resolves them for seeding exactly as FastaDb does
+ *    (seq/fasta_db.rs:86-105) and keeps the originals for traceback.
+ *  - Coordinates are 0-based, half-open; subject coordinates of minus-strand
+ *    HSPs are on the plus strand (hits.rs:165-192).
+ *  - Every function returns RMB_OK (0) or an error code; the message of the
+ *    last error is available from rmb_last_error().  No exception or abort
+ *    crosses this boundary (the convention of the reference's only C ABI,
+ *    rmblast-stats/rmstats/csrc/alp_shim.cpp:44-117).
+ *  - One context per GPU; calls on one context must be serialised by the
+ *    caller.  The caller owns all host buffers; the library owns device memory
+ *    for the life of the context.
+ */
+#ifndef RMBLAST_HIP_H
+#define RMBLAST_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RMB_OK 0
+#define RMB_ERR_ARG 1       /* invalid argument (message says which) */
+#define RMB_ERR_DEVICE 2    /* HIP runtime or kernel failure */
+#define RMB_ERR_STATE 3     /* call out of order (e.g. search before library load) */
+#define RMB_ERR_CAPACITY 4  /* a device work limit was exceeded; message names it */
+
+/* Search parameters: the rmblastn options in scope (rmblast-lib/src/options.rs:12-104,
+ * rmblastn/src/main.rs:128-281).  dust must be 0: DUST masking is done by the
+ * caller on the query before the call (filter/dust.rs), or not at all. */
+typedef struct rmb_params {
+  int32_t gap_open;
+  int32_t gap_extend;
+  int32_t word_size;
+  int32_t xdrop_ungap;
+  int32_t xdrop_gap;
+  int32_t xdrop_gap_final;
+  int32_t min_raw_gapped_score;
+  int32_t ungapped_cutoff; /* < 0: NCBI cutoff_score_max (options.rs:80-82); else the KA-derived value */
+  int32_t complexity_adjust;
+  int32_t dust;
+} rmb_params;
+
+/* Substitution matrix in BLASTNA order, as ScoreMatrix holds it after parsing
+ * (matrix.rs:62-72): scores[q][s], base frequencies (A,C,G,T at 0..3) and the
+ * lambda used by complexity adjustment (matrix.rs:306-354). */
+typedef struct rmb_matrix {
+  int32_t scores[16][16];
+  double freqs[16];
+  double lambda;
+} rmb_matrix;
+
+/* One final HSP (hits.rs:165-192 Hsp without the aligned strings). */
+typedef struct rmb_hsp {
+  uint32_t query;     /* index into the query batch */
+  uint32_t subject;   /* index into the loaded library */
+  int32_t score;
+  uint32_t q_start, q_end;
+  uint32_t s_start, s_end;
+  uint32_t strand;    /* 0 plus, 1 minus */
+  uint32_t ops_offset; /* first edit op in the ops array */
+  uint32_t ops_count;
+} rmb_hsp;
+
+/* Edit op: low 2 bits = kind (0 Sub, 1 GapInQuery, 2 GapInSubject; hits.rs:123-134),
+ * upper 30 bits = run length. Ops are in forward query order. */
+typedef uint32_t rmb_op;
+
+typedef struct rmb_ctx rmb_ctx;
+
+/* Create a context on HIP device `device`.  Replaces the per-search state the
+ * reference builds in rmblastn main (main.rs:516-700): parameters + matrix. */
+int rmb_ctx_create(int device, const rmb_params* params, const rmb_matrix* matrix, rmb_ctx** out);
+int rmb_ctx_destroy(rmb_ctx* ctx);
+
+/* Copy of the last error message (NUL-terminated, truncated to len). */
+int rmb_last_error(const rmb_ctx* ctx, char* buf, size_t len);
+
+/* Load the subject library (the TE families): n sequences with their original
+ * ambiguity codes.  Replaces SubjectDb::open + get_full_sequence_blastna +
+ * get_prepared (seq/mod.rs:155-290, seq/fasta_db.rs:86-132). */
+int rmb_library_load(rmb_ctx* ctx, uint32_t n, const uint8_t* const* seqs, const uint32_t* lens);
+
+/* Search a batch of query records against every loaded subject, both strands.
+ * Replaces build_query_lookup (engine.rs:606) + search_with_query_lookup over
+ * all subjects (main.rs:1205-1223).  Queries longer than 2^31-1 are refused.
+ * Results are held by the context until the next search. */
+int rmb_search(rmb_ctx* ctx, uint32_t n_queries, const uint8_t* const* queries, const uint32_t* lens);
+
+/* Two-call result retrieval: sizes, then copy into caller buffers. */
+int rmb_results_size(const rmb_ctx* ctx, uint64_t* n_hsps, uint64_t* n_ops);
+int rmb_results_copy(const rmb_ctx* ctx, rmb_hsp* hsps, rmb_op* ops);
+
+/* Stage counters of the last search (the reference's diagnostics counters,
+ * engine.rs:28-37 and gapped.rs:25-36): [0] seeds that reached the diagonal
+ * hash, [1] ungapped hits kept, [2] preliminary DPs run, [3] traceback DPs run,
+ * [4] score-only DP cells, [5] traceback DP cells, [6] final HSPs,
+ * [7] ungapped extension steps.  Stage wall times in microseconds:
+ * [8] lookup build, [9] seeding + ungapped, [10] preliminary gapped,
+ * [11] traceback, [12] total. */
+int rmb_stats(const rmb_ctx* ctx, uint64_t* out, int n);
+
+/* ---- parity taps (one stage at a time; same inputs as rmb_search) -------- */
+
+/* Seeding + diagonal dedup + ungapped extension for query 0 of a one-query
+ * batch against subject `subject`: the sorted ungapped hits the reference's
+ * search_with_query_lookup_ungapped collects (engine.rs:817-832, order of
+ * sort_ungapped_like_ncbi :1280-1309) in the record frame: per hit
+ * {q_seed, s_seed, q_start, q_end, s_start, s_end, score, strand}.
+ * Two-call: pass out = NULL to get the count in *n. */
+int rmb_tap_ungapped(rmb_ctx* ctx, const uint8_t* query, uint32_t q_len, uint32_t subject, uint32_t* out,
+                     uint64_t* n);
+
+/* Score-only X-drop half alignment (gapped.rs:124-337, align_ex_score_only):
+ * out = {score, a_len, b_len}. */
+int rmb_tap_align_score_only(rmb_ctx* ctx, const uint8_t* a, uint32_t m, const uint8_t* b, uint32_t n, int32_t xdrop,
+                             int32_t reverse, int64_t out[3]);
+
+/* X-drop half alignment with traceback (gapped.rs:450-757, align_ex):
+ * out = {score, a_len, b_len, n_ops}; ops receives up to max_ops ops. */
+int rmb_tap_align_ex(rmb_ctx* ctx, const uint8_t* a, uint32_t m, const uint8_t* b, uint32_t n, int32_t xdrop,
+                     int32_t reverse, int64_t out[4], rmb_op* ops, uint32_t max_ops);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RMBLAST_HIP_H */

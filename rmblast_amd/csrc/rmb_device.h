@@ -1,0 +1,82 @@
+// Shared device/host record layouts of librmblast_hip.
+#pragma once
+#include <cstdint>
+
+namespace rmb {
+
+// Phase-1 launch geometry.  One workgroup owns one (record, subject) work item
+// at a time; SEEDCAP seeds are staged in LDS per pass; the 512 diagonal-hash
+// buckets are owned two per thread.
+constexpr int kP1Threads = 256;
+constexpr int kSeedCap = 1024;
+constexpr int kHashLds = 6;     // cells per bucket held in LDS
+constexpr int kHashGlobal = 250; // further cells per bucket in the workgroup's global spill area
+constexpr int kBuckets = 512;
+
+struct RecDesc {        // one query record (a chunk of the genome)
+  uint64_t q_base;      // offset of its sentinel-wrapped arrays in the query pools
+  uint64_t cell_base;   // offset of its (4^lw + 1) cell starts in the cell pool
+  uint32_t q_len;       // bases, without sentinels
+  uint32_t pad;
+};
+
+struct SubjDesc {       // one library sequence
+  uint64_t s_base;      // offset of its sentinel-wrapped arrays in the subject pools
+  uint32_t s_len;
+  uint32_t pad;
+};
+
+struct P1Params {
+  int32_t word_size, lut_width, scan_step, xdrop_ungap, cutoff;
+  uint32_t n_subj;
+  uint64_t n_items;
+  uint64_t out_cap;
+};
+
+struct UngHit {         // an ungapped hit at or above the cutoff, record frame
+  uint32_t item;        // rec * n_subj + subj
+  uint32_t q_seed, s_seed;
+  uint32_t q_start, q_end, s_start, s_end;
+  int32_t score;
+  uint32_t strand;      // 0 plus, 1 minus (record frame: forward query, RC subject)
+};
+
+// Counters accumulated by the kernels (u64 atomics).
+enum Counter { C_SEED_EVALS = 0, C_UNG_HITS = 1, C_UNG_STEPS = 2, C_DP_CELLS = 3, C_NCOUNTERS = 8 };
+
+// Error bits written by kernels.
+enum ErrBit : uint32_t {
+  E_HASH_FULL = 1u, E_OUT_FULL = 2u, E_BAND_WIDE = 4u, E_TB_SPACE = 8u
+};
+
+struct DpTask {         // one half of an X-drop alignment, lane-serial
+  const uint8_t* a;     // outer sequence (rows)
+  const uint8_t* b;     // inner sequence (columns, the band)
+  uint32_t m, n;
+  int32_t xdrop;
+  uint32_t reverse;
+};
+
+struct DpResult {
+  int32_t score;
+  uint32_t a_len, b_len;
+  uint32_t flags;       // ErrBit E_BAND_WIDE when the band outgrew the ring
+  uint64_t cells;       // DP cells evaluated (the reference's TOTAL_DP_CELLS unit)
+  uint32_t tb_bytes;    // traceback bytes through the best row
+  uint32_t tb_rows;     // traceback rows through the best row (row 0 included)
+};
+
+struct TbTask {         // traceback pass: a DpTask plus its storage
+  DpTask t;
+  uint64_t flat_off;    // byte offset of its traceback bytes
+  uint64_t row_off;     // u32-pair offset of its row table
+  uint64_t ops_off;     // u32 offset of its op buffer
+  uint32_t stop_row;    // best_a from the score pass
+  uint32_t best_b;
+  uint32_t ops_cap;
+  uint32_t pad;
+};
+
+struct TbResult { uint32_t n_ops; uint32_t flags; };
+
+}  // namespace rmb

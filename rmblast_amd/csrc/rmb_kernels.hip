@@ -1,0 +1,679 @@
+// HIP kernels of librmblast_hip for gfx950 (wave64, 160 KB LDS per CU).
+//
+//  k_lut_keys      lookup-table build, stage 1: one (record, word, offset)
+//                  key per window of the two-strand query buffer
+//                  (blast/nalookup.rs:128-181, blast/mblookup.rs:367-438)
+//  k_phase1        seeding + exact diagonal-hash dedup + ungapped X-drop for
+//                  (record, subject) work items pulled from a device queue
+//                  (search/engine.rs:1063-1277, diag_hash.rs, ungapped.rs)
+//  k_dp_score      lane-serial score-only X-drop half alignment
+//                  (search/gapped.rs:124-337)
+//  k_dp_traceback  the same recurrence with 1-byte traceback cells and the
+//                  traceback walk (search/gapped.rs:450-757)
+#include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include "rmb_device.h"
+
+namespace rmb {
+
+static constexpr int32_t kMinInt = INT32_MIN / 2;  // gapped.rs:47
+__constant__ uint8_t c_complement[16] = {3, 2, 1, 0, 5, 4, 7, 6, 8, 9, 13, 12, 11, 10, 14, 15};
+
+// --------------------------------------------------------------- helpers ----
+__device__ __forceinline__ uint32_t block_incl_scan256(uint32_t v, uint32_t* wsum) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(v, o, 64);
+    if (lane >= o) v += y;
+  }
+  if (lane == 63) wsum[wid] = v;
+  __syncthreads();
+  uint32_t add = 0;
+  for (int w = 0; w < wid; ++w) add += wsum[w];
+  __syncthreads();
+  return v + add;
+}
+
+// Ungapped X-drop from (q_off, s_off) over sentinel-wrapped sequences:
+// left side fixed X, right side adaptive X (ungapped.rs:223-306).
+struct UngOut { int32_t score; uint32_t q_beg, q_end, steps; };
+__device__ __forceinline__ UngOut ungapped_ext(const uint8_t* __restrict__ q, int32_t q_real_len,
+                                               const uint8_t* __restrict__ s, int32_t s_real_len, int32_t q_off,
+                                               int32_t s_off, const int* mat, int32_t xdrop) {
+  const int32_t x = -xdrop;
+  int32_t score = 0, sum = 0, q_beg = q_off;
+  uint32_t steps = 0;
+  for (int32_t qi = q_off - 1, si = s_off - 1; qi >= 0 && si >= 0; --qi, --si) {
+    const uint8_t qb = q[1 + qi], sb = s[1 + si];
+    if (qb == 15 || sb == 15) break;
+    ++steps;
+    sum += mat[(qb << 4) | sb];
+    if (sum > 0) {
+      q_beg = qi;
+      score += sum;
+      sum = 0;
+    } else if (sum < x) {
+      break;
+    }
+  }
+  int32_t q_end = q_off, xc = x;
+  sum = 0;
+  for (int32_t qi = q_off, si = s_off; qi < q_real_len && si < s_real_len; ++qi, ++si) {
+    const uint8_t qb = q[1 + qi], sb = s[1 + si];
+    if (qb == 15 || sb == 15) break;
+    ++steps;
+    sum += mat[(qb << 4) | sb];
+    if (sum > 0) {
+      q_end = qi + 1;
+      score += sum;
+      xc = (-score > x) ? -score : x;
+      sum = 0;
+    } else if (sum < xc) {
+      break;
+    }
+  }
+  return UngOut{score, (uint32_t)q_beg, (uint32_t)q_end, steps};
+}
+
+// --------------------------------------------------------- lookup build -----
+// One key per window start p of record r's two-strand buffer [fwd | N | rc]
+// (engine.rs:630-649).  Valid windows lie inside one strand, hold no code >= 4
+// and belong to a strand of at least word_size bases; invalid ones get the
+// sort-last key.  Small tables list offsets ascending per cell, megablast
+// tables descending (mblookup.rs:431-432): the radix sort is stable, so the
+// value order fed in decides it.
+__global__ void k_lut_keys(const RecDesc* __restrict__ recs, uint32_t n_recs, const uint64_t* __restrict__ elem_base,
+                           const uint8_t* __restrict__ qext, int lw, int word_size, int descending,
+                           uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t* __restrict__ counts,
+                           uint64_t invalid_key, uint64_t total) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  // locate the record by binary search over elem_base (n_recs + 1 entries)
+  uint32_t lo = 0, hi = n_recs;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (elem_base[mid] <= e) lo = mid; else hi = mid;
+  }
+  const uint32_t r = lo;
+  const RecDesc R = recs[r];
+  const int64_t n = R.q_len;
+  const int64_t k = (int64_t)(e - elem_base[r]);  // 0 .. 2n
+  const int64_t p = descending ? (2 * n - k) : k;
+  keys[e] = invalid_key;
+  vals[e] = (uint32_t)p;
+  if (n < word_size) return;
+  const int64_t last = p + lw - 1;
+  const bool in_fwd = last <= n - 1;
+  const bool in_rc = p >= n + 1 && last <= 2 * n;
+  if (!in_fwd && !in_rc) return;
+  const uint8_t* Q = qext + R.q_base;
+  uint32_t code = 0;
+  for (int i = 0; i < lw; ++i) {
+    const int64_t pp = p + i;
+    const uint8_t b = pp < n ? Q[1 + pp] : c_complement[Q[2 * n + 1 - pp] & 15];
+    if (b & 0xFC) return;
+    code = (code << 2) | b;
+  }
+  keys[e] = ((uint64_t)r << (2 * lw)) | code;
+  atomicAdd(&counts[((uint64_t)r << (2 * lw)) + code], 1u);
+}
+
+// ------------------------------------------------------------ Phase 1 -------
+// One workgroup per (record, subject) item at a time, items from a device
+// queue.  Per pass it stages up to kSeedCap seeds of the reference scan order
+// (subject offset ascending, then cell order) in LDS, extends every seed
+// speculatively (the extension is a pure function of the seed), then replays
+// the reference's 512-bucket diagonal hash exactly: buckets are independent,
+// so each thread replays its two buckets' seeds in scan order.  Seeds the
+// replay keeps and that reach the cutoff are emitted.
+enum SeedFlag : uint8_t { F_VALID = 1, F_HIT = 2, F_EMIT = 4 };
+
+__global__ __launch_bounds__(kP1Threads) void k_phase1(
+    P1Params P, const RecDesc* __restrict__ recs, const SubjDesc* __restrict__ subjs,
+    const uint8_t* __restrict__ qfwd, const uint8_t* __restrict__ qrc, const uint8_t* __restrict__ qext,
+    const uint8_t* __restrict__ splus, const uint8_t* __restrict__ src, const uint32_t* __restrict__ cells,
+    const uint32_t* __restrict__ entries, const int32_t* __restrict__ gmat, UngHit* __restrict__ out,
+    unsigned long long* __restrict__ out_count, uint2* __restrict__ spill, unsigned long long* __restrict__ queue,
+    unsigned long long* __restrict__ counters, uint32_t* __restrict__ err) {
+  __shared__ int mat[256];
+  __shared__ uint32_t s_cum[kP1Threads], s_cst[kP1Threads], wsum[4];
+  __shared__ uint32_t seed_s[kSeedCap], seed_q[kSeedCap];
+  __shared__ int32_t sd_diag[kSeedCap];
+  __shared__ uint32_t sd_chk[kSeedCap], sd_level[kSeedCap];
+  __shared__ uint16_t sd_steps[kSeedCap], blist[kSeedCap];
+  __shared__ uint8_t sd_flag[kSeedCap];
+  __shared__ uint32_t bcnt[kBuckets], bst[kBuckets];
+  __shared__ int32_t hdiag[kBuckets * kHashLds];
+  __shared__ uint32_t hlev[kBuckets * kHashLds];
+  __shared__ uint16_t hcnt[kBuckets];
+  __shared__ unsigned long long s_item;
+  __shared__ uint32_t s_next_slot, s_next_k;
+
+  const int tid = threadIdx.x;
+  for (int i = tid; i < 256; i += kP1Threads) mat[i] = gmat[i];
+  uint2* my_spill = spill + (size_t)blockIdx.x * kBuckets * kHashGlobal;
+  const int32_t ws = P.word_size, lw = P.lut_width, step = P.scan_step;
+  const int32_t ext_needed = ws - lw;
+  const int32_t window = 1 - ws;
+  unsigned long long n_evals = 0, n_steps = 0, n_hits = 0;
+
+  for (;;) {
+    __syncthreads();
+    if (tid == 0) s_item = atomicAdd(queue, 1ull);
+    __syncthreads();
+    const unsigned long long item = s_item;
+    if (item >= P.n_items) break;
+    const uint32_t r = (uint32_t)(item / P.n_subj), f = (uint32_t)(item % P.n_subj);
+    const RecDesc R = recs[r];
+    const SubjDesc S = subjs[f];
+    const int32_t q_len = (int32_t)R.q_len, s_len = (int32_t)S.s_len;
+    if (q_len < ws || s_len < ws) continue;
+    const uint8_t* Q = qfwd + R.q_base;
+    const uint8_t* QR = qrc + R.q_base;
+    const uint8_t* QE = qext + R.q_base;
+    const uint8_t* SP = splus + S.s_base;
+    const uint8_t* SR = src + S.s_base;
+    const uint32_t* CS = cells + R.cell_base;
+    const int32_t end_range = s_len - lw;
+    const uint32_t n_slots = (uint32_t)(end_range / step) + 1;
+    const int32_t c1 = q_len + 1;
+    for (int b = tid; b < kBuckets; b += kP1Threads) hcnt[b] = 0;
+    uint32_t slot0 = 0, kcur = 0;
+    __syncthreads();
+    while (slot0 < n_slots) {
+      // ---- A: hit counts of the next kP1Threads scan positions
+      uint32_t cnt = 0, cst = 0;
+      const uint32_t slot = slot0 + tid;
+      if (slot < n_slots) {
+        const int32_t s = (int32_t)slot * step;
+        uint32_t idx = 0;
+        for (int k = 0; k < lw; ++k) idx = (idx << 2) | (SP[1 + s + k] & 3);
+        const uint32_t a = CS[idx], b = CS[idx + 1];
+        cst = a;
+        cnt = b - a;
+      }
+      if (tid == 0) { cst += kcur; cnt -= kcur; }
+      s_cst[tid] = cst;
+      const uint32_t inc = block_incl_scan256(cnt, wsum);
+      s_cum[tid] = inc;
+      __syncthreads();
+      const uint32_t total = s_cum[kP1Threads - 1];
+      const uint32_t take = total < (uint32_t)kSeedCap ? total : (uint32_t)kSeedCap;
+      // ---- B: materialise seeds in scan order
+      for (uint32_t i = tid; i < take; i += kP1Threads) {
+        int lo = 0, hi = kP1Threads - 1;
+        while (lo < hi) {
+          int mid = (lo + hi) >> 1;
+          if (s_cum[mid] > i) hi = mid; else lo = mid + 1;
+        }
+        const uint32_t prev = lo ? s_cum[lo - 1] : 0;
+        seed_s[i] = (slot0 + lo) * (uint32_t)step;
+        seed_q[i] = entries[s_cst[lo] + (i - prev)];
+      }
+      if (tid == 0) {
+        if (total <= (uint32_t)kSeedCap) {
+          s_next_slot = slot0 + kP1Threads;
+          s_next_k = 0;
+        } else {
+          int lo = 0, hi = kP1Threads - 1;
+          while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (s_cum[mid] > take) hi = mid; else lo = mid + 1;
+          }
+          const uint32_t prev = lo ? s_cum[lo - 1] : 0;
+          s_next_slot = slot0 + lo;
+          s_next_k = (take - prev) + (lo == 0 ? kcur : 0);
+        }
+      }
+      for (int b = tid; b < kBuckets; b += kP1Threads) bcnt[b] = 0;
+      __syncthreads();
+      // ---- C: exact word extension, diagonal, speculative ungapped extension
+      for (uint32_t i = tid; i < take; i += kP1Threads) {
+        const int32_t s_fwd = (int32_t)seed_s[i];
+        const int32_t q0c = (int32_t)seed_q[i];
+        uint8_t flag = 0;
+        int32_t diag = 0;
+        uint32_t chk = 0, level = 0, steps = 0;
+        if (q0c < c1) {
+          int32_t q_off = q0c, s_off = s_fwd;
+          bool ok = true;
+          if (ext_needed > 0) {  // engine.rs:1130-1153
+            int32_t el = 0, er = 0;
+            while (el < ext_needed) {
+              const int32_t qi = q0c - el - 1, si = s_fwd - el - 1;
+              if (qi < 0 || si < 0) break;
+              const uint8_t qb = QE[1 + qi], sb = SP[1 + si];
+              if (qb >= 4 || sb >= 4 || qb != sb) break;
+              ++el;
+            }
+            while (el + er < ext_needed) {
+              const int32_t qi = q0c + lw + er, si = s_fwd + lw + er;
+              if (qi >= q_len || si >= s_len) break;
+              const uint8_t qb = QE[1 + qi], sb = SP[1 + si];
+              if (qb >= 4 || sb >= 4 || qb != sb) break;
+              ++er;
+            }
+            ok = el + er >= ext_needed;
+            q_off = q0c - el;
+            s_off = s_fwd - el;
+          }
+          if (ok) {
+            diag = (int32_t)((uint32_t)s_off - (uint32_t)q_off);
+            chk = (uint32_t)s_off;
+            const UngOut u = ungapped_ext(Q, q_len + 1, SP, s_len + 1, q_off, s_off, mat, P.xdrop_ungap);
+            const bool hit = u.score >= P.cutoff;
+            level = hit ? (uint32_t)s_off + (u.q_end - (uint32_t)q_off) : (uint32_t)(s_off + ws);
+            steps = u.steps;
+            flag = F_VALID | (hit ? F_HIT : 0);
+          }
+        } else {
+          const int32_t rcq = q0c - c1;
+          const int32_t q0 = q_len - rcq - lw, s0 = s_len - s_fwd - lw;
+          int32_t q_off = q0, s_off = s0;
+          bool ok = q0 >= 0 && s0 >= 0;
+          if (ok && ext_needed > 0) {  // engine.rs:1201-1221
+            int32_t el = 0, er = 0;
+            while (er < ext_needed) {
+              const int32_t qi = q0 + lw + er, si = s0 + lw + er;
+              if (qi >= q_len || si >= s_len) break;
+              const uint8_t qb = QE[1 + qi], sb = SR[1 + si];
+              if (qb >= 4 || sb >= 4 || qb != sb) break;
+              ++er;
+            }
+            while (el + er < ext_needed) {
+              const int32_t qi = q0 - el - 1, si = s0 - el - 1;
+              if (qi < 0 || si < 0) break;
+              const uint8_t qb = QE[1 + qi], sb = SR[1 + si];
+              if (qb >= 4 || sb >= 4 || qb != sb) break;
+              ++el;
+            }
+            ok = el + er >= ext_needed;
+            q_off = q0 + er;
+            s_off = s0 + er;
+          }
+          if (ok) {
+            diag = (int32_t)((uint32_t)s_fwd - (uint32_t)q0c);
+            const uint32_t s_fwd_eff = (uint32_t)(s_len - s_off - lw);
+            chk = s_fwd_eff;
+            const int32_t qn = q_len - 1 - q_off, sn = s_len - 1 - s_off;
+            const UngOut u = ungapped_ext(QR, q_len + 1, SP, s_len + 1, qn, sn, mat, P.xdrop_ungap);
+            const bool hit = u.score >= P.cutoff;
+            level = hit ? (uint32_t)sn + (u.q_end - (uint32_t)qn) : s_fwd_eff + (uint32_t)ws;
+            steps = u.steps;
+            flag = F_VALID | (hit ? F_HIT : 0);
+          }
+        }
+        sd_diag[i] = diag;
+        sd_chk[i] = chk;
+        sd_level[i] = level;
+        sd_steps[i] = (uint16_t)(steps < 65535u ? steps : 65535u);
+        sd_flag[i] = flag;
+        if (flag) atomicAdd(&bcnt[(uint32_t)diag & (kBuckets - 1)], 1u);
+      }
+      __syncthreads();
+      // ---- D: bucket lists (counting sort; each owner re-sorts by seed index)
+      {
+        const uint32_t v0 = bcnt[2 * tid], v1 = bcnt[2 * tid + 1];
+        const uint32_t incl = block_incl_scan256(v0 + v1, wsum);
+        bst[2 * tid] = incl - v0 - v1;
+        bst[2 * tid + 1] = incl - v1;
+      }
+      __syncthreads();
+      for (uint32_t i = tid; i < take; i += kP1Threads)
+        if (sd_flag[i]) blist[atomicAdd(&bst[(uint32_t)sd_diag[i] & (kBuckets - 1)], 1u)] = (uint16_t)i;
+      __syncthreads();
+      // ---- E: exact replay of each bucket (diag_hash.rs:57-111)
+      for (int bb = tid; bb < kBuckets; bb += kP1Threads) {
+        const uint32_t nb = bcnt[bb];
+        if (!nb) continue;
+        const uint32_t st = bst[bb] - nb;
+        for (uint32_t x = st + 1; x < st + nb; ++x) {
+          const uint16_t v = blist[x];
+          uint32_t y = x;
+          while (y > st && blist[y - 1] > v) { blist[y] = blist[y - 1]; --y; }
+          blist[y] = v;
+        }
+        uint32_t cnt_b = hcnt[bb];
+        int32_t* HD = hdiag + bb * kHashLds;
+        uint32_t* HL = hlev + bb * kHashLds;
+        uint2* GS = my_spill + (size_t)bb * kHashGlobal;
+        for (uint32_t x = st; x < st + nb; ++x) {
+          const uint32_t i = blist[x];
+          const int32_t d = sd_diag[i];
+          const uint32_t sc = sd_chk[i];
+          // get(): newest cell first
+          uint32_t lev = 0;
+          for (int32_t k = (int32_t)cnt_b - 1; k >= 0; --k) {
+            int32_t cd;
+            uint32_t cl;
+            if (k < kHashLds) { cd = HD[k]; cl = HL[k]; }
+            else { const uint2 g = GS[k - kHashLds]; cd = (int32_t)g.x; cl = g.y; }
+            if (cd == d) { lev = cl; break; }
+          }
+          if (sc < lev) continue;
+          ++n_evals;
+          n_steps += sd_steps[i];
+          if (sd_flag[i] & F_HIT) sd_flag[i] |= F_EMIT;
+          // insert(): same diagonal, else first stale cell, else a new head cell
+          const uint32_t L = sd_level[i];
+          bool done = false;
+          for (int32_t k = (int32_t)cnt_b - 1; k >= 0 && !done; --k) {
+            int32_t cd;
+            uint32_t cl;
+            if (k < kHashLds) { cd = HD[k]; cl = HL[k]; }
+            else { const uint2 g = GS[k - kHashLds]; cd = (int32_t)g.x; cl = g.y; }
+            if (cd == d || (int32_t)(sc - cl) > window) {
+              if (k < kHashLds) { HD[k] = d; HL[k] = L; }
+              else GS[k - kHashLds] = make_uint2((uint32_t)d, L);
+              done = true;
+            }
+          }
+          if (!done) {
+            if (cnt_b >= (uint32_t)(kHashLds + kHashGlobal)) {
+              atomicOr(err, (uint32_t)E_HASH_FULL);
+            } else {
+              if (cnt_b < (uint32_t)kHashLds) { HD[cnt_b] = d; HL[cnt_b] = L; }
+              else GS[cnt_b - kHashLds] = make_uint2((uint32_t)d, L);
+              ++cnt_b;
+            }
+          }
+        }
+        hcnt[bb] = (uint16_t)cnt_b;
+      }
+      __syncthreads();
+      // ---- F: emit kept hits in the record frame
+      for (uint32_t i = tid; i < take; i += kP1Threads) {
+        if (!(sd_flag[i] & F_EMIT)) continue;
+        const int32_t s_fwd = (int32_t)seed_s[i];
+        const int32_t q0c = (int32_t)seed_q[i];
+        UngHit h;
+        h.item = (uint32_t)item;
+        if (q0c < c1) {
+          const int32_t q_off = q0c - ((int32_t)seed_s[i] - (int32_t)sd_chk[i]);
+          const int32_t s_off = (int32_t)sd_chk[i];
+          const UngOut u = ungapped_ext(Q, q_len + 1, SP, s_len + 1, q_off, s_off, mat, P.xdrop_ungap);
+          h.q_seed = (uint32_t)q_off;
+          h.s_seed = (uint32_t)s_off;
+          h.q_start = u.q_beg;
+          h.q_end = u.q_end;
+          h.s_start = (uint32_t)(s_off - (q_off - (int32_t)u.q_beg));
+          h.s_end = (uint32_t)s_off + (u.q_end - (uint32_t)q_off);
+          h.score = u.score;
+          h.strand = 0;
+        } else {
+          const uint32_t s_fwd_eff = sd_chk[i];
+          const int32_t s_off = s_len - (int32_t)s_fwd_eff - lw;
+          const int32_t q_off = (q_len - (q0c - c1) - lw) + (s_off - (s_len - s_fwd - lw));
+          const int32_t qn = q_len - 1 - q_off, sn = s_len - 1 - s_off;
+          const UngOut u = ungapped_ext(QR, q_len + 1, SP, s_len + 1, qn, sn, mat, P.xdrop_ungap);
+          const uint32_t rs0 = (uint32_t)(sn - (qn - (int32_t)u.q_beg));
+          const uint32_t rs1 = (uint32_t)sn + (u.q_end - (uint32_t)qn);
+          h.q_seed = (uint32_t)q_off;
+          h.s_seed = (uint32_t)s_off;
+          h.q_start = (uint32_t)q_len - u.q_end;
+          h.q_end = (uint32_t)q_len - u.q_beg;
+          h.s_start = (uint32_t)s_len - rs1;
+          h.s_end = (uint32_t)s_len - rs0;
+          h.score = u.score;
+          h.strand = 1;
+        }
+        ++n_hits;
+        const unsigned long long pos = atomicAdd(out_count, 1ull);
+        if (pos < P.out_cap) out[pos] = h;
+        else atomicOr(err, (uint32_t)E_OUT_FULL);
+      }
+      __syncthreads();
+      slot0 = s_next_slot;
+      kcur = s_next_k;
+    }
+  }
+  atomicAdd(&counters[C_SEED_EVALS], n_evals);
+  atomicAdd(&counters[C_UNG_STEPS], n_steps);
+  atomicAdd(&counters[C_UNG_HITS], n_hits);
+}
+
+// --------------------------------------------------------- X-drop DP ---------
+// One lane runs one half alignment in the reference's row-major order, so every
+// prune decision and band bound is the reference's.  The band [first_b, b_size)
+// lives in a per-lane ring of 2^wlog2 cells indexed by the absolute column; a
+// band wider than the ring sets E_BAND_WIDE and the host reruns the task with a
+// wider ring.  With TB, one traceback byte per band cell is kept for rows up to
+// stop_row (the best row from the score pass) and the walk runs at the end.
+static constexpr uint8_t SCRIPT_GAP_IN_A = 0, SCRIPT_SUB = 3, SCRIPT_GAP_IN_B = 6, SCRIPT_OP_MASK = 7,
+                         SCRIPT_EXTEND_GAP_A = 0x10, SCRIPT_EXTEND_GAP_B = 0x40;
+
+template <bool TB>
+__device__ void dp_half(const DpTask& t, const int* mat, int32_t go, int32_t ge, int2* __restrict__ ring,
+                        uint32_t wmask, DpResult& res, uint8_t* __restrict__ flat, uint32_t* __restrict__ rows,
+                        uint32_t stop_row) {
+  res.score = 0; res.a_len = 0; res.b_len = 0; res.flags = 0; res.cells = 0; res.tb_bytes = 0; res.tb_rows = 0;
+  const uint32_t m = t.m, n = t.n;
+  if (m == 0 || n == 0) return;
+  const bool rev = t.reverse != 0;
+  const int32_t oe = go + ge;
+  const int32_t xdrop = t.xdrop > oe ? t.xdrop : oe;
+  uint32_t flat_len = 0;
+  ring[0] = make_int2(0, -oe);
+  if (TB) { rows[0] = 0; rows[1] = 0; flat[flat_len] = SCRIPT_GAP_IN_B; }
+  ++flat_len;
+  int32_t score = -oe;
+  uint32_t b_size = 1;
+  for (uint32_t j = 1; j <= n; ++j) {
+    if (score < -xdrop) break;
+    if (j > wmask) { res.flags |= E_BAND_WIDE; return; }
+    ring[j & wmask] = make_int2(score, score - oe);
+    if (TB) flat[flat_len] = SCRIPT_GAP_IN_A;
+    ++flat_len;
+    score -= ge;
+    b_size = j + 1;
+  }
+  int32_t best = 0;
+  uint32_t best_a = 0, best_b = 0, first_b = 0;
+  uint64_t cells = 0;
+  uint32_t bytes_at_best = flat_len, rows_at_best = 1;
+  for (uint32_t ai = 0; ai < m; ++ai) {
+    if (TB && ai >= stop_row) break;
+    const int* row = mat + ((t.a[rev ? m - 1 - ai : ai] & 15) << 4);
+    if (TB) { rows[2 * (ai + 1)] = flat_len; rows[2 * (ai + 1) + 1] = first_b; }
+    int32_t sc = kMinInt, sgr = kMinInt;
+    uint32_t last_b = first_b;
+    bool best_here = false;
+    cells += b_size - first_b;
+    const uint8_t* bp = t.b;
+    for (uint32_t bi = first_b; bi < b_size; ++bi) {
+      int2 c = ring[bi & wmask];
+      const uint32_t bb = bi < n ? (bp[rev ? n - 1 - bi : bi] & 15) : 0;
+      int32_t sgc = c.y;
+      const int32_t next = c.x + row[bb];
+      uint8_t op = SCRIPT_SUB;
+      int32_t cs = sc;
+      if (sgc > cs) { cs = sgc; op = SCRIPT_GAP_IN_B; }
+      if (sgr > cs) { cs = sgr; op = SCRIPT_GAP_IN_A; }
+      if (best - cs > xdrop) {
+        if (bi == first_b) ++first_b;
+        else ring[bi & wmask].x = kMinInt;
+      } else {
+        last_b = bi;
+        if (cs > best) { best = cs; best_a = ai + 1; best_b = bi; best_here = true; }
+        sgr -= ge;
+        sgc -= ge;
+        const int32_t open = cs - oe;
+        if (sgc < open) c.y = open;
+        else { c.y = sgc; op |= SCRIPT_EXTEND_GAP_B; }
+        c.x = cs;
+        ring[bi & wmask] = c;
+        if (sgr < open) sgr = open;
+        else op |= SCRIPT_EXTEND_GAP_A;
+      }
+      sc = next;
+      if (TB) flat[flat_len] = op;
+      ++flat_len;
+    }
+    if (first_b >= b_size) {
+      if (best_here) { bytes_at_best = flat_len; rows_at_best = ai + 2; }
+      break;
+    }
+    if (last_b + 1 < b_size) {
+      b_size = last_b + 1;
+    } else {
+      while (sgr >= best - xdrop && b_size <= n) {
+        if (b_size - first_b > wmask) { res.flags |= E_BAND_WIDE; return; }
+        ring[b_size & wmask] = make_int2(sgr, sgr - oe);
+        if (TB) flat[flat_len] = SCRIPT_GAP_IN_A;
+        ++flat_len;
+        sgr -= ge;
+        ++b_size;
+      }
+    }
+    if (b_size <= n) {
+      if (b_size - first_b > wmask) { res.flags |= E_BAND_WIDE; return; }
+      ring[b_size & wmask] = make_int2(kMinInt, kMinInt);
+      ++b_size;
+    }
+    if (best_here) { bytes_at_best = flat_len; rows_at_best = ai + 2; }
+  }
+  res.score = best;
+  res.a_len = best_a;
+  res.b_len = best_b;
+  res.cells = cells;
+  res.tb_bytes = bytes_at_best;
+  res.tb_rows = rows_at_best;
+}
+
+__global__ __launch_bounds__(256) void k_dp_score(const DpTask* __restrict__ tasks, uint32_t n_tasks,
+                                                  const int32_t* __restrict__ gmat, int32_t go, int32_t ge,
+                                                  int2* __restrict__ ring, uint32_t wlog2,
+                                                  DpResult* __restrict__ out, unsigned long long* counters) {
+  __shared__ int mat[256];
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) mat[i] = gmat[i];
+  __syncthreads();
+  const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+  int2* my = ring + ((size_t)gt << wlog2);
+  unsigned long long cells = 0;
+  for (uint32_t i = gt; i < n_tasks; i += nt) {
+    DpResult r;
+    dp_half<false>(tasks[i], mat, go, ge, my, (1u << wlog2) - 1, r, nullptr, nullptr, 0);
+    out[i] = r;
+    cells += r.cells;
+  }
+  atomicAdd(&counters[C_DP_CELLS], cells);
+}
+
+__global__ __launch_bounds__(256) void k_dp_traceback(const TbTask* __restrict__ tasks, uint32_t n_tasks,
+                                                      const int32_t* __restrict__ gmat, int32_t go, int32_t ge,
+                                                      int2* __restrict__ ring, uint32_t wlog2,
+                                                      uint8_t* __restrict__ flat_pool, uint32_t* __restrict__ row_pool,
+                                                      uint32_t* __restrict__ ops_pool, TbResult* __restrict__ out) {
+  __shared__ int mat[256];
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) mat[i] = gmat[i];
+  __syncthreads();
+  const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+  int2* my = ring + ((size_t)gt << wlog2);
+  for (uint32_t i = gt; i < n_tasks; i += nt) {
+    const TbTask T = tasks[i];
+    uint8_t* flat = flat_pool + T.flat_off;
+    uint32_t* rows = row_pool + 2 * T.row_off;
+    uint32_t* ops = ops_pool + T.ops_off;
+    DpResult r;
+    dp_half<true>(T.t, mat, go, ge, my, (1u << wlog2) - 1, r, flat, rows, T.stop_row);
+    TbResult o{0, r.flags};
+    if (!r.flags) {
+      // traceback walk (gapped.rs:692-757); ops are collected backwards, run-length encoded
+      uint32_t a = T.stop_row, b = T.best_b, nops = 0;
+      const uint32_t n_rows = T.stop_row + 1;
+      const uint32_t flat_total = r.tb_bytes;
+      uint8_t prev = SCRIPT_SUB;
+      uint32_t last_kind = 3, run = 0;
+      const uint32_t budget = a + b;
+      for (uint32_t stepi = 0; stepi < budget && (a | b); ++stepi) {
+        const uint32_t off = rows[2 * a], fb = rows[2 * a + 1];
+        const uint32_t row_end = a + 1 < n_rows ? rows[2 * (a + 1)] : flat_total;
+        uint8_t raw = SCRIPT_GAP_IN_A;
+        if (b >= fb && off + (b - fb) < row_end) raw = flat[off + (b - fb)];
+        uint8_t op;
+        if ((prev & SCRIPT_OP_MASK) == SCRIPT_GAP_IN_A)
+          op = (raw & SCRIPT_EXTEND_GAP_A) ? SCRIPT_GAP_IN_A : (raw & SCRIPT_OP_MASK);
+        else if ((prev & SCRIPT_OP_MASK) == SCRIPT_GAP_IN_B)
+          op = (raw & SCRIPT_EXTEND_GAP_B) ? SCRIPT_GAP_IN_B : (raw & SCRIPT_OP_MASK);
+        else
+          op = raw & SCRIPT_OP_MASK;
+        prev = op;
+        uint32_t kind;
+        if (op == SCRIPT_GAP_IN_A) { kind = 1; if (b) --b; }
+        else if (op == SCRIPT_GAP_IN_B) { kind = 2; if (a) --a; }
+        else { kind = 0; if (a) --a; if (b) --b; }
+        if (kind == last_kind) {
+          ++run;
+        } else {
+          if (run) {
+            if (nops >= T.ops_cap) { o.flags |= E_TB_SPACE; break; }
+            ops[nops++] = last_kind | (run << 2);
+          }
+          last_kind = kind;
+          run = 1;
+        }
+      }
+      if (!(o.flags & E_TB_SPACE) && run) {
+        if (nops >= T.ops_cap) o.flags |= E_TB_SPACE;
+        else ops[nops++] = last_kind | (run << 2);
+      }
+      if (a | b) o.flags |= E_TB_SPACE;
+      for (uint32_t x = 0, y = nops ? nops - 1 : 0; x < y; ++x, --y) {
+        const uint32_t tmp = ops[x];
+        ops[x] = ops[y];
+        ops[y] = tmp;
+      }
+      o.n_ops = nops;
+    }
+    out[i] = o;
+  }
+}
+
+// ---------------------------------------------------------- launchers --------
+hipError_t launch_lut_keys(const RecDesc* recs, uint32_t n_recs, const uint64_t* elem_base, const uint8_t* qext,
+                           int lw, int ws, int desc, uint64_t* keys, uint32_t* vals, uint32_t* counts,
+                           uint64_t invalid_key, uint64_t total, hipStream_t st) {
+  const uint32_t blocks = (uint32_t)((total + 255) / 256);
+  hipLaunchKernelGGL(k_lut_keys, dim3(blocks), dim3(256), 0, st, recs, n_recs, elem_base, qext, lw, ws, desc, keys,
+                     vals, counts, invalid_key, total);
+  return hipGetLastError();
+}
+
+hipError_t sort_pairs_u64(void* temp, size_t& temp_bytes, const uint64_t* kin, uint64_t* kout, const uint32_t* vin,
+                          uint32_t* vout, uint64_t n, int end_bit, hipStream_t st) {
+  return hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, kin, kout, vin, vout, (int)n, 0, end_bit, st);
+}
+
+hipError_t exclusive_sum_u32(void* temp, size_t& temp_bytes, const uint32_t* in, uint32_t* out, uint64_t n,
+                             hipStream_t st) {
+  return hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, in, out, (int)n, st);
+}
+
+hipError_t launch_phase1(int blocks, P1Params P, const RecDesc* recs, const SubjDesc* subjs, const uint8_t* qfwd,
+                         const uint8_t* qrc, const uint8_t* qext, const uint8_t* splus, const uint8_t* src,
+                         const uint32_t* cells, const uint32_t* entries, const int32_t* gmat, UngHit* out,
+                         unsigned long long* out_count, uint2* spill, unsigned long long* queue,
+                         unsigned long long* counters, uint32_t* err, hipStream_t st) {
+  hipLaunchKernelGGL(k_phase1, dim3(blocks), dim3(kP1Threads), 0, st, P, recs, subjs, qfwd, qrc, qext, splus, src,
+                     cells, entries, gmat, out, out_count, spill, queue, counters, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_dp_score(int blocks, const DpTask* tasks, uint32_t n, const int32_t* gmat, int32_t go, int32_t ge,
+                           int2* ring, uint32_t wlog2, DpResult* out, unsigned long long* counters, hipStream_t st) {
+  hipLaunchKernelGGL(k_dp_score, dim3(blocks), dim3(256), 0, st, tasks, n, gmat, go, ge, ring, wlog2, out, counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_dp_traceback(int blocks, const TbTask* tasks, uint32_t n, const int32_t* gmat, int32_t go,
+                               int32_t ge, int2* ring, uint32_t wlog2, uint8_t* flat, uint32_t* rows, uint32_t* ops,
+                               TbResult* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_dp_traceback, dim3(blocks), dim3(256), 0, st, tasks, n, gmat, go, ge, ring, wlog2, flat, rows,
+                     ops, out);
+  return hipGetLastError();
+}
+
+}  // namespace rmb

@@ -1,0 +1,996 @@
+// librmblast_hip: context, device memory, stage orchestration and the C ABI.
+//
+// Stage chain of one rmb_search call (all records x all subjects, both strands):
+//   1. lookup tables for every record on the GPU (k_lut_keys + radix sort + scan)
+//   2. k_phase1 over (record, subject) items from a device queue -> ungapped hits
+//   3. per item, NCBI order of the hits; Phase 2a as ordered rounds: the host
+//      walks each item's hits against its containment trees, the GPU runs the
+//      score-only X-drop DPs of the hits that need one (k_dp_score)
+//   4. Phase 2b: every prelim's traceback DP on the GPU (score pass, then the
+//      traceback pass with exact-size storage), ordered replay on the host
+//      (containment, complexity adjustment, common-endpoint purge)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstdarg>
+#include <cstring>
+#include <functional>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/rmblast_hip.h"
+#include "rmb_device.h"
+#include "rmb_replay.h"
+
+namespace rmb {
+hipError_t launch_lut_keys(const RecDesc*, uint32_t, const uint64_t*, const uint8_t*, int, int, int, uint64_t*,
+                           uint32_t*, uint32_t*, uint64_t, uint64_t, hipStream_t);
+hipError_t sort_pairs_u64(void*, size_t&, const uint64_t*, uint64_t*, const uint32_t*, uint32_t*, uint64_t, int,
+                          hipStream_t);
+hipError_t exclusive_sum_u32(void*, size_t&, const uint32_t*, uint32_t*, uint64_t, hipStream_t);
+hipError_t launch_phase1(int, P1Params, const RecDesc*, const SubjDesc*, const uint8_t*, const uint8_t*,
+                         const uint8_t*, const uint8_t*, const uint8_t*, const uint32_t*, const uint32_t*,
+                         const int32_t*, UngHit*, unsigned long long*, uint2*, unsigned long long*,
+                         unsigned long long*, uint32_t*, hipStream_t);
+hipError_t launch_dp_score(int, const DpTask*, uint32_t, const int32_t*, int32_t, int32_t, int2*, uint32_t,
+                           DpResult*, unsigned long long*, hipStream_t);
+hipError_t launch_dp_traceback(int, const TbTask*, uint32_t, const int32_t*, int32_t, int32_t, int2*, uint32_t,
+                               uint8_t*, uint32_t*, uint32_t*, TbResult*, hipStream_t);
+
+struct Failure : std::runtime_error {
+  int code;
+  Failure(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+static void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw Failure(RMB_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Grow-only device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  template <class T>
+  T* get(size_t n_elems) {
+    const size_t bytes = std::max<size_t>(n_elems * sizeof(T), 16);
+    if (bytes > cap) {
+      if (p) hip_check(hipFree(p), "hipFree");
+      p = nullptr;
+      hip_check(hipMalloc(&p, bytes), "hipMalloc");
+      cap = bytes;
+    }
+    return (T*)p;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+static void parallel_for(size_t n, const std::function<void(size_t)>& fn) {
+  if (n == 0) return;
+  unsigned nt = std::max(1u, std::thread::hardware_concurrency());
+  nt = (unsigned)std::min<size_t>(nt, n);
+  if (nt == 1) {
+    for (size_t i = 0; i < n; ++i) fn(i);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  std::vector<std::thread> pool;
+  std::exception_ptr failure;
+  std::atomic<bool> failed{false};
+  for (unsigned t = 0; t < nt; ++t)
+    pool.emplace_back([&]() {
+      try {
+        for (size_t i; !failed && (i = next.fetch_add(1)) < n;) fn(i);
+      } catch (...) {
+        if (!failed.exchange(true)) failure = std::current_exception();
+      }
+    });
+  for (auto& th : pool) th.join();
+  if (failure) std::rethrow_exception(failure);
+}
+
+// NCBI lookup width thresholds for a two-strand query of `e` entries.
+static int lut_width_for(int ws, size_t e) {
+  switch (ws) {
+    case 4: case 5: case 6: return ws;
+    case 7: return e < 250 ? 6 : 7;
+    case 8: return e < 8500 ? 7 : 8;
+    case 9: return e < 1250 ? 7 : (e < 21000 ? 8 : 9);
+    case 10: return e < 1250 ? 7 : (e < 8500 ? 8 : (e < 18000 ? 9 : 10));
+    case 11: return e < 12000 ? 8 : (e < 180000 ? 10 : 11);
+    case 12: return e < 8500 ? 8 : (e < 18000 ? 9 : (e < 60000 ? 10 : (e < 900000 ? 11 : 12)));
+    default: return e < 8500 ? 8 : (e < 300000 ? 11 : 12);
+  }
+}
+
+// makeblastdb's CRandom lagged-Fibonacci generator, seeded by the sequence
+// length (seq/twobit.rs:50-134): one draw per ambiguous base, 5' to 3'.
+class AmbiguityResolver {
+ public:
+  explicit AmbiguityResolver(uint32_t seed) {
+    st_[0] = seed;
+    for (int i = 1; i < 33; ++i) st_[i] = 1103515245u * st_[i - 1] + 12345u;
+    for (int i = 0; i < 330; ++i) next();
+  }
+  uint8_t base_for(uint8_t code) {
+    static const uint8_t sets[16][4] = {{0}, {0}, {0}, {0}, {2, 0, 2}, {2, 1, 3}, {2, 0, 1}, {2, 2, 3},
+                                       {2, 0, 3}, {2, 1, 2}, {3, 1, 2, 3}, {3, 0, 2, 3}, {3, 0, 1, 3},
+                                       {3, 0, 1, 2}, {4}, {0}};
+    const uint32_t r = next() >> 1;
+    if (code == 14) return (uint8_t)(r & 3);
+    return sets[code][1 + r % sets[code][0]];
+  }
+
+ private:
+  uint32_t st_[33];
+  int j_ = 12, k_ = 32;
+  uint32_t next() {
+    const uint32_t r = st_[k_] + st_[j_];
+    st_[k_] = r;
+    --k_;
+    --j_;
+    if (k_ < 0) k_ = 32;
+    else if (j_ < 0) j_ = 32;
+    return r;
+  }
+};
+
+struct Subject {
+  std::vector<uint8_t> plus, rc, align, align_rc;  // sentinel-wrapped
+  std::vector<uint8_t> n_mask, n_mask_rc;
+};
+
+static std::vector<uint8_t> wrap(const uint8_t* s, uint32_t n) {
+  std::vector<uint8_t> v(n + 2);
+  v[0] = 15;
+  for (uint32_t i = 0; i < n; ++i) v[1 + i] = s[i] & 15;
+  v[n + 1] = 15;
+  return v;
+}
+static std::vector<uint8_t> revcomp_wrapped(const std::vector<uint8_t>& v) {
+  std::vector<uint8_t> r(v.size());
+  for (size_t i = 0; i < v.size(); ++i) r[i] = kBlastnaComplement[v[v.size() - 1 - i] & 15];
+  return r;
+}
+
+}  // namespace rmb
+
+using namespace rmb;
+
+struct rmb_ctx {
+  int device = 0;
+  hipStream_t st = nullptr;
+  rmb_params P{};
+  Scoring M{};
+  std::string err;
+  int32_t* d_mat = nullptr;
+  int cu_count = 256;
+  // library
+  std::vector<Subject> subj;
+  std::vector<SubjDesc> sdesc;
+  DevBuf d_splus, d_src, d_salign, d_sdesc;
+  // results of the last search
+  std::vector<rmb_hsp> hsps;
+  std::vector<rmb_op> ops;
+  uint64_t stats[16] = {0};
+  // scratch
+  DevBuf b_qfwd, b_qrc, b_recs, b_elem, b_keys, b_keys2, b_vals, b_vals2, b_counts, b_cells, b_sort_tmp;
+  DevBuf b_hits, b_hitcount, b_spill, b_queue, b_counters, b_err;
+  DevBuf b_tasks, b_dpres, b_ring, b_ring_big, b_tbtasks, b_tbres, b_flat, b_rows, b_ops;
+};
+
+namespace rmb {
+
+static const int kSpecDepth = 6;        // Phase-2a DPs requested per item per round
+static const uint32_t kRingLog2 = 10;   // per-lane band ring for the common case
+static const uint32_t kRingBigLog2 = 17;
+
+struct Timer {
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  uint64_t us() const {
+    return (uint64_t)std::chrono::duration_cast<std::chrono::microseconds>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+// Run a list of lane-serial score-only DPs; tasks whose band outgrows the
+// common ring are rerun once with a ring of 2^kRingBigLog2 cells.
+static void run_dp_score(rmb_ctx* c, const std::vector<DpTask>& tasks, std::vector<DpResult>& res) {
+  res.assign(tasks.size(), DpResult{});
+  if (tasks.empty()) return;
+  unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
+  const size_t chunk = 1u << 20;
+  for (size_t base = 0; base < tasks.size(); base += chunk) {
+    const uint32_t n = (uint32_t)std::min(chunk, tasks.size() - base);
+    DpTask* dt = c->b_tasks.get<DpTask>(n);
+    DpResult* dr = c->b_dpres.get<DpResult>(n);
+    hip_check(hipMemcpyAsync(dt, tasks.data() + base, n * sizeof(DpTask), hipMemcpyHostToDevice, c->st), "copy tasks");
+    const uint32_t lanes = std::min<uint32_t>((uint32_t)c->cu_count * 4 * 64, ((n + 255) / 256) * 256);
+    int2* ring = c->b_ring.get<int2>((size_t)lanes << kRingLog2);
+    hip_check(launch_dp_score((int)(lanes / 256), dt, n, c->d_mat, c->P.gap_open, c->P.gap_extend, ring, kRingLog2,
+                              dr, ctr, c->st),
+              "k_dp_score");
+    hip_check(hipMemcpyAsync(res.data() + base, dr, n * sizeof(DpResult), hipMemcpyDeviceToHost, c->st), "copy dp");
+  }
+  hip_check(hipStreamSynchronize(c->st), "dp sync");
+  std::vector<uint32_t> wide;
+  for (size_t i = 0; i < res.size(); ++i)
+    if (res[i].flags & E_BAND_WIDE) wide.push_back((uint32_t)i);
+  if (wide.empty()) return;
+  std::vector<DpTask> wt(wide.size());
+  for (size_t i = 0; i < wide.size(); ++i) wt[i] = tasks[wide[i]];
+  const uint32_t n = (uint32_t)wt.size();
+  DpTask* dt = c->b_tasks.get<DpTask>(n);
+  DpResult* dr = c->b_dpres.get<DpResult>(n);
+  hip_check(hipMemcpyAsync(dt, wt.data(), n * sizeof(DpTask), hipMemcpyHostToDevice, c->st), "copy wide");
+  const uint32_t lanes = std::min<uint32_t>(4096, ((n + 255) / 256) * 256);
+  int2* ring = c->b_ring_big.get<int2>((size_t)lanes << kRingBigLog2);
+  hip_check(launch_dp_score((int)(lanes / 256), dt, n, c->d_mat, c->P.gap_open, c->P.gap_extend, ring, kRingBigLog2,
+                            dr, ctr, c->st),
+            "k_dp_score wide");
+  std::vector<DpResult> wr(n);
+  hip_check(hipMemcpyAsync(wr.data(), dr, n * sizeof(DpResult), hipMemcpyDeviceToHost, c->st), "copy wide res");
+  hip_check(hipStreamSynchronize(c->st), "dp wide sync");
+  for (size_t i = 0; i < wide.size(); ++i) {
+    if (wr[i].flags & E_BAND_WIDE)
+      throw Failure(RMB_ERR_CAPACITY, "X-drop band wider than 2^17 cells");
+    res[wide[i]] = wr[i];
+  }
+}
+
+struct HalfTb { int32_t score; uint32_t a_len, b_len; Script ops; };
+
+// Traceback DPs: score pass for sizes, then the traceback pass in batches
+// bounded by the device storage they need.
+static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::vector<HalfTb>& out) {
+  out.assign(tasks.size(), HalfTb{0, 0, 0, {}});
+  std::vector<DpResult> sres;
+  run_dp_score(c, tasks, sres);
+  std::vector<uint32_t> need;
+  for (size_t i = 0; i < tasks.size(); ++i) {
+    out[i].score = sres[i].score;
+    out[i].a_len = sres[i].a_len;
+    out[i].b_len = sres[i].b_len;
+    if (sres[i].a_len > 0 || sres[i].b_len > 0) need.push_back((uint32_t)i);
+  }
+  const uint64_t kBudget = 1ull << 31;
+  size_t pos = 0;
+  while (pos < need.size()) {
+    std::vector<TbTask> batch;
+    uint64_t flat = 0, rows = 0, opsn = 0;
+    while (pos < need.size()) {
+      const uint32_t i = need[pos];
+      const DpResult& r = sres[i];
+      const uint64_t ocap = (uint64_t)r.a_len + r.b_len + 1;
+      if (!batch.empty() && flat + r.tb_bytes > kBudget) break;
+      TbTask t{};
+      t.t = tasks[i];
+      t.flat_off = flat;
+      t.row_off = rows;
+      t.ops_off = opsn;
+      t.stop_row = r.a_len;
+      t.best_b = r.b_len;
+      t.ops_cap = (uint32_t)ocap;
+      flat += r.tb_bytes;
+      rows += r.tb_rows;
+      opsn += ocap;
+      batch.push_back(t);
+      ++pos;
+    }
+    const uint32_t n = (uint32_t)batch.size();
+    TbTask* dt = c->b_tbtasks.get<TbTask>(n);
+    TbResult* dr = c->b_tbres.get<TbResult>(n);
+    uint8_t* dflat = c->b_flat.get<uint8_t>(flat + 16);
+    uint32_t* drows = c->b_rows.get<uint32_t>(2 * rows + 16);
+    uint32_t* dops = c->b_ops.get<uint32_t>(opsn + 16);
+    hip_check(hipMemcpyAsync(dt, batch.data(), n * sizeof(TbTask), hipMemcpyHostToDevice, c->st), "copy tb tasks");
+    std::vector<TbResult> tr(n);
+    const uint32_t lanes = std::min<uint32_t>((uint32_t)c->cu_count * 4 * 64, ((n + 255) / 256) * 256);
+    int2* ring = c->b_ring.get<int2>((size_t)lanes << kRingLog2);
+    hip_check(launch_dp_traceback((int)(lanes / 256), dt, n, c->d_mat, c->P.gap_open, c->P.gap_extend, ring,
+                                  kRingLog2, dflat, drows, dops, dr, c->st),
+              "k_dp_traceback");
+    hip_check(hipMemcpyAsync(tr.data(), dr, n * sizeof(TbResult), hipMemcpyDeviceToHost, c->st), "copy tb res");
+    hip_check(hipStreamSynchronize(c->st), "tb sync");
+    // rerun band-wide tasks with the big ring
+    std::vector<uint32_t> redo;
+    for (uint32_t k = 0; k < n; ++k)
+      if (tr[k].flags & E_BAND_WIDE) redo.push_back(k);
+    if (!redo.empty()) {
+      std::vector<TbTask> rb(redo.size());
+      for (size_t k = 0; k < redo.size(); ++k) rb[k] = batch[redo[k]];
+      const uint32_t rn = (uint32_t)rb.size();
+      TbTask* rdt = c->b_tbtasks.get<TbTask>(std::max<size_t>(n, rn));
+      hip_check(hipMemcpyAsync(rdt, rb.data(), rn * sizeof(TbTask), hipMemcpyHostToDevice, c->st), "copy tb redo");
+      const uint32_t rl = std::min<uint32_t>(4096, ((rn + 255) / 256) * 256);
+      int2* big = c->b_ring_big.get<int2>((size_t)rl << kRingBigLog2);
+      hip_check(launch_dp_traceback((int)(rl / 256), rdt, rn, c->d_mat, c->P.gap_open, c->P.gap_extend, big,
+                                    kRingBigLog2, dflat, drows, dops, dr, c->st),
+                "k_dp_traceback wide");
+      std::vector<TbResult> rr(rn);
+      hip_check(hipMemcpyAsync(rr.data(), dr, rn * sizeof(TbResult), hipMemcpyDeviceToHost, c->st), "copy redo");
+      hip_check(hipStreamSynchronize(c->st), "tb redo sync");
+      for (size_t k = 0; k < redo.size(); ++k) tr[redo[k]] = rr[k];
+    }
+    std::vector<uint32_t> hops(opsn + 1);
+    hip_check(hipMemcpy(hops.data(), dops, opsn * sizeof(uint32_t), hipMemcpyDeviceToHost), "copy ops");
+    for (uint32_t k = 0; k < n; ++k) {
+      if (tr[k].flags) throw Failure(RMB_ERR_CAPACITY, "traceback storage or band limit exceeded");
+      const uint32_t i = need[pos - n + k];
+      Script& s = out[i].ops;
+      const uint32_t* o = hops.data() + batch[k].ops_off;
+      for (uint32_t j = 0; j < tr[k].n_ops; ++j) script_push(s, (uint8_t)(o[j] & 3), o[j] >> 2);
+    }
+  }
+}
+
+struct ItemState {
+  uint32_t item = 0;
+  std::vector<UngRec> hits;
+  size_t cursor = 0;
+  std::vector<PreRec> pre;
+  std::vector<FinalRec> fin;
+  // Phase-2a round bookkeeping
+  std::vector<uint32_t> req;     // hit indices with a DP this round
+  size_t task0 = 0;              // first task of this item in the round
+};
+
+// Phase 1 for a group of records sharing one lookup width.
+static void phase1_group(rmb_ctx* c, const std::vector<uint32_t>& recs_idx, const std::vector<RecDesc>& all_recs,
+                         int lw, const uint8_t* d_qfwd, const uint8_t* d_qrc, std::vector<UngHit>& hits_out) {
+  const int ws = c->P.word_size;
+  const uint32_t R = (uint32_t)recs_idx.size();
+  const uint64_t cells = 1ull << (2 * lw);
+  std::vector<RecDesc> recs(R);
+  std::vector<uint64_t> elem_base(R + 1, 0);
+  for (uint32_t i = 0; i < R; ++i) {
+    recs[i] = all_recs[recs_idx[i]];
+    recs[i].cell_base = (uint64_t)i * cells;
+    elem_base[i + 1] = elem_base[i] + 2ull * recs[i].q_len + 1;
+  }
+  const uint64_t E = elem_base[R];
+  RecDesc* drecs = c->b_recs.get<RecDesc>(R);
+  uint64_t* delem = c->b_elem.get<uint64_t>(R + 1);
+  hip_check(hipMemcpyAsync(drecs, recs.data(), R * sizeof(RecDesc), hipMemcpyHostToDevice, c->st), "copy recs");
+  hip_check(hipMemcpyAsync(delem, elem_base.data(), (R + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, c->st),
+            "copy elem");
+  uint64_t* keys = c->b_keys.get<uint64_t>(E);
+  uint64_t* keys2 = c->b_keys2.get<uint64_t>(E);
+  uint32_t* vals = c->b_vals.get<uint32_t>(E);
+  uint32_t* vals2 = c->b_vals2.get<uint32_t>(E);
+  const uint64_t ncell = (uint64_t)R * cells + 1;
+  uint32_t* counts = c->b_counts.get<uint32_t>(ncell);
+  uint32_t* cstart = c->b_cells.get<uint32_t>(ncell);
+  hip_check(hipMemsetAsync(counts, 0, ncell * sizeof(uint32_t), c->st), "memset counts");
+  const uint64_t invalid = (uint64_t)R << (2 * lw);
+  int end_bit = 2 * lw;
+  while ((1ull << (end_bit - 2 * lw)) <= R) ++end_bit;
+  const int desc = lw > 8 ? 1 : 0;
+  hip_check(launch_lut_keys(drecs, R, delem, d_qfwd, lw, ws, desc, keys, vals, counts, invalid, E, c->st), "lut keys");
+  size_t tmp1 = 0, tmp2 = 0;
+  hip_check(sort_pairs_u64(nullptr, tmp1, keys, keys2, vals, vals2, E, end_bit, c->st), "sort size");
+  hip_check(exclusive_sum_u32(nullptr, tmp2, counts, cstart, ncell, c->st), "scan size");
+  void* tmp = c->b_sort_tmp.get<uint8_t>(std::max(tmp1, tmp2));
+  hip_check(sort_pairs_u64(tmp, tmp1, keys, keys2, vals, vals2, E, end_bit, c->st), "radix sort");
+  hip_check(exclusive_sum_u32(tmp, tmp2, counts, cstart, ncell, c->st), "scan");
+  // Phase 1
+  const uint32_t nsub = (uint32_t)c->subj.size();
+  P1Params P{};
+  P.word_size = ws;
+  P.lut_width = lw;
+  P.scan_step = ws - lw + 1;
+  P.xdrop_ungap = c->P.xdrop_ungap;
+  P.cutoff = c->P.ungapped_cutoff >= 0 ? c->P.ungapped_cutoff
+                                      : (c->P.min_raw_gapped_score > 0 ? c->P.min_raw_gapped_score / 2 : 1);
+  P.n_subj = nsub;
+  P.n_items = (uint64_t)R * nsub;
+  const int blocks = std::max(1, std::min<int>(c->cu_count * 2, (int)std::min<uint64_t>(P.n_items, 1u << 20)));
+  uint2* spill = c->b_spill.get<uint2>((size_t)blocks * kBuckets * kHashGlobal);
+  unsigned long long* queue = c->b_queue.get<unsigned long long>(1);
+  unsigned long long* hitcount = c->b_hitcount.get<unsigned long long>(1);
+  unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
+  uint32_t* err = c->b_err.get<uint32_t>(1);
+  uint64_t cap = std::max<uint64_t>(1u << 16, c->b_hits.cap / sizeof(UngHit));
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    UngHit* dh = c->b_hits.get<UngHit>(cap);
+    cap = c->b_hits.cap / sizeof(UngHit);
+    P.out_cap = cap;
+    hip_check(hipMemsetAsync(queue, 0, sizeof(unsigned long long), c->st), "memset queue");
+    hip_check(hipMemsetAsync(hitcount, 0, sizeof(unsigned long long), c->st), "memset hitcount");
+    hip_check(hipMemsetAsync(err, 0, sizeof(uint32_t), c->st), "memset err");
+    hip_check(launch_phase1(blocks, P, drecs, (const SubjDesc*)c->d_sdesc.p, d_qfwd, d_qrc, d_qfwd,
+                            (const uint8_t*)c->d_splus.p, (const uint8_t*)c->d_src.p, cstart, vals2, c->d_mat, dh,
+                            hitcount, spill, queue, ctr, err, c->st),
+              "k_phase1");
+    unsigned long long nh = 0;
+    uint32_t e = 0;
+    hip_check(hipMemcpyAsync(&nh, hitcount, sizeof nh, hipMemcpyDeviceToHost, c->st), "copy hitcount");
+    hip_check(hipMemcpyAsync(&e, err, sizeof e, hipMemcpyDeviceToHost, c->st), "copy err");
+    hip_check(hipStreamSynchronize(c->st), "phase1 sync");
+    if (e & E_HASH_FULL) throw Failure(RMB_ERR_CAPACITY, "diagonal hash bucket exceeded 256 cells");
+    if (nh > cap) {  // output larger than the buffer: grow and rerun the group
+      cap = nh + nh / 4;
+      hip_check(hipMemsetAsync(ctr, 0, C_NCOUNTERS * sizeof(unsigned long long), c->st), "reset counters");
+      continue;
+    }
+    const size_t base = hits_out.size();
+    hits_out.resize(base + nh);
+    hip_check(hipMemcpy(hits_out.data() + base, dh, nh * sizeof(UngHit), hipMemcpyDeviceToHost), "copy hits");
+    for (size_t i = base; i < hits_out.size(); ++i) {  // item ids are group-local: map to batch records
+      const uint32_t r = hits_out[i].item / nsub, f = hits_out[i].item % nsub;
+      hits_out[i].item = recs_idx[r] * nsub + f;
+    }
+    return;
+  }
+  throw Failure(RMB_ERR_CAPACITY, "ungapped hit buffer could not be sized");
+}
+
+static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const uint32_t* lens) {
+  Timer total;
+  if (c->subj.empty()) throw Failure(RMB_ERR_STATE, "no library loaded");
+  std::memset(c->stats, 0, sizeof c->stats);
+  c->hsps.clear();
+  c->ops.clear();
+  const int ws = c->P.word_size;
+  const uint32_t nsub = (uint32_t)c->subj.size();
+  // ---- query pools
+  std::vector<std::vector<uint8_t>> qf(nq), qr(nq);
+  std::vector<RecDesc> recs(nq);
+  uint64_t off = 0;
+  for (uint32_t i = 0; i < nq; ++i) {
+    if (lens[i] > 0x7FFFFFFFu) throw Failure(RMB_ERR_ARG, "query longer than 2^31-1 bases");
+    qf[i] = wrap(qs[i], lens[i]);
+    qr[i] = revcomp_wrapped(qf[i]);
+    recs[i].q_base = off;
+    recs[i].q_len = lens[i];
+    off += lens[i] + 2;
+  }
+  uint8_t* d_qfwd = c->b_qfwd.get<uint8_t>(off);
+  uint8_t* d_qrc = c->b_qrc.get<uint8_t>(off);
+  {
+    std::vector<uint8_t> pf(off), pr(off);
+    for (uint32_t i = 0; i < nq; ++i) {
+      std::memcpy(pf.data() + recs[i].q_base, qf[i].data(), qf[i].size());
+      std::memcpy(pr.data() + recs[i].q_base, qr[i].data(), qr[i].size());
+    }
+    hip_check(hipMemcpy(d_qfwd, pf.data(), off, hipMemcpyHostToDevice), "copy qfwd");
+    hip_check(hipMemcpy(d_qrc, pr.data(), off, hipMemcpyHostToDevice), "copy qrc");
+  }
+  unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
+  hip_check(hipMemset(ctr, 0, C_NCOUNTERS * sizeof(unsigned long long)), "memset counters");
+  // ---- Phase 1 by lookup width; groups are bounded by the cell table size
+  Timer t1;
+  std::vector<UngHit> hits;
+  std::vector<int> lw_of(nq);
+  for (uint32_t i = 0; i < nq; ++i) lw_of[i] = lut_width_for(ws, 2ull * lens[i]);
+  for (int lw = 4; lw <= 12; ++lw) {
+    std::vector<uint32_t> grp;
+    const uint64_t cells = 1ull << (2 * lw);
+    uint64_t elems = 0;
+    for (uint32_t i = 0; i <= nq; ++i) {
+      const bool flush = i == nq || (!grp.empty() && ((grp.size() + 1) * cells > (1ull << 28) ||
+                                                      elems + 2ull * lens[i] + 1 > (1ull << 31)));
+      if (flush && !grp.empty()) {
+        phase1_group(c, grp, recs, lw, d_qfwd, d_qrc, hits);
+        grp.clear();
+        elems = 0;
+      }
+      if (i < nq && lw_of[i] == lw) {
+        grp.push_back(i);
+        elems += 2ull * lens[i] + 1;
+      }
+    }
+  }
+  c->stats[9] = t1.us();
+  // ---- group hits by item, NCBI order per item
+  std::vector<ItemState> items;
+  {
+    std::vector<uint64_t> keyed(hits.size());
+    for (size_t i = 0; i < hits.size(); ++i) keyed[i] = ((uint64_t)hits[i].item << 32) | i;
+    std::sort(keyed.begin(), keyed.end());
+    for (size_t i = 0; i < keyed.size();) {
+      const uint32_t it = (uint32_t)(keyed[i] >> 32);
+      ItemState s;
+      s.item = it;
+      size_t j = i;
+      for (; j < keyed.size() && (uint32_t)(keyed[j] >> 32) == it; ++j) {
+        const UngHit& h = hits[(uint32_t)keyed[j]];
+        s.hits.push_back(UngRec{h.q_seed, h.s_seed, h.q_start, h.q_end, h.s_start, h.s_end, h.score, (uint8_t)h.strand});
+      }
+      items.push_back(std::move(s));
+      i = j;
+    }
+    hits.clear();
+    hits.shrink_to_fit();
+  }
+  parallel_for(items.size(), [&](size_t k) {
+    ItemState& s = items[k];
+    order_ungapped(s.hits, lens[s.item / nsub], c->subj[s.item % nsub].plus.size() - 2);
+  });
+  // ---- Phase 2a: ordered rounds of preliminary score-only DPs
+  Timer t2;
+  const int32_t cutoff = std::max(c->P.min_raw_gapped_score, 1);
+  const uint8_t* d_splus = (const uint8_t*)c->d_splus.p;
+  {
+    std::vector<HspTree> tp, tm;
+    tp.reserve(items.size());
+    tm.reserve(items.size());
+    for (auto& s : items) {
+      const uint32_t ql = lens[s.item / nsub], sl = (uint32_t)c->subj[s.item % nsub].plus.size() - 2;
+      tp.emplace_back(ql + 1, sl + 1);
+      tm.emplace_back(ql + 1, sl + 1);
+    }
+    std::vector<uint32_t> active(items.size());
+    for (size_t i = 0; i < items.size(); ++i) active[i] = (uint32_t)i;
+    uint64_t n_prelim = 0;
+    while (!active.empty()) {
+      // choose this round's DPs: the next uncovered hit (needed) plus a few
+      // likely-needed followers
+      parallel_for(active.size(), [&](size_t a) {
+        ItemState& s = items[active[a]];
+        s.req.clear();
+        while (s.cursor < s.hits.size()) {
+          const UngRec& u = s.hits[s.cursor];
+          const uint32_t sl = (uint32_t)c->subj[s.item % nsub].plus.size() - 2;
+          const uint32_t ss = u.minus ? sl - u.s_end : u.s_start, se = u.minus ? sl - u.s_start : u.s_end;
+          if (!(u.minus ? tm : tp)[active[a]].covers(u.q_start, u.q_end, ss, se, u.score, !u.minus)) break;
+          ++s.cursor;
+        }
+        for (size_t k = s.cursor; k < s.hits.size() && s.req.size() < (size_t)kSpecDepth; ++k) {
+          const UngRec& u = s.hits[k];
+          const uint32_t sl = (uint32_t)c->subj[s.item % nsub].plus.size() - 2;
+          const uint32_t ss = u.minus ? sl - u.s_end : u.s_start, se = u.minus ? sl - u.s_start : u.s_end;
+          if (k == s.cursor || !(u.minus ? tm : tp)[active[a]].covers(u.q_start, u.q_end, ss, se, u.score, !u.minus))
+            s.req.push_back((uint32_t)k);
+        }
+      });
+      std::vector<DpTask> tasks;
+      for (uint32_t a : active) {
+        ItemState& s = items[a];
+        s.task0 = tasks.size();
+        const uint32_t r = s.item / nsub, f = s.item % nsub;
+        const uint32_t ql = lens[r], sl = (uint32_t)c->subj[f].plus.size() - 2;
+        for (uint32_t k : s.req) {
+          const UngRec& u = s.hits[k];
+          uint32_t pq, ps, tq, ts;
+          pivot_of(u, ql, sl, (uint32_t)lw_of[r], pq, ps, tq, ts);
+          const uint8_t* qa = (u.minus ? d_qrc : d_qfwd) + recs[r].q_base + 1;
+          const uint8_t* sa = d_splus + c->sdesc[f].s_base + 1;
+          const int32_t xd = std::min(c->P.xdrop_gap, u.score);
+          tasks.push_back(DpTask{sa, qa, ps, pq, xd, 1});
+          tasks.push_back(DpTask{sa + ps, qa + pq, sl > ps ? sl - ps : 0, ql > pq ? ql - pq : 0, xd, 0});
+        }
+      }
+      n_prelim += tasks.size() / 2;
+      std::vector<DpResult> res;
+      run_dp_score(c, tasks, res);
+      std::vector<uint32_t> still;
+      std::vector<uint8_t> keep_active(active.size(), 0);
+      parallel_for(active.size(), [&](size_t a) {
+        ItemState& s = items[active[a]];
+        const uint32_t r = s.item / nsub, f = s.item % nsub;
+        const uint32_t ql = lens[r], sl = (uint32_t)c->subj[f].plus.size() - 2;
+        size_t ri = 0;
+        while (s.cursor < s.hits.size()) {
+          const UngRec& u = s.hits[s.cursor];
+          const uint32_t ss = u.minus ? sl - u.s_end : u.s_start, se = u.minus ? sl - u.s_start : u.s_end;
+          HspTree& t = (u.minus ? tm : tp)[active[a]];
+          if (t.covers(u.q_start, u.q_end, ss, se, u.score, !u.minus)) { ++s.cursor; continue; }
+          while (ri < s.req.size() && s.req[ri] < s.cursor) ++ri;
+          if (ri >= s.req.size() || s.req[ri] != s.cursor) break;  // no DP yet: next round
+          const DpResult& L = res[s.task0 + 2 * ri];
+          const DpResult& Rr = res[s.task0 + 2 * ri + 1];
+          const int32_t sc = L.score + Rr.score;
+          uint32_t pq, ps, tq, ts;
+          pivot_of(u, ql, sl, (uint32_t)lw_of[r], pq, ps, tq, ts);
+          ++s.cursor;
+          if (sc < 0) continue;
+          uint32_t q0 = pq - L.b_len, q1 = pq + Rr.b_len, s0 = ps - L.a_len, s1 = ps + Rr.a_len;
+          uint32_t qs_, qe_, ss_, se_;
+          if (u.minus) { qs_ = ql - q1; qe_ = ql - q0; ss_ = sl - s1; se_ = sl - s0; }
+          else { qs_ = q0; qe_ = q1; ss_ = s0; se_ = s1; }
+          if (sc < cutoff) continue;
+          const uint32_t nss = u.minus ? sl - se_ : ss_, nse = u.minus ? sl - ss_ : se_;
+          t.add(qs_, qe_, nss, nse, sc, !u.minus);
+          s.pre.push_back(PreRec{tq, ts, qs_, qe_, ss_, se_, sc, u.minus});
+        }
+        keep_active[a] = s.cursor < s.hits.size();
+      });
+      for (size_t a = 0; a < active.size(); ++a)
+        if (keep_active[a]) still.push_back(active[a]);
+      active.swap(still);
+    }
+    c->stats[2] = n_prelim;
+  }
+  parallel_for(items.size(), [&](size_t k) {
+    ItemState& s = items[k];
+    purge_prelims(s.pre, (uint32_t)c->subj[s.item % nsub].plus.size() - 2);
+  });
+  c->stats[10] = t2.us();
+  // ---- Phase 2b: traceback
+  Timer t3;
+  {
+    struct TbRef { uint32_t item; uint32_t pre; uint32_t tq, ts; };
+    std::vector<size_t> first(items.size() + 1, 0);
+    parallel_for(items.size(), [&](size_t k) {
+      ItemState& s = items[k];
+      const uint32_t ql = lens[s.item / nsub];
+      const uint32_t sl = (uint32_t)c->subj[s.item % nsub].align.size() - 2;
+      std::stable_sort(s.pre.begin(), s.pre.end(), [&](const PreRec& a, const PreRec& b) {
+        const uint32_t fsa = a.minus ? sl - a.s_end : a.s_start, fsb = b.minus ? sl - b.s_end : b.s_start;
+        const uint32_t fea = a.minus ? sl - a.s_start : a.s_end, feb = b.minus ? sl - b.s_start : b.s_end;
+        const uint32_t qoa = a.minus ? ql - a.q_end : a.q_start, qob = b.minus ? ql - b.q_end : b.q_start;
+        const uint32_t qea = a.minus ? ql - a.q_start : a.q_end, qeb = b.minus ? ql - b.q_start : b.q_end;
+        if (a.score != b.score) return a.score > b.score;
+        if (fsa != fsb) return fsa < fsb;
+        if (fea != feb) return fea > feb;
+        if (qoa != qob) return qoa < qob;
+        return qea > qeb;
+      });
+    });
+    for (size_t k = 0; k < items.size(); ++k) first[k + 1] = first[k] + items[k].pre.size();
+    std::vector<TbRef> refs(first.back());
+    std::vector<DpTask> tasks(2 * first.back());
+    const uint8_t* d_salign = (const uint8_t*)c->d_salign.p;
+    parallel_for(items.size(), [&](size_t k) {
+      ItemState& s = items[k];
+      const uint32_t r = s.item / nsub, f = s.item % nsub;
+      const uint32_t ql = lens[r];
+      const Subject& S = c->subj[f];
+      const uint32_t sl = (uint32_t)S.align.size() - 2;
+      for (size_t p = 0; p < s.pre.size(); ++p) {
+        const PreRec& h = s.pre[p];
+        const std::vector<uint8_t>& sv = h.minus ? S.align_rc : S.align;
+        uint32_t nq, ns;
+        improve_seed(qf[r].data() + 1, ql, sv.data() + 1, sl, h.q_seed, h.s_seed, h.q_start, h.q_end, h.s_start,
+                     h.s_end, h.minus, nq, ns);
+        const uint32_t tq = h.minus ? ql - 1 - nq : nq, ts = h.minus ? sl - 1 - ns : ns;
+        const size_t g = first[k] + p;
+        refs[g] = TbRef{(uint32_t)k, (uint32_t)p, tq, ts};
+        const uint8_t* qa = (h.minus ? d_qrc : d_qfwd) + recs[r].q_base + 1;
+        const uint8_t* sa = d_salign + c->sdesc[f].s_base + 1;
+        tasks[2 * g] = DpTask{qa, sa, tq + 1, ts + 1, c->P.xdrop_gap_final, 1};
+        const uint32_t rm = ql > tq + 1 ? ql - tq - 1 : 0, rn = sl > ts + 1 ? sl - ts - 1 : 0;
+        tasks[2 * g + 1] = DpTask{qa + tq + 1, sa + ts + 1, rn ? rm : 0, rm ? rn : 0, c->P.xdrop_gap_final, 0};
+      }
+    });
+    std::vector<HalfTb> halves;
+    run_dp_traceback(c, tasks, halves);
+    c->stats[3] = refs.size();
+    const int32_t go = c->P.gap_open, ge = c->P.gap_extend;
+    parallel_for(items.size(), [&](size_t k) {
+      ItemState& s = items[k];
+      const uint32_t r = s.item / nsub, f = s.item % nsub;
+      const uint32_t ql = lens[r];
+      const Subject& S = c->subj[f];
+      const uint32_t sl = (uint32_t)S.align.size() - 2;
+      HspTree tbp(ql + 1, sl + 1), tbm(ql + 1, sl + 1, true);
+      for (size_t p = 0; p < s.pre.size(); ++p) {
+        const PreRec& h = s.pre[p];
+        const bool plus = !h.minus;
+        const uint32_t pss = plus ? h.s_start : sl - h.s_end, pse = plus ? h.s_end : sl - h.s_start;
+        const uint32_t cqs = plus ? h.q_start : ql - h.q_end, cqe = plus ? h.q_end : ql - h.q_start;
+        if ((plus ? tbp : tbm).covers(cqs, cqe, pss, pse, h.score, plus)) continue;
+        const size_t g = first[k] + p;
+        const TbRef& ref = refs[g];
+        const HalfTb& L = halves[2 * g];
+        const HalfTb& R = halves[2 * g + 1];
+        int32_t sc = L.score + R.score;
+        uint32_t q0 = ref.tq + 1 - L.a_len, s0 = ref.ts + 1 - L.b_len;
+        uint32_t q1 = ref.tq + 1 + R.a_len, s1 = ref.ts + 1 + R.b_len;
+        Script ops;
+        for (auto it = L.ops.rbegin(); it != L.ops.rend(); ++it) script_push(ops, it->kind, it->len);
+        for (const Run& x : R.ops) script_push(ops, x.kind, x.len);
+        while (!ops.empty() && ops.front().kind != K_SUB) {  // engine.rs:3016-3033
+          sc += go + (int32_t)ops.front().len * ge;
+          if (ops.front().kind == K_GAP_Q) s0 += ops.front().len; else q0 += ops.front().len;
+          ops.erase(ops.begin());
+        }
+        while (!ops.empty() && ops.back().kind != K_SUB) {
+          sc += go + (int32_t)ops.back().len * ge;
+          if (ops.back().kind == K_GAP_Q) s1 -= ops.back().len; else q1 -= ops.back().len;
+          ops.pop_back();
+        }
+        if (sc < 0) continue;
+        FinalRec fr;
+        fr.minus = h.minus;
+        if (h.minus) {
+          fr.q_start = ql - q1;
+          fr.q_end = ql - q0;
+          fr.s_start = s0;  // forward subject
+          fr.s_end = s1;
+          std::reverse(ops.begin(), ops.end());
+        } else {
+          fr.q_start = q0;
+          fr.q_end = q1;
+          fr.s_start = s0;
+          fr.s_end = s1;
+        }
+        if (c->P.complexity_adjust) {
+          int32_t adj;
+          if (!complexity_adjust(sc, qf[r].data(), qf[r].size(), fr.q_start, ops, c->M, cutoff, h.minus, adj))
+            continue;
+          sc = adj;
+        }
+        fr.score = sc;
+        fr.ops = std::move(ops);
+        const uint32_t tqs = plus ? fr.q_start : ql - fr.q_end, tqe = plus ? fr.q_end : ql - fr.q_start;
+        (plus ? tbp : tbm).add(tqs, tqe, fr.s_start, fr.s_end, sc, true);
+        s.fin.push_back(std::move(fr));
+      }
+      finish_subject(s.fin, qf[r].data(), ql, S.align.data(), S.align_rc.data(), sl, S.n_mask, S.n_mask_rc, c->M,
+                     go, ge, cutoff);
+    });
+  }
+  c->stats[11] = t3.us();
+  // ---- results, item order
+  for (auto& s : items) {
+    for (auto& h : s.fin) {
+      rmb_hsp o{};
+      o.query = s.item / nsub;
+      o.subject = s.item % nsub;
+      o.score = h.score;
+      o.q_start = h.q_start;
+      o.q_end = h.q_end;
+      o.s_start = h.s_start;
+      o.s_end = h.s_end;
+      o.strand = h.minus;
+      o.ops_offset = (uint32_t)c->ops.size();
+      o.ops_count = (uint32_t)h.ops.size();
+      for (const Run& x : h.ops) c->ops.push_back((uint32_t)x.kind | (x.len << 2));
+      c->hsps.push_back(o);
+    }
+  }
+  unsigned long long hc[C_NCOUNTERS];
+  hip_check(hipMemcpy(hc, ctr, sizeof hc, hipMemcpyDeviceToHost), "copy counters");
+  c->stats[0] = hc[C_SEED_EVALS];
+  c->stats[1] = hc[C_UNG_HITS];
+  c->stats[7] = hc[C_UNG_STEPS];
+  c->stats[4] = hc[C_DP_CELLS];  // all DP cells (score passes of both phases)
+  c->stats[6] = c->hsps.size();
+  c->stats[12] = total.us();
+}
+
+}  // namespace rmb
+
+// ============================================================== C ABI =====
+template <class F>
+static int guarded(rmb_ctx* c, F&& fn) {
+  try {
+    fn();
+    return RMB_OK;
+  } catch (const Failure& f) {
+    if (c) c->err = f.what();
+    return f.code;
+  } catch (const std::exception& e) {
+    if (c) c->err = e.what();
+    return RMB_ERR_DEVICE;
+  } catch (...) {
+    if (c) c->err = "unknown failure";
+    return RMB_ERR_DEVICE;
+  }
+}
+
+extern "C" {
+
+int rmb_ctx_create(int device, const rmb_params* params, const rmb_matrix* matrix, rmb_ctx** out) {
+  if (!params || !matrix || !out) return RMB_ERR_ARG;
+  *out = nullptr;
+  rmb_ctx* c = new rmb_ctx();
+  int rc = guarded(c, [&]() {
+    if (params->dust) throw Failure(RMB_ERR_ARG, "dust must be applied by the caller (params.dust = 0)");
+    if (params->word_size < 4) throw Failure(RMB_ERR_ARG, "word_size below 4");
+    if (params->gap_extend < 1 || params->gap_open < 0) throw Failure(RMB_ERR_ARG, "bad gap costs");
+    int n = 0;
+    hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (device < 0 || device >= n) throw Failure(RMB_ERR_ARG, "no such HIP device");
+    c->device = device;
+    hip_check(hipSetDevice(device), "hipSetDevice");
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+    c->cu_count = prop.multiProcessorCount;
+    hip_check(hipStreamCreate(&c->st), "hipStreamCreate");
+    c->P = *params;
+    std::memcpy(c->M.s, matrix->scores, sizeof c->M.s);
+    std::memcpy(c->M.freqs, matrix->freqs, sizeof c->M.freqs);
+    c->M.lambda = matrix->lambda;
+    hip_check(hipMalloc(&c->d_mat, 256 * sizeof(int32_t)), "hipMalloc matrix");
+    hip_check(hipMemcpy(c->d_mat, matrix->scores, 256 * sizeof(int32_t), hipMemcpyHostToDevice), "copy matrix");
+  });
+  if (rc != RMB_OK) {
+    // keep the context so the caller can read the message
+    *out = c;
+    return rc;
+  }
+  *out = c;
+  return RMB_OK;
+}
+
+int rmb_ctx_destroy(rmb_ctx* c) {
+  if (!c) return RMB_OK;
+  (void)hipSetDevice(c->device);
+  for (DevBuf* b : {&c->d_splus, &c->d_src, &c->d_salign, &c->d_sdesc, &c->b_qfwd, &c->b_qrc, &c->b_recs, &c->b_elem,
+                    &c->b_keys, &c->b_keys2, &c->b_vals, &c->b_vals2, &c->b_counts, &c->b_cells, &c->b_sort_tmp,
+                    &c->b_hits, &c->b_hitcount, &c->b_spill, &c->b_queue, &c->b_counters, &c->b_err, &c->b_tasks,
+                    &c->b_dpres, &c->b_ring, &c->b_ring_big, &c->b_tbtasks, &c->b_tbres, &c->b_flat, &c->b_rows,
+                    &c->b_ops})
+    b->release();
+  if (c->d_mat) (void)hipFree(c->d_mat);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+  return RMB_OK;
+}
+
+int rmb_last_error(const rmb_ctx* c, char* buf, size_t len) {
+  if (!c || !buf || len == 0) return RMB_ERR_ARG;
+  std::strncpy(buf, c->err.c_str(), len - 1);
+  buf[len - 1] = 0;
+  return RMB_OK;
+}
+
+int rmb_library_load(rmb_ctx* c, uint32_t n, const uint8_t* const* seqs, const uint32_t* lens) {
+  if (!c) return RMB_ERR_ARG;
+  return guarded(c, [&]() {
+    if (n == 0 || !seqs || !lens) throw Failure(RMB_ERR_ARG, "empty library");
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    c->subj.assign(n, Subject{});
+    c->sdesc.assign(n, SubjDesc{});
+    uint64_t off = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+      if (lens[i] > 0x7FFFFFFFu) throw Failure(RMB_ERR_ARG, "subject longer than 2^31-1 bases");
+      Subject& S = c->subj[i];
+      std::vector<uint8_t> raw = wrap(seqs[i], lens[i]);
+      S.plus = raw;
+      AmbiguityResolver* rng = nullptr;
+      std::vector<AmbiguityResolver> holder;
+      for (uint32_t k = 0; k < lens[i]; ++k) {
+        const uint8_t code = raw[1 + k];
+        if (code >= 4 && code <= 14) {
+          if (!rng) { holder.emplace_back(lens[i]); rng = &holder.back(); }
+          S.plus[1 + k] = rng->base_for(code);
+          if (S.n_mask.empty()) S.n_mask.assign(lens[i], 0);
+          S.n_mask[k] = code;
+        }
+      }
+      S.rc = revcomp_wrapped(S.plus);
+      S.align = raw;  // original codes (the plus strand with ambiguity restored)
+      S.align_rc = revcomp_wrapped(S.align);
+      if (!S.n_mask.empty()) {
+        S.n_mask_rc.resize(lens[i]);
+        for (uint32_t k = 0; k < lens[i]; ++k) {
+          const uint8_t code = S.n_mask[lens[i] - 1 - k];
+          S.n_mask_rc[k] = code ? kBlastnaComplement[code] : 0;
+        }
+      }
+      c->sdesc[i].s_base = off;
+      c->sdesc[i].s_len = lens[i];
+      off += lens[i] + 2;
+    }
+    std::vector<uint8_t> pp(off), pr(off), pa(off);
+    for (uint32_t i = 0; i < n; ++i) {
+      std::memcpy(pp.data() + c->sdesc[i].s_base, c->subj[i].plus.data(), c->subj[i].plus.size());
+      std::memcpy(pr.data() + c->sdesc[i].s_base, c->subj[i].rc.data(), c->subj[i].rc.size());
+      std::memcpy(pa.data() + c->sdesc[i].s_base, c->subj[i].align.data(), c->subj[i].align.size());
+    }
+    hip_check(hipMemcpy(c->d_splus.get<uint8_t>(off), pp.data(), off, hipMemcpyHostToDevice), "copy subjects");
+    hip_check(hipMemcpy(c->d_src.get<uint8_t>(off), pr.data(), off, hipMemcpyHostToDevice), "copy subjects rc");
+    hip_check(hipMemcpy(c->d_salign.get<uint8_t>(off), pa.data(), off, hipMemcpyHostToDevice), "copy subjects align");
+    hip_check(hipMemcpy(c->d_sdesc.get<SubjDesc>(n), c->sdesc.data(), n * sizeof(SubjDesc), hipMemcpyHostToDevice),
+              "copy subject table");
+  });
+}
+
+int rmb_search(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const uint32_t* lens) {
+  if (!c) return RMB_ERR_ARG;
+  return guarded(c, [&]() {
+    if (nq == 0 || !qs || !lens) throw Failure(RMB_ERR_ARG, "empty query batch");
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    search_impl(c, nq, qs, lens);
+  });
+}
+
+int rmb_results_size(const rmb_ctx* c, uint64_t* n_hsps, uint64_t* n_ops) {
+  if (!c || !n_hsps || !n_ops) return RMB_ERR_ARG;
+  *n_hsps = c->hsps.size();
+  *n_ops = c->ops.size();
+  return RMB_OK;
+}
+
+int rmb_results_copy(const rmb_ctx* c, rmb_hsp* hsps, rmb_op* ops) {
+  if (!c) return RMB_ERR_ARG;
+  if (hsps && !c->hsps.empty()) std::memcpy(hsps, c->hsps.data(), c->hsps.size() * sizeof(rmb_hsp));
+  if (ops && !c->ops.empty()) std::memcpy(ops, c->ops.data(), c->ops.size() * sizeof(rmb_op));
+  return RMB_OK;
+}
+
+int rmb_stats(const rmb_ctx* c, uint64_t* out, int n) {
+  if (!c || !out) return RMB_ERR_ARG;
+  for (int i = 0; i < n && i < 16; ++i) out[i] = c->stats[i];
+  return RMB_OK;
+}
+
+int rmb_tap_ungapped(rmb_ctx* c, const uint8_t* query, uint32_t q_len, uint32_t subject, uint32_t* out, uint64_t* n) {
+  if (!c || !query || !n) return RMB_ERR_ARG;
+  return guarded(c, [&]() {
+    if (subject >= c->subj.size()) throw Failure(RMB_ERR_ARG, "subject index out of range");
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    std::vector<uint8_t> qf = wrap(query, q_len), qr = revcomp_wrapped(qf);
+    uint8_t* d_qfwd = c->b_qfwd.get<uint8_t>(qf.size());
+    uint8_t* d_qrc = c->b_qrc.get<uint8_t>(qr.size());
+    hip_check(hipMemcpy(d_qfwd, qf.data(), qf.size(), hipMemcpyHostToDevice), "copy q");
+    hip_check(hipMemcpy(d_qrc, qr.data(), qr.size(), hipMemcpyHostToDevice), "copy qrc");
+    std::vector<RecDesc> recs(1);
+    recs[0].q_base = 0;
+    recs[0].q_len = q_len;
+    std::vector<UngHit> hits;
+    unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
+    hip_check(hipMemset(ctr, 0, C_NCOUNTERS * sizeof(unsigned long long)), "memset counters");
+    const int lw = lut_width_for(c->P.word_size, 2ull * q_len);
+    if (q_len >= (uint32_t)c->P.word_size) phase1_group(c, {0u}, recs, lw, d_qfwd, d_qrc, hits);
+    std::vector<UngRec> v;
+    for (auto& h : hits)
+      if (h.item % c->subj.size() == subject)
+        v.push_back(UngRec{h.q_seed, h.s_seed, h.q_start, h.q_end, h.s_start, h.s_end, h.score, (uint8_t)h.strand});
+    order_ungapped(v, q_len, (uint32_t)c->subj[subject].plus.size() - 2);
+    if (out) {
+      const uint64_t cap = *n;
+      for (size_t i = 0; i < v.size() && i < cap; ++i) {
+        uint32_t* o = out + 8 * i;
+        o[0] = v[i].q_seed; o[1] = v[i].s_seed; o[2] = v[i].q_start; o[3] = v[i].q_end;
+        o[4] = v[i].s_start; o[5] = v[i].s_end; o[6] = (uint32_t)v[i].score; o[7] = v[i].minus;
+      }
+    }
+    *n = v.size();
+  });
+}
+
+static void tap_upload(rmb_ctx* c, const uint8_t* a, uint32_t m, const uint8_t* b, uint32_t n, const uint8_t*& da,
+                       const uint8_t*& db) {
+  uint8_t* d = c->b_qfwd.get<uint8_t>((size_t)m + n + 2);
+  hip_check(hipMemcpy(d, a, m, hipMemcpyHostToDevice), "copy a");
+  hip_check(hipMemcpy(d + m + 1, b, n, hipMemcpyHostToDevice), "copy b");
+  da = d;
+  db = d + m + 1;
+}
+
+int rmb_tap_align_score_only(rmb_ctx* c, const uint8_t* a, uint32_t m, const uint8_t* b, uint32_t n, int32_t xdrop,
+                             int32_t reverse, int64_t out[3]) {
+  if (!c || !out || (m && !a) || (n && !b)) return RMB_ERR_ARG;
+  return guarded(c, [&]() {
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    const uint8_t *da, *db;
+    tap_upload(c, a, m, b, n, da, db);
+    std::vector<DpTask> t{DpTask{da, db, m, n, xdrop, (uint32_t)(reverse != 0)}};
+    std::vector<DpResult> r;
+    run_dp_score(c, t, r);
+    out[0] = r[0].score;
+    out[1] = r[0].a_len;
+    out[2] = r[0].b_len;
+  });
+}
+
+int rmb_tap_align_ex(rmb_ctx* c, const uint8_t* a, uint32_t m, const uint8_t* b, uint32_t n, int32_t xdrop,
+                     int32_t reverse, int64_t out[4], rmb_op* ops, uint32_t max_ops) {
+  if (!c || !out || (m && !a) || (n && !b)) return RMB_ERR_ARG;
+  return guarded(c, [&]() {
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    const uint8_t *da, *db;
+    tap_upload(c, a, m, b, n, da, db);
+    std::vector<DpTask> t{DpTask{da, db, m, n, xdrop, (uint32_t)(reverse != 0)}};
+    std::vector<HalfTb> r;
+    run_dp_traceback(c, t, r);
+    out[0] = r[0].score;
+    out[1] = r[0].a_len;
+    out[2] = r[0].b_len;
+    out[3] = (int64_t)r[0].ops.size();
+    for (size_t i = 0; i < r[0].ops.size() && i < max_ops; ++i) ops[i] = r[0].ops[i].kind | (r[0].ops[i].len << 2);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,439 @@
+// rmblastn front end over librmblast_hip (include/rmblast_hip.h).
+//
+// Keeps the reference CLI's flags and NCBI spellings (rmblastn/src/main.rs:64-281)
+// and its outfmt 6 fields (rmblast-lib/src/output.rs:142-363).  The GPU library
+// returns each (query, subject) pair's final HSPs; this program does what the
+// reference does after the search: mask level across subjects
+// (engine.rs:863-974), NCBI output order (main.rs:1230-1234, engine.rs:998-1017),
+// alignment statistics (stats.rs:52-243) and the tabular writer.
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/rmblast_hip.h"
+
+namespace {
+
+const char kIupac[] = "ACGTRYMKWSBDHVN-";
+const uint8_t kComp[16] = {3, 2, 1, 0, 5, 4, 7, 6, 8, 9, 13, 12, 11, 10, 14, 15};
+
+uint8_t code_of(char ch) {
+  switch (std::toupper((unsigned char)ch)) {
+    case 'A': return 0; case 'C': return 1; case 'G': return 2; case 'T': case 'U': return 3;
+    case 'R': return 4; case 'Y': return 5; case 'M': return 6; case 'K': return 7; case 'W': return 8;
+    case 'S': return 9; case 'B': return 10; case 'D': return 11; case 'H': return 12; case 'V': return 13;
+    default: return 14;  // N, X, '-' and anything else read as N (encoding.rs:19-52)
+  }
+}
+
+struct Record { std::string id; std::vector<uint8_t> bases; };
+
+bool load_fasta(const std::string& path, std::vector<Record>& out, std::string& err) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) { err = "cannot open " + path; return false; }
+  std::string line;
+  Record cur;
+  bool open = false;
+  while (std::getline(f, line)) {
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    if (!line.empty() && line[0] == '>') {
+      if (open) out.push_back(std::move(cur));
+      cur = Record{};
+      std::istringstream is(line.substr(1));
+      is >> cur.id;
+      if (cur.id.empty()) { err = "'>' with no sequence id"; return false; }
+      open = true;
+      continue;
+    }
+    size_t a = line.find_first_not_of(" \t"), b = line.find_last_not_of(" \t");
+    if (a == std::string::npos) continue;
+    if (!open) { err = "sequence data before the first header"; return false; }
+    for (size_t i = a; i <= b; ++i) cur.bases.push_back(code_of(line[i]));
+  }
+  if (open) out.push_back(std::move(cur));
+  return true;
+}
+
+// RMBlast matrix file (matrix.rs:76-213): FREQS comment, header row, rows.
+bool load_matrix(const std::string& path, rmb_matrix& m, std::string& err) {
+  std::ifstream f(path);
+  if (!f) { err = "cannot open matrix " + path; return false; }
+  auto slot = [](char c) -> int {
+    const char* s = "ACGTRYMKWSBDHVN";
+    const char* p = std::strchr(s, c);
+    return (p && c) ? (int)(p - s) : 15;
+  };
+  for (auto& r : m.scores)
+    for (auto& v : r) v = -32768;
+  std::memset(m.freqs, 0, sizeof m.freqs);
+  std::vector<char> cols;
+  int row = 0;
+  bool freqs = false;
+  std::string line;
+  while (std::getline(f, line)) {
+    std::istringstream is(line);
+    std::string tok;
+    if (!(is >> tok)) continue;
+    if (tok[0] == '#') {
+      size_t at = line.find("FREQS");
+      if (at == std::string::npos) continue;
+      std::istringstream fs(line.substr(at + 5));
+      std::string b;
+      double v;
+      int got = 0;
+      while (fs >> b >> v) {
+        const int k = slot((char)std::toupper((unsigned char)b[0]));
+        if (k > 3 || b.size() != 1) { err = "bad FREQS"; return false; }
+        m.freqs[k] = v;
+        ++got;
+      }
+      if (got != 4) { err = "FREQS needs A, C, G and T"; return false; }
+      freqs = true;
+      continue;
+    }
+    if (cols.empty()) {
+      do cols.push_back((char)std::toupper((unsigned char)tok[0])); while (is >> tok);
+      continue;
+    }
+    if (row >= (int)cols.size()) { err = "more rows than columns"; return false; }
+    std::vector<int> vals;
+    bool first = true;
+    do {
+      char* e = nullptr;
+      long v = std::strtol(tok.c_str(), &e, 10);
+      if (*e) {
+        if (first && tok.size() == 1) { first = false; continue; }  // row label
+        err = "score '" + tok + "' is not an integer";
+        return false;
+      }
+      first = false;
+      vals.push_back((int)v);
+    } while (is >> tok);
+    if (vals.size() != cols.size()) { err = "row length differs from the header"; return false; }
+    for (size_t c = 0; c < cols.size(); ++c) m.scores[slot(cols[row])][slot(cols[c])] = vals[c];
+    ++row;
+  }
+  if (!freqs) { err = "no FREQS line"; return false; }
+  if (row != (int)cols.size()) { err = "matrix must be square"; return false; }
+  for (int i = 0; i < 16; ++i) { m.scores[15][i] = INT32_MIN / 2; m.scores[i][15] = INT32_MIN / 2; }
+  double expected = 0;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) expected += m.freqs[i] * m.freqs[j] * m.scores[i][j];
+  if (expected >= 0) { err = "expected pair score must be negative"; return false; }
+  // lambda: sum_ij f_i f_j exp(lambda s_ij) = 1, doubling then bisection to 1e-5 (matrix.rs:306-354)
+  auto g = [&](double lam) {
+    double acc = 0;
+    for (int i = 0; i < 16; ++i)
+      for (int j = 0; j < 16; ++j)
+        if (m.freqs[i] > 0.0 && m.freqs[j] > 0.0) acc += m.freqs[i] * m.freqs[j] * std::exp(lam * (double)m.scores[i][j]);
+    return acc;
+  };
+  double lo = 0, x = 0.5;
+  m.lambda = 0;
+  while (g(x) < 1.0) {
+    lo = x;
+    x *= 2;
+    if (x > 1000.0) return true;
+  }
+  double hi = x;
+  while (hi - lo > 0.00001) {
+    x = (lo + hi) / 2;
+    if (g(x) >= 1.0) hi = x; else lo = x;
+  }
+  m.lambda = x;
+  return true;
+}
+
+struct Stats { uint32_t mism = 0, qgap = 0, sgap = 0, ti = 0, tv = 0, cpg = 0; double kdiv = 0, cpgk = 0; float ps = 0, pq = 0, pd = 0; };
+
+// stats.rs:52-243 with the subject as the ancestral sequence.
+Stats align_stats(const std::string& q, const std::string& s) {
+  auto pure = [](char c) { return c == 'A' || c == 'C' || c == 'G' || c == 'T'; };
+  auto bits = [](char c) -> int {
+    switch (c) {
+      case 'A': return 1; case 'C': return 2; case 'G': return 4; case 'T': return 8; case 'R': return 5;
+      case 'Y': return 10; case 'M': return 3; case 'K': return 12; case 'W': return 9; case 'S': return 6;
+      case 'B': return 14; case 'D': return 13; case 'H': return 11; case 'V': return 7; default: return 15;
+    }
+  };
+  enum { ID, TI, TV, IUBM, IUBS, GAP, NONE };
+  auto kind = [&](char a, char d) {
+    if (a == '-' || d == '-') return (int)GAP;
+    if (a == d) return (int)ID;
+    if (!pure(a) || !pure(d)) return ((pure(a) != pure(d)) && (bits(a) & bits(d))) ? (int)IUBM : (int)IUBS;
+    return ((a == 'A' || a == 'G') == (d == 'A' || d == 'G')) ? (int)TI : (int)TV;
+  };
+  Stats st;
+  uint32_t pairs = 0, qlen = 0, dlen = 0;
+  double cti = 0;
+  char prev = ' ';
+  int prevk = NONE;
+  for (size_t i = 0; i < s.size(); ++i) {
+    const char a = s[i], d = q[i];
+    if (d != '-') ++qlen;
+    if (a != '-') ++dlen;
+    if (prev == 'C' && a == 'G') {
+      ++st.cpg;
+      const int k = kind(a, d);
+      if (prevk == TI && k == TI) cti -= 1.0;
+      else if (prevk == TI || k == TI) cti -= 0.9;
+    }
+    if (a == '-' || d == '-') {
+      if (d == '-') ++st.qgap; else ++st.sgap;
+      if (a != '-') { prev = a; prevk = GAP; }
+      continue;
+    }
+    const int k = kind(a, d);
+    const bool both = pure(a) && pure(d);
+    if (k == ID) { if (both) ++pairs; }
+    else if (k == TI) { ++st.mism; if (both) { ++pairs; ++st.ti; cti += 1.0; } }
+    else if (k == TV) { ++st.mism; if (both) { ++pairs; ++st.tv; } }
+    else if (k == IUBS) ++st.mism;
+    prev = a;
+    prevk = k;
+  }
+  auto kimura = [](double p, double qq) {
+    const double o = (1.0 - 2.0 * p - qq) * std::sqrt(1.0 - 2.0 * qq);
+    return o > 0.0 ? std::fabs(-0.5 * std::log(o)) * 100.0 : 100.0;
+  };
+  if (pairs < 1) { st.kdiv = st.cpgk = 100.0; }
+  else {
+    const double n = pairs;
+    st.kdiv = kimura(st.ti / n, st.tv / n);
+    st.cpgk = kimura(cti / n, st.tv / n);
+  }
+  st.ps = qlen ? ((float)st.mism / (float)qlen) * 100.0f : 0.0f;
+  st.pq = qlen ? ((float)st.qgap / (float)qlen) * 100.0f : 0.0f;
+  st.pd = dlen ? ((float)st.sgap / (float)dlen) * 100.0f : 0.0f;
+  return st;
+}
+
+struct Row {
+  rmb_hsp h;
+  uint32_t q_len, s_len;
+  std::string qa, sa;  // aligned IUPAC strings
+};
+
+std::string norm(std::string a) {
+  if (a.size() > 1 && a[0] == '-' && a[1] != '-') a = "-" + a;
+  for (auto& c : a) if (c == '_') c = '-';
+  return a;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  std::map<std::string, std::string> kv;
+  bool complexity = false;
+  for (int i = 1; i < argc; ++i) {
+    const std::string f = norm(argv[i]);
+    if (f == "--complexity-adjust") { complexity = true; continue; }
+    if (i + 1 >= argc) { fprintf(stderr, "error: %s needs a value\n", argv[i]); return 2; }
+    kv[f] = argv[++i];
+  }
+  auto opt = [&](const char* k, const char* d) { auto it = kv.find(k); return it == kv.end() ? std::string(d) : it->second; };
+  for (const char* req : {"--db", "--query", "--gapopen", "--gapextend"})
+    if (!kv.count(req)) { fprintf(stderr, "error: %s is required\n", req); return 2; }
+  rmb_params P{};
+  P.gap_open = std::stoi(kv["--gapopen"]);
+  P.gap_extend = std::stoi(kv["--gapextend"]);
+  P.word_size = std::stoi(opt("--word-size", "11"));
+  P.xdrop_ungap = std::stoi(opt("--xdrop-ungap", "20"));
+  P.xdrop_gap = std::stoi(opt("--xdrop-gap", "30"));
+  P.xdrop_gap_final = std::stoi(opt("--xdrop-gap-final", "100"));
+  P.min_raw_gapped_score = std::stoi(opt("--min-raw-gapped-score", "0"));
+  P.ungapped_cutoff = -1;
+  P.complexity_adjust = complexity;
+  const std::string dust = opt("--dust", "yes");
+  if (dust != "no" && dust != "false") {
+    fprintf(stderr, "error: --dust yes is not available in this build; pass --dust no\n");
+    return 1;
+  }
+  int mask_level = std::stoi(opt("--mask-level", "-1"));
+  if (mask_level < 0) mask_level = 101;
+  std::vector<std::string> fields;
+  {
+    std::istringstream is(opt("--outfmt", "6 score qseqid qstart qend sstrand sseqid sstart send"));
+    std::string w;
+    is >> w;
+    if (w != "6") { fprintf(stderr, "error: only -outfmt 6 <fields> is supported\n"); return 1; }
+    while (is >> w) fields.push_back(w);
+  }
+  std::string err;
+  rmb_matrix M;
+  if (!load_matrix(opt("--matrix", ""), M, err)) { fprintf(stderr, "error: %s\n", err.c_str()); return 1; }
+  std::vector<Record> db, qs;
+  if (!load_fasta(kv["--db"], db, err) || !load_fasta(kv["--query"], qs, err)) {
+    fprintf(stderr, "error: %s\n", err.c_str());
+    return 1;
+  }
+  std::map<std::string, size_t> oid;
+  for (size_t i = 0; i < db.size(); ++i)
+    if (!oid.emplace(db[i].id, i).second) { fprintf(stderr, "error: duplicate subject id %s\n", db[i].id.c_str()); return 1; }
+  rmb_ctx* ctx = nullptr;
+  int rc = rmb_ctx_create(std::stoi(opt("--device", "0")), &P, &M, &ctx);
+  auto fail = [&](const char* what) {
+    char buf[512] = {0};
+    if (ctx) rmb_last_error(ctx, buf, sizeof buf);
+    fprintf(stderr, "error: %s: %s\n", what, buf);
+    rmb_ctx_destroy(ctx);
+    return 1;
+  };
+  if (rc) return fail("device");
+  {
+    std::vector<const uint8_t*> ptr;
+    std::vector<uint32_t> len;
+    for (auto& r : db) { ptr.push_back(r.bases.data()); len.push_back((uint32_t)r.bases.size()); }
+    if (rmb_library_load(ctx, (uint32_t)db.size(), ptr.data(), len.data())) return fail("library");
+  }
+  std::vector<const uint8_t*> qp;
+  std::vector<uint32_t> ql;
+  for (auto& r : qs) { qp.push_back(r.bases.data()); ql.push_back((uint32_t)r.bases.size()); }
+  if (qs.empty()) { rmb_ctx_destroy(ctx); return 0; }
+  if (rmb_search(ctx, (uint32_t)qs.size(), qp.data(), ql.data())) return fail("search");
+  uint64_t nh = 0, no = 0;
+  rmb_results_size(ctx, &nh, &no);
+  std::vector<rmb_hsp> H(nh);
+  std::vector<rmb_op> O(no);
+  rmb_results_copy(ctx, H.data(), O.data());
+  std::vector<std::vector<Row>> per_query(qs.size());
+  for (const rmb_hsp& h : H) {
+    const Record& Q = qs[h.query];
+    const Record& S = db[h.subject];
+    Row r;
+    r.h = h;
+    r.q_len = (uint32_t)Q.bases.size();
+    r.s_len = (uint32_t)S.bases.size();
+    // aligned columns in the forward query frame; the minus subject is read reverse-complemented
+    size_t qi = h.q_start, si = h.strand ? S.bases.size() - h.s_end : h.s_start;
+    for (uint32_t k = 0; k < h.ops_count; ++k) {
+      const uint32_t kindv = O[h.ops_offset + k] & 3, n = O[h.ops_offset + k] >> 2;
+      for (uint32_t j = 0; j < n; ++j) {
+        if (kindv != 1) r.qa += kIupac[Q.bases[qi++] & 15]; else r.qa += '-';
+        if (kindv != 2) {
+          const uint8_t b = h.strand ? kComp[S.bases[S.bases.size() - 1 - si] & 15] : (S.bases[si] & 15);
+          ++si;
+          r.sa += kIupac[b];
+        } else {
+          r.sa += '-';
+        }
+      }
+    }
+    per_query[h.query].push_back(std::move(r));
+  }
+  std::string out;
+  for (size_t qi = 0; qi < qs.size(); ++qi) {
+    std::vector<Row>& rows = per_query[qi];
+    auto qoff = [](const Row& r) { return r.h.strand ? r.q_len - r.h.q_end : r.h.q_start; };
+    auto qend = [](const Row& r) { return r.h.strand ? r.q_len - r.h.q_start : r.h.q_end; };
+    auto better = [&](const Row& a, const Row& b) {  // ScoreCompareHSPs (hits.rs:208-216); <0 if a first
+      if (a.h.score != b.h.score) return a.h.score > b.h.score ? -1 : 1;
+      if (a.h.s_start != b.h.s_start) return a.h.s_start < b.h.s_start ? -1 : 1;
+      if (a.h.s_end != b.h.s_end) return a.h.s_end > b.h.s_end ? -1 : 1;
+      if (qoff(a) != qoff(b)) return qoff(a) < qoff(b) ? -1 : 1;
+      if (qend(a) != qend(b)) return qend(a) > qend(b) ? -1 : 1;
+      return 0;
+    };
+    if (mask_level < 100) {  // engine.rs:863-974, cross-subject form
+      std::sort(rows.begin(), rows.end(), [&](const Row& a, const Row& b) {
+        if (a.h.score != b.h.score) return a.h.score > b.h.score;
+        if (a.h.subject != b.h.subject) return a.h.subject > b.h.subject;
+        return better(a, b) < 0;
+      });
+      struct Span { uint32_t s, e; int32_t sc; };
+      std::vector<Span> kept;  // by start
+      uint32_t widest = 0;
+      std::vector<Row> survivors;
+      for (Row& r : rows) {
+        uint32_t s = r.h.q_start, e = r.h.q_end;
+        if (r.h.strand) { ++s; ++e; }
+        const int64_t span = (int64_t)e - s;
+        if (span == 0) continue;
+        const bool touch = mask_level == 0;
+        const uint32_t hi = touch ? e : e - 1, lo = s > widest ? s - widest : 0;
+        bool masked = false;
+        auto it = std::lower_bound(kept.begin(), kept.end(), lo, [](const Span& x, uint32_t v) { return x.s < v; });
+        for (; it != kept.end() && it->s <= hi && !masked; ++it) {
+          if (it->e < s || (it->e == s && !touch) || it->sc < r.h.score) continue;
+          const int64_t ov = (int64_t)std::min(it->e, e) - (int64_t)std::max(it->s, s);
+          if ((int64_t)(100.0 * ((double)ov / (double)span)) >= mask_level) masked = true;
+        }
+        if (masked) continue;
+        kept.insert(std::upper_bound(kept.begin(), kept.end(), s, [](uint32_t v, const Span& x) { return v < x.s; }),
+                    Span{s, e, r.h.score});
+        widest = std::max(widest, e - s);
+        survivors.push_back(std::move(r));
+      }
+      rows.swap(survivors);
+    }
+    std::sort(rows.begin(), rows.end(), [&](const Row& a, const Row& b) {  // main.rs:1230-1234
+      const std::string &ia = db[a.h.subject].id, &ib = db[b.h.subject].id;
+      if (ia != ib) return ia < ib;
+      if (a.h.strand != b.h.strand) return a.h.strand < b.h.strand;
+      return a.h.q_start < b.h.q_start;
+    });
+    std::map<uint32_t, int32_t> top;  // engine.rs:998-1017
+    for (auto& r : rows) {
+      auto it = top.emplace(r.h.subject, INT32_MIN).first;
+      it->second = std::max(it->second, r.h.score);
+    }
+    std::stable_sort(rows.begin(), rows.end(), [&](const Row& a, const Row& b) {
+      const int32_t ta = top[a.h.subject], tb = top[b.h.subject];
+      if (ta != tb) return ta > tb;
+      if (a.h.subject != b.h.subject) return a.h.subject > b.h.subject;
+      return better(a, b) < 0;
+    });
+    char buf[64];
+    for (const Row& r : rows) {
+      const Stats st = align_stats(r.qa, r.sa);
+      for (size_t k = 0; k < fields.size(); ++k) {
+        if (k) out += '\t';
+        const std::string& f = fields[k];
+        const rmb_hsp& h = r.h;
+        if (f == "score") out += std::to_string(h.score);
+        else if (f == "perc_sub") { snprintf(buf, sizeof buf, "%.2f", (double)st.ps); out += buf; }
+        else if (f == "perc_query_gap") { snprintf(buf, sizeof buf, "%.2f", (double)st.pq); out += buf; }
+        else if (f == "perc_db_gap") { snprintf(buf, sizeof buf, "%.2f", (double)st.pd); out += buf; }
+        else if (f == "qseqid") out += qs[qi].id;
+        else if (f == "qstart") out += std::to_string(h.q_start + 1);
+        else if (f == "qend") out += std::to_string(h.q_end);
+        else if (f == "qlen") out += std::to_string(r.q_len);
+        else if (f == "sstrand") out += h.strand ? "minus" : "plus";
+        else if (f == "sseqid") out += db[h.subject].id;
+        else if (f == "sstart") out += std::to_string(h.strand ? h.s_end : h.s_start + 1);
+        else if (f == "send") out += std::to_string(h.strand ? h.s_start + 1 : h.s_end);
+        else if (f == "slen") out += std::to_string(r.s_len);
+        else if (f == "kdiv") { snprintf(buf, sizeof buf, "%.2f", st.kdiv); out += buf; }
+        else if (f == "cpg_kdiv") { snprintf(buf, sizeof buf, "%.2f", st.cpgk); out += buf; }
+        else if (f == "transi") out += std::to_string(st.tv);     // NCBI column quirks (output.rs:338-340)
+        else if (f == "transv") out += std::to_string(st.cpg);
+        else if (f == "cpg_sites") out += std::to_string(st.ti);
+        else if (f == "qseq") out += r.qa;
+        else if (f == "sseq") out += r.sa;
+        else if (f == "evalue") out += "1.0";
+        else if (f == "bitscore") out += " 0.0";
+        else { fprintf(stderr, "error: unknown -outfmt field '%s'\n", f.c_str()); return 1; }
+      }
+      out += '\n';
+    }
+  }
+  fwrite(out.data(), 1, out.size(), stdout);
+  if (std::getenv("RMB_STATS")) {
+    uint64_t s[16];
+    rmb_stats(ctx, s, 16);
+    fprintf(stderr, "seed_evals %llu ungapped_hits %llu prelim_dps %llu tb_dps %llu dp_cells %llu hsps %llu "
+            "ung_steps %llu t_seed_us %llu t_prelim_us %llu t_tb_us %llu t_total_us %llu\n",
+            (unsigned long long)s[0], (unsigned long long)s[1], (unsigned long long)s[2], (unsigned long long)s[3],
+            (unsigned long long)s[4], (unsigned long long)s[6], (unsigned long long)s[7], (unsigned long long)s[9],
+            (unsigned long long)s[10], (unsigned long long)s[11], (unsigned long long)s[12]);
+  }
+  rmb_ctx_destroy(ctx);
+  return 0;
+}
