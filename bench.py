@@ -1,0 +1,167 @@
+"""Benchmark: the exact search path at scale on one MI355X.
+
+Workload (the exact-path-at-scale configuration): a 100 Mbp synthetic genome
+split into 1,667 records of 60 kbp, searched on both strands against 1,000
+synthetic families (log-normal lengths, 100-15,000 bp), word size 7, gap
+costs 20/5, X-drop 20/30/100, min_raw_gapped_score 200, complexity adjustment,
+no DUST.  One step = one rmb_search over all records (lookup build, seeding,
+ungapped, preliminary and traceback DP, ordered replay), with the library
+already resident on the GPU.  Metric: query Mbp x families per second.
+
+Prints one JSON line.  --gpus N with N > 1 is refused: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+INT32_VALU_PEAK = 256 * 64 * 2.4e9   # 256 CUs x 64 lanes x 2.4 GHz, int32 ops/s
+OPS_PER_UNGAPPED_STEP = 6
+OPS_PER_DP_CELL = 12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--records", type=int, default=1667, help="records of the 100 Mbp genome to search")
+    ap.add_argument("--families", type=int, default=1000)
+    ap.add_argument("--cpu-records", type=int, default=12, help="records in the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported", file=sys.stderr)
+        return 2
+    import rmblast_amd
+    from rmblast_amd import synth
+
+    t0 = time.time()
+    w = synth.scale_config()
+    w = synth.subset(w, a.records, a.families)
+    gen_s = time.time() - t0
+    print(f"[bench] workload generated in {gen_s:.1f}s: {len(w.records)} records x {len(w.families)} families",
+          file=sys.stderr, flush=True)
+    tmp = tempfile.mkdtemp(prefix="rmb_bench_")
+    mpath = os.path.join(tmp, "synth.matrix")
+    with open(mpath, "w") as f:
+        f.write(synth.MATRIX_TEXT)
+    m = rmblast_amd.load_matrix(mpath)
+    params = dict(gap_open=20, gap_extend=5, word_size=7, xdrop_ungap=20, xdrop_gap=30, xdrop_gap_final=100,
+                  min_raw_gapped_score=200, complexity_adjust=True)
+    s = rmblast_amd.Searcher(m, **params)
+    s.load_library([f[1] for f in w.families])
+    queries = [r[1] for r in w.records]
+    mbp = sum(len(q) for q in queries) / 1e6
+    nfam = len(w.families)
+
+    for i in range(a.warmup):
+        s.search(queries)
+        print(f"[bench] warmup {i + 1}/{a.warmup} done", file=sys.stderr, flush=True)
+    times, stats = [], []
+    for i in range(a.steps):
+        t = time.perf_counter()
+        hsps, ops = s.search(queries)
+        times.append(time.perf_counter() - t)
+        stats.append(s.stats())
+        print(f"[bench] step {i + 1}/{a.steps}: {times[-1]:.3f}s, {len(hsps)} HSPs", file=sys.stderr, flush=True)
+    total = sum(times)
+    step_s = total / a.steps
+    st = stats[-1]
+    value = mbp * nfam / step_s
+
+    # roofline of the dominant kernel (the seeding kernel): bytes it must stream
+    # per launch = for every (record, family) item the family's packed strand
+    # (L/4 bytes) + one 4-byte lookup entry per seed + one 4-byte cell start per
+    # scanned position; achieved = bytes / HIP-event kernel time.
+    fam_len = np.array([len(f[1]) for f in w.families], dtype=np.float64)
+    positions = float(len(queries)) * float(np.maximum(fam_len - 6, 0).sum())
+    algo_bytes = float(len(queries)) * float(np.ceil(fam_len / 4).sum()) + 4.0 * st["seed_evals"] + 4.0 * positions
+    k_ns = max(st["t_phase1_kernel_ns"], 1)
+    achieved = algo_bytes / (k_ns * 1e-9) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "phase1_traffic.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch_measured")
+    valu_ops = OPS_PER_UNGAPPED_STEP * st["ungapped_steps"]
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "kernel": "k_phase1", "kernel_ms": round(k_ns / 1e6, 3),
+                "int32_valu": {"ops": valu_ops, "achieved_tops": round(valu_ops / (k_ns * 1e-9) / 1e12, 3),
+                               "peak_tops": round(INT32_VALU_PEAK / 1e12, 1),
+                               "frac": round(valu_ops / (k_ns * 1e-9) / INT32_VALU_PEAK, 5)}}
+    dp_s = (st["t_prelim_us"] + st["t_traceback_us"]) / 1e6
+    gcups = st["dp_cells"] / dp_s / 1e9 if dp_s > 0 else 0.0
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(w, tmp, mpath, a.cpu_records)
+    line = {
+        "metric": "query Mbp x families per second (exact search path)",
+        "value": round(value, 2),
+        "unit": "Mbp*families/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(step_s * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded generator, rmblast_amd/synth.py)",
+        "config": {"workload": "exact path at scale: 1,667 x 60 kbp records x 1,000 families, ws 7",
+                   "records": len(queries), "families": nfam, "query_mbp": round(mbp, 3), "word_size": 7,
+                   "min_raw_gapped_score": 200, "xdrop": [20, 30, 100], "complexity_adjust": True, "dust": False},
+        "roofline": roofline,
+        "gapped_gcups": round(gcups, 3),
+        "stage_ms": {k: round(st[k] / 1e3, 2) for k in ("t_seed_us", "t_prelim_us", "t_traceback_us", "t_total_us")},
+        "counters": {k: st[k] for k in ("seed_evals", "ungapped_hits", "prelim_dps", "traceback_dps", "dp_cells",
+                                         "hsps", "ungapped_steps")},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+def cpu_baseline(w, tmp, mpath, n_records):
+    """The CPU oracle (a restatement of the reference's search path) on the
+    first n_records records x all families, threads over subjects per record
+    like rmblastn's default thread mode."""
+    from rmblast_amd import synth
+    exe = os.path.join(ROOT, "oracle", "_build", "rmblastn_oracle")
+    if not os.path.exists(exe):
+        return None
+    cores = min(16, os.cpu_count() or 1)
+    q = os.path.join(tmp, "cpu_q.fa")
+    db = os.path.join(tmp, "cpu_db.fa")
+    synth.to_fasta(w.records[:n_records], q)
+    synth.to_fasta(w.families, db)
+    args = [exe, "--db", db, "--query", q, "--matrix", mpath, "--gapopen", "20", "--gapextend", "5", "--word-size",
+            "7", "--min-raw-gapped-score", "200", "--complexity-adjust", "--dust", "no", "--num-threads", str(cores),
+            "--outfmt", "6 score qseqid qstart qend sstrand sseqid sstart send"]
+    r = subprocess.run(args, capture_output=True, text=True, env=dict(os.environ, RMB_ORACLE_TIME="1"),
+                       timeout=900)
+    if r.returncode != 0:
+        return None
+    secs = float([l for l in r.stderr.splitlines() if l.startswith("oracle_search_seconds")][0].split()[1])
+    mbp = sum(len(x[1]) for x in w.records[:n_records]) / 1e6
+    return {"value": round(mbp * len(w.families) / secs, 2), "unit": "Mbp*families/s", "cores": cores,
+            "kind": "port", "seconds": round(secs, 3),
+            "sample": f"first {n_records} records ({mbp:.2f} Mbp) x {len(w.families)} families, same parameters"}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -90,6 +90,12 @@ typedef uint32_t rmb_op;
 
 typedef struct rmb_ctx rmb_ctx;
 
+/* Parse an RMBlast matrix file (FREQS comment, header row, one row per
+ * symbol) into BLASTNA order with NCBI's defaults for absent entries and the
+ * sentinel row/column, and estimate lambda.  Replaces ScoreMatrix::from_file
+ * (matrix.rs:76-216, lambda :306-354).  err receives a message on failure. */
+int rmb_matrix_load(const char* path, rmb_matrix* out, char* err, size_t err_len);
+
 /* Create a context on HIP device `device`.  Replaces the per-search state the
  * reference builds in rmblastn main (main.rs:516-700): parameters + matrix. */
 int rmb_ctx_create(int device, const rmb_params* params, const rmb_matrix* matrix, rmb_ctx** out);
@@ -119,7 +125,9 @@ int rmb_results_copy(const rmb_ctx* ctx, rmb_hsp* hsps, rmb_op* ops);
  * [4] score-only DP cells, [5] traceback DP cells, [6] final HSPs,
  * [7] ungapped extension steps.  Stage wall times in microseconds:
  * [8] lookup build, [9] seeding + ungapped, [10] preliminary gapped,
- * [11] traceback, [12] total. */
+ * [11] traceback, [12] total; [13] time inside the seeding kernel in
+ * nanoseconds (HIP events on the stream it is launched on), [15] launches of
+ * the seeding kernel. */
 int rmb_stats(const rmb_ctx* ctx, uint64_t* out, int n);
 
 /* ---- parity taps (one stage at a time; same inputs as rmb_search) -------- */

@@ -68,3 +68,88 @@ void orc_ncbi_random(uint32_t seed, const uint8_t* codes, int n, uint8_t* out) {
 }
 
 }
+
+extern "C" {
+
+// engine.rs:796-832 search_with_query_lookup_ungapped for one pair; q, s are raw
+// BLASTNA (no sentinels).  out: 8 u32 per hit {q_seed, s_seed, q_start, q_end,
+// s_start, s_end, score, strand} in the record frame, sorted like NCBI.
+int orc_ungapped_hits(const uint8_t* q, size_t qn, const uint8_t* s, size_t sn, int word_size, int xdrop_ungap,
+                      int min_raw, const int32_t* matrix, uint32_t* out, int cap) {
+  Matrix m = mat_from(matrix);
+  Params p;
+  p.word_size = word_size;
+  p.xdrop_ungap = xdrop_ungap;
+  p.min_raw_gapped_score = min_raw;
+  std::vector<uint8_t> qv(q, q + qn), sv(s, s + sn);
+  qv.insert(qv.begin(), 15); qv.push_back(15);
+  sv.insert(sv.begin(), 15); sv.push_back(15);
+  QueryLookup ql = build_query_lookup(qv, p);
+  PreparedSubject ps = prepare_subject(sv);
+  std::vector<UngappedRecord> ung;
+  Taps t;
+  t.ungapped = &ung;
+  search_one_subject(ql, qv, revcomp(qv), "q", ps, "s", p, m, &t);
+  for (int i = 0; i < (int)ung.size() && i < cap; ++i) {
+    const UngappedRecord& u = ung[i];
+    uint32_t* o = out + 8 * i;
+    o[0] = u.q_seed; o[1] = u.s_seed; o[2] = u.q_start; o[3] = u.q_end; o[4] = u.s_start; o[5] = u.s_end;
+    o[6] = (uint32_t)u.score; o[7] = u.strand;
+  }
+  return (int)ung.size();
+}
+
+// complexity_adjust.rs:57-102 with the matrix file's FREQS and lambda
+int orc_adjusted_score(const char* matrix_path, int32_t raw, const uint8_t* bases, int n, double* out) {
+  Matrix m; std::string err;
+  if (!matrix_from_file(matrix_path, m, err)) return 1;
+  *out = compute_adjusted_score(raw, std::vector<uint8_t>(bases, bases + n), m);
+  return 0;
+}
+
+// engine.rs:863-974 per-subject form; hsps: 6 u32 per HSP {score, q_start, q_end, s_start, s_end, strand}
+int orc_mask_level(const uint32_t* h, int n, uint32_t q_len, int mask_level, uint8_t* keep) {
+  std::vector<AlignResult> r(n);
+  for (int i = 0; i < n; ++i) {
+    const uint32_t* x = h + 6 * i;
+    r[i].hsp = Hsp{(int32_t)x[0], x[1], x[2], q_len, x[3], x[4], 0, x[5] ? MINUS : PLUS, {}, {}, {}};
+    r[i].subject_id = std::to_string(i);
+  }
+  apply_mask_level(r, mask_level, {});
+  for (int i = 0; i < n; ++i) keep[i] = 0;
+  for (auto& x : r) keep[std::stoi(x.subject_id)] = 1;
+  return (int)r.size();
+}
+
+// engine.rs:1969-2083; prelims: 6 u32 {score, q_start, q_end, s_start, s_end, strand}
+int orc_purge_prelims(const uint32_t* h, int n, uint32_t sa_len, uint8_t* keep) {
+  std::vector<PrelimHsp> v(n);
+  for (int i = 0; i < n; ++i) {
+    const uint32_t* x = h + 6 * i;
+    v[i] = PrelimHsp{(uint32_t)i, 0, x[1], x[2], x[3], x[4], (int32_t)x[0], x[5] ? MINUS : PLUS};
+  }
+  purge_prelim_common_endpoints(v, sa_len);
+  for (int i = 0; i < n; ++i) keep[i] = 0;
+  for (auto& p : v) keep[p.q_seed] = 1;
+  return (int)v.size();
+}
+
+// engine.rs:2953-3039 on sentinel-free query/subject; out = {score, q0, q1, s0, s1, ok}
+void orc_gapped_bidir(const uint8_t* q, size_t qn, const uint8_t* s, size_t sn, uint32_t q_seed, uint32_t s_seed,
+                      int go, int ge, int x, const int32_t* matrix, int64_t* out) {
+  Matrix m = mat_from(matrix);
+  std::vector<uint8_t> qv(q, q + qn), sv(s, s + sn);
+  qv.insert(qv.begin(), 15); qv.push_back(15);
+  sv.insert(sv.begin(), 15); sv.push_back(15);
+  int32_t sc = 0; uint32_t q0 = 0, q1 = 0, s0 = 0, s1 = 0; EditOps ops;
+  bool ok = gapped_extend_bidirectional(qv, sv, q_seed, s_seed, go, ge, x, m, sc, q0, q1, s0, s1, ops, nullptr);
+  out[0] = sc; out[1] = q0; out[2] = q1; out[3] = s0; out[4] = s1; out[5] = ok;
+}
+
+}
+
+extern "C" void orc_improve_seed(const uint8_t* q, size_t qn, const uint8_t* s, size_t sn, const uint32_t* in6,
+                                 int minus, uint32_t* out2) {
+  // in6 = {q_seed, s_seed, prelim q_start, q_end, s_start, s_end}
+  improve_seed(q, qn, s, sn, in6[0], in6[1], in6[2], in6[3], in6[4], in6[5], minus != 0, out2[0], out2[1]);
+}

@@ -281,7 +281,7 @@ static std::vector<PrelimHsp> run_phase2a(const std::vector<uint8_t>& query, con
 }
 
 // ------------------------------------------------------------ improve_seed ----
-static void improve_seed(const uint8_t* query, size_t qlen, const uint8_t* subject, size_t slen, uint32_t q_seed32,
+void improve_seed(const uint8_t* query, size_t qlen, const uint8_t* subject, size_t slen, uint32_t q_seed32,
                          uint32_t s_seed32, uint32_t pqs, uint32_t pqe, uint32_t pss, uint32_t pse, bool minus,
                          uint32_t& oq, uint32_t& os) {
   // engine.rs:383-572

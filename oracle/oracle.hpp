@@ -198,6 +198,11 @@ std::vector<AlignResult> search_query(const std::vector<uint8_t>& query, const s
 // output.rs:294-363 outfmt 6
 std::string format_tabular(const AlignResult& r, const std::vector<std::string>& fields);
 
+// engine.rs:383-572 (query/subject without sentinels)
+void improve_seed(const uint8_t* query, size_t qlen, const uint8_t* subject, size_t slen, uint32_t q_seed,
+                  uint32_t s_seed, uint32_t pqs, uint32_t pqe, uint32_t pss, uint32_t pse, bool minus, uint32_t& oq,
+                  uint32_t& os);
+
 // unit-level entry points used by tests
 void purge_prelim_common_endpoints(std::vector<PrelimHsp>& v, uint32_t sa_len);
 int purge_hsps_with_common_endpoints(std::vector<AlignResult*>& arr, bool purge);

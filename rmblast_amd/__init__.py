@@ -1,0 +1,221 @@
+"""rmblast_amd: MI355X (gfx950) search path of rmblastn behind a C ABI.
+
+This package is a thin ctypes binding of librmblast_hip.so
+(include/rmblast_hip.h), mirroring the reference's rmblast-lib search API
+(rmblast-lib/src/search/mod.rs:9-21): a context holds the parameters and the
+matrix, the library (TE families) is loaded once, and `search` runs a batch of
+query records against it on the GPU.  There is no CPU fallback: importing the
+package fails if the HIP library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "librmblast_hip.so")
+CLI_PATH = os.path.join(_HERE, "_build", "rmblastn")
+
+EXPORTED = [
+    "rmb_matrix_load", "rmb_ctx_create", "rmb_ctx_destroy", "rmb_last_error", "rmb_library_load",
+    "rmb_search", "rmb_results_size", "rmb_results_copy", "rmb_stats", "rmb_tap_ungapped",
+    "rmb_tap_align_score_only", "rmb_tap_align_ex",
+]
+
+
+class RmbParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "gap_open", "gap_extend", "word_size", "xdrop_ungap", "xdrop_gap", "xdrop_gap_final",
+        "min_raw_gapped_score", "ungapped_cutoff", "complexity_adjust", "dust")]
+
+
+class RmbMatrix(ctypes.Structure):
+    _fields_ = [("scores", (ctypes.c_int32 * 16) * 16), ("freqs", ctypes.c_double * 16),
+                ("lambda_", ctypes.c_double)]
+
+
+HSP_DTYPE = np.dtype([("query", "<u4"), ("subject", "<u4"), ("score", "<i4"), ("q_start", "<u4"),
+                      ("q_end", "<u4"), ("s_start", "<u4"), ("s_end", "<u4"), ("strand", "<u4"),
+                      ("ops_offset", "<u4"), ("ops_count", "<u4")])
+
+STAT_NAMES = ["seed_evals", "ungapped_hits", "prelim_dps", "traceback_dps", "dp_cells", "tb_cells",
+              "hsps", "ungapped_steps", "t_lookup_us", "t_seed_us", "t_prelim_us", "t_traceback_us",
+              "t_total_us", "t_phase1_kernel_ns", "reserved", "phase1_launches"]
+
+
+class RmbError(RuntimeError):
+    pass
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    if not os.path.exists(path):
+        raise RmbError(f"{path} is missing: build it with `python -m rmblast_amd.build`")
+    lib = ctypes.CDLL(path)
+    P = ctypes.POINTER
+    u8p = P(ctypes.c_uint8)
+    lib.rmb_matrix_load.argtypes = [ctypes.c_char_p, P(RmbMatrix), ctypes.c_char_p, ctypes.c_size_t]
+    lib.rmb_ctx_create.argtypes = [ctypes.c_int, P(RmbParams), P(RmbMatrix), P(ctypes.c_void_p)]
+    lib.rmb_ctx_destroy.argtypes = [ctypes.c_void_p]
+    lib.rmb_last_error.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
+    lib.rmb_library_load.argtypes = [ctypes.c_void_p, ctypes.c_uint32, P(u8p), P(ctypes.c_uint32)]
+    lib.rmb_search.argtypes = [ctypes.c_void_p, ctypes.c_uint32, P(u8p), P(ctypes.c_uint32)]
+    lib.rmb_results_size.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), P(ctypes.c_uint64)]
+    lib.rmb_results_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    lib.rmb_stats.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), ctypes.c_int]
+    lib.rmb_tap_ungapped.argtypes = [ctypes.c_void_p, u8p, ctypes.c_uint32, ctypes.c_uint32,
+                                     P(ctypes.c_uint32), P(ctypes.c_uint64)]
+    lib.rmb_tap_align_score_only.argtypes = [ctypes.c_void_p, u8p, ctypes.c_uint32, u8p, ctypes.c_uint32,
+                                             ctypes.c_int32, ctypes.c_int32, P(ctypes.c_int64)]
+    lib.rmb_tap_align_ex.argtypes = [ctypes.c_void_p, u8p, ctypes.c_uint32, u8p, ctypes.c_uint32,
+                                     ctypes.c_int32, ctypes.c_int32, P(ctypes.c_int64), P(ctypes.c_uint32),
+                                     ctypes.c_uint32]
+    return lib
+
+
+_LIB = None
+
+
+def lib() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is None:
+        _LIB = load_library()
+    return _LIB
+
+
+def load_matrix(path: str) -> RmbMatrix:
+    m = RmbMatrix()
+    err = ctypes.create_string_buffer(256)
+    if lib().rmb_matrix_load(path.encode(), ctypes.byref(m), err, 256):
+        raise RmbError(f"matrix {path}: {err.value.decode()}")
+    return m
+
+
+def _u8ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+
+
+class Searcher:
+    """One GPU context: parameters + matrix + loaded library.
+
+    Mirrors build_query_lookup + search_with_query_lookup (engine.rs:606-729)
+    over every subject, as rmblastn's search_db_parallel drives them
+    (rmblastn/src/main.rs:1133-1234)."""
+
+    def __init__(self, matrix: RmbMatrix, *, gap_open=20, gap_extend=5, word_size=11, xdrop_ungap=20,
+                 xdrop_gap=30, xdrop_gap_final=100, min_raw_gapped_score=0, ungapped_cutoff=-1,
+                 complexity_adjust=False, device=0):
+        self._lib = lib()
+        p = RmbParams(gap_open, gap_extend, word_size, xdrop_ungap, xdrop_gap, xdrop_gap_final,
+                      min_raw_gapped_score, ungapped_cutoff, int(bool(complexity_adjust)), 0)
+        self.params = p
+        self.matrix = matrix
+        ctx = ctypes.c_void_p()
+        rc = self._lib.rmb_ctx_create(device, ctypes.byref(p), ctypes.byref(matrix), ctypes.byref(ctx))
+        self._ctx = ctx
+        self._check(rc, "rmb_ctx_create")
+        self._keep = []
+
+    def _check(self, rc, what):
+        if rc:
+            buf = ctypes.create_string_buffer(512)
+            if self._ctx:
+                self._lib.rmb_last_error(self._ctx, buf, 512)
+            raise RmbError(f"{what} failed ({rc}): {buf.value.decode()}")
+
+    def close(self):
+        if self._ctx:
+            self._lib.rmb_ctx_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load_library(self, seqs):
+        arrs = [np.ascontiguousarray(s, dtype=np.uint8) for s in seqs]
+        ptrs = (ctypes.POINTER(ctypes.c_uint8) * len(arrs))(*[_u8ptr(a) for a in arrs])
+        lens = (ctypes.c_uint32 * len(arrs))(*[len(a) for a in arrs])
+        self._check(self._lib.rmb_library_load(self._ctx, len(arrs), ptrs, lens), "rmb_library_load")
+        self.subjects = arrs
+
+    def search(self, queries):
+        arrs = [np.ascontiguousarray(q, dtype=np.uint8) for q in queries]
+        ptrs = (ctypes.POINTER(ctypes.c_uint8) * len(arrs))(*[_u8ptr(a) for a in arrs])
+        lens = (ctypes.c_uint32 * len(arrs))(*[len(a) for a in arrs])
+        self._check(self._lib.rmb_search(self._ctx, len(arrs), ptrs, lens), "rmb_search")
+        nh, no = ctypes.c_uint64(), ctypes.c_uint64()
+        self._lib.rmb_results_size(self._ctx, ctypes.byref(nh), ctypes.byref(no))
+        hsps = np.zeros(nh.value, dtype=HSP_DTYPE)
+        ops = np.zeros(no.value, dtype=np.uint32)
+        self._lib.rmb_results_copy(self._ctx, hsps.ctypes.data_as(ctypes.c_void_p),
+                                   ops.ctypes.data_as(ctypes.c_void_p))
+        return hsps, ops
+
+    def stats(self):
+        out = (ctypes.c_uint64 * 16)()
+        self._lib.rmb_stats(self._ctx, out, 16)
+        return dict(zip(STAT_NAMES, list(out)))
+
+    def tap_ungapped(self, query, subject: int):
+        q = np.ascontiguousarray(query, dtype=np.uint8)
+        n = ctypes.c_uint64(0)
+        self._check(self._lib.rmb_tap_ungapped(self._ctx, _u8ptr(q), len(q), subject, None, ctypes.byref(n)),
+                    "rmb_tap_ungapped")
+        out = np.zeros((max(n.value, 1), 8), dtype=np.uint32)
+        self._check(self._lib.rmb_tap_ungapped(self._ctx, _u8ptr(q), len(q), subject,
+                                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                               ctypes.byref(n)), "rmb_tap_ungapped")
+        return out[:n.value]
+
+    def tap_align_score_only(self, a, b, xdrop, reverse):
+        a = np.ascontiguousarray(a, dtype=np.uint8)
+        b = np.ascontiguousarray(b, dtype=np.uint8)
+        out = (ctypes.c_int64 * 3)()
+        self._check(self._lib.rmb_tap_align_score_only(self._ctx, _u8ptr(a), len(a), _u8ptr(b), len(b), xdrop,
+                                                       int(reverse), out), "rmb_tap_align_score_only")
+        return tuple(out)
+
+    def tap_align_ex(self, a, b, xdrop, reverse, max_ops=100000):
+        a = np.ascontiguousarray(a, dtype=np.uint8)
+        b = np.ascontiguousarray(b, dtype=np.uint8)
+        out = (ctypes.c_int64 * 4)()
+        ops = np.zeros(max_ops, dtype=np.uint32)
+        self._check(self._lib.rmb_tap_align_ex(self._ctx, _u8ptr(a), len(a), _u8ptr(b), len(b), xdrop,
+                                               int(reverse), out,
+                                               ops.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), max_ops),
+                    "rmb_tap_align_ex")
+        score, alen, blen, nops = tuple(out)
+        return score, alen, blen, [(int(o & 3), int(o >> 2)) for o in ops[:nops]]
+
+
+_CODES = np.full(256, 14, dtype=np.uint8)
+for _i, _c in enumerate("ACGTRYMKWSBDHVN"):
+    _CODES[ord(_c)] = _i
+    _CODES[ord(_c.lower())] = _i
+_CODES[ord("U")] = _CODES[ord("u")] = 3
+
+
+def encode(text: str) -> np.ndarray:
+    """IUPAC text -> BLASTNA codes (unknown letters, '-' and 'X' read as N)."""
+    return _CODES[np.frombuffer(text.encode(), dtype=np.uint8)]
+
+
+def read_fasta(path: str):
+    recs, name, buf = [], None, []
+    with open(path) as f:
+        for line in f:
+            line = line.rstrip("\r\n")
+            if line.startswith(">"):
+                if name is not None:
+                    recs.append((name, encode("".join(buf))))
+                name = line[1:].split()[0]
+                buf = []
+            elif line.strip():
+                buf.append(line.strip())
+    if name is not None:
+        recs.append((name, encode("".join(buf))))
+    return recs

@@ -12,6 +12,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
+#include <fstream>
+#include <sstream>
 #include <atomic>
 #include <chrono>
 #include <cstdarg>
@@ -180,6 +183,7 @@ struct rmb_ctx {
   std::vector<rmb_hsp> hsps;
   std::vector<rmb_op> ops;
   uint64_t stats[16] = {0};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // scratch
   DevBuf b_qfwd, b_qrc, b_recs, b_elem, b_keys, b_keys2, b_vals, b_vals2, b_counts, b_cells, b_sort_tmp;
   DevBuf b_hits, b_hitcount, b_spill, b_queue, b_counters, b_err;
@@ -404,15 +408,23 @@ static void phase1_group(rmb_ctx* c, const std::vector<uint32_t>& recs_idx, cons
     hip_check(hipMemsetAsync(queue, 0, sizeof(unsigned long long), c->st), "memset queue");
     hip_check(hipMemsetAsync(hitcount, 0, sizeof(unsigned long long), c->st), "memset hitcount");
     hip_check(hipMemsetAsync(err, 0, sizeof(uint32_t), c->st), "memset err");
+    hip_check(hipEventRecord(c->ev0, c->st), "event");
     hip_check(launch_phase1(blocks, P, drecs, (const SubjDesc*)c->d_sdesc.p, d_qfwd, d_qrc, d_qfwd,
                             (const uint8_t*)c->d_splus.p, (const uint8_t*)c->d_src.p, cstart, vals2, c->d_mat, dh,
                             hitcount, spill, queue, ctr, err, c->st),
               "k_phase1");
+    hip_check(hipEventRecord(c->ev1, c->st), "event");
     unsigned long long nh = 0;
     uint32_t e = 0;
     hip_check(hipMemcpyAsync(&nh, hitcount, sizeof nh, hipMemcpyDeviceToHost, c->st), "copy hitcount");
     hip_check(hipMemcpyAsync(&e, err, sizeof e, hipMemcpyDeviceToHost, c->st), "copy err");
     hip_check(hipStreamSynchronize(c->st), "phase1 sync");
+    {
+      float ms = 0;
+      hip_check(hipEventElapsedTime(&ms, c->ev0, c->ev1), "event time");
+      c->stats[13] += (uint64_t)(ms * 1e6);
+      c->stats[15] += 1;
+    }
     if (e & E_HASH_FULL) throw Failure(RMB_ERR_CAPACITY, "diagonal hash bucket exceeded 256 cells");
     if (nh > cap) {  // output larger than the buffer: grow and rerun the group
       cap = nh + nh / 4;
@@ -759,6 +771,99 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
 
 }  // namespace rmb
 
+namespace rmb {
+// RMBlast matrix file (matrix.rs:76-213): FREQS comment, header row, rows.
+static bool parse_matrix(const std::string& path, rmb_matrix& m, std::string& err) {
+  std::ifstream f(path);
+  if (!f) { err = "cannot open matrix " + path; return false; }
+  auto slot = [](char c) -> int {
+    const char* s = "ACGTRYMKWSBDHVN";
+    const char* p = std::strchr(s, c);
+    return (p && c) ? (int)(p - s) : 15;
+  };
+  for (auto& r : m.scores)
+    for (auto& v : r) v = -32768;
+  std::memset(m.freqs, 0, sizeof m.freqs);
+  std::vector<char> cols;
+  int row = 0;
+  bool freqs = false;
+  std::string line;
+  while (std::getline(f, line)) {
+    std::istringstream is(line);
+    std::string tok;
+    if (!(is >> tok)) continue;
+    if (tok[0] == '#') {
+      size_t at = line.find("FREQS");
+      if (at == std::string::npos) continue;
+      std::istringstream fs(line.substr(at + 5));
+      std::string b;
+      double v;
+      int got = 0;
+      while (fs >> b >> v) {
+        const int k = slot((char)std::toupper((unsigned char)b[0]));
+        if (k > 3 || b.size() != 1) { err = "bad FREQS"; return false; }
+        m.freqs[k] = v;
+        ++got;
+      }
+      if (got != 4) { err = "FREQS needs A, C, G and T"; return false; }
+      freqs = true;
+      continue;
+    }
+    if (cols.empty()) {
+      do cols.push_back((char)std::toupper((unsigned char)tok[0])); while (is >> tok);
+      continue;
+    }
+    if (row >= (int)cols.size()) { err = "more rows than columns"; return false; }
+    std::vector<int> vals;
+    bool first = true;
+    do {
+      char* e = nullptr;
+      long v = std::strtol(tok.c_str(), &e, 10);
+      if (*e) {
+        if (first && tok.size() == 1) { first = false; continue; }  // row label
+        err = "score '" + tok + "' is not an integer";
+        return false;
+      }
+      first = false;
+      vals.push_back((int)v);
+    } while (is >> tok);
+    if (vals.size() != cols.size()) { err = "row length differs from the header"; return false; }
+    for (size_t c = 0; c < cols.size(); ++c) m.scores[slot(cols[row])][slot(cols[c])] = vals[c];
+    ++row;
+  }
+  if (!freqs) { err = "no FREQS line"; return false; }
+  if (row != (int)cols.size()) { err = "matrix must be square"; return false; }
+  for (int i = 0; i < 16; ++i) { m.scores[15][i] = INT32_MIN / 2; m.scores[i][15] = INT32_MIN / 2; }
+  double expected = 0;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) expected += m.freqs[i] * m.freqs[j] * m.scores[i][j];
+  if (expected >= 0) { err = "expected pair score must be negative"; return false; }
+  // lambda: sum_ij f_i f_j exp(lambda s_ij) = 1, doubling then bisection to 1e-5 (matrix.rs:306-354)
+  auto g = [&](double lam) {
+    double acc = 0;
+    for (int i = 0; i < 16; ++i)
+      for (int j = 0; j < 16; ++j)
+        if (m.freqs[i] > 0.0 && m.freqs[j] > 0.0) acc += m.freqs[i] * m.freqs[j] * std::exp(lam * (double)m.scores[i][j]);
+    return acc;
+  };
+  double lo = 0, x = 0.5;
+  m.lambda = 0;
+  while (g(x) < 1.0) {
+    lo = x;
+    x *= 2;
+    if (x > 1000.0) return true;
+  }
+  double hi = x;
+  while (hi - lo > 0.00001) {
+    x = (lo + hi) / 2;
+    if (g(x) >= 1.0) hi = x; else lo = x;
+  }
+  m.lambda = x;
+  return true;
+}
+
+}  // namespace rmb
+
 // ============================================================== C ABI =====
 template <class F>
 static int guarded(rmb_ctx* c, F&& fn) {
@@ -779,6 +884,16 @@ static int guarded(rmb_ctx* c, F&& fn) {
 
 extern "C" {
 
+int rmb_matrix_load(const char* path, rmb_matrix* out, char* err, size_t err_len) {
+  if (!path || !out) return RMB_ERR_ARG;
+  std::string e;
+  if (!rmb::parse_matrix(path, *out, e)) {
+    if (err && err_len) { std::strncpy(err, e.c_str(), err_len - 1); err[err_len - 1] = 0; }
+    return RMB_ERR_ARG;
+  }
+  return RMB_OK;
+}
+
 int rmb_ctx_create(int device, const rmb_params* params, const rmb_matrix* matrix, rmb_ctx** out) {
   if (!params || !matrix || !out) return RMB_ERR_ARG;
   *out = nullptr;
@@ -796,6 +911,8 @@ int rmb_ctx_create(int device, const rmb_params* params, const rmb_matrix* matri
     hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
     c->cu_count = prop.multiProcessorCount;
     hip_check(hipStreamCreate(&c->st), "hipStreamCreate");
+    hip_check(hipEventCreate(&c->ev0), "hipEventCreate");
+    hip_check(hipEventCreate(&c->ev1), "hipEventCreate");
     c->P = *params;
     std::memcpy(c->M.s, matrix->scores, sizeof c->M.s);
     std::memcpy(c->M.freqs, matrix->freqs, sizeof c->M.freqs);
@@ -822,6 +939,8 @@ int rmb_ctx_destroy(rmb_ctx* c) {
                     &c->b_ops})
     b->release();
   if (c->d_mat) (void)hipFree(c->d_mat);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
   return RMB_OK;

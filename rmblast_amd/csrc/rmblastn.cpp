@@ -61,96 +61,6 @@ bool load_fasta(const std::string& path, std::vector<Record>& out, std::string& 
   return true;
 }
 
-// RMBlast matrix file (matrix.rs:76-213): FREQS comment, header row, rows.
-bool load_matrix(const std::string& path, rmb_matrix& m, std::string& err) {
-  std::ifstream f(path);
-  if (!f) { err = "cannot open matrix " + path; return false; }
-  auto slot = [](char c) -> int {
-    const char* s = "ACGTRYMKWSBDHVN";
-    const char* p = std::strchr(s, c);
-    return (p && c) ? (int)(p - s) : 15;
-  };
-  for (auto& r : m.scores)
-    for (auto& v : r) v = -32768;
-  std::memset(m.freqs, 0, sizeof m.freqs);
-  std::vector<char> cols;
-  int row = 0;
-  bool freqs = false;
-  std::string line;
-  while (std::getline(f, line)) {
-    std::istringstream is(line);
-    std::string tok;
-    if (!(is >> tok)) continue;
-    if (tok[0] == '#') {
-      size_t at = line.find("FREQS");
-      if (at == std::string::npos) continue;
-      std::istringstream fs(line.substr(at + 5));
-      std::string b;
-      double v;
-      int got = 0;
-      while (fs >> b >> v) {
-        const int k = slot((char)std::toupper((unsigned char)b[0]));
-        if (k > 3 || b.size() != 1) { err = "bad FREQS"; return false; }
-        m.freqs[k] = v;
-        ++got;
-      }
-      if (got != 4) { err = "FREQS needs A, C, G and T"; return false; }
-      freqs = true;
-      continue;
-    }
-    if (cols.empty()) {
-      do cols.push_back((char)std::toupper((unsigned char)tok[0])); while (is >> tok);
-      continue;
-    }
-    if (row >= (int)cols.size()) { err = "more rows than columns"; return false; }
-    std::vector<int> vals;
-    bool first = true;
-    do {
-      char* e = nullptr;
-      long v = std::strtol(tok.c_str(), &e, 10);
-      if (*e) {
-        if (first && tok.size() == 1) { first = false; continue; }  // row label
-        err = "score '" + tok + "' is not an integer";
-        return false;
-      }
-      first = false;
-      vals.push_back((int)v);
-    } while (is >> tok);
-    if (vals.size() != cols.size()) { err = "row length differs from the header"; return false; }
-    for (size_t c = 0; c < cols.size(); ++c) m.scores[slot(cols[row])][slot(cols[c])] = vals[c];
-    ++row;
-  }
-  if (!freqs) { err = "no FREQS line"; return false; }
-  if (row != (int)cols.size()) { err = "matrix must be square"; return false; }
-  for (int i = 0; i < 16; ++i) { m.scores[15][i] = INT32_MIN / 2; m.scores[i][15] = INT32_MIN / 2; }
-  double expected = 0;
-  for (int i = 0; i < 4; ++i)
-    for (int j = 0; j < 4; ++j) expected += m.freqs[i] * m.freqs[j] * m.scores[i][j];
-  if (expected >= 0) { err = "expected pair score must be negative"; return false; }
-  // lambda: sum_ij f_i f_j exp(lambda s_ij) = 1, doubling then bisection to 1e-5 (matrix.rs:306-354)
-  auto g = [&](double lam) {
-    double acc = 0;
-    for (int i = 0; i < 16; ++i)
-      for (int j = 0; j < 16; ++j)
-        if (m.freqs[i] > 0.0 && m.freqs[j] > 0.0) acc += m.freqs[i] * m.freqs[j] * std::exp(lam * (double)m.scores[i][j]);
-    return acc;
-  };
-  double lo = 0, x = 0.5;
-  m.lambda = 0;
-  while (g(x) < 1.0) {
-    lo = x;
-    x *= 2;
-    if (x > 1000.0) return true;
-  }
-  double hi = x;
-  while (hi - lo > 0.00001) {
-    x = (lo + hi) / 2;
-    if (g(x) >= 1.0) hi = x; else lo = x;
-  }
-  m.lambda = x;
-  return true;
-}
-
 struct Stats { uint32_t mism = 0, qgap = 0, sgap = 0, ti = 0, tv = 0, cpg = 0; double kdiv = 0, cpgk = 0; float ps = 0, pq = 0, pd = 0; };
 
 // stats.rs:52-243 with the subject as the ancestral sequence.
@@ -268,7 +178,10 @@ int main(int argc, char** argv) {
   }
   std::string err;
   rmb_matrix M;
-  if (!load_matrix(opt("--matrix", ""), M, err)) { fprintf(stderr, "error: %s\n", err.c_str()); return 1; }
+  {
+    char mbuf[256] = {0};
+    if (rmb_matrix_load(opt("--matrix", "").c_str(), &M, mbuf, sizeof mbuf)) { fprintf(stderr, "error: matrix: %s\n", mbuf); return 1; }
+  }
   std::vector<Record> db, qs;
   if (!load_fasta(kv["--db"], db, err) || !load_fasta(kv["--query"], qs, err)) {
     fprintf(stderr, "error: %s\n", err.c_str());

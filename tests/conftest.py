@@ -1,0 +1,93 @@
+import ctypes
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+ORACLE_BIN = os.path.join(ROOT, "oracle", "_build", "rmblastn_oracle")
+ORACLE_LIB = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+CLI = os.path.join(ROOT, "rmblast_amd", "_build", "rmblastn")
+REFERENCE = "/root/reference"
+
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+RM_FIELDS = ("6 score perc_sub perc_query_gap perc_db_gap qseqid qstart qend qlen sstrand sseqid sstart send "
+             "slen kdiv cpg_kdiv transi transv cpg_sites qseq sseq")
+# The parameters every reference fixture run shares (rmblastn/tests/cli.rs:107-117).
+RM_SEARCH = ["--gapopen", "20", "--gapextend", "5", "--xdrop-ungap", "400", "--xdrop-gap-final", "200",
+             "--xdrop-gap", "100", "--min-raw-gapped-score", "200"]
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP path)")
+
+
+def golden(name):
+    return os.path.join(GOLDEN, name)
+
+
+def run(cmd, timeout=600, env=None):
+    e = dict(os.environ)
+    if env:
+        e.update(env)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, env=e)
+    return r
+
+
+@pytest.fixture(scope="session")
+def oracle_lib():
+    if not os.path.exists(ORACLE_LIB):
+        pytest.fail("oracle not built: run `python -m rmblast_amd.build` (or make -C oracle)")
+    L = ctypes.CDLL(ORACLE_LIB)
+    P = ctypes.POINTER
+    L.orc_extend_ungapped.argtypes = [P(ctypes.c_uint8), ctypes.c_size_t, P(ctypes.c_uint8), ctypes.c_size_t,
+                                      ctypes.c_uint32, ctypes.c_uint32, P(ctypes.c_int32), ctypes.c_int,
+                                      P(ctypes.c_int32)]
+    L.orc_align_score_only.argtypes = [P(ctypes.c_uint8), P(ctypes.c_uint8), ctypes.c_size_t, ctypes.c_size_t,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_int32), ctypes.c_int,
+                                       P(ctypes.c_int64)]
+    L.orc_align_ex.argtypes = [P(ctypes.c_uint8), P(ctypes.c_uint8), ctypes.c_size_t, ctypes.c_size_t,
+                               ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_int32), ctypes.c_int,
+                               P(ctypes.c_int64), P(ctypes.c_uint32), ctypes.c_int]
+    L.orc_diag_hash_run.argtypes = [P(ctypes.c_int32), ctypes.c_int, ctypes.c_int, P(ctypes.c_uint32)]
+    L.orc_ncbi_random.argtypes = [ctypes.c_uint32, P(ctypes.c_uint8), ctypes.c_int, P(ctypes.c_uint8)]
+    L.orc_ungapped_hits.argtypes = [P(ctypes.c_uint8), ctypes.c_size_t, P(ctypes.c_uint8), ctypes.c_size_t,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_int32),
+                                    P(ctypes.c_uint32), ctypes.c_int]
+    L.orc_adjusted_score.argtypes = [ctypes.c_char_p, ctypes.c_int32, P(ctypes.c_uint8), ctypes.c_int,
+                                     P(ctypes.c_double)]
+    L.orc_mask_level.argtypes = [P(ctypes.c_uint32), ctypes.c_int, ctypes.c_uint32, ctypes.c_int,
+                                 P(ctypes.c_uint8)]
+    L.orc_purge_prelims.argtypes = [P(ctypes.c_uint32), ctypes.c_int, ctypes.c_uint32, P(ctypes.c_uint8)]
+    L.orc_gapped_bidir.argtypes = [P(ctypes.c_uint8), ctypes.c_size_t, P(ctypes.c_uint8), ctypes.c_size_t,
+                                   ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   P(ctypes.c_int32), P(ctypes.c_int64)]
+    L.orc_improve_seed.argtypes = [P(ctypes.c_uint8), ctypes.c_size_t, P(ctypes.c_uint8), ctypes.c_size_t,
+                                   P(ctypes.c_uint32), ctypes.c_int, P(ctypes.c_uint32)]
+    return L
+
+
+def ptr(a, ct=ctypes.c_uint8):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def simple_matrix(match, mismatch, other=0, sentinel=None):
+    """16x16 BLASTNA matrix: match/mismatch on ACGT, `other` elsewhere."""
+    m = np.full((16, 16), other, dtype=np.int32)
+    for i in range(4):
+        for j in range(4):
+            m[i, j] = match if i == j else mismatch
+    if sentinel is not None:
+        m[15, :] = sentinel
+        m[:, 15] = sentinel
+    return np.ascontiguousarray(m)
+
+
+def seq_from_fasta(path):
+    with open(path) as f:
+        return "".join(l.strip() for l in f if not l.startswith(">"))

@@ -1,0 +1,62 @@
+"""The C-ABI library loads and exports every entry point include/rmblast_hip.h
+declares; host-side argument errors come back as error codes (no GPU call)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import CLI, ROOT
+
+HEADER = os.path.join(ROOT, "include", "rmblast_hip.h")
+
+
+def declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^int (rmb_\w+)\(", text, re.M)))
+
+
+def test_header_declares_the_boundary():
+    names = declared()
+    for n in ("rmb_ctx_create", "rmb_ctx_destroy", "rmb_library_load", "rmb_search", "rmb_results_size",
+              "rmb_results_copy", "rmb_last_error", "rmb_matrix_load", "rmb_tap_ungapped",
+              "rmb_tap_align_score_only", "rmb_tap_align_ex", "rmb_stats"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    import rmblast_amd
+    lib = rmblast_amd.load_library()
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing
+    assert set(rmblast_amd.EXPORTED) <= set(declared())
+
+
+def test_cli_was_built():
+    assert os.access(CLI, os.X_OK)
+
+
+def test_matrix_load_parses_and_rejects(tmp_path):
+    import rmblast_amd
+    m = rmblast_amd.load_matrix(os.path.join(ROOT, "tests", "golden", "comparison.matrix"))
+    assert m.scores[0][0] == 9 and m.scores[15][0] == -(2 ** 30)
+    assert abs(m.freqs[0] - 0.265) < 1e-12 and 0.1 < m.lambda_ < 0.2
+    bad = tmp_path / "bad.matrix"
+    bad.write_text("   A C\nA 1 -1\nC -1 1\n")       # no FREQS line: NCBI refuses it
+    with pytest.raises(rmblast_amd.RmbError):
+        rmblast_amd.load_matrix(str(bad))
+
+
+def test_null_arguments_are_errors_not_crashes():
+    import rmblast_amd
+    lib = rmblast_amd.load_library()
+    assert lib.rmb_ctx_create(0, None, None, None) != 0
+    assert lib.rmb_search(None, 0, None, None) != 0
+    assert lib.rmb_results_size(None, None, None) != 0
+    assert lib.rmb_ctx_destroy(None) == 0
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    import rmblast_amd
+    with pytest.raises(rmblast_amd.RmbError):
+        rmblast_amd.load_library(str(tmp_path / "nope.so"))

@@ -1,0 +1,198 @@
+"""HIP path parity: the product (librmblast_hip.so through its C ABI, and the
+rmblastn front end on top of it) against the NCBI golden and the CPU oracle.
+Integer work throughout, so every comparison is byte/bit exact."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import rmblast_amd
+from rmblast_amd import synth
+from conftest import CLI, ORACLE_BIN, RM_FIELDS, RM_SEARCH, golden, ptr, run, seq_from_fasta, simple_matrix
+
+pytestmark = pytest.mark.gpu
+
+
+def both(args, tmo=300):
+    g = run([CLI] + args, timeout=tmo)
+    assert g.returncode == 0, g.stderr
+    o = run([ORACLE_BIN] + args, timeout=tmo)
+    assert o.returncode == 0, o.stderr
+    return g.stdout, o.stdout
+
+
+def test_nomask_nodust_golden_on_gpu():
+    g = run([CLI, "--db", golden("db.fa"), "--query", golden("query.fa"), "--matrix", golden("comparison.matrix"),
+             "--word-size", "14", *RM_SEARCH, "--outfmt", RM_FIELDS, "--mask-level", "101", "--dust", "no"])
+    assert g.returncode == 0, g.stderr
+    assert g.stdout == open(golden("expected_nomask_nodust.ncbi-2.17.1.tsv")).read()
+
+
+@pytest.mark.parametrize("ws", [7, 8, 11, 14, 19])
+@pytest.mark.parametrize("extra", [["--mask-level", "80", "--complexity-adjust"], ["--mask-level", "101"]])
+def test_fixture_runs_match_oracle(ws, extra):
+    g, o = both(["--db", golden("db.fa"), "--query", golden("query.fa"), "--matrix", golden("comparison.matrix"),
+                 "--word-size", str(ws), *RM_SEARCH, "--outfmt", RM_FIELDS, "--dust", "no", *extra])
+    assert g == o and g.count("\n") > 20
+
+
+@pytest.mark.parametrize("db,query,matrix", [
+    ("db_dash.fa", "query.fa", "comparison.matrix"),      # subject ambiguity: NcbiRandom + n_mask restore
+    ("db.fa", "query_dash.fa", "comparison.matrix"),      # query N runs
+    ("db.fa", "query.fa", "asym_freqs.matrix"),           # minus-strand complexity adjust
+    ("db.fa", "query.fa", "identity.matrix"),
+])
+def test_ambiguity_and_matrices_match_oracle(db, query, matrix):
+    g, o = both(["--db", golden(db), "--query", golden(query), "--matrix", golden(matrix), "--word-size", "11",
+                 *RM_SEARCH, "--outfmt", RM_FIELDS, "--dust", "no", "--mask-level", "80", "--complexity-adjust"])
+    assert g == o
+
+
+@pytest.mark.parametrize("ws", [7, 14])
+def test_chr22_excerpt_matches_oracle(tmp_path, ws):
+    db = tmp_path / "db.fa"
+    db.write_text(open(golden("alu.fa")).read() + open(golden("db.fa")).read())
+    g, o = both(["--db", str(db), "--query", golden("chr22_30000000_30200000.fa"), "--matrix",
+                 golden("comparison.matrix"), "--word-size", str(ws), "--gapopen", "20", "--gapextend", "5",
+                 "--min-raw-gapped-score", "200", "--mask-level", "80", "--dust", "no",
+                 "--outfmt", RM_FIELDS])
+    assert g == o and g.count("\n") > 10
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_synthetic_parity_config_matches_oracle(tmp_path, seed):
+    w = synth.parity_config(seed=seed, genome_len=300_000, n_families=20, n_copies=600)
+    synth.to_fasta(w.records, tmp_path / "q.fa")
+    synth.to_fasta(w.families, tmp_path / "db.fa")
+    (tmp_path / "m.matrix").write_text(synth.MATRIX_TEXT)
+    g, o = both(["--db", str(tmp_path / "db.fa"), "--query", str(tmp_path / "q.fa"), "--matrix",
+                 str(tmp_path / "m.matrix"), "--word-size", "7", "--gapopen", "20", "--gapextend", "5",
+                 "--min-raw-gapped-score", "200", "--mask-level", "80", "--complexity-adjust", "--dust", "no",
+                 "--outfmt", RM_FIELDS])
+    assert g == o and g.count("\n") > 100
+
+
+def test_gappy_divergent_and_microsatellites_match_oracle(tmp_path):
+    rng = np.random.default_rng(5)
+    fams = synth.make_families(rng, 6, [5000, 8000, 6000, 7000, 9000, 5500])
+    g0, _ = synth.insert_copies(rng, 200_000, fams, 60, (500, 4000), (0.25, 0.40), (0.05, 0.08))
+    g0 = g0.copy()
+    for k in range(40):                              # 5% microsatellite tracts, period 2-6
+        p = int(rng.integers(2, 7))
+        unit = synth.random_bases(rng, p)
+        L = 250
+        at = int(rng.integers(0, len(g0) - L))
+        g0[at:at + L] = np.tile(unit, L // p + 1)[:L]
+    fams.append(("sat", np.tile(np.array([1, 0], dtype=np.uint8), 400)))
+    synth.to_fasta([("q", g0)], tmp_path / "q.fa")
+    synth.to_fasta(fams, tmp_path / "db.fa")
+    (tmp_path / "m.matrix").write_text(synth.MATRIX_TEXT)
+    g, o = both(["--db", str(tmp_path / "db.fa"), "--query", str(tmp_path / "q.fa"), "--matrix",
+                 str(tmp_path / "m.matrix"), "--word-size", "7", "--gapopen", "20", "--gapextend", "5",
+                 "--min-raw-gapped-score", "200", "--xdrop-gap", "120", "--xdrop-gap-final", "400",
+                 "--complexity-adjust", "--dust", "no", "--outfmt", RM_FIELDS])
+    assert g == o and g.count("\n") > 20
+
+
+def test_edge_cases_short_and_empty_records(tmp_path):
+    q = tmp_path / "q.fa"
+    q.write_text(">empty\n\n>short\nACGTA\n>nrun\n" + "N" * 500 + "\n>ok\n" +
+                 seq_from_fasta(golden("query.fa"))[:3000] + "\n")
+    db = tmp_path / "db.fa"
+    db.write_text(">tiny\nACG\n" + open(golden("db.fa")).read())
+    g, o = both(["--db", str(db), "--query", str(q), "--matrix", golden("comparison.matrix"), "--word-size", "7",
+                 *RM_SEARCH, "--outfmt", RM_FIELDS, "--dust", "no", "--mask-level", "80"])
+    assert g == o
+
+
+def test_repeat_runs_are_deterministic(tmp_path):
+    args = ["--db", golden("db.fa"), "--query", golden("query.fa"), "--matrix", golden("comparison.matrix"),
+            "--word-size", "7", *RM_SEARCH, "--outfmt", RM_FIELDS, "--dust", "no"]
+    a = run([CLI] + args)
+    b = run([CLI] + args)
+    assert a.returncode == 0 and a.stdout == b.stdout
+
+
+# ---- stage taps through the C ABI ---------------------------------------------------
+def searcher_with(mat16, **kw):
+    m = rmblast_amd.RmbMatrix()
+    for i in range(16):
+        for j in range(16):
+            m.scores[i][j] = int(mat16[i, j])
+    return rmblast_amd.Searcher(m, **kw)
+
+
+def test_tap_word_finder_known_answer():
+    # engine.rs:3431-3490 (NCBI testNuclWordFinder) on the GPU seeding kernel
+    mat = simple_matrix(1, -3, other=0)
+    s = searcher_with(mat, word_size=19, xdrop_ungap=11, min_raw_gapped_score=28)
+    subj = rmblast_amd.encode(seq_from_fasta(golden("gi33383640_xylanase_subject.fa")))
+    s.load_library([subj])
+    q = rmblast_amd.encode(seq_from_fasta(golden("gi3090_xylanase_query.fa")))
+    hits = s.tap_ungapped(q, 0)
+    plus = sorted((int(h[2]), int(h[4]), int(h[3] - h[2]), int(h[6])) for h in hits if h[7] == 0)
+    assert plus == sorted([(233, 0, 31, 31), (1037, 66, 945, 853), (263, 27, 685, 541), (1911, 811, 101, 73),
+                           (1782, 940, 63, 51)])
+
+
+@pytest.mark.parametrize("ws", [7, 11, 14])
+def test_tap_ungapped_matches_oracle(oracle_lib, ws):
+    w = synth.parity_config(seed=21, genome_len=120_000, n_families=4, n_copies=120)
+    m = rmblast_amd.load_matrix(golden("comparison.matrix"))
+    mat = np.ascontiguousarray(np.array([[m.scores[i][j] for j in range(16)] for i in range(16)], dtype=np.int32))
+    s = rmblast_amd.Searcher(m, word_size=ws, xdrop_ungap=20, min_raw_gapped_score=60)
+    s.load_library([f[1] for f in w.families])
+    q = w.records[0][1]
+    for f in range(len(w.families)):
+        got = s.tap_ungapped(q, f)
+        subj = w.families[f][1]
+        out = np.zeros((200000, 8), dtype=np.uint32)
+        n = oracle_lib.orc_ungapped_hits(ptr(q), len(q), ptr(subj), len(subj), ws, 20, 60,
+                                         ptr(mat, ctypes.c_int32), ptr(out, ctypes.c_uint32), 200000)
+        assert n > 0 and np.array_equal(got, out[:n]), (ws, f, len(got), n)
+
+
+def random_pair(rng):
+    kind = rng.integers(0, 4)
+    la = int(rng.integers(1, 3000))
+    a = rng.integers(0, 4, la).astype(np.uint8)
+    if kind == 0:        # mutated copy with indels
+        b = synth.mutate(rng, a, float(rng.uniform(0.0, 0.3)), float(rng.uniform(0, 0.08)))
+    elif kind == 1:      # unrelated
+        b = rng.integers(0, 4, int(rng.integers(1, 3000))).astype(np.uint8)
+    elif kind == 2:      # tandem repeat against tandem repeat: wide bands
+        unit = rng.integers(0, 4, int(rng.integers(1, 7))).astype(np.uint8)
+        a = np.tile(unit, la // len(unit) + 1)[:la]
+        b = synth.mutate(rng, np.tile(unit, 400)[:int(rng.integers(50, 2500))], 0.05, 0.02)
+    else:                # ambiguity codes sprinkled in
+        b = synth.mutate(rng, a, 0.1, 0.02)
+        b[rng.random(len(b)) < 0.02] = 14
+    if len(b) == 0:
+        b = np.zeros(1, dtype=np.uint8)
+    return a, b
+
+
+def test_tap_dp_kernels_match_oracle(oracle_lib):
+    m = rmblast_amd.load_matrix(golden("comparison.matrix"))
+    mat = np.ascontiguousarray(np.array([[m.scores[i][j] for j in range(16)] for i in range(16)], dtype=np.int32))
+    rng = np.random.default_rng(99)
+    gaps = [(20, 5), (4, 4), (8, 2)]
+    ctxs = {g: rmblast_amd.Searcher(m, gap_open=g[0], gap_extend=g[1]) for g in gaps}
+    for trial in range(120):
+        a, b = random_pair(rng)
+        go, ge = gaps[trial % 3]
+        s = ctxs[(go, ge)]
+        x = [30, 100, 400, 10 ** 6][trial % 4]
+        for rev in (0, 1):
+            want = (ctypes.c_int64 * 3)()
+            oracle_lib.orc_align_score_only(ptr(a), ptr(b), len(a), len(b), go, ge, x, ptr(mat, ctypes.c_int32), rev,
+                                            want)
+            assert s.tap_align_score_only(a, b, x, rev) == tuple(want), (trial, rev)
+            o = (ctypes.c_int64 * 4)()
+            ops = np.zeros(2 * 20000, dtype=np.uint32)
+            n = oracle_lib.orc_align_ex(ptr(a), ptr(b), len(a), len(b), go, ge, x, ptr(mat, ctypes.c_int32), rev, o,
+                                        ptr(ops, ctypes.c_uint32), 20000)
+            sc, al, bl, gops = s.tap_align_ex(a, b, x, rev)
+            assert (sc, al, bl) == tuple(o[:3]), (trial, rev)
+            assert gops == [(int(ops[2 * k]), int(ops[2 * k + 1])) for k in range(n)], (trial, rev)
