@@ -126,8 +126,14 @@ int rmb_results_copy(const rmb_ctx* ctx, rmb_hsp* hsps, rmb_op* ops);
  * [7] ungapped extension steps.  Stage wall times in microseconds:
  * [8] lookup build, [9] seeding + ungapped, [10] preliminary gapped,
  * [11] traceback, [12] total; [13] time inside the seeding kernel in
- * nanoseconds (HIP events on the stream it is launched on), [15] launches of
- * the seeding kernel. */
+ * nanoseconds (HIP events on the stream it is launched on), [14] time inside
+ * the gapped kernels (preliminary, score and traceback DP) in nanoseconds,
+ * [15] launches of the seeding kernel, [16] seeds staged by the seeding
+ * kernel, [17..22] seeding-kernel stage time (scan, seed fill, extension,
+ * bucket sort, hash replay, emit) in 100 MHz ticks summed over workgroups,
+ * [23] reserved.  Entries past `n` are not written; RMB_NSTATS is the full
+ * length. */
+#define RMB_NSTATS 24
 int rmb_stats(const rmb_ctx* ctx, uint64_t* out, int n);
 
 /* ---- parity taps (one stage at a time; same inputs as rmb_search) -------- */

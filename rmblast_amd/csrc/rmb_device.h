@@ -8,10 +8,11 @@ namespace rmb {
 // at a time; SEEDCAP seeds are staged in LDS per pass; the 512 diagonal-hash
 // buckets are owned two per thread.
 constexpr int kP1Threads = 256;
-constexpr int kSeedCap = 1024;
-constexpr int kHashLds = 6;     // cells per bucket held in LDS
-constexpr int kHashGlobal = 250; // further cells per bucket in the workgroup's global spill area
+constexpr int kSeedCap = 768;
+constexpr int kHashLds = 2;      // cells per bucket held in LDS
+constexpr int kHashGlobal = 254; // further cells per bucket in the workgroup's global spill area
 constexpr int kBuckets = 512;
+constexpr int kXcds = 8;         // MI355X: 8 XCDs, workgroups dispatched round-robin over them
 
 struct RecDesc {        // one query record (a chunk of the genome)
   uint64_t q_base;      // offset of its sentinel-wrapped arrays in the query pools
@@ -42,7 +43,25 @@ struct UngHit {         // an ungapped hit at or above the cutoff, record frame
 };
 
 // Counters accumulated by the kernels (u64 atomics).
-enum Counter { C_SEED_EVALS = 0, C_UNG_HITS = 1, C_UNG_STEPS = 2, C_DP_CELLS = 3, C_NCOUNTERS = 8 };
+enum Counter {
+  C_SEED_EVALS = 0, C_UNG_HITS = 1, C_UNG_STEPS = 2, C_DP_CELLS = 3, C_PRELIM_CELLS = 4, C_PRELIM_DPS = 5,
+  C_SEEDS = 6,          // seeds staged by Phase 1 (valid or not)
+  C_P1_STAGE = 8,       // 6 counters: Phase-1 stage A..F time, 100 MHz ticks summed over workgroups
+  C_NCOUNTERS = 16
+};
+
+struct PreHit {         // an ungapped hit whose preliminary DP reached the gapped cutoff
+  UngHit u;
+  int32_t score;        // preliminary gapped score
+  uint32_t q_start, q_end, s_start, s_end;  // record frame (RC subject for minus)
+  uint32_t tq, ts;      // traceback seed (record frame)
+};
+
+struct PrelimParams {
+  int32_t xdrop_gap, gap_open, gap_extend, cutoff, lut_width;
+  uint32_t n_subj;
+  uint64_t n_hits, out_cap, redo_cap;
+};
 
 // Error bits written by kernels.
 enum ErrBit : uint32_t {

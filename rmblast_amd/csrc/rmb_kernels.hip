@@ -37,17 +37,59 @@ __device__ __forceinline__ uint32_t block_incl_scan256(uint32_t v, uint32_t* wsu
   return v + add;
 }
 
-// Ungapped X-drop from (q_off, s_off) over sentinel-wrapped sequences:
-// left side fixed X, right side adaptive X (ungapped.rs:223-306).
+// Byte reader over a 16-byte-aligned device pool through aligned 16-byte
+// vector loads: one load per 16 consecutive bytes instead of one per byte.
+// Pools are allocated with 64 bytes of slack so every aligned chunk that holds
+// a valid byte lies inside the allocation.  The byte is picked with a 64-bit
+// mask blend and a shift: a select between the vector's halves is turned back
+// into a lane-varying vector index, which is lowered through scratch memory.
+struct Win16 {
+  const uint8_t* __restrict__ base;
+  int64_t chunk;
+  uint64_t lo, hi;
+  __device__ __forceinline__ explicit Win16(const uint8_t* b) : base(b), chunk(-1), lo(0), hi(0) {}
+  __device__ __forceinline__ uint32_t at(int64_t i) {
+    const int64_t c = i >> 4;
+    if (c != chunk) {
+      chunk = c;
+      const uint4 v = *reinterpret_cast<const uint4*>(base + (c << 4));
+      lo = ((uint64_t)v.y << 32) | v.x;
+      hi = ((uint64_t)v.w << 32) | v.z;
+    }
+    const uint32_t k = (uint32_t)i & 15u;
+    const uint64_t m = 0ull - (uint64_t)((k >> 3) & 1u);  // all ones for the high half
+    const uint64_t h = lo ^ ((lo ^ hi) & m);
+    return (uint32_t)(h >> ((k & 7u) * 8u)) & 0xFFu;
+  }
+};
+
+// Bytes [o, o + 16) of the 32-byte little-endian concatenation lo|hi, o < 16:
+// a two-level word select and one v_alignbyte per output word.
+__device__ __forceinline__ uint4 funnel16(const uint4 lo, const uint4 hi, uint32_t o) {
+  const bool b8 = (o & 8u) != 0, b4 = (o & 4u) != 0;
+  const uint32_t ob = o & 3u;
+  const uint32_t t0 = b8 ? lo.z : lo.x, t1 = b8 ? lo.w : lo.y, t2 = b8 ? hi.x : lo.z, t3 = b8 ? hi.y : lo.w,
+                 t4 = b8 ? hi.z : hi.x, t5 = b8 ? hi.w : hi.y;
+  const uint32_t u0 = b4 ? t1 : t0, u1 = b4 ? t2 : t1, u2 = b4 ? t3 : t2, u3 = b4 ? t4 : t3, u4 = b4 ? t5 : t4;
+  return make_uint4(__builtin_amdgcn_alignbyte(u1, u0, ob), __builtin_amdgcn_alignbyte(u2, u1, ob),
+                    __builtin_amdgcn_alignbyte(u3, u2, ob), __builtin_amdgcn_alignbyte(u4, u3, ob));
+}
+
+// Ungapped X-drop from (q_off, s_off) over sentinel-wrapped sequences stored at
+// q_base / s_base of their pools: left side fixed X, right side adaptive X
+// (ungapped.rs:223-306).
 struct UngOut { int32_t score; uint32_t q_beg, q_end, steps; };
-__device__ __forceinline__ UngOut ungapped_ext(const uint8_t* __restrict__ q, int32_t q_real_len,
-                                               const uint8_t* __restrict__ s, int32_t s_real_len, int32_t q_off,
-                                               int32_t s_off, const int* mat, int32_t xdrop) {
+__device__ __forceinline__ UngOut ungapped_ext(const uint8_t* __restrict__ qpool, uint64_t q_base,
+                                               int32_t q_real_len, const uint8_t* __restrict__ spool,
+                                               uint64_t s_base, int32_t s_real_len, int32_t q_off, int32_t s_off,
+                                               const int* mat, int32_t xdrop) {
   const int32_t x = -xdrop;
   int32_t score = 0, sum = 0, q_beg = q_off;
   uint32_t steps = 0;
+  Win16 wq(qpool), ws(spool);
+  const int64_t qb0 = (int64_t)q_base + 1, sb0 = (int64_t)s_base + 1;
   for (int32_t qi = q_off - 1, si = s_off - 1; qi >= 0 && si >= 0; --qi, --si) {
-    const uint8_t qb = q[1 + qi], sb = s[1 + si];
+    const uint32_t qb = wq.at(qb0 + qi), sb = ws.at(sb0 + si);
     if (qb == 15 || sb == 15) break;
     ++steps;
     sum += mat[(qb << 4) | sb];
@@ -62,7 +104,7 @@ __device__ __forceinline__ UngOut ungapped_ext(const uint8_t* __restrict__ q, in
   int32_t q_end = q_off, xc = x;
   sum = 0;
   for (int32_t qi = q_off, si = s_off; qi < q_real_len && si < s_real_len; ++qi, ++si) {
-    const uint8_t qb = q[1 + qi], sb = s[1 + si];
+    const uint32_t qb = wq.at(qb0 + qi), sb = ws.at(sb0 + si);
     if (qb == 15 || sb == 15) break;
     ++steps;
     sum += mat[(qb << 4) | sb];
@@ -76,6 +118,98 @@ __device__ __forceinline__ UngOut ungapped_ext(const uint8_t* __restrict__ q, in
     }
   }
   return UngOut{score, (uint32_t)q_beg, (uint32_t)q_end, steps};
+}
+
+// The same extension when each side ends within kShortReach steps, which
+// covers all but a small fraction of seeds: the 32 bytes on either side of
+// the seed in both sequences come in with ten independent 16-byte loads and
+// every step then runs from registers.  `smat` is the scoring matrix with
+// kSentScore in row and column 15, so reaching a sentinel ends a side through
+// the X-drop test (the step is then not counted, as the sentinel test comes
+// first in ungapped_ext).  Returns false when a side is still running at the
+// end of its window; the caller then runs ungapped_ext.
+static constexpr int32_t kSentScore = -(1 << 20);
+static constexpr int kShortReach = 32;
+
+__device__ __forceinline__ bool ungapped_short(const uint8_t* __restrict__ qpool, int64_t qa,
+                                               const uint8_t* __restrict__ spool, int64_t sa, int32_t q_off,
+                                               const int* smat, int32_t xdrop, UngOut& out) {
+  const uint4* QV = reinterpret_cast<const uint4*>(qpool);
+  const uint4* SV = reinterpret_cast<const uint4*>(spool);
+  const int64_t cq = qa >> 4, cs = sa >> 4;
+  // chunks below the pool start only hold bytes that sit past the leading
+  // sentinel of the first record, which the extension never reads
+  const uint4 q0 = QV[cq >= 2 ? cq - 2 : 0], q1 = QV[cq >= 1 ? cq - 1 : 0], q2 = QV[cq], q3 = QV[cq + 1],
+              q4 = QV[cq + 2];
+  const uint4 s0 = SV[cs >= 2 ? cs - 2 : 0], s1 = SV[cs >= 1 ? cs - 1 : 0], s2 = SV[cs], s3 = SV[cs + 1],
+              s4 = SV[cs + 2];
+  const uint32_t oq = (uint32_t)qa & 15u, os = (uint32_t)sa & 15u;
+  // left windows: bytes qa-32 .. qa-1; right windows: bytes qa .. qa+31
+  const uint4 la = funnel16(q0, q1, oq), lb = funnel16(q1, q2, oq);
+  const uint4 ta = funnel16(s0, s1, os), tb = funnel16(s1, s2, os);
+  const uint4 ra = funnel16(q2, q3, oq), rb = funnel16(q3, q4, oq);
+  const uint4 ua = funnel16(s2, s3, os), ub = funnel16(s3, s4, os);
+  const int32_t x = -xdrop;
+  int32_t score = 0, sum = 0, q_beg = q_off;
+  uint32_t steps = 0;
+  {
+    // left, consumed from qa-1 downwards
+    const uint32_t lq[8] = {la.x, la.y, la.z, la.w, lb.x, lb.y, lb.z, lb.w};
+    const uint32_t ls[8] = {ta.x, ta.y, ta.z, ta.w, tb.x, tb.y, tb.z, tb.w};
+    bool run = true;
+#pragma unroll
+    for (int w = 7; w >= 0; --w) {
+#pragma unroll
+      for (int b = 3; b >= 0; --b) {
+        if (run) {
+          const uint32_t qb = (lq[w] >> (8 * b)) & 0xFFu, sb = (ls[w] >> (8 * b)) & 0xFFu;
+          const int32_t v = smat[(qb << 4) | sb];
+          ++steps;
+          sum += v;
+          if (sum > 0) {
+            q_beg = q_off - (kShortReach - (4 * w + b));
+            score += sum;
+            sum = 0;
+          } else if (sum < x) {
+            run = false;
+            if (v == kSentScore) --steps;
+          }
+        }
+      }
+    }
+    if (run) return false;
+  }
+  int32_t q_end = q_off, xc = x;
+  sum = 0;
+  {
+    const uint32_t rq[8] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y, rb.z, rb.w};
+    const uint32_t rs[8] = {ua.x, ua.y, ua.z, ua.w, ub.x, ub.y, ub.z, ub.w};
+    bool run = true;
+#pragma unroll
+    for (int w = 0; w < 8; ++w) {
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        if (run) {
+          const uint32_t qb = (rq[w] >> (8 * b)) & 0xFFu, sb = (rs[w] >> (8 * b)) & 0xFFu;
+          const int32_t v = smat[(qb << 4) | sb];
+          ++steps;
+          sum += v;
+          if (sum > 0) {
+            q_end = q_off + 4 * w + b + 1;
+            score += sum;
+            xc = (-score > x) ? -score : x;
+            sum = 0;
+          } else if (sum < xc) {
+            run = false;
+            if (v == kSentScore) --steps;
+          }
+        }
+      }
+    }
+    if (run) return false;
+  }
+  out = UngOut{score, (uint32_t)q_beg, (uint32_t)q_end, steps};
+  return true;
 }
 
 // --------------------------------------------------------- lookup build -----
@@ -131,13 +265,13 @@ __global__ void k_lut_keys(const RecDesc* __restrict__ recs, uint32_t n_recs, co
 // replay keeps and that reach the cutoff are emitted.
 enum SeedFlag : uint8_t { F_VALID = 1, F_HIT = 2, F_EMIT = 4 };
 
-__global__ __launch_bounds__(kP1Threads) void k_phase1(
+__global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
     P1Params P, const RecDesc* __restrict__ recs, const SubjDesc* __restrict__ subjs,
     const uint8_t* __restrict__ qfwd, const uint8_t* __restrict__ qrc, const uint8_t* __restrict__ qext,
     const uint8_t* __restrict__ splus, const uint8_t* __restrict__ src, const uint32_t* __restrict__ cells,
     const uint32_t* __restrict__ entries, const int32_t* __restrict__ gmat, UngHit* __restrict__ out,
-    unsigned long long* __restrict__ out_count, uint2* __restrict__ spill, unsigned long long* __restrict__ queue,
-    unsigned long long* __restrict__ counters, uint32_t* __restrict__ err) {
+    unsigned long long* __restrict__ out_count, uint2* __restrict__ spill, uint4* __restrict__ hitres,
+    unsigned long long* __restrict__ queue, unsigned long long* __restrict__ counters, uint32_t* __restrict__ err) {
   __shared__ int mat[256];
   __shared__ uint32_t s_cum[kP1Threads], s_cst[kP1Threads], wsum[4];
   __shared__ uint32_t seed_s[kSeedCap], seed_q[kSeedCap];
@@ -153,26 +287,50 @@ __global__ __launch_bounds__(kP1Threads) void k_phase1(
   __shared__ uint32_t s_next_slot, s_next_k;
 
   const int tid = threadIdx.x;
-  for (int i = tid; i < 256; i += kP1Threads) mat[i] = gmat[i];
+  const uint32_t xcd = blockIdx.x % kXcds;
+  for (int i = tid; i < 256; i += kP1Threads) mat[i] = ((i >> 4) == 15 || (i & 15) == 15) ? kSentScore : gmat[i];
   uint2* my_spill = spill + (size_t)blockIdx.x * kBuckets * kHashGlobal;
+  uint4* my_hitres = hitres + (size_t)blockIdx.x * kSeedCap;  // extension results of this pass's hits
   const int32_t ws = P.word_size, lw = P.lut_width, step = P.scan_step;
   const int32_t ext_needed = ws - lw;
   const int32_t window = 1 - ws;
-  unsigned long long n_evals = 0, n_steps = 0, n_hits = 0;
+  unsigned long long n_evals = 0, n_steps = 0, n_hits = 0, n_seeds = 0;
+  // per-stage time (thread 0, 100 MHz clock read after each barrier)
+  unsigned long long t_stage[6] = {0, 0, 0, 0, 0, 0}, t_prev = wall_clock64();
+  auto tick = [&](int k) {
+    if (tid == 0) {
+      const unsigned long long t = wall_clock64();
+      t_stage[k] += t - t_prev;
+      t_prev = t;
+    }
+  };
 
   for (;;) {
     __syncthreads();
-    if (tid == 0) s_item = atomicAdd(queue, 1ull);
+    if (tid == 0) {
+      // XCD-aware queues: queue x hands out the items of records r = x (mod 8),
+      // record-major, so one XCD's workgroups share a record's lookup table in
+      // their L2; an XCD whose queue is empty takes work from the others.
+      unsigned long long it = ~0ull;
+      const uint32_t n_recs = (uint32_t)(P.n_items / P.n_subj);
+      for (int k = 0; k < kXcds && it == ~0ull; ++k) {
+        const uint32_t x = (xcd + k) % kXcds;
+        const unsigned long long n_x = (unsigned long long)(n_recs > x ? (n_recs - x + kXcds - 1) / kXcds : 0) *
+                                       P.n_subj;
+        if (n_x == 0) continue;
+        const unsigned long long t = atomicAdd(&queue[x], 1ull);
+        if (t < n_x) it = (unsigned long long)(x + kXcds * (t / P.n_subj)) * P.n_subj + t % P.n_subj;
+      }
+      s_item = it;
+    }
     __syncthreads();
     const unsigned long long item = s_item;
-    if (item >= P.n_items) break;
+    if (item == ~0ull) break;
     const uint32_t r = (uint32_t)(item / P.n_subj), f = (uint32_t)(item % P.n_subj);
     const RecDesc R = recs[r];
     const SubjDesc S = subjs[f];
     const int32_t q_len = (int32_t)R.q_len, s_len = (int32_t)S.s_len;
     if (q_len < ws || s_len < ws) continue;
-    const uint8_t* Q = qfwd + R.q_base;
-    const uint8_t* QR = qrc + R.q_base;
     const uint8_t* QE = qext + R.q_base;
     const uint8_t* SP = splus + S.s_base;
     const uint8_t* SR = src + S.s_base;
@@ -200,8 +358,10 @@ __global__ __launch_bounds__(kP1Threads) void k_phase1(
       const uint32_t inc = block_incl_scan256(cnt, wsum);
       s_cum[tid] = inc;
       __syncthreads();
+      tick(0);
       const uint32_t total = s_cum[kP1Threads - 1];
       const uint32_t take = total < (uint32_t)kSeedCap ? total : (uint32_t)kSeedCap;
+      n_seeds += tid == 0 ? take : 0;
       // ---- B: materialise seeds in scan order
       for (uint32_t i = tid; i < take; i += kP1Threads) {
         int lo = 0, hi = kP1Threads - 1;
@@ -230,6 +390,7 @@ __global__ __launch_bounds__(kP1Threads) void k_phase1(
       }
       for (int b = tid; b < kBuckets; b += kP1Threads) bcnt[b] = 0;
       __syncthreads();
+      tick(1);
       // ---- C: exact word extension, diagonal, speculative ungapped extension
       for (uint32_t i = tid; i < take; i += kP1Threads) {
         const int32_t s_fwd = (int32_t)seed_s[i];
@@ -263,8 +424,13 @@ __global__ __launch_bounds__(kP1Threads) void k_phase1(
           if (ok) {
             diag = (int32_t)((uint32_t)s_off - (uint32_t)q_off);
             chk = (uint32_t)s_off;
-            const UngOut u = ungapped_ext(Q, q_len + 1, SP, s_len + 1, q_off, s_off, mat, P.xdrop_ungap);
+            UngOut u;
+            if (!ungapped_short(qfwd, (int64_t)R.q_base + 1 + q_off, splus, (int64_t)S.s_base + 1 + s_off, q_off, mat,
+                                P.xdrop_ungap, u))
+              u = ungapped_ext(qfwd, R.q_base, q_len + 1, splus, S.s_base, s_len + 1, q_off, s_off, mat,
+                               P.xdrop_ungap);
             const bool hit = u.score >= P.cutoff;
+            if (hit) my_hitres[i] = make_uint4(u.q_beg, u.q_end, (uint32_t)u.score, 0u);
             level = hit ? (uint32_t)s_off + (u.q_end - (uint32_t)q_off) : (uint32_t)(s_off + ws);
             steps = u.steps;
             flag = F_VALID | (hit ? F_HIT : 0);
@@ -299,8 +465,12 @@ __global__ __launch_bounds__(kP1Threads) void k_phase1(
             const uint32_t s_fwd_eff = (uint32_t)(s_len - s_off - lw);
             chk = s_fwd_eff;
             const int32_t qn = q_len - 1 - q_off, sn = s_len - 1 - s_off;
-            const UngOut u = ungapped_ext(QR, q_len + 1, SP, s_len + 1, qn, sn, mat, P.xdrop_ungap);
+            UngOut u;
+            if (!ungapped_short(qrc, (int64_t)R.q_base + 1 + qn, splus, (int64_t)S.s_base + 1 + sn, qn, mat,
+                                P.xdrop_ungap, u))
+              u = ungapped_ext(qrc, R.q_base, q_len + 1, splus, S.s_base, s_len + 1, qn, sn, mat, P.xdrop_ungap);
             const bool hit = u.score >= P.cutoff;
+            if (hit) my_hitres[i] = make_uint4(u.q_beg, u.q_end, (uint32_t)u.score, 0u);
             level = hit ? (uint32_t)sn + (u.q_end - (uint32_t)qn) : s_fwd_eff + (uint32_t)ws;
             steps = u.steps;
             flag = F_VALID | (hit ? F_HIT : 0);
@@ -314,6 +484,7 @@ __global__ __launch_bounds__(kP1Threads) void k_phase1(
         if (flag) atomicAdd(&bcnt[(uint32_t)diag & (kBuckets - 1)], 1u);
       }
       __syncthreads();
+      tick(2);
       // ---- D: bucket lists (counting sort; each owner re-sorts by seed index)
       {
         const uint32_t v0 = bcnt[2 * tid], v1 = bcnt[2 * tid + 1];
@@ -325,6 +496,7 @@ __global__ __launch_bounds__(kP1Threads) void k_phase1(
       for (uint32_t i = tid; i < take; i += kP1Threads)
         if (sd_flag[i]) blist[atomicAdd(&bst[(uint32_t)sd_diag[i] & (kBuckets - 1)], 1u)] = (uint16_t)i;
       __syncthreads();
+      tick(3);
       // ---- E: exact replay of each bucket (diag_hash.rs:57-111)
       for (int bb = tid; bb < kBuckets; bb += kP1Threads) {
         const uint32_t nb = bcnt[bb];
@@ -384,6 +556,7 @@ __global__ __launch_bounds__(kP1Threads) void k_phase1(
         hcnt[bb] = (uint16_t)cnt_b;
       }
       __syncthreads();
+      tick(4);
       // ---- F: emit kept hits in the record frame
       for (uint32_t i = tid; i < take; i += kP1Threads) {
         if (!(sd_flag[i] & F_EMIT)) continue;
@@ -394,7 +567,8 @@ __global__ __launch_bounds__(kP1Threads) void k_phase1(
         if (q0c < c1) {
           const int32_t q_off = q0c - ((int32_t)seed_s[i] - (int32_t)sd_chk[i]);
           const int32_t s_off = (int32_t)sd_chk[i];
-          const UngOut u = ungapped_ext(Q, q_len + 1, SP, s_len + 1, q_off, s_off, mat, P.xdrop_ungap);
+          const uint4 hv = my_hitres[i];  // {q_beg, q_end, score} from stage C
+          const UngOut u{(int32_t)hv.z, hv.x, hv.y, 0u};
           h.q_seed = (uint32_t)q_off;
           h.s_seed = (uint32_t)s_off;
           h.q_start = u.q_beg;
@@ -408,7 +582,8 @@ __global__ __launch_bounds__(kP1Threads) void k_phase1(
           const int32_t s_off = s_len - (int32_t)s_fwd_eff - lw;
           const int32_t q_off = (q_len - (q0c - c1) - lw) + (s_off - (s_len - s_fwd - lw));
           const int32_t qn = q_len - 1 - q_off, sn = s_len - 1 - s_off;
-          const UngOut u = ungapped_ext(QR, q_len + 1, SP, s_len + 1, qn, sn, mat, P.xdrop_ungap);
+          const uint4 hv = my_hitres[i];
+          const UngOut u{(int32_t)hv.z, hv.x, hv.y, 0u};
           const uint32_t rs0 = (uint32_t)(sn - (qn - (int32_t)u.q_beg));
           const uint32_t rs1 = (uint32_t)sn + (u.q_end - (uint32_t)qn);
           h.q_seed = (uint32_t)q_off;
@@ -426,6 +601,7 @@ __global__ __launch_bounds__(kP1Threads) void k_phase1(
         else atomicOr(err, (uint32_t)E_OUT_FULL);
       }
       __syncthreads();
+      tick(5);
       slot0 = s_next_slot;
       kcur = s_next_k;
     }
@@ -433,6 +609,10 @@ __global__ __launch_bounds__(kP1Threads) void k_phase1(
   atomicAdd(&counters[C_SEED_EVALS], n_evals);
   atomicAdd(&counters[C_UNG_STEPS], n_steps);
   atomicAdd(&counters[C_UNG_HITS], n_hits);
+  if (tid == 0) {
+    atomicAdd(&counters[C_SEEDS], n_seeds);
+    for (int k = 0; k < 6; ++k) atomicAdd(&counters[C_P1_STAGE + k], t_stage[k]);
+  }
 }
 
 // --------------------------------------------------------- X-drop DP ---------
@@ -632,7 +812,107 @@ __global__ __launch_bounds__(256) void k_dp_traceback(const TbTask* __restrict__
   }
 }
 
+// ------------------------------------------------- fused Phase 2a DPs -------
+// Phase-2a pivot of an ungapped hit (engine.rs:1850-1895), record frame in,
+// DP frame out: (pq, ps) splits the score-only alignment, (tq, ts) is the
+// traceback seed kept for Phase 2b.
+__device__ __forceinline__ void prelim_pivot(const UngHit& u, uint32_t qa_len, uint32_t sa_len, uint32_t lw,
+                                             uint32_t& pq, uint32_t& ps, uint32_t& tq, uint32_t& ts) {
+  if (!u.strand) {
+    const bool shift = u.s_end >= u.s_seed + 8;
+    tq = u.q_seed + (shift ? 3u : 0u);
+    ts = u.s_seed + (shift ? 3u : 0u);
+    const uint32_t up = 4 - (ts & 3);
+    ps = ts + up;
+    pq = tq + up;
+    if (ps >= sa_len || pq >= qa_len) { ps -= 4; pq -= 4; }
+  } else {
+    const uint32_t saved_fwd = sa_len - u.s_seed - lw;
+    const bool shift = (sa_len - u.s_start) >= saved_fwd + 8;
+    tq = shift ? u.q_seed + lw - 4 : u.q_seed;
+    ts = shift ? u.s_seed + lw - 4 : u.s_seed;
+    const uint32_t qn = (qa_len - u.q_seed - lw) + (shift ? 3u : 0u);
+    const uint32_t sn = (sa_len - u.s_seed - lw) + (shift ? 3u : 0u);
+    const uint32_t up = 4 - (sn & 3);
+    pq = qn + up;
+    ps = sn + up;
+    if (pq > qa_len || ps > sa_len) { pq -= 4; ps -= 4; }
+  }
+}
+
+// One lane per ungapped hit: pivot, both score-only halves (subject outer,
+// gapped.rs:360-418), and the cutoff test.  The DP is a pure function of the
+// hit, and a hit whose DP falls below the cutoff never changes the containment
+// state of the ordered Phase-2a loop (engine.rs:1317-1411), so only the hits
+// that reach the cutoff are kept for the ordered replay.  Hits whose band
+// outgrows the lane's ring go to `redo` for the wide-ring path.
+__global__ __launch_bounds__(256) void k_prelim_fused(
+    PrelimParams P, const UngHit* __restrict__ hits, const RecDesc* __restrict__ recs,
+    const SubjDesc* __restrict__ subjs, const uint8_t* __restrict__ qfwd, const uint8_t* __restrict__ qrc,
+    const uint8_t* __restrict__ splus, const int32_t* __restrict__ gmat, int2* __restrict__ ring, uint32_t wlog2,
+    PreHit* __restrict__ out, unsigned long long* __restrict__ out_count, UngHit* __restrict__ redo,
+    unsigned long long* __restrict__ redo_count, unsigned long long* __restrict__ counters,
+    uint32_t* __restrict__ err) {
+  __shared__ int mat[256];
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) mat[i] = gmat[i];
+  __syncthreads();
+  const uint64_t gt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (uint64_t)gridDim.x * blockDim.x;
+  int2* my = ring + (gt << wlog2);
+  const uint32_t wmask = (1u << wlog2) - 1;
+  unsigned long long cells = 0, dps = 0;
+  for (uint64_t i = gt; i < P.n_hits; i += nt) {
+    const UngHit u = hits[i];
+    const uint32_t r = u.item / P.n_subj, f = u.item % P.n_subj;
+    const RecDesc R = recs[r];
+    const SubjDesc S = subjs[f];
+    const uint32_t ql = R.q_len, sl = S.s_len;
+    uint32_t pq, ps, tq, ts;
+    prelim_pivot(u, ql, sl, (uint32_t)P.lut_width, pq, ps, tq, ts);
+    const uint8_t* qa = (u.strand ? qrc : qfwd) + R.q_base + 1;
+    const uint8_t* sa = splus + S.s_base + 1;
+    const int32_t xd = P.xdrop_gap < u.score ? P.xdrop_gap : u.score;
+    DpTask left{sa, qa, ps, pq, xd, 1};
+    DpTask right{sa + ps, qa + pq, sl > ps ? sl - ps : 0, ql > pq ? ql - pq : 0, xd, 0};
+    DpResult L, Rr;
+    dp_half<false>(left, mat, P.gap_open, P.gap_extend, my, wmask, L, nullptr, nullptr, 0);
+    dp_half<false>(right, mat, P.gap_open, P.gap_extend, my, wmask, Rr, nullptr, nullptr, 0);
+    ++dps;
+    cells += L.cells + Rr.cells;
+    if ((L.flags | Rr.flags) & E_BAND_WIDE) {
+      const unsigned long long k = atomicAdd(redo_count, 1ull);
+      if (k < P.redo_cap) redo[k] = u;
+      else atomicOr(err, (uint32_t)E_OUT_FULL);
+      continue;
+    }
+    const int32_t sc = L.score + Rr.score;
+    if (sc < 0 || sc < P.cutoff) continue;
+    const uint32_t q0 = pq - L.b_len, q1 = pq + Rr.b_len, s0 = ps - L.a_len, s1 = ps + Rr.a_len;
+    PreHit h;
+    h.u = u;
+    h.score = sc;
+    if (u.strand) { h.q_start = ql - q1; h.q_end = ql - q0; h.s_start = sl - s1; h.s_end = sl - s0; }
+    else { h.q_start = q0; h.q_end = q1; h.s_start = s0; h.s_end = s1; }
+    h.tq = tq;
+    h.ts = ts;
+    const unsigned long long k = atomicAdd(out_count, 1ull);
+    if (k < P.out_cap) out[k] = h;
+    else atomicOr(err, (uint32_t)E_OUT_FULL);
+  }
+  atomicAdd(&counters[C_PRELIM_CELLS], cells);
+  atomicAdd(&counters[C_PRELIM_DPS], dps);
+}
+
 // ---------------------------------------------------------- launchers --------
+hipError_t launch_prelim_fused(int blocks, PrelimParams P, const UngHit* hits, const RecDesc* recs,
+                               const SubjDesc* subjs, const uint8_t* qfwd, const uint8_t* qrc, const uint8_t* splus,
+                               const int32_t* gmat, int2* ring, uint32_t wlog2, PreHit* out,
+                               unsigned long long* out_count, UngHit* redo, unsigned long long* redo_count,
+                               unsigned long long* counters, uint32_t* err, hipStream_t st) {
+  hipLaunchKernelGGL(k_prelim_fused, dim3(blocks), dim3(256), 0, st, P, hits, recs, subjs, qfwd, qrc, splus, gmat,
+                     ring, wlog2, out, out_count, redo, redo_count, counters, err);
+  return hipGetLastError();
+}
+
 hipError_t launch_lut_keys(const RecDesc* recs, uint32_t n_recs, const uint64_t* elem_base, const uint8_t* qext,
                            int lw, int ws, int desc, uint64_t* keys, uint32_t* vals, uint32_t* counts,
                            uint64_t invalid_key, uint64_t total, hipStream_t st) {
@@ -655,10 +935,10 @@ hipError_t exclusive_sum_u32(void* temp, size_t& temp_bytes, const uint32_t* in,
 hipError_t launch_phase1(int blocks, P1Params P, const RecDesc* recs, const SubjDesc* subjs, const uint8_t* qfwd,
                          const uint8_t* qrc, const uint8_t* qext, const uint8_t* splus, const uint8_t* src,
                          const uint32_t* cells, const uint32_t* entries, const int32_t* gmat, UngHit* out,
-                         unsigned long long* out_count, uint2* spill, unsigned long long* queue,
+                         unsigned long long* out_count, uint2* spill, uint4* hitres, unsigned long long* queue,
                          unsigned long long* counters, uint32_t* err, hipStream_t st) {
   hipLaunchKernelGGL(k_phase1, dim3(blocks), dim3(kP1Threads), 0, st, P, recs, subjs, qfwd, qrc, qext, splus, src,
-                     cells, entries, gmat, out, out_count, spill, queue, counters, err);
+                     cells, entries, gmat, out, out_count, spill, hitres, queue, counters, err);
   return hipGetLastError();
 }
 

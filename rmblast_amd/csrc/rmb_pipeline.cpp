@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <fstream>
 #include <sstream>
@@ -37,12 +38,16 @@ hipError_t sort_pairs_u64(void*, size_t&, const uint64_t*, uint64_t*, const uint
 hipError_t exclusive_sum_u32(void*, size_t&, const uint32_t*, uint32_t*, uint64_t, hipStream_t);
 hipError_t launch_phase1(int, P1Params, const RecDesc*, const SubjDesc*, const uint8_t*, const uint8_t*,
                          const uint8_t*, const uint8_t*, const uint8_t*, const uint32_t*, const uint32_t*,
-                         const int32_t*, UngHit*, unsigned long long*, uint2*, unsigned long long*,
+                         const int32_t*, UngHit*, unsigned long long*, uint2*, uint4*, unsigned long long*,
                          unsigned long long*, uint32_t*, hipStream_t);
 hipError_t launch_dp_score(int, const DpTask*, uint32_t, const int32_t*, int32_t, int32_t, int2*, uint32_t,
                            DpResult*, unsigned long long*, hipStream_t);
 hipError_t launch_dp_traceback(int, const TbTask*, uint32_t, const int32_t*, int32_t, int32_t, int2*, uint32_t,
                                uint8_t*, uint32_t*, uint32_t*, TbResult*, hipStream_t);
+hipError_t launch_prelim_fused(int, PrelimParams, const UngHit*, const RecDesc*, const SubjDesc*, const uint8_t*,
+                               const uint8_t*, const uint8_t*, const int32_t*, int2*, uint32_t, PreHit*,
+                               unsigned long long*, UngHit*, unsigned long long*, unsigned long long*, uint32_t*,
+                               hipStream_t);
 
 struct Failure : std::runtime_error {
   int code;
@@ -63,7 +68,9 @@ struct DevBuf {
     if (bytes > cap) {
       if (p) hip_check(hipFree(p), "hipFree");
       p = nullptr;
-      hip_check(hipMalloc(&p, bytes), "hipMalloc");
+      // 64 bytes of tail: the kernels' aligned 16-byte sequence windows may
+      // read up to 15 bytes past the last sentinel of a pool.
+      hip_check(hipMalloc(&p, bytes + 64), "hipMalloc");
       cap = bytes;
     }
     return (T*)p;
@@ -182,12 +189,13 @@ struct rmb_ctx {
   // results of the last search
   std::vector<rmb_hsp> hsps;
   std::vector<rmb_op> ops;
-  uint64_t stats[16] = {0};
+  uint64_t stats[RMB_NSTATS] = {0};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // scratch
   DevBuf b_qfwd, b_qrc, b_recs, b_elem, b_keys, b_keys2, b_vals, b_vals2, b_counts, b_cells, b_sort_tmp;
-  DevBuf b_hits, b_hitcount, b_spill, b_queue, b_counters, b_err;
+  DevBuf b_hits, b_hitcount, b_spill, b_hitres, b_queue, b_counters, b_err;
   DevBuf b_tasks, b_dpres, b_ring, b_ring_big, b_tbtasks, b_tbres, b_flat, b_rows, b_ops;
+  DevBuf b_pre, b_redo, b_precount;
 };
 
 namespace rmb {
@@ -203,6 +211,19 @@ struct Timer {
   }
 };
 
+// Runs one kernel launch between two events on the context's stream and adds
+// its duration in nanoseconds to stats[slot].
+template <class F>
+static void timed_launch(rmb_ctx* c, int slot, F&& launch) {
+  hip_check(hipEventRecord(c->ev0, c->st), "event");
+  launch();
+  hip_check(hipEventRecord(c->ev1, c->st), "event");
+  hip_check(hipEventSynchronize(c->ev1), "event sync");
+  float ms = 0;
+  hip_check(hipEventElapsedTime(&ms, c->ev0, c->ev1), "event time");
+  c->stats[slot] += (uint64_t)((double)ms * 1e6);
+}
+
 // Run a list of lane-serial score-only DPs; tasks whose band outgrows the
 // common ring are rerun once with a ring of 2^kRingBigLog2 cells.
 static void run_dp_score(rmb_ctx* c, const std::vector<DpTask>& tasks, std::vector<DpResult>& res) {
@@ -217,9 +238,11 @@ static void run_dp_score(rmb_ctx* c, const std::vector<DpTask>& tasks, std::vect
     hip_check(hipMemcpyAsync(dt, tasks.data() + base, n * sizeof(DpTask), hipMemcpyHostToDevice, c->st), "copy tasks");
     const uint32_t lanes = std::min<uint32_t>((uint32_t)c->cu_count * 4 * 64, ((n + 255) / 256) * 256);
     int2* ring = c->b_ring.get<int2>((size_t)lanes << kRingLog2);
-    hip_check(launch_dp_score((int)(lanes / 256), dt, n, c->d_mat, c->P.gap_open, c->P.gap_extend, ring, kRingLog2,
-                              dr, ctr, c->st),
-              "k_dp_score");
+    timed_launch(c, 14, [&] {
+      hip_check(launch_dp_score((int)(lanes / 256), dt, n, c->d_mat, c->P.gap_open, c->P.gap_extend, ring,
+                                kRingLog2, dr, ctr, c->st),
+                "k_dp_score");
+    });
     hip_check(hipMemcpyAsync(res.data() + base, dr, n * sizeof(DpResult), hipMemcpyDeviceToHost, c->st), "copy dp");
   }
   hip_check(hipStreamSynchronize(c->st), "dp sync");
@@ -235,9 +258,11 @@ static void run_dp_score(rmb_ctx* c, const std::vector<DpTask>& tasks, std::vect
   hip_check(hipMemcpyAsync(dt, wt.data(), n * sizeof(DpTask), hipMemcpyHostToDevice, c->st), "copy wide");
   const uint32_t lanes = std::min<uint32_t>(4096, ((n + 255) / 256) * 256);
   int2* ring = c->b_ring_big.get<int2>((size_t)lanes << kRingBigLog2);
-  hip_check(launch_dp_score((int)(lanes / 256), dt, n, c->d_mat, c->P.gap_open, c->P.gap_extend, ring, kRingBigLog2,
-                            dr, ctr, c->st),
-            "k_dp_score wide");
+  timed_launch(c, 14, [&] {
+    hip_check(launch_dp_score((int)(lanes / 256), dt, n, c->d_mat, c->P.gap_open, c->P.gap_extend, ring,
+                              kRingBigLog2, dr, ctr, c->st),
+              "k_dp_score wide");
+  });
   std::vector<DpResult> wr(n);
   hip_check(hipMemcpyAsync(wr.data(), dr, n * sizeof(DpResult), hipMemcpyDeviceToHost, c->st), "copy wide res");
   hip_check(hipStreamSynchronize(c->st), "dp wide sync");
@@ -297,9 +322,11 @@ static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::
     std::vector<TbResult> tr(n);
     const uint32_t lanes = std::min<uint32_t>((uint32_t)c->cu_count * 4 * 64, ((n + 255) / 256) * 256);
     int2* ring = c->b_ring.get<int2>((size_t)lanes << kRingLog2);
-    hip_check(launch_dp_traceback((int)(lanes / 256), dt, n, c->d_mat, c->P.gap_open, c->P.gap_extend, ring,
-                                  kRingLog2, dflat, drows, dops, dr, c->st),
-              "k_dp_traceback");
+    timed_launch(c, 14, [&] {
+      hip_check(launch_dp_traceback((int)(lanes / 256), dt, n, c->d_mat, c->P.gap_open, c->P.gap_extend, ring,
+                                    kRingLog2, dflat, drows, dops, dr, c->st),
+                "k_dp_traceback");
+    });
     hip_check(hipMemcpyAsync(tr.data(), dr, n * sizeof(TbResult), hipMemcpyDeviceToHost, c->st), "copy tb res");
     hip_check(hipStreamSynchronize(c->st), "tb sync");
     // rerun band-wide tasks with the big ring
@@ -314,9 +341,11 @@ static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::
       hip_check(hipMemcpyAsync(rdt, rb.data(), rn * sizeof(TbTask), hipMemcpyHostToDevice, c->st), "copy tb redo");
       const uint32_t rl = std::min<uint32_t>(4096, ((rn + 255) / 256) * 256);
       int2* big = c->b_ring_big.get<int2>((size_t)rl << kRingBigLog2);
-      hip_check(launch_dp_traceback((int)(rl / 256), rdt, rn, c->d_mat, c->P.gap_open, c->P.gap_extend, big,
-                                    kRingBigLog2, dflat, drows, dops, dr, c->st),
-                "k_dp_traceback wide");
+      timed_launch(c, 14, [&] {
+        hip_check(launch_dp_traceback((int)(rl / 256), rdt, rn, c->d_mat, c->P.gap_open, c->P.gap_extend, big,
+                                      kRingBigLog2, dflat, drows, dops, dr, c->st),
+                  "k_dp_traceback wide");
+      });
       std::vector<TbResult> rr(rn);
       hip_check(hipMemcpyAsync(rr.data(), dr, rn * sizeof(TbResult), hipMemcpyDeviceToHost, c->st), "copy redo");
       hip_check(hipStreamSynchronize(c->st), "tb redo sync");
@@ -336,18 +365,104 @@ static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::
 
 struct ItemState {
   uint32_t item = 0;
-  std::vector<UngRec> hits;
-  size_t cursor = 0;
+  std::vector<uint32_t> surv;    // indices of the item's surviving hits
   std::vector<PreRec> pre;
   std::vector<FinalRec> fin;
-  // Phase-2a round bookkeeping
-  std::vector<uint32_t> req;     // hit indices with a DP this round
-  size_t task0 = 0;              // first task of this item in the round
 };
 
-// Phase 1 for a group of records sharing one lookup width.
+// Preliminary gapped DPs of every ungapped hit of a group, on the GPU; the hits
+// reaching the gapped cutoff come back with their DP results (item ids mapped
+// to batch records).  Hits whose band outgrew the lane ring are redone here
+// with the wide ring.
+static void prelim_group(rmb_ctx* c, const std::vector<RecDesc>& recs, const std::vector<uint32_t>& recs_idx,
+                         const RecDesc* drecs, int lw, const uint8_t* d_qfwd, const uint8_t* d_qrc,
+                         const UngHit* dh, uint64_t nh, std::vector<PreHit>& out) {
+  const uint32_t nsub = (uint32_t)c->subj.size();
+  const int32_t cutoff = std::max(c->P.min_raw_gapped_score, 1);
+  PrelimParams PP{};
+  PP.xdrop_gap = c->P.xdrop_gap;
+  PP.gap_open = c->P.gap_open;
+  PP.gap_extend = c->P.gap_extend;
+  PP.cutoff = cutoff;
+  PP.lut_width = lw;
+  PP.n_subj = nsub;
+  PP.n_hits = nh;
+  PP.out_cap = nh;
+  PP.redo_cap = std::max<uint64_t>(4096, nh / 8);
+  PreHit* dout = c->b_pre.get<PreHit>(PP.out_cap);
+  UngHit* dredo = c->b_redo.get<UngHit>(PP.redo_cap);
+  unsigned long long* cnt = c->b_precount.get<unsigned long long>(2);
+  uint32_t* err = c->b_err.get<uint32_t>(1);
+  unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
+  const uint64_t lanes = std::min<uint64_t>((uint64_t)c->cu_count * 8 * 64, ((nh + 255) / 256) * 256);
+  int2* ring = c->b_ring.get<int2>((size_t)lanes << kRingLog2);
+  hip_check(hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned long long), c->st), "memset prelim counts");
+  hip_check(hipMemsetAsync(err, 0, sizeof(uint32_t), c->st), "memset err");
+  timed_launch(c, 14, [&] {
+    hip_check(launch_prelim_fused((int)(lanes / 256), PP, dh, drecs, (const SubjDesc*)c->d_sdesc.p, d_qfwd, d_qrc,
+                                  (const uint8_t*)c->d_splus.p, c->d_mat, ring, kRingLog2, dout, cnt, dredo, cnt + 1,
+                                  ctr, err, c->st),
+              "k_prelim_fused");
+  });
+  unsigned long long n[2];
+  uint32_t e = 0;
+  hip_check(hipMemcpyAsync(n, cnt, sizeof n, hipMemcpyDeviceToHost, c->st), "copy prelim counts");
+  hip_check(hipMemcpyAsync(&e, err, sizeof e, hipMemcpyDeviceToHost, c->st), "copy err");
+  hip_check(hipStreamSynchronize(c->st), "prelim sync");
+  if (e & E_OUT_FULL) throw Failure(RMB_ERR_CAPACITY, "preliminary redo list overflow");
+  const size_t base = out.size();
+  out.resize(base + n[0]);
+  hip_check(hipMemcpy(out.data() + base, dout, n[0] * sizeof(PreHit), hipMemcpyDeviceToHost), "copy prelims");
+  std::vector<UngHit> redo(n[1]);
+  if (n[1]) hip_check(hipMemcpy(redo.data(), dredo, n[1] * sizeof(UngHit), hipMemcpyDeviceToHost), "copy redo");
+  for (size_t i = base; i < out.size(); ++i) {
+    const uint32_t r = out[i].u.item / nsub, f = out[i].u.item % nsub;
+    out[i].u.item = recs_idx[r] * nsub + f;
+  }
+  if (redo.empty()) return;
+  std::vector<DpTask> tasks;
+  std::vector<std::array<uint32_t, 4>> piv;
+  for (const UngHit& h : redo) {
+    const uint32_t r = h.item / nsub, f = h.item % nsub;
+    const uint32_t ql = recs[r].q_len, sl = c->sdesc[f].s_len;
+    UngRec u{h.q_seed, h.s_seed, h.q_start, h.q_end, h.s_start, h.s_end, h.score, (uint8_t)h.strand};
+    uint32_t pq, ps, tq, ts;
+    pivot_of(u, ql, sl, (uint32_t)lw, pq, ps, tq, ts);
+    piv.push_back({pq, ps, tq, ts});
+    const uint8_t* qa = (h.strand ? d_qrc : d_qfwd) + recs[r].q_base + 1;
+    const uint8_t* sa = (const uint8_t*)c->d_splus.p + c->sdesc[f].s_base + 1;
+    const int32_t xd = std::min(c->P.xdrop_gap, h.score);
+    tasks.push_back(DpTask{sa, qa, ps, pq, xd, 1});
+    tasks.push_back(DpTask{sa + ps, qa + pq, sl > ps ? sl - ps : 0, ql > pq ? ql - pq : 0, xd, 0});
+  }
+  std::vector<DpResult> res;
+  run_dp_score(c, tasks, res);
+  for (size_t k = 0; k < redo.size(); ++k) {
+    const UngHit& h = redo[k];
+    const DpResult &L = res[2 * k], &R = res[2 * k + 1];
+    const int32_t sc = L.score + R.score;
+    if (sc < 0 || sc < cutoff) continue;
+    const uint32_t r = h.item / nsub, f = h.item % nsub;
+    const uint32_t ql = recs[r].q_len, sl = c->sdesc[f].s_len;
+    const auto& pv = piv[k];
+    const uint32_t q0 = pv[0] - L.b_len, q1 = pv[0] + R.b_len, s0 = pv[1] - L.a_len, s1 = pv[1] + R.a_len;
+    PreHit p;
+    p.u = h;
+    p.u.item = recs_idx[r] * nsub + f;
+    p.score = sc;
+    if (h.strand) { p.q_start = ql - q1; p.q_end = ql - q0; p.s_start = sl - s1; p.s_end = sl - s0; }
+    else { p.q_start = q0; p.q_end = q1; p.s_start = s0; p.s_end = s1; }
+    p.tq = pv[2];
+    p.ts = pv[3];
+    out.push_back(p);
+  }
+}
+
+// Phase 1 for a group of records sharing one lookup width; with pre_out, also
+// the preliminary DPs of its hits.
 static void phase1_group(rmb_ctx* c, const std::vector<uint32_t>& recs_idx, const std::vector<RecDesc>& all_recs,
-                         int lw, const uint8_t* d_qfwd, const uint8_t* d_qrc, std::vector<UngHit>& hits_out) {
+                         int lw, const uint8_t* d_qfwd, const uint8_t* d_qrc, std::vector<UngHit>* hits_out,
+                         std::vector<PreHit>* pre_out) {
   const int ws = c->P.word_size;
   const uint32_t R = (uint32_t)recs_idx.size();
   const uint64_t cells = 1ull << (2 * lw);
@@ -394,24 +509,31 @@ static void phase1_group(rmb_ctx* c, const std::vector<uint32_t>& recs_idx, cons
                                       : (c->P.min_raw_gapped_score > 0 ? c->P.min_raw_gapped_score / 2 : 1);
   P.n_subj = nsub;
   P.n_items = (uint64_t)R * nsub;
-  const int blocks = std::max(1, std::min<int>(c->cu_count * 2, (int)std::min<uint64_t>(P.n_items, 1u << 20)));
+  // ~36 KB of LDS per workgroup: four resident per CU.  Workgroups are dealt
+  // round-robin over the XCDs, so blockIdx % kXcds names the XCD.
+  const int blocks = std::max(1, std::min<int>(c->cu_count * 4, (int)std::min<uint64_t>(P.n_items, 1u << 20)));
   uint2* spill = c->b_spill.get<uint2>((size_t)blocks * kBuckets * kHashGlobal);
-  unsigned long long* queue = c->b_queue.get<unsigned long long>(1);
+  uint4* hitres = c->b_hitres.get<uint4>((size_t)blocks * kSeedCap);
+  unsigned long long* queue = c->b_queue.get<unsigned long long>(kXcds);
   unsigned long long* hitcount = c->b_hitcount.get<unsigned long long>(1);
   unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
   uint32_t* err = c->b_err.get<uint32_t>(1);
   uint64_t cap = std::max<uint64_t>(1u << 16, c->b_hits.cap / sizeof(UngHit));
+  // counters as they stood before this group, restored when the group reruns
+  unsigned long long ctr_saved[C_NCOUNTERS];
+  hip_check(hipMemcpyAsync(ctr_saved, ctr, sizeof ctr_saved, hipMemcpyDeviceToHost, c->st), "save counters");
+  hip_check(hipStreamSynchronize(c->st), "save counters sync");
   for (int attempt = 0; attempt < 3; ++attempt) {
     UngHit* dh = c->b_hits.get<UngHit>(cap);
     cap = c->b_hits.cap / sizeof(UngHit);
     P.out_cap = cap;
-    hip_check(hipMemsetAsync(queue, 0, sizeof(unsigned long long), c->st), "memset queue");
+    hip_check(hipMemsetAsync(queue, 0, kXcds * sizeof(unsigned long long), c->st), "memset queue");
     hip_check(hipMemsetAsync(hitcount, 0, sizeof(unsigned long long), c->st), "memset hitcount");
     hip_check(hipMemsetAsync(err, 0, sizeof(uint32_t), c->st), "memset err");
     hip_check(hipEventRecord(c->ev0, c->st), "event");
     hip_check(launch_phase1(blocks, P, drecs, (const SubjDesc*)c->d_sdesc.p, d_qfwd, d_qrc, d_qfwd,
                             (const uint8_t*)c->d_splus.p, (const uint8_t*)c->d_src.p, cstart, vals2, c->d_mat, dh,
-                            hitcount, spill, queue, ctr, err, c->st),
+                            hitcount, spill, hitres, queue, ctr, err, c->st),
               "k_phase1");
     hip_check(hipEventRecord(c->ev1, c->st), "event");
     unsigned long long nh = 0;
@@ -428,16 +550,20 @@ static void phase1_group(rmb_ctx* c, const std::vector<uint32_t>& recs_idx, cons
     if (e & E_HASH_FULL) throw Failure(RMB_ERR_CAPACITY, "diagonal hash bucket exceeded 256 cells");
     if (nh > cap) {  // output larger than the buffer: grow and rerun the group
       cap = nh + nh / 4;
-      hip_check(hipMemsetAsync(ctr, 0, C_NCOUNTERS * sizeof(unsigned long long), c->st), "reset counters");
+      hip_check(hipMemcpyAsync(ctr, ctr_saved, sizeof ctr_saved, hipMemcpyHostToDevice, c->st), "restore counters");
+      hip_check(hipStreamSynchronize(c->st), "restore counters sync");
       continue;
     }
-    const size_t base = hits_out.size();
-    hits_out.resize(base + nh);
-    hip_check(hipMemcpy(hits_out.data() + base, dh, nh * sizeof(UngHit), hipMemcpyDeviceToHost), "copy hits");
-    for (size_t i = base; i < hits_out.size(); ++i) {  // item ids are group-local: map to batch records
-      const uint32_t r = hits_out[i].item / nsub, f = hits_out[i].item % nsub;
-      hits_out[i].item = recs_idx[r] * nsub + f;
+    if (hits_out) {
+      const size_t base = hits_out->size();
+      hits_out->resize(base + nh);
+      hip_check(hipMemcpy(hits_out->data() + base, dh, nh * sizeof(UngHit), hipMemcpyDeviceToHost), "copy hits");
+      for (size_t i = base; i < hits_out->size(); ++i) {  // item ids are group-local: map to batch records
+        const uint32_t r = (*hits_out)[i].item / nsub, f = (*hits_out)[i].item % nsub;
+        (*hits_out)[i].item = recs_idx[r] * nsub + f;
+      }
     }
+    if (pre_out && nh) prelim_group(c, recs, recs_idx, drecs, lw, d_qfwd, d_qrc, dh, nh, *pre_out);
     return;
   }
   throw Failure(RMB_ERR_CAPACITY, "ungapped hit buffer could not be sized");
@@ -478,7 +604,7 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
   hip_check(hipMemset(ctr, 0, C_NCOUNTERS * sizeof(unsigned long long)), "memset counters");
   // ---- Phase 1 by lookup width; groups are bounded by the cell table size
   Timer t1;
-  std::vector<UngHit> hits;
+  std::vector<PreHit> survivors;
   std::vector<int> lw_of(nq);
   for (uint32_t i = 0; i < nq; ++i) lw_of[i] = lut_width_for(ws, 2ull * lens[i]);
   for (int lw = 4; lw <= 12; ++lw) {
@@ -489,7 +615,7 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
       const bool flush = i == nq || (!grp.empty() && ((grp.size() + 1) * cells > (1ull << 28) ||
                                                       elems + 2ull * lens[i] + 1 > (1ull << 31)));
       if (flush && !grp.empty()) {
-        phase1_group(c, grp, recs, lw, d_qfwd, d_qrc, hits);
+        phase1_group(c, grp, recs, lw, d_qfwd, d_qrc, nullptr, &survivors);
         grp.clear();
         elems = 0;
       }
@@ -500,130 +626,51 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
     }
   }
   c->stats[9] = t1.us();
-  // ---- group hits by item, NCBI order per item
+  // ---- Phase 2a: ordered replay of the hits whose preliminary DP reached the
+  // cutoff, in NCBI's ungapped-hit order per item (engine.rs:1280-1411)
+  Timer t2;
+  const int32_t cutoff = std::max(c->P.min_raw_gapped_score, 1);
   std::vector<ItemState> items;
   {
-    std::vector<uint64_t> keyed(hits.size());
-    for (size_t i = 0; i < hits.size(); ++i) keyed[i] = ((uint64_t)hits[i].item << 32) | i;
+    std::vector<uint64_t> keyed(survivors.size());
+    for (size_t i = 0; i < survivors.size(); ++i) keyed[i] = ((uint64_t)survivors[i].u.item << 32) | i;
     std::sort(keyed.begin(), keyed.end());
     for (size_t i = 0; i < keyed.size();) {
       const uint32_t it = (uint32_t)(keyed[i] >> 32);
       ItemState s;
       s.item = it;
       size_t j = i;
-      for (; j < keyed.size() && (uint32_t)(keyed[j] >> 32) == it; ++j) {
-        const UngHit& h = hits[(uint32_t)keyed[j]];
-        s.hits.push_back(UngRec{h.q_seed, h.s_seed, h.q_start, h.q_end, h.s_start, h.s_end, h.score, (uint8_t)h.strand});
-      }
+      for (; j < keyed.size() && (uint32_t)(keyed[j] >> 32) == it; ++j) s.surv.push_back((uint32_t)keyed[j]);
       items.push_back(std::move(s));
       i = j;
     }
-    hits.clear();
-    hits.shrink_to_fit();
   }
   parallel_for(items.size(), [&](size_t k) {
     ItemState& s = items[k];
-    order_ungapped(s.hits, lens[s.item / nsub], c->subj[s.item % nsub].plus.size() - 2);
-  });
-  // ---- Phase 2a: ordered rounds of preliminary score-only DPs
-  Timer t2;
-  const int32_t cutoff = std::max(c->P.min_raw_gapped_score, 1);
-  const uint8_t* d_splus = (const uint8_t*)c->d_splus.p;
-  {
-    std::vector<HspTree> tp, tm;
-    tp.reserve(items.size());
-    tm.reserve(items.size());
-    for (auto& s : items) {
-      const uint32_t ql = lens[s.item / nsub], sl = (uint32_t)c->subj[s.item % nsub].plus.size() - 2;
-      tp.emplace_back(ql + 1, sl + 1);
-      tm.emplace_back(ql + 1, sl + 1);
+    const uint32_t ql = lens[s.item / nsub], sl = (uint32_t)c->subj[s.item % nsub].plus.size() - 2;
+    std::vector<UngRec> u(s.surv.size());
+    for (size_t i = 0; i < s.surv.size(); ++i) {
+      const UngHit& h = survivors[s.surv[i]].u;
+      u[i] = UngRec{h.q_seed, h.s_seed, h.q_start, h.q_end, h.s_start, h.s_end, h.score, (uint8_t)h.strand};
     }
-    std::vector<uint32_t> active(items.size());
-    for (size_t i = 0; i < items.size(); ++i) active[i] = (uint32_t)i;
-    uint64_t n_prelim = 0;
-    while (!active.empty()) {
-      // choose this round's DPs: the next uncovered hit (needed) plus a few
-      // likely-needed followers
-      parallel_for(active.size(), [&](size_t a) {
-        ItemState& s = items[active[a]];
-        s.req.clear();
-        while (s.cursor < s.hits.size()) {
-          const UngRec& u = s.hits[s.cursor];
-          const uint32_t sl = (uint32_t)c->subj[s.item % nsub].plus.size() - 2;
-          const uint32_t ss = u.minus ? sl - u.s_end : u.s_start, se = u.minus ? sl - u.s_start : u.s_end;
-          if (!(u.minus ? tm : tp)[active[a]].covers(u.q_start, u.q_end, ss, se, u.score, !u.minus)) break;
-          ++s.cursor;
-        }
-        for (size_t k = s.cursor; k < s.hits.size() && s.req.size() < (size_t)kSpecDepth; ++k) {
-          const UngRec& u = s.hits[k];
-          const uint32_t sl = (uint32_t)c->subj[s.item % nsub].plus.size() - 2;
-          const uint32_t ss = u.minus ? sl - u.s_end : u.s_start, se = u.minus ? sl - u.s_start : u.s_end;
-          if (k == s.cursor || !(u.minus ? tm : tp)[active[a]].covers(u.q_start, u.q_end, ss, se, u.score, !u.minus))
-            s.req.push_back((uint32_t)k);
-        }
-      });
-      std::vector<DpTask> tasks;
-      for (uint32_t a : active) {
-        ItemState& s = items[a];
-        s.task0 = tasks.size();
-        const uint32_t r = s.item / nsub, f = s.item % nsub;
-        const uint32_t ql = lens[r], sl = (uint32_t)c->subj[f].plus.size() - 2;
-        for (uint32_t k : s.req) {
-          const UngRec& u = s.hits[k];
-          uint32_t pq, ps, tq, ts;
-          pivot_of(u, ql, sl, (uint32_t)lw_of[r], pq, ps, tq, ts);
-          const uint8_t* qa = (u.minus ? d_qrc : d_qfwd) + recs[r].q_base + 1;
-          const uint8_t* sa = d_splus + c->sdesc[f].s_base + 1;
-          const int32_t xd = std::min(c->P.xdrop_gap, u.score);
-          tasks.push_back(DpTask{sa, qa, ps, pq, xd, 1});
-          tasks.push_back(DpTask{sa + ps, qa + pq, sl > ps ? sl - ps : 0, ql > pq ? ql - pq : 0, xd, 0});
-        }
-      }
-      n_prelim += tasks.size() / 2;
-      std::vector<DpResult> res;
-      run_dp_score(c, tasks, res);
-      std::vector<uint32_t> still;
-      std::vector<uint8_t> keep_active(active.size(), 0);
-      parallel_for(active.size(), [&](size_t a) {
-        ItemState& s = items[active[a]];
-        const uint32_t r = s.item / nsub, f = s.item % nsub;
-        const uint32_t ql = lens[r], sl = (uint32_t)c->subj[f].plus.size() - 2;
-        size_t ri = 0;
-        while (s.cursor < s.hits.size()) {
-          const UngRec& u = s.hits[s.cursor];
-          const uint32_t ss = u.minus ? sl - u.s_end : u.s_start, se = u.minus ? sl - u.s_start : u.s_end;
-          HspTree& t = (u.minus ? tm : tp)[active[a]];
-          if (t.covers(u.q_start, u.q_end, ss, se, u.score, !u.minus)) { ++s.cursor; continue; }
-          while (ri < s.req.size() && s.req[ri] < s.cursor) ++ri;
-          if (ri >= s.req.size() || s.req[ri] != s.cursor) break;  // no DP yet: next round
-          const DpResult& L = res[s.task0 + 2 * ri];
-          const DpResult& Rr = res[s.task0 + 2 * ri + 1];
-          const int32_t sc = L.score + Rr.score;
-          uint32_t pq, ps, tq, ts;
-          pivot_of(u, ql, sl, (uint32_t)lw_of[r], pq, ps, tq, ts);
-          ++s.cursor;
-          if (sc < 0) continue;
-          uint32_t q0 = pq - L.b_len, q1 = pq + Rr.b_len, s0 = ps - L.a_len, s1 = ps + Rr.a_len;
-          uint32_t qs_, qe_, ss_, se_;
-          if (u.minus) { qs_ = ql - q1; qe_ = ql - q0; ss_ = sl - s1; se_ = sl - s0; }
-          else { qs_ = q0; qe_ = q1; ss_ = s0; se_ = s1; }
-          if (sc < cutoff) continue;
-          const uint32_t nss = u.minus ? sl - se_ : ss_, nse = u.minus ? sl - ss_ : se_;
-          t.add(qs_, qe_, nss, nse, sc, !u.minus);
-          s.pre.push_back(PreRec{tq, ts, qs_, qe_, ss_, se_, sc, u.minus});
-        }
-        keep_active[a] = s.cursor < s.hits.size();
-      });
-      for (size_t a = 0; a < active.size(); ++a)
-        if (keep_active[a]) still.push_back(active[a]);
-      active.swap(still);
+    std::vector<uint32_t> ord(u.size());
+    for (size_t i = 0; i < ord.size(); ++i) ord[i] = (uint32_t)i;
+    std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return ung_before(u[a], u[b], ql, sl); });
+    HspTree tp(ql + 1, sl + 1), tm(ql + 1, sl + 1);
+    for (uint32_t i : ord) {
+      const UngRec& x = u[i];
+      const uint32_t css = x.minus ? sl - x.s_end : x.s_start, cse = x.minus ? sl - x.s_start : x.s_end;
+      HspTree& t = x.minus ? tm : tp;
+      if (t.covers(x.q_start, x.q_end, css, cse, x.score, !x.minus)) continue;
+      const PreHit& p = survivors[s.surv[i]];
+      const uint32_t nss = x.minus ? sl - p.s_end : p.s_start, nse = x.minus ? sl - p.s_start : p.s_end;
+      t.add(p.q_start, p.q_end, nss, nse, p.score, !x.minus);
+      s.pre.push_back(PreRec{p.tq, p.ts, p.q_start, p.q_end, p.s_start, p.s_end, p.score, x.minus});
     }
-    c->stats[2] = n_prelim;
-  }
-  parallel_for(items.size(), [&](size_t k) {
-    ItemState& s = items[k];
-    purge_prelims(s.pre, (uint32_t)c->subj[s.item % nsub].plus.size() - 2);
+    purge_prelims(s.pre, sl);
   });
+  survivors.clear();
+  survivors.shrink_to_fit();
   c->stats[10] = t2.us();
   // ---- Phase 2b: traceback
   Timer t3;
@@ -764,9 +811,12 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
   c->stats[0] = hc[C_SEED_EVALS];
   c->stats[1] = hc[C_UNG_HITS];
   c->stats[7] = hc[C_UNG_STEPS];
-  c->stats[4] = hc[C_DP_CELLS];  // all DP cells (score passes of both phases)
+  c->stats[2] = hc[C_PRELIM_DPS];
+  c->stats[4] = hc[C_PRELIM_CELLS] + hc[C_DP_CELLS];  // DP cells of both phases (score passes)
   c->stats[6] = c->hsps.size();
   c->stats[12] = total.us();
+  c->stats[16] = hc[C_SEEDS];
+  for (int k = 0; k < 6; ++k) c->stats[17 + k] = hc[C_P1_STAGE + k];
 }
 
 }  // namespace rmb
@@ -934,9 +984,9 @@ int rmb_ctx_destroy(rmb_ctx* c) {
   (void)hipSetDevice(c->device);
   for (DevBuf* b : {&c->d_splus, &c->d_src, &c->d_salign, &c->d_sdesc, &c->b_qfwd, &c->b_qrc, &c->b_recs, &c->b_elem,
                     &c->b_keys, &c->b_keys2, &c->b_vals, &c->b_vals2, &c->b_counts, &c->b_cells, &c->b_sort_tmp,
-                    &c->b_hits, &c->b_hitcount, &c->b_spill, &c->b_queue, &c->b_counters, &c->b_err, &c->b_tasks,
+                    &c->b_hits, &c->b_hitcount, &c->b_spill, &c->b_hitres, &c->b_queue, &c->b_counters, &c->b_err, &c->b_tasks,
                     &c->b_dpres, &c->b_ring, &c->b_ring_big, &c->b_tbtasks, &c->b_tbres, &c->b_flat, &c->b_rows,
-                    &c->b_ops})
+                    &c->b_ops, &c->b_pre, &c->b_redo, &c->b_precount})
     b->release();
   if (c->d_mat) (void)hipFree(c->d_mat);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1030,7 +1080,7 @@ int rmb_results_copy(const rmb_ctx* c, rmb_hsp* hsps, rmb_op* ops) {
 
 int rmb_stats(const rmb_ctx* c, uint64_t* out, int n) {
   if (!c || !out) return RMB_ERR_ARG;
-  for (int i = 0; i < n && i < 16; ++i) out[i] = c->stats[i];
+  for (int i = 0; i < n && i < RMB_NSTATS; ++i) out[i] = c->stats[i];
   return RMB_OK;
 }
 
@@ -1051,7 +1101,7 @@ int rmb_tap_ungapped(rmb_ctx* c, const uint8_t* query, uint32_t q_len, uint32_t 
     unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
     hip_check(hipMemset(ctr, 0, C_NCOUNTERS * sizeof(unsigned long long)), "memset counters");
     const int lw = lut_width_for(c->P.word_size, 2ull * q_len);
-    if (q_len >= (uint32_t)c->P.word_size) phase1_group(c, {0u}, recs, lw, d_qfwd, d_qrc, hits);
+    if (q_len >= (uint32_t)c->P.word_size) phase1_group(c, {0u}, recs, lw, d_qfwd, d_qrc, &hits, nullptr);
     std::vector<UngRec> v;
     for (auto& h : hits)
       if (h.item % c->subj.size() == subject)

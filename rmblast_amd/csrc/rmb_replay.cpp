@@ -20,19 +20,22 @@ void script_push(Script& s, uint8_t kind, uint32_t len) {
 static inline uint32_t sub_sat(uint32_t a, uint32_t b) { return a > b ? a - b : 0; }
 
 // ------------------------------------------------------------ ordering ------
-void order_ungapped(std::vector<UngRec>& v, uint32_t q_len, uint32_t s_len) {
+static inline std::tuple<int32_t, uint32_t, uint32_t, uint64_t, uint32_t> ung_key(const UngRec& a, uint32_t q_len,
+                                                                                  uint32_t s_len) {
   // score DESC, fwd s_start ASC, length DESC, combined-query key ASC, seed key ASC
-  struct Key { int32_t neg_score; uint32_t ss; uint32_t neg_len; uint64_t qk; uint32_t sk; };
-  auto key = [&](const UngRec& a) {
-    Key k;
-    k.neg_score = -a.score;
-    k.ss = a.minus ? s_len - a.s_end : a.s_start;
-    k.neg_len = ~(a.s_end - a.s_start);
-    k.qk = a.minus ? ((uint64_t)q_len + 1) + (q_len > a.q_end ? (uint64_t)(q_len - a.q_end) : 0) : a.q_start;
-    k.sk = a.minus ? s_len - a.s_seed : a.s_seed;
-    return std::make_tuple(k.neg_score, k.ss, k.neg_len, k.qk, k.sk);
-  };
-  std::sort(v.begin(), v.end(), [&](const UngRec& a, const UngRec& b) { return key(a) < key(b); });
+  const uint32_t ss = a.minus ? s_len - a.s_end : a.s_start;
+  const uint64_t qk = a.minus ? ((uint64_t)q_len + 1) + (q_len > a.q_end ? (uint64_t)(q_len - a.q_end) : 0)
+                              : a.q_start;
+  const uint32_t sk = a.minus ? s_len - a.s_seed : a.s_seed;
+  return std::make_tuple(-a.score, ss, ~(a.s_end - a.s_start), qk, sk);
+}
+
+bool ung_before(const UngRec& a, const UngRec& b, uint32_t q_len, uint32_t s_len) {
+  return ung_key(a, q_len, s_len) < ung_key(b, q_len, s_len);
+}
+
+void order_ungapped(std::vector<UngRec>& v, uint32_t q_len, uint32_t s_len) {
+  std::sort(v.begin(), v.end(), [&](const UngRec& a, const UngRec& b) { return ung_before(a, b, q_len, s_len); });
 }
 
 void pivot_of(const UngRec& u, uint32_t qa_len, uint32_t sa_len, uint32_t lw, uint32_t& pq, uint32_t& ps,

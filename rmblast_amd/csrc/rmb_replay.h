@@ -39,6 +39,7 @@ struct FinalRec {
 };
 
 // NCBI ungapped-hit order (engine.rs:1280-1309).
+bool ung_before(const UngRec& a, const UngRec& b, uint32_t q_len, uint32_t s_len);
 void order_ungapped(std::vector<UngRec>& v, uint32_t q_len, uint32_t s_len);
 // Phase-2a pivot and traceback seed (engine.rs:1850-1895).
 void pivot_of(const UngRec& u, uint32_t qa_len, uint32_t sa_len, uint32_t lw, uint32_t& pq, uint32_t& ps,
