@@ -108,11 +108,11 @@ def main():
     # DPs over the time inside the gapped kernels (HIP events on their stream)
     g_ns = st["t_gapped_kernel_ns"]
     gcups = st["dp_cells"] / g_ns if g_ns > 0 else 0.0
-    ticks = [st[k] for k in ("p1_ticks_scan", "p1_ticks_fill", "p1_ticks_extend", "p1_ticks_bucket",
+    ticks = [st[k] for k in ("p1_ticks_scan", "p1_ticks_reserve", "p1_ticks_extend", "p1_ticks_bucket",
                              "p1_ticks_replay", "p1_ticks_emit")]
     tick_sum = max(sum(ticks), 1)
-    p1_stages = {n: round(t / tick_sum, 4) for n, t in zip(("scan", "fill", "extend", "bucket", "replay", "emit"),
-                                                          ticks)}
+    p1_stages = {n: round(t / tick_sum, 4)
+                 for n, t in zip(("scan", "reserve", "extend", "bucket", "replay", "emit"), ticks)}
 
     cpu = None
     if not a.no_cpu:

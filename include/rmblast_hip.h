@@ -129,8 +129,9 @@ int rmb_results_copy(const rmb_ctx* ctx, rmb_hsp* hsps, rmb_op* ops);
  * nanoseconds (HIP events on the stream it is launched on), [14] time inside
  * the gapped kernels (preliminary, score and traceback DP) in nanoseconds,
  * [15] launches of the seeding kernel, [16] seeds staged by the seeding
- * kernel, [17..22] seeding-kernel stage time (scan, seed fill, extension,
- * bucket sort, hash replay, emit) in 100 MHz ticks summed over workgroups,
+ * kernel, [17..22] seeding-kernel stage time (scan, output-slot reservation,
+ * seed fill + extension, bucket sort, hash replay, emit) in 100 MHz ticks
+ * summed over workgroups,
  * [23] reserved.  Entries past `n` are not written; RMB_NSTATS is the full
  * length. */
 #define RMB_NSTATS 24

@@ -5,11 +5,11 @@
 namespace rmb {
 
 // Phase-1 launch geometry.  One workgroup owns one (record, subject) work item
-// at a time; SEEDCAP seeds are staged in LDS per pass; the 512 diagonal-hash
+// at a time; kSeedCap seeds are staged in LDS per pass; the 512 diagonal-hash
 // buckets are owned two per thread.
 constexpr int kP1Threads = 256;
 constexpr int kSeedCap = 768;
-constexpr int kHashLds = 2;      // cells per bucket held in LDS
+constexpr int kHashLds = 2;      // cells per bucket held in the owner thread's registers
 constexpr int kHashGlobal = 254; // further cells per bucket in the workgroup's global spill area
 constexpr int kBuckets = 512;
 constexpr int kXcds = 8;         // MI355X: 8 XCDs, workgroups dispatched round-robin over them
@@ -33,6 +33,8 @@ struct P1Params {
   uint64_t n_items;
   uint64_t out_cap;
 };
+
+constexpr uint32_t kNoItem = 0xFFFFFFFFu;  // UngHit::item of an output slot reserved but left unused
 
 struct UngHit {         // an ungapped hit at or above the cutoff, record frame
   uint32_t item;        // rec * n_subj + subj
@@ -97,5 +99,10 @@ struct TbTask {         // traceback pass: a DpTask plus its storage
 };
 
 struct TbResult { uint32_t n_ops; uint32_t flags; };
+
+struct PackTask {       // copy n ops of one traceback from the op pool into the packed list
+  uint64_t src, dst;
+  uint32_t n, pad;
+};
 
 }  // namespace rmb

@@ -149,64 +149,81 @@ __device__ __forceinline__ bool ungapped_short(const uint8_t* __restrict__ qpool
   const uint4 ta = funnel16(s0, s1, os), tb = funnel16(s1, s2, os);
   const uint4 ra = funnel16(q2, q3, oq), rb = funnel16(q3, q4, oq);
   const uint4 ua = funnel16(s2, s3, os), ub = funnel16(s3, s4, os);
+  // matrix index bytes (q << 4 | s), four per word
+  uint32_t li[8], ri[8];
+  {
+    const uint32_t lq[8] = {la.x, la.y, la.z, la.w, lb.x, lb.y, lb.z, lb.w};
+    const uint32_t ls[8] = {ta.x, ta.y, ta.z, ta.w, tb.x, tb.y, tb.z, tb.w};
+    const uint32_t rq[8] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y, rb.z, rb.w};
+    const uint32_t rs[8] = {ua.x, ua.y, ua.z, ua.w, ub.x, ub.y, ub.z, ub.w};
+#pragma unroll
+    for (int w = 0; w < 8; ++w) {
+      li[w] = ((lq[w] << 4) & 0xF0F0F0F0u) | (ls[w] & 0x0F0F0F0Fu);
+      ri[w] = ((rq[w] << 4) & 0xF0F0F0F0u) | (rs[w] & 0x0F0F0F0Fu);
+    }
+  }
+  // Steps run in groups of eight: the group's matrix lookups are issued
+  // together, the X-drop state advances branch-free under a per-lane `run`
+  // flag, and a group starts only while some lane of the wave still runs.
   const int32_t x = -xdrop;
   int32_t score = 0, sum = 0, q_beg = q_off;
   uint32_t steps = 0;
   {
-    // left, consumed from qa-1 downwards
-    const uint32_t lq[8] = {la.x, la.y, la.z, la.w, lb.x, lb.y, lb.z, lb.w};
-    const uint32_t ls[8] = {ta.x, ta.y, ta.z, ta.w, tb.x, tb.y, tb.z, tb.w};
+    // left: step k reads query offset q_off - 1 - k (window byte 31 - k)
     bool run = true;
 #pragma unroll
-    for (int w = 7; w >= 0; --w) {
+    for (int g = 0; g < kShortReach / 8; ++g) {
+      if (!__any(run)) break;
+      int32_t v[8];
 #pragma unroll
-      for (int b = 3; b >= 0; --b) {
-        if (run) {
-          const uint32_t qb = (lq[w] >> (8 * b)) & 0xFFu, sb = (ls[w] >> (8 * b)) & 0xFFu;
-          const int32_t v = smat[(qb << 4) | sb];
-          ++steps;
-          sum += v;
-          if (sum > 0) {
-            q_beg = q_off - (kShortReach - (4 * w + b));
-            score += sum;
-            sum = 0;
-          } else if (sum < x) {
-            run = false;
-            if (v == kSentScore) --steps;
-          }
-        }
+      for (int j = 0; j < 8; ++j) {
+        const int p = kShortReach - 1 - (8 * g + j);
+        v[j] = smat[(li[p >> 2] >> (8 * (p & 3))) & 0xFFu];
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int32_t ns = sum + v[j];
+        const bool up = run && ns > 0;
+        const bool stop = run && ns < x;
+        score = up ? score + ns : score;
+        q_beg = up ? q_off - 1 - (8 * g + j) : q_beg;
+        sum = run ? (ns > 0 ? 0 : ns) : sum;
+        steps += run ? 1u : 0u;
+        run = run && !stop;
       }
     }
     if (run) return false;
+    if (sum <= kSentScore / 2) --steps;  // ended on a sentinel: that step is not counted
   }
   int32_t q_end = q_off, xc = x;
   sum = 0;
   {
-    const uint32_t rq[8] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y, rb.z, rb.w};
-    const uint32_t rs[8] = {ua.x, ua.y, ua.z, ua.w, ub.x, ub.y, ub.z, ub.w};
+    // right: step k reads query offset q_off + k (window byte k)
     bool run = true;
 #pragma unroll
-    for (int w = 0; w < 8; ++w) {
+    for (int g = 0; g < kShortReach / 8; ++g) {
+      if (!__any(run)) break;
+      int32_t v[8];
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        if (run) {
-          const uint32_t qb = (rq[w] >> (8 * b)) & 0xFFu, sb = (rs[w] >> (8 * b)) & 0xFFu;
-          const int32_t v = smat[(qb << 4) | sb];
-          ++steps;
-          sum += v;
-          if (sum > 0) {
-            q_end = q_off + 4 * w + b + 1;
-            score += sum;
-            xc = (-score > x) ? -score : x;
-            sum = 0;
-          } else if (sum < xc) {
-            run = false;
-            if (v == kSentScore) --steps;
-          }
-        }
+      for (int j = 0; j < 8; ++j) {
+        const int p = 8 * g + j;
+        v[j] = smat[(ri[p >> 2] >> (8 * (p & 3))) & 0xFFu];
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int32_t ns = sum + v[j];
+        const bool up = run && ns > 0;
+        const bool stop = run && ns < xc;
+        score = up ? score + ns : score;
+        q_end = up ? q_off + 8 * g + j + 1 : q_end;
+        xc = up ? ((-score > x) ? -score : x) : xc;
+        sum = run ? (ns > 0 ? 0 : ns) : sum;
+        steps += run ? 1u : 0u;
+        run = run && !stop;
       }
     }
     if (run) return false;
+    if (sum <= kSentScore / 2) --steps;
   }
   out = UngOut{score, (uint32_t)q_beg, (uint32_t)q_end, steps};
   return true;
@@ -263,7 +280,9 @@ __global__ void k_lut_keys(const RecDesc* __restrict__ recs, uint32_t n_recs, co
 // the reference's 512-bucket diagonal hash exactly: buckets are independent,
 // so each thread replays its two buckets' seeds in scan order.  Seeds the
 // replay keeps and that reach the cutoff are emitted.
-enum SeedFlag : uint8_t { F_VALID = 1, F_HIT = 2, F_EMIT = 4 };
+enum SeedFlag : uint8_t { F_VALID = 1, F_HIT = 2, F_EMIT = 4, F_MINUS = 8 };
+constexpr uint32_t kHitBuf = 255;    // hit extension results kept in LDS per pass; slot 255 = in global memory
+constexpr uint32_t kOutChunk = 256;  // output slots a workgroup reserves at a time
 
 __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
     P1Params P, const RecDesc* __restrict__ recs, const SubjDesc* __restrict__ subjs,
@@ -274,23 +293,22 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
     unsigned long long* __restrict__ queue, unsigned long long* __restrict__ counters, uint32_t* __restrict__ err) {
   __shared__ int mat[256];
   __shared__ uint32_t s_cum[kP1Threads], s_cst[kP1Threads], wsum[4];
-  __shared__ uint32_t seed_s[kSeedCap], seed_q[kSeedCap];
-  __shared__ int32_t sd_diag[kSeedCap];
-  __shared__ uint32_t sd_chk[kSeedCap], sd_level[kSeedCap];
-  __shared__ uint16_t sd_steps[kSeedCap], blist[kSeedCap];
-  __shared__ uint8_t sd_flag[kSeedCap];
+  __shared__ uint4 sd[kSeedCap];  // per seed: {diagonal, hash check offset, level, steps | flags << 16 | hit slot << 24}
+  __shared__ uint16_t blist[kSeedCap];
   __shared__ uint32_t bcnt[kBuckets], bst[kBuckets];
-  __shared__ int32_t hdiag[kBuckets * kHashLds];
-  __shared__ uint32_t hlev[kBuckets * kHashLds];
-  __shared__ uint16_t hcnt[kBuckets];
-  __shared__ unsigned long long s_item;
-  __shared__ uint32_t s_next_slot, s_next_k;
+  __shared__ uint4 hbuf[kHitBuf];  // per hit: {q_beg, q_end, score, extension-frame query offset}
+  __shared__ unsigned long long s_item, s_out_next, s_pass_a, s_pass_b;
+  __shared__ uint32_t s_out_left, s_pass_an, s_nhit, s_nemit, s_emit_i;
 
   const int tid = threadIdx.x;
   const uint32_t xcd = blockIdx.x % kXcds;
   for (int i = tid; i < 256; i += kP1Threads) mat[i] = ((i >> 4) == 15 || (i & 15) == 15) ? kSentScore : gmat[i];
+  if (tid == 0) {
+    s_out_next = 0;
+    s_out_left = 0;
+  }
   uint2* my_spill = spill + (size_t)blockIdx.x * kBuckets * kHashGlobal;
-  uint4* my_hitres = hitres + (size_t)blockIdx.x * kSeedCap;  // extension results of this pass's hits
+  uint4* my_hitres = hitres + (size_t)blockIdx.x * kSeedCap;  // hits beyond the LDS buffer
   const int32_t ws = P.word_size, lw = P.lut_width, step = P.scan_step;
   const int32_t ext_needed = ws - lw;
   const int32_t window = 1 - ws;
@@ -322,7 +340,11 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
         if (t < n_x) it = (unsigned long long)(x + kXcds * (t / P.n_subj)) * P.n_subj + t % P.n_subj;
       }
       s_item = it;
+      s_nhit = 0;
+      s_nemit = 0;
+      s_emit_i = 0;
     }
+    for (int b = tid; b < kBuckets; b += kP1Threads) bcnt[b] = 0;
     __syncthreads();
     const unsigned long long item = s_item;
     if (item == ~0ull) break;
@@ -338,69 +360,70 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
     const int32_t end_range = s_len - lw;
     const uint32_t n_slots = (uint32_t)(end_range / step) + 1;
     const int32_t c1 = q_len + 1;
-    for (int b = tid; b < kBuckets; b += kP1Threads) hcnt[b] = 0;
-    uint32_t slot0 = 0, kcur = 0;
-    __syncthreads();
-    while (slot0 < n_slots) {
-      // ---- A: hit counts of the next kP1Threads scan positions
-      uint32_t cnt = 0, cst = 0;
-      const uint32_t slot = slot0 + tid;
-      if (slot < n_slots) {
-        const int32_t s = (int32_t)slot * step;
-        uint32_t idx = 0;
-        for (int k = 0; k < lw; ++k) idx = (idx << 2) | (SP[1 + s + k] & 3);
-        const uint32_t a = CS[idx], b = CS[idx + 1];
-        cst = a;
-        cnt = b - a;
+    // diagonal hash: thread tid owns buckets tid and tid + kP1Threads for the
+    // whole item; each bucket's first two cells live in these registers, the
+    // rest in the workgroup's spill area
+    uint32_t hc[2] = {0u, 0u}, hl0[2] = {0u, 0u}, hl1[2] = {0u, 0u};
+    int32_t hd0[2] = {0, 0}, hd1[2] = {0, 0};
+    // The scan runs over windows of kP1Threads subject positions; a window's
+    // seeds (subject offset ascending, then cell order) are taken kSeedCap
+    // per pass, so each window is scanned once.
+    uint32_t w0 = 0, wk = 0, wtotal = 0;
+    bool scanned = false;
+    for (;;) {
+      if (!scanned) {
+        if (w0 >= n_slots) break;
+        // ---- A: hit counts of the window's scan positions
+        uint32_t cnt = 0, cst = 0;
+        const uint32_t slot = w0 + tid;
+        if (slot < n_slots) {
+          const int32_t s = (int32_t)slot * step;
+          uint32_t idx = 0;
+          for (int k = 0; k < lw; ++k) idx = (idx << 2) | (SP[1 + s + k] & 3);
+          const uint32_t a = CS[idx], b = CS[idx + 1];
+          cst = a;
+          cnt = b - a;
+        }
+        s_cst[tid] = cst;
+        const uint32_t inc = block_incl_scan256(cnt, wsum);
+        s_cum[tid] = inc;
+        __syncthreads();
+        wtotal = s_cum[kP1Threads - 1];
+        wk = 0;
+        scanned = true;
+        tick(0);
+        if (wtotal == 0) {
+          w0 += kP1Threads;
+          scanned = false;
+          continue;
+        }
       }
-      if (tid == 0) { cst += kcur; cnt -= kcur; }
-      s_cst[tid] = cst;
-      const uint32_t inc = block_incl_scan256(cnt, wsum);
-      s_cum[tid] = inc;
-      __syncthreads();
-      tick(0);
-      const uint32_t total = s_cum[kP1Threads - 1];
-      const uint32_t take = total < (uint32_t)kSeedCap ? total : (uint32_t)kSeedCap;
+      const uint32_t take = (wtotal - wk) < (uint32_t)kSeedCap ? (wtotal - wk) : (uint32_t)kSeedCap;
       n_seeds += tid == 0 ? take : 0;
-      // ---- B: materialise seeds in scan order
+      // ---- B+C: materialise each seed, exact word extension, diagonal,
+      // speculative ungapped extension (the lane that fills a seed extends it)
       for (uint32_t i = tid; i < take; i += kP1Threads) {
+        const uint32_t g = wk + i;  // seed index within the window
         int lo = 0, hi = kP1Threads - 1;
         while (lo < hi) {
-          int mid = (lo + hi) >> 1;
-          if (s_cum[mid] > i) hi = mid; else lo = mid + 1;
+          const int mid = (lo + hi) >> 1;
+          if (s_cum[mid] > g) hi = mid; else lo = mid + 1;
         }
         const uint32_t prev = lo ? s_cum[lo - 1] : 0;
-        seed_s[i] = (slot0 + lo) * (uint32_t)step;
-        seed_q[i] = entries[s_cst[lo] + (i - prev)];
-      }
-      if (tid == 0) {
-        if (total <= (uint32_t)kSeedCap) {
-          s_next_slot = slot0 + kP1Threads;
-          s_next_k = 0;
-        } else {
-          int lo = 0, hi = kP1Threads - 1;
-          while (lo < hi) {
-            int mid = (lo + hi) >> 1;
-            if (s_cum[mid] > take) hi = mid; else lo = mid + 1;
-          }
-          const uint32_t prev = lo ? s_cum[lo - 1] : 0;
-          s_next_slot = slot0 + lo;
-          s_next_k = (take - prev) + (lo == 0 ? kcur : 0);
-        }
-      }
-      for (int b = tid; b < kBuckets; b += kP1Threads) bcnt[b] = 0;
-      __syncthreads();
-      tick(1);
-      // ---- C: exact word extension, diagonal, speculative ungapped extension
-      for (uint32_t i = tid; i < take; i += kP1Threads) {
-        const int32_t s_fwd = (int32_t)seed_s[i];
-        const int32_t q0c = (int32_t)seed_q[i];
-        uint8_t flag = 0;
+        const int32_t s_fwd = (int32_t)((w0 + (uint32_t)lo) * (uint32_t)step);
+        const int32_t q0c = (int32_t)entries[s_cst[lo] + (g - prev)];
+        uint32_t flag = 0;
         int32_t diag = 0;
-        uint32_t chk = 0, level = 0, steps = 0;
+        uint32_t chk = 0, level = 0, steps = 0, hslot = kHitBuf;
+        // Strand-specific part: exact word extension, diagonal, hash check
+        // offset and the extension frame (eq, es).  Plus strand: forward query
+        // and subject; minus strand: reverse-complement query against the
+        // forward subject, anchored at the word's last base.
+        bool ok;
+        int32_t eq = 0, es = 0;
         if (q0c < c1) {
           int32_t q_off = q0c, s_off = s_fwd;
-          bool ok = true;
+          ok = true;
           if (ext_needed > 0) {  // engine.rs:1130-1153
             int32_t el = 0, er = 0;
             while (el < ext_needed) {
@@ -421,25 +444,15 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
             q_off = q0c - el;
             s_off = s_fwd - el;
           }
-          if (ok) {
-            diag = (int32_t)((uint32_t)s_off - (uint32_t)q_off);
-            chk = (uint32_t)s_off;
-            UngOut u;
-            if (!ungapped_short(qfwd, (int64_t)R.q_base + 1 + q_off, splus, (int64_t)S.s_base + 1 + s_off, q_off, mat,
-                                P.xdrop_ungap, u))
-              u = ungapped_ext(qfwd, R.q_base, q_len + 1, splus, S.s_base, s_len + 1, q_off, s_off, mat,
-                               P.xdrop_ungap);
-            const bool hit = u.score >= P.cutoff;
-            if (hit) my_hitres[i] = make_uint4(u.q_beg, u.q_end, (uint32_t)u.score, 0u);
-            level = hit ? (uint32_t)s_off + (u.q_end - (uint32_t)q_off) : (uint32_t)(s_off + ws);
-            steps = u.steps;
-            flag = F_VALID | (hit ? F_HIT : 0);
-          }
+          diag = (int32_t)((uint32_t)s_off - (uint32_t)q_off);
+          chk = (uint32_t)s_off;
+          eq = q_off;
+          es = s_off;
         } else {
           const int32_t rcq = q0c - c1;
           const int32_t q0 = q_len - rcq - lw, s0 = s_len - s_fwd - lw;
           int32_t q_off = q0, s_off = s0;
-          bool ok = q0 >= 0 && s0 >= 0;
+          ok = q0 >= 0 && s0 >= 0;
           if (ok && ext_needed > 0) {  // engine.rs:1201-1221
             int32_t el = 0, er = 0;
             while (er < ext_needed) {
@@ -460,31 +473,40 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
             q_off = q0 + er;
             s_off = s0 + er;
           }
-          if (ok) {
-            diag = (int32_t)((uint32_t)s_fwd - (uint32_t)q0c);
-            const uint32_t s_fwd_eff = (uint32_t)(s_len - s_off - lw);
-            chk = s_fwd_eff;
-            const int32_t qn = q_len - 1 - q_off, sn = s_len - 1 - s_off;
-            UngOut u;
-            if (!ungapped_short(qrc, (int64_t)R.q_base + 1 + qn, splus, (int64_t)S.s_base + 1 + sn, qn, mat,
-                                P.xdrop_ungap, u))
-              u = ungapped_ext(qrc, R.q_base, q_len + 1, splus, S.s_base, s_len + 1, qn, sn, mat, P.xdrop_ungap);
-            const bool hit = u.score >= P.cutoff;
-            if (hit) my_hitres[i] = make_uint4(u.q_beg, u.q_end, (uint32_t)u.score, 0u);
-            level = hit ? (uint32_t)sn + (u.q_end - (uint32_t)qn) : s_fwd_eff + (uint32_t)ws;
-            steps = u.steps;
-            flag = F_VALID | (hit ? F_HIT : 0);
-          }
+          diag = (int32_t)((uint32_t)s_fwd - (uint32_t)q0c);
+          chk = (uint32_t)(s_len - s_off - lw);
+          eq = q_len - 1 - q_off;
+          es = s_len - 1 - s_off;
         }
-        sd_diag[i] = diag;
-        sd_chk[i] = chk;
-        sd_level[i] = level;
-        sd_steps[i] = (uint16_t)(steps < 65535u ? steps : 65535u);
-        sd_flag[i] = flag;
-        if (flag) atomicAdd(&bcnt[(uint32_t)diag & (kBuckets - 1)], 1u);
+        if (ok) {
+          // one extension call for both strands, so a wave holding seeds of
+          // both runs it once
+          const uint8_t* qpool = q0c < c1 ? qfwd : qrc;
+          UngOut u;
+          if (!ungapped_short(qpool, (int64_t)R.q_base + 1 + eq, splus, (int64_t)S.s_base + 1 + es, eq, mat,
+                              P.xdrop_ungap, u))
+            u = ungapped_ext(qpool, R.q_base, q_len + 1, splus, S.s_base, s_len + 1, eq, es, mat, P.xdrop_ungap);
+          const bool hit = u.score >= P.cutoff;
+          if (hit) {
+            const uint4 hv = make_uint4(u.q_beg, u.q_end, (uint32_t)u.score, (uint32_t)eq);
+            hslot = atomicAdd(&s_nhit, 1u);
+            if (hslot < kHitBuf) hbuf[hslot] = hv;
+            else { hslot = kHitBuf; my_hitres[i] = hv; }
+          }
+          level = hit ? (uint32_t)es + (u.q_end - (uint32_t)eq) : chk + (uint32_t)ws;
+          steps = u.steps;
+          flag = F_VALID | (hit ? F_HIT : 0) | (q0c < c1 ? 0 : F_MINUS);
+          atomicAdd(&bcnt[(uint32_t)diag & (kBuckets - 1)], 1u);
+        } else {
+          diag = 0;
+          chk = 0;
+        }
+        sd[i] = make_uint4((uint32_t)diag, chk, level,
+                           (steps < 65535u ? steps : 65535u) | (flag << 16) | (hslot << 24));
       }
       __syncthreads();
       tick(2);
+      if (tid == 0) s_nhit = 0;  // every slot of this pass is taken; F reads hbuf, not the count
       // ---- D: bucket lists (counting sort; each owner re-sorts by seed index)
       {
         const uint32_t v0 = bcnt[2 * tid], v1 = bcnt[2 * tid + 1];
@@ -493,12 +515,16 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
         bst[2 * tid + 1] = incl - v1;
       }
       __syncthreads();
-      for (uint32_t i = tid; i < take; i += kP1Threads)
-        if (sd_flag[i]) blist[atomicAdd(&bst[(uint32_t)sd_diag[i] & (kBuckets - 1)], 1u)] = (uint16_t)i;
+      for (uint32_t i = tid; i < take; i += kP1Threads) {
+        const uint4 e = sd[i];
+        if ((e.w >> 16) & F_VALID) blist[atomicAdd(&bst[e.x & (kBuckets - 1)], 1u)] = (uint16_t)i;
+      }
       __syncthreads();
       tick(3);
       // ---- E: exact replay of each bucket (diag_hash.rs:57-111)
-      for (int bb = tid; bb < kBuckets; bb += kP1Threads) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int bb = tid + h * kP1Threads;
         const uint32_t nb = bcnt[bb];
         if (!nb) continue;
         const uint32_t st = bst[bb] - nb;
@@ -508,103 +534,139 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
           while (y > st && blist[y - 1] > v) { blist[y] = blist[y - 1]; --y; }
           blist[y] = v;
         }
-        uint32_t cnt_b = hcnt[bb];
-        int32_t* HD = hdiag + bb * kHashLds;
-        uint32_t* HL = hlev + bb * kHashLds;
+        uint32_t cnt_b = hc[h], l0 = hl0[h], l1 = hl1[h];
+        int32_t d0 = hd0[h], d1 = hd1[h];
         uint2* GS = my_spill + (size_t)bb * kHashGlobal;
         for (uint32_t x = st; x < st + nb; ++x) {
           const uint32_t i = blist[x];
-          const int32_t d = sd_diag[i];
-          const uint32_t sc = sd_chk[i];
+          const uint4 e = sd[i];
+          const int32_t d = (int32_t)e.x;
+          const uint32_t sc = e.y;
           // get(): newest cell first
           uint32_t lev = 0;
-          for (int32_t k = (int32_t)cnt_b - 1; k >= 0; --k) {
-            int32_t cd;
-            uint32_t cl;
-            if (k < kHashLds) { cd = HD[k]; cl = HL[k]; }
-            else { const uint2 g = GS[k - kHashLds]; cd = (int32_t)g.x; cl = g.y; }
-            if (cd == d) { lev = cl; break; }
+          bool found = false;
+          for (int32_t k = (int32_t)cnt_b - 1; k >= kHashLds; --k) {
+            const uint2 gv = GS[k - kHashLds];
+            if ((int32_t)gv.x == d) { lev = gv.y; found = true; break; }
           }
+          if (!found && cnt_b > 1 && d1 == d) { lev = l1; found = true; }
+          if (!found && cnt_b > 0 && d0 == d) lev = l0;
           if (sc < lev) continue;
           ++n_evals;
-          n_steps += sd_steps[i];
-          if (sd_flag[i] & F_HIT) sd_flag[i] |= F_EMIT;
+          n_steps += e.w & 0xFFFFu;
+          if ((e.w >> 16) & F_HIT) {
+            sd[i].w = e.w | ((uint32_t)F_EMIT << 16);
+            atomicAdd(&s_nemit, 1u);
+          }
           // insert(): same diagonal, else first stale cell, else a new head cell
-          const uint32_t L = sd_level[i];
+          const uint32_t L = e.z;
           bool done = false;
-          for (int32_t k = (int32_t)cnt_b - 1; k >= 0 && !done; --k) {
-            int32_t cd;
-            uint32_t cl;
-            if (k < kHashLds) { cd = HD[k]; cl = HL[k]; }
-            else { const uint2 g = GS[k - kHashLds]; cd = (int32_t)g.x; cl = g.y; }
-            if (cd == d || (int32_t)(sc - cl) > window) {
-              if (k < kHashLds) { HD[k] = d; HL[k] = L; }
-              else GS[k - kHashLds] = make_uint2((uint32_t)d, L);
+          for (int32_t k = (int32_t)cnt_b - 1; k >= kHashLds && !done; --k) {
+            const uint2 gv = GS[k - kHashLds];
+            if ((int32_t)gv.x == d || (int32_t)(sc - gv.y) > window) {
+              GS[k - kHashLds] = make_uint2((uint32_t)d, L);
               done = true;
             }
           }
+          if (!done && cnt_b > 1 && (d1 == d || (int32_t)(sc - l1) > window)) { d1 = d; l1 = L; done = true; }
+          if (!done && cnt_b > 0 && (d0 == d || (int32_t)(sc - l0) > window)) { d0 = d; l0 = L; done = true; }
           if (!done) {
             if (cnt_b >= (uint32_t)(kHashLds + kHashGlobal)) {
               atomicOr(err, (uint32_t)E_HASH_FULL);
             } else {
-              if (cnt_b < (uint32_t)kHashLds) { HD[cnt_b] = d; HL[cnt_b] = L; }
+              if (cnt_b == 0) { d0 = d; l0 = L; }
+              else if (cnt_b == 1) { d1 = d; l1 = L; }
               else GS[cnt_b - kHashLds] = make_uint2((uint32_t)d, L);
               ++cnt_b;
             }
           }
         }
-        hcnt[bb] = (uint16_t)cnt_b;
+        hc[h] = cnt_b; hl0[h] = l0; hl1[h] = l1; hd0[h] = d0; hd1[h] = d1;
       }
       __syncthreads();
       tick(4);
-      // ---- F: emit kept hits in the record frame
-      for (uint32_t i = tid; i < take; i += kP1Threads) {
-        if (!(sd_flag[i] & F_EMIT)) continue;
-        const int32_t s_fwd = (int32_t)seed_s[i];
-        const int32_t q0c = (int32_t)seed_q[i];
-        UngHit h;
-        h.item = (uint32_t)item;
-        if (q0c < c1) {
-          const int32_t q_off = q0c - ((int32_t)seed_s[i] - (int32_t)sd_chk[i]);
-          const int32_t s_off = (int32_t)sd_chk[i];
-          const uint4 hv = my_hitres[i];  // {q_beg, q_end, score} from stage C
-          const UngOut u{(int32_t)hv.z, hv.x, hv.y, 0u};
-          h.q_seed = (uint32_t)q_off;
-          h.s_seed = (uint32_t)s_off;
-          h.q_start = u.q_beg;
-          h.q_end = u.q_end;
-          h.s_start = (uint32_t)(s_off - (q_off - (int32_t)u.q_beg));
-          h.s_end = (uint32_t)s_off + (u.q_end - (uint32_t)q_off);
-          h.score = u.score;
-          h.strand = 0;
-        } else {
-          const uint32_t s_fwd_eff = sd_chk[i];
-          const int32_t s_off = s_len - (int32_t)s_fwd_eff - lw;
-          const int32_t q_off = (q_len - (q0c - c1) - lw) + (s_off - (s_len - s_fwd - lw));
-          const int32_t qn = q_len - 1 - q_off, sn = s_len - 1 - s_off;
-          const uint4 hv = my_hitres[i];
-          const UngOut u{(int32_t)hv.z, hv.x, hv.y, 0u};
-          const uint32_t rs0 = (uint32_t)(sn - (qn - (int32_t)u.q_beg));
-          const uint32_t rs1 = (uint32_t)sn + (u.q_end - (uint32_t)qn);
-          h.q_seed = (uint32_t)q_off;
-          h.s_seed = (uint32_t)s_off;
-          h.q_start = (uint32_t)q_len - u.q_end;
-          h.q_end = (uint32_t)q_len - u.q_beg;
-          h.s_start = (uint32_t)s_len - rs1;
-          h.s_end = (uint32_t)s_len - rs0;
-          h.score = u.score;
-          h.strand = 1;
+      // ---- F: output slots for the pass's kept hits: the rest of the
+      // workgroup's reserved chunk, then a new chunk (one global atomic per
+      // kOutChunk hits)
+      const uint32_t nemit = s_nemit;
+      if (nemit) {
+        if (tid == 0) {
+          if (nemit <= s_out_left) {
+            s_pass_a = s_out_next;
+            s_pass_an = nemit;
+            s_pass_b = 0;
+            s_out_next += nemit;
+            s_out_left -= nemit;
+          } else {
+            const uint32_t rest = nemit - s_out_left;
+            const uint32_t chunk = rest > kOutChunk ? rest : kOutChunk;
+            const unsigned long long nb = atomicAdd(out_count, (unsigned long long)chunk);
+            s_pass_a = s_out_next;
+            s_pass_an = s_out_left;
+            s_pass_b = nb;
+            s_out_next = nb + rest;
+            s_out_left = chunk - rest;
+          }
         }
-        ++n_hits;
-        const unsigned long long pos = atomicAdd(out_count, 1ull);
-        if (pos < P.out_cap) out[pos] = h;
-        else atomicOr(err, (uint32_t)E_OUT_FULL);
+        __syncthreads();
+        tick(1);
+        // emit kept hits in the record frame
+        for (uint32_t i = tid; i < take; i += kP1Threads) {
+          const uint4 e = sd[i];
+          if (!((e.w >> 16) & F_EMIT)) continue;
+          const uint32_t hs = e.w >> 24;
+          const uint4 hv = hs < kHitBuf ? hbuf[hs] : my_hitres[i];
+          const int32_t eq = (int32_t)hv.w;
+          const bool minus = ((e.w >> 16) & F_MINUS) != 0;
+          const int32_t es = minus ? lw - 1 + (int32_t)e.y : (int32_t)e.y;
+          const uint32_t qb = hv.x, qe = hv.y;
+          UngHit h;
+          h.item = (uint32_t)item;
+          h.score = (int32_t)hv.z;
+          if (!minus) {
+            h.q_seed = (uint32_t)eq;
+            h.s_seed = (uint32_t)es;
+            h.q_start = qb;
+            h.q_end = qe;
+            h.s_start = (uint32_t)(es - (eq - (int32_t)qb));
+            h.s_end = (uint32_t)es + (qe - (uint32_t)eq);
+            h.strand = 0;
+          } else {
+            const uint32_t rs0 = (uint32_t)(es - (eq - (int32_t)qb));
+            const uint32_t rs1 = (uint32_t)es + (qe - (uint32_t)eq);
+            h.q_seed = (uint32_t)(q_len - 1 - eq);
+            h.s_seed = (uint32_t)(s_len - 1 - es);
+            h.q_start = (uint32_t)q_len - qe;
+            h.q_end = (uint32_t)q_len - qb;
+            h.s_start = (uint32_t)s_len - rs1;
+            h.s_end = (uint32_t)s_len - rs0;
+            h.strand = 1;
+          }
+          ++n_hits;
+          const uint32_t k = atomicAdd(&s_emit_i, 1u);
+          const unsigned long long pos = k < s_pass_an ? s_pass_a + k : s_pass_b + (k - s_pass_an);
+          if (pos < P.out_cap) out[pos] = h;
+          else atomicOr(err, (uint32_t)E_OUT_FULL);
+        }
       }
+      for (int b = tid; b < kBuckets; b += kP1Threads) bcnt[b] = 0;
       __syncthreads();
       tick(5);
-      slot0 = s_next_slot;
-      kcur = s_next_k;
+      if (tid == 0) {  // read and updated only between this pass's E and F barriers
+        s_nemit = 0;
+        s_emit_i = 0;
+      }
+      wk += take;
+      if (wk == wtotal) {
+        w0 += kP1Threads;
+        scanned = false;
+      }
     }
+  }
+  // mark the unused rest of the last reserved chunk
+  for (uint32_t k = tid; k < s_out_left; k += kP1Threads) {
+    const unsigned long long pos = s_out_next + k;
+    if (pos < P.out_cap) out[pos].item = kNoItem;
   }
   atomicAdd(&counters[C_SEED_EVALS], n_evals);
   atomicAdd(&counters[C_UNG_STEPS], n_steps);
@@ -812,6 +874,16 @@ __global__ __launch_bounds__(256) void k_dp_traceback(const TbTask* __restrict__
   }
 }
 
+// Gathers each traceback's ops from its worst-case slot of the op pool into one
+// packed list, so only the ops themselves cross PCIe.
+__global__ __launch_bounds__(256) void k_pack_ops(const PackTask* __restrict__ tasks, uint32_t n_tasks,
+                                                  const uint32_t* __restrict__ ops_pool, uint32_t* __restrict__ packed) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_tasks) return;
+  const PackTask t = tasks[i];
+  for (uint32_t j = 0; j < t.n; ++j) packed[t.dst + j] = ops_pool[t.src + j];
+}
+
 // ------------------------------------------------- fused Phase 2a DPs -------
 // Phase-2a pivot of an ungapped hit (engine.rs:1850-1895), record frame in,
 // DP frame out: (pq, ps) splits the score-only alignment, (tq, ts) is the
@@ -862,6 +934,7 @@ __global__ __launch_bounds__(256) void k_prelim_fused(
   unsigned long long cells = 0, dps = 0;
   for (uint64_t i = gt; i < P.n_hits; i += nt) {
     const UngHit u = hits[i];
+    if (u.item == kNoItem) continue;  // reserved slot the seeding kernel left unused
     const uint32_t r = u.item / P.n_subj, f = u.item % P.n_subj;
     const RecDesc R = recs[r];
     const SubjDesc S = subjs[f];
@@ -945,6 +1018,13 @@ hipError_t launch_phase1(int blocks, P1Params P, const RecDesc* recs, const Subj
 hipError_t launch_dp_score(int blocks, const DpTask* tasks, uint32_t n, const int32_t* gmat, int32_t go, int32_t ge,
                            int2* ring, uint32_t wlog2, DpResult* out, unsigned long long* counters, hipStream_t st) {
   hipLaunchKernelGGL(k_dp_score, dim3(blocks), dim3(256), 0, st, tasks, n, gmat, go, ge, ring, wlog2, out, counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_ops(const PackTask* tasks, uint32_t n, const uint32_t* ops_pool, uint32_t* packed,
+                           hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pack_ops, dim3((n + 255) / 256), dim3(256), 0, st, tasks, n, ops_pool, packed);
   return hipGetLastError();
 }
 

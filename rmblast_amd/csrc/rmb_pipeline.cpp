@@ -19,6 +19,8 @@
 #include <atomic>
 #include <chrono>
 #include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <stdexcept>
@@ -40,6 +42,7 @@ hipError_t launch_phase1(int, P1Params, const RecDesc*, const SubjDesc*, const u
                          const uint8_t*, const uint8_t*, const uint8_t*, const uint32_t*, const uint32_t*,
                          const int32_t*, UngHit*, unsigned long long*, uint2*, uint4*, unsigned long long*,
                          unsigned long long*, uint32_t*, hipStream_t);
+hipError_t launch_pack_ops(const PackTask*, uint32_t, const uint32_t*, uint32_t*, hipStream_t);
 hipError_t launch_dp_score(int, const DpTask*, uint32_t, const int32_t*, int32_t, int32_t, int2*, uint32_t,
                            DpResult*, unsigned long long*, hipStream_t);
 hipError_t launch_dp_traceback(int, const TbTask*, uint32_t, const int32_t*, int32_t, int32_t, int2*, uint32_t,
@@ -82,9 +85,22 @@ struct DevBuf {
   }
 };
 
+// Host worker threads: RMB_THREADS if set, else the hardware threads, at most 16
+// (one GPU's share of the host on a multi-GPU node).
+static unsigned host_threads() {
+  static const unsigned n = [] {
+    if (const char* e = std::getenv("RMB_THREADS")) {
+      const long v = std::strtol(e, nullptr, 10);
+      if (v > 0) return (unsigned)std::min<long>(v, 1024);
+    }
+    return std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+  }();
+  return n;
+}
+
 static void parallel_for(size_t n, const std::function<void(size_t)>& fn) {
   if (n == 0) return;
-  unsigned nt = std::max(1u, std::thread::hardware_concurrency());
+  unsigned nt = host_threads();
   nt = (unsigned)std::min<size_t>(nt, n);
   if (nt == 1) {
     for (size_t i = 0; i < n; ++i) fn(i);
@@ -195,7 +211,7 @@ struct rmb_ctx {
   DevBuf b_qfwd, b_qrc, b_recs, b_elem, b_keys, b_keys2, b_vals, b_vals2, b_counts, b_cells, b_sort_tmp;
   DevBuf b_hits, b_hitcount, b_spill, b_hitres, b_queue, b_counters, b_err;
   DevBuf b_tasks, b_dpres, b_ring, b_ring_big, b_tbtasks, b_tbres, b_flat, b_rows, b_ops;
-  DevBuf b_pre, b_redo, b_precount;
+  DevBuf b_pre, b_redo, b_precount, b_packtasks, b_packed;
 };
 
 namespace rmb {
@@ -351,13 +367,26 @@ static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::
       hip_check(hipStreamSynchronize(c->st), "tb redo sync");
       for (size_t k = 0; k < redo.size(); ++k) tr[redo[k]] = rr[k];
     }
-    std::vector<uint32_t> hops(opsn + 1);
-    hip_check(hipMemcpy(hops.data(), dops, opsn * sizeof(uint32_t), hipMemcpyDeviceToHost), "copy ops");
+    // pack the ops on the GPU and copy back only those
+    std::vector<PackTask> pk(n);
+    uint64_t packed_n = 0;
     for (uint32_t k = 0; k < n; ++k) {
       if (tr[k].flags) throw Failure(RMB_ERR_CAPACITY, "traceback storage or band limit exceeded");
+      pk[k] = PackTask{batch[k].ops_off, packed_n, tr[k].n_ops, 0};
+      packed_n += tr[k].n_ops;
+    }
+    PackTask* dpk = c->b_packtasks.get<PackTask>(n);
+    uint32_t* dpacked = c->b_packed.get<uint32_t>(packed_n + 1);
+    hip_check(hipMemcpyAsync(dpk, pk.data(), n * sizeof(PackTask), hipMemcpyHostToDevice, c->st), "copy pack");
+    hip_check(launch_pack_ops(dpk, n, dops, dpacked, c->st), "k_pack_ops");
+    std::vector<uint32_t> hops(packed_n + 1);
+    hip_check(hipMemcpyAsync(hops.data(), dpacked, packed_n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->st),
+              "copy ops");
+    hip_check(hipStreamSynchronize(c->st), "ops sync");
+    for (uint32_t k = 0; k < n; ++k) {
       const uint32_t i = need[pos - n + k];
       Script& s = out[i].ops;
-      const uint32_t* o = hops.data() + batch[k].ops_off;
+      const uint32_t* o = hops.data() + pk[k].dst;
       for (uint32_t j = 0; j < tr[k].n_ops; ++j) script_push(s, (uint8_t)(o[j] & 3), o[j] >> 2);
     }
   }
@@ -558,10 +587,15 @@ static void phase1_group(rmb_ctx* c, const std::vector<uint32_t>& recs_idx, cons
       const size_t base = hits_out->size();
       hits_out->resize(base + nh);
       hip_check(hipMemcpy(hits_out->data() + base, dh, nh * sizeof(UngHit), hipMemcpyDeviceToHost), "copy hits");
+      size_t w = base;
       for (size_t i = base; i < hits_out->size(); ++i) {  // item ids are group-local: map to batch records
-        const uint32_t r = (*hits_out)[i].item / nsub, f = (*hits_out)[i].item % nsub;
-        (*hits_out)[i].item = recs_idx[r] * nsub + f;
+        UngHit h = (*hits_out)[i];
+        if (h.item == kNoItem) continue;  // reserved slot left unused
+        const uint32_t r = h.item / nsub, f = h.item % nsub;
+        h.item = recs_idx[r] * nsub + f;
+        (*hits_out)[w++] = h;
       }
+      hits_out->resize(w);
     }
     if (pre_out && nh) prelim_group(c, recs, recs_idx, drecs, lw, d_qfwd, d_qrc, dh, nh, *pre_out);
     return;
@@ -674,6 +708,7 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
   c->stats[10] = t2.us();
   // ---- Phase 2b: traceback
   Timer t3;
+  uint64_t tb_marks[4] = {0, 0, 0, 0};  // sort, seeds + tasks, DP on the GPU, replay (us since t3)
   {
     struct TbRef { uint32_t item; uint32_t pre; uint32_t tq, ts; };
     std::vector<size_t> first(items.size() + 1, 0);
@@ -693,6 +728,7 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
         return qea > qeb;
       });
     });
+    tb_marks[0] = t3.us();
     for (size_t k = 0; k < items.size(); ++k) first[k + 1] = first[k] + items[k].pre.size();
     std::vector<TbRef> refs(first.back());
     std::vector<DpTask> tasks(2 * first.back());
@@ -719,8 +755,10 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
         tasks[2 * g + 1] = DpTask{qa + tq + 1, sa + ts + 1, rn ? rm : 0, rm ? rn : 0, c->P.xdrop_gap_final, 0};
       }
     });
+    tb_marks[1] = t3.us();
     std::vector<HalfTb> halves;
     run_dp_traceback(c, tasks, halves);
+    tb_marks[2] = t3.us();
     c->stats[3] = refs.size();
     const int32_t go = c->P.gap_open, ge = c->P.gap_extend;
     parallel_for(items.size(), [&](size_t k) {
@@ -788,6 +826,12 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
     });
   }
   c->stats[11] = t3.us();
+  tb_marks[3] = c->stats[11];
+  if (std::getenv("RMB_STAGE_TIMES"))
+    std::fprintf(stderr,
+                 "[rmb] phase2b ms: sort %.1f  seeds+tasks %.1f  dp %.1f  replay %.1f  (prelims %zu, hits in 2a %zu)\n",
+                 tb_marks[0] / 1e3, (tb_marks[1] - tb_marks[0]) / 1e3, (tb_marks[2] - tb_marks[1]) / 1e3,
+                 (tb_marks[3] - tb_marks[2]) / 1e3, (size_t)c->stats[3] / 2 * 2, (size_t)c->stats[1]);
   // ---- results, item order
   for (auto& s : items) {
     for (auto& h : s.fin) {
@@ -986,7 +1030,7 @@ int rmb_ctx_destroy(rmb_ctx* c) {
                     &c->b_keys, &c->b_keys2, &c->b_vals, &c->b_vals2, &c->b_counts, &c->b_cells, &c->b_sort_tmp,
                     &c->b_hits, &c->b_hitcount, &c->b_spill, &c->b_hitres, &c->b_queue, &c->b_counters, &c->b_err, &c->b_tasks,
                     &c->b_dpres, &c->b_ring, &c->b_ring_big, &c->b_tbtasks, &c->b_tbres, &c->b_flat, &c->b_rows,
-                    &c->b_ops, &c->b_pre, &c->b_redo, &c->b_precount})
+                    &c->b_ops, &c->b_pre, &c->b_redo, &c->b_precount, &c->b_packtasks, &c->b_packed})
     b->release();
   if (c->d_mat) (void)hipFree(c->d_mat);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
