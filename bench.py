@@ -39,7 +39,24 @@ def parse():
     ap.add_argument("--cpu-records", type=int, default=300,
                     help="records in the CPU baseline sample (about 15 s of 16-core work)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB three times (the reference "
+                         "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    import torch
+    n = mib << 20
+    src = torch.empty(n, dtype=torch.uint8, device="cuda")
+    src.fill_(1)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"[bench] copied {n} bytes x 3", file=sys.stderr)
+    return 0
 
 
 def main():
@@ -47,6 +64,8 @@ def main():
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported", file=sys.stderr)
         return 2
+    if a.copy_calibration:
+        return copy_calibration(a.copy_calibration)
     import rmblast_amd
     from rmblast_amd import synth
 
@@ -84,18 +103,19 @@ def main():
     st = stats[-1]
     value = mbp * nfam / step_s
 
-    # roofline of the dominant kernel (the seeding kernel): bytes it must stream
-    # per launch = for every (record, family) item the family's packed strand
-    # (L/4 bytes) + one 4-byte lookup entry per seed + one 4-byte cell start per
-    # scanned position; achieved = bytes / HIP-event kernel time.
+    # roofline of the dominant kernel (the seeding kernel): algorithmic scan
+    # bytes = ceil(L/4) of the streamed sequence per (streamed sequence, lookup
+    # table) pair; here every family is streamed against each record's table,
+    # so records x sum_f ceil(L_f/4); achieved = bytes / HIP-event kernel time.
     fam_len = np.array([len(f[1]) for f in w.families], dtype=np.float64)
-    positions = float(len(queries)) * float(np.maximum(fam_len - 6, 0).sum())
-    algo_bytes = float(len(queries)) * float(np.ceil(fam_len / 4).sum()) + 4.0 * st["seed_evals"] + 4.0 * positions
+    algo_bytes = float(len(queries)) * float(np.ceil(fam_len / 4).sum())
     k_ns = max(st["t_phase1_kernel_ns"], 1)
     achieved = algo_bytes / (k_ns * 1e-9) / 1e9
+    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on the
+    # full workload, scaled by a plain copy's readings (profiles/)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "phase1_traffic.json")
-    if os.path.exists(tpath):
+    if os.path.exists(tpath) and len(queries) == 1667 and nfam == 1000:
         traffic = json.load(open(tpath)).get("hbm_bytes_per_launch_measured")
     valu_ops = OPS_PER_UNGAPPED_STEP * st["ungapped_steps"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
