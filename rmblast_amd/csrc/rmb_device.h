@@ -70,6 +70,11 @@ enum ErrBit : uint32_t {
   E_HASH_FULL = 1u, E_OUT_FULL = 2u, E_BAND_WIDE = 4u, E_TB_SPACE = 8u
 };
 
+// Per-lane DP band ring: 2^wlog2 cells plus one 128-byte line of padding, so
+// the same ring cell of different lanes does not fall on the same L2 channel.
+constexpr uint32_t kRingPad = 16;
+inline constexpr uint64_t ring_stride(uint32_t wlog2) { return (1ull << wlog2) + kRingPad; }
+
 struct DpTask {         // one half of an X-drop alignment, lane-serial
   const uint8_t* a;     // outer sequence (rows)
   const uint8_t* b;     // inner sequence (columns, the band)

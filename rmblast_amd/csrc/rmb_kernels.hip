@@ -793,7 +793,7 @@ __global__ __launch_bounds__(256) void k_dp_score(const DpTask* __restrict__ tas
   for (int i = threadIdx.x; i < 256; i += blockDim.x) mat[i] = gmat[i];
   __syncthreads();
   const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
-  int2* my = ring + ((size_t)gt << wlog2);
+  int2* my = ring + (size_t)gt * ring_stride(wlog2);
   unsigned long long cells = 0;
   for (uint32_t i = gt; i < n_tasks; i += nt) {
     DpResult r;
@@ -813,7 +813,7 @@ __global__ __launch_bounds__(256) void k_dp_traceback(const TbTask* __restrict__
   for (int i = threadIdx.x; i < 256; i += blockDim.x) mat[i] = gmat[i];
   __syncthreads();
   const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
-  int2* my = ring + ((size_t)gt << wlog2);
+  int2* my = ring + (size_t)gt * ring_stride(wlog2);
   for (uint32_t i = gt; i < n_tasks; i += nt) {
     const TbTask T = tasks[i];
     uint8_t* flat = flat_pool + T.flat_off;
@@ -929,7 +929,7 @@ __global__ __launch_bounds__(256) void k_prelim_fused(
   for (int i = threadIdx.x; i < 256; i += blockDim.x) mat[i] = gmat[i];
   __syncthreads();
   const uint64_t gt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (uint64_t)gridDim.x * blockDim.x;
-  int2* my = ring + (gt << wlog2);
+  int2* my = ring + gt * ring_stride(wlog2);
   const uint32_t wmask = (1u << wlog2) - 1;
   unsigned long long cells = 0, dps = 0;
   for (uint64_t i = gt; i < P.n_hits; i += nt) {

@@ -253,7 +253,7 @@ static void run_dp_score(rmb_ctx* c, const std::vector<DpTask>& tasks, std::vect
     DpResult* dr = c->b_dpres.get<DpResult>(n);
     hip_check(hipMemcpyAsync(dt, tasks.data() + base, n * sizeof(DpTask), hipMemcpyHostToDevice, c->st), "copy tasks");
     const uint32_t lanes = std::min<uint32_t>((uint32_t)c->cu_count * 4 * 64, ((n + 255) / 256) * 256);
-    int2* ring = c->b_ring.get<int2>((size_t)lanes << kRingLog2);
+    int2* ring = c->b_ring.get<int2>((size_t)lanes * ring_stride(kRingLog2));
     timed_launch(c, 14, [&] {
       hip_check(launch_dp_score((int)(lanes / 256), dt, n, c->d_mat, c->P.gap_open, c->P.gap_extend, ring,
                                 kRingLog2, dr, ctr, c->st),
@@ -273,7 +273,7 @@ static void run_dp_score(rmb_ctx* c, const std::vector<DpTask>& tasks, std::vect
   DpResult* dr = c->b_dpres.get<DpResult>(n);
   hip_check(hipMemcpyAsync(dt, wt.data(), n * sizeof(DpTask), hipMemcpyHostToDevice, c->st), "copy wide");
   const uint32_t lanes = std::min<uint32_t>(4096, ((n + 255) / 256) * 256);
-  int2* ring = c->b_ring_big.get<int2>((size_t)lanes << kRingBigLog2);
+  int2* ring = c->b_ring_big.get<int2>((size_t)lanes * ring_stride(kRingBigLog2));
   timed_launch(c, 14, [&] {
     hip_check(launch_dp_score((int)(lanes / 256), dt, n, c->d_mat, c->P.gap_open, c->P.gap_extend, ring,
                               kRingBigLog2, dr, ctr, c->st),
@@ -337,7 +337,7 @@ static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::
     hip_check(hipMemcpyAsync(dt, batch.data(), n * sizeof(TbTask), hipMemcpyHostToDevice, c->st), "copy tb tasks");
     std::vector<TbResult> tr(n);
     const uint32_t lanes = std::min<uint32_t>((uint32_t)c->cu_count * 4 * 64, ((n + 255) / 256) * 256);
-    int2* ring = c->b_ring.get<int2>((size_t)lanes << kRingLog2);
+    int2* ring = c->b_ring.get<int2>((size_t)lanes * ring_stride(kRingLog2));
     timed_launch(c, 14, [&] {
       hip_check(launch_dp_traceback((int)(lanes / 256), dt, n, c->d_mat, c->P.gap_open, c->P.gap_extend, ring,
                                     kRingLog2, dflat, drows, dops, dr, c->st),
@@ -356,7 +356,7 @@ static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::
       TbTask* rdt = c->b_tbtasks.get<TbTask>(std::max<size_t>(n, rn));
       hip_check(hipMemcpyAsync(rdt, rb.data(), rn * sizeof(TbTask), hipMemcpyHostToDevice, c->st), "copy tb redo");
       const uint32_t rl = std::min<uint32_t>(4096, ((rn + 255) / 256) * 256);
-      int2* big = c->b_ring_big.get<int2>((size_t)rl << kRingBigLog2);
+      int2* big = c->b_ring_big.get<int2>((size_t)rl * ring_stride(kRingBigLog2));
       timed_launch(c, 14, [&] {
         hip_check(launch_dp_traceback((int)(rl / 256), rdt, rn, c->d_mat, c->P.gap_open, c->P.gap_extend, big,
                                       kRingBigLog2, dflat, drows, dops, dr, c->st),
@@ -424,7 +424,7 @@ static void prelim_group(rmb_ctx* c, const std::vector<RecDesc>& recs, const std
   uint32_t* err = c->b_err.get<uint32_t>(1);
   unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
   const uint64_t lanes = std::min<uint64_t>((uint64_t)c->cu_count * 8 * 64, ((nh + 255) / 256) * 256);
-  int2* ring = c->b_ring.get<int2>((size_t)lanes << kRingLog2);
+  int2* ring = c->b_ring.get<int2>((size_t)lanes * ring_stride(kRingLog2));
   hip_check(hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned long long), c->st), "memset prelim counts");
   hip_check(hipMemsetAsync(err, 0, sizeof(uint32_t), c->st), "memset err");
   timed_launch(c, 14, [&] {
