@@ -115,6 +115,13 @@ int rmb_library_load(rmb_ctx* ctx, uint32_t n, const uint8_t* const* seqs, const
  * Results are held by the context until the next search. */
 int rmb_search(rmb_ctx* ctx, uint32_t n_queries, const uint8_t* const* queries, const uint32_t* lens);
 
+/* Search only the listed (query, subject) pairs of a batch: the reference's
+ * search_with_query_lookup for one query and one subject (engine.rs:715-729),
+ * over many pairs at once.  Duplicate pairs are searched once; results as for
+ * rmb_search, in (query, subject) order. */
+int rmb_search_pairs(rmb_ctx* ctx, uint32_t n_queries, const uint8_t* const* queries, const uint32_t* lens,
+                     uint64_t n_pairs, const uint32_t* pair_query, const uint32_t* pair_subject);
+
 /* Two-call result retrieval: sizes, then copy into caller buffers. */
 int rmb_results_size(const rmb_ctx* ctx, uint64_t* n_hsps, uint64_t* n_ops);
 int rmb_results_copy(const rmb_ctx* ctx, rmb_hsp* hsps, rmb_op* ops);

@@ -20,7 +20,7 @@ CLI_PATH = os.path.join(_HERE, "_build", "rmblastn")
 
 EXPORTED = [
     "rmb_matrix_load", "rmb_ctx_create", "rmb_ctx_destroy", "rmb_last_error", "rmb_library_load",
-    "rmb_search", "rmb_results_size", "rmb_results_copy", "rmb_stats", "rmb_tap_ungapped",
+    "rmb_search", "rmb_search_pairs", "rmb_results_size", "rmb_results_copy", "rmb_stats", "rmb_tap_ungapped",
     "rmb_tap_align_score_only", "rmb_tap_align_ex",
 ]
 
@@ -63,6 +63,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rmb_last_error.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
     lib.rmb_library_load.argtypes = [ctypes.c_void_p, ctypes.c_uint32, P(u8p), P(ctypes.c_uint32)]
     lib.rmb_search.argtypes = [ctypes.c_void_p, ctypes.c_uint32, P(u8p), P(ctypes.c_uint32)]
+    lib.rmb_search_pairs.argtypes = [ctypes.c_void_p, ctypes.c_uint32, P(u8p), P(ctypes.c_uint32), ctypes.c_uint64,
+                                     P(ctypes.c_uint32), P(ctypes.c_uint32)]
     lib.rmb_results_size.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), P(ctypes.c_uint64)]
     lib.rmb_results_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     lib.rmb_stats.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), ctypes.c_int]
@@ -149,6 +151,23 @@ class Searcher:
         ptrs = (ctypes.POINTER(ctypes.c_uint8) * len(arrs))(*[_u8ptr(a) for a in arrs])
         lens = (ctypes.c_uint32 * len(arrs))(*[len(a) for a in arrs])
         self._check(self._lib.rmb_search(self._ctx, len(arrs), ptrs, lens), "rmb_search")
+        return self._results()
+
+    def search_pairs(self, queries, pair_query, pair_subject):
+        """Search only the listed (query index, subject index) pairs."""
+        arrs = [np.ascontiguousarray(q, dtype=np.uint8) for q in queries]
+        ptrs = (ctypes.POINTER(ctypes.c_uint8) * len(arrs))(*[_u8ptr(a) for a in arrs])
+        lens = (ctypes.c_uint32 * len(arrs))(*[len(a) for a in arrs])
+        pq = np.ascontiguousarray(pair_query, dtype=np.uint32)
+        ps = np.ascontiguousarray(pair_subject, dtype=np.uint32)
+        assert pq.shape == ps.shape
+        self._check(self._lib.rmb_search_pairs(self._ctx, len(arrs), ptrs, lens, len(pq),
+                                               pq.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                               ps.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))),
+                    "rmb_search_pairs")
+        return self._results()
+
+    def _results(self):
         nh, no = ctypes.c_uint64(), ctypes.c_uint64()
         self._lib.rmb_results_size(self._ctx, ctypes.byref(nh), ctypes.byref(no))
         hsps = np.zeros(nh.value, dtype=HSP_DTYPE)

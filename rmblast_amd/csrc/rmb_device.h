@@ -30,8 +30,9 @@ struct SubjDesc {       // one library sequence
 struct P1Params {
   int32_t word_size, lut_width, scan_step, xdrop_ungap, cutoff;
   uint32_t n_subj;
-  uint64_t n_items;
+  uint64_t n_items;          // records x subjects, or the length of item_list
   uint64_t out_cap;
+  const uint32_t* item_list; // null: every (record, subject) item; else these item ids
 };
 
 constexpr uint32_t kNoItem = 0xFFFFFFFFu;  // UngHit::item of an output slot reserved but left unused

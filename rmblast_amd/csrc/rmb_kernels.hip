@@ -330,14 +330,24 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
       // record-major, so one XCD's workgroups share a record's lookup table in
       // their L2; an XCD whose queue is empty takes work from the others.
       unsigned long long it = ~0ull;
-      const uint32_t n_recs = (uint32_t)(P.n_items / P.n_subj);
-      for (int k = 0; k < kXcds && it == ~0ull; ++k) {
-        const uint32_t x = (xcd + k) % kXcds;
-        const unsigned long long n_x = (unsigned long long)(n_recs > x ? (n_recs - x + kXcds - 1) / kXcds : 0) *
-                                       P.n_subj;
-        if (n_x == 0) continue;
-        const unsigned long long t = atomicAdd(&queue[x], 1ull);
-        if (t < n_x) it = (unsigned long long)(x + kXcds * (t / P.n_subj)) * P.n_subj + t % P.n_subj;
+      if (P.item_list) {  // listed items: the list is cut into kXcds contiguous slices
+        for (int k = 0; k < kXcds && it == ~0ull; ++k) {
+          const uint32_t x = (xcd + k) % kXcds;
+          const unsigned long long lo = P.n_items * x / kXcds, hi = P.n_items * (x + 1) / kXcds;
+          if (hi == lo) continue;
+          const unsigned long long t = atomicAdd(&queue[x], 1ull);
+          if (t < hi - lo) it = P.item_list[lo + t];
+        }
+      } else {
+        const uint32_t n_recs = (uint32_t)(P.n_items / P.n_subj);
+        for (int k = 0; k < kXcds && it == ~0ull; ++k) {
+          const uint32_t x = (xcd + k) % kXcds;
+          const unsigned long long n_x =
+              (unsigned long long)(n_recs > x ? (n_recs - x + kXcds - 1) / kXcds : 0) * P.n_subj;
+          if (n_x == 0) continue;
+          const unsigned long long t = atomicAdd(&queue[x], 1ull);
+          if (t < n_x) it = (unsigned long long)(x + kXcds * (t / P.n_subj)) * P.n_subj + t % P.n_subj;
+        }
       }
       s_item = it;
       s_nhit = 0;

@@ -211,7 +211,7 @@ struct rmb_ctx {
   DevBuf b_qfwd, b_qrc, b_recs, b_elem, b_keys, b_keys2, b_vals, b_vals2, b_counts, b_cells, b_sort_tmp;
   DevBuf b_hits, b_hitcount, b_spill, b_hitres, b_queue, b_counters, b_err;
   DevBuf b_tasks, b_dpres, b_ring, b_ring_big, b_tbtasks, b_tbres, b_flat, b_rows, b_ops;
-  DevBuf b_pre, b_redo, b_precount, b_packtasks, b_packed;
+  DevBuf b_pre, b_redo, b_precount, b_packtasks, b_packed, b_items;
 };
 
 namespace rmb {
@@ -491,7 +491,7 @@ static void prelim_group(rmb_ctx* c, const std::vector<RecDesc>& recs, const std
 // the preliminary DPs of its hits.
 static void phase1_group(rmb_ctx* c, const std::vector<uint32_t>& recs_idx, const std::vector<RecDesc>& all_recs,
                          int lw, const uint8_t* d_qfwd, const uint8_t* d_qrc, std::vector<UngHit>* hits_out,
-                         std::vector<PreHit>* pre_out) {
+                         std::vector<PreHit>* pre_out, const std::vector<uint32_t>* items = nullptr) {
   const int ws = c->P.word_size;
   const uint32_t R = (uint32_t)recs_idx.size();
   const uint64_t cells = 1ull << (2 * lw);
@@ -538,6 +538,15 @@ static void phase1_group(rmb_ctx* c, const std::vector<uint32_t>& recs_idx, cons
                                       : (c->P.min_raw_gapped_score > 0 ? c->P.min_raw_gapped_score / 2 : 1);
   P.n_subj = nsub;
   P.n_items = (uint64_t)R * nsub;
+  P.item_list = nullptr;
+  if (items) {  // group-local item ids (record index in the group * nsub + subject)
+    if (items->empty()) return;
+    uint32_t* dl = c->b_items.get<uint32_t>(items->size());
+    hip_check(hipMemcpyAsync(dl, items->data(), items->size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->st),
+              "copy items");
+    P.item_list = dl;
+    P.n_items = items->size();
+  }
   // ~36 KB of LDS per workgroup: four resident per CU.  Workgroups are dealt
   // round-robin over the XCDs, so blockIdx % kXcds names the XCD.
   const int blocks = std::max(1, std::min<int>(c->cu_count * 4, (int)std::min<uint64_t>(P.n_items, 1u << 20)));
@@ -603,7 +612,9 @@ static void phase1_group(rmb_ctx* c, const std::vector<uint32_t>& recs_idx, cons
   throw Failure(RMB_ERR_CAPACITY, "ungapped hit buffer could not be sized");
 }
 
-static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const uint32_t* lens) {
+// pairs: null for every (query, subject) pair, else the item ids q * nsub + s to search
+static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const uint32_t* lens,
+                        const std::vector<uint64_t>* pairs = nullptr) {
   Timer total;
   if (c->subj.empty()) throw Failure(RMB_ERR_STATE, "no library loaded");
   std::memset(c->stats, 0, sizeof c->stats);
@@ -649,7 +660,17 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
       const bool flush = i == nq || (!grp.empty() && ((grp.size() + 1) * cells > (1ull << 28) ||
                                                       elems + 2ull * lens[i] + 1 > (1ull << 31)));
       if (flush && !grp.empty()) {
-        phase1_group(c, grp, recs, lw, d_qfwd, d_qrc, nullptr, &survivors);
+        if (pairs) {
+          std::vector<uint32_t> pos(nq, UINT32_MAX), local;
+          for (uint32_t k = 0; k < grp.size(); ++k) pos[grp[k]] = k;
+          for (uint64_t it : *pairs) {
+            const uint32_t q = (uint32_t)(it / nsub), f = (uint32_t)(it % nsub);
+            if (pos[q] != UINT32_MAX) local.push_back(pos[q] * nsub + f);
+          }
+          phase1_group(c, grp, recs, lw, d_qfwd, d_qrc, nullptr, &survivors, &local);
+        } else {
+          phase1_group(c, grp, recs, lw, d_qfwd, d_qrc, nullptr, &survivors);
+        }
         grp.clear();
         elems = 0;
       }
@@ -1030,7 +1051,7 @@ int rmb_ctx_destroy(rmb_ctx* c) {
                     &c->b_keys, &c->b_keys2, &c->b_vals, &c->b_vals2, &c->b_counts, &c->b_cells, &c->b_sort_tmp,
                     &c->b_hits, &c->b_hitcount, &c->b_spill, &c->b_hitres, &c->b_queue, &c->b_counters, &c->b_err, &c->b_tasks,
                     &c->b_dpres, &c->b_ring, &c->b_ring_big, &c->b_tbtasks, &c->b_tbres, &c->b_flat, &c->b_rows,
-                    &c->b_ops, &c->b_pre, &c->b_redo, &c->b_precount, &c->b_packtasks, &c->b_packed})
+                    &c->b_ops, &c->b_pre, &c->b_redo, &c->b_precount, &c->b_packtasks, &c->b_packed, &c->b_items})
     b->release();
   if (c->d_mat) (void)hipFree(c->d_mat);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1105,6 +1126,27 @@ int rmb_search(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const uint32_t
     if (nq == 0 || !qs || !lens) throw Failure(RMB_ERR_ARG, "empty query batch");
     hip_check(hipSetDevice(c->device), "hipSetDevice");
     search_impl(c, nq, qs, lens);
+  });
+}
+
+int rmb_search_pairs(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const uint32_t* lens, uint64_t n_pairs,
+                     const uint32_t* pair_query, const uint32_t* pair_subject) {
+  if (!c) return RMB_ERR_ARG;
+  return guarded(c, [&]() {
+    if (nq == 0 || !qs || !lens) throw Failure(RMB_ERR_ARG, "empty query batch");
+    if (n_pairs && (!pair_query || !pair_subject)) throw Failure(RMB_ERR_ARG, "null pair list");
+    if (c->subj.empty()) throw Failure(RMB_ERR_STATE, "no library loaded");
+    const uint64_t nsub = c->subj.size();
+    if ((uint64_t)nq * nsub > UINT32_MAX) throw Failure(RMB_ERR_ARG, "more than 2^32 query x subject pairs");
+    std::vector<uint64_t> items(n_pairs);
+    for (uint64_t k = 0; k < n_pairs; ++k) {
+      if (pair_query[k] >= nq || pair_subject[k] >= nsub) throw Failure(RMB_ERR_ARG, "pair index out of range");
+      items[k] = (uint64_t)pair_query[k] * nsub + pair_subject[k];
+    }
+    std::sort(items.begin(), items.end());
+    items.erase(std::unique(items.begin(), items.end()), items.end());
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    search_impl(c, nq, qs, lens, &items);
   });
 }
 

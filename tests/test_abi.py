@@ -1,6 +1,8 @@
-"""The C-ABI library loads and exports every entry point include/rmblast_hip.h
-declares; host-side argument errors come back as error codes (no GPU call)."""
+"""The C-ABI library loads and exports every entry point the headers under
+include/ declare; host-side argument errors come back as error codes (no GPU
+call)."""
 import ctypes
+import glob
 import os
 import re
 
@@ -8,19 +10,22 @@ import pytest
 
 from conftest import CLI, ROOT
 
-HEADER = os.path.join(ROOT, "include", "rmblast_hip.h")
+HEADERS = sorted(glob.glob(os.path.join(ROOT, "include", "*.h")))
 
 
 def declared():
-    text = open(HEADER).read()
-    return sorted(set(re.findall(r"^int (rmb_\w+)\(", text, re.M)))
+    names = set()
+    for h in HEADERS:
+        names |= set(re.findall(r"^int (rmb_\w+)\(", open(h).read(), re.M))
+    return sorted(names)
 
 
 def test_header_declares_the_boundary():
     names = declared()
     for n in ("rmb_ctx_create", "rmb_ctx_destroy", "rmb_library_load", "rmb_search", "rmb_results_size",
               "rmb_results_copy", "rmb_last_error", "rmb_matrix_load", "rmb_tap_ungapped",
-              "rmb_tap_align_score_only", "rmb_tap_align_ex", "rmb_stats"):
+              "rmb_tap_align_score_only", "rmb_tap_align_ex", "rmb_stats", "rmb_search_pairs",
+              "rmb_triage_create", "rmb_triage_forward_backward", "rmb_triage_logits", "rmb_kmer_profiles"):
         assert n in names
 
 

@@ -1,0 +1,151 @@
+"""Learned triage on the GPU, through the C ABI: the BF16 MFMA GEMM against
+numpy for every operand layout, the k-mer profiles, the model's logits, loss
+and gradients against the CPU oracle (oracle/triage_ref.py, same rounding
+points), training reduces the loss, and the pair search used by the pruned
+path returns exactly the full search's HSPs for the pairs it is given."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import triage_ref as T  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def model():
+    from rmblast_amd.triage import Triage
+    m = Triage(seed=3)
+    yield m
+    m.close()
+
+
+def _bf16_rand(rng, shape, scale=1.0):
+    return T.bf16_bits(rng.standard_normal(shape).astype(np.float32) * scale)
+
+
+@pytest.mark.parametrize("ta,tb", [(False, True), (True, False), (False, False), (True, True)])
+@pytest.mark.parametrize("M,N,K", [(70, 45, 100), (64, 64, 32), (5, 130, 2080), (129, 7, 3)])
+def test_gemm_tap_matches_numpy(model, ta, tb, M, N, K):
+    rng = np.random.default_rng(M * 7 + N * 3 + K)
+    A = _bf16_rand(rng, (K, M) if ta else (M, K))
+    B = _bf16_rand(rng, (N, K) if tb else (K, N))
+    C = model.tap_gemm(ta, tb, M, N, K, A, A.shape[1], B, B.shape[1])
+    a = T.from_bf16_bits(A).astype(np.float64)
+    b = T.from_bf16_bits(B).astype(np.float64)
+    ref = (a.T if ta else a) @ (b.T if tb else b)
+    np.testing.assert_allclose(C, ref, rtol=1e-4, atol=1e-4 * np.sqrt(K))
+
+
+def test_kmer_profiles_match_oracle(model):
+    rng = np.random.default_rng(11)
+    seq = rng.integers(0, 4, 9000).astype(np.uint8)
+    seq[rng.choice(9000, 40, replace=False)] = 14
+    starts = np.array([0, 1000, 2500, 7000, 8995, 4000], dtype=np.uint64)
+    lens = np.array([2000, 2000, 3000, 2000, 5, 0], dtype=np.uint32)
+    got = model.profiles(seq, starts, lens)
+    for k in range(len(starts)):
+        ref = T.bf16_bits(T.kmer_profile(seq, int(starts[k]), int(lens[k])))
+        diff = np.abs(got[k].astype(np.int32) - ref.astype(np.int32))
+        assert diff.max() <= 1, k  # log1pf and numpy's log1p may differ by one BF16 step
+
+
+def test_forward_backward_matches_oracle(model):
+    rng = np.random.default_rng(2)
+    nw, nf = 37, 5
+    xw = T.bf16_bits(np.log1p(rng.poisson(1.0, (nw, T.KMER_DIM))).astype(np.float32))
+    xf = T.bf16_bits(np.log1p(rng.poisson(1.5, (nf, T.KMER_DIM))).astype(np.float32))
+    lab = (rng.random(nw * nf) < 0.2).astype(np.uint8)
+    flat = T.init_params(4) * 0.5
+    flat[T.layout()[0]["a"][0]] = 0.05
+    model.set_params(flat)
+    loss, grads = model.forward_backward(xw, xf, lab, pos_weight=4.0, want_grads=True)
+    logits = model.logits(xw, xf).reshape(-1)
+    z, loss_ref, g_ref = T.forward_backward(flat, T.from_bf16_bits(xw), T.from_bf16_bits(xf), lab, pos_weight=4.0)
+    np.testing.assert_allclose(logits, z, rtol=2e-3, atol=2e-3)
+    assert abs(loss - loss_ref) <= 1e-3 * max(1.0, abs(loss_ref))
+    off, _ = T.layout()
+    for k, (o, shape) in off.items():
+        n = int(np.prod(shape)) if shape else 1
+        a, b = grads[o:o + n].astype(np.float64), g_ref[o:o + n].astype(np.float64)
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+        assert rel < 2e-2, (k, rel)
+
+
+def test_training_reduces_loss(model):
+    rng = np.random.default_rng(9)
+    nw, nf = 64, 8
+    # positives: the family profile is mixed into the window profile
+    fam = rng.poisson(2.0, (nf, T.KMER_DIM))
+    win = rng.poisson(1.0, (nw, T.KMER_DIM))
+    lab = np.zeros((nw, nf), dtype=np.uint8)
+    for i in range(nw):
+        j = rng.integers(0, nf)
+        if rng.random() < 0.5:
+            win[i] += fam[j]
+            lab[i, j] = 1
+    xw = T.bf16_bits(np.log1p(win).astype(np.float32))
+    xf = T.bf16_bits(np.log1p(fam).astype(np.float32))
+    model.set_params(T.init_params(6) * 0.3)
+    first = None
+    for step in range(60):
+        loss, _ = model.forward_backward(xw, xf, lab, pos_weight=4.0)
+        first = loss if first is None else first
+        model.adam_step(1e-3)
+    assert loss < 0.5 * first, (first, loss)
+
+
+@pytest.fixture(scope="module")
+def small_instance():
+    import rmblast_amd
+    from rmblast_amd import synth
+    import tempfile
+    w = synth.parity_config(seed=21, genome_len=120_000, n_families=12, n_copies=240)
+    d = tempfile.mkdtemp()
+    mp = os.path.join(d, "synth.matrix")
+    open(mp, "w").write(synth.MATRIX_TEXT)
+    m = rmblast_amd.load_matrix(mp)
+    s = rmblast_amd.Searcher(m, gap_open=20, gap_extend=5, word_size=7, min_raw_gapped_score=200,
+                             complexity_adjust=True)
+    s.load_library([f[1] for f in w.families])
+    return s, w
+
+
+def test_search_pairs_equals_full_search_on_those_pairs(small_instance):
+    from rmblast_amd.triage import hsp_key
+    s, w = small_instance
+    q = w.records[0][1]
+    half = len(q) // 2
+    queries = [q[:half], q[half:]]
+    full, fops = s.search(queries)
+    full_keys = {(int(h["query"]),) + hsp_key(h, fops) for h in full}
+    nf = len(w.families)
+    pq = [0, 0, 1, 1, 1, 0]
+    ps = [0, 3, 1, 5, nf - 1, 3]  # duplicates are searched once
+    got, gops = s.search_pairs(queries, pq, ps)
+    got_keys = {(int(h["query"]),) + hsp_key(h, gops) for h in got}
+    want = {k for k in full_keys if (k[0], k[1]) in set(zip(pq, ps))}
+    assert got_keys == want
+    allp, aops = s.search_pairs(queries, [i for i in range(2) for _ in range(nf)], list(range(nf)) * 2)
+    assert {(int(h["query"]),) + hsp_key(h, aops) for h in allp} == full_keys
+
+
+def test_pruned_search_with_true_labels_recovers_the_full_search(small_instance):
+    from rmblast_amd.triage import Instance, labels_from_hsps, pruned_search, windows, hsp_key
+    s, w = small_instance
+    q = w.records[0][1]
+    full, ops = s.search([q])
+    st, ln = windows(len(q))
+    lab = labels_from_hsps(full, st, ln, len(w.families))
+    inst = Instance(seq=q, families=[f[1] for f in w.families], starts=st, lens=ln, hsps=full, ops=ops,
+                    labels=lab)
+    keys, n_pairs, _ = pruned_search(s, inst, lab.astype(bool), margin=2000)
+    full_keys = {hsp_key(h, ops) for h in full}
+    recall = len(full_keys & keys) / max(1, len(full_keys))
+    assert recall >= 0.99, recall
+    assert n_pairs > 0
