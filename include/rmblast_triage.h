@@ -4,7 +4,8 @@
  * The triage has no counterpart in the reference: it is the pruning step the
  * reference does not have, placed before seeding.  It scores (query window,
  * family) pairs from canonical 6-mer profiles with a two-tower MLP
- * (2080 -> 512 -> 256 per tower, dot-plus-two-layer head: 256 -> 64 -> 1),
+ * (2080 -> 512 -> 256 per tower, dot-plus-two-layer head: 256 -> 64 -> 1, plus
+ * a learned weight on the cosine of the raw profiles),
  * BF16 weights and GEMM operands with FP32 accumulation, trained on the GPU
  * by the library's own forward and backward kernels.  Pairs it passes go
  * through the exact search (rmb_search_pairs in rmblast_hip.h), so every HSP
@@ -36,7 +37,7 @@ int rmb_triage_destroy(rmb_triage* t);
 int rmb_triage_last_error(const rmb_triage* t, char* buf, size_t len);
 
 /* Flat FP32 parameter vector (window tower W1 b1 W2 b2, family tower, head
- * Wh1 bh1 wh2 bh2 a; matrices row-major [out][in]).  set_params resets the
+ * Wh1 bh1 wh2 bh2 a c; matrices row-major [out][in]).  set_params resets the
  * optimiser state. */
 int rmb_triage_n_params(const rmb_triage* t, uint64_t* n);
 int rmb_triage_get_params(const rmb_triage* t, float* out);
@@ -49,8 +50,9 @@ int rmb_kmer_profiles(rmb_triage* t, const uint8_t* seq, uint64_t seq_len, const
                       const uint32_t* lens, uint32_t n, uint16_t* out);
 
 /* Forward and backward over all nw x nf pairs of one batch (pair p = i*nf + j,
- * labels[p] 0/1): weighted logistic loss (weight pos_weight on positives,
- * normalised by the total weight).  The gradient is kept for
+ * labels[p] 0 negative, 1 positive, 2 ignored): weighted logistic loss
+ * (weight pos_weight on positives, 1 on negatives, normalised by the total
+ * weight).  The gradient is kept for
  * rmb_triage_adam_step and copied to grads (n_params floats) when non-NULL. */
 int rmb_triage_forward_backward(rmb_triage* t, const uint16_t* xw, uint32_t nw, const uint16_t* xf, uint32_t nf,
                                 const uint8_t* labels, float pos_weight, float* loss, float* grads);

@@ -81,7 +81,8 @@ def layout():
     off["wh2"] = (h + HEAD * EMB + HEAD, (HEAD,))
     off["bh2"] = (h + HEAD * EMB + 2 * HEAD, ())
     off["a"] = (h + HEAD * EMB + 2 * HEAD + 1, ())
-    total = h + HEAD * EMB + 2 * HEAD + 2
+    off["c"] = (h + HEAD * EMB + 2 * HEAD + 2, ())
+    total = h + HEAD * EMB + 2 * HEAD + 3
     return off, total
 
 
@@ -105,8 +106,9 @@ def pack(d: dict, dtype=np.float32) -> np.ndarray:
 
 # ------------------------------------------------------- forward / backward --
 def forward_backward(flat, xw, xf, labels=None, pos_weight=1.0, bf16=True, dtype=np.float32):
-    """Logits of all nw x nf pairs (pair p = i*nf + j); with labels also the
-    weighted logistic loss and the gradient as a flat vector."""
+    """Logits of all nw x nf pairs (pair p = i*nf + j); with labels (0
+    negative, 1 positive, 2 ignored) also the weighted logistic loss and the
+    gradient as a flat vector."""
     r = bf16_round if bf16 else (lambda x: x)
     P_ = unpack(np.asarray(flat, dtype=dtype))
     xw = np.asarray(xw, dtype=dtype)
@@ -124,19 +126,24 @@ def forward_backward(flat, xw, xf, labels=None, pos_weight=1.0, bf16=True, dtype
     prod = ew[:, None, :] * ef[None, :, :]
     P = r(prod).reshape(nw * nf, EMB)
     dot = prod.sum(-1).reshape(-1)
+    xr_w, xr_f = r(xw), r(xf)
+    nrm = np.linalg.norm(xr_w, axis=1)[:, None] * np.linalg.norm(xr_f, axis=1)[None, :]
+    cos = np.where(nrm > 0, (xr_w @ xr_f.T) / np.where(nrm > 0, nrm, 1), 0).reshape(-1)
     hh = P @ r(P_["Wh1"]).T + P_["bh1"]
     act = np.maximum(hh, 0)
-    z = P_["bh2"] + P_["a"] * dot + act @ P_["wh2"]
+    z = P_["bh2"] + P_["a"] * dot + P_["c"] * cos + act @ P_["wh2"]
     if labels is None:
         return z
-    y = np.asarray(labels, dtype=dtype).reshape(-1)
-    wt = np.where(y > 0, pos_weight, 1.0).astype(dtype)
-    wsum = wt.sum()
+    lab = np.asarray(labels).reshape(-1)
+    y = (lab == 1).astype(dtype)
+    wt = np.where(lab == 1, pos_weight, np.where(lab == 0, 1.0, 0.0)).astype(dtype)
+    wsum = max(wt.sum(), 1.0)
     loss = (wt * (np.maximum(z, 0) - z * y + np.log1p(np.exp(-np.abs(z))))).sum() / wsum
     g = wt * (1 / (1 + np.exp(-z)) - y) / wsum
     G = {}
     G["bh2"] = g.sum()
     G["a"] = (g * dot).sum()
+    G["c"] = (g * cos).sum()
     G["wh2"] = act.T @ g
     dh = (hh > 0) * g[:, None] * P_["wh2"][None, :]
     G["bh1"] = dh.sum(0)

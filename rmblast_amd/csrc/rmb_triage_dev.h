@@ -22,8 +22,8 @@ struct Layout {
   static constexpr size_t b2(int t) { return tower(t) + W1 + B1 + W2; }
   static constexpr size_t HEAD = 2 * TOWER;
   static constexpr size_t wh1() { return HEAD; }                         // [kHead][kEmb]
-  static constexpr size_t hp() { return HEAD + (size_t)kHead * kEmb; }   // bh1[64], wh2[64], bh2, a
-  static constexpr size_t HP = 2 * kHead + 2;
+  static constexpr size_t hp() { return HEAD + (size_t)kHead * kEmb; }   // bh1[64], wh2[64], bh2, a, c
+  static constexpr size_t HP = 2 * kHead + 3;
   static constexpr size_t TOTAL = HEAD + (size_t)kHead * kEmb + HP;
 };
 

@@ -26,8 +26,9 @@ hipError_t launch_relu_bwd(float*, const float*, size_t, uint16_t*, hipStream_t)
 hipError_t launch_to_bf16(const float*, size_t, uint16_t*, hipStream_t);
 hipError_t launch_colsum(const float*, int, int, float*, hipStream_t);
 hipError_t launch_pair_product(const float*, const float*, int, int, uint16_t*, hipStream_t);
-hipError_t launch_head(const float*, const float*, const float*, int, int, const float*, const uint8_t*, float, float,
-                       float*, float*, float*, uint16_t*, float*, float*, hipStream_t);
+hipError_t launch_head(const float*, const float*, const float*, const float*, int, int, const float*, const uint8_t*,
+                       float, float, float*, float*, float*, uint16_t*, float*, float*, hipStream_t);
+hipError_t launch_cosine(const uint16_t*, int, const uint16_t*, int, float*, float*, hipStream_t);
 hipError_t launch_embed_grad(const float*, const float*, const float*, const float*, int, int, const float*, float*,
                              float*, hipStream_t);
 hipError_t launch_adam(float*, uint16_t*, const float*, float*, float*, size_t, float, float, float, float, float,
@@ -103,7 +104,7 @@ struct rmb_triage {
   bool have_grad = false;
   // scratch
   Buf<uint16_t> xw, xf, P, h1bf_w, h1bf_f, dHbf, dEbf, dH1bf, prof;
-  Buf<float> h1_w, h1_f, e_w, e_f, Hp, logits, g, dH, dP, dEw, dEf, dH1, loss;
+  Buf<float> h1_w, h1_f, e_w, e_f, Hp, logits, g, dH, dP, dEw, dEf, dH1, loss, cosr, norms;
   Buf<uint8_t> labels, seq;
   Buf<uint64_t> starts;
   Buf<uint32_t> lens;
@@ -177,6 +178,8 @@ void run(rmb_triage* t, const uint16_t* hxw, uint32_t nw, const uint16_t* hxf, u
   uint16_t* P = t->P.get(np * kEmb);
   float* Hp = t->Hp.get(np * kHead);
   float* logits = t->logits.get(np);
+  float* cosr = t->cosr.get(np);
+  ck(launch_cosine(xw, (int)nw, xf, (int)nf, t->norms.get((size_t)nw + nf), cosr, t->st), "cosine");
   ck(launch_pair_product(ew, ef, (int)nw, (int)nf, P, t->st), "pair product");
   ck(launch_gemm(false, true, (int)np, kHead, kEmb, P, kEmb, t->wbf + Layout::wh1(), kEmb, Hp, kHead, false, nullptr,
                  t->st), "gemm wh1");
@@ -187,9 +190,12 @@ void run(rmb_triage* t, const uint16_t* hxw, uint32_t nw, const uint16_t* hxf, u
   if (hlab) {
     lab = t->labels.get(np);
     ck(hipMemcpyAsync(lab, hlab, np, hipMemcpyHostToDevice, t->st), "copy labels");
-    size_t npos = 0;
-    for (size_t p = 0; p < np; ++p) npos += hlab[p] ? 1 : 0;
-    wsum = (float)npos * pos_weight + (float)(np - npos);
+    size_t npos = 0, nneg = 0;
+    for (size_t p = 0; p < np; ++p) {
+      npos += hlab[p] == 1 ? 1 : 0;
+      nneg += hlab[p] == 0 ? 1 : 0;
+    }
+    wsum = std::max(1.f, (float)npos * pos_weight + (float)nneg);
     g = t->g.get(np);
     dH = t->dH.get(np * kHead);
     dHbf = t->dHbf.get(np * kHead);
@@ -197,7 +203,7 @@ void run(rmb_triage* t, const uint16_t* hxw, uint32_t nw, const uint16_t* hxf, u
     ck(hipMemsetAsync(t->grad, 0, Layout::TOTAL * sizeof(float), t->st), "zero grad");
     ck(hipMemsetAsync(loss, 0, sizeof(float), t->st), "zero loss");
   }
-  ck(launch_head(Hp, ew, ef, (int)nw, (int)nf, t->w + Layout::hp(), lab, pos_weight, wsum, logits, g, dH, dHbf,
+  ck(launch_head(Hp, ew, ef, cosr, (int)nw, (int)nf, t->w + Layout::hp(), lab, pos_weight, wsum, logits, g, dH, dHbf,
                  t->grad + Layout::hp() + kHead, loss, t->st), "head");
   if (hlab) {
     // head backward
@@ -233,6 +239,7 @@ void init_params(rmb_triage* t, uint64_t seed) {
   for (int k = 0; k < kHead; ++k) h[Layout::hp() + kHead + k] = uni(ao);
   h[Layout::hp() + 2 * kHead] = -2.f;  // bh2: start near a 12% prior
   h[Layout::hp() + 2 * kHead + 1] = 0.f;  // a
+  h[Layout::hp() + 2 * kHead + 2] = 0.f;  // c
   rmb_triage_set_params(t, h.data());
 }
 
@@ -276,7 +283,7 @@ int rmb_triage_destroy(rmb_triage* t) {
     if (p) (void)hipFree(p);
   t->xw.release(); t->xf.release(); t->P.release(); t->h1bf_w.release(); t->h1bf_f.release(); t->dHbf.release();
   t->dEbf.release(); t->dH1bf.release(); t->prof.release(); t->h1_w.release(); t->h1_f.release(); t->e_w.release();
-  t->e_f.release(); t->Hp.release(); t->logits.release(); t->g.release(); t->dH.release(); t->dP.release();
+  t->e_f.release(); t->cosr.release(); t->norms.release(); t->Hp.release(); t->logits.release(); t->g.release(); t->dH.release(); t->dP.release();
   t->dEw.release(); t->dEf.release(); t->dH1.release(); t->loss.release(); t->labels.release(); t->seq.release();
   t->starts.release(); t->lens.release();
   if (t->st) (void)hipStreamDestroy(t->st);

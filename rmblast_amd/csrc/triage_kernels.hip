@@ -5,7 +5,8 @@
 // Model (two towers, dot-plus-two-layer head), per (window w, family f):
 //   e_w = W2w relu(W1w x_w + b1w) + b2w          x: log1p canonical 6-mer counts
 //   e_f = W2f relu(W1f x_f + b1f) + b2f          (2080 -> 512 -> 256 per tower)
-//   logit = a * <e_w, e_f> + wh2 . relu(Wh1 (e_w * e_f) + bh1) + bh2    (256 -> 64 -> 1)
+//   logit = a * <e_w, e_f> + c * cos(x_w, x_f) + wh2 . relu(Wh1 (e_w * e_f) + bh1) + bh2
+// (head 256 -> 64 -> 1; cos is the cosine of the raw profiles)
 // Matrices are held as FP32 masters and BF16 copies; GEMM operands are BF16,
 // accumulation and everything else FP32.
 #include <hip/hip_runtime.h>
@@ -170,25 +171,27 @@ __global__ void k_pair_product(const float* __restrict__ Ew, const float* __rest
   P[t] = f2bf(Ew[(size_t)i * kEmb + d] * Ef[(size_t)j * kEmb + d]);
 }
 
-// Head, one thread per pair.  Hp holds Wh1 (e_w * e_f) (no bias).  Writes the
-// logit; with labels also the loss terms and the gradients: g = dL/dlogit,
-// dH (pre-activation gradient of the hidden layer, FP32 and BF16) and the
-// workgroup's partial sums of dwh2, dbh1 is left to a column sum of dH, and
-// dbh2 / da.  Loss: weighted logistic loss, weights pos_weight for positives
-// and 1 for negatives, normalised by wsum.
+// Head, one thread per pair.  Hp holds Wh1 (e_w * e_f) (no bias), cosr the
+// raw-profile cosine.  Writes the logit; with labels also the loss terms and
+// the gradients: g = dL/dlogit, dH (pre-activation gradient of the hidden
+// layer, FP32 and BF16) and the workgroup's partial sums of dwh2, dbh2, da and
+// dc (dbh1 is left to a column sum of dH).  Loss: weighted logistic loss,
+// weight pos_weight for label 1, 1 for label 0, 0 for label 2 (ignored),
+// normalised by wsum.
 __global__ __launch_bounds__(256) void k_head(const float* __restrict__ Hp, const float* __restrict__ Ew,
-                                              const float* __restrict__ Ef, int nw, int nf,
-                                              const float* __restrict__ hp,  // bh1[64], wh2[64], bh2, a
+                                              const float* __restrict__ Ef, const float* __restrict__ cosr,
+                                              int nw, int nf,
+                                              const float* __restrict__ hp,  // bh1[64], wh2[64], bh2, a, c
                                               const uint8_t* __restrict__ labels, float pos_weight, float wsum,
                                               float* __restrict__ logits, float* __restrict__ g_out,
                                               float* __restrict__ dH, uint16_t* __restrict__ dHbf,
-                                              float* __restrict__ grad_hp,  // dwh2[64], dbh2, da (accumulated)
+                                              float* __restrict__ grad_hp,  // dwh2[64], dbh2, da, dc (accumulated)
                                               float* __restrict__ loss) {
   __shared__ float s_dwh2[kHead];
-  __shared__ float s_misc[2];
+  __shared__ float s_misc[3];
   __shared__ float s_loss;
   for (int k = threadIdx.x; k < kHead; k += blockDim.x) s_dwh2[k] = 0.f;
-  if (threadIdx.x < 2) s_misc[threadIdx.x] = 0.f;
+  if (threadIdx.x < 3) s_misc[threadIdx.x] = 0.f;
   if (threadIdx.x == 0) s_loss = 0.f;
   __syncthreads();
   const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -197,10 +200,10 @@ __global__ __launch_bounds__(256) void k_head(const float* __restrict__ Hp, cons
     const int i = (int)(p / nf), j = (int)(p % nf);
     const float* bh1 = hp;
     const float* wh2 = hp + kHead;
-    const float bh2 = hp[2 * kHead], a = hp[2 * kHead + 1];
+    const float bh2 = hp[2 * kHead], a = hp[2 * kHead + 1], c = hp[2 * kHead + 2];
     float dot = 0.f;
     for (int d = 0; d < kEmb; ++d) dot += Ew[(size_t)i * kEmb + d] * Ef[(size_t)j * kEmb + d];
-    float z = bh2 + a * dot;
+    float z = bh2 + a * dot + c * cosr[p];
     const float* h = Hp + p * kHead;
     for (int k = 0; k < kHead; ++k) {
       const float hk = h[k] + bh1[k];
@@ -208,8 +211,8 @@ __global__ __launch_bounds__(256) void k_head(const float* __restrict__ Hp, cons
     }
     logits[p] = z;
     if (labels) {
-      const float y = labels[p] ? 1.f : 0.f;
-      const float wt = labels[p] ? pos_weight : 1.f;
+      const float y = labels[p] == 1 ? 1.f : 0.f;
+      const float wt = labels[p] == 1 ? pos_weight : (labels[p] == 0 ? 1.f : 0.f);
       // softplus(z) - y z, computed stably
       const float l = fmaxf(z, 0.f) - z * y + log1pf(expf(-fabsf(z)));
       const float sg = 1.f / (1.f + expf(-z));
@@ -218,6 +221,7 @@ __global__ __launch_bounds__(256) void k_head(const float* __restrict__ Hp, cons
       atomicAdd(&s_loss, wt * l / wsum);
       atomicAdd(&s_misc[0], g);        // dbh2
       atomicAdd(&s_misc[1], g * dot);  // da
+      atomicAdd(&s_misc[2], g * cosr[p]);  // dc
       for (int k = 0; k < kHead; ++k) {
         const float hk = h[k] + bh1[k];
         const float act = hk > 0.f ? hk : 0.f;
@@ -234,9 +238,31 @@ __global__ __launch_bounds__(256) void k_head(const float* __restrict__ Hp, cons
     if (threadIdx.x == 0) {
       atomicAdd(&grad_hp[kHead], s_misc[0]);
       atomicAdd(&grad_hp[kHead + 1], s_misc[1]);
+      atomicAdd(&grad_hp[kHead + 2], s_misc[2]);
       atomicAdd(loss, s_loss);
     }
   }
+}
+
+// Row norms of BF16 profile rows.
+__global__ void k_row_norm(const uint16_t* __restrict__ X, int n, float* __restrict__ out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  float s = 0.f;
+  for (int k = 0; k < kKmerDim; ++k) {
+    const float x = __uint_as_float((uint32_t)X[(size_t)r * kKmerDim + k] << 16);
+    s += x * x;
+  }
+  out[r] = sqrtf(s);
+}
+
+// cos[i nf + j] = D[i][j] / (|x_w[i]| |x_f[j]|), in place on D
+__global__ void k_cosine(float* __restrict__ D, const float* __restrict__ nw_norm, const float* __restrict__ nf_norm,
+                         int nw, int nf) {
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (size_t)nw * nf) return;
+  const float den = nw_norm[p / nf] * nf_norm[p % nf];
+  D[p] = den > 0.f ? D[p] / den : 0.f;
 }
 
 // dE_w[i][d] = sum_j (dP[i nf + j][d] + a g[i nf + j]) e_f[j][d], and the
@@ -339,13 +365,24 @@ hipError_t launch_pair_product(const float* Ew, const float* Ef, int nw, int nf,
   return hipGetLastError();
 }
 
-hipError_t launch_head(const float* Hp, const float* Ew, const float* Ef, int nw, int nf, const float* hp,
-                       const uint8_t* labels, float pos_weight, float wsum, float* logits, float* g, float* dH,
-                       uint16_t* dHbf, float* grad_hp, float* loss, hipStream_t st) {
+hipError_t launch_head(const float* Hp, const float* Ew, const float* Ef, const float* cosr, int nw, int nf,
+                       const float* hp, const uint8_t* labels, float pos_weight, float wsum, float* logits, float* g,
+                       float* dH, uint16_t* dHbf, float* grad_hp, float* loss, hipStream_t st) {
   const size_t n = (size_t)nw * nf;
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_head, dim3(nblk(n)), dim3(256), 0, st, Hp, Ew, Ef, nw, nf, hp, labels, pos_weight, wsum,
-                     logits, g, dH, dHbf, grad_hp, loss);
+  hipLaunchKernelGGL(k_head, dim3(nblk(n)), dim3(256), 0, st, Hp, Ew, Ef, cosr, nw, nf, hp, labels, pos_weight,
+                     wsum, logits, g, dH, dHbf, grad_hp, loss);
+  return hipGetLastError();
+}
+
+hipError_t launch_cosine(const uint16_t* Xw, int nw, const uint16_t* Xf, int nf, float* norms, float* D,
+                         hipStream_t st) {
+  if (!nw || !nf) return hipSuccess;
+  hipError_t e = launch_gemm(false, true, nw, nf, kKmerDim, Xw, kKmerDim, Xf, kKmerDim, D, nf, false, nullptr, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_row_norm, dim3(nblk((size_t)nw)), dim3(256), 0, st, Xw, nw, norms);
+  hipLaunchKernelGGL(k_row_norm, dim3(nblk((size_t)nf)), dim3(256), 0, st, Xf, nf, norms + nw);
+  hipLaunchKernelGGL(k_cosine, dim3(nblk((size_t)nw * nf)), dim3(256), 0, st, D, norms, norms + nw, nw, nf);
   return hipGetLastError();
 }
 

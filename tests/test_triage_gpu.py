@@ -61,8 +61,10 @@ def test_forward_backward_matches_oracle(model):
     xw = T.bf16_bits(np.log1p(rng.poisson(1.0, (nw, T.KMER_DIM))).astype(np.float32))
     xf = T.bf16_bits(np.log1p(rng.poisson(1.5, (nf, T.KMER_DIM))).astype(np.float32))
     lab = (rng.random(nw * nf) < 0.2).astype(np.uint8)
+    lab[::7] = 2  # ignored pairs
     flat = T.init_params(4) * 0.5
     flat[T.layout()[0]["a"][0]] = 0.05
+    flat[T.layout()[0]["c"][0]] = 0.7
     model.set_params(flat)
     loss, grads = model.forward_backward(xw, xf, lab, pos_weight=4.0, want_grads=True)
     logits = model.logits(xw, xf).reshape(-1)
@@ -149,3 +151,23 @@ def test_pruned_search_with_true_labels_recovers_the_full_search(small_instance)
     recall = len(full_keys & keys) / max(1, len(full_keys))
     assert recall >= 0.99, recall
     assert n_pairs > 0
+
+
+def test_trained_network_gains_over_cosine_only():
+    """The method without the network (a logistic fit on the cosine of the
+    profiles) against the trained network warm-started from it: on held-out
+    genomes the network's pair loss is lower."""
+    from rmblast_amd import triage as tr
+    searcher = tr.make_searcher()
+    model = tr.Triage(seed=5)
+    train_set = [tr.make_instance(searcher, model, 500 + k) for k in range(8)]
+    val_set = [tr.make_instance(searcher, model, 600 + k) for k in range(2)]
+    th = tr.fit_cosine_logistic(train_set, 4.0)
+    tr.warm_start_from_cosine(model, th)
+    tr.train(model, train_set, epochs=10, lr=1e-3, pos_weight=4.0, val=val_set)
+    cos_val = np.mean([tr._weighted_logistic_loss(th[0] * tr.cosine_scores(v) + th[1], v.train_labels, 4.0)
+                       for v in val_set])
+    net_val = np.mean([tr._weighted_logistic_loss(model.logits(v.xw, v.xf).astype(np.float64), v.train_labels, 4.0)
+                       for v in val_set])
+    model.close()
+    assert net_val < cos_val, (net_val, cos_val)

@@ -62,8 +62,10 @@ def _batch(seed, nw=3, nf=2):
     xf = np.log1p(rng.poisson(1.0, (nf, T.KMER_DIM))).astype(np.float64)
     lab = (rng.random(nw * nf) < 0.4).astype(np.uint8)
     lab[0] = 1
+    lab[-1] = 2  # ignored
     flat = T.init_params(seed).astype(np.float64)
     flat[T.layout()[0]["a"][0]] = 0.01
+    flat[T.layout()[0]["c"][0]] = 0.3
     return flat, xw, xf, lab
 
 
