@@ -48,8 +48,10 @@ extern "C" {
 #define RMB_ERR_CAPACITY 4  /* a device work limit was exceeded; message names it */
 
 /* Search parameters: the rmblastn options in scope (rmblast-lib/src/options.rs:12-104,
- * rmblastn/src/main.rs:128-281).  dust must be 0: DUST masking is done by the
- * caller on the query before the call (filter/dust.rs), or not at all. */
+ * rmblastn/src/main.rs:128-281).  dust = 1 applies symmetric DUST (window 64,
+ * level 20, linker 1) to each query on the host, as build_query_lookup does
+ * (engine.rs:606-619): the lookup table and the exact word extension read the
+ * masked query, ungapped and gapped extension the query itself. */
 typedef struct rmb_params {
   int32_t gap_open;
   int32_t gap_extend;
@@ -121,6 +123,12 @@ int rmb_search(rmb_ctx* ctx, uint32_t n_queries, const uint8_t* const* queries, 
  * rmb_search, in (query, subject) order. */
 int rmb_search_pairs(rmb_ctx* ctx, uint32_t n_queries, const uint8_t* const* queries, const uint32_t* lens,
                      uint64_t n_pairs, const uint32_t* pair_query, const uint32_t* pair_subject);
+
+/* DUST intervals of BLASTNA bases (no sentinels): [first, last] pairs, 0-based
+ * inclusive, into out (2 x *n_out entries on input); *n_out receives the
+ * count.  Pass out = NULL to get the count.  Replaces dust_mask / the BLAST
+ * filter path of rmblast-lib/src/filter/dust.rs:339-484.  Host only. */
+int rmb_dust_intervals(const uint8_t* bases, uint64_t n, uint64_t* out, uint64_t* n_out);
 
 /* Two-call result retrieval: sizes, then copy into caller buffers. */
 int rmb_results_size(const rmb_ctx* ctx, uint64_t* n_hsps, uint64_t* n_ops);

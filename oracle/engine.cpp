@@ -880,7 +880,7 @@ std::vector<AlignResult> search_one_subject(const QueryLookup& ql, const std::ve
   const uint32_t s_len = subj.plus.size() >= 2 ? (uint32_t)subj.plus.size() - 2 : 0;
   if (q_len < (uint32_t)p.word_size || s_len < (uint32_t)p.word_size) return {};
   std::vector<UngappedRecord> ung;
-  collect_ungapped(ql, query, query_rc, query, subj, p, m, ung, taps);
+  collect_ungapped(ql, query, query_rc, ql.masked.empty() ? query : ql.masked, subj, p, m, ung, taps);
   sort_ungapped_like_ncbi(ung, q_len, s_len);
   if (taps && taps->ungapped) *taps->ungapped = ung;
   if (ung.empty()) return {};

@@ -5,6 +5,7 @@
 // buffer [fwd | N | rc]).  Both tables index every lut_width window of a
 // location that contains no code >= 4; a location shorter than word_length is
 // skipped (nalookup.rs:368-371, mblookup.rs:389-392).
+#include "../rmblast_amd/csrc/rmb_dust.h"
 #include "oracle.hpp"
 
 #include <algorithm>
@@ -64,8 +65,13 @@ static Lookup build_table(const std::vector<uint8_t>& seq, const std::vector<std
   return L;
 }
 
-QueryLookup build_query_lookup(const std::vector<uint8_t>& q, const Params& p) {  // engine.rs:656-662
+QueryLookup build_query_lookup(const std::vector<uint8_t>& query, const Params& p) {  // engine.rs:606-662
   QueryLookup ql;
+  if (p.dust && query.size() > 2) {  // filter/dust.rs:462-484, mask value 14
+    ql.masked = query;
+    rmb::dust_mask(ql.masked.data() + 1, ql.masked.size() - 2);
+  }
+  const std::vector<uint8_t>& q = ql.masked.empty() ? query : ql.masked;
   size_t n = q.size() >= 2 ? q.size() - 2 : 0;
   ql.fwd_n = (uint32_t)n;
   ql.c1_start = (uint32_t)n + 1;

@@ -45,7 +45,6 @@ int main(int argc, char** argv) {
   p.mask_level = ml < 0 ? 101 : ml;
   std::string dust = get("--dust", "yes");
   p.dust = (dust == "yes" || dust == "true");
-  if (p.dust) { fprintf(stderr, "rmblastn_oracle: --dust yes is not implemented in the oracle\n"); return 1; }
   int threads = std::stoi(get("--num-threads", "1"));
   std::string dump = get("--dump", "");
   std::vector<std::string> fields;

@@ -91,8 +91,11 @@ int choose_lut_width(int word_size, size_t approx_entries);   // nalookup.rs:254
 struct QueryLookup {                                           // engine.rs:579-690
   Lookup lut;
   uint32_t fwd_n = 0, c1_start = 0;
+  std::vector<uint8_t> masked;  // DUST-masked query (sentinel-wrapped); empty with dust off
 };
-QueryLookup build_query_lookup(const std::vector<uint8_t>& masked_query, const Params& p);
+// engine.rs:606-619: with dust on, the table is built from the DUST-masked
+// query and exact word extension reads it; extensions use the query itself.
+QueryLookup build_query_lookup(const std::vector<uint8_t>& query, const Params& p);
 
 // ---- core kernels ------------------------------------------------------------
 struct Ungapped { int32_t score; uint32_t q_start, q_end, s_start, s_end; };

@@ -20,7 +20,7 @@ CLI_PATH = os.path.join(_HERE, "_build", "rmblastn")
 
 EXPORTED = [
     "rmb_matrix_load", "rmb_ctx_create", "rmb_ctx_destroy", "rmb_last_error", "rmb_library_load",
-    "rmb_search", "rmb_search_pairs", "rmb_results_size", "rmb_results_copy", "rmb_stats", "rmb_tap_ungapped",
+    "rmb_search", "rmb_search_pairs", "rmb_dust_intervals", "rmb_results_size", "rmb_results_copy", "rmb_stats", "rmb_tap_ungapped",
     "rmb_tap_align_score_only", "rmb_tap_align_ex",
 ]
 
@@ -68,6 +68,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rmb_results_size.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), P(ctypes.c_uint64)]
     lib.rmb_results_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     lib.rmb_stats.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), ctypes.c_int]
+    lib.rmb_dust_intervals.argtypes = [u8p, ctypes.c_uint64, P(ctypes.c_uint64), P(ctypes.c_uint64)]
     lib.rmb_tap_ungapped.argtypes = [ctypes.c_void_p, u8p, ctypes.c_uint32, ctypes.c_uint32,
                                      P(ctypes.c_uint32), P(ctypes.c_uint64)]
     lib.rmb_tap_align_score_only.argtypes = [ctypes.c_void_p, u8p, ctypes.c_uint32, u8p, ctypes.c_uint32,
@@ -94,6 +95,19 @@ def load_matrix(path: str) -> RmbMatrix:
     if lib().rmb_matrix_load(path.encode(), ctypes.byref(m), err, 256):
         raise RmbError(f"matrix {path}: {err.value.decode()}")
     return m
+
+
+def dust_intervals(bases) -> list:
+    """Symmetric DUST intervals [(first, last), ...] (0-based, inclusive) of
+    BLASTNA bases without sentinels (host code of the library)."""
+    b = np.ascontiguousarray(bases, dtype=np.uint8)
+    n = ctypes.c_uint64(0)
+    ptr = b.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    if lib().rmb_dust_intervals(ptr, len(b), None, ctypes.byref(n)):
+        raise RmbError("rmb_dust_intervals failed")
+    out = np.zeros(2 * max(n.value, 1), dtype=np.uint64)
+    lib().rmb_dust_intervals(ptr, len(b), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(n))
+    return [(int(out[2 * k]), int(out[2 * k + 1])) for k in range(n.value)]
 
 
 def _u8ptr(a: np.ndarray):

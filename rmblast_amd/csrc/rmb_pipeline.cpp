@@ -30,6 +30,7 @@
 
 #include "../../include/rmblast_hip.h"
 #include "rmb_device.h"
+#include "rmb_dust.h"
 #include "rmb_replay.h"
 
 namespace rmb {
@@ -211,7 +212,7 @@ struct rmb_ctx {
   DevBuf b_qfwd, b_qrc, b_recs, b_elem, b_keys, b_keys2, b_vals, b_vals2, b_counts, b_cells, b_sort_tmp;
   DevBuf b_hits, b_hitcount, b_spill, b_hitres, b_queue, b_counters, b_err;
   DevBuf b_tasks, b_dpres, b_ring, b_ring_big, b_tbtasks, b_tbres, b_flat, b_rows, b_ops;
-  DevBuf b_pre, b_redo, b_precount, b_packtasks, b_packed, b_items;
+  DevBuf b_pre, b_redo, b_precount, b_packtasks, b_packed, b_items, b_qext;
 };
 
 namespace rmb {
@@ -490,8 +491,9 @@ static void prelim_group(rmb_ctx* c, const std::vector<RecDesc>& recs, const std
 // Phase 1 for a group of records sharing one lookup width; with pre_out, also
 // the preliminary DPs of its hits.
 static void phase1_group(rmb_ctx* c, const std::vector<uint32_t>& recs_idx, const std::vector<RecDesc>& all_recs,
-                         int lw, const uint8_t* d_qfwd, const uint8_t* d_qrc, std::vector<UngHit>* hits_out,
-                         std::vector<PreHit>* pre_out, const std::vector<uint32_t>* items = nullptr) {
+                         int lw, const uint8_t* d_qfwd, const uint8_t* d_qrc, const uint8_t* d_qext,
+                         std::vector<UngHit>* hits_out, std::vector<PreHit>* pre_out,
+                         const std::vector<uint32_t>* items = nullptr) {
   const int ws = c->P.word_size;
   const uint32_t R = (uint32_t)recs_idx.size();
   const uint64_t cells = 1ull << (2 * lw);
@@ -520,7 +522,7 @@ static void phase1_group(rmb_ctx* c, const std::vector<uint32_t>& recs_idx, cons
   int end_bit = 2 * lw;
   while ((1ull << (end_bit - 2 * lw)) <= R) ++end_bit;
   const int desc = lw > 8 ? 1 : 0;
-  hip_check(launch_lut_keys(drecs, R, delem, d_qfwd, lw, ws, desc, keys, vals, counts, invalid, E, c->st), "lut keys");
+  hip_check(launch_lut_keys(drecs, R, delem, d_qext, lw, ws, desc, keys, vals, counts, invalid, E, c->st), "lut keys");
   size_t tmp1 = 0, tmp2 = 0;
   hip_check(sort_pairs_u64(nullptr, tmp1, keys, keys2, vals, vals2, E, end_bit, c->st), "sort size");
   hip_check(exclusive_sum_u32(nullptr, tmp2, counts, cstart, ncell, c->st), "scan size");
@@ -569,7 +571,7 @@ static void phase1_group(rmb_ctx* c, const std::vector<uint32_t>& recs_idx, cons
     hip_check(hipMemsetAsync(hitcount, 0, sizeof(unsigned long long), c->st), "memset hitcount");
     hip_check(hipMemsetAsync(err, 0, sizeof(uint32_t), c->st), "memset err");
     hip_check(hipEventRecord(c->ev0, c->st), "event");
-    hip_check(launch_phase1(blocks, P, drecs, (const SubjDesc*)c->d_sdesc.p, d_qfwd, d_qrc, d_qfwd,
+    hip_check(launch_phase1(blocks, P, drecs, (const SubjDesc*)c->d_sdesc.p, d_qfwd, d_qrc, d_qext,
                             (const uint8_t*)c->d_splus.p, (const uint8_t*)c->d_src.p, cstart, vals2, c->d_mat, dh,
                             hitcount, spill, hitres, queue, ctr, err, c->st),
               "k_phase1");
@@ -612,6 +614,21 @@ static void phase1_group(rmb_ctx* c, const std::vector<uint32_t>& recs_idx, cons
   throw Failure(RMB_ERR_CAPACITY, "ungapped hit buffer could not be sized");
 }
 
+// DUST-masked copy of the query pool (filter/dust.rs:462-484: masked bases set
+// to 14), uploaded to the context's masked pool.
+static const uint8_t* upload_dusted(rmb_ctx* c, const std::vector<std::vector<uint8_t>>& qf,
+                                    const std::vector<RecDesc>& recs, uint64_t off) {
+  std::vector<uint8_t> pm(off);
+  parallel_for(qf.size(), [&](size_t i) {
+    uint8_t* dst = pm.data() + recs[i].q_base;
+    std::memcpy(dst, qf[i].data(), qf[i].size());
+    if (recs[i].q_len) dust_mask(dst + 1, recs[i].q_len);
+  });
+  uint8_t* d = c->b_qext.get<uint8_t>(off);
+  hip_check(hipMemcpy(d, pm.data(), off, hipMemcpyHostToDevice), "copy masked query");
+  return d;
+}
+
 // pairs: null for every (query, subject) pair, else the item ids q * nsub + s to search
 static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const uint32_t* lens,
                         const std::vector<uint64_t>* pairs = nullptr) {
@@ -645,6 +662,11 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
     hip_check(hipMemcpy(d_qfwd, pf.data(), off, hipMemcpyHostToDevice), "copy qfwd");
     hip_check(hipMemcpy(d_qrc, pr.data(), off, hipMemcpyHostToDevice), "copy qrc");
   }
+  // With DUST on, the lookup tables and the exact word extension read the
+  // masked query; ungapped and gapped extension read the query itself
+  // (engine.rs:606-619).
+  const uint8_t* d_qext = d_qfwd;
+  if (c->P.dust) d_qext = upload_dusted(c, qf, recs, off);
   unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
   hip_check(hipMemset(ctr, 0, C_NCOUNTERS * sizeof(unsigned long long)), "memset counters");
   // ---- Phase 1 by lookup width; groups are bounded by the cell table size
@@ -667,9 +689,9 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
             const uint32_t q = (uint32_t)(it / nsub), f = (uint32_t)(it % nsub);
             if (pos[q] != UINT32_MAX) local.push_back(pos[q] * nsub + f);
           }
-          phase1_group(c, grp, recs, lw, d_qfwd, d_qrc, nullptr, &survivors, &local);
+          phase1_group(c, grp, recs, lw, d_qfwd, d_qrc, d_qext, nullptr, &survivors, &local);
         } else {
-          phase1_group(c, grp, recs, lw, d_qfwd, d_qrc, nullptr, &survivors);
+          phase1_group(c, grp, recs, lw, d_qfwd, d_qrc, d_qext, nullptr, &survivors);
         }
         grp.clear();
         elems = 0;
@@ -1014,7 +1036,7 @@ int rmb_ctx_create(int device, const rmb_params* params, const rmb_matrix* matri
   *out = nullptr;
   rmb_ctx* c = new rmb_ctx();
   int rc = guarded(c, [&]() {
-    if (params->dust) throw Failure(RMB_ERR_ARG, "dust must be applied by the caller (params.dust = 0)");
+    if (params->dust != 0 && params->dust != 1) throw Failure(RMB_ERR_ARG, "dust must be 0 or 1");
     if (params->word_size < 4) throw Failure(RMB_ERR_ARG, "word_size below 4");
     if (params->gap_extend < 1 || params->gap_open < 0) throw Failure(RMB_ERR_ARG, "bad gap costs");
     int n = 0;
@@ -1051,7 +1073,7 @@ int rmb_ctx_destroy(rmb_ctx* c) {
                     &c->b_keys, &c->b_keys2, &c->b_vals, &c->b_vals2, &c->b_counts, &c->b_cells, &c->b_sort_tmp,
                     &c->b_hits, &c->b_hitcount, &c->b_spill, &c->b_hitres, &c->b_queue, &c->b_counters, &c->b_err, &c->b_tasks,
                     &c->b_dpres, &c->b_ring, &c->b_ring_big, &c->b_tbtasks, &c->b_tbres, &c->b_flat, &c->b_rows,
-                    &c->b_ops, &c->b_pre, &c->b_redo, &c->b_precount, &c->b_packtasks, &c->b_packed, &c->b_items})
+                    &c->b_ops, &c->b_pre, &c->b_redo, &c->b_precount, &c->b_packtasks, &c->b_packed, &c->b_items, &c->b_qext})
     b->release();
   if (c->d_mat) (void)hipFree(c->d_mat);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1150,6 +1172,24 @@ int rmb_search_pairs(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const ui
   });
 }
 
+int rmb_dust_intervals(const uint8_t* bases, uint64_t n, uint64_t* out, uint64_t* n_out) {
+  if ((!bases && n) || !n_out) return RMB_ERR_ARG;
+  try {
+    const std::vector<std::pair<size_t, size_t>> iv = dust_intervals(bases, n);
+    if (out) {
+      const uint64_t cap = *n_out;
+      for (size_t k = 0; k < iv.size() && k < cap; ++k) {
+        out[2 * k] = iv[k].first;
+        out[2 * k + 1] = iv[k].second;
+      }
+    }
+    *n_out = iv.size();
+    return RMB_OK;
+  } catch (...) {
+    return RMB_ERR_DEVICE;
+  }
+}
+
 int rmb_results_size(const rmb_ctx* c, uint64_t* n_hsps, uint64_t* n_ops) {
   if (!c || !n_hsps || !n_ops) return RMB_ERR_ARG;
   *n_hsps = c->hsps.size();
@@ -1187,7 +1227,8 @@ int rmb_tap_ungapped(rmb_ctx* c, const uint8_t* query, uint32_t q_len, uint32_t 
     unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
     hip_check(hipMemset(ctr, 0, C_NCOUNTERS * sizeof(unsigned long long)), "memset counters");
     const int lw = lut_width_for(c->P.word_size, 2ull * q_len);
-    if (q_len >= (uint32_t)c->P.word_size) phase1_group(c, {0u}, recs, lw, d_qfwd, d_qrc, &hits, nullptr);
+    const uint8_t* d_qext = c->P.dust ? upload_dusted(c, {qf}, recs, qf.size()) : d_qfwd;
+    if (q_len >= (uint32_t)c->P.word_size) phase1_group(c, {0u}, recs, lw, d_qfwd, d_qrc, d_qext, &hits, nullptr);
     std::vector<UngRec> v;
     for (auto& h : hits)
       if (h.item % c->subj.size() == subject)

@@ -161,11 +161,17 @@ int main(int argc, char** argv) {
   P.min_raw_gapped_score = std::stoi(opt("--min-raw-gapped-score", "0"));
   P.ungapped_cutoff = -1;
   P.complexity_adjust = complexity;
+  // --dust yes|true|no|false (rmblastn/src/main.rs:330-342); NCBI's
+  // 'level window linker' form is rejected
   const std::string dust = opt("--dust", "yes");
-  if (dust != "no" && dust != "false") {
-    fprintf(stderr, "error: --dust yes is not available in this build; pass --dust no\n");
+  int dust_on = 0;
+  if (dust == "yes" || dust == "true") {
+    dust_on = 1;
+  } else if (dust != "no" && dust != "false") {
+    fprintf(stderr, "error: --dust '%s': expected yes, no, true or false\n", dust.c_str());
     return 1;
   }
+  P.dust = dust_on;
   int mask_level = std::stoi(opt("--mask-level", "-1"));
   if (mask_level < 0) mask_level = 101;
   std::vector<std::string> fields;

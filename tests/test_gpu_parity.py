@@ -196,3 +196,53 @@ def test_tap_dp_kernels_match_oracle(oracle_lib):
             sc, al, bl, gops = s.tap_align_ex(a, b, x, rev)
             assert (sc, al, bl) == tuple(o[:3]), (trial, rev)
             assert gops == [(int(ops[2 * k]), int(ops[2 * k + 1])) for k in range(n)], (trial, rev)
+
+
+# the NCBI goldens that run with DUST on (cli.rs:147-299), through the product CLI
+GPU_DUST_GOLDENS = [
+    ("rm_defaults", "query.fa", "db.fa", "comparison.matrix", ["--mask-level", "80", "--complexity-adjust"]),
+    ("query_dash", "query_dash.fa", "db.fa", "comparison.matrix", ["--mask-level", "80", "--complexity-adjust"]),
+    ("db_dash", "query.fa", "db_dash.fa", "comparison.matrix", ["--mask-level", "80", "--complexity-adjust"]),
+    ("asym_freqs", "query.fa", "db.fa", "asym_freqs.matrix", ["--mask-level", "101", "--complexity-adjust"]),
+]
+
+
+@pytest.mark.parametrize("name,query,db,matrix,extra", GPU_DUST_GOLDENS)
+def test_dust_goldens_on_gpu(name, query, db, matrix, extra):
+    g = run([CLI, "--db", golden(db), "--query", golden(query), "--matrix", golden(matrix), "--word-size", "14",
+             *RM_SEARCH, "--outfmt", RM_FIELDS, *extra, "--dust", "yes"])
+    assert g.returncode == 0, g.stderr
+    assert g.stdout == open(golden(f"expected_{name}.ncbi-2.17.1.tsv")).read()
+
+
+def test_ncbi_defaults_golden_on_gpu():
+    g = run([CLI, "--db", golden("db.fa"), "--query", golden("query.fa"), "--matrix", golden("comparison.matrix"),
+             "--gapopen", "20", "--gapextend", "5", "--outfmt", RM_FIELDS])
+    assert g.returncode == 0, g.stderr
+    assert g.stdout == open(golden("expected_defaults.ncbi-2.17.1.tsv")).read()
+
+
+def test_dust_on_synthetic_matches_oracle(tmp_path):
+    # word size 7 with DUST on the parity-configuration generator, with
+    # microsatellites so DUST has something to mask
+    from rmblast_amd import synth
+    w = synth.parity_config(seed=13, genome_len=150_000, n_families=15, n_copies=300)
+    q = list(w.records[0][1])
+    rng = np.random.default_rng(3)
+    for k in range(40):  # (CA)n, (A)n and (TTG)n tracts
+        at = int(rng.integers(0, len(q) - 200))
+        unit = [[1, 0], [0], [3, 3, 2]][k % 3]
+        for i in range(int(rng.integers(30, 150))):
+            q[at + i] = unit[i % len(unit)]
+    qp, dp, mp = tmp_path / "q.fa", tmp_path / "db.fa", tmp_path / "m.matrix"
+    synth.to_fasta([(w.records[0][0], np.array(q, dtype=np.uint8))], str(qp))
+    synth.to_fasta(w.families, str(dp))
+    mp.write_text(synth.MATRIX_TEXT)
+    args = ["--db", str(dp), "--query", str(qp), "--matrix", str(mp), "--gapopen", "20", "--gapextend", "5",
+            "--word-size", "7", "--min-raw-gapped-score", "200", "--mask-level", "80", "--complexity-adjust",
+            "--dust", "yes", "--outfmt", RM_FIELDS]
+    g = run([CLI] + args)
+    o = run([ORACLE_BIN, "--num-threads", "8"] + args, timeout=1200)
+    assert g.returncode == 0 and o.returncode == 0, (g.stderr, o.stderr)
+    assert g.stdout == o.stdout
+    assert g.stdout.count("\n") > 50

@@ -1,8 +1,10 @@
 """The CPU oracle against the reference's NCBI RMBlast 2.17.1 goldens.
 
 rmblastn/tests/cli.rs:107-300 produced expected_*.ncbi-2.17.1.tsv with real
-NCBI rmblastn.  The oracle has no DUST filter, so only the dust-free golden
-(nomask_nodust) is an end-to-end pin; the others run with `-dust yes`.
+NCBI rmblastn: mask level off and dust off, RepeatMasker's options with DUST,
+'-' in either sequence, a matrix with asymmetric frequencies, and NCBI's
+defaults (word size 11).  The multi-chunk chr22 golden needs the chunked
+query path, which is not built.
 """
 import os
 
@@ -51,3 +53,33 @@ def test_thread_count_does_not_change_output():
             "--word-size", "7", *RM_SEARCH, "--outfmt", RM_FIELDS, "--mask-level", "80", "--complexity-adjust",
             "--dust", "no"]
     assert oracle(base + ["--num-threads", "1"]) == oracle(base + ["--num-threads", "4"])
+
+
+# cli.rs:147-299: the goldens that run with DUST on
+DUST_GOLDENS = [
+    ("rm_defaults", "query.fa", "db.fa", "comparison.matrix", ["--mask-level", "80", "--complexity-adjust"], 33),
+    ("query_dash", "query_dash.fa", "db.fa", "comparison.matrix", ["--mask-level", "80", "--complexity-adjust"], 24),
+    ("db_dash", "query.fa", "db_dash.fa", "comparison.matrix", ["--mask-level", "80", "--complexity-adjust"], 34),
+    ("asym_freqs", "query.fa", "db.fa", "asym_freqs.matrix", ["--mask-level", "101", "--complexity-adjust"], 181),
+]
+
+
+def dust_golden_args(query, db, matrix, extra):
+    return ["--db", golden(db), "--query", golden(query), "--matrix", golden(matrix), "--word-size", "14",
+            *RM_SEARCH, "--outfmt", RM_FIELDS, *extra, "--dust", "yes"]
+
+
+@pytest.mark.parametrize("name,query,db,matrix,extra,rows", DUST_GOLDENS)
+def test_dust_goldens_match_ncbi(name, query, db, matrix, extra, rows):
+    out = oracle(dust_golden_args(query, db, matrix, extra))
+    want = open(golden(f"expected_{name}.ncbi-2.17.1.tsv")).read()
+    assert out.count("\n") == rows
+    assert out == want
+
+
+def test_ncbi_defaults_match():
+    # cli.rs:293-300: only the gap costs given; word size 11, dust on, no mask level
+    out = oracle(["--db", golden("db.fa"), "--query", golden("query.fa"), "--matrix", golden("comparison.matrix"),
+                  "--gapopen", "20", "--gapextend", "5", "--outfmt", RM_FIELDS])
+    assert out.count("\n") == 350
+    assert out == open(golden("expected_defaults.ncbi-2.17.1.tsv")).read()
