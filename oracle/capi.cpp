@@ -153,3 +153,34 @@ extern "C" void orc_improve_seed(const uint8_t* q, size_t qn, const uint8_t* s, 
   // in6 = {q_seed, s_seed, prelim q_start, q_end, s_start, s_end}
   improve_seed(q, qn, s, sn, in6[0], in6[1], in6[2], in6[3], in6[4], in6[5], minus != 0, out2[0], out2[1]);
 }
+
+// main.rs:1150-1193: {chunk size, chunk count, step}
+extern "C" void orc_chunk_plan(uint64_t n, uint64_t* out3) {
+  const ChunkPlan c = chunk_plan((size_t)n);
+  out3[0] = c.size;
+  out3[1] = c.count;
+  out3[2] = c.step;
+}
+
+// main.rs:1016-1085; prelims as 8 u32 {q_seed, s_seed, q_start, q_end, s_start, s_end, score, strand};
+// out holds n_acc + n_new entries; returns the merged count
+extern "C" int orc_merge_chunk_prelims(const uint32_t* acc, int n_acc, const uint32_t* nw, int n_new, uint32_t split,
+                                       uint32_t* out) {
+  auto unpack = [](const uint32_t* x, int n) {
+    std::vector<PrelimHsp> v(n);
+    for (int i = 0; i < n; ++i) {
+      const uint32_t* r = x + 8 * i;
+      v[i] = PrelimHsp{r[0], r[1], r[2], r[3], r[4], r[5], (int32_t)r[6], r[7] ? MINUS : PLUS};
+    }
+    return v;
+  };
+  std::vector<PrelimHsp> a = unpack(acc, n_acc);
+  merge_chunk_prelims(a, unpack(nw, n_new), split);
+  for (size_t i = 0; i < a.size(); ++i) {
+    const PrelimHsp& h = a[i];
+    const uint32_t r[8] = {h.q_seed, h.s_seed, h.q_start, h.q_end, h.s_start, h.s_end, (uint32_t)h.score,
+                           h.strand == MINUS ? 1u : 0u};
+    std::copy(r, r + 8, out + 8 * i);
+  }
+  return (int)a.size();
+}

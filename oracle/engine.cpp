@@ -4,6 +4,7 @@
 // :1063-1309 (seeding + ungapped + sort), :1317-1762 (Phase 2a/2b/2c),
 // :1850-1895 (pivots), :1969-2083 (prelim purge), :2092-2823 (cut, reevaluate,
 // common-endpoint purge) and rmblastn/src/main.rs:1133-1234 (single-chunk driver).
+#include "../rmblast_amd/csrc/rmb_dust.h"
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -985,26 +986,169 @@ void sort_hit_list_order(std::vector<AlignResult>& r, const std::vector<std::str
   });
 }
 
+// ---------------------------------------------------------- multi-chunk ------
+// NCBI's query split (rmblastn/src/main.rs:1150-1190): initial chunks of
+// 1,000,000 bases overlapping by 100; more than one chunk from 1,999,800 bases.
+ChunkPlan chunk_plan(size_t n) {
+  const size_t kInit = 1000000, kOverlap = 100;
+  ChunkPlan c;
+  const size_t num = n / (kInit - kOverlap);
+  if (num <= 1) {
+    c.size = n;
+    c.count = 1;
+  } else {
+    size_t cs = (n + (num - 1) * kOverlap) / num;
+    if (num < (cs > kOverlap ? cs - kOverlap : 0)) ++cs;
+    c.size = cs;
+    c.count = num;
+  }
+  c.step = c.size > kOverlap ? c.size - kOverlap : 0;
+  return c;
+}
+
+// Cross-chunk merge of a later chunk's prelims into the accumulated ones
+// (main.rs:1016-1085): plus strand only; each accumulated candidate, best
+// first, absorbs every new candidate within 10 diagonals whose start or end
+// falls in its box, growing as it goes.
+void merge_chunk_prelims(std::vector<PrelimHsp>& acc, std::vector<PrelimHsp> nw, uint32_t split) {
+  const uint32_t kOverlap = 100;
+  const int64_t kDiagClose = 10;
+  auto before = [](const PrelimHsp& a, const PrelimHsp& b) {
+    if (a.score != b.score) return a.score > b.score;
+    if (a.s_start != b.s_start) return a.s_start < b.s_start;
+    if (a.s_end != b.s_end) return a.s_end > b.s_end;
+    if (a.q_start != b.q_start) return a.q_start < b.q_start;
+    return a.q_end > b.q_end;
+  };
+  std::vector<size_t> ai, ni;
+  for (size_t i = 0; i < acc.size(); ++i)
+    if (acc[i].strand == PLUS && acc[i].q_end > split) ai.push_back(i);
+  for (size_t j = 0; j < nw.size(); ++j)
+    if (nw[j].strand == PLUS && nw[j].q_start < split + kOverlap) ni.push_back(j);
+  std::stable_sort(ai.begin(), ai.end(), [&](size_t x, size_t y) { return before(acc[x], acc[y]); });
+  std::stable_sort(ni.begin(), ni.end(), [&](size_t x, size_t y) { return before(nw[x], nw[y]); });
+  std::vector<char> absorbed(nw.size(), 0);
+  for (size_t i : ai) {
+    PrelimHsp& h1 = acc[i];
+    for (size_t j : ni) {
+      if (absorbed[j]) continue;
+      const PrelimHsp& h2 = nw[j];
+      const int64_t end_diag = (int64_t)h1.q_end - h1.s_end, start_diag = (int64_t)h2.q_start - h2.s_start;
+      if (std::llabs(end_diag - start_diag) >= kDiagClose) continue;
+      const bool start_in = h2.q_start >= h1.q_start && h2.q_start <= h1.q_end && h2.s_start >= h1.s_start &&
+                            h2.s_start <= h1.s_end;
+      const bool end_in = h2.q_end >= h1.q_start && h2.q_end <= h1.q_end && h2.s_end >= h1.s_start &&
+                          h2.s_end <= h1.s_end;
+      if (!start_in && !end_in) continue;
+      const double density = (double)(h1.score + h2.score) / (double)((h1.q_end - h1.q_start) + (h2.q_end - h2.q_start));
+      h1.q_start = std::min(h1.q_start, h2.q_start);
+      h1.s_start = std::min(h1.s_start, h2.s_start);
+      h1.q_end = std::max(h1.q_end, h2.q_end);
+      h1.s_end = std::max(h1.s_end, h2.s_end);
+      if (h2.score > h1.score) {
+        h1.q_seed = h2.q_seed;
+        h1.s_seed = h2.s_seed;
+        h1.score = h2.score;
+      }
+      h1.score = std::max(h1.score, (int32_t)(density * (double)(h1.q_end - h1.q_start)));
+      absorbed[j] = 1;
+    }
+  }
+  for (size_t j = 0; j < nw.size(); ++j)
+    if (!absorbed[j]) acc.push_back(nw[j]);
+}
+
+// Phase 1 + 2a of one chunk against one subject, prelims shifted to
+// full-query coordinates (engine.rs:1775-1831, search_phase2a)
+static std::vector<PrelimHsp> chunk_phase2a(const QueryLookup& ql, const std::vector<uint8_t>& chunk,
+                                            const std::vector<uint8_t>& chunk_rc,
+                                            const std::vector<uint8_t>& masked_chunk, const PreparedSubject& subj,
+                                            const Params& p, const Matrix& m, uint32_t offset) {
+  const uint32_t q_len = (uint32_t)chunk.size() - 2, s_len = (uint32_t)subj.plus.size() - 2;
+  if (q_len < (uint32_t)p.word_size || s_len < (uint32_t)p.word_size) return {};
+  std::vector<UngappedRecord> ung;
+  collect_ungapped(ql, chunk, chunk_rc, masked_chunk, subj, p, m, ung, nullptr);
+  sort_ungapped_like_ncbi(ung, q_len, s_len);
+  if (ung.empty()) return {};
+  std::vector<PrelimHsp> pre =
+      run_phase2a(chunk, chunk_rc, subj.plus, q_len, s_len, ung, p, m, (uint32_t)ql.lut.lut_width, nullptr);
+  for (auto& h : pre) {
+    h.q_start += offset;
+    h.q_end += offset;
+    h.q_seed += offset;
+  }
+  return pre;
+}
+
+template <class F>
+static void for_subjects(size_t n, int threads, F&& fn) {
+  if (threads <= 1) {
+    for (size_t i = 0; i < n; ++i) fn(i);
+    return;
+  }
+  std::vector<std::thread> pool;
+  std::atomic<size_t> next{0};
+  for (int t = 0; t < threads; ++t)
+    pool.emplace_back([&]() {
+      for (size_t i; (i = next.fetch_add(1)) < n;) fn(i);
+    });
+  for (auto& th : pool) th.join();
+}
+
 std::vector<AlignResult> search_query(const std::vector<uint8_t>& query, const std::string& query_id,
                                       const std::vector<PreparedSubject>& subjects,
                                       const std::vector<std::string>& names, const Params& p, const Matrix& m,
                                       int threads) {
-  // rmblastn/src/main.rs:1133-1234 (single chunk)
-  QueryLookup ql = build_query_lookup(query, p);
+  // rmblastn/src/main.rs:1133-1430
   std::vector<uint8_t> qrc = revcomp(query);
   std::vector<std::vector<AlignResult>> per(subjects.size());
-  if (threads <= 1) {
-    for (size_t i = 0; i < subjects.size(); ++i)
+  const size_t n = query.size() >= 2 ? query.size() - 2 : 0;
+  const ChunkPlan plan = chunk_plan(n);
+  if (plan.count <= 1) {
+    QueryLookup ql = build_query_lookup(query, p);
+    for_subjects(subjects.size(), threads, [&](size_t i) {
       per[i] = search_one_subject(ql, query, qrc, query_id, subjects[i], names[i], p, m, nullptr);
+    });
   } else {
-    std::vector<std::thread> pool;
-    std::atomic<size_t> next{0};
-    for (int t = 0; t < threads; ++t)
-      pool.emplace_back([&]() {
-        for (size_t i; (i = next.fetch_add(1)) < subjects.size();)
-          per[i] = search_one_subject(ql, query, qrc, query_id, subjects[i], names[i], p, m, nullptr);
+    // The whole query is DUST-masked once, with NCBI's chunked-interval
+    // off-by-one (dust.rs:614-631: one more base after each interval); each
+    // chunk's table is built from its slice of the masked query and exact
+    // word extension reads it; Phase 2b runs on the whole query.
+    std::vector<uint8_t> masked = query;
+    if (p.dust && n) {
+      for (const auto& iv : rmb::dust_mask(masked.data() + 1, n))
+        if (iv.second + 1 < n) masked[iv.second + 2] = 14;
+    }
+    Params nodust = p;
+    nodust.dust = false;
+    std::vector<std::vector<PrelimHsp>> acc(subjects.size());
+    for (size_t k = 0; k < plan.count; ++k) {
+      const size_t cs = k * plan.step;
+      if (cs >= n) break;
+      const size_t ce = std::min(cs + plan.size, n);
+      if (ce - cs < (size_t)p.word_size) break;
+      std::vector<uint8_t> chunk, mchunk;
+      chunk.push_back(query.front());
+      chunk.insert(chunk.end(), query.begin() + 1 + cs, query.begin() + 1 + ce);
+      chunk.push_back(query.back());
+      mchunk.push_back(masked.front());
+      mchunk.insert(mchunk.end(), masked.begin() + 1 + cs, masked.begin() + 1 + ce);
+      mchunk.push_back(masked.back());
+      const QueryLookup ql = build_query_lookup(mchunk, nodust);
+      const std::vector<uint8_t> crc = revcomp(chunk);
+      std::vector<std::vector<PrelimHsp>> got(subjects.size());
+      for_subjects(subjects.size(), threads, [&](size_t i) {
+        got[i] = chunk_phase2a(ql, chunk, crc, mchunk, subjects[i], p, m, (uint32_t)cs);
       });
-    for (auto& th : pool) th.join();
+      for (size_t i = 0; i < subjects.size(); ++i) {
+        if (k == 0) acc[i] = std::move(got[i]);
+        else merge_chunk_prelims(acc[i], std::move(got[i]), (uint32_t)cs);
+      }
+    }
+    for_subjects(subjects.size(), threads, [&](size_t i) {
+      if (!acc[i].empty())
+        per[i] = traceback_phase(query, qrc, query_id, subjects[i], names[i], std::move(acc[i]), p, m, nullptr);
+    });
   }
   std::vector<AlignResult> res;
   for (auto& v : per)

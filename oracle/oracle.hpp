@@ -185,6 +185,10 @@ struct Taps {                       // optional stage outputs for parity tests
 };
 
 // engine.rs:734-790 single-chunk search of one query against one subject
+struct ChunkPlan { size_t size = 0, count = 1, step = 0; };   // rmblastn/src/main.rs:1150-1190
+ChunkPlan chunk_plan(size_t query_len);
+// rmblastn/src/main.rs:1016-1085
+void merge_chunk_prelims(std::vector<PrelimHsp>& acc, std::vector<PrelimHsp> nw, uint32_t split);
 std::vector<AlignResult> search_one_subject(const QueryLookup& ql, const std::vector<uint8_t>& query,
                                             const std::vector<uint8_t>& query_rc, const std::string& query_id,
                                             const PreparedSubject& subj, const std::string& subject_id,

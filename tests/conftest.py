@@ -31,6 +31,23 @@ def golden(name):
     return os.path.join(GOLDEN, name)
 
 
+def golden_unzipped(name, dest):
+    """Decompress tests/golden/<name>.gz into directory `dest`; returns the path."""
+    import gzip
+    out = os.path.join(str(dest), name)
+    with gzip.open(golden(name + ".gz"), "rb") as f, open(out, "wb") as g:
+        g.write(f.read())
+    return out
+
+
+# chr22:30,000,000-33,000,000 of the reference's tests/data/chr22.fa, the query
+# of rmblastn/tests/cli.rs:252-274 (three chunks of the split query path)
+CHR22_3MB = "chr22_30000000_33000000.fa"
+CHUNKED_SEARCH = ["--matrix", golden("comparison.matrix"), "--gapopen", "20", "--gapextend", "5",
+                  "--word-size", "7", "--min-raw-gapped-score", "200", "--mask-level", "80",
+                  "--outfmt", "6 score qseqid qstart qend sstrand sseqid sstart send"]
+
+
 def run(cmd, timeout=600, env=None):
     e = dict(os.environ)
     if env:
@@ -69,6 +86,10 @@ def oracle_lib():
                                    P(ctypes.c_int32), P(ctypes.c_int64)]
     L.orc_improve_seed.argtypes = [P(ctypes.c_uint8), ctypes.c_size_t, P(ctypes.c_uint8), ctypes.c_size_t,
                                    P(ctypes.c_uint32), ctypes.c_int, P(ctypes.c_uint32)]
+    L.orc_chunk_plan.argtypes = [ctypes.c_uint64, P(ctypes.c_uint64)]
+    L.orc_merge_chunk_prelims.argtypes = [P(ctypes.c_uint32), ctypes.c_int, P(ctypes.c_uint32), ctypes.c_int,
+                                          ctypes.c_uint32, P(ctypes.c_uint32)]
+    L.orc_merge_chunk_prelims.restype = ctypes.c_int
     return L
 
 

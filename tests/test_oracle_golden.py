@@ -3,8 +3,8 @@
 rmblastn/tests/cli.rs:107-300 produced expected_*.ncbi-2.17.1.tsv with real
 NCBI rmblastn: mask level off and dust off, RepeatMasker's options with DUST,
 '-' in either sequence, a matrix with asymmetric frequencies, and NCBI's
-defaults (word size 11).  The multi-chunk chr22 golden needs the chunked
-query path, which is not built.
+defaults (word size 11).  The multi-chunk chr22 golden is checked in
+test_oracle_chunks.py.
 """
 import os
 
