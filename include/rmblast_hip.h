@@ -113,8 +113,10 @@ int rmb_library_load(rmb_ctx* ctx, uint32_t n, const uint8_t* const* seqs, const
 
 /* Search a batch of query records against every loaded subject, both strands.
  * Replaces build_query_lookup (engine.rs:606) + search_with_query_lookup over
- * all subjects (main.rs:1205-1223).  Queries longer than 2^31-1 are refused.
- * Results are held by the context until the next search. */
+ * all subjects (main.rs:1205-1223).  A query of 1,999,800 bases or more is
+ * searched as NCBI's overlapping chunks with the cross-chunk merge, as
+ * search_db_parallel does (main.rs:1150-1300).  Queries longer than 2^31-1
+ * are refused.  Results are held by the context until the next search. */
 int rmb_search(rmb_ctx* ctx, uint32_t n_queries, const uint8_t* const* queries, const uint32_t* lens);
 
 /* Search only the listed (query, subject) pairs of a batch: the reference's

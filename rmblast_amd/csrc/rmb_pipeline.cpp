@@ -31,6 +31,8 @@
 #include "../../include/rmblast_hip.h"
 #include "rmb_device.h"
 #include "rmb_dust.h"
+
+#include <map>
 #include "rmb_replay.h"
 
 namespace rmb {
@@ -614,18 +616,41 @@ static void phase1_group(rmb_ctx* c, const std::vector<uint32_t>& recs_idx, cons
   throw Failure(RMB_ERR_CAPACITY, "ungapped hit buffer could not be sized");
 }
 
+// The records a query is searched as through Phase 2a: the whole query, or
+// the chunks of NCBI's split (rmblastn/src/main.rs:1238-1300).
+struct SearchRecs {
+  std::vector<RecDesc> recs;        // search records, grouped by query in order
+  std::vector<uint32_t> query;      // query of each search record
+  std::vector<uint32_t> offset;     // its first base in the query
+  std::vector<uint32_t> first;      // search records of query i: [first[i], first[i + 1])
+  std::vector<uint64_t> whole_base; // pool offset of each whole query (Phase 2b)
+  uint64_t pool = 0;                // pool bytes
+};
+
 // DUST-masked copy of the query pool (filter/dust.rs:462-484: masked bases set
-// to 14), uploaded to the context's masked pool.
+// to 14), uploaded to the context's masked pool.  A split query is masked
+// whole, with one more base after each interval (NCBI's chunked-interval
+// off-by-one, dust.rs:603-631), and its chunks are cut from that mask.
 static const uint8_t* upload_dusted(rmb_ctx* c, const std::vector<std::vector<uint8_t>>& qf,
-                                    const std::vector<RecDesc>& recs, uint64_t off) {
-  std::vector<uint8_t> pm(off);
+                                    const SearchRecs& sr) {
+  std::vector<uint8_t> pm(sr.pool, 15);
   parallel_for(qf.size(), [&](size_t i) {
-    uint8_t* dst = pm.data() + recs[i].q_base;
-    std::memcpy(dst, qf[i].data(), qf[i].size());
-    if (recs[i].q_len) dust_mask(dst + 1, recs[i].q_len);
+    const uint32_t n = (uint32_t)qf[i].size() - 2;
+    const uint32_t k0 = sr.first[i], k1 = sr.first[i + 1];
+    if (k1 - k0 == 1) {
+      uint8_t* dst = pm.data() + sr.recs[k0].q_base;
+      std::memcpy(dst, qf[i].data(), qf[i].size());
+      if (n) dust_mask(dst + 1, n);
+      return;
+    }
+    std::vector<uint8_t> m(qf[i].begin() + 1, qf[i].end() - 1);
+    for (const auto& iv : dust_mask(m.data(), n))
+      if (iv.second + 1 < n) m[iv.second + 1] = 14;
+    for (uint32_t k = k0; k < k1; ++k)
+      std::memcpy(pm.data() + sr.recs[k].q_base + 1, m.data() + sr.offset[k], sr.recs[k].q_len);
   });
-  uint8_t* d = c->b_qext.get<uint8_t>(off);
-  hip_check(hipMemcpy(d, pm.data(), off, hipMemcpyHostToDevice), "copy masked query");
+  uint8_t* d = c->b_qext.get<uint8_t>(sr.pool);
+  hip_check(hipMemcpy(d, pm.data(), sr.pool, hipMemcpyHostToDevice), "copy masked query");
   return d;
 }
 
@@ -639,56 +664,92 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
   c->ops.clear();
   const int ws = c->P.word_size;
   const uint32_t nsub = (uint32_t)c->subj.size();
-  // ---- query pools
-  std::vector<std::vector<uint8_t>> qf(nq), qr(nq);
-  std::vector<RecDesc> recs(nq);
-  uint64_t off = 0;
+  // ---- query pools.  A query of 1,999,800 bases or more is searched
+  // through Phase 2a as NCBI's overlapping chunks, each its own sentinel-
+  // wrapped record; Phase 2b aligns on the whole query, stored after them.
+  std::vector<std::vector<uint8_t>> qf(nq);
+  SearchRecs sr;
+  sr.first.assign(nq + 1, 0);
+  sr.whole_base.assign(nq, 0);
+  std::vector<QuerySplit> split(nq);
   for (uint32_t i = 0; i < nq; ++i) {
     if (lens[i] > 0x7FFFFFFFu) throw Failure(RMB_ERR_ARG, "query longer than 2^31-1 bases");
     qf[i] = wrap(qs[i], lens[i]);
-    qr[i] = revcomp_wrapped(qf[i]);
-    recs[i].q_base = off;
-    recs[i].q_len = lens[i];
-    off += lens[i] + 2;
-  }
-  uint8_t* d_qfwd = c->b_qfwd.get<uint8_t>(off);
-  uint8_t* d_qrc = c->b_qrc.get<uint8_t>(off);
-  {
-    std::vector<uint8_t> pf(off), pr(off);
-    for (uint32_t i = 0; i < nq; ++i) {
-      std::memcpy(pf.data() + recs[i].q_base, qf[i].data(), qf[i].size());
-      std::memcpy(pr.data() + recs[i].q_base, qr[i].data(), qr[i].size());
+    split[i] = query_split(lens[i]);
+    sr.first[i] = (uint32_t)sr.recs.size();
+    if (split[i].count <= 1) {
+      sr.whole_base[i] = sr.pool;
+      sr.recs.push_back(RecDesc{sr.pool, 0, lens[i], 0});
+      sr.query.push_back(i);
+      sr.offset.push_back(0);
+      sr.pool += lens[i] + 2;
+      continue;
     }
-    hip_check(hipMemcpy(d_qfwd, pf.data(), off, hipMemcpyHostToDevice), "copy qfwd");
-    hip_check(hipMemcpy(d_qrc, pr.data(), off, hipMemcpyHostToDevice), "copy qrc");
+    for (uint32_t k = 0; k < split[i].count; ++k) {
+      const uint32_t cs = k * split[i].step;
+      if (cs >= lens[i]) break;
+      const uint32_t ce = std::min<uint64_t>((uint64_t)cs + split[i].size, lens[i]);
+      if (ce - cs < (uint32_t)ws) break;  // main.rs:1256-1258
+      sr.recs.push_back(RecDesc{sr.pool, 0, ce - cs, 0});
+      sr.query.push_back(i);
+      sr.offset.push_back(cs);
+      sr.pool += ce - cs + 2;
+    }
+    sr.whole_base[i] = sr.pool;
+    sr.pool += lens[i] + 2;
+  }
+  sr.first[nq] = (uint32_t)sr.recs.size();
+  const uint32_t nsr = (uint32_t)sr.recs.size();
+  std::vector<RecDesc>& recs = sr.recs;
+  uint8_t* d_qfwd = c->b_qfwd.get<uint8_t>(sr.pool);
+  uint8_t* d_qrc = c->b_qrc.get<uint8_t>(sr.pool);
+  {
+    std::vector<uint8_t> pf(sr.pool), pr(sr.pool);
+    auto put = [&](uint64_t base, const uint8_t* wrapped, size_t n) {  // n bases between two sentinels
+      uint8_t* f = pf.data() + base;
+      uint8_t* r = pr.data() + base;
+      f[0] = r[0] = f[n + 1] = r[n + 1] = 15;
+      std::memcpy(f + 1, wrapped + 1, n);
+      for (size_t j = 0; j < n; ++j) r[1 + j] = kBlastnaComplement[wrapped[n - j] & 15];
+    };
+    parallel_for(nq, [&](size_t i) {
+      put(sr.whole_base[i], qf[i].data(), lens[i]);
+      if (sr.first[i + 1] - sr.first[i] > 1)
+        for (uint32_t k = sr.first[i]; k < sr.first[i + 1]; ++k)
+          put(recs[k].q_base, qf[i].data() + sr.offset[k], recs[k].q_len);
+    });
+    hip_check(hipMemcpy(d_qfwd, pf.data(), sr.pool, hipMemcpyHostToDevice), "copy qfwd");
+    hip_check(hipMemcpy(d_qrc, pr.data(), sr.pool, hipMemcpyHostToDevice), "copy qrc");
   }
   // With DUST on, the lookup tables and the exact word extension read the
   // masked query; ungapped and gapped extension read the query itself
   // (engine.rs:606-619).
   const uint8_t* d_qext = d_qfwd;
-  if (c->P.dust) d_qext = upload_dusted(c, qf, recs, off);
+  if (c->P.dust) d_qext = upload_dusted(c, qf, sr);
   unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
   hip_check(hipMemset(ctr, 0, C_NCOUNTERS * sizeof(unsigned long long)), "memset counters");
   // ---- Phase 1 by lookup width; groups are bounded by the cell table size
   Timer t1;
   std::vector<PreHit> survivors;
-  std::vector<int> lw_of(nq);
-  for (uint32_t i = 0; i < nq; ++i) lw_of[i] = lut_width_for(ws, 2ull * lens[i]);
+  std::vector<int> lw_of(nsr);
+  for (uint32_t i = 0; i < nsr; ++i) lw_of[i] = lut_width_for(ws, 2ull * recs[i].q_len);
   for (int lw = 4; lw <= 12; ++lw) {
     std::vector<uint32_t> grp;
     const uint64_t cells = 1ull << (2 * lw);
     uint64_t elems = 0;
-    for (uint32_t i = 0; i <= nq; ++i) {
-      const bool flush = i == nq || (!grp.empty() && ((grp.size() + 1) * cells > (1ull << 28) ||
-                                                      elems + 2ull * lens[i] + 1 > (1ull << 31)));
+    for (uint32_t i = 0; i <= nsr; ++i) {
+      const bool flush = i == nsr || (!grp.empty() && ((grp.size() + 1) * cells > (1ull << 28) ||
+                                                       elems + 2ull * recs[i].q_len + 1 > (1ull << 31)));
       if (flush && !grp.empty()) {
-        if (pairs) {
-          std::vector<uint32_t> pos(nq, UINT32_MAX), local;
+        if (pairs) {  // pairs name queries: search every record of the query
+          std::vector<uint32_t> pos(nsr, UINT32_MAX), local;
           for (uint32_t k = 0; k < grp.size(); ++k) pos[grp[k]] = k;
           for (uint64_t it : *pairs) {
             const uint32_t q = (uint32_t)(it / nsub), f = (uint32_t)(it % nsub);
-            if (pos[q] != UINT32_MAX) local.push_back(pos[q] * nsub + f);
+            for (uint32_t k = sr.first[q]; k < sr.first[q + 1]; ++k)
+              if (pos[k] != UINT32_MAX) local.push_back(pos[k] * nsub + f);
           }
+          std::sort(local.begin(), local.end());
           phase1_group(c, grp, recs, lw, d_qfwd, d_qrc, d_qext, nullptr, &survivors, &local);
         } else {
           phase1_group(c, grp, recs, lw, d_qfwd, d_qrc, d_qext, nullptr, &survivors);
@@ -696,9 +757,9 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
         grp.clear();
         elems = 0;
       }
-      if (i < nq && lw_of[i] == lw) {
+      if (i < nsr && lw_of[i] == lw) {
         grp.push_back(i);
-        elems += 2ull * lens[i] + 1;
+        elems += 2ull * recs[i].q_len + 1;
       }
     }
   }
@@ -724,7 +785,7 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
   }
   parallel_for(items.size(), [&](size_t k) {
     ItemState& s = items[k];
-    const uint32_t ql = lens[s.item / nsub], sl = (uint32_t)c->subj[s.item % nsub].plus.size() - 2;
+    const uint32_t ql = recs[s.item / nsub].q_len, sl = (uint32_t)c->subj[s.item % nsub].plus.size() - 2;
     std::vector<UngRec> u(s.surv.size());
     for (size_t i = 0; i < s.surv.size(); ++i) {
       const UngHit& h = survivors[s.surv[i]].u;
@@ -748,6 +809,38 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
   });
   survivors.clear();
   survivors.shrink_to_fit();
+  // ---- items from search records to queries: a split query's prelims move
+  // to query coordinates and fold chunk by chunk into those of the chunks
+  // before (main.rs:1270-1300; items are in search-record order, so the
+  // chunks of a query arrive in order)
+  if (nsr != nq) {
+    std::vector<ItemState> merged;
+    merged.reserve(items.size());
+    for (size_t k = 0; k < items.size();) {
+      const uint32_t q = sr.query[items[k].item / nsub];
+      if (sr.first[q + 1] - sr.first[q] == 1) {
+        items[k].item = q * nsub + items[k].item % nsub;
+        merged.push_back(std::move(items[k++]));
+        continue;
+      }
+      std::map<uint32_t, ItemState> acc;  // by subject
+      for (; k < items.size() && sr.query[items[k].item / nsub] == q; ++k) {
+        ItemState& s = items[k];
+        const uint32_t chunk = s.item / nsub, f = s.item % nsub, off = sr.offset[chunk];
+        for (PreRec& h : s.pre) {
+          h.q_seed += off;
+          h.q_start += off;
+          h.q_end += off;
+        }
+        ItemState& a = acc[f];
+        a.item = q * nsub + f;
+        if (chunk == sr.first[q]) a.pre = std::move(s.pre);
+        else merge_chunk_prelims(a.pre, std::move(s.pre), off);
+      }
+      for (auto& kv : acc) merged.push_back(std::move(kv.second));
+    }
+    items = std::move(merged);
+  }
   c->stats[10] = t2.us();
   // ---- Phase 2b: traceback
   Timer t3;
@@ -791,7 +884,7 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
         const uint32_t tq = h.minus ? ql - 1 - nq : nq, ts = h.minus ? sl - 1 - ns : ns;
         const size_t g = first[k] + p;
         refs[g] = TbRef{(uint32_t)k, (uint32_t)p, tq, ts};
-        const uint8_t* qa = (h.minus ? d_qrc : d_qfwd) + recs[r].q_base + 1;
+        const uint8_t* qa = (h.minus ? d_qrc : d_qfwd) + sr.whole_base[r] + 1;
         const uint8_t* sa = d_salign + c->sdesc[f].s_base + 1;
         tasks[2 * g] = DpTask{qa, sa, tq + 1, ts + 1, c->P.xdrop_gap_final, 1};
         const uint32_t rm = ql > tq + 1 ? ql - tq - 1 : 0, rn = sl > ts + 1 ? sl - ts - 1 : 0;
@@ -1227,7 +1320,14 @@ int rmb_tap_ungapped(rmb_ctx* c, const uint8_t* query, uint32_t q_len, uint32_t 
     unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
     hip_check(hipMemset(ctr, 0, C_NCOUNTERS * sizeof(unsigned long long)), "memset counters");
     const int lw = lut_width_for(c->P.word_size, 2ull * q_len);
-    const uint8_t* d_qext = c->P.dust ? upload_dusted(c, {qf}, recs, qf.size()) : d_qfwd;
+    SearchRecs one;  // the query as one record, whatever its length
+    one.recs = recs;
+    one.query = {0};
+    one.offset = {0};
+    one.first = {0, 1};
+    one.whole_base = {0};
+    one.pool = qf.size();
+    const uint8_t* d_qext = c->P.dust ? upload_dusted(c, {qf}, one) : d_qfwd;
     if (q_len >= (uint32_t)c->P.word_size) phase1_group(c, {0u}, recs, lw, d_qfwd, d_qrc, d_qext, &hits, nullptr);
     std::vector<UngRec> v;
     for (auto& h : hits)

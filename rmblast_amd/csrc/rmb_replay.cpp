@@ -700,3 +700,66 @@ void finish_subject(std::vector<FinalRec>& hsps, const uint8_t* qw, uint32_t qa_
 }
 
 }  // namespace rmb
+
+namespace rmb {
+
+QuerySplit query_split(uint32_t n) {
+  constexpr uint32_t kChunk = 1000000, kOverlap = 100;
+  const uint32_t count = n / (kChunk - kOverlap);
+  if (count <= 1) return QuerySplit{n, 1, n > kOverlap ? n - kOverlap : 0};
+  uint64_t size = ((uint64_t)n + (uint64_t)(count - 1) * kOverlap) / count;
+  if (count + kOverlap < size) ++size;  // NCBI: num_chunks < chunk_size - overlap
+  return QuerySplit{(uint32_t)size, count, (uint32_t)size - kOverlap};
+}
+
+void merge_chunk_prelims(std::vector<PreRec>& acc, std::vector<PreRec>&& later, uint32_t split) {
+  // Only plus-strand pieces that reach into the 100-base overlap take part:
+  // accumulated ones ending past the split, later ones starting before its end.
+  // Each accumulated piece, best first, absorbs every later piece on a
+  // diagonal within 10 of its end whose start or end lies in its box; the box
+  // grows as it absorbs, and the score becomes the larger of the better piece
+  // and the pair's score density over the grown query span.
+  auto ranks_before = [](const PreRec& a, const PreRec& b) {
+    return std::make_tuple(-(int64_t)a.score, a.s_start, ~a.s_end, a.q_start, ~a.q_end) <
+           std::make_tuple(-(int64_t)b.score, b.s_start, ~b.s_end, b.q_start, ~b.q_end);
+  };
+  std::vector<uint32_t> lhs, rhs;
+  for (uint32_t i = 0; i < acc.size(); ++i)
+    if (!acc[i].minus && acc[i].q_end > split) lhs.push_back(i);
+  for (uint32_t j = 0; j < later.size(); ++j)
+    if (!later[j].minus && later[j].q_start < split + 100u) rhs.push_back(j);
+  std::stable_sort(lhs.begin(), lhs.end(), [&](uint32_t x, uint32_t y) { return ranks_before(acc[x], acc[y]); });
+  std::stable_sort(rhs.begin(), rhs.end(), [&](uint32_t x, uint32_t y) { return ranks_before(later[x], later[y]); });
+  std::vector<uint8_t> gone(later.size(), 0);
+  auto inside = [](const PreRec& box, uint32_t q, uint32_t s) {
+    return q >= box.q_start && q <= box.q_end && s >= box.s_start && s <= box.s_end;
+  };
+  for (uint32_t i : lhs) {
+    PreRec& a = acc[i];
+    for (uint32_t j : rhs) {
+      if (gone[j]) continue;
+      const PreRec& b = later[j];
+      const int64_t d = ((int64_t)a.q_end - a.s_end) - ((int64_t)b.q_start - b.s_start);
+      if (d <= -10 || d >= 10) continue;
+      if (!inside(a, b.q_start, b.s_start) && !inside(a, b.q_end, b.s_end)) continue;
+      const double density =
+          (double)(a.score + b.score) / (double)((a.q_end - a.q_start) + (b.q_end - b.q_start));
+      a.q_start = std::min(a.q_start, b.q_start);
+      a.q_end = std::max(a.q_end, b.q_end);
+      a.s_start = std::min(a.s_start, b.s_start);
+      a.s_end = std::max(a.s_end, b.s_end);
+      if (b.score > a.score) {
+        a.score = b.score;
+        a.q_seed = b.q_seed;
+        a.s_seed = b.s_seed;
+      }
+      const int32_t spread = (int32_t)(density * (double)(a.q_end - a.q_start));
+      if (spread > a.score) a.score = spread;
+      gone[j] = 1;
+    }
+  }
+  for (uint32_t j = 0; j < later.size(); ++j)
+    if (!gone[j]) acc.push_back(later[j]);
+}
+
+}  // namespace rmb

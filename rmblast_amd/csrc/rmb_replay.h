@@ -46,6 +46,13 @@ void pivot_of(const UngRec& u, uint32_t qa_len, uint32_t sa_len, uint32_t lw, ui
               uint32_t& tq, uint32_t& ts);
 // Blast_HSPListPurgeHSPsWithCommonEndpoints on prelims (engine.rs:1969-2083).
 void purge_prelims(std::vector<PreRec>& v, uint32_t sa_len);
+// NCBI's query split (rmblastn/src/main.rs:1150-1193): chunks of `size` bases
+// starting every `step`; one chunk below 1,999,800 bases.
+struct QuerySplit { uint32_t size, count, step; };
+QuerySplit query_split(uint32_t n);
+// Folds a later chunk's prelims (query coordinates) into the accumulated ones
+// at chunk start `split` (rmblastn/src/main.rs:1016-1085).
+void merge_chunk_prelims(std::vector<PreRec>& acc, std::vector<PreRec>&& later, uint32_t split);
 // Traceback seed inside a prelim (engine.rs:383-572).
 void improve_seed(const uint8_t* q, size_t qlen, const uint8_t* s, size_t slen, uint32_t q_seed, uint32_t s_seed,
                   uint32_t pqs, uint32_t pqe, uint32_t pss, uint32_t pse, bool minus, uint32_t& oq, uint32_t& os);
