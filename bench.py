@@ -156,10 +156,11 @@ def main():
         "roofline": roofline,
         "gapped_gcups": round(gcups, 3),
         "gapped_kernel_ms": round(g_ns / 1e6, 3),
+        "prelim_kernel_ms": round(st["t_prelim_kernel_ns"] / 1e6, 3),
         "stage_ms": {k: round(st[k] / 1e3, 2) for k in ("t_seed_us", "t_prelim_us", "t_traceback_us", "t_total_us")},
         "phase1_stage_frac": p1_stages,
         "counters": {k: st[k] for k in ("seeds_staged", "seed_evals", "ungapped_hits", "prelim_dps",
-                                         "traceback_dps", "dp_cells", "hsps", "ungapped_steps")},
+                                         "traceback_dps", "dp_cells", "tb_cells", "hsps", "ungapped_steps")},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -139,7 +139,8 @@ int rmb_results_copy(const rmb_ctx* ctx, rmb_hsp* hsps, rmb_op* ops);
 /* Stage counters of the last search (the reference's diagnostics counters,
  * engine.rs:28-37 and gapped.rs:25-36): [0] seeds that reached the diagonal
  * hash, [1] ungapped hits kept, [2] preliminary DPs run, [3] traceback DPs run,
- * [4] score-only DP cells, [5] traceback DP cells, [6] final HSPs,
+ * [4] score-only DP cells (both phases), [5] of those, the traceback phase's
+ * (its score pass; the traceback pass fills the same band), [6] final HSPs,
  * [7] ungapped extension steps.  Stage wall times in microseconds:
  * [8] lookup build, [9] seeding + ungapped, [10] preliminary gapped,
  * [11] traceback, [12] total; [13] time inside the seeding kernel in
@@ -149,7 +150,8 @@ int rmb_results_copy(const rmb_ctx* ctx, rmb_hsp* hsps, rmb_op* ops);
  * kernel, [17..22] seeding-kernel stage time (scan, output-slot reservation,
  * seed fill + extension, bucket sort, hash replay, emit) in 100 MHz ticks
  * summed over workgroups,
- * [23] reserved.  Entries past `n` are not written; RMB_NSTATS is the full
+ * [23] time inside the preliminary DP kernel in nanoseconds (part of [14]).
+ * Entries past `n` are not written; RMB_NSTATS is the full
  * length. */
 #define RMB_NSTATS 24
 int rmb_stats(const rmb_ctx* ctx, uint64_t* out, int n);

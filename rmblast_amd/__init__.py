@@ -44,7 +44,7 @@ STAT_NAMES = ["seed_evals", "ungapped_hits", "prelim_dps", "traceback_dps", "dp_
               "hsps", "ungapped_steps", "t_lookup_us", "t_seed_us", "t_prelim_us", "t_traceback_us",
               "t_total_us", "t_phase1_kernel_ns", "t_gapped_kernel_ns", "phase1_launches", "seeds_staged",
               "p1_ticks_scan", "p1_ticks_reserve", "p1_ticks_extend", "p1_ticks_bucket", "p1_ticks_replay",
-              "p1_ticks_emit", "reserved"]
+              "p1_ticks_emit", "t_prelim_kernel_ns"]
 
 
 class RmbError(RuntimeError):

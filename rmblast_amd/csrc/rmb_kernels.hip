@@ -690,17 +690,19 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
 // --------------------------------------------------------- X-drop DP ---------
 // One lane runs one half alignment in the reference's row-major order, so every
 // prune decision and band bound is the reference's.  The band [first_b, b_size)
-// lives in a per-lane ring of 2^wlog2 cells indexed by the absolute column; a
-// band wider than the ring sets E_BAND_WIDE and the host reruns the task with a
-// wider ring.  With TB, one traceback byte per band cell is kept for rows up to
+// lives in a per-lane ring of 2^wlog2 cells indexed by the absolute column,
+// cell k at ring[k * RS] (RS = 1: a lane's own stretch of global memory; RS =
+// the workgroup size: an LDS ring interleaved across the lanes); a band wider
+// than the ring sets E_BAND_WIDE and the host reruns the task with a wider
+// ring.  With TB, one traceback byte per band cell is kept for rows up to
 // stop_row (the best row from the score pass) and the walk runs at the end.
 static constexpr uint8_t SCRIPT_GAP_IN_A = 0, SCRIPT_SUB = 3, SCRIPT_GAP_IN_B = 6, SCRIPT_OP_MASK = 7,
                          SCRIPT_EXTEND_GAP_A = 0x10, SCRIPT_EXTEND_GAP_B = 0x40;
 
-template <bool TB>
-__device__ void dp_half(const DpTask& t, const int* mat, int32_t go, int32_t ge, int2* __restrict__ ring,
-                        uint32_t wmask, DpResult& res, uint8_t* __restrict__ flat, uint32_t* __restrict__ rows,
-                        uint32_t stop_row) {
+template <bool TB, uint32_t RS = 1>
+__device__ __forceinline__ void dp_half(const DpTask& t, const int* mat, int32_t go, int32_t ge,
+                                        int2* __restrict__ ring, uint32_t wmask, DpResult& res,
+                                        uint8_t* __restrict__ flat, uint32_t* __restrict__ rows, uint32_t stop_row) {
   res.score = 0; res.a_len = 0; res.b_len = 0; res.flags = 0; res.cells = 0; res.tb_bytes = 0; res.tb_rows = 0;
   const uint32_t m = t.m, n = t.n;
   if (m == 0 || n == 0) return;
@@ -716,7 +718,7 @@ __device__ void dp_half(const DpTask& t, const int* mat, int32_t go, int32_t ge,
   for (uint32_t j = 1; j <= n; ++j) {
     if (score < -xdrop) break;
     if (j > wmask) { res.flags |= E_BAND_WIDE; return; }
-    ring[j & wmask] = make_int2(score, score - oe);
+    ring[(j & wmask) * RS] = make_int2(score, score - oe);
     if (TB) flat[flat_len] = SCRIPT_GAP_IN_A;
     ++flat_len;
     score -= ge;
@@ -736,7 +738,7 @@ __device__ void dp_half(const DpTask& t, const int* mat, int32_t go, int32_t ge,
     cells += b_size - first_b;
     const uint8_t* bp = t.b;
     for (uint32_t bi = first_b; bi < b_size; ++bi) {
-      int2 c = ring[bi & wmask];
+      int2 c = ring[(bi & wmask) * RS];
       const uint32_t bb = bi < n ? (bp[rev ? n - 1 - bi : bi] & 15) : 0;
       int32_t sgc = c.y;
       const int32_t next = c.x + row[bb];
@@ -746,7 +748,7 @@ __device__ void dp_half(const DpTask& t, const int* mat, int32_t go, int32_t ge,
       if (sgr > cs) { cs = sgr; op = SCRIPT_GAP_IN_A; }
       if (best - cs > xdrop) {
         if (bi == first_b) ++first_b;
-        else ring[bi & wmask].x = kMinInt;
+        else ring[(bi & wmask) * RS].x = kMinInt;
       } else {
         last_b = bi;
         if (cs > best) { best = cs; best_a = ai + 1; best_b = bi; best_here = true; }
@@ -756,7 +758,7 @@ __device__ void dp_half(const DpTask& t, const int* mat, int32_t go, int32_t ge,
         if (sgc < open) c.y = open;
         else { c.y = sgc; op |= SCRIPT_EXTEND_GAP_B; }
         c.x = cs;
-        ring[bi & wmask] = c;
+        ring[(bi & wmask) * RS] = c;
         if (sgr < open) sgr = open;
         else op |= SCRIPT_EXTEND_GAP_A;
       }
@@ -773,7 +775,7 @@ __device__ void dp_half(const DpTask& t, const int* mat, int32_t go, int32_t ge,
     } else {
       while (sgr >= best - xdrop && b_size <= n) {
         if (b_size - first_b > wmask) { res.flags |= E_BAND_WIDE; return; }
-        ring[b_size & wmask] = make_int2(sgr, sgr - oe);
+        ring[(b_size & wmask) * RS] = make_int2(sgr, sgr - oe);
         if (TB) flat[flat_len] = SCRIPT_GAP_IN_A;
         ++flat_len;
         sgr -= ge;
@@ -782,7 +784,7 @@ __device__ void dp_half(const DpTask& t, const int* mat, int32_t go, int32_t ge,
     }
     if (b_size <= n) {
       if (b_size - first_b > wmask) { res.flags |= E_BAND_WIDE; return; }
-      ring[b_size & wmask] = make_int2(kMinInt, kMinInt);
+      ring[(b_size & wmask) * RS] = make_int2(kMinInt, kMinInt);
       ++b_size;
     }
     if (best_here) { bytes_at_best = flat_len; rows_at_best = ai + 2; }
@@ -900,49 +902,58 @@ __global__ __launch_bounds__(256) void k_pack_ops(const PackTask* __restrict__ t
 // traceback seed kept for Phase 2b.
 __device__ __forceinline__ void prelim_pivot(const UngHit& u, uint32_t qa_len, uint32_t sa_len, uint32_t lw,
                                              uint32_t& pq, uint32_t& ps, uint32_t& tq, uint32_t& ts) {
-  if (!u.strand) {
-    const bool shift = u.s_end >= u.s_seed + 8;
-    tq = u.q_seed + (shift ? 3u : 0u);
-    ts = u.s_seed + (shift ? 3u : 0u);
-    const uint32_t up = 4 - (ts & 3);
-    ps = ts + up;
-    pq = tq + up;
-    if (ps >= sa_len || pq >= qa_len) { ps -= 4; pq -= 4; }
-  } else {
-    const uint32_t saved_fwd = sa_len - u.s_seed - lw;
-    const bool shift = (sa_len - u.s_start) >= saved_fwd + 8;
-    tq = shift ? u.q_seed + lw - 4 : u.q_seed;
-    ts = shift ? u.s_seed + lw - 4 : u.s_seed;
-    const uint32_t qn = (qa_len - u.q_seed - lw) + (shift ? 3u : 0u);
-    const uint32_t sn = (sa_len - u.s_seed - lw) + (shift ? 3u : 0u);
-    const uint32_t up = 4 - (sn & 3);
-    pq = qn + up;
-    ps = sn + up;
-    if (pq > qa_len || ps > sa_len) { pq -= 4; ps -= 4; }
-  }
+  // both strands computed into locals and selected, so no output is written
+  // through a strand-dependent reference (which would go through scratch)
+  const bool p_shift = u.s_end >= u.s_seed + 8;
+  const uint32_t p_tq = u.q_seed + (p_shift ? 3u : 0u), p_ts = u.s_seed + (p_shift ? 3u : 0u);
+  const uint32_t p_up = 4 - (p_ts & 3);
+  const uint32_t p_ps0 = p_ts + p_up, p_pq0 = p_tq + p_up;
+  const bool p_back = p_ps0 >= sa_len || p_pq0 >= qa_len;
+  const uint32_t saved_fwd = sa_len - u.s_seed - lw;
+  const bool m_shift = (sa_len - u.s_start) >= saved_fwd + 8;
+  const uint32_t m_tq = m_shift ? u.q_seed + lw - 4 : u.q_seed, m_ts = m_shift ? u.s_seed + lw - 4 : u.s_seed;
+  const uint32_t qn = (qa_len - u.q_seed - lw) + (m_shift ? 3u : 0u);
+  const uint32_t sn = (sa_len - u.s_seed - lw) + (m_shift ? 3u : 0u);
+  const uint32_t m_up = 4 - (sn & 3);
+  const uint32_t m_pq0 = qn + m_up, m_ps0 = sn + m_up;
+  const bool m_back = m_pq0 > qa_len || m_ps0 > sa_len;
+  const bool minus = u.strand != 0;
+  tq = minus ? m_tq : p_tq;
+  ts = minus ? m_ts : p_ts;
+  pq = minus ? m_pq0 - (m_back ? 4u : 0u) : p_pq0 - (p_back ? 4u : 0u);
+  ps = minus ? m_ps0 - (m_back ? 4u : 0u) : p_ps0 - (p_back ? 4u : 0u);
 }
 
 // One lane per ungapped hit: pivot, both score-only halves (subject outer,
 // gapped.rs:360-418), and the cutoff test.  The DP is a pure function of the
 // hit, and a hit whose DP falls below the cutoff never changes the containment
 // state of the ordered Phase-2a loop (engine.rs:1317-1411), so only the hits
-// that reach the cutoff are kept for the ordered replay.  Hits whose band
-// outgrows the lane's ring go to `redo` for the wide-ring path.
-__global__ __launch_bounds__(256) void k_prelim_fused(
-    PrelimParams P, const UngHit* __restrict__ hits, const RecDesc* __restrict__ recs,
-    const SubjDesc* __restrict__ subjs, const uint8_t* __restrict__ qfwd, const uint8_t* __restrict__ qrc,
-    const uint8_t* __restrict__ splus, const int32_t* __restrict__ gmat, int2* __restrict__ ring, uint32_t wlog2,
-    PreHit* __restrict__ out, unsigned long long* __restrict__ out_count, UngHit* __restrict__ redo,
-    unsigned long long* __restrict__ redo_count, unsigned long long* __restrict__ counters,
+// that reach the cutoff are kept for the ordered replay.  Preliminary bands
+// (X-drop 30 by default) mostly stay a few cells wide: the first pass gives
+// each lane a ring of 2^kPrelimRingLog2 cells of LDS, interleaved across the
+// workgroup's lanes, and lists the hits whose band outgrows it (by index) for
+// a second pass over per-lane rings of 2^wlog2 cells in global memory.
+static constexpr uint32_t kPrelimThreads = 128, kPrelimRingLog2 = 4;
+
+template <bool LDS_RING>
+__global__ __launch_bounds__(kPrelimThreads) void k_prelim_fused(
+    PrelimParams P, const UngHit* __restrict__ hits, const uint32_t* __restrict__ idx, uint64_t n_tasks,
+    const RecDesc* __restrict__ recs, const SubjDesc* __restrict__ subjs, const uint8_t* __restrict__ qfwd,
+    const uint8_t* __restrict__ qrc, const uint8_t* __restrict__ splus, const int32_t* __restrict__ gmat,
+    int2* __restrict__ gring, uint32_t wlog2, PreHit* __restrict__ out, unsigned long long* __restrict__ out_count,
+    uint32_t* __restrict__ redo, unsigned long long* __restrict__ redo_count, unsigned long long* __restrict__ counters,
     uint32_t* __restrict__ err) {
   __shared__ int mat[256];
+  __shared__ int2 lring[LDS_RING ? (kPrelimThreads << kPrelimRingLog2) : 1];
   for (int i = threadIdx.x; i < 256; i += blockDim.x) mat[i] = gmat[i];
   __syncthreads();
   const uint64_t gt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (uint64_t)gridDim.x * blockDim.x;
-  int2* my = ring + gt * ring_stride(wlog2);
-  const uint32_t wmask = (1u << wlog2) - 1;
+  constexpr uint32_t RS = LDS_RING ? kPrelimThreads : 1;
+  int2* my = LDS_RING ? lring + threadIdx.x : gring + gt * ring_stride(wlog2);
+  const uint32_t wmask = LDS_RING ? (1u << kPrelimRingLog2) - 1 : (1u << wlog2) - 1;
   unsigned long long cells = 0, dps = 0;
-  for (uint64_t i = gt; i < P.n_hits; i += nt) {
+  for (uint64_t k = gt; k < n_tasks; k += nt) {
+    const uint32_t i = idx ? idx[k] : (uint32_t)k;
     const UngHit u = hits[i];
     if (u.item == kNoItem) continue;  // reserved slot the seeding kernel left unused
     const uint32_t r = u.item / P.n_subj, f = u.item % P.n_subj;
@@ -957,16 +968,16 @@ __global__ __launch_bounds__(256) void k_prelim_fused(
     DpTask left{sa, qa, ps, pq, xd, 1};
     DpTask right{sa + ps, qa + pq, sl > ps ? sl - ps : 0, ql > pq ? ql - pq : 0, xd, 0};
     DpResult L, Rr;
-    dp_half<false>(left, mat, P.gap_open, P.gap_extend, my, wmask, L, nullptr, nullptr, 0);
-    dp_half<false>(right, mat, P.gap_open, P.gap_extend, my, wmask, Rr, nullptr, nullptr, 0);
-    ++dps;
-    cells += L.cells + Rr.cells;
+    dp_half<false, RS>(left, mat, P.gap_open, P.gap_extend, my, wmask, L, nullptr, nullptr, 0);
+    dp_half<false, RS>(right, mat, P.gap_open, P.gap_extend, my, wmask, Rr, nullptr, nullptr, 0);
     if ((L.flags | Rr.flags) & E_BAND_WIDE) {
-      const unsigned long long k = atomicAdd(redo_count, 1ull);
-      if (k < P.redo_cap) redo[k] = u;
+      const unsigned long long kk = atomicAdd(redo_count, 1ull);
+      if (kk < P.redo_cap) redo[kk] = i;
       else atomicOr(err, (uint32_t)E_OUT_FULL);
       continue;
     }
+    ++dps;
+    cells += L.cells + Rr.cells;
     const int32_t sc = L.score + Rr.score;
     if (sc < 0 || sc < P.cutoff) continue;
     const uint32_t q0 = pq - L.b_len, q1 = pq + Rr.b_len, s0 = ps - L.a_len, s1 = ps + Rr.a_len;
@@ -977,8 +988,8 @@ __global__ __launch_bounds__(256) void k_prelim_fused(
     else { h.q_start = q0; h.q_end = q1; h.s_start = s0; h.s_end = s1; }
     h.tq = tq;
     h.ts = ts;
-    const unsigned long long k = atomicAdd(out_count, 1ull);
-    if (k < P.out_cap) out[k] = h;
+    const unsigned long long kk = atomicAdd(out_count, 1ull);
+    if (kk < P.out_cap) out[kk] = h;
     else atomicOr(err, (uint32_t)E_OUT_FULL);
   }
   atomicAdd(&counters[C_PRELIM_CELLS], cells);
@@ -986,13 +997,18 @@ __global__ __launch_bounds__(256) void k_prelim_fused(
 }
 
 // ---------------------------------------------------------- launchers --------
-hipError_t launch_prelim_fused(int blocks, PrelimParams P, const UngHit* hits, const RecDesc* recs,
-                               const SubjDesc* subjs, const uint8_t* qfwd, const uint8_t* qrc, const uint8_t* splus,
-                               const int32_t* gmat, int2* ring, uint32_t wlog2, PreHit* out,
-                               unsigned long long* out_count, UngHit* redo, unsigned long long* redo_count,
-                               unsigned long long* counters, uint32_t* err, hipStream_t st) {
-  hipLaunchKernelGGL(k_prelim_fused, dim3(blocks), dim3(256), 0, st, P, hits, recs, subjs, qfwd, qrc, splus, gmat,
-                     ring, wlog2, out, out_count, redo, redo_count, counters, err);
+hipError_t launch_prelim_fused(int blocks, bool lds_ring, PrelimParams P, const UngHit* hits, const uint32_t* idx,
+                               uint64_t n_tasks, const RecDesc* recs, const SubjDesc* subjs, const uint8_t* qfwd,
+                               const uint8_t* qrc, const uint8_t* splus, const int32_t* gmat, int2* gring,
+                               uint32_t wlog2, PreHit* out, unsigned long long* out_count, uint32_t* redo,
+                               unsigned long long* redo_count, unsigned long long* counters, uint32_t* err,
+                               hipStream_t st) {
+  if (lds_ring)
+    hipLaunchKernelGGL(k_prelim_fused<true>, dim3(blocks), dim3(kPrelimThreads), 0, st, P, hits, idx, n_tasks, recs,
+                       subjs, qfwd, qrc, splus, gmat, gring, wlog2, out, out_count, redo, redo_count, counters, err);
+  else
+    hipLaunchKernelGGL(k_prelim_fused<false>, dim3(blocks), dim3(kPrelimThreads), 0, st, P, hits, idx, n_tasks, recs,
+                       subjs, qfwd, qrc, splus, gmat, gring, wlog2, out, out_count, redo, redo_count, counters, err);
   return hipGetLastError();
 }
 

@@ -50,10 +50,10 @@ hipError_t launch_dp_score(int, const DpTask*, uint32_t, const int32_t*, int32_t
                            DpResult*, unsigned long long*, hipStream_t);
 hipError_t launch_dp_traceback(int, const TbTask*, uint32_t, const int32_t*, int32_t, int32_t, int2*, uint32_t,
                                uint8_t*, uint32_t*, uint32_t*, TbResult*, hipStream_t);
-hipError_t launch_prelim_fused(int, PrelimParams, const UngHit*, const RecDesc*, const SubjDesc*, const uint8_t*,
-                               const uint8_t*, const uint8_t*, const int32_t*, int2*, uint32_t, PreHit*,
-                               unsigned long long*, UngHit*, unsigned long long*, unsigned long long*, uint32_t*,
-                               hipStream_t);
+hipError_t launch_prelim_fused(int, bool, PrelimParams, const UngHit*, const uint32_t*, uint64_t, const RecDesc*,
+                               const SubjDesc*, const uint8_t*, const uint8_t*, const uint8_t*, const int32_t*, int2*,
+                               uint32_t, PreHit*, unsigned long long*, uint32_t*, unsigned long long*,
+                               unsigned long long*, uint32_t*, hipStream_t);
 
 struct Failure : std::runtime_error {
   int code;
@@ -209,6 +209,7 @@ struct rmb_ctx {
   std::vector<rmb_hsp> hsps;
   std::vector<rmb_op> ops;
   uint64_t stats[RMB_NSTATS] = {0};
+  uint64_t host_prelim_dps = 0;  // preliminary DPs of the last search run through run_dp_score
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // scratch
   DevBuf b_qfwd, b_qrc, b_recs, b_elem, b_keys, b_keys2, b_vals, b_vals2, b_counts, b_cells, b_sort_tmp;
@@ -243,11 +244,43 @@ static void timed_launch(rmb_ctx* c, int slot, F&& launch) {
   c->stats[slot] += (uint64_t)((double)ms * 1e6);
 }
 
+// Lanes of a wave run their DPs in lockstep, so a wave lasts as long as its
+// longest DP: tasks are launched longest first (by `work`, an estimate),
+// which puts DPs of like size in one wave.  The order is a counting sort on
+// quarter-octave classes of the work (within 19% of each other), stable.
+static std::vector<uint32_t> longest_first(const std::vector<uint64_t>& work) {
+  auto cls = [](uint64_t w) -> uint32_t {
+    if (w < 4) return (uint32_t)w;
+    const uint32_t e = 63u - (uint32_t)__builtin_clzll(w);  // >= 2
+    return 4 * (e - 1) + (uint32_t)((w >> (e - 2)) & 3u);   // 4 .. 255
+  };
+  std::vector<uint32_t> start(257, 0), ord(work.size());
+  for (uint64_t w : work) ++start[255 - cls(w) + 1];
+  for (int k = 0; k < 256; ++k) start[k + 1] += start[k];
+  for (uint32_t i = 0; i < work.size(); ++i) ord[start[255 - cls(work[i])]++] = i;
+  return ord;
+}
+
 // Run a list of lane-serial score-only DPs; tasks whose band outgrows the
-// common ring are rerun once with a ring of 2^kRingBigLog2 cells.
-static void run_dp_score(rmb_ctx* c, const std::vector<DpTask>& tasks, std::vector<DpResult>& res) {
-  res.assign(tasks.size(), DpResult{});
-  if (tasks.empty()) return;
+// common ring are rerun once with a ring of 2^kRingBigLog2 cells.  With
+// `work`, tasks run longest first.
+static void run_dp_score(rmb_ctx* c, const std::vector<DpTask>& tasks_in, std::vector<DpResult>& res,
+                         const std::vector<uint64_t>* work = nullptr) {
+  Timer tt;
+  res.assign(tasks_in.size(), DpResult{});
+  if (tasks_in.empty()) return;
+  std::vector<uint32_t> ord;
+  std::vector<DpTask> sorted;
+  if (work) {
+    ord = longest_first(*work);
+    sorted.resize(tasks_in.size());
+    for (size_t k = 0; k < ord.size(); ++k) sorted[k] = tasks_in[ord[k]];
+  }
+  const std::vector<DpTask>& tasks = work ? sorted : tasks_in;
+  std::vector<DpResult> sres;
+  std::vector<DpResult>& res_o = work ? sres : res;
+  if (work) sres.assign(tasks.size(), DpResult{});
+  const uint64_t t_prep = tt.us();
   unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
   const size_t chunk = 1u << 20;
   for (size_t base = 0; base < tasks.size(); base += chunk) {
@@ -262,50 +295,73 @@ static void run_dp_score(rmb_ctx* c, const std::vector<DpTask>& tasks, std::vect
                                 kRingLog2, dr, ctr, c->st),
                 "k_dp_score");
     });
-    hip_check(hipMemcpyAsync(res.data() + base, dr, n * sizeof(DpResult), hipMemcpyDeviceToHost, c->st), "copy dp");
+    hip_check(hipMemcpyAsync(res_o.data() + base, dr, n * sizeof(DpResult), hipMemcpyDeviceToHost, c->st), "copy dp");
   }
   hip_check(hipStreamSynchronize(c->st), "dp sync");
+  const uint64_t t_gpu = tt.us();
   std::vector<uint32_t> wide;
-  for (size_t i = 0; i < res.size(); ++i)
-    if (res[i].flags & E_BAND_WIDE) wide.push_back((uint32_t)i);
-  if (wide.empty()) return;
-  std::vector<DpTask> wt(wide.size());
-  for (size_t i = 0; i < wide.size(); ++i) wt[i] = tasks[wide[i]];
-  const uint32_t n = (uint32_t)wt.size();
-  DpTask* dt = c->b_tasks.get<DpTask>(n);
-  DpResult* dr = c->b_dpres.get<DpResult>(n);
-  hip_check(hipMemcpyAsync(dt, wt.data(), n * sizeof(DpTask), hipMemcpyHostToDevice, c->st), "copy wide");
-  const uint32_t lanes = std::min<uint32_t>(4096, ((n + 255) / 256) * 256);
-  int2* ring = c->b_ring_big.get<int2>((size_t)lanes * ring_stride(kRingBigLog2));
-  timed_launch(c, 14, [&] {
-    hip_check(launch_dp_score((int)(lanes / 256), dt, n, c->d_mat, c->P.gap_open, c->P.gap_extend, ring,
-                              kRingBigLog2, dr, ctr, c->st),
-              "k_dp_score wide");
-  });
-  std::vector<DpResult> wr(n);
-  hip_check(hipMemcpyAsync(wr.data(), dr, n * sizeof(DpResult), hipMemcpyDeviceToHost, c->st), "copy wide res");
-  hip_check(hipStreamSynchronize(c->st), "dp wide sync");
-  for (size_t i = 0; i < wide.size(); ++i) {
-    if (wr[i].flags & E_BAND_WIDE)
-      throw Failure(RMB_ERR_CAPACITY, "X-drop band wider than 2^17 cells");
-    res[wide[i]] = wr[i];
+  for (size_t i = 0; i < res_o.size(); ++i)
+    if (res_o[i].flags & E_BAND_WIDE) wide.push_back((uint32_t)i);
+  if (!wide.empty()) {
+    std::vector<DpTask> wt(wide.size());
+    for (size_t i = 0; i < wide.size(); ++i) wt[i] = tasks[wide[i]];
+    const uint32_t n = (uint32_t)wt.size();
+    DpTask* dt = c->b_tasks.get<DpTask>(n);
+    DpResult* dr = c->b_dpres.get<DpResult>(n);
+    hip_check(hipMemcpyAsync(dt, wt.data(), n * sizeof(DpTask), hipMemcpyHostToDevice, c->st), "copy wide");
+    const uint32_t lanes = std::min<uint32_t>(4096, ((n + 255) / 256) * 256);
+    int2* ring = c->b_ring_big.get<int2>((size_t)lanes * ring_stride(kRingBigLog2));
+    timed_launch(c, 14, [&] {
+      hip_check(launch_dp_score((int)(lanes / 256), dt, n, c->d_mat, c->P.gap_open, c->P.gap_extend, ring,
+                                kRingBigLog2, dr, ctr, c->st),
+                "k_dp_score wide");
+    });
+    std::vector<DpResult> wr(n);
+    hip_check(hipMemcpyAsync(wr.data(), dr, n * sizeof(DpResult), hipMemcpyDeviceToHost, c->st), "copy wide res");
+    hip_check(hipStreamSynchronize(c->st), "dp wide sync");
+    for (size_t i = 0; i < wide.size(); ++i) {
+      if (wr[i].flags & E_BAND_WIDE)
+        throw Failure(RMB_ERR_CAPACITY, "X-drop band wider than 2^17 cells");
+      res_o[wide[i]] = wr[i];
+    }
   }
+  if (work)
+    for (size_t k = 0; k < ord.size(); ++k) res[ord[k]] = sres[k];
+  if (work && std::getenv("RMB_STAGE_TIMES"))
+    std::fprintf(stderr, "[rmb] score DPs ms: order %.1f  launch+copy %.1f  wide+reorder %.1f  (%zu tasks, %zu wide)\n",
+                 t_prep / 1e3, (t_gpu - t_prep) / 1e3, (tt.us() - t_gpu) / 1e3, tasks.size(), wide.size());
 }
 
-struct HalfTb { int32_t score; uint32_t a_len, b_len; Script ops; };
+// One traceback half: score, lengths, and its run-length ops (kind | len << 2,
+// in alignment order from the seed outwards) at ops_at in the op pool.
+struct HalfTb { int32_t score; uint32_t a_len, b_len, n_ops; uint64_t ops_at; };
 
 // Traceback DPs: score pass for sizes, then the traceback pass in batches
 // bounded by the device storage they need.
-static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::vector<HalfTb>& out) {
-  out.assign(tasks.size(), HalfTb{0, 0, 0, {}});
+static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::vector<HalfTb>& out,
+                             std::vector<uint32_t>& op_pool, const std::vector<uint64_t>* work = nullptr) {
+  out.assign(tasks.size(), HalfTb{0, 0, 0, 0, 0});
+  op_pool.clear();
+  Timer tt;
+  uint64_t t_tb = 0, t_unpack = 0;
   std::vector<DpResult> sres;
-  run_dp_score(c, tasks, sres);
+  run_dp_score(c, tasks, sres, work);
+  const uint64_t t_score = tt.us();
   std::vector<uint32_t> need;
   for (size_t i = 0; i < tasks.size(); ++i) {
     out[i].score = sres[i].score;
     out[i].a_len = sres[i].a_len;
     out[i].b_len = sres[i].b_len;
     if (sres[i].a_len > 0 || sres[i].b_len > 0) need.push_back((uint32_t)i);
+  }
+  // the score pass measured each band: the traceback pass runs longest first
+  {
+    std::vector<uint64_t> tb(need.size());
+    for (size_t k = 0; k < need.size(); ++k) tb[k] = sres[need[k]].tb_bytes;
+    const std::vector<uint32_t> ord = longest_first(tb);
+    std::vector<uint32_t> sorted(need.size());
+    for (size_t k = 0; k < ord.size(); ++k) sorted[k] = need[ord[k]];
+    need.swap(sorted);
   }
   const uint64_t kBudget = 1ull << 31;
   size_t pos = 0;
@@ -332,6 +388,7 @@ static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::
       ++pos;
     }
     const uint32_t n = (uint32_t)batch.size();
+    const uint64_t t_b0 = tt.us();
     TbTask* dt = c->b_tbtasks.get<TbTask>(n);
     TbResult* dr = c->b_tbres.get<TbResult>(n);
     uint8_t* dflat = c->b_flat.get<uint8_t>(flat + 16);
@@ -382,17 +439,24 @@ static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::
     uint32_t* dpacked = c->b_packed.get<uint32_t>(packed_n + 1);
     hip_check(hipMemcpyAsync(dpk, pk.data(), n * sizeof(PackTask), hipMemcpyHostToDevice, c->st), "copy pack");
     hip_check(launch_pack_ops(dpk, n, dops, dpacked, c->st), "k_pack_ops");
-    std::vector<uint32_t> hops(packed_n + 1);
-    hip_check(hipMemcpyAsync(hops.data(), dpacked, packed_n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->st),
+    const uint64_t pool_at = op_pool.size();
+    op_pool.resize(pool_at + packed_n);
+    hip_check(hipMemcpyAsync(op_pool.data() + pool_at, dpacked, packed_n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             c->st),
               "copy ops");
     hip_check(hipStreamSynchronize(c->st), "ops sync");
+    const uint64_t t_b1 = tt.us();
     for (uint32_t k = 0; k < n; ++k) {
-      const uint32_t i = need[pos - n + k];
-      Script& s = out[i].ops;
-      const uint32_t* o = hops.data() + pk[k].dst;
-      for (uint32_t j = 0; j < tr[k].n_ops; ++j) script_push(s, (uint8_t)(o[j] & 3), o[j] >> 2);
+      HalfTb& h = out[need[pos - n + k]];
+      h.ops_at = pool_at + pk[k].dst;
+      h.n_ops = tr[k].n_ops;
     }
+    t_tb += t_b1 - t_b0;
+    t_unpack += tt.us() - t_b1;
   }
+  if (std::getenv("RMB_STAGE_TIMES"))
+    std::fprintf(stderr, "[rmb] traceback DPs ms: score pass %.1f  traceback pass %.1f  unpack %.1f  total %.1f\n",
+                 t_score / 1e3, t_tb / 1e3, t_unpack / 1e3, tt.us() / 1e3);
 }
 
 struct ItemState {
@@ -420,33 +484,57 @@ static void prelim_group(rmb_ctx* c, const std::vector<RecDesc>& recs, const std
   PP.n_subj = nsub;
   PP.n_hits = nh;
   PP.out_cap = nh;
-  PP.redo_cap = std::max<uint64_t>(4096, nh / 8);
+  PP.redo_cap = nh;
+  if (nh > 0xFFFFFFFFull) throw Failure(RMB_ERR_CAPACITY, "more than 2^32 ungapped hits in one group");
   PreHit* dout = c->b_pre.get<PreHit>(PP.out_cap);
-  UngHit* dredo = c->b_redo.get<UngHit>(PP.redo_cap);
-  unsigned long long* cnt = c->b_precount.get<unsigned long long>(2);
+  uint32_t* dredo = c->b_redo.get<uint32_t>(2 * nh);  // hit indices: wide bands, then bands wider still
+  unsigned long long* cnt = c->b_precount.get<unsigned long long>(3);
   uint32_t* err = c->b_err.get<uint32_t>(1);
   unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
-  const uint64_t lanes = std::min<uint64_t>((uint64_t)c->cu_count * 8 * 64, ((nh + 255) / 256) * 256);
-  int2* ring = c->b_ring.get<int2>((size_t)lanes * ring_stride(kRingLog2));
-  hip_check(hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned long long), c->st), "memset prelim counts");
+  hip_check(hipMemsetAsync(cnt, 0, 3 * sizeof(unsigned long long), c->st), "memset prelim counts");
   hip_check(hipMemsetAsync(err, 0, sizeof(uint32_t), c->st), "memset err");
+  const uint64_t before = c->stats[14];
+  // pass 1: 128-lane workgroups with a 16 KB LDS band ring each, eight per CU
+  const int blocks = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->cu_count * 8, (nh + 127) / 128));
   timed_launch(c, 14, [&] {
-    hip_check(launch_prelim_fused((int)(lanes / 256), PP, dh, drecs, (const SubjDesc*)c->d_sdesc.p, d_qfwd, d_qrc,
-                                  (const uint8_t*)c->d_splus.p, c->d_mat, ring, kRingLog2, dout, cnt, dredo, cnt + 1,
+    hip_check(launch_prelim_fused(blocks, true, PP, dh, nullptr, nh, drecs, (const SubjDesc*)c->d_sdesc.p, d_qfwd,
+                                  d_qrc, (const uint8_t*)c->d_splus.p, c->d_mat, nullptr, 0, dout, cnt, dredo, cnt + 1,
                                   ctr, err, c->st),
               "k_prelim_fused");
   });
-  unsigned long long n[2];
+  unsigned long long n[3];
   uint32_t e = 0;
   hip_check(hipMemcpyAsync(n, cnt, sizeof n, hipMemcpyDeviceToHost, c->st), "copy prelim counts");
+  hip_check(hipStreamSynchronize(c->st), "prelim sync");
+  if (n[1]) {
+    // pass 2: the hits whose band outgrew the LDS ring, with per-lane rings
+    // of 2^kRingLog2 cells in global memory
+    const uint64_t lanes = std::min<uint64_t>((uint64_t)c->cu_count * 8 * 128, ((n[1] + 127) / 128) * 128);
+    int2* ring = c->b_ring.get<int2>((size_t)lanes * ring_stride(kRingLog2));
+    timed_launch(c, 14, [&] {
+      hip_check(launch_prelim_fused((int)(lanes / 128), false, PP, dh, dredo, n[1], drecs,
+                                    (const SubjDesc*)c->d_sdesc.p, d_qfwd, d_qrc, (const uint8_t*)c->d_splus.p,
+                                    c->d_mat, ring, kRingLog2, dout, cnt, dredo + nh, cnt + 2, ctr, err, c->st),
+                "k_prelim_fused wide");
+    });
+    hip_check(hipMemcpyAsync(n, cnt, sizeof n, hipMemcpyDeviceToHost, c->st), "copy prelim counts");
+  }
   hip_check(hipMemcpyAsync(&e, err, sizeof e, hipMemcpyDeviceToHost, c->st), "copy err");
   hip_check(hipStreamSynchronize(c->st), "prelim sync");
-  if (e & E_OUT_FULL) throw Failure(RMB_ERR_CAPACITY, "preliminary redo list overflow");
+  c->stats[23] += c->stats[14] - before;
+  if (e & E_OUT_FULL) throw Failure(RMB_ERR_CAPACITY, "preliminary DP output overflow");
   const size_t base = out.size();
   out.resize(base + n[0]);
   hip_check(hipMemcpy(out.data() + base, dout, n[0] * sizeof(PreHit), hipMemcpyDeviceToHost), "copy prelims");
-  std::vector<UngHit> redo(n[1]);
-  if (n[1]) hip_check(hipMemcpy(redo.data(), dredo, n[1] * sizeof(UngHit), hipMemcpyDeviceToHost), "copy redo");
+  // bands wider than 2^kRingLog2 cells: the host-driven path with wider rings
+  std::vector<UngHit> redo(n[2]);
+  if (n[2]) {
+    std::vector<uint32_t> ri(n[2]);
+    hip_check(hipMemcpy(ri.data(), dredo + nh, n[2] * sizeof(uint32_t), hipMemcpyDeviceToHost), "copy redo");
+    for (size_t k = 0; k < ri.size(); ++k)
+      hip_check(hipMemcpy(&redo[k], dh + ri[k], sizeof(UngHit), hipMemcpyDeviceToHost), "copy redo hit");
+    c->host_prelim_dps += n[2];
+  }
   for (size_t i = base; i < out.size(); ++i) {
     const uint32_t r = out[i].u.item / nsub, f = out[i].u.item % nsub;
     out[i].u.item = recs_idx[r] * nsub + f;
@@ -660,6 +748,7 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
   Timer total;
   if (c->subj.empty()) throw Failure(RMB_ERR_STATE, "no library loaded");
   std::memset(c->stats, 0, sizeof c->stats);
+  c->host_prelim_dps = 0;
   c->hsps.clear();
   c->ops.clear();
   const int ws = c->P.word_size;
@@ -868,6 +957,7 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
     for (size_t k = 0; k < items.size(); ++k) first[k + 1] = first[k] + items[k].pre.size();
     std::vector<TbRef> refs(first.back());
     std::vector<DpTask> tasks(2 * first.back());
+    std::vector<uint64_t> work(tasks.size());  // rows each half is expected to cover: the prelim's extent
     const uint8_t* d_salign = (const uint8_t*)c->d_salign.p;
     parallel_for(items.size(), [&](size_t k) {
       ItemState& s = items[k];
@@ -889,11 +979,15 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
         tasks[2 * g] = DpTask{qa, sa, tq + 1, ts + 1, c->P.xdrop_gap_final, 1};
         const uint32_t rm = ql > tq + 1 ? ql - tq - 1 : 0, rn = sl > ts + 1 ? sl - ts - 1 : 0;
         tasks[2 * g + 1] = DpTask{qa + tq + 1, sa + ts + 1, rn ? rm : 0, rm ? rn : 0, c->P.xdrop_gap_final, 0};
+        const uint32_t fqs = h.minus ? ql - h.q_end : h.q_start, fqe = h.minus ? ql - h.q_start : h.q_end;
+        work[2 * g] = tq + 1 > fqs ? tq + 1 - fqs : 1;
+        work[2 * g + 1] = fqe > tq + 1 ? fqe - tq - 1 : 1;
       }
     });
     tb_marks[1] = t3.us();
     std::vector<HalfTb> halves;
-    run_dp_traceback(c, tasks, halves);
+    std::vector<uint32_t> op_pool;
+    run_dp_traceback(c, tasks, halves, op_pool, &work);
     tb_marks[2] = t3.us();
     c->stats[3] = refs.size();
     const int32_t go = c->P.gap_open, ge = c->P.gap_extend;
@@ -918,8 +1012,14 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
         uint32_t q0 = ref.tq + 1 - L.a_len, s0 = ref.ts + 1 - L.b_len;
         uint32_t q1 = ref.tq + 1 + R.a_len, s1 = ref.ts + 1 + R.b_len;
         Script ops;
-        for (auto it = L.ops.rbegin(); it != L.ops.rend(); ++it) script_push(ops, it->kind, it->len);
-        for (const Run& x : R.ops) script_push(ops, x.kind, x.len);
+        for (uint32_t j = L.n_ops; j-- > 0;) {
+          const uint32_t o = op_pool[L.ops_at + j];
+          script_push(ops, (uint8_t)(o & 3), o >> 2);
+        }
+        for (uint32_t j = 0; j < R.n_ops; ++j) {
+          const uint32_t o = op_pool[R.ops_at + j];
+          script_push(ops, (uint8_t)(o & 3), o >> 2);
+        }
         while (!ops.empty() && ops.front().kind != K_SUB) {  // engine.rs:3016-3033
           sc += go + (int32_t)ops.front().len * ge;
           if (ops.front().kind == K_GAP_Q) s0 += ops.front().len; else q0 += ops.front().len;
@@ -991,8 +1091,9 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
   c->stats[0] = hc[C_SEED_EVALS];
   c->stats[1] = hc[C_UNG_HITS];
   c->stats[7] = hc[C_UNG_STEPS];
-  c->stats[2] = hc[C_PRELIM_DPS];
+  c->stats[2] = hc[C_PRELIM_DPS] + c->host_prelim_dps;
   c->stats[4] = hc[C_PRELIM_CELLS] + hc[C_DP_CELLS];  // DP cells of both phases (score passes)
+  c->stats[5] = hc[C_DP_CELLS];
   c->stats[6] = c->hsps.size();
   c->stats[12] = total.us();
   c->stats[16] = hc[C_SEEDS];
@@ -1380,12 +1481,18 @@ int rmb_tap_align_ex(rmb_ctx* c, const uint8_t* a, uint32_t m, const uint8_t* b,
     tap_upload(c, a, m, b, n, da, db);
     std::vector<DpTask> t{DpTask{da, db, m, n, xdrop, (uint32_t)(reverse != 0)}};
     std::vector<HalfTb> r;
-    run_dp_traceback(c, t, r);
+    std::vector<uint32_t> pool;
+    run_dp_traceback(c, t, r, pool);
+    Script sc;
+    for (uint32_t j = 0; j < r[0].n_ops; ++j) {
+      const uint32_t o = pool[r[0].ops_at + j];
+      script_push(sc, (uint8_t)(o & 3), o >> 2);
+    }
     out[0] = r[0].score;
     out[1] = r[0].a_len;
     out[2] = r[0].b_len;
-    out[3] = (int64_t)r[0].ops.size();
-    for (size_t i = 0; i < r[0].ops.size() && i < max_ops; ++i) ops[i] = r[0].ops[i].kind | (r[0].ops[i].len << 2);
+    out[3] = (int64_t)sc.size();
+    for (size_t i = 0; i < sc.size() && i < max_ops; ++i) ops[i] = sc[i].kind | (sc[i].len << 2);
   });
 }
 
