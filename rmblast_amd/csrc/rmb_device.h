@@ -9,8 +9,8 @@ namespace rmb {
 // buckets are owned two per thread.
 constexpr int kP1Threads = 256;
 constexpr int kSeedCap = 768;
-constexpr int kHashLds = 2;      // cells per bucket held in the owner thread's registers
-constexpr int kHashGlobal = 254; // further cells per bucket in the workgroup's global spill area
+constexpr int kHashLds = 3;      // cells per bucket held in the owner thread's registers
+constexpr int kHashGlobal = 253; // further cells per bucket in the workgroup's global spill area
 constexpr int kBuckets = 512;
 constexpr int kXcds = 8;         // MI355X: 8 XCDs, workgroups dispatched round-robin over them
 

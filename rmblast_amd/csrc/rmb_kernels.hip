@@ -272,6 +272,18 @@ __global__ void k_lut_keys(const RecDesc* __restrict__ recs, uint32_t n_recs, co
   atomicAdd(&counts[((uint64_t)r << (2 * lw)) + code], 1u);
 }
 
+// Ascending sort of eight values (Batcher's 19-comparator network).
+__device__ __forceinline__ void sort8(uint32_t (&v)[8]) {
+  constexpr int kNet[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6}, {5, 7}, {1, 2}, {5, 6},
+                               {0, 4}, {1, 5}, {2, 6}, {3, 7}, {2, 4}, {3, 5}, {1, 2}, {3, 4}, {5, 6}};
+#pragma unroll
+  for (int k = 0; k < 19; ++k) {
+    const uint32_t a = v[kNet[k][0]], b = v[kNet[k][1]];
+    v[kNet[k][0]] = a < b ? a : b;
+    v[kNet[k][1]] = a < b ? b : a;
+  }
+}
+
 // ------------------------------------------------------------ Phase 1 -------
 // One workgroup per (record, subject) item at a time, items from a device
 // queue.  Per pass it stages up to kSeedCap seeds of the reference scan order
@@ -371,10 +383,10 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
     const uint32_t n_slots = (uint32_t)(end_range / step) + 1;
     const int32_t c1 = q_len + 1;
     // diagonal hash: thread tid owns buckets tid and tid + kP1Threads for the
-    // whole item; each bucket's first two cells live in these registers, the
-    // rest in the workgroup's spill area
-    uint32_t hc[2] = {0u, 0u}, hl0[2] = {0u, 0u}, hl1[2] = {0u, 0u};
-    int32_t hd0[2] = {0, 0}, hd1[2] = {0, 0};
+    // whole item; each bucket's first kHashLds (3) cells live in these
+    // registers, the rest in the workgroup's spill area
+    uint32_t hc[2] = {0u, 0u}, hl0[2] = {0u, 0u}, hl1[2] = {0u, 0u}, hl2[2] = {0u, 0u};
+    int32_t hd0[2] = {0, 0}, hd1[2] = {0, 0}, hd2[2] = {0, 0};
     // The scan runs over windows of kP1Threads subject positions; a window's
     // seeds (subject offset ascending, then cell order) are taken kSeedCap
     // per pass, so each window is scanned once.
@@ -531,68 +543,124 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
       }
       __syncthreads();
       tick(3);
-      // ---- E: exact replay of each bucket (diag_hash.rs:57-111)
+      // ---- E: exact replay of each bucket (diag_hash.rs:57-111).  Fast path:
+      // a bucket with at most 8 seeds this pass whose cells fit the registers
+      // sorts its seed list in registers and replays it branch-free; the
+      // generic loop takes the rest (more seeds, or cells in the spill area).
+      uint32_t nemit_local = 0;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const int bb = tid + h * kP1Threads;
         const uint32_t nb = bcnt[bb];
-        if (!nb) continue;
         const uint32_t st = bst[bb] - nb;
-        for (uint32_t x = st + 1; x < st + nb; ++x) {
-          const uint16_t v = blist[x];
-          uint32_t y = x;
-          while (y > st && blist[y - 1] > v) { blist[y] = blist[y - 1]; --y; }
-          blist[y] = v;
-        }
-        uint32_t cnt_b = hc[h], l0 = hl0[h], l1 = hl1[h];
-        int32_t d0 = hd0[h], d1 = hd1[h];
-        uint2* GS = my_spill + (size_t)bb * kHashGlobal;
-        for (uint32_t x = st; x < st + nb; ++x) {
-          const uint32_t i = blist[x];
-          const uint4 e = sd[i];
-          const int32_t d = (int32_t)e.x;
-          const uint32_t sc = e.y;
-          // get(): newest cell first
-          uint32_t lev = 0;
-          bool found = false;
-          for (int32_t k = (int32_t)cnt_b - 1; k >= kHashLds; --k) {
-            const uint2 gv = GS[k - kHashLds];
-            if ((int32_t)gv.x == d) { lev = gv.y; found = true; break; }
-          }
-          if (!found && cnt_b > 1 && d1 == d) { lev = l1; found = true; }
-          if (!found && cnt_b > 0 && d0 == d) lev = l0;
-          if (sc < lev) continue;
-          ++n_evals;
-          n_steps += e.w & 0xFFFFu;
-          if ((e.w >> 16) & F_HIT) {
-            sd[i].w = e.w | ((uint32_t)F_EMIT << 16);
-            atomicAdd(&s_nemit, 1u);
-          }
-          // insert(): same diagonal, else first stale cell, else a new head cell
-          const uint32_t L = e.z;
-          bool done = false;
-          for (int32_t k = (int32_t)cnt_b - 1; k >= kHashLds && !done; --k) {
-            const uint2 gv = GS[k - kHashLds];
-            if ((int32_t)gv.x == d || (int32_t)(sc - gv.y) > window) {
-              GS[k - kHashLds] = make_uint2((uint32_t)d, L);
-              done = true;
+        uint32_t cnt_b = hc[h], l0 = hl0[h], l1 = hl1[h], l2 = hl2[h];
+        int32_t d0 = hd0[h], d1 = hd1[h], d2 = hd2[h];
+        bool slow = nb > 8 || cnt_b > (uint32_t)kHashLds;
+        uint32_t done_n = 0;  // seeds of the bucket the fast path replayed
+        if (__any(!slow && nb > 0)) {
+          uint32_t ix[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) ix[j] = (!slow && (uint32_t)j < nb) ? (uint32_t)blist[st + j] : 0xFFFFu;
+          sort8(ix);
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const bool act = !slow && (uint32_t)j < nb;
+            if (!__any(act)) break;
+            const uint32_t i = act ? ix[j] : 0u;
+            const uint4 e = sd[i];
+            const int32_t d = (int32_t)e.x;
+            const uint32_t sc = e.y;
+            // get(): newest cell first
+            const bool f2 = cnt_b > 2 && d2 == d, f1 = cnt_b > 1 && d1 == d, f0 = cnt_b > 0 && d0 == d;
+            const uint32_t lev = f2 ? l2 : (f1 ? l1 : (f0 ? l0 : 0u));
+            const bool eval = act && sc >= lev;
+            // insert(): the newest cell on the same diagonal or stale, else a
+            // new cell; a fourth cell sends the bucket to the generic loop
+            const bool u2 = cnt_b > 2 && (d2 == d || (int32_t)(sc - l2) > window);
+            const bool u1 = !u2 && cnt_b > 1 && (d1 == d || (int32_t)(sc - l1) > window);
+            const bool u0 = !u2 && !u1 && cnt_b > 0 && (d0 == d || (int32_t)(sc - l0) > window);
+            const bool grow = !u2 && !u1 && !u0;
+            const bool to_slow = eval && grow && cnt_b >= (uint32_t)kHashLds;
+            const bool apply = eval && !to_slow;
+            slow = slow || to_slow;
+            done_n = act && !to_slow ? (uint32_t)j + 1 : done_n;
+            n_evals += apply ? 1u : 0u;
+            n_steps += apply ? (e.w & 0xFFFFu) : 0u;
+            if (apply && ((e.w >> 16) & F_HIT)) {
+              sd[i].w = e.w | ((uint32_t)F_EMIT << 16);
+              ++nemit_local;
             }
-          }
-          if (!done && cnt_b > 1 && (d1 == d || (int32_t)(sc - l1) > window)) { d1 = d; l1 = L; done = true; }
-          if (!done && cnt_b > 0 && (d0 == d || (int32_t)(sc - l0) > window)) { d0 = d; l0 = L; done = true; }
-          if (!done) {
-            if (cnt_b >= (uint32_t)(kHashLds + kHashGlobal)) {
-              atomicOr(err, (uint32_t)E_HASH_FULL);
-            } else {
-              if (cnt_b == 0) { d0 = d; l0 = L; }
-              else if (cnt_b == 1) { d1 = d; l1 = L; }
-              else GS[cnt_b - kHashLds] = make_uint2((uint32_t)d, L);
-              ++cnt_b;
-            }
+            const uint32_t L = e.z;
+            const bool w2 = apply && (u2 || (grow && cnt_b == 2));
+            const bool w1 = apply && (u1 || (grow && cnt_b == 1));
+            const bool w0 = apply && (u0 || (grow && cnt_b == 0));
+            d2 = w2 ? d : d2; l2 = w2 ? L : l2;
+            d1 = w1 ? d : d1; l1 = w1 ? L : l1;
+            d0 = w0 ? d : d0; l0 = w0 ? L : l0;
+            cnt_b += (apply && grow) ? 1u : 0u;
           }
         }
-        hc[h] = cnt_b; hl0[h] = l0; hl1[h] = l1; hd0[h] = d0; hd1[h] = d1;
+        if (slow) {
+          // generic replay from seed done_n: the list sorted in LDS, cells
+          // beyond the registers in the workgroup's spill area
+          for (uint32_t x = st + 1; x < st + nb; ++x) {
+            const uint16_t v = blist[x];
+            uint32_t y = x;
+            while (y > st && blist[y - 1] > v) { blist[y] = blist[y - 1]; --y; }
+            blist[y] = v;
+          }
+          uint2* GS = my_spill + (size_t)bb * kHashGlobal;
+          for (uint32_t x = st + done_n; x < st + nb; ++x) {
+            const uint32_t i = blist[x];
+            const uint4 e = sd[i];
+            const int32_t d = (int32_t)e.x;
+            const uint32_t sc = e.y;
+            // get(): newest cell first
+            uint32_t lev = 0;
+            bool found = false;
+            for (int32_t k = (int32_t)cnt_b - 1; k >= kHashLds; --k) {
+              const uint2 gv = GS[k - kHashLds];
+              if ((int32_t)gv.x == d) { lev = gv.y; found = true; break; }
+            }
+            if (!found && cnt_b > 2 && d2 == d) { lev = l2; found = true; }
+            if (!found && cnt_b > 1 && d1 == d) { lev = l1; found = true; }
+            if (!found && cnt_b > 0 && d0 == d) lev = l0;
+            if (sc < lev) continue;
+            ++n_evals;
+            n_steps += e.w & 0xFFFFu;
+            if ((e.w >> 16) & F_HIT) {
+              sd[i].w = e.w | ((uint32_t)F_EMIT << 16);
+              ++nemit_local;
+            }
+            // insert(): same diagonal, else first stale cell, else a new head cell
+            const uint32_t L = e.z;
+            bool done = false;
+            for (int32_t k = (int32_t)cnt_b - 1; k >= kHashLds && !done; --k) {
+              const uint2 gv = GS[k - kHashLds];
+              if ((int32_t)gv.x == d || (int32_t)(sc - gv.y) > window) {
+                GS[k - kHashLds] = make_uint2((uint32_t)d, L);
+                done = true;
+              }
+            }
+            if (!done && cnt_b > 2 && (d2 == d || (int32_t)(sc - l2) > window)) { d2 = d; l2 = L; done = true; }
+            if (!done && cnt_b > 1 && (d1 == d || (int32_t)(sc - l1) > window)) { d1 = d; l1 = L; done = true; }
+            if (!done && cnt_b > 0 && (d0 == d || (int32_t)(sc - l0) > window)) { d0 = d; l0 = L; done = true; }
+            if (!done) {
+              if (cnt_b >= (uint32_t)(kHashLds + kHashGlobal)) {
+                atomicOr(err, (uint32_t)E_HASH_FULL);
+              } else {
+                if (cnt_b == 0) { d0 = d; l0 = L; }
+                else if (cnt_b == 1) { d1 = d; l1 = L; }
+                else if (cnt_b == 2) { d2 = d; l2 = L; }
+                else GS[cnt_b - kHashLds] = make_uint2((uint32_t)d, L);
+                ++cnt_b;
+              }
+            }
+          }
+        }
+        hc[h] = cnt_b; hl0[h] = l0; hl1[h] = l1; hl2[h] = l2; hd0[h] = d0; hd1[h] = d1; hd2[h] = d2;
       }
+      if (nemit_local) atomicAdd(&s_nemit, nemit_local);
       __syncthreads();
       tick(4);
       // ---- F: output slots for the pass's kept hits: the rest of the
@@ -797,42 +865,53 @@ __device__ __forceinline__ void dp_half(const DpTask& t, const int* mat, int32_t
   res.tb_rows = rows_at_best;
 }
 
-__global__ __launch_bounds__(256) void k_dp_score(const DpTask* __restrict__ tasks, uint32_t n_tasks,
-                                                  const int32_t* __restrict__ gmat, int32_t go, int32_t ge,
-                                                  int2* __restrict__ ring, uint32_t wlog2,
-                                                  DpResult* __restrict__ out, unsigned long long* counters) {
+// Traceback-phase bands (X-drop 100 by default) stay within 64 cells: the
+// first attempt gives each lane a 64-cell ring in LDS (64-lane workgroups);
+// wider bands rerun with per-lane rings in global memory.
+static constexpr uint32_t kDpLdsThreads = 64, kDpLdsLog2 = 6;
+
+template <bool LDS_RING>
+__global__ __launch_bounds__(LDS_RING ? kDpLdsThreads : 256) void k_dp_score(
+    const DpTask* __restrict__ tasks, uint32_t n_tasks, const int32_t* __restrict__ gmat, int32_t go, int32_t ge,
+    int2* __restrict__ ring, uint32_t wlog2, DpResult* __restrict__ out, unsigned long long* counters) {
   __shared__ int mat[256];
+  __shared__ int2 lring[LDS_RING ? (kDpLdsThreads << kDpLdsLog2) : 1];
   for (int i = threadIdx.x; i < 256; i += blockDim.x) mat[i] = gmat[i];
   __syncthreads();
   const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
-  int2* my = ring + (size_t)gt * ring_stride(wlog2);
+  constexpr uint32_t RS = LDS_RING ? kDpLdsThreads : 1;
+  int2* my = LDS_RING ? lring + threadIdx.x : ring + (size_t)gt * ring_stride(wlog2);
+  const uint32_t wmask = LDS_RING ? (1u << kDpLdsLog2) - 1 : (1u << wlog2) - 1;
   unsigned long long cells = 0;
   for (uint32_t i = gt; i < n_tasks; i += nt) {
     DpResult r;
-    dp_half<false>(tasks[i], mat, go, ge, my, (1u << wlog2) - 1, r, nullptr, nullptr, 0);
+    dp_half<false, RS>(tasks[i], mat, go, ge, my, wmask, r, nullptr, nullptr, 0);
     out[i] = r;
     cells += r.cells;
   }
   atomicAdd(&counters[C_DP_CELLS], cells);
 }
 
-__global__ __launch_bounds__(256) void k_dp_traceback(const TbTask* __restrict__ tasks, uint32_t n_tasks,
-                                                      const int32_t* __restrict__ gmat, int32_t go, int32_t ge,
-                                                      int2* __restrict__ ring, uint32_t wlog2,
-                                                      uint8_t* __restrict__ flat_pool, uint32_t* __restrict__ row_pool,
-                                                      uint32_t* __restrict__ ops_pool, TbResult* __restrict__ out) {
+template <bool LDS_RING>
+__global__ __launch_bounds__(LDS_RING ? kDpLdsThreads : 256) void k_dp_traceback(
+    const TbTask* __restrict__ tasks, uint32_t n_tasks, const int32_t* __restrict__ gmat, int32_t go, int32_t ge,
+    int2* __restrict__ ring, uint32_t wlog2, uint8_t* __restrict__ flat_pool, uint32_t* __restrict__ row_pool,
+    uint32_t* __restrict__ ops_pool, TbResult* __restrict__ out) {
   __shared__ int mat[256];
+  __shared__ int2 lring[LDS_RING ? (kDpLdsThreads << kDpLdsLog2) : 1];
   for (int i = threadIdx.x; i < 256; i += blockDim.x) mat[i] = gmat[i];
   __syncthreads();
   const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
-  int2* my = ring + (size_t)gt * ring_stride(wlog2);
+  constexpr uint32_t RS = LDS_RING ? kDpLdsThreads : 1;
+  int2* my = LDS_RING ? lring + threadIdx.x : ring + (size_t)gt * ring_stride(wlog2);
+  const uint32_t wmask = LDS_RING ? (1u << kDpLdsLog2) - 1 : (1u << wlog2) - 1;
   for (uint32_t i = gt; i < n_tasks; i += nt) {
     const TbTask T = tasks[i];
     uint8_t* flat = flat_pool + T.flat_off;
     uint32_t* rows = row_pool + 2 * T.row_off;
     uint32_t* ops = ops_pool + T.ops_off;
     DpResult r;
-    dp_half<true>(T.t, mat, go, ge, my, (1u << wlog2) - 1, r, flat, rows, T.stop_row);
+    dp_half<true, RS>(T.t, mat, go, ge, my, wmask, r, flat, rows, T.stop_row);
     TbResult o{0, r.flags};
     if (!r.flags) {
       // traceback walk (gapped.rs:692-757); ops are collected backwards, run-length encoded
@@ -929,7 +1008,7 @@ __device__ __forceinline__ void prelim_pivot(const UngHit& u, uint32_t qa_len, u
 // hit, and a hit whose DP falls below the cutoff never changes the containment
 // state of the ordered Phase-2a loop (engine.rs:1317-1411), so only the hits
 // that reach the cutoff are kept for the ordered replay.  Preliminary bands
-// (X-drop 30 by default) mostly stay a few cells wide: the first pass gives
+// (X-drop 30 by default) almost always fit 16 cells: the first pass gives
 // each lane a ring of 2^kPrelimRingLog2 cells of LDS, interleaved across the
 // workgroup's lanes, and lists the hits whose band outgrows it (by index) for
 // a second pass over per-lane rings of 2^wlog2 cells in global memory.
@@ -1043,7 +1122,13 @@ hipError_t launch_phase1(int blocks, P1Params P, const RecDesc* recs, const Subj
 
 hipError_t launch_dp_score(int blocks, const DpTask* tasks, uint32_t n, const int32_t* gmat, int32_t go, int32_t ge,
                            int2* ring, uint32_t wlog2, DpResult* out, unsigned long long* counters, hipStream_t st) {
-  hipLaunchKernelGGL(k_dp_score, dim3(blocks), dim3(256), 0, st, tasks, n, gmat, go, ge, ring, wlog2, out, counters);
+  // ring == nullptr: LDS rings, `blocks` workgroups of kDpLdsThreads lanes
+  if (!ring)
+    hipLaunchKernelGGL(k_dp_score<true>, dim3(blocks), dim3(kDpLdsThreads), 0, st, tasks, n, gmat, go, ge, ring,
+                       wlog2, out, counters);
+  else
+    hipLaunchKernelGGL(k_dp_score<false>, dim3(blocks), dim3(256), 0, st, tasks, n, gmat, go, ge, ring, wlog2, out,
+                       counters);
   return hipGetLastError();
 }
 
@@ -1057,8 +1142,13 @@ hipError_t launch_pack_ops(const PackTask* tasks, uint32_t n, const uint32_t* op
 hipError_t launch_dp_traceback(int blocks, const TbTask* tasks, uint32_t n, const int32_t* gmat, int32_t go,
                                int32_t ge, int2* ring, uint32_t wlog2, uint8_t* flat, uint32_t* rows, uint32_t* ops,
                                TbResult* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_dp_traceback, dim3(blocks), dim3(256), 0, st, tasks, n, gmat, go, ge, ring, wlog2, flat, rows,
-                     ops, out);
+  // ring == nullptr: LDS rings, `blocks` workgroups of kDpLdsThreads lanes
+  if (!ring)
+    hipLaunchKernelGGL(k_dp_traceback<true>, dim3(blocks), dim3(kDpLdsThreads), 0, st, tasks, n, gmat, go, ge, ring,
+                       wlog2, flat, rows, ops, out);
+  else
+    hipLaunchKernelGGL(k_dp_traceback<false>, dim3(blocks), dim3(256), 0, st, tasks, n, gmat, go, ge, ring, wlog2,
+                       flat, rows, ops, out);
   return hipGetLastError();
 }
 

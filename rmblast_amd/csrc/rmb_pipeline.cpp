@@ -288,11 +288,11 @@ static void run_dp_score(rmb_ctx* c, const std::vector<DpTask>& tasks_in, std::v
     DpTask* dt = c->b_tasks.get<DpTask>(n);
     DpResult* dr = c->b_dpres.get<DpResult>(n);
     hip_check(hipMemcpyAsync(dt, tasks.data() + base, n * sizeof(DpTask), hipMemcpyHostToDevice, c->st), "copy tasks");
-    const uint32_t lanes = std::min<uint32_t>((uint32_t)c->cu_count * 4 * 64, ((n + 255) / 256) * 256);
-    int2* ring = c->b_ring.get<int2>((size_t)lanes * ring_stride(kRingLog2));
+    // 64-lane workgroups with a 32 KB LDS band ring each, four per CU
+    const uint32_t blocks = std::min<uint32_t>((uint32_t)c->cu_count * 4, (n + 63) / 64);
     timed_launch(c, 14, [&] {
-      hip_check(launch_dp_score((int)(lanes / 256), dt, n, c->d_mat, c->P.gap_open, c->P.gap_extend, ring,
-                                kRingLog2, dr, ctr, c->st),
+      hip_check(launch_dp_score((int)blocks, dt, n, c->d_mat, c->P.gap_open, c->P.gap_extend, nullptr, 0, dr, ctr,
+                                c->st),
                 "k_dp_score");
     });
     hip_check(hipMemcpyAsync(res_o.data() + base, dr, n * sizeof(DpResult), hipMemcpyDeviceToHost, c->st), "copy dp");
@@ -396,11 +396,10 @@ static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::
     uint32_t* dops = c->b_ops.get<uint32_t>(opsn + 16);
     hip_check(hipMemcpyAsync(dt, batch.data(), n * sizeof(TbTask), hipMemcpyHostToDevice, c->st), "copy tb tasks");
     std::vector<TbResult> tr(n);
-    const uint32_t lanes = std::min<uint32_t>((uint32_t)c->cu_count * 4 * 64, ((n + 255) / 256) * 256);
-    int2* ring = c->b_ring.get<int2>((size_t)lanes * ring_stride(kRingLog2));
+    const uint32_t blocks = std::min<uint32_t>((uint32_t)c->cu_count * 4, (n + 63) / 64);
     timed_launch(c, 14, [&] {
-      hip_check(launch_dp_traceback((int)(lanes / 256), dt, n, c->d_mat, c->P.gap_open, c->P.gap_extend, ring,
-                                    kRingLog2, dflat, drows, dops, dr, c->st),
+      hip_check(launch_dp_traceback((int)blocks, dt, n, c->d_mat, c->P.gap_open, c->P.gap_extend, nullptr, 0, dflat,
+                                    drows, dops, dr, c->st),
                 "k_dp_traceback");
     });
     hip_check(hipMemcpyAsync(tr.data(), dr, n * sizeof(TbResult), hipMemcpyDeviceToHost, c->st), "copy tb res");
