@@ -64,6 +64,8 @@ struct PrelimParams {
   int32_t xdrop_gap, gap_open, gap_extend, cutoff, lut_width;
   uint32_t n_subj;
   uint64_t n_hits, out_cap, redo_cap;
+  const unsigned long long* n_dev;  // when set, the task count is min(*n_dev, n_hits)
+  const uint32_t* rec_ids;          // group-local record -> search record, for output item ids
 };
 
 // Error bits written by kernels.

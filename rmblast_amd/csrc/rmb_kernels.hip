@@ -1011,7 +1011,10 @@ __device__ __forceinline__ void prelim_pivot(const UngHit& u, uint32_t qa_len, u
 // (X-drop 30 by default) almost always fit 16 cells: the first pass gives
 // each lane a ring of 2^kPrelimRingLog2 cells of LDS, interleaved across the
 // workgroup's lanes, and lists the hits whose band outgrows it (by index) for
-// a second pass over per-lane rings of 2^wlog2 cells in global memory.
+// a second pass over per-lane rings of 2^wlog2 cells in global memory, which
+// hands the hits wider still back as records.  Output item ids name search
+// records (P.rec_ids); the task count may come from device memory (P.n_dev),
+// so the passes queue behind the seeding kernel without a host round trip.
 static constexpr uint32_t kPrelimThreads = 128, kPrelimRingLog2 = 4;
 
 template <bool LDS_RING>
@@ -1020,8 +1023,8 @@ __global__ __launch_bounds__(kPrelimThreads) void k_prelim_fused(
     const RecDesc* __restrict__ recs, const SubjDesc* __restrict__ subjs, const uint8_t* __restrict__ qfwd,
     const uint8_t* __restrict__ qrc, const uint8_t* __restrict__ splus, const int32_t* __restrict__ gmat,
     int2* __restrict__ gring, uint32_t wlog2, PreHit* __restrict__ out, unsigned long long* __restrict__ out_count,
-    uint32_t* __restrict__ redo, unsigned long long* __restrict__ redo_count, unsigned long long* __restrict__ counters,
-    uint32_t* __restrict__ err) {
+    uint32_t* __restrict__ redo, UngHit* __restrict__ redo_hits, unsigned long long* __restrict__ redo_count,
+    unsigned long long* __restrict__ counters, uint32_t* __restrict__ err) {
   __shared__ int mat[256];
   __shared__ int2 lring[LDS_RING ? (kPrelimThreads << kPrelimRingLog2) : 1];
   for (int i = threadIdx.x; i < 256; i += blockDim.x) mat[i] = gmat[i];
@@ -1031,6 +1034,7 @@ __global__ __launch_bounds__(kPrelimThreads) void k_prelim_fused(
   int2* my = LDS_RING ? lring + threadIdx.x : gring + gt * ring_stride(wlog2);
   const uint32_t wmask = LDS_RING ? (1u << kPrelimRingLog2) - 1 : (1u << wlog2) - 1;
   unsigned long long cells = 0, dps = 0;
+  if (P.n_dev) n_tasks = *P.n_dev < n_tasks ? *P.n_dev : n_tasks;
   for (uint64_t k = gt; k < n_tasks; k += nt) {
     const uint32_t i = idx ? idx[k] : (uint32_t)k;
     const UngHit u = hits[i];
@@ -1051,8 +1055,15 @@ __global__ __launch_bounds__(kPrelimThreads) void k_prelim_fused(
     dp_half<false, RS>(right, mat, P.gap_open, P.gap_extend, my, wmask, Rr, nullptr, nullptr, 0);
     if ((L.flags | Rr.flags) & E_BAND_WIDE) {
       const unsigned long long kk = atomicAdd(redo_count, 1ull);
-      if (kk < P.redo_cap) redo[kk] = i;
-      else atomicOr(err, (uint32_t)E_OUT_FULL);
+      if (kk < P.redo_cap) {
+        if (LDS_RING) {
+          redo[kk] = i;
+        } else {
+          UngHit v = u;
+          v.item = P.rec_ids[r] * P.n_subj + f;
+          redo_hits[kk] = v;
+        }
+      }
       continue;
     }
     ++dps;
@@ -1062,6 +1073,7 @@ __global__ __launch_bounds__(kPrelimThreads) void k_prelim_fused(
     const uint32_t q0 = pq - L.b_len, q1 = pq + Rr.b_len, s0 = ps - L.a_len, s1 = ps + Rr.a_len;
     PreHit h;
     h.u = u;
+    h.u.item = P.rec_ids[r] * P.n_subj + f;
     h.score = sc;
     if (u.strand) { h.q_start = ql - q1; h.q_end = ql - q0; h.s_start = sl - s1; h.s_end = sl - s0; }
     else { h.q_start = q0; h.q_end = q1; h.s_start = s0; h.s_end = s1; }
@@ -1069,7 +1081,6 @@ __global__ __launch_bounds__(kPrelimThreads) void k_prelim_fused(
     h.ts = ts;
     const unsigned long long kk = atomicAdd(out_count, 1ull);
     if (kk < P.out_cap) out[kk] = h;
-    else atomicOr(err, (uint32_t)E_OUT_FULL);
   }
   atomicAdd(&counters[C_PRELIM_CELLS], cells);
   atomicAdd(&counters[C_PRELIM_DPS], dps);
@@ -1080,14 +1091,16 @@ hipError_t launch_prelim_fused(int blocks, bool lds_ring, PrelimParams P, const 
                                uint64_t n_tasks, const RecDesc* recs, const SubjDesc* subjs, const uint8_t* qfwd,
                                const uint8_t* qrc, const uint8_t* splus, const int32_t* gmat, int2* gring,
                                uint32_t wlog2, PreHit* out, unsigned long long* out_count, uint32_t* redo,
-                               unsigned long long* redo_count, unsigned long long* counters, uint32_t* err,
-                               hipStream_t st) {
+                               UngHit* redo_hits, unsigned long long* redo_count, unsigned long long* counters,
+                               uint32_t* err, hipStream_t st) {
   if (lds_ring)
     hipLaunchKernelGGL(k_prelim_fused<true>, dim3(blocks), dim3(kPrelimThreads), 0, st, P, hits, idx, n_tasks, recs,
-                       subjs, qfwd, qrc, splus, gmat, gring, wlog2, out, out_count, redo, redo_count, counters, err);
+                       subjs, qfwd, qrc, splus, gmat, gring, wlog2, out, out_count, redo, redo_hits, redo_count,
+                       counters, err);
   else
     hipLaunchKernelGGL(k_prelim_fused<false>, dim3(blocks), dim3(kPrelimThreads), 0, st, P, hits, idx, n_tasks, recs,
-                       subjs, qfwd, qrc, splus, gmat, gring, wlog2, out, out_count, redo, redo_count, counters, err);
+                       subjs, qfwd, qrc, splus, gmat, gring, wlog2, out, out_count, redo, redo_hits, redo_count,
+                       counters, err);
   return hipGetLastError();
 }
 

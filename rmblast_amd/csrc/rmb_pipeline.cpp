@@ -33,6 +33,7 @@
 #include "rmb_dust.h"
 
 #include <map>
+#include <unordered_map>
 #include "rmb_replay.h"
 
 namespace rmb {
@@ -52,7 +53,7 @@ hipError_t launch_dp_traceback(int, const TbTask*, uint32_t, const int32_t*, int
                                uint8_t*, uint32_t*, uint32_t*, TbResult*, hipStream_t);
 hipError_t launch_prelim_fused(int, bool, PrelimParams, const UngHit*, const uint32_t*, uint64_t, const RecDesc*,
                                const SubjDesc*, const uint8_t*, const uint8_t*, const uint8_t*, const int32_t*, int2*,
-                               uint32_t, PreHit*, unsigned long long*, uint32_t*, unsigned long long*,
+                               uint32_t, PreHit*, unsigned long long*, uint32_t*, UngHit*, unsigned long long*,
                                unsigned long long*, uint32_t*, hipStream_t);
 
 struct Failure : std::runtime_error {
@@ -176,6 +177,14 @@ struct Subject {
   std::vector<uint8_t> n_mask, n_mask_rc;
 };
 
+// Device outputs a queued Phase-1 batch keeps until the host has read them;
+// the context holds two, so one batch is read while the next one runs.
+struct P1Slot {
+  DevBuf pre, redo_hits, cnt, hitcnt, ctr, err;
+  std::vector<hipEvent_t> ev;  // per group: k_phase1, prelim pass 1, prelim pass 2 (start, stop)
+  hipEvent_t done = nullptr;
+};
+
 static std::vector<uint8_t> wrap(const uint8_t* s, uint32_t n) {
   std::vector<uint8_t> v(n + 2);
   v[0] = 15;
@@ -195,7 +204,10 @@ using namespace rmb;
 
 struct rmb_ctx {
   int device = 0;
-  hipStream_t st = nullptr;
+  hipStream_t st = nullptr;   // Phase 2b and the taps
+  hipStream_t st1 = nullptr;  // seeding and preliminary DPs
+  P1Slot slot[2];
+  double hits_per_item = 0;   // most ungapped hits per searched item seen, sizes the hit buffers
   rmb_params P{};
   Scoring M{};
   std::string err;
@@ -215,12 +227,13 @@ struct rmb_ctx {
   DevBuf b_qfwd, b_qrc, b_recs, b_elem, b_keys, b_keys2, b_vals, b_vals2, b_counts, b_cells, b_sort_tmp;
   DevBuf b_hits, b_hitcount, b_spill, b_hitres, b_queue, b_counters, b_err;
   DevBuf b_tasks, b_dpres, b_ring, b_ring_big, b_tbtasks, b_tbres, b_flat, b_rows, b_ops;
-  DevBuf b_pre, b_redo, b_precount, b_packtasks, b_packed, b_items, b_qext;
+  DevBuf b_pre, b_redo, b_precount, b_packtasks, b_packed, b_items, b_qext, b_recids;
 };
 
 namespace rmb {
 
-static const int kSpecDepth = 6;        // Phase-2a DPs requested per item per round
+static const int kMaxBatches = 8;                   // Phase-1 batches a search is cut into (RMB_BATCHES overrides)
+static const uint64_t kBatchMinBases = 4000000;     // bases per batch at least
 static const uint32_t kRingLog2 = 10;   // per-lane band ring for the common case
 static const uint32_t kRingBigLog2 = 17;
 
@@ -465,139 +478,45 @@ struct ItemState {
   std::vector<FinalRec> fin;
 };
 
-// Preliminary gapped DPs of every ungapped hit of a group, on the GPU; the hits
-// reaching the gapped cutoff come back with their DP results (item ids mapped
-// to batch records).  Hits whose band outgrew the lane ring are redone here
-// with the wide ring.
-static void prelim_group(rmb_ctx* c, const std::vector<RecDesc>& recs, const std::vector<uint32_t>& recs_idx,
-                         const RecDesc* drecs, int lw, const uint8_t* d_qfwd, const uint8_t* d_qrc,
-                         const UngHit* dh, uint64_t nh, std::vector<PreHit>& out) {
-  const uint32_t nsub = (uint32_t)c->subj.size();
-  const int32_t cutoff = std::max(c->P.min_raw_gapped_score, 1);
-  PrelimParams PP{};
-  PP.xdrop_gap = c->P.xdrop_gap;
-  PP.gap_open = c->P.gap_open;
-  PP.gap_extend = c->P.gap_extend;
-  PP.cutoff = cutoff;
-  PP.lut_width = lw;
-  PP.n_subj = nsub;
-  PP.n_hits = nh;
-  PP.out_cap = nh;
-  PP.redo_cap = nh;
-  if (nh > 0xFFFFFFFFull) throw Failure(RMB_ERR_CAPACITY, "more than 2^32 ungapped hits in one group");
-  PreHit* dout = c->b_pre.get<PreHit>(PP.out_cap);
-  uint32_t* dredo = c->b_redo.get<uint32_t>(2 * nh);  // hit indices: wide bands, then bands wider still
-  unsigned long long* cnt = c->b_precount.get<unsigned long long>(3);
-  uint32_t* err = c->b_err.get<uint32_t>(1);
-  unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
-  hip_check(hipMemsetAsync(cnt, 0, 3 * sizeof(unsigned long long), c->st), "memset prelim counts");
-  hip_check(hipMemsetAsync(err, 0, sizeof(uint32_t), c->st), "memset err");
-  const uint64_t before = c->stats[14];
-  // pass 1: 128-lane workgroups with a 16 KB LDS band ring each, eight per CU
-  const int blocks = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->cu_count * 8, (nh + 127) / 128));
-  timed_launch(c, 14, [&] {
-    hip_check(launch_prelim_fused(blocks, true, PP, dh, nullptr, nh, drecs, (const SubjDesc*)c->d_sdesc.p, d_qfwd,
-                                  d_qrc, (const uint8_t*)c->d_splus.p, c->d_mat, nullptr, 0, dout, cnt, dredo, cnt + 1,
-                                  ctr, err, c->st),
-              "k_prelim_fused");
-  });
-  unsigned long long n[3];
-  uint32_t e = 0;
-  hip_check(hipMemcpyAsync(n, cnt, sizeof n, hipMemcpyDeviceToHost, c->st), "copy prelim counts");
-  hip_check(hipStreamSynchronize(c->st), "prelim sync");
-  if (n[1]) {
-    // pass 2: the hits whose band outgrew the LDS ring, with per-lane rings
-    // of 2^kRingLog2 cells in global memory
-    const uint64_t lanes = std::min<uint64_t>((uint64_t)c->cu_count * 8 * 128, ((n[1] + 127) / 128) * 128);
-    int2* ring = c->b_ring.get<int2>((size_t)lanes * ring_stride(kRingLog2));
-    timed_launch(c, 14, [&] {
-      hip_check(launch_prelim_fused((int)(lanes / 128), false, PP, dh, dredo, n[1], drecs,
-                                    (const SubjDesc*)c->d_sdesc.p, d_qfwd, d_qrc, (const uint8_t*)c->d_splus.p,
-                                    c->d_mat, ring, kRingLog2, dout, cnt, dredo + nh, cnt + 2, ctr, err, c->st),
-                "k_prelim_fused wide");
-    });
-    hip_check(hipMemcpyAsync(n, cnt, sizeof n, hipMemcpyDeviceToHost, c->st), "copy prelim counts");
-  }
-  hip_check(hipMemcpyAsync(&e, err, sizeof e, hipMemcpyDeviceToHost, c->st), "copy err");
-  hip_check(hipStreamSynchronize(c->st), "prelim sync");
-  c->stats[23] += c->stats[14] - before;
-  if (e & E_OUT_FULL) throw Failure(RMB_ERR_CAPACITY, "preliminary DP output overflow");
-  const size_t base = out.size();
-  out.resize(base + n[0]);
-  hip_check(hipMemcpy(out.data() + base, dout, n[0] * sizeof(PreHit), hipMemcpyDeviceToHost), "copy prelims");
-  // bands wider than 2^kRingLog2 cells: the host-driven path with wider rings
-  std::vector<UngHit> redo(n[2]);
-  if (n[2]) {
-    std::vector<uint32_t> ri(n[2]);
-    hip_check(hipMemcpy(ri.data(), dredo + nh, n[2] * sizeof(uint32_t), hipMemcpyDeviceToHost), "copy redo");
-    for (size_t k = 0; k < ri.size(); ++k)
-      hip_check(hipMemcpy(&redo[k], dh + ri[k], sizeof(UngHit), hipMemcpyDeviceToHost), "copy redo hit");
-    c->host_prelim_dps += n[2];
-  }
-  for (size_t i = base; i < out.size(); ++i) {
-    const uint32_t r = out[i].u.item / nsub, f = out[i].u.item % nsub;
-    out[i].u.item = recs_idx[r] * nsub + f;
-  }
-  if (redo.empty()) return;
-  std::vector<DpTask> tasks;
-  std::vector<std::array<uint32_t, 4>> piv;
-  for (const UngHit& h : redo) {
-    const uint32_t r = h.item / nsub, f = h.item % nsub;
-    const uint32_t ql = recs[r].q_len, sl = c->sdesc[f].s_len;
-    UngRec u{h.q_seed, h.s_seed, h.q_start, h.q_end, h.s_start, h.s_end, h.score, (uint8_t)h.strand};
-    uint32_t pq, ps, tq, ts;
-    pivot_of(u, ql, sl, (uint32_t)lw, pq, ps, tq, ts);
-    piv.push_back({pq, ps, tq, ts});
-    const uint8_t* qa = (h.strand ? d_qrc : d_qfwd) + recs[r].q_base + 1;
-    const uint8_t* sa = (const uint8_t*)c->d_splus.p + c->sdesc[f].s_base + 1;
-    const int32_t xd = std::min(c->P.xdrop_gap, h.score);
-    tasks.push_back(DpTask{sa, qa, ps, pq, xd, 1});
-    tasks.push_back(DpTask{sa + ps, qa + pq, sl > ps ? sl - ps : 0, ql > pq ? ql - pq : 0, xd, 0});
-  }
-  std::vector<DpResult> res;
-  run_dp_score(c, tasks, res);
-  for (size_t k = 0; k < redo.size(); ++k) {
-    const UngHit& h = redo[k];
-    const DpResult &L = res[2 * k], &R = res[2 * k + 1];
-    const int32_t sc = L.score + R.score;
-    if (sc < 0 || sc < cutoff) continue;
-    const uint32_t r = h.item / nsub, f = h.item % nsub;
-    const uint32_t ql = recs[r].q_len, sl = c->sdesc[f].s_len;
-    const auto& pv = piv[k];
-    const uint32_t q0 = pv[0] - L.b_len, q1 = pv[0] + R.b_len, s0 = pv[1] - L.a_len, s1 = pv[1] + R.a_len;
-    PreHit p;
-    p.u = h;
-    p.u.item = recs_idx[r] * nsub + f;
-    p.score = sc;
-    if (h.strand) { p.q_start = ql - q1; p.q_end = ql - q0; p.s_start = sl - s1; p.s_end = sl - s0; }
-    else { p.q_start = q0; p.q_end = q1; p.s_start = s0; p.s_end = s1; }
-    p.tq = pv[2];
-    p.ts = pv[3];
-    out.push_back(p);
-  }
-}
+// ---- Phase 1 and the preliminary DPs, queued per batch of search records ----
+//
+// A batch runs on the context's first stream as one queue of kernels per
+// lookup-width group: table build, k_phase1, and the two preliminary-DP
+// passes, which read their task counts from device memory, so nothing waits
+// on the host.  The host reads a batch back while the next batch runs, and
+// meanwhile replays and aligns the batch before (Phase 2a/2b on the second
+// stream and the host threads).
 
-// Phase 1 for a group of records sharing one lookup width; with pre_out, also
-// the preliminary DPs of its hits.
-static void phase1_group(rmb_ctx* c, const std::vector<uint32_t>& recs_idx, const std::vector<RecDesc>& all_recs,
-                         int lw, const uint8_t* d_qfwd, const uint8_t* d_qrc, const uint8_t* d_qext,
-                         std::vector<UngHit>* hits_out, std::vector<PreHit>* pre_out,
-                         const std::vector<uint32_t>* items = nullptr) {
-  const int ws = c->P.word_size;
-  const uint32_t R = (uint32_t)recs_idx.size();
+struct P1Group {
+  int lw = 0;
+  std::vector<uint32_t> recs_idx;  // search records
+  std::vector<uint32_t> items;     // group-local items to search (listed pairs); unused when !listed
+  bool listed = false;
+  uint64_t hits_cap = 0;
+  // host copies that queued transfers read
+  std::vector<RecDesc> recs;
+  std::vector<uint64_t> elem_base;
+};
+
+struct P1Batch {
+  int slot = 0;
+  std::vector<P1Group> groups;
+  uint64_t redo_cap = 0;
+};
+
+// Per-group seeding launch: lookup table for the group's records, then
+// k_phase1 writing up to `cap` hits to dh (count at *hitcount).
+static void launch_group_phase1(rmb_ctx* c, hipStream_t st, P1Group& g, const uint8_t* d_qfwd, const uint8_t* d_qrc,
+                                const uint8_t* d_qext, UngHit* dh, uint64_t cap, unsigned long long* hitcount,
+                                unsigned long long* ctr, uint32_t* err, hipEvent_t ev_a, hipEvent_t ev_b) {
+  const int ws = c->P.word_size, lw = g.lw;
+  const uint32_t R = (uint32_t)g.recs_idx.size();
   const uint64_t cells = 1ull << (2 * lw);
-  std::vector<RecDesc> recs(R);
-  std::vector<uint64_t> elem_base(R + 1, 0);
-  for (uint32_t i = 0; i < R; ++i) {
-    recs[i] = all_recs[recs_idx[i]];
-    recs[i].cell_base = (uint64_t)i * cells;
-    elem_base[i + 1] = elem_base[i] + 2ull * recs[i].q_len + 1;
-  }
-  const uint64_t E = elem_base[R];
+  const uint64_t E = g.elem_base[R];
   RecDesc* drecs = c->b_recs.get<RecDesc>(R);
   uint64_t* delem = c->b_elem.get<uint64_t>(R + 1);
-  hip_check(hipMemcpyAsync(drecs, recs.data(), R * sizeof(RecDesc), hipMemcpyHostToDevice, c->st), "copy recs");
-  hip_check(hipMemcpyAsync(delem, elem_base.data(), (R + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, c->st),
+  hip_check(hipMemcpyAsync(drecs, g.recs.data(), R * sizeof(RecDesc), hipMemcpyHostToDevice, st), "copy recs");
+  hip_check(hipMemcpyAsync(delem, g.elem_base.data(), (R + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, st),
             "copy elem");
   uint64_t* keys = c->b_keys.get<uint64_t>(E);
   uint64_t* keys2 = c->b_keys2.get<uint64_t>(E);
@@ -606,19 +525,18 @@ static void phase1_group(rmb_ctx* c, const std::vector<uint32_t>& recs_idx, cons
   const uint64_t ncell = (uint64_t)R * cells + 1;
   uint32_t* counts = c->b_counts.get<uint32_t>(ncell);
   uint32_t* cstart = c->b_cells.get<uint32_t>(ncell);
-  hip_check(hipMemsetAsync(counts, 0, ncell * sizeof(uint32_t), c->st), "memset counts");
+  hip_check(hipMemsetAsync(counts, 0, ncell * sizeof(uint32_t), st), "memset counts");
   const uint64_t invalid = (uint64_t)R << (2 * lw);
   int end_bit = 2 * lw;
   while ((1ull << (end_bit - 2 * lw)) <= R) ++end_bit;
   const int desc = lw > 8 ? 1 : 0;
-  hip_check(launch_lut_keys(drecs, R, delem, d_qext, lw, ws, desc, keys, vals, counts, invalid, E, c->st), "lut keys");
+  hip_check(launch_lut_keys(drecs, R, delem, d_qext, lw, ws, desc, keys, vals, counts, invalid, E, st), "lut keys");
   size_t tmp1 = 0, tmp2 = 0;
-  hip_check(sort_pairs_u64(nullptr, tmp1, keys, keys2, vals, vals2, E, end_bit, c->st), "sort size");
-  hip_check(exclusive_sum_u32(nullptr, tmp2, counts, cstart, ncell, c->st), "scan size");
+  hip_check(sort_pairs_u64(nullptr, tmp1, keys, keys2, vals, vals2, E, end_bit, st), "sort size");
+  hip_check(exclusive_sum_u32(nullptr, tmp2, counts, cstart, ncell, st), "scan size");
   void* tmp = c->b_sort_tmp.get<uint8_t>(std::max(tmp1, tmp2));
-  hip_check(sort_pairs_u64(tmp, tmp1, keys, keys2, vals, vals2, E, end_bit, c->st), "radix sort");
-  hip_check(exclusive_sum_u32(tmp, tmp2, counts, cstart, ncell, c->st), "scan");
-  // Phase 1
+  hip_check(sort_pairs_u64(tmp, tmp1, keys, keys2, vals, vals2, E, end_bit, st), "radix sort");
+  hip_check(exclusive_sum_u32(tmp, tmp2, counts, cstart, ncell, st), "scan");
   const uint32_t nsub = (uint32_t)c->subj.size();
   P1Params P{};
   P.word_size = ws;
@@ -630,77 +548,285 @@ static void phase1_group(rmb_ctx* c, const std::vector<uint32_t>& recs_idx, cons
   P.n_subj = nsub;
   P.n_items = (uint64_t)R * nsub;
   P.item_list = nullptr;
-  if (items) {  // group-local item ids (record index in the group * nsub + subject)
-    if (items->empty()) return;
-    uint32_t* dl = c->b_items.get<uint32_t>(items->size());
-    hip_check(hipMemcpyAsync(dl, items->data(), items->size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->st),
+  P.out_cap = cap;
+  if (g.listed) {  // group-local item ids (record index in the group * nsub + subject)
+    uint32_t* dl = c->b_items.get<uint32_t>(g.items.size());
+    hip_check(hipMemcpyAsync(dl, g.items.data(), g.items.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st),
               "copy items");
     P.item_list = dl;
-    P.n_items = items->size();
+    P.n_items = g.items.size();
   }
-  // ~36 KB of LDS per workgroup: four resident per CU.  Workgroups are dealt
+  // ~25 KB of LDS per workgroup: four resident per CU.  Workgroups are dealt
   // round-robin over the XCDs, so blockIdx % kXcds names the XCD.
   const int blocks = std::max(1, std::min<int>(c->cu_count * 4, (int)std::min<uint64_t>(P.n_items, 1u << 20)));
   uint2* spill = c->b_spill.get<uint2>((size_t)blocks * kBuckets * kHashGlobal);
   uint4* hitres = c->b_hitres.get<uint4>((size_t)blocks * kSeedCap);
   unsigned long long* queue = c->b_queue.get<unsigned long long>(kXcds);
-  unsigned long long* hitcount = c->b_hitcount.get<unsigned long long>(1);
-  unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
-  uint32_t* err = c->b_err.get<uint32_t>(1);
-  uint64_t cap = std::max<uint64_t>(1u << 16, c->b_hits.cap / sizeof(UngHit));
-  // counters as they stood before this group, restored when the group reruns
-  unsigned long long ctr_saved[C_NCOUNTERS];
-  hip_check(hipMemcpyAsync(ctr_saved, ctr, sizeof ctr_saved, hipMemcpyDeviceToHost, c->st), "save counters");
-  hip_check(hipStreamSynchronize(c->st), "save counters sync");
-  for (int attempt = 0; attempt < 3; ++attempt) {
-    UngHit* dh = c->b_hits.get<UngHit>(cap);
-    cap = c->b_hits.cap / sizeof(UngHit);
-    P.out_cap = cap;
-    hip_check(hipMemsetAsync(queue, 0, kXcds * sizeof(unsigned long long), c->st), "memset queue");
-    hip_check(hipMemsetAsync(hitcount, 0, sizeof(unsigned long long), c->st), "memset hitcount");
-    hip_check(hipMemsetAsync(err, 0, sizeof(uint32_t), c->st), "memset err");
-    hip_check(hipEventRecord(c->ev0, c->st), "event");
-    hip_check(launch_phase1(blocks, P, drecs, (const SubjDesc*)c->d_sdesc.p, d_qfwd, d_qrc, d_qext,
-                            (const uint8_t*)c->d_splus.p, (const uint8_t*)c->d_src.p, cstart, vals2, c->d_mat, dh,
-                            hitcount, spill, hitres, queue, ctr, err, c->st),
-              "k_phase1");
-    hip_check(hipEventRecord(c->ev1, c->st), "event");
-    unsigned long long nh = 0;
+  hip_check(hipMemsetAsync(queue, 0, kXcds * sizeof(unsigned long long), st), "memset queue");
+  if (ev_a) hip_check(hipEventRecord(ev_a, st), "event");
+  hip_check(launch_phase1(blocks, P, drecs, (const SubjDesc*)c->d_sdesc.p, d_qfwd, d_qrc, d_qext,
+                          (const uint8_t*)c->d_splus.p, (const uint8_t*)c->d_src.p, cstart, vals2, c->d_mat, dh,
+                          hitcount, spill, hitres, queue, ctr, err, st),
+            "k_phase1");
+  if (ev_b) hip_check(hipEventRecord(ev_b, st), "event");
+}
+
+// Estimated hit capacity for a group: hits per searched item seen so far.
+static uint64_t hits_estimate(const rmb_ctx* c, const P1Group& g) {
+  const uint64_t n_items = g.listed ? g.items.size() : (uint64_t)g.recs_idx.size() * c->subj.size();
+  const double per = c->hits_per_item > 0 ? c->hits_per_item : 0.0;
+  return std::max<uint64_t>(1u << 20, (uint64_t)(per * 1.25 * (double)n_items) + (1u << 16));
+}
+
+static void ensure_events(std::vector<hipEvent_t>& ev, size_t n) {
+  while (ev.size() < n) {
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    ev.push_back(e);
+  }
+}
+
+// Queue a batch on the seeding stream.
+static void enqueue_batch(rmb_ctx* c, P1Batch& b, const uint8_t* d_qfwd, const uint8_t* d_qrc,
+                          const uint8_t* d_qext) {
+  hipStream_t st = c->st1;
+  P1Slot& S = c->slot[b.slot];
+  const size_t ng = b.groups.size();
+  uint64_t max_cap = 0, pre_cap = 0;
+  for (auto& g : b.groups) {
+    if (!g.hits_cap) g.hits_cap = hits_estimate(c, g);
+    max_cap = std::max(max_cap, g.hits_cap);
+    pre_cap += g.hits_cap;
+  }
+  if (!b.redo_cap) b.redo_cap = std::max<uint64_t>(4096, pre_cap / 64);
+  if (max_cap > 0xFFFFFFFFull) throw Failure(RMB_ERR_CAPACITY, "more than 2^32 ungapped hits in one group");
+  PreHit* dpre = S.pre.get<PreHit>(pre_cap);
+  UngHit* dredoh = S.redo_hits.get<UngHit>(b.redo_cap);
+  unsigned long long* cnt = S.cnt.get<unsigned long long>(4);           // [0] prelims, [2] wide hits
+  unsigned long long* hitcnt = S.hitcnt.get<unsigned long long>(2 * ng + 2);  // per group: hits, wide (pass 1)
+  unsigned long long* ctr = S.ctr.get<unsigned long long>(C_NCOUNTERS);
+  uint32_t* err = S.err.get<uint32_t>(1);
+  hip_check(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned long long), st), "memset counts");
+  hip_check(hipMemsetAsync(hitcnt, 0, (2 * ng + 2) * sizeof(unsigned long long), st), "memset hit counts");
+  hip_check(hipMemsetAsync(ctr, 0, C_NCOUNTERS * sizeof(unsigned long long), st), "memset counters");
+  hip_check(hipMemsetAsync(err, 0, sizeof(uint32_t), st), "memset err");
+  ensure_events(S.ev, 6 * ng);
+  UngHit* dh = c->b_hits.get<UngHit>(max_cap);
+  uint32_t* dredo = c->b_redo.get<uint32_t>(max_cap);
+  const uint32_t nsub = (uint32_t)c->subj.size();
+  for (size_t gi = 0; gi < ng; ++gi) {
+    P1Group& g = b.groups[gi];
+    if (g.listed && g.items.empty()) continue;
+    launch_group_phase1(c, st, g, d_qfwd, d_qrc, d_qext, dh, g.hits_cap, hitcnt + 2 * gi, ctr, err, S.ev[6 * gi],
+                        S.ev[6 * gi + 1]);
+    const RecDesc* drecs = (const RecDesc*)c->b_recs.p;
+    uint32_t* dids = c->b_recids.get<uint32_t>(g.recs_idx.size());
+    hip_check(hipMemcpyAsync(dids, g.recs_idx.data(), g.recs_idx.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                             st),
+              "copy record ids");
+    PrelimParams PP{};
+    PP.xdrop_gap = c->P.xdrop_gap;
+    PP.gap_open = c->P.gap_open;
+    PP.gap_extend = c->P.gap_extend;
+    PP.cutoff = std::max(c->P.min_raw_gapped_score, 1);
+    PP.lut_width = g.lw;
+    PP.n_subj = nsub;
+    PP.out_cap = pre_cap;
+    PP.rec_ids = dids;
+    // pass 1: 128-lane workgroups with a 16 KB LDS band ring each, eight per CU
+    PP.n_hits = g.hits_cap;
+    PP.n_dev = hitcnt + 2 * gi;
+    PP.redo_cap = g.hits_cap;
+    hip_check(hipEventRecord(S.ev[6 * gi + 2], st), "event");
+    hip_check(launch_prelim_fused(c->cu_count * 8, true, PP, dh, nullptr, g.hits_cap, drecs,
+                                  (const SubjDesc*)c->d_sdesc.p, d_qfwd, d_qrc, (const uint8_t*)c->d_splus.p,
+                                  c->d_mat, nullptr, 0, dpre, cnt, dredo, nullptr, hitcnt + 2 * gi + 1, ctr, err, st),
+              "k_prelim_fused");
+    hip_check(hipEventRecord(S.ev[6 * gi + 3], st), "event");
+    // pass 2: the hits whose band outgrew the LDS ring, with per-lane rings of
+    // 2^kRingLog2 cells in global memory; wider still go back as records
+    const uint64_t lanes = (uint64_t)c->cu_count * 4 * 128;
+    int2* ring = c->b_ring.get<int2>((size_t)lanes * ring_stride(kRingLog2));
+    PP.n_dev = hitcnt + 2 * gi + 1;
+    PP.redo_cap = b.redo_cap;
+    hip_check(hipEventRecord(S.ev[6 * gi + 4], st), "event");
+    hip_check(launch_prelim_fused((int)(lanes / 128), false, PP, dh, dredo, g.hits_cap, drecs,
+                                  (const SubjDesc*)c->d_sdesc.p, d_qfwd, d_qrc, (const uint8_t*)c->d_splus.p,
+                                  c->d_mat, ring, kRingLog2, dpre, cnt, nullptr, dredoh, cnt + 2, ctr, err, st),
+              "k_prelim_fused wide");
+    hip_check(hipEventRecord(S.ev[6 * gi + 5], st), "event");
+  }
+  if (!S.done) hip_check(hipEventCreate(&S.done), "hipEventCreate");
+  hip_check(hipEventRecord(S.done, st), "event");
+}
+
+// Wait for a queued batch and read it back: the prelims (item ids naming
+// search records) are appended to `out`.  A group whose hits outgrew their
+// buffer, or wide hits beyond theirs, rerun the batch with larger buffers.
+static void complete_batch(rmb_ctx* c, P1Batch& b, const std::vector<RecDesc>& recs, const uint8_t* d_qfwd,
+                           const uint8_t* d_qrc, const uint8_t* d_qext, std::vector<PreHit>& out,
+                           uint64_t p1_ctr[C_NCOUNTERS]) {
+  P1Slot& S = c->slot[b.slot];
+  const size_t ng = b.groups.size();
+  for (int attempt = 0;; ++attempt) {
+    // Reads go through the Phase-2b stream after the batch's own event: a
+    // copy on the null stream would also wait for the batch queued behind.
+    hip_check(hipEventSynchronize(S.done), "batch sync");
+    std::vector<unsigned long long> hc(2 * ng + 2), cnt(4), ctr(C_NCOUNTERS);
     uint32_t e = 0;
-    hip_check(hipMemcpyAsync(&nh, hitcount, sizeof nh, hipMemcpyDeviceToHost, c->st), "copy hitcount");
-    hip_check(hipMemcpyAsync(&e, err, sizeof e, hipMemcpyDeviceToHost, c->st), "copy err");
-    hip_check(hipStreamSynchronize(c->st), "phase1 sync");
-    {
-      float ms = 0;
-      hip_check(hipEventElapsedTime(&ms, c->ev0, c->ev1), "event time");
-      c->stats[13] += (uint64_t)(ms * 1e6);
-      c->stats[15] += 1;
-    }
+    hip_check(hipMemcpyAsync(hc.data(), S.hitcnt.p, hc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             c->st),
+              "copy hit counts");
+    hip_check(hipMemcpyAsync(cnt.data(), S.cnt.p, cnt.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             c->st),
+              "copy counts");
+    hip_check(hipMemcpyAsync(ctr.data(), S.ctr.p, ctr.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             c->st),
+              "copy counters");
+    hip_check(hipMemcpyAsync(&e, S.err.p, sizeof e, hipMemcpyDeviceToHost, c->st), "copy err");
+    hip_check(hipStreamSynchronize(c->st), "copy sync");
     if (e & E_HASH_FULL) throw Failure(RMB_ERR_CAPACITY, "diagonal hash bucket exceeded 256 cells");
-    if (nh > cap) {  // output larger than the buffer: grow and rerun the group
-      cap = nh + nh / 4;
-      hip_check(hipMemcpyAsync(ctr, ctr_saved, sizeof ctr_saved, hipMemcpyHostToDevice, c->st), "restore counters");
-      hip_check(hipStreamSynchronize(c->st), "restore counters sync");
+    bool rerun = false;
+    for (size_t gi = 0; gi < ng; ++gi) {
+      P1Group& g = b.groups[gi];
+      const uint64_t n_items = g.listed ? g.items.size() : (uint64_t)g.recs_idx.size() * c->subj.size();
+      if (n_items) c->hits_per_item = std::max(c->hits_per_item, (double)hc[2 * gi] / (double)n_items);
+      if (hc[2 * gi] > g.hits_cap) {
+        g.hits_cap = hc[2 * gi] + hc[2 * gi] / 4;
+        rerun = true;
+      }
+    }
+    if (cnt[2] > b.redo_cap) {
+      b.redo_cap = cnt[2] + cnt[2] / 4;
+      rerun = true;
+    }
+    if (rerun) {
+      if (attempt >= 2) throw Failure(RMB_ERR_CAPACITY, "ungapped hit buffer could not be sized");
+      enqueue_batch(c, b, d_qfwd, d_qrc, d_qext);
       continue;
     }
-    if (hits_out) {
-      const size_t base = hits_out->size();
-      hits_out->resize(base + nh);
-      hip_check(hipMemcpy(hits_out->data() + base, dh, nh * sizeof(UngHit), hipMemcpyDeviceToHost), "copy hits");
-      size_t w = base;
-      for (size_t i = base; i < hits_out->size(); ++i) {  // item ids are group-local: map to batch records
-        UngHit h = (*hits_out)[i];
-        if (h.item == kNoItem) continue;  // reserved slot left unused
-        const uint32_t r = h.item / nsub, f = h.item % nsub;
-        h.item = recs_idx[r] * nsub + f;
-        (*hits_out)[w++] = h;
+    for (int k = 0; k < C_NCOUNTERS; ++k) p1_ctr[k] += ctr[k];
+    for (size_t gi = 0; gi < ng; ++gi) {
+      const P1Group& g = b.groups[gi];
+      if (g.listed && g.items.empty()) continue;
+      float ms = 0;
+      hip_check(hipEventElapsedTime(&ms, S.ev[6 * gi], S.ev[6 * gi + 1]), "event time");
+      c->stats[13] += (uint64_t)((double)ms * 1e6);
+      c->stats[15] += 1;
+      for (int p = 0; p < 2; ++p) {
+        hip_check(hipEventElapsedTime(&ms, S.ev[6 * gi + 2 + 2 * p], S.ev[6 * gi + 3 + 2 * p]), "event time");
+        c->stats[14] += (uint64_t)((double)ms * 1e6);
+        c->stats[23] += (uint64_t)((double)ms * 1e6);
       }
-      hits_out->resize(w);
     }
-    if (pre_out && nh) prelim_group(c, recs, recs_idx, drecs, lw, d_qfwd, d_qrc, dh, nh, *pre_out);
+    const size_t base = out.size();
+    out.resize(base + cnt[0]);
+    hip_check(hipMemcpyAsync(out.data() + base, S.pre.p, cnt[0] * sizeof(PreHit), hipMemcpyDeviceToHost, c->st),
+              "copy prelims");
+    std::vector<UngHit> redo(cnt[2]);
+    if (cnt[2])
+      hip_check(hipMemcpyAsync(redo.data(), S.redo_hits.p, cnt[2] * sizeof(UngHit), hipMemcpyDeviceToHost, c->st),
+                "copy redo");
+    hip_check(hipStreamSynchronize(c->st), "copy sync");
+    if (!cnt[2]) return;
+    // bands wider than 2^kRingLog2 cells: score-only DPs with wider rings
+    c->host_prelim_dps += redo.size();
+    const uint32_t nsub = (uint32_t)c->subj.size();
+    const int32_t cutoff = std::max(c->P.min_raw_gapped_score, 1);
+    std::vector<DpTask> tasks;
+    std::vector<std::array<uint32_t, 4>> piv;
+    for (const UngHit& h : redo) {
+      const uint32_t r = h.item / nsub, f = h.item % nsub;
+      const uint32_t ql = recs[r].q_len, sl = c->sdesc[f].s_len;
+      UngRec u{h.q_seed, h.s_seed, h.q_start, h.q_end, h.s_start, h.s_end, h.score, (uint8_t)h.strand};
+      uint32_t pq, ps, tq, ts;
+      pivot_of(u, ql, sl, (uint32_t)lut_width_for(c->P.word_size, 2ull * ql), pq, ps, tq, ts);
+      piv.push_back({pq, ps, tq, ts});
+      const uint8_t* qa = (h.strand ? d_qrc : d_qfwd) + recs[r].q_base + 1;
+      const uint8_t* sa = (const uint8_t*)c->d_splus.p + c->sdesc[f].s_base + 1;
+      const int32_t xd = std::min(c->P.xdrop_gap, h.score);
+      tasks.push_back(DpTask{sa, qa, ps, pq, xd, 1});
+      tasks.push_back(DpTask{sa + ps, qa + pq, sl > ps ? sl - ps : 0, ql > pq ? ql - pq : 0, xd, 0});
+    }
+    std::vector<DpResult> res;
+    run_dp_score(c, tasks, res);
+    for (size_t k = 0; k < redo.size(); ++k) {
+      const UngHit& h = redo[k];
+      const DpResult &L = res[2 * k], &R = res[2 * k + 1];
+      const int32_t sc = L.score + R.score;
+      if (sc < 0 || sc < cutoff) continue;
+      const uint32_t r = h.item / nsub, f = h.item % nsub;
+      const uint32_t ql = recs[r].q_len, sl = c->sdesc[f].s_len;
+      const auto& pv = piv[k];
+      const uint32_t q0 = pv[0] - L.b_len, q1 = pv[0] + R.b_len, s0 = pv[1] - L.a_len, s1 = pv[1] + R.a_len;
+      PreHit p;
+      p.u = h;
+      p.score = sc;
+      if (h.strand) { p.q_start = ql - q1; p.q_end = ql - q0; p.s_start = sl - s1; p.s_end = sl - s0; }
+      else { p.q_start = q0; p.q_end = q1; p.s_start = s0; p.s_end = s1; }
+      p.tq = pv[2];
+      p.ts = pv[3];
+      out.push_back(p);
+    }
     return;
   }
-  throw Failure(RMB_ERR_CAPACITY, "ungapped hit buffer could not be sized");
+}
+
+// Lookup-width groups of a batch's search records [k0, k1), bounded by the
+// cell table size and the entry count; with `pairs`, only the listed items.
+static std::vector<P1Group> group_records(const rmb_ctx* c, const std::vector<RecDesc>& recs,
+                                          const std::vector<uint32_t>& first, uint32_t k0, uint32_t k1,
+                                          const std::vector<uint64_t>* pairs) {
+  const int ws = c->P.word_size;
+  const uint32_t nsub = (uint32_t)c->subj.size();
+  std::vector<P1Group> out;
+  auto flush = [&](std::vector<uint32_t>& grp, int lw) {
+    if (grp.empty()) return;
+    P1Group g;
+    g.lw = lw;
+    g.recs_idx = grp;
+    const uint32_t R = (uint32_t)grp.size();
+    const uint64_t cells = 1ull << (2 * lw);
+    g.recs.resize(R);
+    g.elem_base.assign(R + 1, 0);
+    for (uint32_t i = 0; i < R; ++i) {
+      g.recs[i] = recs[grp[i]];
+      g.recs[i].cell_base = (uint64_t)i * cells;
+      g.elem_base[i + 1] = g.elem_base[i] + 2ull * g.recs[i].q_len + 1;
+    }
+    if (pairs) {  // pairs name queries: search every record of the query
+      g.listed = true;
+      std::unordered_map<uint32_t, uint32_t> pos;
+      for (uint32_t k = 0; k < R; ++k) pos[grp[k]] = k;
+      for (uint64_t it : *pairs) {
+        const uint32_t q = (uint32_t)(it / nsub), f = (uint32_t)(it % nsub);
+        for (uint32_t k = first[q]; k < first[q + 1]; ++k) {
+          auto p = pos.find(k);
+          if (p != pos.end()) g.items.push_back(p->second * nsub + f);
+        }
+      }
+      std::sort(g.items.begin(), g.items.end());
+    }
+    out.push_back(std::move(g));
+    grp.clear();
+  };
+  for (int lw = 4; lw <= 12; ++lw) {
+    std::vector<uint32_t> grp;
+    const uint64_t cells = 1ull << (2 * lw);
+    uint64_t elems = 0;
+    for (uint32_t i = k0; i < k1; ++i) {
+      if (lut_width_for(ws, 2ull * recs[i].q_len) != lw) continue;
+      if (!grp.empty() && ((grp.size() + 1) * cells > (1ull << 28) ||
+                           elems + 2ull * recs[i].q_len + 1 > (1ull << 31))) {
+        flush(grp, lw);
+        elems = 0;
+      }
+      grp.push_back(i);
+      elems += 2ull * recs[i].q_len + 1;
+    }
+    flush(grp, lw);
+  }
+  return out;
 }
 
 // The records a query is searched as through Phase 2a: the whole query, or
@@ -739,6 +865,254 @@ static const uint8_t* upload_dusted(rmb_ctx* c, const std::vector<std::vector<ui
   uint8_t* d = c->b_qext.get<uint8_t>(sr.pool);
   hip_check(hipMemcpy(d, pm.data(), sr.pool, hipMemcpyHostToDevice), "copy masked query");
   return d;
+}
+
+// Phase 2a of one batch: ordered replay of the hits whose preliminary DP
+// reached the cutoff, in NCBI's ungapped-hit order per item
+// (engine.rs:1280-1411), then split queries' chunks folded together.
+static std::vector<ItemState> phase2a(rmb_ctx* c, const SearchRecs& sr, std::vector<PreHit>& survivors) {
+  const uint32_t nsub = (uint32_t)c->subj.size();
+  const std::vector<RecDesc>& recs = sr.recs;
+  const uint32_t nsr = (uint32_t)recs.size(), nq = (uint32_t)sr.whole_base.size();
+  std::vector<ItemState> items;
+  {
+    std::vector<uint64_t> keyed(survivors.size());
+    for (size_t i = 0; i < survivors.size(); ++i) keyed[i] = ((uint64_t)survivors[i].u.item << 32) | i;
+    std::sort(keyed.begin(), keyed.end());
+    for (size_t i = 0; i < keyed.size();) {
+      const uint32_t it = (uint32_t)(keyed[i] >> 32);
+      ItemState s;
+      s.item = it;
+      size_t j = i;
+      for (; j < keyed.size() && (uint32_t)(keyed[j] >> 32) == it; ++j) s.surv.push_back((uint32_t)keyed[j]);
+      items.push_back(std::move(s));
+      i = j;
+    }
+  }
+  parallel_for(items.size(), [&](size_t k) {
+    ItemState& s = items[k];
+    const uint32_t ql = recs[s.item / nsub].q_len, sl = (uint32_t)c->subj[s.item % nsub].plus.size() - 2;
+    std::vector<UngRec> u(s.surv.size());
+    for (size_t i = 0; i < s.surv.size(); ++i) {
+      const UngHit& h = survivors[s.surv[i]].u;
+      u[i] = UngRec{h.q_seed, h.s_seed, h.q_start, h.q_end, h.s_start, h.s_end, h.score, (uint8_t)h.strand};
+    }
+    std::vector<uint32_t> ord(u.size());
+    for (size_t i = 0; i < ord.size(); ++i) ord[i] = (uint32_t)i;
+    std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return ung_before(u[a], u[b], ql, sl); });
+    HspTree tp(ql + 1, sl + 1), tm(ql + 1, sl + 1);
+    for (uint32_t i : ord) {
+      const UngRec& x = u[i];
+      const uint32_t css = x.minus ? sl - x.s_end : x.s_start, cse = x.minus ? sl - x.s_start : x.s_end;
+      HspTree& t = x.minus ? tm : tp;
+      if (t.covers(x.q_start, x.q_end, css, cse, x.score, !x.minus)) continue;
+      const PreHit& p = survivors[s.surv[i]];
+      const uint32_t nss = x.minus ? sl - p.s_end : p.s_start, nse = x.minus ? sl - p.s_start : p.s_end;
+      t.add(p.q_start, p.q_end, nss, nse, p.score, !x.minus);
+      s.pre.push_back(PreRec{p.tq, p.ts, p.q_start, p.q_end, p.s_start, p.s_end, p.score, x.minus});
+    }
+    purge_prelims(s.pre, sl);
+  });
+  survivors.clear();
+  survivors.shrink_to_fit();
+  // ---- items from search records to queries: a split query's prelims move
+  // to query coordinates and fold chunk by chunk into those of the chunks
+  // before (main.rs:1270-1300; items are in search-record order, so the
+  // chunks of a query arrive in order)
+  if (nsr != nq) {
+    std::vector<ItemState> merged;
+    merged.reserve(items.size());
+    for (size_t k = 0; k < items.size();) {
+      const uint32_t q = sr.query[items[k].item / nsub];
+      if (sr.first[q + 1] - sr.first[q] == 1) {
+        items[k].item = q * nsub + items[k].item % nsub;
+        merged.push_back(std::move(items[k++]));
+        continue;
+      }
+      std::map<uint32_t, ItemState> acc;  // by subject
+      for (; k < items.size() && sr.query[items[k].item / nsub] == q; ++k) {
+        ItemState& s = items[k];
+        const uint32_t chunk = s.item / nsub, f = s.item % nsub, off = sr.offset[chunk];
+        for (PreRec& h : s.pre) {
+          h.q_seed += off;
+          h.q_start += off;
+          h.q_end += off;
+        }
+        ItemState& a = acc[f];
+        a.item = q * nsub + f;
+        if (chunk == sr.first[q]) a.pre = std::move(s.pre);
+        else merge_chunk_prelims(a.pre, std::move(s.pre), off);
+      }
+      for (auto& kv : acc) merged.push_back(std::move(kv.second));
+    }
+    items = std::move(merged);
+  }
+  return items;
+}
+
+// Phase 2b of one batch: traceback DPs on the GPU, then the per-subject
+// replay on the host (engine.rs:1438-1762).
+static void phase2b(rmb_ctx* c, const SearchRecs& sr, const std::vector<std::vector<uint8_t>>& qf,
+                    const uint32_t* lens, std::vector<ItemState>& items, const uint8_t* d_qfwd, const uint8_t* d_qrc,
+                    uint64_t tb_ms[4]) {
+  const uint32_t nsub = (uint32_t)c->subj.size();
+  const int32_t cutoff = std::max(c->P.min_raw_gapped_score, 1);
+  Timer t3;
+  uint64_t tb_marks[4] = {0, 0, 0, 0};  // sort, seeds + tasks, DP on the GPU, replay (us since t3)
+  {
+    struct TbRef { uint32_t item; uint32_t pre; uint32_t tq, ts; };
+    std::vector<size_t> first(items.size() + 1, 0);
+    parallel_for(items.size(), [&](size_t k) {
+      ItemState& s = items[k];
+      const uint32_t ql = lens[s.item / nsub];
+      const uint32_t sl = (uint32_t)c->subj[s.item % nsub].align.size() - 2;
+      std::stable_sort(s.pre.begin(), s.pre.end(), [&](const PreRec& a, const PreRec& b) {
+        const uint32_t fsa = a.minus ? sl - a.s_end : a.s_start, fsb = b.minus ? sl - b.s_end : b.s_start;
+        const uint32_t fea = a.minus ? sl - a.s_start : a.s_end, feb = b.minus ? sl - b.s_start : b.s_end;
+        const uint32_t qoa = a.minus ? ql - a.q_end : a.q_start, qob = b.minus ? ql - b.q_end : b.q_start;
+        const uint32_t qea = a.minus ? ql - a.q_start : a.q_end, qeb = b.minus ? ql - b.q_start : b.q_end;
+        if (a.score != b.score) return a.score > b.score;
+        if (fsa != fsb) return fsa < fsb;
+        if (fea != feb) return fea > feb;
+        if (qoa != qob) return qoa < qob;
+        return qea > qeb;
+      });
+    });
+    tb_marks[0] = t3.us();
+    for (size_t k = 0; k < items.size(); ++k) first[k + 1] = first[k] + items[k].pre.size();
+    std::vector<TbRef> refs(first.back());
+    std::vector<DpTask> tasks(2 * first.back());
+    std::vector<uint64_t> work(tasks.size());  // rows each half is expected to cover: the prelim's extent
+    const uint8_t* d_salign = (const uint8_t*)c->d_salign.p;
+    parallel_for(items.size(), [&](size_t k) {
+      ItemState& s = items[k];
+      const uint32_t r = s.item / nsub, f = s.item % nsub;
+      const uint32_t ql = lens[r];
+      const Subject& S = c->subj[f];
+      const uint32_t sl = (uint32_t)S.align.size() - 2;
+      for (size_t p = 0; p < s.pre.size(); ++p) {
+        const PreRec& h = s.pre[p];
+        const std::vector<uint8_t>& sv = h.minus ? S.align_rc : S.align;
+        uint32_t nq, ns;
+        improve_seed(qf[r].data() + 1, ql, sv.data() + 1, sl, h.q_seed, h.s_seed, h.q_start, h.q_end, h.s_start,
+                     h.s_end, h.minus, nq, ns);
+        const uint32_t tq = h.minus ? ql - 1 - nq : nq, ts = h.minus ? sl - 1 - ns : ns;
+        const size_t g = first[k] + p;
+        refs[g] = TbRef{(uint32_t)k, (uint32_t)p, tq, ts};
+        const uint8_t* qa = (h.minus ? d_qrc : d_qfwd) + sr.whole_base[r] + 1;
+        const uint8_t* sa = d_salign + c->sdesc[f].s_base + 1;
+        tasks[2 * g] = DpTask{qa, sa, tq + 1, ts + 1, c->P.xdrop_gap_final, 1};
+        const uint32_t rm = ql > tq + 1 ? ql - tq - 1 : 0, rn = sl > ts + 1 ? sl - ts - 1 : 0;
+        tasks[2 * g + 1] = DpTask{qa + tq + 1, sa + ts + 1, rn ? rm : 0, rm ? rn : 0, c->P.xdrop_gap_final, 0};
+        const uint32_t fqs = h.minus ? ql - h.q_end : h.q_start, fqe = h.minus ? ql - h.q_start : h.q_end;
+        work[2 * g] = tq + 1 > fqs ? tq + 1 - fqs : 1;
+        work[2 * g + 1] = fqe > tq + 1 ? fqe - tq - 1 : 1;
+      }
+    });
+    tb_marks[1] = t3.us();
+    std::vector<HalfTb> halves;
+    std::vector<uint32_t> op_pool;
+    run_dp_traceback(c, tasks, halves, op_pool, &work);
+    tb_marks[2] = t3.us();
+    c->stats[3] += refs.size();
+    const int32_t go = c->P.gap_open, ge = c->P.gap_extend;
+    parallel_for(items.size(), [&](size_t k) {
+      ItemState& s = items[k];
+      const uint32_t r = s.item / nsub, f = s.item % nsub;
+      const uint32_t ql = lens[r];
+      const Subject& S = c->subj[f];
+      const uint32_t sl = (uint32_t)S.align.size() - 2;
+      HspTree tbp(ql + 1, sl + 1), tbm(ql + 1, sl + 1, true);
+      for (size_t p = 0; p < s.pre.size(); ++p) {
+        const PreRec& h = s.pre[p];
+        const bool plus = !h.minus;
+        const uint32_t pss = plus ? h.s_start : sl - h.s_end, pse = plus ? h.s_end : sl - h.s_start;
+        const uint32_t cqs = plus ? h.q_start : ql - h.q_end, cqe = plus ? h.q_end : ql - h.q_start;
+        if ((plus ? tbp : tbm).covers(cqs, cqe, pss, pse, h.score, plus)) continue;
+        const size_t g = first[k] + p;
+        const TbRef& ref = refs[g];
+        const HalfTb& L = halves[2 * g];
+        const HalfTb& R = halves[2 * g + 1];
+        int32_t sc = L.score + R.score;
+        uint32_t q0 = ref.tq + 1 - L.a_len, s0 = ref.ts + 1 - L.b_len;
+        uint32_t q1 = ref.tq + 1 + R.a_len, s1 = ref.ts + 1 + R.b_len;
+        Script ops;
+        for (uint32_t j = L.n_ops; j-- > 0;) {
+          const uint32_t o = op_pool[L.ops_at + j];
+          script_push(ops, (uint8_t)(o & 3), o >> 2);
+        }
+        for (uint32_t j = 0; j < R.n_ops; ++j) {
+          const uint32_t o = op_pool[R.ops_at + j];
+          script_push(ops, (uint8_t)(o & 3), o >> 2);
+        }
+        while (!ops.empty() && ops.front().kind != K_SUB) {  // engine.rs:3016-3033
+          sc += go + (int32_t)ops.front().len * ge;
+          if (ops.front().kind == K_GAP_Q) s0 += ops.front().len; else q0 += ops.front().len;
+          ops.erase(ops.begin());
+        }
+        while (!ops.empty() && ops.back().kind != K_SUB) {
+          sc += go + (int32_t)ops.back().len * ge;
+          if (ops.back().kind == K_GAP_Q) s1 -= ops.back().len; else q1 -= ops.back().len;
+          ops.pop_back();
+        }
+        if (sc < 0) continue;
+        FinalRec fr;
+        fr.minus = h.minus;
+        if (h.minus) {
+          fr.q_start = ql - q1;
+          fr.q_end = ql - q0;
+          fr.s_start = s0;  // forward subject
+          fr.s_end = s1;
+          std::reverse(ops.begin(), ops.end());
+        } else {
+          fr.q_start = q0;
+          fr.q_end = q1;
+          fr.s_start = s0;
+          fr.s_end = s1;
+        }
+        if (c->P.complexity_adjust) {
+          int32_t adj;
+          if (!complexity_adjust(sc, qf[r].data(), qf[r].size(), fr.q_start, ops, c->M, cutoff, h.minus, adj))
+            continue;
+          sc = adj;
+        }
+        fr.score = sc;
+        fr.ops = std::move(ops);
+        const uint32_t tqs = plus ? fr.q_start : ql - fr.q_end, tqe = plus ? fr.q_end : ql - fr.q_start;
+        (plus ? tbp : tbm).add(tqs, tqe, fr.s_start, fr.s_end, sc, true);
+        s.fin.push_back(std::move(fr));
+      }
+      finish_subject(s.fin, qf[r].data(), ql, S.align.data(), S.align_rc.data(), sl, S.n_mask, S.n_mask_rc, c->M,
+                     go, ge, cutoff);
+    });
+  }
+  tb_marks[3] = t3.us();
+  tb_ms[0] += tb_marks[0];
+  tb_ms[1] += tb_marks[1] - tb_marks[0];
+  tb_ms[2] += tb_marks[2] - tb_marks[1];
+  tb_ms[3] += tb_marks[3] - tb_marks[2];
+}
+
+// Final HSPs of a batch's items, in item order, into the context's results.
+static void emit_results(rmb_ctx* c, const std::vector<ItemState>& items) {
+  const uint32_t nsub = (uint32_t)c->subj.size();
+  for (const auto& s : items) {
+    for (const auto& h : s.fin) {
+      rmb_hsp o{};
+      o.query = s.item / nsub;
+      o.subject = s.item % nsub;
+      o.score = h.score;
+      o.q_start = h.q_start;
+      o.q_end = h.q_end;
+      o.s_start = h.s_start;
+      o.s_end = h.s_end;
+      o.strand = h.minus;
+      o.ops_offset = (uint32_t)c->ops.size();
+      o.ops_count = (uint32_t)h.ops.size();
+      for (const Run& x : h.ops) c->ops.push_back((uint32_t)x.kind | (x.len << 2));
+      c->hsps.push_back(o);
+    }
+  }
 }
 
 // pairs: null for every (query, subject) pair, else the item ids q * nsub + s to search
@@ -816,289 +1190,74 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
   if (c->P.dust) d_qext = upload_dusted(c, qf, sr);
   unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
   hip_check(hipMemset(ctr, 0, C_NCOUNTERS * sizeof(unsigned long long)), "memset counters");
-  // ---- Phase 1 by lookup width; groups are bounded by the cell table size
-  Timer t1;
-  std::vector<PreHit> survivors;
-  std::vector<int> lw_of(nsr);
-  for (uint32_t i = 0; i < nsr; ++i) lw_of[i] = lut_width_for(ws, 2ull * recs[i].q_len);
-  for (int lw = 4; lw <= 12; ++lw) {
-    std::vector<uint32_t> grp;
-    const uint64_t cells = 1ull << (2 * lw);
-    uint64_t elems = 0;
-    for (uint32_t i = 0; i <= nsr; ++i) {
-      const bool flush = i == nsr || (!grp.empty() && ((grp.size() + 1) * cells > (1ull << 28) ||
-                                                       elems + 2ull * recs[i].q_len + 1 > (1ull << 31)));
-      if (flush && !grp.empty()) {
-        if (pairs) {  // pairs name queries: search every record of the query
-          std::vector<uint32_t> pos(nsr, UINT32_MAX), local;
-          for (uint32_t k = 0; k < grp.size(); ++k) pos[grp[k]] = k;
-          for (uint64_t it : *pairs) {
-            const uint32_t q = (uint32_t)(it / nsub), f = (uint32_t)(it % nsub);
-            for (uint32_t k = sr.first[q]; k < sr.first[q + 1]; ++k)
-              if (pos[k] != UINT32_MAX) local.push_back(pos[k] * nsub + f);
-          }
-          std::sort(local.begin(), local.end());
-          phase1_group(c, grp, recs, lw, d_qfwd, d_qrc, d_qext, nullptr, &survivors, &local);
-        } else {
-          phase1_group(c, grp, recs, lw, d_qfwd, d_qrc, d_qext, nullptr, &survivors);
-        }
-        grp.clear();
-        elems = 0;
-      }
-      if (i < nsr && lw_of[i] == lw) {
-        grp.push_back(i);
-        elems += 2ull * recs[i].q_len + 1;
-      }
-    }
-  }
-  c->stats[9] = t1.us();
-  // ---- Phase 2a: ordered replay of the hits whose preliminary DP reached the
-  // cutoff, in NCBI's ungapped-hit order per item (engine.rs:1280-1411)
-  Timer t2;
-  const int32_t cutoff = std::max(c->P.min_raw_gapped_score, 1);
-  std::vector<ItemState> items;
+  // ---- batches of whole queries with about equal bases, pipelined: the GPU
+  // seeds batch b + 1 while the host replays and aligns batch b
+  uint64_t total_bases = 0;
+  for (uint32_t i = 0; i < nsr; ++i) total_bases += recs[i].q_len;
+  int max_batches = kMaxBatches;
+  if (const char* e = std::getenv("RMB_BATCHES")) max_batches = std::max(1, std::atoi(e));
+  const uint64_t per_batch = std::max<uint64_t>(kBatchMinBases, (total_bases + max_batches - 1) / max_batches);
+  std::vector<P1Batch> batches;
   {
-    std::vector<uint64_t> keyed(survivors.size());
-    for (size_t i = 0; i < survivors.size(); ++i) keyed[i] = ((uint64_t)survivors[i].u.item << 32) | i;
-    std::sort(keyed.begin(), keyed.end());
-    for (size_t i = 0; i < keyed.size();) {
-      const uint32_t it = (uint32_t)(keyed[i] >> 32);
-      ItemState s;
-      s.item = it;
-      size_t j = i;
-      for (; j < keyed.size() && (uint32_t)(keyed[j] >> 32) == it; ++j) s.surv.push_back((uint32_t)keyed[j]);
-      items.push_back(std::move(s));
-      i = j;
+    uint32_t k0 = 0;
+    uint64_t acc = 0;
+    for (uint32_t q = 0; q < nq; ++q) {
+      for (uint32_t k = sr.first[q]; k < sr.first[q + 1]; ++k) acc += recs[k].q_len;
+      if (acc >= per_batch || q + 1 == nq) {
+        const uint32_t k1 = sr.first[q + 1];
+        if (k1 > k0) {
+          P1Batch b;
+          b.slot = (int)(batches.size() % 2);
+          b.groups = group_records(c, recs, sr.first, k0, k1, pairs);
+          batches.push_back(std::move(b));
+        }
+        k0 = k1;
+        acc = 0;
+      }
     }
   }
-  parallel_for(items.size(), [&](size_t k) {
-    ItemState& s = items[k];
-    const uint32_t ql = recs[s.item / nsub].q_len, sl = (uint32_t)c->subj[s.item % nsub].plus.size() - 2;
-    std::vector<UngRec> u(s.surv.size());
-    for (size_t i = 0; i < s.surv.size(); ++i) {
-      const UngHit& h = survivors[s.surv[i]].u;
-      u[i] = UngRec{h.q_seed, h.s_seed, h.q_start, h.q_end, h.s_start, h.s_end, h.score, (uint8_t)h.strand};
-    }
-    std::vector<uint32_t> ord(u.size());
-    for (size_t i = 0; i < ord.size(); ++i) ord[i] = (uint32_t)i;
-    std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return ung_before(u[a], u[b], ql, sl); });
-    HspTree tp(ql + 1, sl + 1), tm(ql + 1, sl + 1);
-    for (uint32_t i : ord) {
-      const UngRec& x = u[i];
-      const uint32_t css = x.minus ? sl - x.s_end : x.s_start, cse = x.minus ? sl - x.s_start : x.s_end;
-      HspTree& t = x.minus ? tm : tp;
-      if (t.covers(x.q_start, x.q_end, css, cse, x.score, !x.minus)) continue;
-      const PreHit& p = survivors[s.surv[i]];
-      const uint32_t nss = x.minus ? sl - p.s_end : p.s_start, nse = x.minus ? sl - p.s_start : p.s_end;
-      t.add(p.q_start, p.q_end, nss, nse, p.score, !x.minus);
-      s.pre.push_back(PreRec{p.tq, p.ts, p.q_start, p.q_end, p.s_start, p.s_end, p.score, x.minus});
-    }
-    purge_prelims(s.pre, sl);
-  });
-  survivors.clear();
-  survivors.shrink_to_fit();
-  // ---- items from search records to queries: a split query's prelims move
-  // to query coordinates and fold chunk by chunk into those of the chunks
-  // before (main.rs:1270-1300; items are in search-record order, so the
-  // chunks of a query arrive in order)
-  if (nsr != nq) {
-    std::vector<ItemState> merged;
-    merged.reserve(items.size());
-    for (size_t k = 0; k < items.size();) {
-      const uint32_t q = sr.query[items[k].item / nsub];
-      if (sr.first[q + 1] - sr.first[q] == 1) {
-        items[k].item = q * nsub + items[k].item % nsub;
-        merged.push_back(std::move(items[k++]));
-        continue;
-      }
-      std::map<uint32_t, ItemState> acc;  // by subject
-      for (; k < items.size() && sr.query[items[k].item / nsub] == q; ++k) {
-        ItemState& s = items[k];
-        const uint32_t chunk = s.item / nsub, f = s.item % nsub, off = sr.offset[chunk];
-        for (PreRec& h : s.pre) {
-          h.q_seed += off;
-          h.q_start += off;
-          h.q_end += off;
-        }
-        ItemState& a = acc[f];
-        a.item = q * nsub + f;
-        if (chunk == sr.first[q]) a.pre = std::move(s.pre);
-        else merge_chunk_prelims(a.pre, std::move(s.pre), off);
-      }
-      for (auto& kv : acc) merged.push_back(std::move(kv.second));
-    }
-    items = std::move(merged);
+  uint64_t p1_ctr[C_NCOUNTERS] = {0};
+  uint64_t t_seed = 0, t_2a = 0, t_2b = 0, tb_ms[4] = {0, 0, 0, 0};
+  if (!batches.empty()) {
+    Timer te;
+    enqueue_batch(c, batches[0], d_qfwd, d_qrc, d_qext);
+    t_seed += te.us();
   }
-  c->stats[10] = t2.us();
-  // ---- Phase 2b: traceback
-  Timer t3;
-  uint64_t tb_marks[4] = {0, 0, 0, 0};  // sort, seeds + tasks, DP on the GPU, replay (us since t3)
-  {
-    struct TbRef { uint32_t item; uint32_t pre; uint32_t tq, ts; };
-    std::vector<size_t> first(items.size() + 1, 0);
-    parallel_for(items.size(), [&](size_t k) {
-      ItemState& s = items[k];
-      const uint32_t ql = lens[s.item / nsub];
-      const uint32_t sl = (uint32_t)c->subj[s.item % nsub].align.size() - 2;
-      std::stable_sort(s.pre.begin(), s.pre.end(), [&](const PreRec& a, const PreRec& b) {
-        const uint32_t fsa = a.minus ? sl - a.s_end : a.s_start, fsb = b.minus ? sl - b.s_end : b.s_start;
-        const uint32_t fea = a.minus ? sl - a.s_start : a.s_end, feb = b.minus ? sl - b.s_start : b.s_end;
-        const uint32_t qoa = a.minus ? ql - a.q_end : a.q_start, qob = b.minus ? ql - b.q_end : b.q_start;
-        const uint32_t qea = a.minus ? ql - a.q_start : a.q_end, qeb = b.minus ? ql - b.q_start : b.q_end;
-        if (a.score != b.score) return a.score > b.score;
-        if (fsa != fsb) return fsa < fsb;
-        if (fea != feb) return fea > feb;
-        if (qoa != qob) return qoa < qob;
-        return qea > qeb;
-      });
-    });
-    tb_marks[0] = t3.us();
-    for (size_t k = 0; k < items.size(); ++k) first[k + 1] = first[k] + items[k].pre.size();
-    std::vector<TbRef> refs(first.back());
-    std::vector<DpTask> tasks(2 * first.back());
-    std::vector<uint64_t> work(tasks.size());  // rows each half is expected to cover: the prelim's extent
-    const uint8_t* d_salign = (const uint8_t*)c->d_salign.p;
-    parallel_for(items.size(), [&](size_t k) {
-      ItemState& s = items[k];
-      const uint32_t r = s.item / nsub, f = s.item % nsub;
-      const uint32_t ql = lens[r];
-      const Subject& S = c->subj[f];
-      const uint32_t sl = (uint32_t)S.align.size() - 2;
-      for (size_t p = 0; p < s.pre.size(); ++p) {
-        const PreRec& h = s.pre[p];
-        const std::vector<uint8_t>& sv = h.minus ? S.align_rc : S.align;
-        uint32_t nq, ns;
-        improve_seed(qf[r].data() + 1, ql, sv.data() + 1, sl, h.q_seed, h.s_seed, h.q_start, h.q_end, h.s_start,
-                     h.s_end, h.minus, nq, ns);
-        const uint32_t tq = h.minus ? ql - 1 - nq : nq, ts = h.minus ? sl - 1 - ns : ns;
-        const size_t g = first[k] + p;
-        refs[g] = TbRef{(uint32_t)k, (uint32_t)p, tq, ts};
-        const uint8_t* qa = (h.minus ? d_qrc : d_qfwd) + sr.whole_base[r] + 1;
-        const uint8_t* sa = d_salign + c->sdesc[f].s_base + 1;
-        tasks[2 * g] = DpTask{qa, sa, tq + 1, ts + 1, c->P.xdrop_gap_final, 1};
-        const uint32_t rm = ql > tq + 1 ? ql - tq - 1 : 0, rn = sl > ts + 1 ? sl - ts - 1 : 0;
-        tasks[2 * g + 1] = DpTask{qa + tq + 1, sa + ts + 1, rn ? rm : 0, rm ? rn : 0, c->P.xdrop_gap_final, 0};
-        const uint32_t fqs = h.minus ? ql - h.q_end : h.q_start, fqe = h.minus ? ql - h.q_start : h.q_end;
-        work[2 * g] = tq + 1 > fqs ? tq + 1 - fqs : 1;
-        work[2 * g + 1] = fqe > tq + 1 ? fqe - tq - 1 : 1;
-      }
-    });
-    tb_marks[1] = t3.us();
-    std::vector<HalfTb> halves;
-    std::vector<uint32_t> op_pool;
-    run_dp_traceback(c, tasks, halves, op_pool, &work);
-    tb_marks[2] = t3.us();
-    c->stats[3] = refs.size();
-    const int32_t go = c->P.gap_open, ge = c->P.gap_extend;
-    parallel_for(items.size(), [&](size_t k) {
-      ItemState& s = items[k];
-      const uint32_t r = s.item / nsub, f = s.item % nsub;
-      const uint32_t ql = lens[r];
-      const Subject& S = c->subj[f];
-      const uint32_t sl = (uint32_t)S.align.size() - 2;
-      HspTree tbp(ql + 1, sl + 1), tbm(ql + 1, sl + 1, true);
-      for (size_t p = 0; p < s.pre.size(); ++p) {
-        const PreRec& h = s.pre[p];
-        const bool plus = !h.minus;
-        const uint32_t pss = plus ? h.s_start : sl - h.s_end, pse = plus ? h.s_end : sl - h.s_start;
-        const uint32_t cqs = plus ? h.q_start : ql - h.q_end, cqe = plus ? h.q_end : ql - h.q_start;
-        if ((plus ? tbp : tbm).covers(cqs, cqe, pss, pse, h.score, plus)) continue;
-        const size_t g = first[k] + p;
-        const TbRef& ref = refs[g];
-        const HalfTb& L = halves[2 * g];
-        const HalfTb& R = halves[2 * g + 1];
-        int32_t sc = L.score + R.score;
-        uint32_t q0 = ref.tq + 1 - L.a_len, s0 = ref.ts + 1 - L.b_len;
-        uint32_t q1 = ref.tq + 1 + R.a_len, s1 = ref.ts + 1 + R.b_len;
-        Script ops;
-        for (uint32_t j = L.n_ops; j-- > 0;) {
-          const uint32_t o = op_pool[L.ops_at + j];
-          script_push(ops, (uint8_t)(o & 3), o >> 2);
-        }
-        for (uint32_t j = 0; j < R.n_ops; ++j) {
-          const uint32_t o = op_pool[R.ops_at + j];
-          script_push(ops, (uint8_t)(o & 3), o >> 2);
-        }
-        while (!ops.empty() && ops.front().kind != K_SUB) {  // engine.rs:3016-3033
-          sc += go + (int32_t)ops.front().len * ge;
-          if (ops.front().kind == K_GAP_Q) s0 += ops.front().len; else q0 += ops.front().len;
-          ops.erase(ops.begin());
-        }
-        while (!ops.empty() && ops.back().kind != K_SUB) {
-          sc += go + (int32_t)ops.back().len * ge;
-          if (ops.back().kind == K_GAP_Q) s1 -= ops.back().len; else q1 -= ops.back().len;
-          ops.pop_back();
-        }
-        if (sc < 0) continue;
-        FinalRec fr;
-        fr.minus = h.minus;
-        if (h.minus) {
-          fr.q_start = ql - q1;
-          fr.q_end = ql - q0;
-          fr.s_start = s0;  // forward subject
-          fr.s_end = s1;
-          std::reverse(ops.begin(), ops.end());
-        } else {
-          fr.q_start = q0;
-          fr.q_end = q1;
-          fr.s_start = s0;
-          fr.s_end = s1;
-        }
-        if (c->P.complexity_adjust) {
-          int32_t adj;
-          if (!complexity_adjust(sc, qf[r].data(), qf[r].size(), fr.q_start, ops, c->M, cutoff, h.minus, adj))
-            continue;
-          sc = adj;
-        }
-        fr.score = sc;
-        fr.ops = std::move(ops);
-        const uint32_t tqs = plus ? fr.q_start : ql - fr.q_end, tqe = plus ? fr.q_end : ql - fr.q_start;
-        (plus ? tbp : tbm).add(tqs, tqe, fr.s_start, fr.s_end, sc, true);
-        s.fin.push_back(std::move(fr));
-      }
-      finish_subject(s.fin, qf[r].data(), ql, S.align.data(), S.align_rc.data(), sl, S.n_mask, S.n_mask_rc, c->M,
-                     go, ge, cutoff);
-    });
+  for (size_t b = 0; b < batches.size(); ++b) {
+    Timer t1;
+    if (b + 1 < batches.size()) enqueue_batch(c, batches[b + 1], d_qfwd, d_qrc, d_qext);
+    std::vector<PreHit> survivors;
+    complete_batch(c, batches[b], recs, d_qfwd, d_qrc, d_qext, survivors, p1_ctr);
+    t_seed += t1.us();
+    Timer t2;
+    std::vector<ItemState> items = phase2a(c, sr, survivors);
+    t_2a += t2.us();
+    Timer t3;
+    phase2b(c, sr, qf, lens, items, d_qfwd, d_qrc, tb_ms);
+    emit_results(c, items);
+    t_2b += t3.us();
   }
-  c->stats[11] = t3.us();
-  tb_marks[3] = c->stats[11];
+  c->stats[9] = t_seed;
+  c->stats[10] = t_2a;
+  c->stats[11] = t_2b;
   if (std::getenv("RMB_STAGE_TIMES"))
     std::fprintf(stderr,
-                 "[rmb] phase2b ms: sort %.1f  seeds+tasks %.1f  dp %.1f  replay %.1f  (prelims %zu, hits in 2a %zu)\n",
-                 tb_marks[0] / 1e3, (tb_marks[1] - tb_marks[0]) / 1e3, (tb_marks[2] - tb_marks[1]) / 1e3,
-                 (tb_marks[3] - tb_marks[2]) / 1e3, (size_t)c->stats[3] / 2 * 2, (size_t)c->stats[1]);
-  // ---- results, item order
-  for (auto& s : items) {
-    for (auto& h : s.fin) {
-      rmb_hsp o{};
-      o.query = s.item / nsub;
-      o.subject = s.item % nsub;
-      o.score = h.score;
-      o.q_start = h.q_start;
-      o.q_end = h.q_end;
-      o.s_start = h.s_start;
-      o.s_end = h.s_end;
-      o.strand = h.minus;
-      o.ops_offset = (uint32_t)c->ops.size();
-      o.ops_count = (uint32_t)h.ops.size();
-      for (const Run& x : h.ops) c->ops.push_back((uint32_t)x.kind | (x.len << 2));
-      c->hsps.push_back(o);
-    }
-  }
+                 "[rmb] %zu batches; phase2b ms: sort %.1f  seeds+tasks %.1f  dp %.1f  replay %.1f  (prelims %zu)\n",
+                 batches.size(), tb_ms[0] / 1e3, tb_ms[1] / 1e3, tb_ms[2] / 1e3, tb_ms[3] / 1e3,
+                 (size_t)c->stats[3]);
   unsigned long long hc[C_NCOUNTERS];
   hip_check(hipMemcpy(hc, ctr, sizeof hc, hipMemcpyDeviceToHost), "copy counters");
-  c->stats[0] = hc[C_SEED_EVALS];
-  c->stats[1] = hc[C_UNG_HITS];
-  c->stats[7] = hc[C_UNG_STEPS];
-  c->stats[2] = hc[C_PRELIM_DPS] + c->host_prelim_dps;
-  c->stats[4] = hc[C_PRELIM_CELLS] + hc[C_DP_CELLS];  // DP cells of both phases (score passes)
+  c->stats[0] = p1_ctr[C_SEED_EVALS];
+  c->stats[1] = p1_ctr[C_UNG_HITS];
+  c->stats[7] = p1_ctr[C_UNG_STEPS];
+  c->stats[2] = p1_ctr[C_PRELIM_DPS] + c->host_prelim_dps;
+  c->stats[4] = p1_ctr[C_PRELIM_CELLS] + hc[C_DP_CELLS];  // DP cells of both phases (score passes)
   c->stats[5] = hc[C_DP_CELLS];
   c->stats[6] = c->hsps.size();
   c->stats[12] = total.us();
-  c->stats[16] = hc[C_SEEDS];
-  for (int k = 0; k < 6; ++k) c->stats[17 + k] = hc[C_P1_STAGE + k];
+  c->stats[16] = p1_ctr[C_SEEDS];
+  for (int k = 0; k < 6; ++k) c->stats[17 + k] = p1_ctr[C_P1_STAGE + k];
 }
-
 }  // namespace rmb
 
 namespace rmb {
@@ -1241,6 +1400,7 @@ int rmb_ctx_create(int device, const rmb_params* params, const rmb_matrix* matri
     hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
     c->cu_count = prop.multiProcessorCount;
     hip_check(hipStreamCreate(&c->st), "hipStreamCreate");
+    hip_check(hipStreamCreate(&c->st1), "hipStreamCreate");
     hip_check(hipEventCreate(&c->ev0), "hipEventCreate");
     hip_check(hipEventCreate(&c->ev1), "hipEventCreate");
     c->P = *params;
@@ -1266,12 +1426,19 @@ int rmb_ctx_destroy(rmb_ctx* c) {
                     &c->b_keys, &c->b_keys2, &c->b_vals, &c->b_vals2, &c->b_counts, &c->b_cells, &c->b_sort_tmp,
                     &c->b_hits, &c->b_hitcount, &c->b_spill, &c->b_hitres, &c->b_queue, &c->b_counters, &c->b_err, &c->b_tasks,
                     &c->b_dpres, &c->b_ring, &c->b_ring_big, &c->b_tbtasks, &c->b_tbres, &c->b_flat, &c->b_rows,
-                    &c->b_ops, &c->b_pre, &c->b_redo, &c->b_precount, &c->b_packtasks, &c->b_packed, &c->b_items, &c->b_qext})
+                    &c->b_ops, &c->b_pre, &c->b_redo, &c->b_precount, &c->b_packtasks, &c->b_packed, &c->b_items, &c->b_qext,
+                    &c->b_recids})
     b->release();
+  for (P1Slot& S : c->slot) {
+    for (DevBuf* b : {&S.pre, &S.redo_hits, &S.cnt, &S.hitcnt, &S.ctr, &S.err}) b->release();
+    for (hipEvent_t e : S.ev) (void)hipEventDestroy(e);
+    if (S.done) (void)hipEventDestroy(S.done);
+  }
   if (c->d_mat) (void)hipFree(c->d_mat);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->st) (void)hipStreamDestroy(c->st);
+  if (c->st1) (void)hipStreamDestroy(c->st1);
   delete c;
   return RMB_OK;
 }
@@ -1416,7 +1583,6 @@ int rmb_tap_ungapped(rmb_ctx* c, const uint8_t* query, uint32_t q_len, uint32_t 
     std::vector<RecDesc> recs(1);
     recs[0].q_base = 0;
     recs[0].q_len = q_len;
-    std::vector<UngHit> hits;
     unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
     hip_check(hipMemset(ctr, 0, C_NCOUNTERS * sizeof(unsigned long long)), "memset counters");
     const int lw = lut_width_for(c->P.word_size, 2ull * q_len);
@@ -1428,10 +1594,38 @@ int rmb_tap_ungapped(rmb_ctx* c, const uint8_t* query, uint32_t q_len, uint32_t 
     one.whole_base = {0};
     one.pool = qf.size();
     const uint8_t* d_qext = c->P.dust ? upload_dusted(c, {qf}, one) : d_qfwd;
-    if (q_len >= (uint32_t)c->P.word_size) phase1_group(c, {0u}, recs, lw, d_qfwd, d_qrc, d_qext, &hits, nullptr);
+    std::vector<UngHit> hits;
+    if (q_len >= (uint32_t)c->P.word_size) {
+      P1Group g;
+      g.lw = lw;
+      g.recs_idx = {0};
+      g.recs = recs;
+      g.elem_base = {0, 2ull * q_len + 1};
+      unsigned long long* hitcount = c->b_hitcount.get<unsigned long long>(1);
+      uint32_t* err = c->b_err.get<uint32_t>(1);
+      for (uint64_t cap = 1u << 16;;) {
+        UngHit* dh = c->b_hits.get<UngHit>(cap);
+        hip_check(hipMemsetAsync(hitcount, 0, sizeof(unsigned long long), c->st1), "memset hitcount");
+        hip_check(hipMemsetAsync(err, 0, sizeof(uint32_t), c->st1), "memset err");
+        launch_group_phase1(c, c->st1, g, d_qfwd, d_qrc, d_qext, dh, cap, hitcount, ctr, err, nullptr, nullptr);
+        unsigned long long nh = 0;
+        uint32_t e = 0;
+        hip_check(hipMemcpyAsync(&nh, hitcount, sizeof nh, hipMemcpyDeviceToHost, c->st1), "copy hitcount");
+        hip_check(hipMemcpyAsync(&e, err, sizeof e, hipMemcpyDeviceToHost, c->st1), "copy err");
+        hip_check(hipStreamSynchronize(c->st1), "tap sync");
+        if (e & E_HASH_FULL) throw Failure(RMB_ERR_CAPACITY, "diagonal hash bucket exceeded 256 cells");
+        if (nh > cap) {
+          cap = nh + nh / 4;
+          continue;
+        }
+        hits.resize(nh);
+        hip_check(hipMemcpy(hits.data(), dh, nh * sizeof(UngHit), hipMemcpyDeviceToHost), "copy hits");
+        break;
+      }
+    }
     std::vector<UngRec> v;
     for (auto& h : hits)
-      if (h.item % c->subj.size() == subject)
+      if (h.item != kNoItem && h.item % c->subj.size() == subject)  // skip reserved slots left unused
         v.push_back(UngRec{h.q_seed, h.s_seed, h.q_start, h.q_end, h.s_start, h.s_end, h.score, (uint8_t)h.strand});
     order_ungapped(v, q_len, (uint32_t)c->subj[subject].plus.size() - 2);
     if (out) {
