@@ -8,7 +8,7 @@ namespace rmb {
 // at a time; kSeedCap seeds are staged in LDS per pass; the 512 diagonal-hash
 // buckets are owned two per thread.
 constexpr int kP1Threads = 256;
-constexpr int kSeedCap = 768;
+constexpr int kSeedCap = 1536;
 constexpr int kHashLds = 3;      // cells per bucket held in the owner thread's registers
 constexpr int kHashGlobal = 253; // further cells per bucket in the workgroup's global spill area
 constexpr int kBuckets = 512;

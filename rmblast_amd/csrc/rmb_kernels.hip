@@ -293,7 +293,6 @@ __device__ __forceinline__ void sort8(uint32_t (&v)[8]) {
 // so each thread replays its two buckets' seeds in scan order.  Seeds the
 // replay keeps and that reach the cutoff are emitted.
 enum SeedFlag : uint8_t { F_VALID = 1, F_HIT = 2, F_EMIT = 4, F_MINUS = 8 };
-constexpr uint32_t kHitBuf = 255;    // hit extension results kept in LDS per pass; slot 255 = in global memory
 constexpr uint32_t kOutChunk = 256;  // output slots a workgroup reserves at a time
 
 __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
@@ -305,12 +304,11 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
     unsigned long long* __restrict__ queue, unsigned long long* __restrict__ counters, uint32_t* __restrict__ err) {
   __shared__ int mat[256];
   __shared__ uint32_t s_cum[kP1Threads], s_cst[kP1Threads], wsum[4];
-  __shared__ uint4 sd[kSeedCap];  // per seed: {diagonal, hash check offset, level, steps | flags << 16 | hit slot << 24}
+  __shared__ uint4 sd[kSeedCap];  // per seed: {diagonal, hash check offset, level, steps | flags << 16}
   __shared__ uint16_t blist[kSeedCap];
   __shared__ uint32_t bcnt[kBuckets], bst[kBuckets];
-  __shared__ uint4 hbuf[kHitBuf];  // per hit: {q_beg, q_end, score, extension-frame query offset}
   __shared__ unsigned long long s_item, s_out_next, s_pass_a, s_pass_b;
-  __shared__ uint32_t s_out_left, s_pass_an, s_nhit, s_nemit, s_emit_i;
+  __shared__ uint32_t s_out_left, s_pass_an, s_nemit, s_emit_i;
 
   const int tid = threadIdx.x;
   const uint32_t xcd = blockIdx.x % kXcds;
@@ -320,7 +318,7 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
     s_out_left = 0;
   }
   uint2* my_spill = spill + (size_t)blockIdx.x * kBuckets * kHashGlobal;
-  uint4* my_hitres = hitres + (size_t)blockIdx.x * kSeedCap;  // hits beyond the LDS buffer
+  uint4* my_hitres = hitres + (size_t)blockIdx.x * kSeedCap;  // extension results of the pass's hits, by seed
   const int32_t ws = P.word_size, lw = P.lut_width, step = P.scan_step;
   const int32_t ext_needed = ws - lw;
   const int32_t window = 1 - ws;
@@ -362,7 +360,6 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
         }
       }
       s_item = it;
-      s_nhit = 0;
       s_nemit = 0;
       s_emit_i = 0;
     }
@@ -436,7 +433,7 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
         const int32_t q0c = (int32_t)entries[s_cst[lo] + (g - prev)];
         uint32_t flag = 0;
         int32_t diag = 0;
-        uint32_t chk = 0, level = 0, steps = 0, hslot = kHitBuf;
+        uint32_t chk = 0, level = 0, steps = 0;
         // Strand-specific part: exact word extension, diagonal, hash check
         // offset and the extension frame (eq, es).  Plus strand: forward query
         // and subject; minus strand: reverse-complement query against the
@@ -509,12 +506,7 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
                               P.xdrop_ungap, u))
             u = ungapped_ext(qpool, R.q_base, q_len + 1, splus, S.s_base, s_len + 1, eq, es, mat, P.xdrop_ungap);
           const bool hit = u.score >= P.cutoff;
-          if (hit) {
-            const uint4 hv = make_uint4(u.q_beg, u.q_end, (uint32_t)u.score, (uint32_t)eq);
-            hslot = atomicAdd(&s_nhit, 1u);
-            if (hslot < kHitBuf) hbuf[hslot] = hv;
-            else { hslot = kHitBuf; my_hitres[i] = hv; }
-          }
+          if (hit) my_hitres[i] = make_uint4(u.q_beg, u.q_end, (uint32_t)u.score, (uint32_t)eq);
           level = hit ? (uint32_t)es + (u.q_end - (uint32_t)eq) : chk + (uint32_t)ws;
           steps = u.steps;
           flag = F_VALID | (hit ? F_HIT : 0) | (q0c < c1 ? 0 : F_MINUS);
@@ -524,11 +516,10 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
           chk = 0;
         }
         sd[i] = make_uint4((uint32_t)diag, chk, level,
-                           (steps < 65535u ? steps : 65535u) | (flag << 16) | (hslot << 24));
+                           (steps < 65535u ? steps : 65535u) | (flag << 16));
       }
       __syncthreads();
       tick(2);
-      if (tid == 0) s_nhit = 0;  // every slot of this pass is taken; F reads hbuf, not the count
       // ---- D: bucket lists (counting sort; each owner re-sorts by seed index)
       {
         const uint32_t v0 = bcnt[2 * tid], v1 = bcnt[2 * tid + 1];
@@ -692,8 +683,7 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
         for (uint32_t i = tid; i < take; i += kP1Threads) {
           const uint4 e = sd[i];
           if (!((e.w >> 16) & F_EMIT)) continue;
-          const uint32_t hs = e.w >> 24;
-          const uint4 hv = hs < kHitBuf ? hbuf[hs] : my_hitres[i];
+          const uint4 hv = my_hitres[i];
           const int32_t eq = (int32_t)hv.w;
           const bool minus = ((e.w >> 16) & F_MINUS) != 0;
           const int32_t es = minus ? lw - 1 + (int32_t)e.y : (int32_t)e.y;
