@@ -33,6 +33,7 @@ struct P1Params {
   uint64_t n_items;          // records x subjects, or the length of item_list
   uint64_t out_cap;
   const uint32_t* item_list; // null: every (record, subject) item; else these item ids
+  int32_t known_word;        // whole-word table (ws >= 7, scan step 1) and positive ACGT match scores
 };
 
 constexpr uint32_t kNoItem = 0xFFFFFFFFu;  // UngHit::item of an output slot reserved but left unused

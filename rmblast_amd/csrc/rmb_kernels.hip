@@ -229,6 +229,95 @@ __device__ __forceinline__ bool ungapped_short(const uint8_t* __restrict__ qpool
   return true;
 }
 
+// Fast first tier of the extension, for whole-word tables (ws >= 7) with
+// positive ACGT match scores.  Every base of the seed word matches, so each
+// of its steps raises the score: the state after them is known, and only the
+// flanks need stepping.  Plus-strand seeds start at the word's first base, so
+// the right side opens with 7 word steps; minus-strand seeds start at its last
+// base, so the left side opens with 6 and the right with 1.  Then eight
+// branch-free flank steps per side, from the 32 bytes [qa - 16, qa + 16) of
+// each sequence (three aligned 16-byte loads).  True: both sides ended within
+// those steps and `out` is the full extension's result; false: a side still
+// runs (or the seed lies within 16 bytes of its pool's start), and the caller
+// runs the full extension.
+__device__ __forceinline__ bool ungapped_first(const uint8_t* __restrict__ qpool, int64_t qa,
+                                               const uint8_t* __restrict__ spool, int64_t sa, int32_t q_off,
+                                               bool minus, const int* smat, int32_t xdrop, UngOut& out) {
+  if (qa < 16 || sa < 16) return false;
+  const uint4* QV = reinterpret_cast<const uint4*>(qpool);
+  const uint4* SV = reinterpret_cast<const uint4*>(spool);
+  const int64_t cq = (qa - 16) >> 4, cs = (sa - 16) >> 4;
+  const uint4 q0 = QV[cq], q1 = QV[cq + 1], q2 = QV[cq + 2];
+  const uint4 s0 = SV[cs], s1 = SV[cs + 1], s2 = SV[cs + 2];
+  const uint32_t oq = (uint32_t)qa & 15u, os = (uint32_t)sa & 15u;
+  const uint4 qa16 = funnel16(q0, q1, oq), qb16 = funnel16(q1, q2, oq);
+  const uint4 sa16 = funnel16(s0, s1, os), sb16 = funnel16(s1, s2, os);
+  // index bytes (q << 4 | s); byte 16 is the seed position qa
+  uint32_t ix[8];
+  {
+    const uint32_t qw[8] = {qa16.x, qa16.y, qa16.z, qa16.w, qb16.x, qb16.y, qb16.z, qb16.w};
+    const uint32_t sw[8] = {sa16.x, sa16.y, sa16.z, sa16.w, sb16.x, sb16.y, sb16.z, sb16.w};
+#pragma unroll
+    for (int w = 0; w < 8; ++w) ix[w] = ((qw[w] << 4) & 0xF0F0F0F0u) | (sw[w] & 0x0F0F0F0Fu);
+  }
+  auto sc = [&](int p) { return smat[(ix[p >> 2] >> (8 * (p & 3))) & 0xFFu]; };
+  // the word's steps: plus bytes 16..22 on the right; minus bytes 15..10 on
+  // the left and byte 16 on the right
+  int32_t known_l = 0, known_r = 0;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    const int32_t v = smat[(minus ? (k < 6 ? (ix[(15 - k) >> 2] >> (8 * ((15 - k) & 3)))
+                                          : (ix[4] & 0xFFu))
+                                  : (ix[(16 + k) >> 2] >> (8 * ((16 + k) & 3)))) & 0xFFu];
+    known_l += (minus && k < 6) ? v : 0;
+    known_r += (!minus || k == 6) ? v : 0;
+  }
+  const int32_t x = -xdrop;
+  // left: 8 flank steps after the known ones
+  const int32_t qb0 = minus ? q_off - 6 : q_off;
+  int32_t score = known_l, sum = 0, q_beg = qb0;
+  uint32_t steps = minus ? 6u : 0u;
+  bool run = true;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int32_t v = minus ? sc(9 - j) : sc(15 - j);
+    const int32_t ns = sum + v;
+    const bool up = run && ns > 0;
+    const bool stop = run && ns < x;
+    score = up ? score + ns : score;
+    q_beg = up ? qb0 - 1 - j : q_beg;
+    sum = run ? (ns > 0 ? 0 : ns) : sum;
+    steps += run ? 1u : 0u;
+    run = run && !stop;
+  }
+  if (run) return false;
+  if (sum <= kSentScore / 2) --steps;  // ended on a sentinel: that step is not counted
+  // right: the known steps, then 8 flank steps
+  score += known_r;
+  steps += minus ? 1u : 7u;
+  const int32_t qe0 = minus ? q_off + 1 : q_off + 7;
+  int32_t q_end = qe0, xc = (-score > x) ? -score : x;
+  sum = 0;
+  run = true;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int32_t v = minus ? sc(17 + j) : sc(23 + j);
+    const int32_t ns = sum + v;
+    const bool up = run && ns > 0;
+    const bool stop = run && ns < xc;
+    score = up ? score + ns : score;
+    q_end = up ? qe0 + j + 1 : q_end;
+    xc = up ? ((-score > x) ? -score : x) : xc;
+    sum = run ? (ns > 0 ? 0 : ns) : sum;
+    steps += run ? 1u : 0u;
+    run = run && !stop;
+  }
+  if (run) return false;
+  if (sum <= kSentScore / 2) --steps;
+  out = UngOut{score, (uint32_t)q_beg, (uint32_t)q_end, steps};
+  return true;
+}
+
 // --------------------------------------------------------- lookup build -----
 // One key per window start p of record r's two-strand buffer [fwd | N | rc]
 // (engine.rs:630-649).  Valid windows lie inside one strand, hold no code >= 4
@@ -292,7 +381,7 @@ __device__ __forceinline__ void sort8(uint32_t (&v)[8]) {
 // the reference's 512-bucket diagonal hash exactly: buckets are independent,
 // so each thread replays its two buckets' seeds in scan order.  Seeds the
 // replay keeps and that reach the cutoff are emitted.
-enum SeedFlag : uint8_t { F_VALID = 1, F_HIT = 2, F_EMIT = 4, F_MINUS = 8 };
+enum SeedFlag : uint8_t { F_VALID = 1, F_HIT = 2, F_EMIT = 4, F_MINUS = 8, F_LONG = 16 };
 constexpr uint32_t kOutChunk = 256;  // output slots a workgroup reserves at a time
 
 __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
@@ -306,9 +395,11 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
   __shared__ uint32_t s_cum[kP1Threads], s_cst[kP1Threads], wsum[4];
   __shared__ uint4 sd[kSeedCap];  // per seed: {diagonal, hash check offset, level, steps | flags << 16}
   __shared__ uint16_t blist[kSeedCap];
+  __shared__ uint16_t llist[kSeedCap];  // seeds whose extension outruns the fast first tier
+  __shared__ uint8_t posmap[kSeedCap];  // pass seed -> its scan position in the window
   __shared__ uint32_t bcnt[kBuckets], bst[kBuckets];
   __shared__ unsigned long long s_item, s_out_next, s_pass_a, s_pass_b;
-  __shared__ uint32_t s_out_left, s_pass_an, s_nemit, s_emit_i;
+  __shared__ uint32_t s_out_left, s_pass_an, s_nemit, s_emit_i, s_nlong;
 
   const int tid = threadIdx.x;
   const uint32_t xcd = blockIdx.x % kXcds;
@@ -316,20 +407,27 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
   if (tid == 0) {
     s_out_next = 0;
     s_out_left = 0;
+    s_nlong = 0;
   }
   uint2* my_spill = spill + (size_t)blockIdx.x * kBuckets * kHashGlobal;
   uint4* my_hitres = hitres + (size_t)blockIdx.x * kSeedCap;  // extension results of the pass's hits, by seed
   const int32_t ws = P.word_size, lw = P.lut_width, step = P.scan_step;
   const int32_t ext_needed = ws - lw;
   const int32_t window = 1 - ws;
-  unsigned long long n_evals = 0, n_steps = 0, n_hits = 0, n_seeds = 0;
-  // per-stage time (thread 0, 100 MHz clock read after each barrier)
-  unsigned long long t_stage[6] = {0, 0, 0, 0, 0, 0}, t_prev = wall_clock64();
+  unsigned long long n_evals = 0, n_steps = 0, n_hits = 0;
+  // per-stage time (thread 0, 100 MHz clock read after each barrier) and
+  // seeds staged, kept in LDS: only thread 0 touches them
+  __shared__ unsigned long long s_tick[7], s_nseeds;
+  if (tid == 0) {
+    for (int k = 0; k < 6; ++k) s_tick[k] = 0;
+    s_tick[6] = wall_clock64();
+    s_nseeds = 0;
+  }
   auto tick = [&](int k) {
     if (tid == 0) {
       const unsigned long long t = wall_clock64();
-      t_stage[k] += t - t_prev;
-      t_prev = t;
+      s_tick[k] += t - s_tick[6];
+      s_tick[6] = t;
     }
   };
 
@@ -418,16 +516,20 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
         }
       }
       const uint32_t take = (wtotal - wk) < (uint32_t)kSeedCap ? (wtotal - wk) : (uint32_t)kSeedCap;
-      n_seeds += tid == 0 ? take : 0;
+      if (tid == 0) s_nseeds += take;
+      // the pass's seed -> scan position map: each position's thread writes
+      // the entries of its own seeds
+      {
+        const uint32_t hi = s_cum[tid], lo = tid ? s_cum[tid - 1] : 0u;
+        const uint32_t a = lo > wk ? lo : wk, b = hi < wk + take ? hi : wk + take;
+        for (uint32_t g = a; g < b; ++g) posmap[g - wk] = (uint8_t)tid;
+      }
+      __syncthreads();
       // ---- B+C: materialise each seed, exact word extension, diagonal,
       // speculative ungapped extension (the lane that fills a seed extends it)
       for (uint32_t i = tid; i < take; i += kP1Threads) {
         const uint32_t g = wk + i;  // seed index within the window
-        int lo = 0, hi = kP1Threads - 1;
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (s_cum[mid] > g) hi = mid; else lo = mid + 1;
-        }
+        const int lo = posmap[i];
         const uint32_t prev = lo ? s_cum[lo - 1] : 0;
         const int32_t s_fwd = (int32_t)((w0 + (uint32_t)lo) * (uint32_t)step);
         const int32_t q0c = (int32_t)entries[s_cst[lo] + (g - prev)];
@@ -497,19 +599,26 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
           eq = q_len - 1 - q_off;
           es = s_len - 1 - s_off;
         }
+        bool longer = false;
         if (ok) {
           // one extension call for both strands, so a wave holding seeds of
           // both runs it once
-          const uint8_t* qpool = q0c < c1 ? qfwd : qrc;
+          const bool minus = q0c >= c1;
+          const uint8_t* qpool = minus ? qrc : qfwd;
+          const int64_t qa = (int64_t)R.q_base + 1 + eq, sa = (int64_t)S.s_base + 1 + es;
           UngOut u;
-          if (!ungapped_short(qpool, (int64_t)R.q_base + 1 + eq, splus, (int64_t)S.s_base + 1 + es, eq, mat,
-                              P.xdrop_ungap, u))
-            u = ungapped_ext(qpool, R.q_base, q_len + 1, splus, S.s_base, s_len + 1, eq, es, mat, P.xdrop_ungap);
-          const bool hit = u.score >= P.cutoff;
-          if (hit) my_hitres[i] = make_uint4(u.q_beg, u.q_end, (uint32_t)u.score, (uint32_t)eq);
-          level = hit ? (uint32_t)es + (u.q_end - (uint32_t)eq) : chk + (uint32_t)ws;
-          steps = u.steps;
-          flag = F_VALID | (hit ? F_HIT : 0) | (q0c < c1 ? 0 : F_MINUS);
+          // without the fast tier every seed takes the full extension below
+          if (P.known_word && ungapped_first(qpool, qa, splus, sa, eq, minus, mat, P.xdrop_ungap, u)) {
+            const bool hit = u.score >= P.cutoff;
+            if (hit) my_hitres[i] = make_uint4(u.q_beg, u.q_end, (uint32_t)u.score, (uint32_t)eq);
+            level = hit ? (uint32_t)es + (u.q_end - (uint32_t)eq) : chk + (uint32_t)ws;
+            steps = u.steps;
+            flag = F_VALID | (hit ? F_HIT : 0) | (minus ? F_MINUS : 0);
+          } else {  // the full extension runs below, over a compact list; level holds eq meanwhile
+            longer = true;
+            level = (uint32_t)eq;
+            flag = F_VALID | F_LONG | (minus ? F_MINUS : 0);
+          }
           atomicAdd(&bcnt[(uint32_t)diag & (kBuckets - 1)], 1u);
         } else {
           diag = 0;
@@ -517,9 +626,43 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
         }
         sd[i] = make_uint4((uint32_t)diag, chk, level,
                            (steps < 65535u ? steps : 65535u) | (flag << 16));
+        // the list of longer extensions: one LDS atomic per wave
+        const unsigned long long m = __ballot(longer);
+        if (m) {
+          const uint32_t lane = (uint32_t)tid & 63u;
+          const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
+          uint32_t base = 0;
+          if (lane == leader) base = atomicAdd(&s_nlong, (uint32_t)__popcll(m));
+          base = __shfl(base, (int)leader, 64);
+          if (longer) llist[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)i;
+        }
       }
       __syncthreads();
+      // ---- C: full extensions of the listed seeds
+      const uint32_t n_long = s_nlong;
+      if (n_long) {
+        for (uint32_t j = tid; j < n_long; j += kP1Threads) {
+          const uint32_t i = llist[j];
+          const uint4 e = sd[i];
+          const uint32_t flag = (e.w >> 16) & (uint32_t)(F_VALID | F_MINUS);
+          const bool minus = (flag & F_MINUS) != 0;
+          const int32_t eq = (int32_t)e.z;
+          const int32_t es = minus ? lw - 1 + (int32_t)e.y : (int32_t)e.y;
+          const uint8_t* qpool = minus ? qrc : qfwd;
+          UngOut u;
+          if (!ungapped_short(qpool, (int64_t)R.q_base + 1 + eq, splus, (int64_t)S.s_base + 1 + es, eq, mat,
+                              P.xdrop_ungap, u))
+            u = ungapped_ext(qpool, R.q_base, q_len + 1, splus, S.s_base, s_len + 1, eq, es, mat, P.xdrop_ungap);
+          const bool hit = u.score >= P.cutoff;
+          if (hit) my_hitres[i] = make_uint4(u.q_beg, u.q_end, (uint32_t)u.score, (uint32_t)eq);
+          const uint32_t level = hit ? (uint32_t)es + (u.q_end - (uint32_t)eq) : e.y + (uint32_t)ws;
+          const uint32_t steps = u.steps < 65535u ? u.steps : 65535u;
+          sd[i] = make_uint4(e.x, e.y, level, steps | ((flag | (hit ? F_HIT : 0)) << 16));
+        }
+        __syncthreads();
+      }
       tick(2);
+      if (tid == 0) s_nlong = 0;  // every thread read it before the barrier above
       // ---- D: bucket lists (counting sort; each owner re-sorts by seed index)
       {
         const uint32_t v0 = bcnt[2 * tid], v1 = bcnt[2 * tid + 1];
@@ -740,8 +883,8 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
   atomicAdd(&counters[C_UNG_STEPS], n_steps);
   atomicAdd(&counters[C_UNG_HITS], n_hits);
   if (tid == 0) {
-    atomicAdd(&counters[C_SEEDS], n_seeds);
-    for (int k = 0; k < 6; ++k) atomicAdd(&counters[C_P1_STAGE + k], t_stage[k]);
+    atomicAdd(&counters[C_SEEDS], s_nseeds);
+    for (int k = 0; k < 6; ++k) atomicAdd(&counters[C_P1_STAGE + k], s_tick[k]);
   }
 }
 

@@ -549,6 +549,9 @@ static void launch_group_phase1(rmb_ctx* c, hipStream_t st, P1Group& g, const ui
   P.n_items = (uint64_t)R * nsub;
   P.item_list = nullptr;
   P.out_cap = cap;
+  bool pos_match = true;
+  for (int b = 0; b < 4; ++b) pos_match = pos_match && c->M.s[b][b] > 0;
+  P.known_word = ws >= 7 && lw == ws && pos_match ? 1 : 0;
   if (g.listed) {  // group-local item ids (record index in the group * nsub + subject)
     uint32_t* dl = c->b_items.get<uint32_t>(g.items.size());
     hip_check(hipMemcpyAsync(dl, g.items.data(), g.items.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st),
