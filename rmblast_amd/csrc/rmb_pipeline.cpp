@@ -1193,20 +1193,26 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
   if (c->P.dust) d_qext = upload_dusted(c, qf, sr);
   unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
   hip_check(hipMemset(ctr, 0, C_NCOUNTERS * sizeof(unsigned long long)), "memset counters");
-  // ---- batches of whole queries with about equal bases, pipelined: the GPU
-  // seeds batch b + 1 while the host replays and aligns batch b
+  // ---- batches of whole queries, pipelined: the GPU seeds batch b + 1 while
+  // the host replays and aligns batch b.  Whatever of the last batch's work
+  // the host does after the seeding stream drains is not overlapped, so the
+  // last batch gets a quarter of the others' size: cuts fall at S, 2S, ...,
+  // (B - 1)S with S = total / (B - 3/4).
   uint64_t total_bases = 0;
   for (uint32_t i = 0; i < nsr; ++i) total_bases += recs[i].q_len;
   int max_batches = kMaxBatches;
   if (const char* e = std::getenv("RMB_BATCHES")) max_batches = std::max(1, std::atoi(e));
-  const uint64_t per_batch = std::max<uint64_t>(kBatchMinBases, (total_bases + max_batches - 1) / max_batches);
+  const double span = std::max<double>((double)kBatchMinBases,
+                                       max_batches > 1 ? (double)total_bases / (max_batches - 0.75)
+                                                       : (double)total_bases);
   std::vector<P1Batch> batches;
   {
     uint32_t k0 = 0;
     uint64_t acc = 0;
     for (uint32_t q = 0; q < nq; ++q) {
       for (uint32_t k = sr.first[q]; k < sr.first[q + 1]; ++k) acc += recs[k].q_len;
-      if (acc >= per_batch || q + 1 == nq) {
+      const double cut = span * (double)(batches.size() + 1);
+      if ((double)acc >= cut || q + 1 == nq) {
         const uint32_t k1 = sr.first[q + 1];
         if (k1 > k0) {
           P1Batch b;
@@ -1215,7 +1221,6 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
           batches.push_back(std::move(b));
         }
         k0 = k1;
-        acc = 0;
       }
     }
   }
