@@ -109,6 +109,20 @@ struct TbTask {         // traceback pass: a DpTask plus its storage
 
 struct TbResult { uint32_t n_ops; uint32_t flags; };
 
+struct OnePassResult {  // a half alignment scored and traced back in one pass
+  int32_t score;
+  uint32_t a_len, b_len, n_ops;
+  uint32_t flags;       // ErrBit: E_BAND_WIDE / E_TB_SPACE send the half to the two-pass path
+  uint32_t pad;
+  uint64_t ops_at;      // its ops in the packed pool
+};
+
+// Per-lane arena of the one-pass traceback: row table (8 B per row), then the
+// traceback bytes (16-byte aligned), then the ops (4 B each).
+inline constexpr uint64_t onepass_stride(uint32_t flat_cap, uint32_t rows_cap, uint32_t ops_cap) {
+  return (uint64_t)rows_cap * 8 + ((flat_cap + 15u) & ~15u) + (((uint64_t)ops_cap * 4 + 15u) & ~15ull);
+}
+
 struct PackTask {       // copy n ops of one traceback from the op pool into the packed list
   uint64_t src, dst;
   uint32_t n, pad;

@@ -903,7 +903,8 @@ static constexpr uint8_t SCRIPT_GAP_IN_A = 0, SCRIPT_SUB = 3, SCRIPT_GAP_IN_B = 
 template <bool TB, uint32_t RS = 1>
 __device__ __forceinline__ void dp_half(const DpTask& t, const int* mat, int32_t go, int32_t ge,
                                         int2* __restrict__ ring, uint32_t wmask, DpResult& res,
-                                        uint8_t* __restrict__ flat, uint32_t* __restrict__ rows, uint32_t stop_row) {
+                                        uint8_t* __restrict__ flat, uint32_t* __restrict__ rows, uint32_t stop_row,
+                                        uint32_t flat_cap = 0xFFFFFFFFu, uint32_t rows_cap = 0xFFFFFFFFu) {
   res.score = 0; res.a_len = 0; res.b_len = 0; res.flags = 0; res.cells = 0; res.tb_bytes = 0; res.tb_rows = 0;
   const uint32_t m = t.m, n = t.n;
   if (m == 0 || n == 0) return;
@@ -931,6 +932,10 @@ __device__ __forceinline__ void dp_half(const DpTask& t, const int* mat, int32_t
   uint32_t bytes_at_best = flat_len, rows_at_best = 1;
   for (uint32_t ai = 0; ai < m; ++ai) {
     if (TB && ai >= stop_row) break;
+    // a row adds one row-table entry and at most two rings' worth of bytes: its
+    // band (at most wmask + 1 cells past the row's first column) plus the
+    // columns its first cells drop
+    if (TB && (ai + 2 > rows_cap || flat_len + 2 * wmask + 4 > flat_cap)) { res.flags |= E_TB_SPACE; return; }
     const int* row = mat + ((t.a[rev ? m - 1 - ai : ai] & 15) << 4);
     if (TB) { rows[2 * (ai + 1)] = flat_len; rows[2 * (ai + 1) + 1] = first_b; }
     int32_t sc = kMinInt, sgr = kMinInt;
@@ -998,6 +1003,54 @@ __device__ __forceinline__ void dp_half(const DpTask& t, const int* mat, int32_t
   res.tb_rows = rows_at_best;
 }
 
+// Traceback walk (gapped.rs:692-757) from (a, b), the best row and column,
+// over the rows' traceback bytes: the ops, run-length encoded, collected
+// backwards into ops[0, cap); the count is returned, E_TB_SPACE set in flags
+// when they do not fit or the walk does not reach the origin.
+__device__ __forceinline__ uint32_t tb_walk(const uint8_t* __restrict__ flat, const uint32_t* __restrict__ rows,
+                                            uint32_t a, uint32_t b, uint32_t flat_total, uint32_t* __restrict__ ops,
+                                            uint32_t cap, uint32_t& flags) {
+  uint32_t nops = 0;
+  const uint32_t n_rows = a + 1;
+  uint8_t prev = SCRIPT_SUB;
+  uint32_t last_kind = 3, run = 0;
+  const uint32_t budget = a + b;
+  for (uint32_t stepi = 0; stepi < budget && (a | b); ++stepi) {
+    const uint32_t off = rows[2 * a], fb = rows[2 * a + 1];
+    const uint32_t row_end = a + 1 < n_rows ? rows[2 * (a + 1)] : flat_total;
+    uint8_t raw = SCRIPT_GAP_IN_A;
+    if (b >= fb && off + (b - fb) < row_end) raw = flat[off + (b - fb)];
+    uint8_t op;
+    if ((prev & SCRIPT_OP_MASK) == SCRIPT_GAP_IN_A)
+      op = (raw & SCRIPT_EXTEND_GAP_A) ? SCRIPT_GAP_IN_A : (raw & SCRIPT_OP_MASK);
+    else if ((prev & SCRIPT_OP_MASK) == SCRIPT_GAP_IN_B)
+      op = (raw & SCRIPT_EXTEND_GAP_B) ? SCRIPT_GAP_IN_B : (raw & SCRIPT_OP_MASK);
+    else
+      op = raw & SCRIPT_OP_MASK;
+    prev = op;
+    uint32_t kind;
+    if (op == SCRIPT_GAP_IN_A) { kind = 1; if (b) --b; }
+    else if (op == SCRIPT_GAP_IN_B) { kind = 2; if (a) --a; }
+    else { kind = 0; if (a) --a; if (b) --b; }
+    if (kind == last_kind) {
+      ++run;
+    } else {
+      if (run) {
+        if (nops >= cap) { flags |= E_TB_SPACE; return nops; }
+        ops[nops++] = last_kind | (run << 2);
+      }
+      last_kind = kind;
+      run = 1;
+    }
+  }
+  if (run) {
+    if (nops >= cap) { flags |= E_TB_SPACE; return nops; }
+    ops[nops++] = last_kind | (run << 2);
+  }
+  if (a | b) flags |= E_TB_SPACE;
+  return nops;
+}
+
 // Traceback-phase bands (X-drop 100 by default) stay within 64 cells: the
 // first attempt gives each lane a 64-cell ring in LDS (64-lane workgroups);
 // wider bands rerun with per-lane rings in global memory.
@@ -1047,55 +1100,64 @@ __global__ __launch_bounds__(LDS_RING ? kDpLdsThreads : 256) void k_dp_traceback
     dp_half<true, RS>(T.t, mat, go, ge, my, wmask, r, flat, rows, T.stop_row);
     TbResult o{0, r.flags};
     if (!r.flags) {
-      // traceback walk (gapped.rs:692-757); ops are collected backwards, run-length encoded
-      uint32_t a = T.stop_row, b = T.best_b, nops = 0;
-      const uint32_t n_rows = T.stop_row + 1;
-      const uint32_t flat_total = r.tb_bytes;
-      uint8_t prev = SCRIPT_SUB;
-      uint32_t last_kind = 3, run = 0;
-      const uint32_t budget = a + b;
-      for (uint32_t stepi = 0; stepi < budget && (a | b); ++stepi) {
-        const uint32_t off = rows[2 * a], fb = rows[2 * a + 1];
-        const uint32_t row_end = a + 1 < n_rows ? rows[2 * (a + 1)] : flat_total;
-        uint8_t raw = SCRIPT_GAP_IN_A;
-        if (b >= fb && off + (b - fb) < row_end) raw = flat[off + (b - fb)];
-        uint8_t op;
-        if ((prev & SCRIPT_OP_MASK) == SCRIPT_GAP_IN_A)
-          op = (raw & SCRIPT_EXTEND_GAP_A) ? SCRIPT_GAP_IN_A : (raw & SCRIPT_OP_MASK);
-        else if ((prev & SCRIPT_OP_MASK) == SCRIPT_GAP_IN_B)
-          op = (raw & SCRIPT_EXTEND_GAP_B) ? SCRIPT_GAP_IN_B : (raw & SCRIPT_OP_MASK);
-        else
-          op = raw & SCRIPT_OP_MASK;
-        prev = op;
-        uint32_t kind;
-        if (op == SCRIPT_GAP_IN_A) { kind = 1; if (b) --b; }
-        else if (op == SCRIPT_GAP_IN_B) { kind = 2; if (a) --a; }
-        else { kind = 0; if (a) --a; if (b) --b; }
-        if (kind == last_kind) {
-          ++run;
-        } else {
-          if (run) {
-            if (nops >= T.ops_cap) { o.flags |= E_TB_SPACE; break; }
-            ops[nops++] = last_kind | (run << 2);
-          }
-          last_kind = kind;
-          run = 1;
-        }
-      }
-      if (!(o.flags & E_TB_SPACE) && run) {
-        if (nops >= T.ops_cap) o.flags |= E_TB_SPACE;
-        else ops[nops++] = last_kind | (run << 2);
-      }
-      if (a | b) o.flags |= E_TB_SPACE;
+      const uint32_t nops = tb_walk(flat, rows, T.stop_row, T.best_b, r.tb_bytes, ops, T.ops_cap, o.flags);
       for (uint32_t x = 0, y = nops ? nops - 1 : 0; x < y; ++x, --y) {
         const uint32_t tmp = ops[x];
         ops[x] = ops[y];
         ops[y] = tmp;
       }
       o.n_ops = nops;
+    }    out[i] = o;
+  }
+}
+
+// Score and traceback in one pass: each lane keeps its half alignment's row
+// table, traceback bytes (for every row: the best row is known only at the
+// end) and ops in its own arena, walks back from the best cell, and appends
+// the ops to one packed pool.  A half whose band, rows, bytes or ops outgrow
+// the LDS ring or the arena comes back flagged for the two-pass path.
+template <bool LDS_RING>
+__global__ __launch_bounds__(LDS_RING ? kDpLdsThreads : 256) void k_dp_onepass(
+    const DpTask* __restrict__ tasks, uint32_t n_tasks, const int32_t* __restrict__ gmat, int32_t go, int32_t ge,
+    int2* __restrict__ ring, uint32_t wlog2, uint8_t* __restrict__ arena, uint32_t flat_cap, uint32_t rows_cap,
+    uint32_t ops_cap, uint32_t* __restrict__ ops_pool, unsigned long long* __restrict__ ops_count,
+    unsigned long long ops_pool_cap, OnePassResult* __restrict__ out, unsigned long long* counters) {
+  __shared__ int mat[256];
+  __shared__ int2 lring[LDS_RING ? (kDpLdsThreads << kDpLdsLog2) : 1];
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) mat[i] = gmat[i];
+  __syncthreads();
+  const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+  constexpr uint32_t RS = LDS_RING ? kDpLdsThreads : 1;
+  int2* my = LDS_RING ? lring + threadIdx.x : ring + (size_t)gt * ring_stride(wlog2);
+  const uint32_t wmask = LDS_RING ? (1u << kDpLdsLog2) - 1 : (1u << wlog2) - 1;
+  uint8_t* base = arena + (size_t)gt * onepass_stride(flat_cap, rows_cap, ops_cap);
+  uint32_t* rows = reinterpret_cast<uint32_t*>(base);
+  uint8_t* flat = base + (size_t)rows_cap * 8;
+  uint32_t* ops = reinterpret_cast<uint32_t*>(flat + ((flat_cap + 15u) & ~15u));
+  unsigned long long cells = 0;
+  for (uint32_t i = gt; i < n_tasks; i += nt) {
+    DpResult r;
+    dp_half<true, RS>(tasks[i], mat, go, ge, my, wmask, r, flat, rows, 0xFFFFFFFFu, flat_cap, rows_cap);
+    OnePassResult o{r.score, r.a_len, r.b_len, 0, r.flags, 0};
+    if (!r.flags) {
+      cells += r.cells;
+      if (r.a_len | r.b_len) {
+        const uint32_t n = tb_walk(flat, rows, r.a_len, r.b_len, r.tb_bytes, ops, ops_cap, o.flags);
+        if (!o.flags) {
+          const unsigned long long at = atomicAdd(ops_count, (unsigned long long)n);
+          if (at + n <= ops_pool_cap) {
+            for (uint32_t k = 0; k < n; ++k) ops_pool[at + k] = ops[n - 1 - k];
+            o.n_ops = n;
+            o.ops_at = at;
+          } else {
+            o.flags |= E_TB_SPACE;
+          }
+        }
+      }
     }
     out[i] = o;
   }
+  atomicAdd(&counters[C_DP_CELLS], cells);
 }
 
 // Gathers each traceback's ops from its worst-case slot of the op pool into one
@@ -1275,6 +1337,15 @@ hipError_t launch_dp_score(int blocks, const DpTask* tasks, uint32_t n, const in
   else
     hipLaunchKernelGGL(k_dp_score<false>, dim3(blocks), dim3(256), 0, st, tasks, n, gmat, go, ge, ring, wlog2, out,
                        counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_dp_onepass(int blocks, const DpTask* tasks, uint32_t n, const int32_t* gmat, int32_t go,
+                             int32_t ge, uint8_t* arena, uint32_t flat_cap, uint32_t rows_cap, uint32_t ops_cap,
+                             uint32_t* ops_pool, unsigned long long* ops_count, unsigned long long ops_pool_cap,
+                             OnePassResult* out, unsigned long long* counters, hipStream_t st) {
+  hipLaunchKernelGGL(k_dp_onepass<true>, dim3(blocks), dim3(kDpLdsThreads), 0, st, tasks, n, gmat, go, ge, nullptr,
+                     0, arena, flat_cap, rows_cap, ops_cap, ops_pool, ops_count, ops_pool_cap, out, counters);
   return hipGetLastError();
 }
 

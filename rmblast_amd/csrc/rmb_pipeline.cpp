@@ -51,6 +51,9 @@ hipError_t launch_dp_score(int, const DpTask*, uint32_t, const int32_t*, int32_t
                            DpResult*, unsigned long long*, hipStream_t);
 hipError_t launch_dp_traceback(int, const TbTask*, uint32_t, const int32_t*, int32_t, int32_t, int2*, uint32_t,
                                uint8_t*, uint32_t*, uint32_t*, TbResult*, hipStream_t);
+hipError_t launch_dp_onepass(int, const DpTask*, uint32_t, const int32_t*, int32_t, int32_t, uint8_t*, uint32_t,
+                             uint32_t, uint32_t, uint32_t*, unsigned long long*, unsigned long long, OnePassResult*,
+                             unsigned long long*, hipStream_t);
 hipError_t launch_prelim_fused(int, bool, PrelimParams, const UngHit*, const uint32_t*, uint64_t, const RecDesc*,
                                const SubjDesc*, const uint8_t*, const uint8_t*, const uint8_t*, const int32_t*, int2*,
                                uint32_t, PreHit*, unsigned long long*, uint32_t*, UngHit*, unsigned long long*,
@@ -228,6 +231,7 @@ struct rmb_ctx {
   DevBuf b_hits, b_hitcount, b_spill, b_hitres, b_queue, b_counters, b_err;
   DevBuf b_tasks, b_dpres, b_ring, b_ring_big, b_tbtasks, b_tbres, b_flat, b_rows, b_ops;
   DevBuf b_pre, b_redo, b_precount, b_packtasks, b_packed, b_items, b_qext, b_recids;
+  DevBuf b_arena, b_oppool, b_opres, b_opcount;  // one-pass traceback
 };
 
 namespace rmb {
@@ -349,10 +353,10 @@ static void run_dp_score(rmb_ctx* c, const std::vector<DpTask>& tasks_in, std::v
 // in alignment order from the seed outwards) at ops_at in the op pool.
 struct HalfTb { int32_t score; uint32_t a_len, b_len, n_ops; uint64_t ops_at; };
 
-// Traceback DPs: score pass for sizes, then the traceback pass in batches
-// bounded by the device storage they need.
-static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::vector<HalfTb>& out,
-                             std::vector<uint32_t>& op_pool, const std::vector<uint64_t>* work = nullptr) {
+// Traceback DPs in two passes: score pass for sizes, then the traceback pass
+// in batches bounded by the device storage they need.
+static void run_dp_traceback_2pass(rmb_ctx* c, const std::vector<DpTask>& tasks, std::vector<HalfTb>& out,
+                                   std::vector<uint32_t>& op_pool, const std::vector<uint64_t>* work = nullptr) {
   out.assign(tasks.size(), HalfTb{0, 0, 0, 0, 0});
   op_pool.clear();
   Timer tt;
@@ -469,6 +473,94 @@ static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::
   if (std::getenv("RMB_STAGE_TIMES"))
     std::fprintf(stderr, "[rmb] traceback DPs ms: score pass %.1f  traceback pass %.1f  unpack %.1f  total %.1f\n",
                  t_score / 1e3, t_tb / 1e3, t_unpack / 1e3, tt.us() / 1e3);
+}
+
+// Per-lane arena of the one-pass traceback: rows, traceback bytes and ops of
+// one half alignment.  Halves beyond it (and bands wider than the LDS ring)
+// take the two-pass path.
+constexpr uint32_t kOnePassRows = 8192, kOnePassFlat = 256u << 10, kOnePassOps = 4096;
+constexpr unsigned long long kOnePassPool = 64ull << 20;  // packed ops per launch
+
+// Traceback DPs: one pass (score, traceback bytes, walk) per half in a
+// per-lane arena, longest first; the halves it flags run in two passes.
+static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::vector<HalfTb>& out,
+                             std::vector<uint32_t>& op_pool, const std::vector<uint64_t>* work = nullptr) {
+  out.assign(tasks.size(), HalfTb{0, 0, 0, 0, 0});
+  op_pool.clear();
+  if (tasks.empty()) return;
+  Timer tt;
+  std::vector<uint32_t> ord;
+  if (work) ord = longest_first(*work);
+  else {
+    ord.resize(tasks.size());
+    for (uint32_t i = 0; i < ord.size(); ++i) ord[i] = i;
+  }
+  std::vector<DpTask> sorted(tasks.size());
+  for (size_t k = 0; k < ord.size(); ++k) sorted[k] = tasks[ord[k]];
+  unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
+  std::vector<uint32_t> fallback;
+  const size_t chunk = 1u << 20;
+  for (size_t base = 0; base < sorted.size(); base += chunk) {
+    const uint32_t n = (uint32_t)std::min(chunk, sorted.size() - base);
+    DpTask* dt = c->b_tasks.get<DpTask>(n);
+    OnePassResult* dr = c->b_opres.get<OnePassResult>(n);
+    const uint32_t blocks = std::min<uint32_t>((uint32_t)c->cu_count * 4, (n + 63) / 64);
+    const uint64_t stride = onepass_stride(kOnePassFlat, kOnePassRows, kOnePassOps);
+    uint8_t* arena = c->b_arena.get<uint8_t>((size_t)blocks * 64 * stride);
+    uint32_t* pool = c->b_oppool.get<uint32_t>(kOnePassPool);
+    unsigned long long* pcount = c->b_opcount.get<unsigned long long>(1);
+    hip_check(hipMemsetAsync(pcount, 0, sizeof(unsigned long long), c->st), "clear op count");
+    hip_check(hipMemcpyAsync(dt, sorted.data() + base, n * sizeof(DpTask), hipMemcpyHostToDevice, c->st),
+              "copy tasks");
+    timed_launch(c, 14, [&] {
+      hip_check(launch_dp_onepass((int)blocks, dt, n, c->d_mat, c->P.gap_open, c->P.gap_extend, arena, kOnePassFlat,
+                                  kOnePassRows, kOnePassOps, pool, pcount, kOnePassPool, dr, ctr, c->st),
+                "k_dp_onepass");
+    });
+    std::vector<OnePassResult> r(n);
+    unsigned long long used = 0;
+    hip_check(hipMemcpyAsync(r.data(), dr, n * sizeof(OnePassResult), hipMemcpyDeviceToHost, c->st), "copy onepass");
+    hip_check(hipMemcpyAsync(&used, pcount, sizeof(used), hipMemcpyDeviceToHost, c->st), "copy op count");
+    hip_check(hipStreamSynchronize(c->st), "onepass sync");
+    used = std::min(used, kOnePassPool);
+    const uint64_t pool_at = op_pool.size();
+    op_pool.resize(pool_at + used);
+    if (used) {
+      hip_check(hipMemcpyAsync(op_pool.data() + pool_at, pool, used * sizeof(uint32_t), hipMemcpyDeviceToHost, c->st),
+                "copy ops");
+      hip_check(hipStreamSynchronize(c->st), "ops sync");
+    }
+    for (uint32_t k = 0; k < n; ++k) {
+      const uint32_t i = ord[base + k];
+      if (r[k].flags) {
+        fallback.push_back(i);
+        continue;
+      }
+      out[i] = HalfTb{r[k].score, r[k].a_len, r[k].b_len, r[k].n_ops, pool_at + r[k].ops_at};
+    }
+  }
+  const uint64_t t_one = tt.us();
+  if (!fallback.empty()) {
+    std::vector<DpTask> ft(fallback.size());
+    std::vector<uint64_t> fw;
+    for (size_t k = 0; k < fallback.size(); ++k) ft[k] = tasks[fallback[k]];
+    if (work) {
+      fw.resize(fallback.size());
+      for (size_t k = 0; k < fallback.size(); ++k) fw[k] = (*work)[fallback[k]];
+    }
+    std::vector<HalfTb> fo;
+    std::vector<uint32_t> fpool;
+    run_dp_traceback_2pass(c, ft, fo, fpool, work ? &fw : nullptr);
+    const uint64_t at = op_pool.size();
+    op_pool.insert(op_pool.end(), fpool.begin(), fpool.end());
+    for (size_t k = 0; k < fallback.size(); ++k) {
+      fo[k].ops_at += at;
+      out[fallback[k]] = fo[k];
+    }
+  }
+  if (std::getenv("RMB_STAGE_TIMES"))
+    std::fprintf(stderr, "[rmb] traceback DPs ms: one pass %.1f (%zu halves)  two-pass fallback %.1f (%zu halves)\n",
+                 t_one / 1e3, tasks.size(), (tt.us() - t_one) / 1e3, fallback.size());
 }
 
 struct ItemState {
@@ -1435,7 +1527,7 @@ int rmb_ctx_destroy(rmb_ctx* c) {
                     &c->b_hits, &c->b_hitcount, &c->b_spill, &c->b_hitres, &c->b_queue, &c->b_counters, &c->b_err, &c->b_tasks,
                     &c->b_dpres, &c->b_ring, &c->b_ring_big, &c->b_tbtasks, &c->b_tbres, &c->b_flat, &c->b_rows,
                     &c->b_ops, &c->b_pre, &c->b_redo, &c->b_precount, &c->b_packtasks, &c->b_packed, &c->b_items, &c->b_qext,
-                    &c->b_recids})
+                    &c->b_recids, &c->b_arena, &c->b_oppool, &c->b_opres, &c->b_opcount})
     b->release();
   for (P1Slot& S : c->slot) {
     for (DevBuf* b : {&S.pre, &S.redo_hits, &S.cnt, &S.hitcnt, &S.ctr, &S.err}) b->release();
