@@ -902,7 +902,8 @@ static constexpr uint8_t SCRIPT_GAP_IN_A = 0, SCRIPT_SUB = 3, SCRIPT_GAP_IN_B = 
 
 template <bool TB, uint32_t RS = 1>
 __device__ __forceinline__ void dp_half(const DpTask& t, const int* mat, int32_t go, int32_t ge,
-                                        int2* __restrict__ ring, uint32_t wmask, DpResult& res,
+                                        int2* __restrict__ ring, uint8_t* __restrict__ bring, uint32_t wmask,
+                                        DpResult& res,
                                         uint8_t* __restrict__ flat, uint32_t* __restrict__ rows, uint32_t stop_row,
                                         uint32_t flat_cap = 0xFFFFFFFFu, uint32_t rows_cap = 0xFFFFFFFFu) {
   res.score = 0; res.a_len = 0; res.b_len = 0; res.flags = 0; res.cells = 0; res.tb_bytes = 0; res.tb_rows = 0;
@@ -911,8 +912,15 @@ __device__ __forceinline__ void dp_half(const DpTask& t, const int* mat, int32_t
   const bool rev = t.reverse != 0;
   const int32_t oe = go + ge;
   const int32_t xdrop = t.xdrop > oe ? t.xdrop : oe;
+  // With rings in LDS (RS > 1) the b code of each band column rides in a byte
+  // ring beside its cell, read from global memory once when the column joins
+  // the band; the row loop then reads only LDS, one cell ahead.
+  constexpr bool BR = RS > 1;
+  const uint8_t* bp = t.b;
+  auto bcode = [&](uint32_t col) -> uint32_t { return col < n ? (bp[rev ? n - 1 - col : col] & 15u) : 0u; };
   uint32_t flat_len = 0;
   ring[0] = make_int2(0, -oe);
+  if (BR) bring[0] = (uint8_t)bcode(0);
   if (TB) { rows[0] = 0; rows[1] = 0; flat[flat_len] = SCRIPT_GAP_IN_B; }
   ++flat_len;
   int32_t score = -oe;
@@ -921,6 +929,7 @@ __device__ __forceinline__ void dp_half(const DpTask& t, const int* mat, int32_t
     if (score < -xdrop) break;
     if (j > wmask) { res.flags |= E_BAND_WIDE; return; }
     ring[(j & wmask) * RS] = make_int2(score, score - oe);
+    if (BR) bring[(j & wmask) * RS] = (uint8_t)bcode(j);
     if (TB) flat[flat_len] = SCRIPT_GAP_IN_A;
     ++flat_len;
     score -= ge;
@@ -942,12 +951,18 @@ __device__ __forceinline__ void dp_half(const DpTask& t, const int* mat, int32_t
     uint32_t last_b = first_b;
     bool best_here = false;
     cells += b_size - first_b;
-    const uint8_t* bp = t.b;
+    int2 cn = ring[(first_b & wmask) * RS];
+    uint32_t bn = BR ? (uint32_t)bring[(first_b & wmask) * RS] : bcode(first_b);
     for (uint32_t bi = first_b; bi < b_size; ++bi) {
-      int2 c = ring[(bi & wmask) * RS];
-      const uint32_t bb = bi < n ? (bp[rev ? n - 1 - bi : bi] & 15) : 0;
+      int2 c = cn;
+      const int32_t sub = row[bn];
+      {  // the next cell's inputs, loaded while this one is computed (no write of this row touches its slot)
+        const uint32_t ns = ((bi + 1) & wmask) * RS;
+        cn = ring[ns];
+        bn = BR ? (uint32_t)bring[ns] : bcode(bi + 1);
+      }
       int32_t sgc = c.y;
-      const int32_t next = c.x + row[bb];
+      const int32_t next = c.x + sub;
       uint8_t op = SCRIPT_SUB;
       int32_t cs = sc;
       if (sgc > cs) { cs = sgc; op = SCRIPT_GAP_IN_B; }
@@ -982,6 +997,7 @@ __device__ __forceinline__ void dp_half(const DpTask& t, const int* mat, int32_t
       while (sgr >= best - xdrop && b_size <= n) {
         if (b_size - first_b > wmask) { res.flags |= E_BAND_WIDE; return; }
         ring[(b_size & wmask) * RS] = make_int2(sgr, sgr - oe);
+        if (BR) bring[(b_size & wmask) * RS] = (uint8_t)bcode(b_size);
         if (TB) flat[flat_len] = SCRIPT_GAP_IN_A;
         ++flat_len;
         sgr -= ge;
@@ -991,6 +1007,7 @@ __device__ __forceinline__ void dp_half(const DpTask& t, const int* mat, int32_t
     if (b_size <= n) {
       if (b_size - first_b > wmask) { res.flags |= E_BAND_WIDE; return; }
       ring[(b_size & wmask) * RS] = make_int2(kMinInt, kMinInt);
+      if (BR) bring[(b_size & wmask) * RS] = (uint8_t)bcode(b_size);
       ++b_size;
     }
     if (best_here) { bytes_at_best = flat_len; rows_at_best = ai + 2; }
@@ -1062,16 +1079,18 @@ __global__ __launch_bounds__(LDS_RING ? kDpLdsThreads : 256) void k_dp_score(
     int2* __restrict__ ring, uint32_t wlog2, DpResult* __restrict__ out, unsigned long long* counters) {
   __shared__ int mat[256];
   __shared__ int2 lring[LDS_RING ? (kDpLdsThreads << kDpLdsLog2) : 1];
+  __shared__ uint8_t lbring[LDS_RING ? (kDpLdsThreads << kDpLdsLog2) : 1];
   for (int i = threadIdx.x; i < 256; i += blockDim.x) mat[i] = gmat[i];
   __syncthreads();
   const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
   constexpr uint32_t RS = LDS_RING ? kDpLdsThreads : 1;
   int2* my = LDS_RING ? lring + threadIdx.x : ring + (size_t)gt * ring_stride(wlog2);
+  uint8_t* myb = LDS_RING ? lbring + threadIdx.x : nullptr;
   const uint32_t wmask = LDS_RING ? (1u << kDpLdsLog2) - 1 : (1u << wlog2) - 1;
   unsigned long long cells = 0;
   for (uint32_t i = gt; i < n_tasks; i += nt) {
     DpResult r;
-    dp_half<false, RS>(tasks[i], mat, go, ge, my, wmask, r, nullptr, nullptr, 0);
+    dp_half<false, RS>(tasks[i], mat, go, ge, my, myb, wmask, r, nullptr, nullptr, 0);
     out[i] = r;
     cells += r.cells;
   }
@@ -1085,11 +1104,13 @@ __global__ __launch_bounds__(LDS_RING ? kDpLdsThreads : 256) void k_dp_traceback
     uint32_t* __restrict__ ops_pool, TbResult* __restrict__ out) {
   __shared__ int mat[256];
   __shared__ int2 lring[LDS_RING ? (kDpLdsThreads << kDpLdsLog2) : 1];
+  __shared__ uint8_t lbring[LDS_RING ? (kDpLdsThreads << kDpLdsLog2) : 1];
   for (int i = threadIdx.x; i < 256; i += blockDim.x) mat[i] = gmat[i];
   __syncthreads();
   const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
   constexpr uint32_t RS = LDS_RING ? kDpLdsThreads : 1;
   int2* my = LDS_RING ? lring + threadIdx.x : ring + (size_t)gt * ring_stride(wlog2);
+  uint8_t* myb = LDS_RING ? lbring + threadIdx.x : nullptr;
   const uint32_t wmask = LDS_RING ? (1u << kDpLdsLog2) - 1 : (1u << wlog2) - 1;
   for (uint32_t i = gt; i < n_tasks; i += nt) {
     const TbTask T = tasks[i];
@@ -1097,7 +1118,7 @@ __global__ __launch_bounds__(LDS_RING ? kDpLdsThreads : 256) void k_dp_traceback
     uint32_t* rows = row_pool + 2 * T.row_off;
     uint32_t* ops = ops_pool + T.ops_off;
     DpResult r;
-    dp_half<true, RS>(T.t, mat, go, ge, my, wmask, r, flat, rows, T.stop_row);
+    dp_half<true, RS>(T.t, mat, go, ge, my, myb, wmask, r, flat, rows, T.stop_row);
     TbResult o{0, r.flags};
     if (!r.flags) {
       const uint32_t nops = tb_walk(flat, rows, T.stop_row, T.best_b, r.tb_bytes, ops, T.ops_cap, o.flags);
@@ -1124,11 +1145,13 @@ __global__ __launch_bounds__(LDS_RING ? kDpLdsThreads : 256) void k_dp_onepass(
     unsigned long long ops_pool_cap, OnePassResult* __restrict__ out, unsigned long long* counters) {
   __shared__ int mat[256];
   __shared__ int2 lring[LDS_RING ? (kDpLdsThreads << kDpLdsLog2) : 1];
+  __shared__ uint8_t lbring[LDS_RING ? (kDpLdsThreads << kDpLdsLog2) : 1];
   for (int i = threadIdx.x; i < 256; i += blockDim.x) mat[i] = gmat[i];
   __syncthreads();
   const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
   constexpr uint32_t RS = LDS_RING ? kDpLdsThreads : 1;
   int2* my = LDS_RING ? lring + threadIdx.x : ring + (size_t)gt * ring_stride(wlog2);
+  uint8_t* myb = LDS_RING ? lbring + threadIdx.x : nullptr;
   const uint32_t wmask = LDS_RING ? (1u << kDpLdsLog2) - 1 : (1u << wlog2) - 1;
   uint8_t* base = arena + (size_t)gt * onepass_stride(flat_cap, rows_cap, ops_cap);
   uint32_t* rows = reinterpret_cast<uint32_t*>(base);
@@ -1137,7 +1160,7 @@ __global__ __launch_bounds__(LDS_RING ? kDpLdsThreads : 256) void k_dp_onepass(
   unsigned long long cells = 0;
   for (uint32_t i = gt; i < n_tasks; i += nt) {
     DpResult r;
-    dp_half<true, RS>(tasks[i], mat, go, ge, my, wmask, r, flat, rows, 0xFFFFFFFFu, flat_cap, rows_cap);
+    dp_half<true, RS>(tasks[i], mat, go, ge, my, myb, wmask, r, flat, rows, 0xFFFFFFFFu, flat_cap, rows_cap);
     OnePassResult o{r.score, r.a_len, r.b_len, 0, r.flags, 0};
     if (!r.flags) {
       cells += r.cells;
@@ -1222,11 +1245,13 @@ __global__ __launch_bounds__(kPrelimThreads) void k_prelim_fused(
     unsigned long long* __restrict__ counters, uint32_t* __restrict__ err) {
   __shared__ int mat[256];
   __shared__ int2 lring[LDS_RING ? (kPrelimThreads << kPrelimRingLog2) : 1];
+  __shared__ uint8_t lbring[LDS_RING ? (kPrelimThreads << kPrelimRingLog2) : 1];
   for (int i = threadIdx.x; i < 256; i += blockDim.x) mat[i] = gmat[i];
   __syncthreads();
   const uint64_t gt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (uint64_t)gridDim.x * blockDim.x;
   constexpr uint32_t RS = LDS_RING ? kPrelimThreads : 1;
   int2* my = LDS_RING ? lring + threadIdx.x : gring + gt * ring_stride(wlog2);
+  uint8_t* myb = LDS_RING ? lbring + threadIdx.x : nullptr;
   const uint32_t wmask = LDS_RING ? (1u << kPrelimRingLog2) - 1 : (1u << wlog2) - 1;
   unsigned long long cells = 0, dps = 0;
   if (P.n_dev) n_tasks = *P.n_dev < n_tasks ? *P.n_dev : n_tasks;
@@ -1246,8 +1271,8 @@ __global__ __launch_bounds__(kPrelimThreads) void k_prelim_fused(
     DpTask left{sa, qa, ps, pq, xd, 1};
     DpTask right{sa + ps, qa + pq, sl > ps ? sl - ps : 0, ql > pq ? ql - pq : 0, xd, 0};
     DpResult L, Rr;
-    dp_half<false, RS>(left, mat, P.gap_open, P.gap_extend, my, wmask, L, nullptr, nullptr, 0);
-    dp_half<false, RS>(right, mat, P.gap_open, P.gap_extend, my, wmask, Rr, nullptr, nullptr, 0);
+    dp_half<false, RS>(left, mat, P.gap_open, P.gap_extend, my, myb, wmask, L, nullptr, nullptr, 0);
+    dp_half<false, RS>(right, mat, P.gap_open, P.gap_extend, my, myb, wmask, Rr, nullptr, nullptr, 0);
     if ((L.flags | Rr.flags) & E_BAND_WIDE) {
       const unsigned long long kk = atomicAdd(redo_count, 1ull);
       if (kk < P.redo_cap) {
