@@ -1,6 +1,7 @@
 // Shared device/host record layouts of librmblast_hip.
 #pragma once
 #include <cstdint>
+#include <hip/hip_runtime.h>
 
 namespace rmb {
 
@@ -67,6 +68,7 @@ struct PrelimParams {
   uint64_t n_hits, out_cap, redo_cap;
   const unsigned long long* n_dev;  // when set, the task count is min(*n_dev, n_hits)
   const uint32_t* rec_ids;          // group-local record -> search record, for output item ids
+  unsigned long long* next;         // work counter (zero at launch): waves take hits from it in runs
 };
 
 // Error bits written by kernels.
@@ -115,6 +117,17 @@ struct OnePassResult {  // a half alignment scored and traced back in one pass
   uint32_t flags;       // ErrBit: E_BAND_WIDE / E_TB_SPACE send the half to the two-pass path
   uint32_t pad;
   uint64_t ops_at;      // its ops in the packed pool
+};
+
+// Retry storage of the one-pass traceback: a half whose band outgrows the LDS
+// ring, or whose traceback outgrows the lane's arena, is run again by its lane
+// with one of `slots` global rings of 2^ring_log2 cells and a larger arena,
+// slots handed out by *next (zero at launch).
+struct OnePassBig {
+  int2* ring;
+  uint8_t* arena;
+  unsigned int* next;
+  uint32_t ring_log2, slots, flat_cap, rows_cap, ops_cap;
 };
 
 // Per-lane arena of the one-pass traceback: row table (8 B per row), then the

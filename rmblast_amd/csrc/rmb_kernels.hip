@@ -888,6 +888,15 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
   }
 }
 
+// Work queue shared by a launch: lane 0 of the wave takes `n` items from
+// *next and every lane gets the first one's index.
+__device__ __forceinline__ uint64_t wave_take(unsigned long long* next, uint64_t n) {
+  unsigned long long base = 0;
+  if ((threadIdx.x & 63u) == 0) base = atomicAdd(next, (unsigned long long)n);
+  const uint32_t lo = __shfl((uint32_t)base, 0, 64), hi = __shfl((uint32_t)(base >> 32), 0, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // --------------------------------------------------------- X-drop DP ---------
 // One lane runs one half alignment in the reference's row-major order, so every
 // prune decision and band bound is the reference's.  The band [first_b, b_size)
@@ -1142,7 +1151,8 @@ __global__ __launch_bounds__(LDS_RING ? kDpLdsThreads : 256) void k_dp_onepass(
     const DpTask* __restrict__ tasks, uint32_t n_tasks, const int32_t* __restrict__ gmat, int32_t go, int32_t ge,
     int2* __restrict__ ring, uint32_t wlog2, uint8_t* __restrict__ arena, uint32_t flat_cap, uint32_t rows_cap,
     uint32_t ops_cap, uint32_t* __restrict__ ops_pool, unsigned long long* __restrict__ ops_count,
-    unsigned long long ops_pool_cap, OnePassResult* __restrict__ out, unsigned long long* counters) {
+    unsigned long long ops_pool_cap, OnePassResult* __restrict__ out, unsigned long long* next, OnePassBig big,
+    unsigned long long* counters) {
   __shared__ int mat[256];
   __shared__ int2 lring[LDS_RING ? (kDpLdsThreads << kDpLdsLog2) : 1];
   __shared__ uint8_t lbring[LDS_RING ? (kDpLdsThreads << kDpLdsLog2) : 1];
@@ -1158,18 +1168,39 @@ __global__ __launch_bounds__(LDS_RING ? kDpLdsThreads : 256) void k_dp_onepass(
   uint8_t* flat = base + (size_t)rows_cap * 8;
   uint32_t* ops = reinterpret_cast<uint32_t*>(flat + ((flat_cap + 15u) & ~15u));
   unsigned long long cells = 0;
-  for (uint32_t i = gt; i < n_tasks; i += nt) {
+  // tasks come longest first; each wave takes the next 64 from the queue, so
+  // workgroups that start late (behind another stream's kernel) take less
+  (void)nt;
+  for (;;) {
+    const uint64_t base = wave_take(next, 64);
+    if (base >= n_tasks) break;
+    const uint32_t i = (uint32_t)base + (threadIdx.x & 63u);
+    if (i >= n_tasks) continue;
     DpResult r;
     dp_half<true, RS>(tasks[i], mat, go, ge, my, myb, wmask, r, flat, rows, 0xFFFFFFFFu, flat_cap, rows_cap);
+    uint8_t* tflat = flat;
+    uint32_t *trows = rows, *tops = ops, tops_cap = ops_cap;
+    if (r.flags) {  // band or traceback beyond the lane's storage: once more with a retry slot
+      const uint32_t slot = atomicAdd(big.next, 1u);
+      if (slot < big.slots) {
+        uint8_t* bb = big.arena + (size_t)slot * onepass_stride(big.flat_cap, big.rows_cap, big.ops_cap);
+        trows = reinterpret_cast<uint32_t*>(bb);
+        tflat = bb + (size_t)big.rows_cap * 8;
+        tops = reinterpret_cast<uint32_t*>(tflat + ((big.flat_cap + 15u) & ~15u));
+        tops_cap = big.ops_cap;
+        dp_half<true, 1>(tasks[i], mat, go, ge, big.ring + (size_t)slot * ring_stride(big.ring_log2), nullptr,
+                         (1u << big.ring_log2) - 1, r, tflat, trows, 0xFFFFFFFFu, big.flat_cap, big.rows_cap);
+      }
+    }
     OnePassResult o{r.score, r.a_len, r.b_len, 0, r.flags, 0};
     if (!r.flags) {
       cells += r.cells;
       if (r.a_len | r.b_len) {
-        const uint32_t n = tb_walk(flat, rows, r.a_len, r.b_len, r.tb_bytes, ops, ops_cap, o.flags);
+        const uint32_t n = tb_walk(tflat, trows, r.a_len, r.b_len, r.tb_bytes, tops, tops_cap, o.flags);
         if (!o.flags) {
           const unsigned long long at = atomicAdd(ops_count, (unsigned long long)n);
           if (at + n <= ops_pool_cap) {
-            for (uint32_t k = 0; k < n; ++k) ops_pool[at + k] = ops[n - 1 - k];
+            for (uint32_t k = 0; k < n; ++k) ops_pool[at + k] = tops[n - 1 - k];
             o.n_ops = n;
             o.ops_at = at;
           } else {
@@ -1233,7 +1264,7 @@ __device__ __forceinline__ void prelim_pivot(const UngHit& u, uint32_t qa_len, u
 // hands the hits wider still back as records.  Output item ids name search
 // records (P.rec_ids); the task count may come from device memory (P.n_dev),
 // so the passes queue behind the seeding kernel without a host round trip.
-static constexpr uint32_t kPrelimThreads = 128, kPrelimRingLog2 = 4;
+static constexpr uint32_t kPrelimThreads = 128, kPrelimRingLog2 = 4, kPrelimRun = 4;
 
 template <bool LDS_RING>
 __global__ __launch_bounds__(kPrelimThreads) void k_prelim_fused(
@@ -1255,7 +1286,15 @@ __global__ __launch_bounds__(kPrelimThreads) void k_prelim_fused(
   const uint32_t wmask = LDS_RING ? (1u << kPrelimRingLog2) - 1 : (1u << wlog2) - 1;
   unsigned long long cells = 0, dps = 0;
   if (P.n_dev) n_tasks = *P.n_dev < n_tasks ? *P.n_dev : n_tasks;
-  for (uint64_t k = gt; k < n_tasks; k += nt) {
+  // waves take runs of kPrelimRun x 64 hits from the launch's queue, so
+  // workgroups that start late (behind another stream's kernels) take less
+  (void)nt;
+  for (;;) {
+   const uint64_t base = wave_take(P.next, 64 * kPrelimRun);
+   if (base >= n_tasks) break;
+   for (uint32_t run = 0; run < kPrelimRun; ++run) {
+    const uint64_t k = base + 64 * run + (threadIdx.x & 63u);
+    if (k >= n_tasks) break;
     const uint32_t i = idx ? idx[k] : (uint32_t)k;
     const UngHit u = hits[i];
     if (u.item == kNoItem) continue;  // reserved slot the seeding kernel left unused
@@ -1301,6 +1340,7 @@ __global__ __launch_bounds__(kPrelimThreads) void k_prelim_fused(
     h.ts = ts;
     const unsigned long long kk = atomicAdd(out_count, 1ull);
     if (kk < P.out_cap) out[kk] = h;
+   }
   }
   atomicAdd(&counters[C_PRELIM_CELLS], cells);
   atomicAdd(&counters[C_PRELIM_DPS], dps);
@@ -1368,9 +1408,11 @@ hipError_t launch_dp_score(int blocks, const DpTask* tasks, uint32_t n, const in
 hipError_t launch_dp_onepass(int blocks, const DpTask* tasks, uint32_t n, const int32_t* gmat, int32_t go,
                              int32_t ge, uint8_t* arena, uint32_t flat_cap, uint32_t rows_cap, uint32_t ops_cap,
                              uint32_t* ops_pool, unsigned long long* ops_count, unsigned long long ops_pool_cap,
-                             OnePassResult* out, unsigned long long* counters, hipStream_t st) {
+                             OnePassResult* out, unsigned long long* next, OnePassBig big,
+                             unsigned long long* counters, hipStream_t st) {
   hipLaunchKernelGGL(k_dp_onepass<true>, dim3(blocks), dim3(kDpLdsThreads), 0, st, tasks, n, gmat, go, ge, nullptr,
-                     0, arena, flat_cap, rows_cap, ops_cap, ops_pool, ops_count, ops_pool_cap, out, counters);
+                     0, arena, flat_cap, rows_cap, ops_cap, ops_pool, ops_count, ops_pool_cap, out, next, big,
+                     counters);
   return hipGetLastError();
 }
 

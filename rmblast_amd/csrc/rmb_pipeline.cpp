@@ -53,7 +53,7 @@ hipError_t launch_dp_traceback(int, const TbTask*, uint32_t, const int32_t*, int
                                uint8_t*, uint32_t*, uint32_t*, TbResult*, hipStream_t);
 hipError_t launch_dp_onepass(int, const DpTask*, uint32_t, const int32_t*, int32_t, int32_t, uint8_t*, uint32_t,
                              uint32_t, uint32_t, uint32_t*, unsigned long long*, unsigned long long, OnePassResult*,
-                             unsigned long long*, hipStream_t);
+                             unsigned long long*, OnePassBig, unsigned long long*, hipStream_t);
 hipError_t launch_prelim_fused(int, bool, PrelimParams, const UngHit*, const uint32_t*, uint64_t, const RecDesc*,
                                const SubjDesc*, const uint8_t*, const uint8_t*, const uint8_t*, const int32_t*, int2*,
                                uint32_t, PreHit*, unsigned long long*, uint32_t*, UngHit*, unsigned long long*,
@@ -231,7 +231,7 @@ struct rmb_ctx {
   DevBuf b_hits, b_hitcount, b_spill, b_hitres, b_queue, b_counters, b_err;
   DevBuf b_tasks, b_dpres, b_ring, b_ring_big, b_tbtasks, b_tbres, b_flat, b_rows, b_ops;
   DevBuf b_pre, b_redo, b_precount, b_packtasks, b_packed, b_items, b_qext, b_recids;
-  DevBuf b_arena, b_oppool, b_opres, b_opcount;  // one-pass traceback
+  DevBuf b_arena, b_arena_big, b_oppool, b_opres, b_opcount;  // one-pass traceback
 };
 
 namespace rmb {
@@ -480,6 +480,9 @@ static void run_dp_traceback_2pass(rmb_ctx* c, const std::vector<DpTask>& tasks,
 // take the two-pass path.
 constexpr uint32_t kOnePassRows = 8192, kOnePassFlat = 256u << 10, kOnePassOps = 4096;
 constexpr unsigned long long kOnePassPool = 64ull << 20;  // packed ops per launch
+// retry slots for halves beyond the LDS ring or the lane arena: global rings of
+// 2^kRingBigLog2 cells and 96 MB arenas
+constexpr uint32_t kBigSlots = 32, kBigRows = 1u << 18, kBigFlat = 64u << 20, kBigOps = 1u << 20;
 
 // Traceback DPs: one pass (score, traceback bytes, walk) per half in a
 // per-lane arena, longest first; the halves it flags run in two passes.
@@ -508,13 +511,24 @@ static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::
     const uint64_t stride = onepass_stride(kOnePassFlat, kOnePassRows, kOnePassOps);
     uint8_t* arena = c->b_arena.get<uint8_t>((size_t)blocks * 64 * stride);
     uint32_t* pool = c->b_oppool.get<uint32_t>(kOnePassPool);
-    unsigned long long* pcount = c->b_opcount.get<unsigned long long>(1);
-    hip_check(hipMemsetAsync(pcount, 0, sizeof(unsigned long long), c->st), "clear op count");
+    // [0] ops used, [1] work queue, [2] retry slots taken (low 32 bits)
+    unsigned long long* pcount = c->b_opcount.get<unsigned long long>(3);
+    hip_check(hipMemsetAsync(pcount, 0, 3 * sizeof(unsigned long long), c->st), "clear op count");
+    OnePassBig big{};
+    big.ring = c->b_ring_big.get<int2>((size_t)kBigSlots * ring_stride(kRingBigLog2));
+    big.arena = c->b_arena_big.get<uint8_t>((size_t)kBigSlots * onepass_stride(kBigFlat, kBigRows, kBigOps));
+    big.next = reinterpret_cast<unsigned int*>(pcount + 2);
+    big.ring_log2 = kRingBigLog2;
+    big.slots = kBigSlots;
+    big.flat_cap = kBigFlat;
+    big.rows_cap = kBigRows;
+    big.ops_cap = kBigOps;
     hip_check(hipMemcpyAsync(dt, sorted.data() + base, n * sizeof(DpTask), hipMemcpyHostToDevice, c->st),
               "copy tasks");
     timed_launch(c, 14, [&] {
       hip_check(launch_dp_onepass((int)blocks, dt, n, c->d_mat, c->P.gap_open, c->P.gap_extend, arena, kOnePassFlat,
-                                  kOnePassRows, kOnePassOps, pool, pcount, kOnePassPool, dr, ctr, c->st),
+                                  kOnePassRows, kOnePassOps, pool, pcount, kOnePassPool, dr, pcount + 1, big, ctr,
+                                  c->st),
                 "k_dp_onepass");
     });
     std::vector<OnePassResult> r(n);
@@ -651,7 +665,7 @@ static void launch_group_phase1(rmb_ctx* c, hipStream_t st, P1Group& g, const ui
     P.item_list = dl;
     P.n_items = g.items.size();
   }
-  // ~35 KB of LDS per workgroup: four resident per CU.  Workgroups are dealt
+  // ~40 KB of LDS per workgroup: four resident per CU.  Workgroups are dealt
   // round-robin over the XCDs, so blockIdx % kXcds names the XCD.
   const int blocks = std::max(1, std::min<int>(c->cu_count * 4, (int)std::min<uint64_t>(P.n_items, 1u << 20)));
   uint2* spill = c->b_spill.get<uint2>((size_t)blocks * kBuckets * kHashGlobal);
@@ -697,11 +711,13 @@ static void enqueue_batch(rmb_ctx* c, P1Batch& b, const uint8_t* d_qfwd, const u
   if (max_cap > 0xFFFFFFFFull) throw Failure(RMB_ERR_CAPACITY, "more than 2^32 ungapped hits in one group");
   PreHit* dpre = S.pre.get<PreHit>(pre_cap);
   UngHit* dredoh = S.redo_hits.get<UngHit>(b.redo_cap);
-  unsigned long long* cnt = S.cnt.get<unsigned long long>(4);           // [0] prelims, [2] wide hits
+  // [0] prelims, [2] wide hits, [4 + 2g] and [5 + 2g] group g's two
+  // preliminary passes' work queues
+  unsigned long long* cnt = S.cnt.get<unsigned long long>(4 + 2 * ng);
   unsigned long long* hitcnt = S.hitcnt.get<unsigned long long>(2 * ng + 2);  // per group: hits, wide (pass 1)
   unsigned long long* ctr = S.ctr.get<unsigned long long>(C_NCOUNTERS);
   uint32_t* err = S.err.get<uint32_t>(1);
-  hip_check(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned long long), st), "memset counts");
+  hip_check(hipMemsetAsync(cnt, 0, (4 + 2 * ng) * sizeof(unsigned long long), st), "memset counts");
   hip_check(hipMemsetAsync(hitcnt, 0, (2 * ng + 2) * sizeof(unsigned long long), st), "memset hit counts");
   hip_check(hipMemsetAsync(ctr, 0, C_NCOUNTERS * sizeof(unsigned long long), st), "memset counters");
   hip_check(hipMemsetAsync(err, 0, sizeof(uint32_t), st), "memset err");
@@ -732,6 +748,7 @@ static void enqueue_batch(rmb_ctx* c, P1Batch& b, const uint8_t* d_qfwd, const u
     PP.n_hits = g.hits_cap;
     PP.n_dev = hitcnt + 2 * gi;
     PP.redo_cap = g.hits_cap;
+    PP.next = cnt + 4 + 2 * gi;
     hip_check(hipEventRecord(S.ev[6 * gi + 2], st), "event");
     hip_check(launch_prelim_fused(c->cu_count * 8, true, PP, dh, nullptr, g.hits_cap, drecs,
                                   (const SubjDesc*)c->d_sdesc.p, d_qfwd, d_qrc, (const uint8_t*)c->d_splus.p,
@@ -744,6 +761,7 @@ static void enqueue_batch(rmb_ctx* c, P1Batch& b, const uint8_t* d_qfwd, const u
     int2* ring = c->b_ring.get<int2>((size_t)lanes * ring_stride(kRingLog2));
     PP.n_dev = hitcnt + 2 * gi + 1;
     PP.redo_cap = b.redo_cap;
+    PP.next = cnt + 5 + 2 * gi;
     hip_check(hipEventRecord(S.ev[6 * gi + 4], st), "event");
     hip_check(launch_prelim_fused((int)(lanes / 128), false, PP, dh, dredo, g.hits_cap, drecs,
                                   (const SubjDesc*)c->d_sdesc.p, d_qfwd, d_qrc, (const uint8_t*)c->d_splus.p,
@@ -1318,16 +1336,20 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
   }
   uint64_t p1_ctr[C_NCOUNTERS] = {0};
   uint64_t t_seed = 0, t_2a = 0, t_2b = 0, tb_ms[4] = {0, 0, 0, 0};
-  if (!batches.empty()) {
+  // Two batches in flight on the seeding stream: a batch is queued as soon as
+  // the batch before it in its slot has been read back, so the stream never
+  // waits for the host's Phase 2a/2b work.
+  {
     Timer te;
-    enqueue_batch(c, batches[0], d_qfwd, d_qrc, d_qext);
+    for (size_t b = 0; b < std::min<size_t>(2, batches.size()); ++b)
+      enqueue_batch(c, batches[b], d_qfwd, d_qrc, d_qext);
     t_seed += te.us();
   }
   for (size_t b = 0; b < batches.size(); ++b) {
     Timer t1;
-    if (b + 1 < batches.size()) enqueue_batch(c, batches[b + 1], d_qfwd, d_qrc, d_qext);
     std::vector<PreHit> survivors;
     complete_batch(c, batches[b], recs, d_qfwd, d_qrc, d_qext, survivors, p1_ctr);
+    if (b + 2 < batches.size()) enqueue_batch(c, batches[b + 2], d_qfwd, d_qrc, d_qext);
     t_seed += t1.us();
     Timer t2;
     std::vector<ItemState> items = phase2a(c, sr, survivors);
@@ -1527,7 +1549,7 @@ int rmb_ctx_destroy(rmb_ctx* c) {
                     &c->b_hits, &c->b_hitcount, &c->b_spill, &c->b_hitres, &c->b_queue, &c->b_counters, &c->b_err, &c->b_tasks,
                     &c->b_dpres, &c->b_ring, &c->b_ring_big, &c->b_tbtasks, &c->b_tbres, &c->b_flat, &c->b_rows,
                     &c->b_ops, &c->b_pre, &c->b_redo, &c->b_precount, &c->b_packtasks, &c->b_packed, &c->b_items, &c->b_qext,
-                    &c->b_recids, &c->b_arena, &c->b_oppool, &c->b_opres, &c->b_opcount})
+                    &c->b_recids, &c->b_arena, &c->b_arena_big, &c->b_oppool, &c->b_opres, &c->b_opcount})
     b->release();
   for (P1Slot& S : c->slot) {
     for (DevBuf* b : {&S.pre, &S.redo_hits, &S.cnt, &S.hitcnt, &S.ctr, &S.err}) b->release();
