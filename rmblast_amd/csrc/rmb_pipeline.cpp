@@ -92,6 +92,37 @@ struct DevBuf {
   }
 };
 
+// Pinned host staging buffer, grown on demand (the query pools: uploads run as
+// DMA on a stream instead of through the runtime's pageable-copy path).
+struct HostBuf {
+  uint8_t* p = nullptr;
+  size_t cap = 0;
+  uint8_t* get(size_t bytes) {
+    if (bytes > cap) {
+      if (p) hip_check(hipHostFree(p), "hipHostFree");
+      p = nullptr;
+      hip_check(hipHostMalloc((void**)&p, std::max<size_t>(bytes, 16), hipHostMallocDefault), "hipHostMalloc");
+      cap = std::max<size_t>(bytes, 16);
+    }
+    return p;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+// A sentinel-wrapped query (code 15, the bases, code 15) inside a pool.
+struct SeqView {
+  const uint8_t* p = nullptr;
+  size_t n = 0;
+  const uint8_t* data() const { return p; }
+  size_t size() const { return n; }
+  const uint8_t* begin() const { return p; }
+  const uint8_t* end() const { return p + n; }
+};
+
 // Host worker threads: RMB_THREADS if set, else the hardware threads, at most 16
 // (one GPU's share of the host on a multi-GPU node).
 static unsigned host_threads() {
@@ -227,6 +258,7 @@ struct rmb_ctx {
   uint64_t host_prelim_dps = 0;  // preliminary DPs of the last search run through run_dp_score
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // scratch
+  HostBuf h_qfwd, h_qrc;  // host query pools of the last search
   DevBuf b_qfwd, b_qrc, b_recs, b_elem, b_keys, b_keys2, b_vals, b_vals2, b_counts, b_cells, b_sort_tmp;
   DevBuf b_hits, b_hitcount, b_spill, b_hitres, b_queue, b_counters, b_err;
   DevBuf b_tasks, b_dpres, b_ring, b_ring_big, b_tbtasks, b_tbres, b_flat, b_rows, b_ops;
@@ -957,7 +989,7 @@ struct SearchRecs {
 // to 14), uploaded to the context's masked pool.  A split query is masked
 // whole, with one more base after each interval (NCBI's chunked-interval
 // off-by-one, dust.rs:603-631), and its chunks are cut from that mask.
-static const uint8_t* upload_dusted(rmb_ctx* c, const std::vector<std::vector<uint8_t>>& qf,
+static const uint8_t* upload_dusted(rmb_ctx* c, const std::vector<SeqView>& qf,
                                     const SearchRecs& sr) {
   std::vector<uint8_t> pm(sr.pool, 15);
   parallel_for(qf.size(), [&](size_t i) {
@@ -1065,7 +1097,7 @@ static std::vector<ItemState> phase2a(rmb_ctx* c, const SearchRecs& sr, std::vec
 
 // Phase 2b of one batch: traceback DPs on the GPU, then the per-subject
 // replay on the host (engine.rs:1438-1762).
-static void phase2b(rmb_ctx* c, const SearchRecs& sr, const std::vector<std::vector<uint8_t>>& qf,
+static void phase2b(rmb_ctx* c, const SearchRecs& sr, const std::vector<SeqView>& qf,
                     const uint32_t* lens, std::vector<ItemState>& items, const uint8_t* d_qfwd, const uint8_t* d_qrc,
                     uint64_t tb_ms[4]) {
   const uint32_t nsub = (uint32_t)c->subj.size();
@@ -1242,14 +1274,13 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
   // ---- query pools.  A query of 1,999,800 bases or more is searched
   // through Phase 2a as NCBI's overlapping chunks, each its own sentinel-
   // wrapped record; Phase 2b aligns on the whole query, stored after them.
-  std::vector<std::vector<uint8_t>> qf(nq);
+  std::vector<SeqView> qf(nq);  // the whole queries, wrapped, in the host forward pool
   SearchRecs sr;
   sr.first.assign(nq + 1, 0);
   sr.whole_base.assign(nq, 0);
   std::vector<QuerySplit> split(nq);
   for (uint32_t i = 0; i < nq; ++i) {
     if (lens[i] > 0x7FFFFFFFu) throw Failure(RMB_ERR_ARG, "query longer than 2^31-1 bases");
-    qf[i] = wrap(qs[i], lens[i]);
     split[i] = query_split(lens[i]);
     sr.first[i] = (uint32_t)sr.recs.size();
     if (split[i].count <= 1) {
@@ -1279,22 +1310,26 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
   uint8_t* d_qfwd = c->b_qfwd.get<uint8_t>(sr.pool);
   uint8_t* d_qrc = c->b_qrc.get<uint8_t>(sr.pool);
   {
-    std::vector<uint8_t> pf(sr.pool), pr(sr.pool);
-    auto put = [&](uint64_t base, const uint8_t* wrapped, size_t n) {  // n bases between two sentinels
-      uint8_t* f = pf.data() + base;
-      uint8_t* r = pr.data() + base;
+    // pinned pools filled in parallel from the caller's buffers, uploaded on
+    // the seeding stream (its first kernels queue behind the copies)
+    uint8_t* pf = c->h_qfwd.get(sr.pool);
+    uint8_t* pr = c->h_qrc.get(sr.pool);
+    auto put = [&](uint64_t base, const uint8_t* bases, size_t n) {  // n bases between two sentinels
+      uint8_t* f = pf + base;
+      uint8_t* r = pr + base;
       f[0] = r[0] = f[n + 1] = r[n + 1] = 15;
-      std::memcpy(f + 1, wrapped + 1, n);
-      for (size_t j = 0; j < n; ++j) r[1 + j] = kBlastnaComplement[wrapped[n - j] & 15];
+      for (size_t j = 0; j < n; ++j) f[1 + j] = bases[j] & 15;
+      for (size_t j = 0; j < n; ++j) r[1 + j] = kBlastnaComplement[bases[n - 1 - j] & 15];
     };
     parallel_for(nq, [&](size_t i) {
-      put(sr.whole_base[i], qf[i].data(), lens[i]);
+      put(sr.whole_base[i], qs[i], lens[i]);
       if (sr.first[i + 1] - sr.first[i] > 1)
         for (uint32_t k = sr.first[i]; k < sr.first[i + 1]; ++k)
-          put(recs[k].q_base, qf[i].data() + sr.offset[k], recs[k].q_len);
+          put(recs[k].q_base, qs[i] + sr.offset[k], recs[k].q_len);
     });
-    hip_check(hipMemcpy(d_qfwd, pf.data(), sr.pool, hipMemcpyHostToDevice), "copy qfwd");
-    hip_check(hipMemcpy(d_qrc, pr.data(), sr.pool, hipMemcpyHostToDevice), "copy qrc");
+    for (uint32_t i = 0; i < nq; ++i) qf[i] = SeqView{pf + sr.whole_base[i], (size_t)lens[i] + 2};
+    hip_check(hipMemcpyAsync(d_qfwd, pf, sr.pool, hipMemcpyHostToDevice, c->st1), "copy qfwd");
+    hip_check(hipMemcpyAsync(d_qrc, pr, sr.pool, hipMemcpyHostToDevice, c->st1), "copy qrc");
   }
   // With DUST on, the lookup tables and the exact word extension read the
   // masked query; ungapped and gapped extension read the query itself
@@ -1336,6 +1371,7 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
   }
   uint64_t p1_ctr[C_NCOUNTERS] = {0};
   uint64_t t_seed = 0, t_2a = 0, t_2b = 0, tb_ms[4] = {0, 0, 0, 0};
+  const uint64_t t_setup = total.us();
   // Two batches in flight on the seeding stream: a batch is queued as soon as
   // the batch before it in its slot has been read back, so the stream never
   // waits for the host's Phase 2a/2b work.
@@ -1364,8 +1400,9 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
   c->stats[11] = t_2b;
   if (std::getenv("RMB_STAGE_TIMES"))
     std::fprintf(stderr,
-                 "[rmb] %zu batches; phase2b ms: sort %.1f  seeds+tasks %.1f  dp %.1f  replay %.1f  (prelims %zu)\n",
-                 batches.size(), tb_ms[0] / 1e3, tb_ms[1] / 1e3, tb_ms[2] / 1e3, tb_ms[3] / 1e3,
+                 "[rmb] %zu batches; setup ms %.1f; phase2b ms: sort %.1f  seeds+tasks %.1f  dp %.1f  replay %.1f  "
+                 "(prelims %zu)\n",
+                 batches.size(), t_setup / 1e3, tb_ms[0] / 1e3, tb_ms[1] / 1e3, tb_ms[2] / 1e3, tb_ms[3] / 1e3,
                  (size_t)c->stats[3]);
   unsigned long long hc[C_NCOUNTERS];
   hip_check(hipMemcpy(hc, ctr, sizeof hc, hipMemcpyDeviceToHost), "copy counters");
@@ -1551,6 +1588,8 @@ int rmb_ctx_destroy(rmb_ctx* c) {
                     &c->b_ops, &c->b_pre, &c->b_redo, &c->b_precount, &c->b_packtasks, &c->b_packed, &c->b_items, &c->b_qext,
                     &c->b_recids, &c->b_arena, &c->b_arena_big, &c->b_oppool, &c->b_opres, &c->b_opcount})
     b->release();
+  c->h_qfwd.release();
+  c->h_qrc.release();
   for (P1Slot& S : c->slot) {
     for (DevBuf* b : {&S.pre, &S.redo_hits, &S.cnt, &S.hitcnt, &S.ctr, &S.err}) b->release();
     for (hipEvent_t e : S.ev) (void)hipEventDestroy(e);
@@ -1715,7 +1754,7 @@ int rmb_tap_ungapped(rmb_ctx* c, const uint8_t* query, uint32_t q_len, uint32_t 
     one.first = {0, 1};
     one.whole_base = {0};
     one.pool = qf.size();
-    const uint8_t* d_qext = c->P.dust ? upload_dusted(c, {qf}, one) : d_qfwd;
+    const uint8_t* d_qext = c->P.dust ? upload_dusted(c, {SeqView{qf.data(), qf.size()}}, one) : d_qfwd;
     std::vector<UngHit> hits;
     if (q_len >= (uint32_t)c->P.word_size) {
       P1Group g;
