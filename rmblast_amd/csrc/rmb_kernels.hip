@@ -909,6 +909,44 @@ __device__ __forceinline__ uint64_t wave_take(unsigned long long* next, uint64_t
 static constexpr uint8_t SCRIPT_GAP_IN_A = 0, SCRIPT_SUB = 3, SCRIPT_GAP_IN_B = 6, SCRIPT_OP_MASK = 7,
                          SCRIPT_EXTEND_GAP_A = 0x10, SCRIPT_EXTEND_GAP_B = 0x40;
 
+// Bytes p[idx] (p[n - 1 - idx] when rev) for idx rising by about one per call,
+// through 16-byte aligned blocks: the block after the current one (in the
+// direction of travel) is requested as soon as the current one is entered, so
+// a lane waits on memory only when it jumps.  Blocks are loaded only where they
+// hold bytes of [p, p + n): an aligned block that holds a byte of an allocation
+// lies within the allocation's 256-byte-aligned start and 64-byte tail.
+struct SeqStream {
+  uintptr_t lo, hi;  // readable bytes [lo, hi)
+  int64_t cb;        // current block (address >> 4)
+  uint4 cur, nxt;
+  bool rev;
+  __device__ __forceinline__ void init(const uint8_t* p, uint32_t n, bool r) {
+    lo = (uintptr_t)p;
+    hi = (uintptr_t)p + n;
+    rev = r;
+    cb = -4;
+  }
+  __device__ __forceinline__ uint4 load(int64_t b) const {
+    if (b < (int64_t)(lo >> 4) || b > (int64_t)((hi - 1) >> 4)) return make_uint4(0u, 0u, 0u, 0u);
+    return *reinterpret_cast<const uint4*>((uintptr_t)b << 4);
+  }
+  __device__ __forceinline__ uint32_t at(uint32_t idx) {  // idx < n
+    const uintptr_t addr = rev ? hi - 1 - idx : lo + idx;
+    const int64_t b = (int64_t)(addr >> 4), step = rev ? -1 : 1;
+    if (b != cb) {
+      cur = b == cb + step ? nxt : load(b);
+      nxt = load(b + step);
+      cb = b;
+    }
+    // word (o >> 2) of cur by masks, which keeps cur in registers
+    const uint32_t o = (uint32_t)addr & 15u;
+    const uint32_t m8 = 0u - ((o >> 3) & 1u), m4 = 0u - ((o >> 2) & 1u);
+    const uint32_t a0 = cur.x ^ ((cur.x ^ cur.z) & m8), a1 = cur.y ^ ((cur.y ^ cur.w) & m8);
+    const uint32_t w = a0 ^ ((a0 ^ a1) & m4);
+    return (w >> ((o & 3u) * 8u)) & 0xFFu;
+  }
+};
+
 template <bool TB, uint32_t RS = 1>
 __device__ __forceinline__ void dp_half(const DpTask& t, const int* mat, int32_t go, int32_t ge,
                                         int2* __restrict__ ring, uint8_t* __restrict__ bring, uint32_t wmask,
@@ -925,8 +963,10 @@ __device__ __forceinline__ void dp_half(const DpTask& t, const int* mat, int32_t
   // ring beside its cell, read from global memory once when the column joins
   // the band; the row loop then reads only LDS, one cell ahead.
   constexpr bool BR = RS > 1;
-  const uint8_t* bp = t.b;
-  auto bcode = [&](uint32_t col) -> uint32_t { return col < n ? (bp[rev ? n - 1 - col : col] & 15u) : 0u; };
+  SeqStream as, bs;
+  as.init(t.a, m, rev);
+  bs.init(t.b, n, rev);
+  auto bcode = [&](uint32_t col) -> uint32_t { return col < n ? (bs.at(col) & 15u) : 0u; };
   uint32_t flat_len = 0;
   ring[0] = make_int2(0, -oe);
   if (BR) bring[0] = (uint8_t)bcode(0);
@@ -954,7 +994,7 @@ __device__ __forceinline__ void dp_half(const DpTask& t, const int* mat, int32_t
     // band (at most wmask + 1 cells past the row's first column) plus the
     // columns its first cells drop
     if (TB && (ai + 2 > rows_cap || flat_len + 2 * wmask + 4 > flat_cap)) { res.flags |= E_TB_SPACE; return; }
-    const int* row = mat + ((t.a[rev ? m - 1 - ai : ai] & 15) << 4);
+    const int* row = mat + ((as.at(ai) & 15u) << 4);
     if (TB) { rows[2 * (ai + 1)] = flat_len; rows[2 * (ai + 1) + 1] = first_b; }
     int32_t sc = kMinInt, sgr = kMinInt;
     uint32_t last_b = first_b;
