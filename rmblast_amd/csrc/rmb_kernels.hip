@@ -479,9 +479,14 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
     const int32_t c1 = q_len + 1;
     // diagonal hash: thread tid owns buckets tid and tid + kP1Threads for the
     // whole item; each bucket's first kHashLds (3) cells live in these
-    // registers, the rest in the workgroup's spill area
-    uint32_t hc[2] = {0u, 0u}, hl0[2] = {0u, 0u}, hl1[2] = {0u, 0u}, hl2[2] = {0u, 0u};
-    int32_t hd0[2] = {0, 0}, hd1[2] = {0, 0}, hd2[2] = {0, 0};
+    // registers, the rest in the workgroup's spill area.  A register cell not
+    // in use holds diagonal kNoDiag (no seed's) and level kFarLevel (never
+    // stale while subject offsets stay below 2^30).
+    constexpr int32_t kNoDiag = (int32_t)0x80000000u;
+    constexpr uint32_t kFarLevel = 0x7FFFFFFFu;
+    uint32_t hc[2] = {0u, 0u}, hl0[2] = {kFarLevel, kFarLevel}, hl1[2] = {kFarLevel, kFarLevel},
+             hl2[2] = {kFarLevel, kFarLevel};
+    int32_t hd0[2] = {kNoDiag, kNoDiag}, hd1[2] = {kNoDiag, kNoDiag}, hd2[2] = {kNoDiag, kNoDiag};
     // The scan runs over windows of kP1Threads subject positions; a window's
     // seeds (subject offset ascending, then cell order) are taken kSeedCap
     // per pass, so each window is scanned once.
@@ -690,48 +695,60 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
         uint32_t cnt_b = hc[h], l0 = hl0[h], l1 = hl1[h], l2 = hl2[h];
         int32_t d0 = hd0[h], d1 = hd1[h], d2 = hd2[h];
         bool slow = nb > 8 || cnt_b > (uint32_t)kHashLds;
-        uint32_t done_n = 0;  // seeds of the bucket the fast path replayed
-        if (__any(!slow && nb > 0)) {
+        // seeds of the bucket the fast path replays: all of them, unless the
+        // bucket needs the generic loop from the start or from a fourth cell
+        uint32_t done_n = slow ? 0u : nb;
+        if (__any(done_n > 0)) {
           uint32_t ix[8];
 #pragma unroll
-          for (int j = 0; j < 8; ++j) ix[j] = (!slow && (uint32_t)j < nb) ? (uint32_t)blist[st + j] : 0xFFFFu;
+          for (int j = 0; j < 8; ++j) ix[j] = (uint32_t)j < done_n ? (uint32_t)blist[st + j] : 0xFFFFu;
           sort8(ix);
+          // unused register cells never match and are never stale, so the
+          // tests below need no cell count; flags are written after the loop
+          uint32_t emit_mask = 0, evals = 0, stepsum = 0;
 #pragma unroll
           for (int j = 0; j < 8; ++j) {
-            const bool act = !slow && (uint32_t)j < nb;
+            const bool act = (uint32_t)j < done_n;
             if (!__any(act)) break;
-            const uint32_t i = act ? ix[j] : 0u;
-            const uint4 e = sd[i];
+            const uint4 e = sd[act ? ix[j] : 0u];
             const int32_t d = (int32_t)e.x;
             const uint32_t sc = e.y;
             // get(): newest cell first
-            const bool f2 = cnt_b > 2 && d2 == d, f1 = cnt_b > 1 && d1 == d, f0 = cnt_b > 0 && d0 == d;
-            const uint32_t lev = f2 ? l2 : (f1 ? l1 : (f0 ? l0 : 0u));
-            const bool eval = act && sc >= lev;
+            const bool m2 = d2 == d, m1 = d1 == d, m0 = d0 == d;
+            const uint32_t lev = m2 ? l2 : (m1 ? l1 : (m0 ? l0 : 0u));
+            // (bitwise & | on the tests: lane masks, no branches or selects)
+            const bool eval = act & (sc >= lev);
             // insert(): the newest cell on the same diagonal or stale, else a
             // new cell; a fourth cell sends the bucket to the generic loop
-            const bool u2 = cnt_b > 2 && (d2 == d || (int32_t)(sc - l2) > window);
-            const bool u1 = !u2 && cnt_b > 1 && (d1 == d || (int32_t)(sc - l1) > window);
-            const bool u0 = !u2 && !u1 && cnt_b > 0 && (d0 == d || (int32_t)(sc - l0) > window);
-            const bool grow = !u2 && !u1 && !u0;
-            const bool to_slow = eval && grow && cnt_b >= (uint32_t)kHashLds;
-            const bool apply = eval && !to_slow;
-            slow = slow || to_slow;
-            done_n = act && !to_slow ? (uint32_t)j + 1 : done_n;
-            n_evals += apply ? 1u : 0u;
-            n_steps += apply ? (e.w & 0xFFFFu) : 0u;
-            if (apply && ((e.w >> 16) & F_HIT)) {
-              sd[i].w = e.w | ((uint32_t)F_EMIT << 16);
-              ++nemit_local;
-            }
+            const bool s2 = (int32_t)(sc - l2) > window, s1 = (int32_t)(sc - l1) > window,
+                       s0 = (int32_t)(sc - l0) > window;
+            const bool u2 = m2 | s2;
+            const bool u1 = !u2 & (m1 | s1);
+            const bool u0 = !u2 & !u1 & (m0 | s0);
+            const bool grow = !(u2 | u1 | u0);
+            const bool to_slow = eval & grow & (cnt_b >= (uint32_t)kHashLds);
+            done_n = to_slow ? (uint32_t)j : done_n;
+            const bool apply = eval & !to_slow;
+            evals += apply ? 1u : 0u;
+            stepsum += apply ? (e.w & 0xFFFFu) : 0u;
+            emit_mask |= (apply & (((e.w >> 16) & F_HIT) != 0)) ? (1u << j) : 0u;
             const uint32_t L = e.z;
-            const bool w2 = apply && (u2 || (grow && cnt_b == 2));
-            const bool w1 = apply && (u1 || (grow && cnt_b == 1));
-            const bool w0 = apply && (u0 || (grow && cnt_b == 0));
+            const bool w2 = apply & (u2 | (grow & (cnt_b == 2)));
+            const bool w1 = apply & (u1 | (grow & (cnt_b == 1)));
+            const bool w0 = apply & (u0 | (grow & (cnt_b == 0)));
             d2 = w2 ? d : d2; l2 = w2 ? L : l2;
             d1 = w1 ? d : d1; l1 = w1 ? L : l1;
             d0 = w0 ? d : d0; l0 = w0 ? L : l0;
-            cnt_b += (apply && grow) ? 1u : 0u;
+            cnt_b += (apply & grow) ? 1u : 0u;
+          }
+          slow = slow || done_n < nb;
+          n_evals += evals;
+          n_steps += stepsum;
+          if (emit_mask) {
+            nemit_local += (uint32_t)__popc(emit_mask);
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+              if ((emit_mask >> j) & 1u) sd[ix[j]].w |= (uint32_t)F_EMIT << 16;
           }
         }
         if (slow) {
@@ -1332,11 +1349,19 @@ __global__ __launch_bounds__(kPrelimThreads) void k_prelim_fused(
   for (;;) {
    const uint64_t base = wave_take(P.next, 64 * kPrelimRun);
    if (base >= n_tasks) break;
+   // the lane's next hit is requested before its current one is aligned
+   const uint64_t k0 = base + (threadIdx.x & 63u);
+   uint32_t i_next = k0 < n_tasks ? (idx ? idx[k0] : (uint32_t)k0) : 0u;
+   UngHit u_next = hits[i_next];
    for (uint32_t run = 0; run < kPrelimRun; ++run) {
     const uint64_t k = base + 64 * run + (threadIdx.x & 63u);
     if (k >= n_tasks) break;
-    const uint32_t i = idx ? idx[k] : (uint32_t)k;
-    const UngHit u = hits[i];
+    const uint32_t i = i_next;
+    const UngHit u = u_next;
+    if (run + 1 < kPrelimRun && k + 64 < n_tasks) {
+      i_next = idx ? idx[k + 64] : (uint32_t)(k + 64);
+      u_next = hits[i_next];
+    }
     if (u.item == kNoItem) continue;  // reserved slot the seeding kernel left unused
     const uint32_t r = u.item / P.n_subj, f = u.item % P.n_subj;
     const RecDesc R = recs[r];
