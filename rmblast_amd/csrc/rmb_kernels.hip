@@ -232,17 +232,24 @@ __device__ __forceinline__ bool ungapped_short(const uint8_t* __restrict__ qpool
 // Fast first tier of the extension, for whole-word tables (ws >= 7) with
 // positive ACGT match scores.  Every base of the seed word matches, so each
 // of its steps raises the score: the state after them is known, and only the
-// flanks need stepping.  Plus-strand seeds start at the word's first base, so
-// the right side opens with 7 word steps; minus-strand seeds start at its last
-// base, so the left side opens with 6 and the right with 1.  Then eight
-// branch-free flank steps per side, from the 32 bytes [qa - 16, qa + 16) of
-// each sequence (three aligned 16-byte loads).  True: both sides ended within
-// those steps and `out` is the full extension's result; false: a side still
-// runs (or the seed lies within 16 bytes of its pool's start), and the caller
-// runs the full extension.
+// flanks need stepping.  Plus-strand seeds start at the word's first base (the
+// right side opens with 7 word steps), minus-strand seeds at its last base
+// (6 word steps on the left, 1 on the right): only the sum of the word's
+// scores enters the result, so a minus seed is stepped in a frame moved 6
+// bases left, where its word sits where a plus seed's does and both strands
+// run the same code.  Then eight branch-free flank steps per side, from the
+// 32 bytes [b - 16, b + 16) of each sequence around the frame start b (three
+// aligned 16-byte loads).  True: both sides ended within those steps and `out`
+// is the full extension's result; false: a side still runs (or the frame
+// starts within 16 bytes of its pool's start), and the caller runs the full
+// extension.
 __device__ __forceinline__ bool ungapped_first(const uint8_t* __restrict__ qpool, int64_t qa,
                                                const uint8_t* __restrict__ spool, int64_t sa, int32_t q_off,
                                                bool minus, const int* smat, int32_t xdrop, UngOut& out) {
+  const int32_t shift = minus ? 6 : 0;
+  qa -= shift;
+  sa -= shift;
+  q_off -= shift;
   if (qa < 16 || sa < 16) return false;
   const uint4* QV = reinterpret_cast<const uint4*>(qpool);
   const uint4* SV = reinterpret_cast<const uint4*>(spool);
@@ -252,7 +259,7 @@ __device__ __forceinline__ bool ungapped_first(const uint8_t* __restrict__ qpool
   const uint32_t oq = (uint32_t)qa & 15u, os = (uint32_t)sa & 15u;
   const uint4 qa16 = funnel16(q0, q1, oq), qb16 = funnel16(q1, q2, oq);
   const uint4 sa16 = funnel16(s0, s1, os), sb16 = funnel16(s1, s2, os);
-  // index bytes (q << 4 | s); byte 16 is the seed position qa
+  // index bytes (q << 4 | s); byte 16 is the frame start
   uint32_t ix[8];
   {
     const uint32_t qw[8] = {qa16.x, qa16.y, qa16.z, qa16.w, qb16.x, qb16.y, qb16.z, qb16.w};
@@ -261,48 +268,38 @@ __device__ __forceinline__ bool ungapped_first(const uint8_t* __restrict__ qpool
     for (int w = 0; w < 8; ++w) ix[w] = ((qw[w] << 4) & 0xF0F0F0F0u) | (sw[w] & 0x0F0F0F0Fu);
   }
   auto sc = [&](int p) { return smat[(ix[p >> 2] >> (8 * (p & 3))) & 0xFFu]; };
-  // the word's steps: plus bytes 16..22 on the right; minus bytes 15..10 on
-  // the left and byte 16 on the right
-  int32_t known_l = 0, known_r = 0;
+  // the word's steps: bytes 16..22
+  int32_t known = 0;
 #pragma unroll
-  for (int k = 0; k < 7; ++k) {
-    const int32_t v = smat[(minus ? (k < 6 ? (ix[(15 - k) >> 2] >> (8 * ((15 - k) & 3)))
-                                          : (ix[4] & 0xFFu))
-                                  : (ix[(16 + k) >> 2] >> (8 * ((16 + k) & 3)))) & 0xFFu];
-    known_l += (minus && k < 6) ? v : 0;
-    known_r += (!minus || k == 6) ? v : 0;
-  }
+  for (int k = 16; k < 23; ++k) known += sc(k);
   const int32_t x = -xdrop;
-  // left: 8 flank steps after the known ones
-  const int32_t qb0 = minus ? q_off - 6 : q_off;
-  int32_t score = known_l, sum = 0, q_beg = qb0;
-  uint32_t steps = minus ? 6u : 0u;
+  // left: 8 flank steps from byte 15 down
+  int32_t score = 0, sum = 0, q_beg = q_off;
+  uint32_t steps = 0;
   bool run = true;
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
-    const int32_t v = minus ? sc(9 - j) : sc(15 - j);
-    const int32_t ns = sum + v;
+    const int32_t ns = sum + sc(15 - j);
     const bool up = run && ns > 0;
     const bool stop = run && ns < x;
     score = up ? score + ns : score;
-    q_beg = up ? qb0 - 1 - j : q_beg;
+    q_beg = up ? q_off - 1 - j : q_beg;
     sum = run ? (ns > 0 ? 0 : ns) : sum;
     steps += run ? 1u : 0u;
     run = run && !stop;
   }
   if (run) return false;
   if (sum <= kSentScore / 2) --steps;  // ended on a sentinel: that step is not counted
-  // right: the known steps, then 8 flank steps
-  score += known_r;
-  steps += minus ? 1u : 7u;
-  const int32_t qe0 = minus ? q_off + 1 : q_off + 7;
+  // right: the word's steps, then 8 flank steps from byte 23 up
+  score += known;
+  steps += 7u;
+  const int32_t qe0 = q_off + 7;
   int32_t q_end = qe0, xc = (-score > x) ? -score : x;
   sum = 0;
   run = true;
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
-    const int32_t v = minus ? sc(17 + j) : sc(23 + j);
-    const int32_t ns = sum + v;
+    const int32_t ns = sum + sc(23 + j);
     const bool up = run && ns > 0;
     const bool stop = run && ns < xc;
     score = up ? score + ns : score;
