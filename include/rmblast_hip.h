@@ -139,8 +139,8 @@ int rmb_results_copy(const rmb_ctx* ctx, rmb_hsp* hsps, rmb_op* ops);
 /* Stage counters of the last search (the reference's diagnostics counters,
  * engine.rs:28-37 and gapped.rs:25-36): [0] seeds that reached the diagonal
  * hash, [1] ungapped hits kept, [2] preliminary DPs run, [3] traceback DPs run,
- * [4] score-only DP cells (both phases), [5] of those, the traceback phase's
- * (its score pass; the traceback pass fills the same band), [6] final HSPs,
+ * [4] DP cells (both phases, each band cell counted once), [5] of those, the
+ * traceback phase's, [6] final HSPs,
  * [7] ungapped extension steps.  Stage wall times in microseconds:
  * [8] lookup build, [9] seeding + ungapped, [10] preliminary gapped,
  * [11] traceback, [12] total; [13] time inside the seeding kernel in
