@@ -9,7 +9,8 @@ namespace rmb {
 // at a time; kSeedCap seeds are staged in LDS per pass; the 512 diagonal-hash
 // buckets are owned two per thread.
 constexpr int kP1Threads = 256;
-constexpr int kSeedCap = 1536;
+constexpr int kP1PerCu = 5;     // resident workgroups per CU (LDS and at most 96 VGPRs)
+constexpr int kSeedCap = 1152;  // with it a workgroup needs 31 KB of LDS: five fit a CU
 constexpr int kHashLds = 3;      // cells per bucket held in the owner thread's registers
 constexpr int kHashGlobal = 253; // further cells per bucket in the workgroup's global spill area
 constexpr int kBuckets = 512;

@@ -37,32 +37,6 @@ __device__ __forceinline__ uint32_t block_incl_scan256(uint32_t v, uint32_t* wsu
   return v + add;
 }
 
-// Byte reader over a 16-byte-aligned device pool through aligned 16-byte
-// vector loads: one load per 16 consecutive bytes instead of one per byte.
-// Pools are allocated with 64 bytes of slack so every aligned chunk that holds
-// a valid byte lies inside the allocation.  The byte is picked with a 64-bit
-// mask blend and a shift: a select between the vector's halves is turned back
-// into a lane-varying vector index, which is lowered through scratch memory.
-struct Win16 {
-  const uint8_t* __restrict__ base;
-  int64_t chunk;
-  uint64_t lo, hi;
-  __device__ __forceinline__ explicit Win16(const uint8_t* b) : base(b), chunk(-1), lo(0), hi(0) {}
-  __device__ __forceinline__ uint32_t at(int64_t i) {
-    const int64_t c = i >> 4;
-    if (c != chunk) {
-      chunk = c;
-      const uint4 v = *reinterpret_cast<const uint4*>(base + (c << 4));
-      lo = ((uint64_t)v.y << 32) | v.x;
-      hi = ((uint64_t)v.w << 32) | v.z;
-    }
-    const uint32_t k = (uint32_t)i & 15u;
-    const uint64_t m = 0ull - (uint64_t)((k >> 3) & 1u);  // all ones for the high half
-    const uint64_t h = lo ^ ((lo ^ hi) & m);
-    return (uint32_t)(h >> ((k & 7u) * 8u)) & 0xFFu;
-  }
-};
-
 // Bytes [o, o + 16) of the 32-byte little-endian concatenation lo|hi, o < 16:
 // a two-level word select and one v_alignbyte per output word.
 __device__ __forceinline__ uint4 funnel16(const uint4 lo, const uint4 hi, uint32_t o) {
@@ -75,158 +49,93 @@ __device__ __forceinline__ uint4 funnel16(const uint4 lo, const uint4 hi, uint32
                     __builtin_amdgcn_alignbyte(u3, u2, ob), __builtin_amdgcn_alignbyte(u4, u3, ob));
 }
 
-// Ungapped X-drop from (q_off, s_off) over sentinel-wrapped sequences stored at
-// q_base / s_base of their pools: left side fixed X, right side adaptive X
-// (ungapped.rs:223-306).
-struct UngOut { int32_t score; uint32_t q_beg, q_end, steps; };
-__device__ __forceinline__ UngOut ungapped_ext(const uint8_t* __restrict__ qpool, uint64_t q_base,
-                                               int32_t q_real_len, const uint8_t* __restrict__ spool,
-                                               uint64_t s_base, int32_t s_real_len, int32_t q_off, int32_t s_off,
-                                               const int* mat, int32_t xdrop) {
-  const int32_t x = -xdrop;
-  int32_t score = 0, sum = 0, q_beg = q_off;
-  uint32_t steps = 0;
-  Win16 wq(qpool), ws(spool);
-  const int64_t qb0 = (int64_t)q_base + 1, sb0 = (int64_t)s_base + 1;
-  for (int32_t qi = q_off - 1, si = s_off - 1; qi >= 0 && si >= 0; --qi, --si) {
-    const uint32_t qb = wq.at(qb0 + qi), sb = ws.at(sb0 + si);
-    if (qb == 15 || sb == 15) break;
-    ++steps;
-    sum += mat[(qb << 4) | sb];
-    if (sum > 0) {
-      q_beg = qi;
-      score += sum;
-      sum = 0;
-    } else if (sum < x) {
-      break;
-    }
-  }
-  int32_t q_end = q_off, xc = x;
-  sum = 0;
-  for (int32_t qi = q_off, si = s_off; qi < q_real_len && si < s_real_len; ++qi, ++si) {
-    const uint32_t qb = wq.at(qb0 + qi), sb = ws.at(sb0 + si);
-    if (qb == 15 || sb == 15) break;
-    ++steps;
-    sum += mat[(qb << 4) | sb];
-    if (sum > 0) {
-      q_end = qi + 1;
-      score += sum;
-      xc = (-score > x) ? -score : x;
-      sum = 0;
-    } else if (sum < xc) {
-      break;
-    }
-  }
-  return UngOut{score, (uint32_t)q_beg, (uint32_t)q_end, steps};
+// Bytes [e, e + 16) of a 16-byte-aligned pool (e >= -15), from the two aligned
+// 16-byte blocks that hold them.  A block before the pool start is replaced by
+// block 0: its bytes sit before the pool's leading sentinel, which ends every
+// extension first.  Pools carry 64 bytes of tail, so the blocks after any
+// sentinel lie inside the allocation.
+__device__ __forceinline__ uint4 pool16(const uint4* __restrict__ V, int64_t e) {
+  const int64_t b = e >> 4;  // floor
+  const uint4 lo = V[b < 0 ? 0 : b], hi = V[b + 1];
+  return funnel16(lo, hi, (uint32_t)e & 15u);
 }
 
-// The same extension when each side ends within kShortReach steps, which
-// covers all but a small fraction of seeds: the 32 bytes on either side of
-// the seed in both sequences come in with ten independent 16-byte loads and
-// every step then runs from registers.  `smat` is the scoring matrix with
-// kSentScore in row and column 15, so reaching a sentinel ends a side through
-// the X-drop test (the step is then not counted, as the sentinel test comes
-// first in ungapped_ext).  Returns false when a side is still running at the
-// end of its window; the caller then runs ungapped_ext.
-static constexpr int32_t kSentScore = -(1 << 20);
-static constexpr int kShortReach = 32;
+struct UngOut { int32_t score; uint32_t q_beg, q_end, steps; };
 
-__device__ __forceinline__ bool ungapped_short(const uint8_t* __restrict__ qpool, int64_t qa,
-                                               const uint8_t* __restrict__ spool, int64_t sa, int32_t q_off,
-                                               const int* smat, int32_t xdrop, UngOut& out) {
+// `smat` is the scoring matrix with kSentScore in row and column 15, so
+// reaching a sentinel ends a side through the X-drop test; that step is then
+// taken back, as the reference tests for the sentinel before counting a step.
+static constexpr int32_t kSentScore = -(1 << 20);
+static constexpr int kLongChunk = 8;  // steps per chunk of the full extension
+
+// Ungapped X-drop from (q_off, s_off) over sentinel-wrapped sequences: left
+// side fixed X, right side adaptive X (ungapped.rs:223-306).  qa / sa are the
+// pool offsets of the start position.  Each side runs in chunks of 16 steps:
+// the chunk's 16 bytes of both sequences come in with four aligned loads, its
+// 16 matrix lookups are issued together, and the X-drop state advances
+// branch-free under a per-lane `run` flag until the lane's side ends.
+template <int CH>
+__device__ __forceinline__ UngOut ungapped_chunked(const uint8_t* __restrict__ qpool, int64_t qa,
+                                                   const uint8_t* __restrict__ spool, int64_t sa, int32_t q_off,
+                                                   const int* smat, int32_t xdrop) {
   const uint4* QV = reinterpret_cast<const uint4*>(qpool);
   const uint4* SV = reinterpret_cast<const uint4*>(spool);
-  const int64_t cq = qa >> 4, cs = sa >> 4;
-  // chunks below the pool start only hold bytes that sit past the leading
-  // sentinel of the first record, which the extension never reads
-  const uint4 q0 = QV[cq >= 2 ? cq - 2 : 0], q1 = QV[cq >= 1 ? cq - 1 : 0], q2 = QV[cq], q3 = QV[cq + 1],
-              q4 = QV[cq + 2];
-  const uint4 s0 = SV[cs >= 2 ? cs - 2 : 0], s1 = SV[cs >= 1 ? cs - 1 : 0], s2 = SV[cs], s3 = SV[cs + 1],
-              s4 = SV[cs + 2];
-  const uint32_t oq = (uint32_t)qa & 15u, os = (uint32_t)sa & 15u;
-  // left windows: bytes qa-32 .. qa-1; right windows: bytes qa .. qa+31
-  const uint4 la = funnel16(q0, q1, oq), lb = funnel16(q1, q2, oq);
-  const uint4 ta = funnel16(s0, s1, os), tb = funnel16(s1, s2, os);
-  const uint4 ra = funnel16(q2, q3, oq), rb = funnel16(q3, q4, oq);
-  const uint4 ua = funnel16(s2, s3, os), ub = funnel16(s3, s4, os);
-  // matrix index bytes (q << 4 | s), four per word
-  uint32_t li[8], ri[8];
-  {
-    const uint32_t lq[8] = {la.x, la.y, la.z, la.w, lb.x, lb.y, lb.z, lb.w};
-    const uint32_t ls[8] = {ta.x, ta.y, ta.z, ta.w, tb.x, tb.y, tb.z, tb.w};
-    const uint32_t rq[8] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y, rb.z, rb.w};
-    const uint32_t rs[8] = {ua.x, ua.y, ua.z, ua.w, ub.x, ub.y, ub.z, ub.w};
-#pragma unroll
-    for (int w = 0; w < 8; ++w) {
-      li[w] = ((lq[w] << 4) & 0xF0F0F0F0u) | (ls[w] & 0x0F0F0F0Fu);
-      ri[w] = ((rq[w] << 4) & 0xF0F0F0F0u) | (rs[w] & 0x0F0F0F0Fu);
-    }
-  }
-  // Steps run in groups of eight: the group's matrix lookups are issued
-  // together, the X-drop state advances branch-free under a per-lane `run`
-  // flag, and a group starts only while some lane of the wave still runs.
+  auto index4 = [](uint32_t q, uint32_t s) { return ((q << 4) & 0xF0F0F0F0u) | (s & 0x0F0F0F0Fu); };
   const int32_t x = -xdrop;
   int32_t score = 0, sum = 0, q_beg = q_off;
   uint32_t steps = 0;
-  {
-    // left: step k reads query offset q_off - 1 - k (window byte 31 - k)
-    bool run = true;
+  bool run = true;
+  // left: step k reads offset qa - 1 - k; chunk c holds steps 16c .. 16c + 15
+  // at bytes 15 .. 0 of [qa - 16(c + 1), qa - 16c)
+  for (int32_t c = 0; run; ++c) {
+    const int64_t e = 16 + CH * (int64_t)c;
+    const uint4 qv = pool16(QV, qa - e), sv = pool16(SV, sa - e);
+    const uint32_t iw[4] = {index4(qv.x, sv.x), index4(qv.y, sv.y), index4(qv.z, sv.z), index4(qv.w, sv.w)};
+    int32_t v[CH];
 #pragma unroll
-    for (int g = 0; g < kShortReach / 8; ++g) {
-      if (!__any(run)) break;
-      int32_t v[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int p = kShortReach - 1 - (8 * g + j);
-        v[j] = smat[(li[p >> 2] >> (8 * (p & 3))) & 0xFFu];
-      }
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int32_t ns = sum + v[j];
-        const bool up = run && ns > 0;
-        const bool stop = run && ns < x;
-        score = up ? score + ns : score;
-        q_beg = up ? q_off - 1 - (8 * g + j) : q_beg;
-        sum = run ? (ns > 0 ? 0 : ns) : sum;
-        steps += run ? 1u : 0u;
-        run = run && !stop;
-      }
+    for (int j = 0; j < CH; ++j) {
+      const int p = 15 - j;
+      v[j] = smat[(iw[p >> 2] >> (8 * (p & 3))) & 0xFFu];
     }
-    if (run) return false;
-    if (sum <= kSentScore / 2) --steps;  // ended on a sentinel: that step is not counted
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      const int32_t ns = sum + v[j];
+      const bool up = run && ns > 0;
+      const bool stop = run && ns < x;
+      score = up ? score + ns : score;
+      q_beg = up ? q_off - 1 - (CH * c + j) : q_beg;
+      sum = run ? (ns > 0 ? 0 : ns) : sum;
+      steps += run ? 1u : 0u;
+      run = run && !stop;
+    }
   }
+  if (sum <= kSentScore / 2) --steps;  // ended on a sentinel: that step is not counted
   int32_t q_end = q_off, xc = x;
   sum = 0;
-  {
-    // right: step k reads query offset q_off + k (window byte k)
-    bool run = true;
+  run = true;
+  // right: step k reads offset qa + k; chunk c holds steps 16c .. 16c + 15
+  for (int32_t c = 0; run; ++c) {
+    const int64_t e = CH * (int64_t)c;
+    const uint4 qv = pool16(QV, qa + e), sv = pool16(SV, sa + e);
+    const uint32_t iw[4] = {index4(qv.x, sv.x), index4(qv.y, sv.y), index4(qv.z, sv.z), index4(qv.w, sv.w)};
+    int32_t v[CH];
 #pragma unroll
-    for (int g = 0; g < kShortReach / 8; ++g) {
-      if (!__any(run)) break;
-      int32_t v[8];
+    for (int j = 0; j < CH; ++j) v[j] = smat[(iw[j >> 2] >> (8 * (j & 3))) & 0xFFu];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int p = 8 * g + j;
-        v[j] = smat[(ri[p >> 2] >> (8 * (p & 3))) & 0xFFu];
-      }
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int32_t ns = sum + v[j];
-        const bool up = run && ns > 0;
-        const bool stop = run && ns < xc;
-        score = up ? score + ns : score;
-        q_end = up ? q_off + 8 * g + j + 1 : q_end;
-        xc = up ? ((-score > x) ? -score : x) : xc;
-        sum = run ? (ns > 0 ? 0 : ns) : sum;
-        steps += run ? 1u : 0u;
-        run = run && !stop;
-      }
+    for (int j = 0; j < CH; ++j) {
+      const int32_t ns = sum + v[j];
+      const bool up = run && ns > 0;
+      const bool stop = run && ns < xc;
+      score = up ? score + ns : score;
+      q_end = up ? q_off + CH * c + j + 1 : q_end;
+      xc = up ? ((-score > x) ? -score : x) : xc;
+      sum = run ? (ns > 0 ? 0 : ns) : sum;
+      steps += run ? 1u : 0u;
+      run = run && !stop;
     }
-    if (run) return false;
-    if (sum <= kSentScore / 2) --steps;
   }
-  out = UngOut{score, (uint32_t)q_beg, (uint32_t)q_end, steps};
-  return true;
+  if (sum <= kSentScore / 2) --steps;
+  return UngOut{score, (uint32_t)q_beg, (uint32_t)q_end, steps};
 }
 
 // Fast first tier of the extension, for whole-word tables (ws >= 7) with
@@ -238,8 +147,9 @@ __device__ __forceinline__ bool ungapped_short(const uint8_t* __restrict__ qpool
 // scores enters the result, so a minus seed is stepped in a frame moved 6
 // bases left, where its word sits where a plus seed's does and both strands
 // run the same code.  Then eight branch-free flank steps per side, from the
-// 32 bytes [b - 16, b + 16) of each sequence around the frame start b (three
-// aligned 16-byte loads).  True: both sides ended within those steps and `out`
+// 32 bytes [b - 16, b + 16) of each sequence around the frame start b (two
+// unaligned 16-byte loads each: gfx950 global loads take any byte address).
+// True: both sides ended within those steps and `out`
 // is the full extension's result; false: a side still runs (or the frame
 // starts within 16 bytes of its pool's start), and the caller runs the full
 // extension.
@@ -251,14 +161,11 @@ __device__ __forceinline__ bool ungapped_first(const uint8_t* __restrict__ qpool
   sa -= shift;
   q_off -= shift;
   if (qa < 16 || sa < 16) return false;
-  const uint4* QV = reinterpret_cast<const uint4*>(qpool);
-  const uint4* SV = reinterpret_cast<const uint4*>(spool);
-  const int64_t cq = (qa - 16) >> 4, cs = (sa - 16) >> 4;
-  const uint4 q0 = QV[cq], q1 = QV[cq + 1], q2 = QV[cq + 2];
-  const uint4 s0 = SV[cs], s1 = SV[cs + 1], s2 = SV[cs + 2];
-  const uint32_t oq = (uint32_t)qa & 15u, os = (uint32_t)sa & 15u;
-  const uint4 qa16 = funnel16(q0, q1, oq), qb16 = funnel16(q1, q2, oq);
-  const uint4 sa16 = funnel16(s0, s1, os), sb16 = funnel16(s1, s2, os);
+  uint4 qa16, qb16, sa16, sb16;
+  __builtin_memcpy(&qa16, qpool + qa - 16, 16);
+  __builtin_memcpy(&qb16, qpool + qa, 16);
+  __builtin_memcpy(&sa16, spool + sa - 16, 16);
+  __builtin_memcpy(&sb16, spool + sa, 16);
   // index bytes (q << 4 | s); byte 16 is the frame start
   uint32_t ix[8];
   {
@@ -381,7 +288,7 @@ __device__ __forceinline__ void sort8(uint32_t (&v)[8]) {
 enum SeedFlag : uint8_t { F_VALID = 1, F_HIT = 2, F_EMIT = 4, F_MINUS = 8, F_LONG = 16 };
 constexpr uint32_t kOutChunk = 256;  // output slots a workgroup reserves at a time
 
-__global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
+__global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
     P1Params P, const RecDesc* __restrict__ recs, const SubjDesc* __restrict__ subjs,
     const uint8_t* __restrict__ qfwd, const uint8_t* __restrict__ qrc, const uint8_t* __restrict__ qext,
     const uint8_t* __restrict__ splus, const uint8_t* __restrict__ src, const uint32_t* __restrict__ cells,
@@ -651,10 +558,8 @@ __global__ __launch_bounds__(kP1Threads, 4) void k_phase1(
           const int32_t eq = (int32_t)e.z;
           const int32_t es = minus ? lw - 1 + (int32_t)e.y : (int32_t)e.y;
           const uint8_t* qpool = minus ? qrc : qfwd;
-          UngOut u;
-          if (!ungapped_short(qpool, (int64_t)R.q_base + 1 + eq, splus, (int64_t)S.s_base + 1 + es, eq, mat,
-                              P.xdrop_ungap, u))
-            u = ungapped_ext(qpool, R.q_base, q_len + 1, splus, S.s_base, s_len + 1, eq, es, mat, P.xdrop_ungap);
+          const UngOut u = ungapped_chunked<kLongChunk>(qpool, (int64_t)R.q_base + 1 + eq, splus, (int64_t)S.s_base + 1 + es,
+                                            eq, mat, P.xdrop_ungap);
           const bool hit = u.score >= P.cutoff;
           if (hit) my_hitres[i] = make_uint4(u.q_beg, u.q_end, (uint32_t)u.score, (uint32_t)eq);
           const uint32_t level = hit ? (uint32_t)es + (u.q_end - (uint32_t)eq) : e.y + (uint32_t)ws;

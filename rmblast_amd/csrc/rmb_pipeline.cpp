@@ -697,9 +697,10 @@ static void launch_group_phase1(rmb_ctx* c, hipStream_t st, P1Group& g, const ui
     P.item_list = dl;
     P.n_items = g.items.size();
   }
-  // ~40 KB of LDS per workgroup: four resident per CU.  Workgroups are dealt
-  // round-robin over the XCDs, so blockIdx % kXcds names the XCD.
-  const int blocks = std::max(1, std::min<int>(c->cu_count * 4, (int)std::min<uint64_t>(P.n_items, 1u << 20)));
+  // ~31 KB of LDS per workgroup: kP1PerCu resident per CU.  Workgroups are
+  // dealt round-robin over the XCDs, so blockIdx % kXcds names the XCD.
+  const int blocks =
+      std::max(1, std::min<int>(c->cu_count * kP1PerCu, (int)std::min<uint64_t>(P.n_items, 1u << 20)));
   uint2* spill = c->b_spill.get<uint2>((size_t)blocks * kBuckets * kHashGlobal);
   uint4* hitres = c->b_hitres.get<uint4>((size_t)blocks * kSeedCap);
   unsigned long long* queue = c->b_queue.get<unsigned long long>(kXcds);
