@@ -513,8 +513,13 @@ static void run_dp_traceback_2pass(rmb_ctx* c, const std::vector<DpTask>& tasks,
 constexpr uint32_t kOnePassRows = 8192, kOnePassFlat = 256u << 10, kOnePassOps = 4096;
 constexpr unsigned long long kOnePassPool = 64ull << 20;  // packed ops per launch
 // retry slots for halves beyond the LDS ring or the lane arena: global rings of
-// 2^kRingBigLog2 cells and 96 MB arenas
+// 2^kRingBigLog2 cells and 70 MB arenas (RMB_TB_RETRY_SLOTS lowers the count,
+// down to 0, which sends every such half to the two-pass path)
 constexpr uint32_t kBigSlots = 32, kBigRows = 1u << 18, kBigFlat = 64u << 20, kBigOps = 1u << 20;
+static uint32_t retry_slots() {
+  if (const char* e = std::getenv("RMB_TB_RETRY_SLOTS")) return (uint32_t)std::min<long>(std::max(0L, std::atol(e)), kBigSlots);
+  return kBigSlots;
+}
 
 // Traceback DPs: one pass (score, traceback bytes, walk) per half in a
 // per-lane arena, longest first; the halves it flags run in two passes.
@@ -551,7 +556,7 @@ static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::
     big.arena = c->b_arena_big.get<uint8_t>((size_t)kBigSlots * onepass_stride(kBigFlat, kBigRows, kBigOps));
     big.next = reinterpret_cast<unsigned int*>(pcount + 2);
     big.ring_log2 = kRingBigLog2;
-    big.slots = kBigSlots;
+    big.slots = retry_slots();
     big.flat_cap = kBigFlat;
     big.rows_cap = kBigRows;
     big.ops_cap = kBigOps;

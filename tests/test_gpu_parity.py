@@ -95,6 +95,39 @@ def test_gappy_divergent_and_microsatellites_match_oracle(tmp_path):
     assert g == o and g.count("\n") > 20
 
 
+def test_traceback_two_pass_fallback_matches_oracle(tmp_path):
+    """Halves whose band outgrows the traceback kernel's LDS ring are rerun in
+    the same launch with a retry slot; with no retry slots they take the
+    two-pass path (score pass, then a traceback pass into exactly sized
+    storage).  Both must give the oracle's rows."""
+    rng = np.random.default_rng(11)
+    fams = synth.make_families(rng, 3, [3000, 4000, 2500])
+    g0, _ = synth.insert_copies(rng, 60_000, fams, 20, (400, 2500), (0.20, 0.35), (0.04, 0.08))
+    g0 = g0.copy()
+    for k in range(12):
+        p = int(rng.integers(2, 5))
+        unit = synth.random_bases(rng, p)
+        at = int(rng.integers(0, len(g0) - 300))
+        g0[at:at + 300] = np.tile(unit, 300 // p + 1)[:300]
+    fams.append(("sat", np.tile(np.array([1, 0], dtype=np.uint8), 400)))
+    synth.to_fasta([("q", g0)], tmp_path / "q.fa")
+    synth.to_fasta(fams, tmp_path / "db.fa")
+    (tmp_path / "m.matrix").write_text(synth.MATRIX_TEXT)
+    args = ["--db", str(tmp_path / "db.fa"), "--query", str(tmp_path / "q.fa"), "--matrix",
+            str(tmp_path / "m.matrix"), "--word-size", "7", "--gapopen", "20", "--gapextend", "5",
+            "--min-raw-gapped-score", "200", "--xdrop-gap", "120", "--xdrop-gap-final", "400",
+            "--complexity-adjust", "--dust", "no", "--outfmt", RM_FIELDS]
+    o = run([ORACLE_BIN] + args)
+    assert o.returncode == 0, o.stderr
+    for slots in ("32", "0"):
+        g = run([CLI] + args, env={"RMB_TB_RETRY_SLOTS": slots, "RMB_STAGE_TIMES": "1"})
+        assert g.returncode == 0, g.stderr
+        assert g.stdout == o.stdout and g.stdout.count("\n") > 10
+        if slots == "0":  # the two-pass path ran
+            assert any("two-pass fallback" in l and "(0 halves)" not in l.split("fallback")[1]
+                       for l in g.stderr.splitlines()), g.stderr[-2000:]
+
+
 def test_edge_cases_short_and_empty_records(tmp_path):
     q = tmp_path / "q.fa"
     q.write_text(">empty\n\n>short\nACGTA\n>nrun\n" + "N" * 500 + "\n>ok\n" +
