@@ -6,10 +6,16 @@
 //  k_phase1        seeding + exact diagonal-hash dedup + ungapped X-drop for
 //                  (record, subject) work items pulled from a device queue
 //                  (search/engine.rs:1063-1277, diag_hash.rs, ungapped.rs)
+//  k_prelim_fused  Phase 2a's score-only X-drop DP for every ungapped hit
+//                  (search/engine.rs:1850-1895, search/gapped.rs:360-418)
+//  k_dp_onepass    Phase 2b: score, traceback bytes and traceback walk of a
+//                  half alignment in one lane-serial pass
+//                  (search/gapped.rs:450-757)
 //  k_dp_score      lane-serial score-only X-drop half alignment
 //                  (search/gapped.rs:124-337)
 //  k_dp_traceback  the same recurrence with 1-byte traceback cells and the
-//                  traceback walk (search/gapped.rs:450-757)
+//                  traceback walk, for halves beyond k_dp_onepass's storage
+//                  (search/gapped.rs:450-757)
 #include <hip/hip_runtime.h>
 
 #include <hipcub/hipcub.hpp>
