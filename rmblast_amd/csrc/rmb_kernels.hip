@@ -860,6 +860,11 @@ struct SeqStream {
     const int64_t b = (int64_t)(addr >> 4), step = rev ? -1 : 1;
     if (b != cb) {
       cur = b == cb + step ? nxt : load(b);
+      // cur is needed complete here, where the block changes: the wait for
+      // it then sits in this branch, not at every read of a byte (a vector
+      // memory wait also drains the lane's traceback stores queued since),
+      // and the next block's load goes out after it
+      __asm__ volatile("" ::"v"(cur.x), "v"(cur.y), "v"(cur.z), "v"(cur.w));
       nxt = load(b + step);
       cb = b;
     }
