@@ -999,6 +999,140 @@ __device__ __forceinline__ void dp_half(const DpTask& t, const int* mat, int32_t
   res.tb_rows = rows_at_best;
 }
 
+static constexpr uint32_t kDpLdsThreads = 64, kDpLdsLog2 = 6;
+
+// The traceback DP of k_dp_onepass with 4-byte LDS ring cells, so twice as
+// many lanes fit a CU: H and E as 16-bit offsets from the best score at the
+// start of the row that wrote the cell (every band cell is rewritten in each
+// row it stays in the band, a pruned cell with its E re-based, so one base per
+// row serves).  H of a pruned cell is the sentinel -32768, and E is held at
+// -32767 at the lowest: such an E lies more than 32,000 below the best score,
+// X-drop being far smaller, so it loses every maximum it meets and never keeps
+// a cell; the traceback code of a cell whose every candidate is that low is
+// never read by the walk.  Every other value is exact, so every decision is
+// dp_half's.  Kept cells lie in [base - X - open cost, base + ring cells x
+// largest score]; a task for which that could leave the 16-bit range comes
+// back E_BAND_WIDE, for the retry with dp_half.
+__device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, int32_t go, int32_t ge,
+                                             uint32_t* __restrict__ ring, uint8_t* __restrict__ bring,
+                                             int32_t max_pos, DpResult& res, uint8_t* __restrict__ flat,
+                                             uint32_t* __restrict__ rows, uint32_t flat_cap, uint32_t rows_cap) {
+  constexpr uint32_t RS = kDpLdsThreads, wmask = (1u << kDpLdsLog2) - 1;
+  res.score = 0; res.a_len = 0; res.b_len = 0; res.flags = 0; res.cells = 0; res.tb_bytes = 0; res.tb_rows = 0;
+  const uint32_t m = t.m, n = t.n;
+  if (m == 0 || n == 0) return;
+  const bool rev = t.reverse != 0;
+  const int32_t oe = go + ge;
+  const int32_t xdrop = t.xdrop > oe ? t.xdrop : oe;
+  if ((int64_t)max_pos * (wmask + 1) + xdrop + oe >= 30000) { res.flags |= E_BAND_WIDE; return; }
+  constexpr int32_t kNegH = -32768, kLowE = -32767;
+  auto enc = [](int32_t h, int32_t e) -> uint32_t {
+    return ((uint32_t)h & 0xFFFFu) | ((uint32_t)(e < kLowE ? kLowE : e) << 16);
+  };
+  SeqStream as, bs;
+  as.init(t.a, m, rev);
+  bs.init(t.b, n, rev);
+  auto bcode = [&](uint32_t col) -> uint32_t { return col < n ? (bs.at(col) & 15u) : 0u; };
+  uint32_t flat_len = 0;
+  ring[0] = enc(0, -oe);
+  bring[0] = (uint8_t)bcode(0);
+  rows[0] = 0; rows[1] = 0;
+  flat[flat_len++] = SCRIPT_GAP_IN_B;
+  int32_t score = -oe;
+  uint32_t b_size = 1;
+  for (uint32_t j = 1; j <= n; ++j) {
+    if (score < -xdrop) break;
+    if (j > wmask) { res.flags |= E_BAND_WIDE; return; }
+    ring[(j & wmask) * RS] = enc(score, score - oe);
+    bring[(j & wmask) * RS] = (uint8_t)bcode(j);
+    flat[flat_len++] = SCRIPT_GAP_IN_A;
+    score -= ge;
+    b_size = j + 1;
+  }
+  int32_t best = 0, cbase = 0;  // cbase: base of the cells written in this row
+  uint32_t best_a = 0, best_b = 0, first_b = 0;
+  uint64_t cells = 0;
+  uint32_t bytes_at_best = flat_len, rows_at_best = 1;
+  for (uint32_t ai = 0; ai < m; ++ai) {
+    if (ai + 2 > rows_cap || flat_len + 2 * wmask + 4 > flat_cap) { res.flags |= E_TB_SPACE; return; }
+    const int32_t pbase = cbase;  // the cells read in this row were written with it
+    cbase = best;
+    const int* row = mat + ((as.at(ai) & 15u) << 4);
+    rows[2 * (ai + 1)] = flat_len;
+    rows[2 * (ai + 1) + 1] = first_b;
+    int32_t sc = kMinInt, sgr = kMinInt;
+    uint32_t last_b = first_b;
+    bool best_here = false;
+    cells += b_size - first_b;
+    uint32_t cn = ring[(first_b & wmask) * RS];
+    uint32_t bn = bring[(first_b & wmask) * RS];
+    for (uint32_t bi = first_b; bi < b_size; ++bi) {
+      const uint32_t cv = cn;
+      const int32_t sub = row[bn];
+      {  // the next cell's inputs, loaded while this one is computed
+        const uint32_t ns = ((bi + 1) & wmask) * RS;
+        cn = ring[ns];
+        bn = bring[ns];
+      }
+      const int32_t hr = (int32_t)(int16_t)(uint16_t)(cv & 0xFFFFu);
+      const int32_t cx = hr == kNegH ? kMinInt : hr + pbase;
+      int32_t sgc = (int32_t)(int16_t)(uint16_t)(cv >> 16) + pbase;
+      const int32_t next = cx + sub;
+      uint8_t op = SCRIPT_SUB;
+      int32_t cs = sc;
+      if (sgc > cs) { cs = sgc; op = SCRIPT_GAP_IN_B; }
+      if (sgr > cs) { cs = sgr; op = SCRIPT_GAP_IN_A; }
+      if (best - cs > xdrop) {
+        if (bi == first_b) ++first_b;
+        else ring[(bi & wmask) * RS] = enc(kNegH, sgc - cbase);
+      } else {
+        last_b = bi;
+        if (cs > best) { best = cs; best_a = ai + 1; best_b = bi; best_here = true; }
+        sgr -= ge;
+        sgc -= ge;
+        const int32_t open = cs - oe;
+        int32_t cy;
+        if (sgc < open) cy = open;
+        else { cy = sgc; op |= SCRIPT_EXTEND_GAP_B; }
+        ring[(bi & wmask) * RS] = enc(cs - cbase, cy - cbase);
+        if (sgr < open) sgr = open;
+        else op |= SCRIPT_EXTEND_GAP_A;
+      }
+      sc = next;
+      flat[flat_len++] = op;
+    }
+    if (first_b >= b_size) {
+      if (best_here) { bytes_at_best = flat_len; rows_at_best = ai + 2; }
+      break;
+    }
+    if (last_b + 1 < b_size) {
+      b_size = last_b + 1;
+    } else {
+      while (sgr >= best - xdrop && b_size <= n) {
+        if (b_size - first_b > wmask) { res.flags |= E_BAND_WIDE; return; }
+        ring[(b_size & wmask) * RS] = enc(sgr - cbase, sgr - oe - cbase);
+        bring[(b_size & wmask) * RS] = (uint8_t)bcode(b_size);
+        flat[flat_len++] = SCRIPT_GAP_IN_A;
+        sgr -= ge;
+        ++b_size;
+      }
+    }
+    if (b_size <= n) {
+      if (b_size - first_b > wmask) { res.flags |= E_BAND_WIDE; return; }
+      ring[(b_size & wmask) * RS] = enc(kNegH, kLowE);
+      bring[(b_size & wmask) * RS] = (uint8_t)bcode(b_size);
+      ++b_size;
+    }
+    if (best_here) { bytes_at_best = flat_len; rows_at_best = ai + 2; }
+  }
+  res.score = best;
+  res.a_len = best_a;
+  res.b_len = best_b;
+  res.cells = cells;
+  res.tb_bytes = bytes_at_best;
+  res.tb_rows = rows_at_best;
+}
+
 // Traceback walk (gapped.rs:692-757) from (a, b), the best row and column,
 // over the rows' traceback bytes: the ops, run-length encoded, collected
 // backwards into ops[0, cap); the count is returned, E_TB_SPACE set in flags
@@ -1050,7 +1184,6 @@ __device__ __forceinline__ uint32_t tb_walk(const uint8_t* __restrict__ flat, co
 // Traceback-phase bands (X-drop 100 by default) stay within 64 cells: the
 // first attempt gives each lane a 64-cell ring in LDS (64-lane workgroups);
 // wider bands rerun with per-lane rings in global memory.
-static constexpr uint32_t kDpLdsThreads = 64, kDpLdsLog2 = 6;
 
 template <bool LDS_RING>
 __global__ __launch_bounds__(LDS_RING ? kDpLdsThreads : 256) void k_dp_score(
@@ -1124,15 +1257,26 @@ __global__ __launch_bounds__(LDS_RING ? kDpLdsThreads : 256) void k_dp_onepass(
     unsigned long long ops_pool_cap, OnePassResult* __restrict__ out, unsigned long long* next, OnePassBig big,
     unsigned long long* counters) {
   __shared__ int mat[256];
-  __shared__ int2 lring[LDS_RING ? (kDpLdsThreads << kDpLdsLog2) : 1];
-  __shared__ uint8_t lbring[LDS_RING ? (kDpLdsThreads << kDpLdsLog2) : 1];
-  for (int i = threadIdx.x; i < 256; i += blockDim.x) mat[i] = gmat[i];
+  __shared__ uint32_t lring[kDpLdsThreads << kDpLdsLog2];
+  __shared__ uint8_t lbring[kDpLdsThreads << kDpLdsLog2];
+  __shared__ int s_max_pos;
+  if (threadIdx.x == 0) s_max_pos = 0;
   __syncthreads();
+  int my_max = 0;  // largest score among codes 0..14 (0 if none is positive)
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) {
+    const int v = gmat[i];
+    mat[i] = v;
+    if ((i >> 4) < 15 && (i & 15) < 15) my_max = max(my_max, v);
+  }
+  atomicMax(&s_max_pos, my_max);
+  __syncthreads();
+  const int32_t max_pos = s_max_pos;
   const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
-  constexpr uint32_t RS = LDS_RING ? kDpLdsThreads : 1;
-  int2* my = LDS_RING ? lring + threadIdx.x : ring + (size_t)gt * ring_stride(wlog2);
-  uint8_t* myb = LDS_RING ? lbring + threadIdx.x : nullptr;
-  const uint32_t wmask = LDS_RING ? (1u << kDpLdsLog2) - 1 : (1u << wlog2) - 1;
+  static_assert(LDS_RING, "the one-pass kernel runs with LDS rings");
+  (void)ring;
+  (void)wlog2;
+  uint32_t* my = lring + threadIdx.x;
+  uint8_t* myb = lbring + threadIdx.x;
   uint8_t* base = arena + (size_t)gt * onepass_stride(flat_cap, rows_cap, ops_cap);
   uint32_t* rows = reinterpret_cast<uint32_t*>(base);
   uint8_t* flat = base + (size_t)rows_cap * 8;
@@ -1147,7 +1291,7 @@ __global__ __launch_bounds__(LDS_RING ? kDpLdsThreads : 256) void k_dp_onepass(
     const uint32_t i = (uint32_t)base + (threadIdx.x & 63u);
     if (i >= n_tasks) continue;
     DpResult r;
-    dp_half<true, RS>(tasks[i], mat, go, ge, my, myb, wmask, r, flat, rows, 0xFFFFFFFFu, flat_cap, rows_cap);
+    dp_half_tb16(tasks[i], mat, go, ge, my, myb, max_pos, r, flat, rows, flat_cap, rows_cap);
     uint8_t* tflat = flat;
     uint32_t *trows = rows, *tops = ops, tops_cap = ops_cap;
     if (r.flags) {  // band or traceback beyond the lane's storage: once more with a retry slot

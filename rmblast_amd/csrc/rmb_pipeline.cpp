@@ -512,6 +512,7 @@ static void run_dp_traceback_2pass(rmb_ctx* c, const std::vector<DpTask>& tasks,
 // take the two-pass path.
 constexpr uint32_t kOnePassRows = 8192, kOnePassFlat = 256u << 10, kOnePassOps = 4096;
 constexpr unsigned long long kOnePassPool = 64ull << 20;  // packed ops per launch
+constexpr uint32_t kOnePassPerCu = 7;  // 64-lane workgroups with ~21 KB of LDS: seven fit a CU
 // retry slots for halves beyond the LDS ring or the lane arena: global rings of
 // 2^kRingBigLog2 cells and 70 MB arenas (RMB_TB_RETRY_SLOTS lowers the count,
 // down to 0, which sends every such half to the two-pass path)
@@ -544,7 +545,7 @@ static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::
     const uint32_t n = (uint32_t)std::min(chunk, sorted.size() - base);
     DpTask* dt = c->b_tasks.get<DpTask>(n);
     OnePassResult* dr = c->b_opres.get<OnePassResult>(n);
-    const uint32_t blocks = std::min<uint32_t>((uint32_t)c->cu_count * 4, (n + 63) / 64);
+    const uint32_t blocks = std::min<uint32_t>((uint32_t)c->cu_count * kOnePassPerCu, (n + 63) / 64);
     const uint64_t stride = onepass_stride(kOnePassFlat, kOnePassRows, kOnePassOps);
     uint8_t* arena = c->b_arena.get<uint8_t>((size_t)blocks * 64 * stride);
     uint32_t* pool = c->b_oppool.get<uint32_t>(kOnePassPool);
