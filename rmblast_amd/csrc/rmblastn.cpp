@@ -1,21 +1,25 @@
 // rmblastn front end over librmblast_hip (include/rmblast_hip.h).
 //
 // Keeps the reference CLI's flags and NCBI spellings (rmblastn/src/main.rs:64-281)
-// and its outfmt 6 fields (rmblast-lib/src/output.rs:142-363).  The GPU library
+// its outfmt 6 fields (rmblast-lib/src/output.rs:142-363) and its pairwise
+// output, outfmt 0, the default (output.rs:365-720).  The GPU library
 // returns each (query, subject) pair's final HSPs; this program does what the
 // reference does after the search: mask level across subjects
 // (engine.rs:863-974), NCBI output order (main.rs:1230-1234, engine.rs:998-1017),
-// alignment statistics (stats.rs:52-243) and the tabular writer.
+// alignment statistics (stats.rs:52-243) and the writers.
 #include <algorithm>
 #include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <ctime>
 #include <fstream>
 #include <map>
 #include <sstream>
 #include <string>
 #include <vector>
+
+#include <sys/stat.h>
 
 #include "../../include/rmblast_hip.h"
 
@@ -33,7 +37,8 @@ uint8_t code_of(char ch) {
   }
 }
 
-struct Record { std::string id; std::vector<uint8_t> bases; };
+// defline: the header line after '>' with its line ending removed
+struct Record { std::string id, defline; std::vector<uint8_t> bases; };
 
 bool load_fasta(const std::string& path, std::vector<Record>& out, std::string& err) {
   std::ifstream f(path, std::ios::binary);
@@ -46,7 +51,8 @@ bool load_fasta(const std::string& path, std::vector<Record>& out, std::string& 
     if (!line.empty() && line[0] == '>') {
       if (open) out.push_back(std::move(cur));
       cur = Record{};
-      std::istringstream is(line.substr(1));
+      cur.defline = line.substr(1);
+      std::istringstream is(cur.defline);
       is >> cur.id;
       if (cur.id.empty()) { err = "'>' with no sequence id"; return false; }
       open = true;
@@ -61,7 +67,7 @@ bool load_fasta(const std::string& path, std::vector<Record>& out, std::string& 
   return true;
 }
 
-struct Stats { uint32_t mism = 0, qgap = 0, sgap = 0, ti = 0, tv = 0, cpg = 0; double kdiv = 0, cpgk = 0; float ps = 0, pq = 0, pd = 0; };
+struct Stats { uint32_t ident = 0, mism = 0, qgap = 0, sgap = 0, ti = 0, tv = 0, cpg = 0; double kdiv = 0, cpgk = 0; float ps = 0, pq = 0, pd = 0; };
 
 // stats.rs:52-243 with the subject as the ancestral sequence.
 Stats align_stats(const std::string& q, const std::string& s) {
@@ -102,7 +108,7 @@ Stats align_stats(const std::string& q, const std::string& s) {
     }
     const int k = kind(a, d);
     const bool both = pure(a) && pure(d);
-    if (k == ID) { if (both) ++pairs; }
+    if (k == ID) { ++st.ident; if (both) ++pairs; }
     else if (k == TI) { ++st.mism; if (both) { ++pairs; ++st.ti; cti += 1.0; } }
     else if (k == TV) { ++st.mism; if (both) { ++pairs; ++st.tv; } }
     else if (k == IUBS) ++st.mism;
@@ -135,6 +141,180 @@ std::string norm(std::string a) {
   if (a.size() > 1 && a[0] == '-' && a[1] != '-') a = "-" + a;
   for (auto& c : a) if (c == '_') c = '-';
   return a;
+}
+
+// ---------------------------------------------------------------- outfmt 0 --
+
+// 11207 -> "11,207" (output.rs:28-37)
+std::string commas(uint64_t n) {
+  const std::string d = std::to_string(n);
+  std::string o;
+  for (size_t i = 0; i < d.size(); ++i) {
+    if (i && (d.size() - i) % 3 == 0) o += ',';
+    o += d[i];
+  }
+  return o;
+}
+
+bool ascii_space(char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\f' || c == '\r'; }
+
+// NCBI's flat-file wrap of the database title (output.rs:44-102): each line
+// ends at the best break within `width` columns -- a newline, else the last
+// space, else just after the last comma, else just after the last hyphen,
+// else wherever the width runs out; spaces after a space break are dropped.
+std::vector<std::string> wrap_flat(const std::string& s, size_t width) {
+  enum { kForced, kHyphen, kComma, kSpace, kNewline };
+  std::vector<std::string> lines;
+  const size_t len = s.size();
+  size_t pos = 0;
+  while (pos < len) {
+    size_t nl = s.find('\n', pos);
+    if (nl == std::string::npos) nl = len;
+    const size_t start = nl - pos <= width ? nl : pos;
+    size_t col = 0, at = SIZE_MAX, p = start;
+    int best = kForced;
+    for (; p < len && col <= width; ++p, ++col) {
+      const char c = s[p];
+      if (c == '\n') { at = p; best = kNewline; break; }
+      int kind = kForced;
+      size_t cut = p;
+      if (ascii_space(c)) kind = kSpace;
+      else if (c == ',' && col < width && p + 1 < len) { kind = kComma; ++cut; }
+      else if (c == '-' && p + 1 < len && col < width) { kind = kHyphen; ++cut; }
+      if (kind >= best && cut > start) { at = cut; best = kind; }
+    }
+    if (best != kNewline && col <= width) at = len;
+    at = std::max(std::min(at, len), pos + 1);
+    lines.push_back(s.substr(pos, at - pos));
+    pos = at;
+    if (best == kSpace) {
+      while (pos < len && s[pos] == ' ') ++pos;
+      if (pos < len && s[pos] == '\n') ++pos;
+    } else if (best == kNewline) {
+      ++pos;
+    }
+  }
+  if (lines.empty()) lines.emplace_back();
+  return lines;
+}
+
+// Subject title wrap (output.rs:108-126): once a line passes 60 characters,
+// break after the next whitespace, which stays on the line.
+std::string wrap_title(const std::string& s) {
+  if (s.size() <= 60) return s;
+  std::string o;
+  bool due = false;
+  for (size_t i = 0; i < s.size(); ++i) {
+    if (i && i % 60 == 0) due = true;
+    o += s[i];
+    if (due && ascii_space(s[i])) { o += '\n'; due = false; }
+  }
+  return o;
+}
+
+// NCBI's GetPercentMatch (output.rs:369-374): 100 only when n == d
+unsigned percent_match(uint32_t n, uint32_t d) {
+  if (d == 0) return 0;
+  if (n == d) return 100;
+  return std::min(99u, (unsigned)(0.5 + 100.0 * n / d));
+}
+
+std::string header_block(const std::string& db_name, size_t n_seqs, uint64_t letters) {  // output.rs:431-452
+  std::string o = "RMBLASTN 2.17.1+\n\n\n"
+                  "Reference: Robert M. Hubley, Arian Smit\n"
+                  "RMBlast - RepeatMasker Search Engine\n"
+                  "2010 <http://www.repeatmasker.org>\n\n"
+                  "Reference: Stephen F. Altschul, Thomas L. Madden, Alejandro A.\n"
+                  "Schaffer, Jinghui Zhang, Zheng Zhang, Webb Miller, and David J.\n"
+                  "Lipman (1997), \"Gapped BLAST and PSI-BLAST: a new generation of\n"
+                  "protein database search programs\", Nucleic Acids Res. 25:3389-3402.\n\n\n\n"
+                  "Database: ";
+  for (const auto& l : wrap_flat(db_name, 68)) o += l + "\n";
+  o += "           " + commas(n_seqs) + " sequences; " + commas(letters) + " total letters\n\n\n\n";
+  return o;
+}
+
+// "Posted date" line value from the database file's modification time, in
+// UTC: "Mon D, YYYY  H:MM AM/PM" (output.rs:383-420)
+std::string posted_date(time_t t) {
+  static const char* kMon[12] = {"Jan", "Feb", "Mar", "Apr", "May", "Jun", "Jul", "Aug", "Sep", "Oct", "Nov", "Dec"};
+  struct tm u;
+  gmtime_r(&t, &u);
+  const int h12 = u.tm_hour % 12 == 0 ? 12 : u.tm_hour % 12;
+  char b[64];
+  snprintf(b, sizeof b, "%s %d, %d  %d:%02d %s", kMon[u.tm_mon], u.tm_mday, u.tm_year + 1900, h12, u.tm_min,
+           u.tm_hour < 12 ? "AM" : "PM");
+  return b;
+}
+
+std::string footer_block(const std::string& db_name, size_t n_seqs, uint64_t letters, const time_t* mtime,
+                         int gap_open, int gap_extend) {  // output.rs:652-680
+  std::string o = "  Database: ";
+  for (const auto& l : wrap_flat(db_name, 68)) o += l + "\n";
+  if (mtime) o += "    Posted date:  " + posted_date(*mtime) + "\n";
+  o += "  Number of letters in database: " + commas(letters) + "\n";
+  o += "  Number of sequences in database:  " + commas(n_seqs) + "\n\n\n\n";
+  char b[96];
+  snprintf(b, sizeof b, "Matrix: blastn matrix 0 0\nGap Penalties: Existence: %d, Extension: %.2f\n", gap_open,
+           (double)gap_extend);
+  return o + b;
+}
+
+// One HSP (output.rs:510-649): the statistics block, then 60-column rows of
+// query, match line and subject.  Query bases inside the DUST intervals
+// `mask` ([first, last] pairs, sorted) print in lower case.
+void hsp_block(std::string& o, const Row& r, const Stats& st, const std::vector<uint64_t>& mask) {
+  const rmb_hsp& h = r.h;
+  const uint32_t n = (uint32_t)r.qa.size(), gaps = st.qgap + st.sgap;
+  char b[256];
+  snprintf(b, sizeof b,
+           " Score = %d\n Substitutions = %.2f%%, Query Gaps = %.2f%%, DB Gaps = %.2f%%\n"
+           " TransI/TransV = %u/%u,  CpG_sites = %u\n Kimura Div = %.2f%%, Kimura Div CpG Adjusted = %.2f%%\n",
+           h.score, (double)st.ps, (double)st.pq, (double)st.pd, st.ti, st.tv, st.cpg, st.kdiv, st.cpgk);
+  o += b;
+  snprintf(b, sizeof b, " Identities = %u/%u (%u%%), Gaps = %u/%u (%u%%)\n Strand=%s\n\n", st.ident, n,
+           percent_match(st.ident, n), gaps, n, percent_match(gaps, n), h.strand ? "Plus/Minus" : "Plus/Plus");
+  o += b;
+  std::string qa = r.qa;
+  if (!mask.empty()) {
+    const size_t nm = mask.size() / 2;
+    size_t m = 0;
+    uint64_t q = h.q_start;
+    while (m < nm && mask[2 * m + 1] < q) ++m;
+    for (char& c : qa) {
+      if (c == '-') continue;
+      while (m < nm && mask[2 * m + 1] < q) ++m;
+      if (m < nm && mask[2 * m] <= q) c = (char)std::tolower((unsigned char)c);
+      ++q;
+    }
+  }
+  const uint32_t top = std::max(h.q_end, h.s_end);
+  const int w = (int)std::to_string(top).size();
+  uint32_t qp = h.q_start + 1, sp = h.strand ? h.s_end : h.s_start + 1;
+  for (uint32_t i = 0; i < n; i += 60) {
+    const uint32_t e = std::min(n, i + 60);
+    uint32_t qn = 0, sn = 0;
+    std::string mid;
+    for (uint32_t k = i; k < e; ++k) {
+      const char q = qa[k], s = r.sa[k];
+      qn += q != '-';
+      sn += s != '-';
+      mid += (q != '-' && s != '-' && std::toupper((unsigned char)q) == std::toupper((unsigned char)s)) ? '|' : ' ';
+    }
+    const uint32_t qe = qn ? qp + qn - 1 : qp;
+    const uint32_t se = sn ? (h.strand ? sp - (sn - 1) : sp + sn - 1) : sp;
+    snprintf(b, sizeof b, "Query  %-*u  ", w, qp);
+    o += b;
+    o.append(qa, i, e - i);
+    o += "  " + std::to_string(qe) + "\n" + std::string(9 + w, ' ') + mid + "\n";
+    snprintf(b, sizeof b, "Sbjct  %-*u  ", w, sp);
+    o += b;
+    o.append(r.sa, i, e - i);
+    o += "  " + std::to_string(se) + "\n\n";
+    qp += qn;
+    sp = h.strand ? (sp >= sn ? sp - sn : 0) : sp + sn;
+  }
+  o += '\n';
 }
 
 }  // namespace
@@ -174,12 +354,21 @@ int main(int argc, char** argv) {
   P.dust = dust_on;
   int mask_level = std::stoi(opt("--mask-level", "-1"));
   if (mask_level < 0) mask_level = 101;
+  // --outfmt: "0" pairwise (the default) or "6 <fields>" (main.rs:201-206,380-391)
   std::vector<std::string> fields;
+  bool pairwise = false;
   {
-    std::istringstream is(opt("--outfmt", "6 score qseqid qstart qend sstrand sseqid sstart send"));
+    const std::string spec = opt("--outfmt", "0");
+    std::istringstream is(spec);
     std::string w;
     is >> w;
-    if (w != "6") { fprintf(stderr, "error: only -outfmt 6 <fields> is supported\n"); return 1; }
+    if (w == "0") {
+      pairwise = true;
+      if (is >> w) { fprintf(stderr, "error: --outfmt '%s': format 0 (pairwise) takes no fields\n", spec.c_str()); return 1; }
+    } else if (w != "6") {
+      fprintf(stderr, "error: --outfmt '%s': only 0 and 6 <fields> are supported\n", spec.c_str());
+      return 1;
+    }
     while (is >> w) fields.push_back(w);
   }
   std::string err;
@@ -248,6 +437,10 @@ int main(int argc, char** argv) {
     per_query[h.query].push_back(std::move(r));
   }
   std::string out;
+  uint64_t db_letters = 0;
+  for (const Record& r : db) db_letters += r.bases.size();
+  const std::string& db_name = kv["--db"];
+  if (pairwise) out += header_block(db_name, db.size(), db_letters);
   for (size_t qi = 0; qi < qs.size(); ++qi) {
     std::vector<Row>& rows = per_query[qi];
     auto qoff = [](const Row& r) { return r.h.strand ? r.q_len - r.h.q_end : r.h.q_start; };
@@ -309,6 +502,37 @@ int main(int argc, char** argv) {
       if (a.h.subject != b.h.subject) return a.h.subject > b.h.subject;
       return better(a, b) < 0;
     });
+    if (pairwise) {  // output.rs:685-720; rows are grouped by subject already
+      std::string ql = qs[qi].defline;
+      while (!ql.empty() && ascii_space(ql.back())) ql.pop_back();
+      out += "Query= " + ql + "\n\nLength=" + std::to_string(qs[qi].bases.size()) + "\n\n";
+      if (rows.empty()) {
+        out += "\n***** No hits found *****\n\n\n\n\n";
+        continue;
+      }
+      std::vector<uint64_t> mask;
+      if (dust_on) {
+        const Record& Q = qs[qi];
+        uint64_t nm = 0;
+        if (rmb_dust_intervals(Q.bases.data(), Q.bases.size(), nullptr, &nm)) return fail("dust");
+        mask.resize(2 * nm);
+        if (nm && rmb_dust_intervals(Q.bases.data(), Q.bases.size(), mask.data(), &nm)) return fail("dust");
+      }
+      for (size_t k = 0; k < rows.size(); ++k) {
+        const Row& r = rows[k];
+        if (k == 0 || r.h.subject != rows[k - 1].h.subject) {
+          // ">id title": the defline past the id and one character (output.rs:487-508)
+          const Record& S = db[r.h.subject];
+          std::string title;
+          if (S.defline.compare(0, S.id.size(), S.id) == 0 && S.defline.size() > S.id.size())
+            title = S.defline.substr(S.id.size() + 1);
+          out += ">" + S.id + " " + wrap_title(title) + "\nLength=" + std::to_string(r.s_len) + "\n\n";
+        }
+        hsp_block(out, r, align_stats(r.qa, r.sa), mask);
+      }
+      out += "\n\n";
+      continue;
+    }
     char buf[64];
     for (const Row& r : rows) {
       const Stats st = align_stats(r.qa, r.sa);
@@ -342,6 +566,11 @@ int main(int argc, char** argv) {
       }
       out += '\n';
     }
+  }
+  if (pairwise) {
+    struct stat sb;
+    const bool have = stat(db_name.c_str(), &sb) == 0;
+    out += footer_block(db_name, db.size(), db_letters, have ? &sb.st_mtime : nullptr, P.gap_open, P.gap_extend);
   }
   fwrite(out.data(), 1, out.size(), stdout);
   if (std::getenv("RMB_STATS")) {

@@ -3,6 +3,8 @@ rmblastn front end on top of it) against the NCBI golden and the CPU oracle.
 Integer work throughout, so every comparison is byte/bit exact."""
 import ctypes
 import os
+import shutil
+import subprocess
 
 import numpy as np
 import pytest
@@ -279,3 +281,60 @@ def test_dust_on_synthetic_matches_oracle(tmp_path):
     assert g.returncode == 0 and o.returncode == 0, (g.stderr, o.stderr)
     assert g.stdout == o.stdout
     assert g.stdout.count("\n") > 50
+
+
+# ---- pairwise output (outfmt 0, the default) ------------------------------------------
+RM_PAIRWISE = ["--word-size", "14", "--mask-level", "80", "--complexity-adjust", "--min-raw-gapped-score", "200",
+               "--xdrop-ungap", "400", "--xdrop-gap-final", "200", "--xdrop-gap", "100"]
+
+
+def pairwise_run(cwd, db, extra=()):
+    r = subprocess.run([CLI, "--db", db, "--query", "query.fa", "--matrix", "comparison.matrix", "--gapopen", "20",
+                        "--gapextend", "5", "--num-threads", "2", *extra],
+                       cwd=str(cwd), capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+@pytest.mark.parametrize("name,extra", [("defaults", []), ("rm", RM_PAIRWISE)])
+def test_pairwise_goldens_on_gpu(tmp_path, name, extra):
+    """NCBI 2.17.1 pairwise output on the fixtures, run from their directory
+    so the database title is "db.fa"; the "Posted date" line (the database
+    file's time) is dropped on both sides (rmblastn/tests/cli.rs:190-212)."""
+    for f in ("db.fa", "query.fa", "comparison.matrix"):
+        shutil.copy(golden(f), tmp_path / f)
+    out = pairwise_run(tmp_path, "db.fa", extra)
+    got = "".join(l + "\n" for l in out.splitlines() if not l.startswith("    Posted date:"))
+    assert got == open(golden(f"expected_pairwise_{name}.ncbi-2.17.1.txt")).read()
+
+
+def test_pairwise_titles_wrap_like_ncbi(tmp_path):
+    """The known answers of rmblast-lib/src/output.rs:735-765: a database path
+    wrapped at 68 columns after a hyphen, a subject title broken after the
+    first space past each 60th character, a long hyphenated word never
+    broken, an id-only defline printed with its trailing space."""
+    shutil.copy(golden("query.fa"), tmp_path / "query.fa")
+    shutil.copy(golden("comparison.matrix"), tmp_path / "comparison.matrix")
+    recs = []
+    for line in open(golden("db.fa")).read().splitlines():
+        if line.startswith(">"):
+            recs.append([line[1:], []])
+        else:
+            recs[-1][1].append(line)
+    long5 = "id5 x averyveryveryveryveryveryveryveryveryveryveryveryverylongwordwithnospaces-and-hyphens-here-too"
+    recs[0][0] = "id3 " + "a very long description with many words " * 4
+    recs[1][0] = long5
+    recs[2][0] = "onlyid"
+    d = "databases/repeat-libraries/2026-11-release/curated-families-with-long-names"
+    (tmp_path / d).mkdir(parents=True)
+    (tmp_path / d / "db.fa").write_text("".join(">" + h + "\n" + "\n".join(s) + "\n" for h, s in recs))
+    out = pairwise_run(tmp_path, d + "/db.fa")
+    wrapped = "databases/repeat-libraries/2026-11-release/curated-families-with-\nlong-names/db.fa\n"
+    assert out.startswith("RMBLASTN 2.17.1+\n")
+    assert "\nDatabase: " + wrapped + "           17 sequences; 11,207 total letters\n" in out
+    assert "\n  Database: " + wrapped in out
+    assert ("\n>id3 a very long description with many words a very long description \n"
+            "with many words a very long description with many words a \n"
+            "very long description with many words \nLength=") in out
+    assert "\n>" + long5 + "\nLength=" in out
+    assert "\n>onlyid \nLength=" in out
