@@ -15,6 +15,7 @@
 #include <ctime>
 #include <fstream>
 #include <map>
+#include <set>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -362,14 +363,33 @@ int main(int argc, char** argv) {
     std::istringstream is(spec);
     std::string w;
     is >> w;
+    static const char* kFields[] = {"score", "perc_sub", "perc_query_gap", "perc_db_gap", "qseqid", "qstart",
+                                    "qend", "qlen", "sstrand", "sseqid", "sstart", "send", "slen", "kdiv",
+                                    "cpg_kdiv", "transi", "transv", "cpg_sites", "qseq", "sseq", "evalue",
+                                    "bitscore"};
+    std::string names;
+    for (const char* f : kFields) names += std::string(names.empty() ? "" : " ") + f;
     if (w == "0") {
       pairwise = true;
       if (is >> w) { fprintf(stderr, "error: --outfmt '%s': format 0 (pairwise) takes no fields\n", spec.c_str()); return 1; }
-    } else if (w != "6") {
-      fprintf(stderr, "error: --outfmt '%s': only 0 and 6 <fields> are supported\n", spec.c_str());
+    } else if (w != "6") {  // output.rs:223-242
+      fprintf(stderr, "error: --outfmt '%s': output format '%s' is not supported; use 0 (pairwise) or 6 <fields> (tabular)\n",
+              spec.c_str(), w.c_str());
       return 1;
     }
-    while (is >> w) fields.push_back(w);
+    while (is >> w) {
+      if (std::find_if(std::begin(kFields), std::end(kFields), [&](const char* f) { return w == f; }) == std::end(kFields)) {
+        fprintf(stderr, "error: --outfmt '%s': unknown -outfmt field '%s'. Supported fields: %s\n", spec.c_str(), w.c_str(),
+                names.c_str());
+        return 1;
+      }
+      fields.push_back(w);
+    }
+    if (!pairwise && fields.empty()) {
+      fprintf(stderr, "error: --outfmt '%s': -outfmt 6 needs a field list; NCBI's default columns are not supported. "
+              "Fields: %s\n", spec.c_str(), names.c_str());
+      return 1;
+    }
   }
   std::string err;
   rmb_matrix M;
@@ -381,6 +401,28 @@ int main(int argc, char** argv) {
   if (!load_fasta(kv["--db"], db, err) || !load_fasta(kv["--query"], qs, err)) {
     fprintf(stderr, "error: %s\n", err.c_str());
     return 1;
+  }
+  if (db.empty()) { fprintf(stderr, "error: database '%s' contains no sequences\n", kv["--db"].c_str()); return 1; }
+  if (kv.count("--gilist")) {  // main.rs:578-600, 1444-1470: subjects named in the file, one per line
+    std::ifstream f(kv["--gilist"]);
+    if (!f) { fprintf(stderr, "error: opening gilist '%s'\n", kv["--gilist"].c_str()); return 1; }
+    std::set<std::string> keep;
+    std::string line;
+    while (std::getline(f, line)) {
+      const size_t a = line.find_first_not_of(" \t\r\n\f\v"), b = line.find_last_not_of(" \t\r\n\f\v");
+      if (a == std::string::npos) continue;
+      const std::string e = line.substr(a, b - a + 1);
+      // a bare GI number also names the sequence "gi|<n>"
+      if (std::all_of(e.begin(), e.end(), [](char c) { return c >= '0' && c <= '9'; })) keep.insert("gi|" + e);
+      keep.insert(e);
+    }
+    std::vector<Record> sel;
+    for (Record& r : db)
+      if (keep.count(r.id)) sel.push_back(std::move(r));
+    db.swap(sel);
+    if (db.empty())
+      fprintf(stderr, "Warning: [rmblastn] the gilist '%s' matches no sequence in the database; nothing to search\n",
+              kv["--gilist"].c_str());
   }
   std::map<std::string, size_t> oid;
   for (size_t i = 0; i < db.size(); ++i)
@@ -395,22 +437,23 @@ int main(int argc, char** argv) {
     return 1;
   };
   if (rc) return fail("device");
-  {
+  uint64_t nh = 0, no = 0;
+  std::vector<rmb_hsp> H;
+  std::vector<rmb_op> O;
+  if (!db.empty() && !qs.empty()) {  // a gilist may select no subject: a search with no hits
     std::vector<const uint8_t*> ptr;
     std::vector<uint32_t> len;
     for (auto& r : db) { ptr.push_back(r.bases.data()); len.push_back((uint32_t)r.bases.size()); }
     if (rmb_library_load(ctx, (uint32_t)db.size(), ptr.data(), len.data())) return fail("library");
+    std::vector<const uint8_t*> qp;
+    std::vector<uint32_t> ql;
+    for (auto& r : qs) { qp.push_back(r.bases.data()); ql.push_back((uint32_t)r.bases.size()); }
+    if (rmb_search(ctx, (uint32_t)qs.size(), qp.data(), ql.data())) return fail("search");
+    rmb_results_size(ctx, &nh, &no);
+    H.resize(nh);
+    O.resize(no);
+    rmb_results_copy(ctx, H.data(), O.data());
   }
-  std::vector<const uint8_t*> qp;
-  std::vector<uint32_t> ql;
-  for (auto& r : qs) { qp.push_back(r.bases.data()); ql.push_back((uint32_t)r.bases.size()); }
-  if (qs.empty()) { rmb_ctx_destroy(ctx); return 0; }
-  if (rmb_search(ctx, (uint32_t)qs.size(), qp.data(), ql.data())) return fail("search");
-  uint64_t nh = 0, no = 0;
-  rmb_results_size(ctx, &nh, &no);
-  std::vector<rmb_hsp> H(nh);
-  std::vector<rmb_op> O(no);
-  rmb_results_copy(ctx, H.data(), O.data());
   std::vector<std::vector<Row>> per_query(qs.size());
   for (const rmb_hsp& h : H) {
     const Record& Q = qs[h.query];
@@ -574,7 +617,7 @@ int main(int argc, char** argv) {
   }
   fwrite(out.data(), 1, out.size(), stdout);
   if (std::getenv("RMB_STATS")) {
-    uint64_t s[16];
+    uint64_t s[16] = {0};
     rmb_stats(ctx, s, 16);
     fprintf(stderr, "seed_evals %llu ungapped_hits %llu prelim_dps %llu tb_dps %llu dp_cells %llu hsps %llu "
             "ung_steps %llu t_seed_us %llu t_prelim_us %llu t_tb_us %llu t_total_us %llu\n",

@@ -65,3 +65,31 @@ def test_missing_library_fails_loudly(tmp_path):
     import rmblast_amd
     with pytest.raises(rmblast_amd.RmbError):
         rmblast_amd.load_library(str(tmp_path / "nope.so"))
+
+
+@pytest.mark.parametrize("fmt,needle", [
+    ("6", "needs a field list"),
+    ("7 score", "output format '7' is not supported"),
+    ("10 score qseqid", "output format '10' is not supported"),
+    ("6 score pident qseqid", "unknown -outfmt field 'pident'"),
+    ("6 std", "unknown -outfmt field 'std'"),
+    ("0 score", "takes no fields"),
+])
+def test_cli_rejects_unsupported_outfmt(fmt, needle):
+    """The front end refuses these before it touches a device, naming the
+    problem (rmblastn/tests/cli.rs:465-485)."""
+    import subprocess
+    g = os.path.join(ROOT, "tests", "golden")
+    r = subprocess.run([CLI, "--db", os.path.join(g, "db.fa"), "--query", os.path.join(g, "query.fa"), "--matrix",
+                        os.path.join(g, "comparison.matrix"), "--gapopen", "20", "--gapextend", "5", "--outfmt", fmt],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and needle in r.stderr and r.stdout == ""
+
+
+def test_cli_requires_gap_costs():
+    import subprocess
+    g = os.path.join(ROOT, "tests", "golden")
+    r = subprocess.run([CLI, "--db", os.path.join(g, "db.fa"), "--query", os.path.join(g, "query.fa"), "--matrix",
+                        os.path.join(g, "comparison.matrix"), "--gapopen", "20"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "--gapextend" in r.stderr

@@ -338,3 +338,30 @@ def test_pairwise_titles_wrap_like_ncbi(tmp_path):
             "very long description with many words \nLength=") in out
     assert "\n>" + long5 + "\nLength=" in out
     assert "\n>onlyid \nLength=" in out
+
+
+def test_gilist_selects_subjects(tmp_path):
+    """--gilist keeps the subjects it names (lines trimmed; a bare number also
+    names "gi|<n>"); one that names nothing is a search with no hits and a
+    warning (rmblastn/src/main.rs:578-600,1444-1470; rmblastn/tests/cli.rs:453-463)."""
+    recs = []
+    for line in open(golden("db.fa")).read().splitlines():
+        if line.startswith(">"):
+            recs.append([f"gi|{len(recs) + 1} " + line[1:], []])
+        else:
+            recs[-1][1].append(line)
+    db = tmp_path / "db.fa"
+    db.write_text("".join(">" + h + "\n" + "\n".join(s) + "\n" for h, s in recs))
+    args = ["--db", str(db), "--query", golden("query.fa"), "--matrix", golden("comparison.matrix"), "--gapopen", "20",
+            "--gapextend", "5", "--outfmt", "6 score qseqid qstart qend sstrand sseqid sstart send"]
+    full = run([CLI] + args)
+    assert full.returncode == 0, full.stderr
+    for text, keep in (("3\n  gi|7 \n\n", {"gi|3", "gi|7"}), ("no-such-id\n", set())):
+        gl = tmp_path / "gilist"
+        gl.write_text(text)
+        r = run([CLI] + args + ["--gilist", str(gl)])
+        assert r.returncode == 0, r.stderr
+        want = "".join(l + "\n" for l in full.stdout.splitlines() if l.split("\t")[5] in keep)
+        assert r.stdout == want
+        assert bool(keep) or "matches no sequence" in r.stderr
+    assert sum(1 for l in full.stdout.splitlines() if l.split("\t")[5] in {"gi|3", "gi|7"}) > 5
