@@ -73,6 +73,7 @@ struct UngOut { int32_t score; uint32_t q_beg, q_end, steps; };
 // taken back, as the reference tests for the sentinel before counting a step.
 static constexpr int32_t kSentScore = -(1 << 20);
 static constexpr int kLongChunk = 8;  // steps per chunk of the full extension
+static constexpr int kFlankSteps = 5;  // flank steps per side of the fast first tier
 
 // Ungapped X-drop from (q_off, s_off) over sentinel-wrapped sequences: left
 // side fixed X, right side adaptive X (ungapped.rs:223-306).  qa / sa are the
@@ -152,9 +153,12 @@ __device__ __forceinline__ UngOut ungapped_chunked(const uint8_t* __restrict__ q
 // (6 word steps on the left, 1 on the right): only the sum of the word's
 // scores enters the result, so a minus seed is stepped in a frame moved 6
 // bases left, where its word sits where a plus seed's does and both strands
-// run the same code.  Then eight branch-free flank steps per side, from the
-// 32 bytes [b - 16, b + 16) of each sequence around the frame start b (two
-// unaligned 16-byte loads each: gfx950 global loads take any byte address).
+// run the same code.  Then kFlankSteps branch-free flank steps per side, from
+// the 24 bytes [b - 8, b + 16) of each sequence around the frame start b (an
+// unaligned 8-byte and 16-byte load each: gfx950 global loads take any byte
+// address).  Five steps suit random-like flanks, where a side ends after
+// about three: each step costs every lane of the wave, and seeds that need
+// more go to the full extension over a compact list.
 // True: both sides ended within those steps and `out`
 // is the full extension's result; false: a side still runs (or the frame
 // starts within 16 bytes of its pool's start), and the caller runs the full
@@ -167,18 +171,21 @@ __device__ __forceinline__ bool ungapped_first(const uint8_t* __restrict__ qpool
   sa -= shift;
   q_off -= shift;
   if (qa < 16 || sa < 16) return false;
-  uint4 qa16, qb16, sa16, sb16;
-  __builtin_memcpy(&qa16, qpool + qa - 16, 16);
+  static_assert(kFlankSteps <= 8, "the flank windows hold 8 bytes per side");
+  uint2 qa8, sa8;
+  uint4 qb16, sb16;
+  __builtin_memcpy(&qa8, qpool + qa - 8, 8);
   __builtin_memcpy(&qb16, qpool + qa, 16);
-  __builtin_memcpy(&sa16, spool + sa - 16, 16);
+  __builtin_memcpy(&sa8, spool + sa - 8, 8);
   __builtin_memcpy(&sb16, spool + sa, 16);
-  // index bytes (q << 4 | s); byte 16 is the frame start
+  // index bytes (q << 4 | s) at byte positions 8..31; byte 16 is the frame start
   uint32_t ix[8];
   {
-    const uint32_t qw[8] = {qa16.x, qa16.y, qa16.z, qa16.w, qb16.x, qb16.y, qb16.z, qb16.w};
-    const uint32_t sw[8] = {sa16.x, sa16.y, sa16.z, sa16.w, sb16.x, sb16.y, sb16.z, sb16.w};
+    const uint32_t qw[6] = {qa8.x, qa8.y, qb16.x, qb16.y, qb16.z, qb16.w};
+    const uint32_t sw[6] = {sa8.x, sa8.y, sb16.x, sb16.y, sb16.z, sb16.w};
+    ix[0] = ix[1] = 0;
 #pragma unroll
-    for (int w = 0; w < 8; ++w) ix[w] = ((qw[w] << 4) & 0xF0F0F0F0u) | (sw[w] & 0x0F0F0F0Fu);
+    for (int w = 0; w < 6; ++w) ix[w + 2] = ((qw[w] << 4) & 0xF0F0F0F0u) | (sw[w] & 0x0F0F0F0Fu);
   }
   auto sc = [&](int p) { return smat[(ix[p >> 2] >> (8 * (p & 3))) & 0xFFu]; };
   // the word's steps: bytes 16..22
@@ -186,12 +193,12 @@ __device__ __forceinline__ bool ungapped_first(const uint8_t* __restrict__ qpool
 #pragma unroll
   for (int k = 16; k < 23; ++k) known += sc(k);
   const int32_t x = -xdrop;
-  // left: 8 flank steps from byte 15 down
+  // left: flank steps from byte 15 down
   int32_t score = 0, sum = 0, q_beg = q_off;
   uint32_t steps = 0;
   bool run = true;
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
+  for (int j = 0; j < kFlankSteps; ++j) {
     const int32_t ns = sum + sc(15 - j);
     const bool up = run && ns > 0;
     const bool stop = run && ns < x;
@@ -203,7 +210,7 @@ __device__ __forceinline__ bool ungapped_first(const uint8_t* __restrict__ qpool
   }
   if (run) return false;
   if (sum <= kSentScore / 2) --steps;  // ended on a sentinel: that step is not counted
-  // right: the word's steps, then 8 flank steps from byte 23 up
+  // right: the word's steps, then flank steps from byte 23 up
   score += known;
   steps += 7u;
   const int32_t qe0 = q_off + 7;
@@ -211,7 +218,7 @@ __device__ __forceinline__ bool ungapped_first(const uint8_t* __restrict__ qpool
   sum = 0;
   run = true;
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
+  for (int j = 0; j < kFlankSteps; ++j) {
     const int32_t ns = sum + sc(23 + j);
     const bool up = run && ns > 0;
     const bool stop = run && ns < xc;
