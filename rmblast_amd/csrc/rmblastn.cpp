@@ -14,6 +14,7 @@
 #include <cstring>
 #include <ctime>
 #include <fstream>
+#include <iterator>
 #include <map>
 #include <set>
 #include <sstream>
@@ -66,6 +67,88 @@ bool load_fasta(const std::string& path, std::vector<Record>& out, std::string& 
   }
   if (open) out.push_back(std::move(cur));
   return true;
+}
+
+// UCSC .2bit database (seq/twobit.rs:187-300): header, index of names and
+// record offsets, then per record dnaSize, N blocks, mask blocks (ignored: case
+// does not matter), a reserved word and the bases packed four to a byte, T C A
+// G = 0 1 2 3 from the high bits.  N-block bases read as N, which the library
+// resolves exactly as the reference resolves them (one NcbiRandom draw per
+// position, seeded by the length, 5' to 3': twobit.rs:346-384); blocks must
+// therefore be ascending and disjoint, as faToTwoBit writes them.
+bool load_2bit(const std::string& path, std::vector<Record>& out, std::string& err) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) { err = "cannot open " + path; return false; }
+  const std::string d((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+  bool swap = false;
+  auto r32 = [&](size_t pos, uint32_t& v) {
+    if (pos > d.size() || d.size() - pos < 4) return false;
+    uint32_t x;
+    memcpy(&x, d.data() + pos, 4);
+    v = swap ? __builtin_bswap32(x) : x;
+    return true;
+  };
+  uint32_t magic = 0, version = 0, count = 0;
+  if (!r32(0, magic)) { err = path + ": shorter than the 16-byte 2bit header"; return false; }
+  if (magic == 0x4327411Au) swap = true;
+  else if (magic != 0x1A412743u) { err = path + ": not a 2bit file"; return false; }
+  if (!r32(4, version) || !r32(8, count) || d.size() < 16) { err = path + ": truncated 2bit header"; return false; }
+  if (version != 0) { err = path + ": unsupported 2bit version"; return false; }
+  if (count > (d.size() - 16) / 5) { err = path + ": 2bit index larger than the file"; return false; }
+  static const uint8_t kFromUcsc[4] = {3, 1, 0, 2};
+  std::vector<std::pair<std::string, uint32_t>> index;
+  std::set<std::string> seen;
+  size_t pos = 16;
+  for (uint32_t i = 0; i < count; ++i) {
+    if (pos >= d.size()) { err = path + ": truncated 2bit index"; return false; }
+    const size_t nl = (uint8_t)d[pos++];
+    uint32_t off = 0;
+    if (d.size() - pos < nl || !r32(pos + nl, off)) { err = path + ": truncated 2bit index"; return false; }
+    index.emplace_back(d.substr(pos, nl), off);
+    if (!seen.insert(index.back().first).second) { err = path + ": duplicate name " + index.back().first; return false; }
+    pos += nl + 4;
+  }
+  for (const auto& [name, off] : index) {
+    uint32_t n = 0, nb = 0, mb = 0, reserved = 0;
+    if (!r32(off, n) || !r32((size_t)off + 4, nb)) { err = path + ": truncated record " + name; return false; }
+    const size_t ns = (size_t)off + 8, nz = ns + 4ull * nb;
+    if (!r32(nz + 4ull * nb, mb) || !r32(nz + 4ull * nb + 4 + 8ull * mb, reserved)) {
+      err = path + ": truncated record " + name;
+      return false;
+    }
+    const size_t dna = nz + 4ull * nb + 8 + 8ull * mb;
+    if (n > 0x7FFFFFFFu || d.size() < dna || d.size() - dna < (n + 3ull) / 4) {
+      err = path + ": record " + name + " runs past the end of the file";
+      return false;
+    }
+    Record r;
+    r.id = r.defline = name;
+    r.bases.resize(n);
+    for (uint32_t p = 0; p < n; ++p)
+      r.bases[p] = kFromUcsc[((uint8_t)d[dna + p / 4] >> (6 - 2 * (p % 4))) & 3];
+    uint32_t prev_end = 0;
+    for (uint32_t b = 0; b < nb; ++b) {
+      uint32_t s = 0, z = 0;
+      r32(ns + 4ull * b, s);
+      r32(nz + 4ull * b, z);
+      const uint32_t e = (uint32_t)std::min<uint64_t>((uint64_t)s + z, n), s0 = std::min(s, e);
+      if (s0 < prev_end) { err = path + ": N blocks of " + name + " overlap or are out of order"; return false; }
+      std::fill(r.bases.begin() + s0, r.bases.begin() + e, (uint8_t)14);
+      prev_end = e;
+    }
+    out.push_back(std::move(r));
+  }
+  return true;
+}
+
+bool load_db(const std::string& path, std::vector<Record>& out, std::string& err) {
+  std::ifstream f(path, std::ios::binary);
+  char m[4] = {0};
+  if (f && f.read(m, 4) && ((uint8_t)m[0] == 0x43 && (uint8_t)m[1] == 0x27 && (uint8_t)m[2] == 0x41 && (uint8_t)m[3] == 0x1A))
+    return load_2bit(path, out, err);
+  if (f && (uint8_t)m[0] == 0x1A && (uint8_t)m[1] == 0x41 && (uint8_t)m[2] == 0x27 && (uint8_t)m[3] == 0x43)
+    return load_2bit(path, out, err);
+  return load_fasta(path, out, err);
 }
 
 struct Stats { uint32_t ident = 0, mism = 0, qgap = 0, sgap = 0, ti = 0, tv = 0, cpg = 0; double kdiv = 0, cpgk = 0; float ps = 0, pq = 0, pd = 0; };
@@ -398,7 +481,7 @@ int main(int argc, char** argv) {
     if (rmb_matrix_load(opt("--matrix", "").c_str(), &M, mbuf, sizeof mbuf)) { fprintf(stderr, "error: matrix: %s\n", mbuf); return 1; }
   }
   std::vector<Record> db, qs;
-  if (!load_fasta(kv["--db"], db, err) || !load_fasta(kv["--query"], qs, err)) {
+  if (!load_db(kv["--db"], db, err) || !load_fasta(kv["--query"], qs, err)) {
     fprintf(stderr, "error: %s\n", err.c_str());
     return 1;
   }

@@ -112,3 +112,32 @@ def simple_matrix(match, mismatch, other=0, sentinel=None):
 def seq_from_fasta(path):
     with open(path) as f:
         return "".join(l.strip() for l in f if not l.startswith(">"))
+
+
+def write_2bit(path, records, swapped=False):
+    """UCSC .2bit file of (name, bases) records: bases as text, N runs stored
+    as N blocks, lower case as mask blocks, the rest packed T C A G = 0 1 2 3
+    from the high bits (the layout rmblast-lib/src/seq/twobit.rs:1-12 reads)."""
+    import re
+    import struct
+    e = ">" if swapped else "<"
+    code = {"T": 0, "C": 1, "A": 2, "G": 3, "N": 0}
+    names = [n.encode() for n, _ in records]
+    off = 16 + sum(1 + len(n) + 4 for n in names)
+    index, body = b"", b""
+    for (name, seq), nb in zip(records, names):
+        up = seq.upper()
+        nblk = [(m.start(), m.end() - m.start()) for m in re.finditer("N+", up)]
+        mblk = [(m.start(), m.end() - m.start()) for m in re.finditer("[a-z]+", seq)]
+        rec = struct.pack(e + "II", len(seq), len(nblk))
+        rec += b"".join(struct.pack(e + "I", s) for s, _ in nblk) + b"".join(struct.pack(e + "I", z) for _, z in nblk)
+        rec += struct.pack(e + "I", len(mblk))
+        rec += b"".join(struct.pack(e + "I", s) for s, _ in mblk) + b"".join(struct.pack(e + "I", z) for _, z in mblk)
+        rec += struct.pack(e + "I", 0)
+        packed = bytearray((len(seq) + 3) // 4)
+        for i, ch in enumerate(up):
+            packed[i // 4] |= code[ch] << (6 - 2 * (i % 4))
+        index += struct.pack("B", len(nb)) + nb + struct.pack(e + "I", off + len(body))
+        body += rec + bytes(packed)
+    with open(path, "wb") as f:
+        f.write(struct.pack(e + "IIII", 0x1A412743, 0, len(records), 0) + index + body)

@@ -365,3 +365,45 @@ def test_gilist_selects_subjects(tmp_path):
         assert r.stdout == want
         assert bool(keep) or "matches no sequence" in r.stderr
     assert sum(1 for l in full.stdout.splitlines() if l.split("\t")[5] in {"gi|3", "gi|7"}) > 5
+
+
+def test_2bit_database_matches_fasta(tmp_path):
+    """A .2bit library gives the rows of the same library in FASTA: the
+    reference's alu.2bit against its FASTA on a chr22 excerpt, and families
+    with N runs (N blocks, resolved like FASTA N: twobit.rs:346-384) and
+    soft-masked stretches, in both byte orders (seq/twobit.rs:187-300)."""
+    import random
+    from conftest import write_2bit
+    q = golden("chr22_30000000_30200000.fa")
+    common = ["--matrix", golden("comparison.matrix"), "--word-size", "14", *RM_SEARCH, "--outfmt", RM_FIELDS,
+              "--dust", "no"]
+    a = run([CLI, "--db", golden("alu.2bit"), "--query", q] + common)
+    b = run([CLI, "--db", golden("alu.fa"), "--query", q] + common)
+    assert a.returncode == 0 and b.returncode == 0, (a.stderr, b.stderr)
+    assert a.stdout == b.stdout and a.stdout.count("\n") > 5
+    recs = []
+    for line in open(golden("db.fa")).read().splitlines():
+        if line.startswith(">"):
+            recs.append([line[1:].split()[0], ""])
+        else:
+            recs[-1][1] += line
+    rng = random.Random(5)
+    for k in (0, 2, 6, 9):
+        s = recs[k][1]
+        for _ in range(3):
+            at, z = rng.randrange(0, len(s) - 40), rng.randrange(1, 30)
+            s = s[:at] + "N" * z + s[at + z:]
+        at = rng.randrange(0, len(s) - 60)
+        recs[k][1] = s[:at] + s[at:at + 50].lower() + s[at + 50:]
+    (tmp_path / "db.fa").write_text("".join(f">{n}\n{s}\n" for n, s in recs))
+    write_2bit(tmp_path / "db.2bit", recs)
+    write_2bit(tmp_path / "db_swapped.2bit", recs, swapped=True)
+    args = ["--query", golden("query.fa"), "--matrix", golden("comparison.matrix"), "--word-size", "11",
+            "--gapopen", "20", "--gapextend", "5", "--outfmt", RM_FIELDS]
+    outs = []
+    for db in ("db.fa", "db.2bit", "db_swapped.2bit"):
+        r = run([CLI, "--db", str(tmp_path / db)] + args)
+        assert r.returncode == 0, r.stderr
+        outs.append(r.stdout)
+    assert outs[0] == outs[1] == outs[2] and outs[0].count("\n") > 100
+    assert any("N" in l.split("\t")[-1] for l in outs[0].splitlines())  # N restored in the subject rows
