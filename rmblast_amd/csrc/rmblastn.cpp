@@ -466,6 +466,11 @@ int main(int argc, char** argv) {
                 names.c_str());
         return 1;
       }
+      if (w == "evalue" || w == "bitscore") {  // ka_stats.rs: E-values need the statistics fit
+        fprintf(stderr, "error: --outfmt '%s': field '%s' needs Karlin-Altschul statistics, which this build does "
+                "not compute\n", spec.c_str(), w.c_str());
+        return 1;
+      }
       fields.push_back(w);
     }
     if (!pairwise && fields.empty()) {
@@ -686,8 +691,6 @@ int main(int argc, char** argv) {
         else if (f == "cpg_sites") out += std::to_string(st.ti);
         else if (f == "qseq") out += r.qa;
         else if (f == "sseq") out += r.sa;
-        else if (f == "evalue") out += "1.0";
-        else if (f == "bitscore") out += " 0.0";
         else { fprintf(stderr, "error: unknown -outfmt field '%s'\n", f.c_str()); return 1; }
       }
       out += '\n';
