@@ -8,6 +8,11 @@ no DUST.  One step = one rmb_search over all records (lookup build, seeding,
 ungapped, preliminary and traceback DP, ordered replay), with the library
 already resident on the GPU.  Metric: query Mbp x families per second.
 
+--workload gappy measures the divergent/gappy stress configuration instead:
+one 10 Mbp record (the split query path) against 300 families of 5-15 kbp,
+copies 25-40% diverged with 5-8% indels, 5% microsatellite tracts, X-drop
+gapped/final 120/400; its CPU baseline sample is the record's first 1.5 Mbp.
+
 Prints one JSON line.  --gpus N with N > 1 is refused: this build runs on one GPU.
 """
 import argparse
@@ -34,6 +39,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=("scale", "gappy"), default="scale")
     ap.add_argument("--records", type=int, default=1667, help="records of the 100 Mbp genome to search")
     ap.add_argument("--families", type=int, default=1000)
     ap.add_argument("--cpu-records", type=int, default=300,
@@ -70,8 +76,12 @@ def main():
     from rmblast_amd import synth
 
     t0 = time.time()
-    w = synth.scale_config()
-    w = synth.subset(w, a.records, a.families)
+    gappy = a.workload == "gappy"
+    if gappy:
+        w = synth.gappy_config()
+    else:
+        w = synth.scale_config()
+        w = synth.subset(w, a.records, a.families)
     gen_s = time.time() - t0
     print(f"[bench] workload generated in {gen_s:.1f}s: {len(w.records)} records x {len(w.families)} families",
           file=sys.stderr, flush=True)
@@ -80,8 +90,9 @@ def main():
     with open(mpath, "w") as f:
         f.write(synth.MATRIX_TEXT)
     m = rmblast_amd.load_matrix(mpath)
-    params = dict(gap_open=20, gap_extend=5, word_size=7, xdrop_ungap=20, xdrop_gap=30, xdrop_gap_final=100,
-                  min_raw_gapped_score=200, complexity_adjust=True)
+    xdrop = [20, 120, 400] if gappy else [20, 30, 100]
+    params = dict(gap_open=20, gap_extend=5, word_size=7, xdrop_ungap=xdrop[0], xdrop_gap=xdrop[1],
+                  xdrop_gap_final=xdrop[2], min_raw_gapped_score=200, complexity_adjust=True)
     s = rmblast_amd.Searcher(m, **params)
     s.load_library([f[1] for f in w.families])
     queries = [r[1] for r in w.records]
@@ -136,7 +147,15 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(w, tmp, mpath, a.cpu_records)
+        if gappy:
+            sample = [(w.records[0][0], w.records[0][1][:1_500_000])]
+            cpu = cpu_baseline(sample, w.families, tmp, mpath, xdrop, "the record's first 1.5 Mbp")
+        else:
+            cpu = cpu_baseline(w.records[:a.cpu_records], w.families, tmp, mpath, xdrop,
+                               f"first {a.cpu_records} records")
+    workload = ("divergent/gappy stress: 1 x 10 Mbp record x 300 families of 5-15 kbp, 25-40% divergence, "
+                "5-8% indels, X-drop 4x, ws 7" if gappy else
+                "exact path at scale: 1,667 x 60 kbp records x 1,000 families, ws 7")
     line = {
         "metric": "query Mbp x families per second (exact search path)",
         "value": round(value, 2),
@@ -150,9 +169,9 @@ def main():
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic (seeded generator, rmblast_amd/synth.py)",
-        "config": {"workload": "exact path at scale: 1,667 x 60 kbp records x 1,000 families, ws 7",
+        "config": {"workload": workload,
                    "records": len(queries), "families": nfam, "query_mbp": round(mbp, 3), "word_size": 7,
-                   "min_raw_gapped_score": 200, "xdrop": [20, 30, 100], "complexity_adjust": True, "dust": False},
+                   "min_raw_gapped_score": 200, "xdrop": xdrop, "complexity_adjust": True, "dust": False},
         "roofline": roofline,
         "gapped_gcups": round(gcups, 3),
         "gapped_kernel_ms": round(g_ns / 1e6, 3),
@@ -167,10 +186,10 @@ def main():
     return 0
 
 
-def cpu_baseline(w, tmp, mpath, n_records):
+def cpu_baseline(records, families, tmp, mpath, xdrop, what):
     """The CPU oracle (a restatement of the reference's search path) on the
-    first n_records records x all families, threads over subjects per record
-    like rmblastn's default thread mode."""
+    sample `records` x all families, threads over subjects per record like
+    rmblastn's default thread mode."""
     from rmblast_amd import synth
     exe = os.path.join(ROOT, "oracle", "_build", "rmblastn_oracle")
     if not os.path.exists(exe):
@@ -178,20 +197,21 @@ def cpu_baseline(w, tmp, mpath, n_records):
     cores = min(16, os.cpu_count() or 1)
     q = os.path.join(tmp, "cpu_q.fa")
     db = os.path.join(tmp, "cpu_db.fa")
-    synth.to_fasta(w.records[:n_records], q)
-    synth.to_fasta(w.families, db)
+    synth.to_fasta(records, q)
+    synth.to_fasta(families, db)
     args = [exe, "--db", db, "--query", q, "--matrix", mpath, "--gapopen", "20", "--gapextend", "5", "--word-size",
-            "7", "--min-raw-gapped-score", "200", "--complexity-adjust", "--dust", "no", "--num-threads", str(cores),
+            "7", "--min-raw-gapped-score", "200", "--xdrop-ungap", str(xdrop[0]), "--xdrop-gap", str(xdrop[1]),
+            "--xdrop-gap-final", str(xdrop[2]), "--complexity-adjust", "--dust", "no", "--num-threads", str(cores),
             "--outfmt", "6 score qseqid qstart qend sstrand sseqid sstart send"]
     r = subprocess.run(args, capture_output=True, text=True, env=dict(os.environ, RMB_ORACLE_TIME="1"),
                        timeout=900)
     if r.returncode != 0:
         return None
     secs = float([l for l in r.stderr.splitlines() if l.startswith("oracle_search_seconds")][0].split()[1])
-    mbp = sum(len(x[1]) for x in w.records[:n_records]) / 1e6
-    return {"value": round(mbp * len(w.families) / secs, 2), "unit": "Mbp*families/s", "cores": cores,
+    mbp = sum(len(x[1]) for x in records) / 1e6
+    return {"value": round(mbp * len(families) / secs, 2), "unit": "Mbp*families/s", "cores": cores,
             "kind": "port", "seconds": round(secs, 3),
-            "sample": f"first {n_records} records ({mbp:.2f} Mbp) x {len(w.families)} families, same parameters"}
+            "sample": f"{what} ({mbp:.2f} Mbp) x {len(families)} families, same parameters"}
 
 
 if __name__ == "__main__":

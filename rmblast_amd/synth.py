@@ -139,6 +139,25 @@ def scale_config(seed=2, n_records=1667, record_len=60_000, n_families=1000, n_c
     return Workload(recs, fams, inserts)
 
 
+def gappy_config(seed=4, genome_len=10_000_000, n_families=300, n_copies=4000, frag_range=(300, 5000),
+                 tract_frac=0.05):
+    """The divergent/gappy stress configuration: one 10 Mbp record (the split
+    query path) against 300 families of 5-15 kbp; copies 25-40% diverged with
+    5-8% indels, fragments uniform in frag_range; then microsatellite tracts
+    (period 2-6, 50-500 bp) overwrite tract_frac of the genome."""
+    rng = np.random.default_rng(seed)
+    fams = make_families(rng, n_families, rng.integers(5000, 15001, n_families))
+    g, ins = insert_copies(rng, genome_len, fams, n_copies, frag_range, (0.25, 0.40), (0.05, 0.08))
+    covered = 0
+    while covered < tract_frac * genome_len:
+        n = int(rng.integers(50, 501))
+        unit = random_bases(rng, int(rng.integers(2, 7)))
+        at = int(rng.integers(0, genome_len - n))
+        g[at:at + n] = np.tile(unit, n // len(unit) + 1)[:n]
+        covered += n
+    return Workload([("chr1", g)], fams, [(0,) + x for x in ins])
+
+
 def subset(w: Workload, n_records: int, n_families: int | None = None) -> Workload:
     fams = w.families if n_families is None else w.families[:n_families]
     return Workload(w.records[:n_records], fams, [x for x in w.inserts if x[0] < n_records])

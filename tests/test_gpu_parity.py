@@ -407,3 +407,21 @@ def test_2bit_database_matches_fasta(tmp_path):
         outs.append(r.stdout)
     assert outs[0] == outs[1] == outs[2] and outs[0].count("\n") > 100
     assert any("N" in l.split("\t")[-1] for l in outs[0].splitlines())  # N restored in the subject rows
+
+
+def test_gappy_stress_config_matches_oracle(tmp_path):
+    """The divergent/gappy stress configuration's parameter point (families of
+    5-15 kbp, copies 25-40% diverged with 5-8% indels, 5% microsatellites,
+    X-drop 120/400) on a 1.5 Mbp genome: identical rows to the oracle."""
+    w = synth.gappy_config(genome_len=1_500_000, n_copies=600)
+    synth.to_fasta(w.records, str(tmp_path / "q.fa"))
+    synth.to_fasta(w.families, str(tmp_path / "db.fa"))
+    (tmp_path / "m.matrix").write_text(synth.MATRIX_TEXT)
+    args = ["--db", str(tmp_path / "db.fa"), "--query", str(tmp_path / "q.fa"), "--matrix", str(tmp_path / "m.matrix"),
+            "--word-size", "7", "--gapopen", "20", "--gapextend", "5", "--min-raw-gapped-score", "200",
+            "--xdrop-gap", "120", "--xdrop-gap-final", "400", "--complexity-adjust", "--mask-level", "80",
+            "--dust", "no", "--outfmt", RM_FIELDS]
+    g = run([CLI] + args)
+    o = run([ORACLE_BIN, "--num-threads", "16"] + args, timeout=900)
+    assert g.returncode == 0 and o.returncode == 0, (g.stderr, o.stderr)
+    assert g.stdout == o.stdout and g.stdout.count("\n") > 100
