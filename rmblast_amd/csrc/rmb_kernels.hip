@@ -1007,6 +1007,9 @@ __device__ __forceinline__ void dp_half(const DpTask& t, const int* mat, int32_t
 }
 
 static constexpr uint32_t kDpLdsThreads = 64, kDpLdsLog2 = 6;
+// wider one-pass rings (81 KB of LDS per workgroup, one per CU), for
+// X-drops whose bands outgrow 64 cells
+static constexpr uint32_t kDpLdsWideLog2 = 8;
 
 // The traceback DP of k_dp_onepass with 4-byte LDS ring cells, so twice as
 // many lanes fit a CU: H and E as 16-bit offsets from the best score at the
@@ -1020,11 +1023,12 @@ static constexpr uint32_t kDpLdsThreads = 64, kDpLdsLog2 = 6;
 // dp_half's.  Kept cells lie in [base - X - open cost, base + ring cells x
 // largest score]; a task for which that could leave the 16-bit range comes
 // back E_BAND_WIDE, for the retry with dp_half.
+template <uint32_t RLOG2>
 __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, int32_t go, int32_t ge,
                                              uint32_t* __restrict__ ring, uint8_t* __restrict__ bring,
                                              int32_t max_pos, DpResult& res, uint8_t* __restrict__ flat,
                                              uint32_t* __restrict__ rows, uint32_t flat_cap, uint32_t rows_cap) {
-  constexpr uint32_t RS = kDpLdsThreads, wmask = (1u << kDpLdsLog2) - 1;
+  constexpr uint32_t RS = kDpLdsThreads, wmask = (1u << RLOG2) - 1;
   res.score = 0; res.a_len = 0; res.b_len = 0; res.flags = 0; res.cells = 0; res.tb_bytes = 0; res.tb_rows = 0;
   const uint32_t m = t.m, n = t.n;
   if (m == 0 || n == 0) return;
@@ -1256,16 +1260,16 @@ __global__ __launch_bounds__(LDS_RING ? kDpLdsThreads : 256) void k_dp_traceback
 // end) and ops in its own arena, walks back from the best cell, and appends
 // the ops to one packed pool.  A half whose band, rows, bytes or ops outgrow
 // the LDS ring or the arena comes back flagged for the two-pass path.
-template <bool LDS_RING>
-__global__ __launch_bounds__(LDS_RING ? kDpLdsThreads : 256) void k_dp_onepass(
+template <bool LDS_RING, uint32_t RLOG2>
+__global__ __launch_bounds__(kDpLdsThreads) void k_dp_onepass(
     const DpTask* __restrict__ tasks, uint32_t n_tasks, const int32_t* __restrict__ gmat, int32_t go, int32_t ge,
     int2* __restrict__ ring, uint32_t wlog2, uint8_t* __restrict__ arena, uint32_t flat_cap, uint32_t rows_cap,
     uint32_t ops_cap, uint32_t* __restrict__ ops_pool, unsigned long long* __restrict__ ops_count,
     unsigned long long ops_pool_cap, OnePassResult* __restrict__ out, unsigned long long* next, OnePassBig big,
     unsigned long long* counters) {
   __shared__ int mat[256];
-  __shared__ uint32_t lring[kDpLdsThreads << kDpLdsLog2];
-  __shared__ uint8_t lbring[kDpLdsThreads << kDpLdsLog2];
+  __shared__ uint32_t lring[kDpLdsThreads << RLOG2];
+  __shared__ uint8_t lbring[kDpLdsThreads << RLOG2];
   __shared__ int s_max_pos;
   if (threadIdx.x == 0) s_max_pos = 0;
   __syncthreads();
@@ -1298,7 +1302,7 @@ __global__ __launch_bounds__(LDS_RING ? kDpLdsThreads : 256) void k_dp_onepass(
     const uint32_t i = (uint32_t)base + (threadIdx.x & 63u);
     if (i >= n_tasks) continue;
     DpResult r;
-    dp_half_tb16(tasks[i], mat, go, ge, my, myb, max_pos, r, flat, rows, flat_cap, rows_cap);
+    dp_half_tb16<RLOG2>(tasks[i], mat, go, ge, my, myb, max_pos, r, flat, rows, flat_cap, rows_cap);
     uint8_t* tflat = flat;
     uint32_t *trows = rows, *tops = ops, tops_cap = ops_cap;
     if (r.flags) {  // band or traceback beyond the lane's storage: once more with a retry slot
@@ -1540,10 +1544,15 @@ hipError_t launch_dp_onepass(int blocks, const DpTask* tasks, uint32_t n, const 
                              int32_t ge, uint8_t* arena, uint32_t flat_cap, uint32_t rows_cap, uint32_t ops_cap,
                              uint32_t* ops_pool, unsigned long long* ops_count, unsigned long long ops_pool_cap,
                              OnePassResult* out, unsigned long long* next, OnePassBig big,
-                             unsigned long long* counters, hipStream_t st) {
-  hipLaunchKernelGGL(k_dp_onepass<true>, dim3(blocks), dim3(kDpLdsThreads), 0, st, tasks, n, gmat, go, ge, nullptr,
-                     0, arena, flat_cap, rows_cap, ops_cap, ops_pool, ops_count, ops_pool_cap, out, next, big,
-                     counters);
+                             unsigned long long* counters, bool wide, hipStream_t st) {
+  if (wide)
+    hipLaunchKernelGGL((k_dp_onepass<true, kDpLdsWideLog2>), dim3(blocks), dim3(kDpLdsThreads), 0, st, tasks, n, gmat,
+                       go, ge, nullptr, 0, arena, flat_cap, rows_cap, ops_cap, ops_pool, ops_count, ops_pool_cap, out,
+                       next, big, counters);
+  else
+    hipLaunchKernelGGL((k_dp_onepass<true, kDpLdsLog2>), dim3(blocks), dim3(kDpLdsThreads), 0, st, tasks, n, gmat, go,
+                       ge, nullptr, 0, arena, flat_cap, rows_cap, ops_cap, ops_pool, ops_count, ops_pool_cap, out,
+                       next, big, counters);
   return hipGetLastError();
 }
 

@@ -53,7 +53,7 @@ hipError_t launch_dp_traceback(int, const TbTask*, uint32_t, const int32_t*, int
                                uint8_t*, uint32_t*, uint32_t*, TbResult*, hipStream_t);
 hipError_t launch_dp_onepass(int, const DpTask*, uint32_t, const int32_t*, int32_t, int32_t, uint8_t*, uint32_t,
                              uint32_t, uint32_t, uint32_t*, unsigned long long*, unsigned long long, OnePassResult*,
-                             unsigned long long*, OnePassBig, unsigned long long*, hipStream_t);
+                             unsigned long long*, OnePassBig, unsigned long long*, bool, hipStream_t);
 hipError_t launch_prelim_fused(int, bool, PrelimParams, const UngHit*, const uint32_t*, uint64_t, const RecDesc*,
                                const SubjDesc*, const uint8_t*, const uint8_t*, const uint8_t*, const int32_t*, int2*,
                                uint32_t, PreHit*, unsigned long long*, uint32_t*, UngHit*, unsigned long long*,
@@ -511,8 +511,21 @@ static void run_dp_traceback_2pass(rmb_ctx* c, const std::vector<DpTask>& tasks,
 // one half alignment.  Halves beyond it (and bands wider than the LDS ring)
 // take the two-pass path.
 constexpr uint32_t kOnePassRows = 8192, kOnePassFlat = 256u << 10, kOnePassOps = 4096;
+// lane arenas of the wide-ring launches (one workgroup per CU, so 16,384 lanes)
+constexpr uint32_t kOnePassWideRows = 32768, kOnePassWideFlat = 1u << 20, kOnePassWideOps = 16384;
 constexpr unsigned long long kOnePassPool = 64ull << 20;  // packed ops per launch
 constexpr uint32_t kOnePassPerCu = 7;  // 64-lane workgroups with ~21 KB of LDS: seven fit a CU
+constexpr uint32_t kOnePassWidePerCu = 1;  // with the 256-cell rings, ~81 KB: one fits a CU
+// A gap of g bases costs open + g x extend, so a band reaches about
+// (X - open) / extend cells to either side of its best cell: past 48 cells
+// in all the one-pass kernel takes its wider rings (RMB_TB_WIDE=0 keeps the
+// 64-cell rings).
+static bool onepass_wide(const rmb_ctx* c) {
+  if (const char* e = std::getenv("RMB_TB_WIDE"))
+    if (std::atoi(e) == 0) return false;
+  const int ge = std::max(1, (int)c->P.gap_extend);
+  return 2 * std::max(0, (int)c->P.xdrop_gap_final - (int)c->P.gap_open) / ge + 1 > 48;
+}
 // retry slots for halves beyond the LDS ring or the lane arena: global rings of
 // 2^kRingBigLog2 cells and 70 MB arenas (RMB_TB_RETRY_SLOTS lowers the count,
 // down to 0, which sends every such half to the two-pass path)
@@ -545,8 +558,12 @@ static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::
     const uint32_t n = (uint32_t)std::min(chunk, sorted.size() - base);
     DpTask* dt = c->b_tasks.get<DpTask>(n);
     OnePassResult* dr = c->b_opres.get<OnePassResult>(n);
-    const uint32_t blocks = std::min<uint32_t>((uint32_t)c->cu_count * kOnePassPerCu, (n + 63) / 64);
-    const uint64_t stride = onepass_stride(kOnePassFlat, kOnePassRows, kOnePassOps);
+    const bool wide = onepass_wide(c);
+    const uint32_t blocks = std::min<uint32_t>((uint32_t)c->cu_count * (wide ? kOnePassWidePerCu : kOnePassPerCu),
+                                               (n + 63) / 64);
+    const uint32_t flat_cap = wide ? kOnePassWideFlat : kOnePassFlat, rows_cap = wide ? kOnePassWideRows : kOnePassRows,
+                   ops_cap = wide ? kOnePassWideOps : kOnePassOps;
+    const uint64_t stride = onepass_stride(flat_cap, rows_cap, ops_cap);
     uint8_t* arena = c->b_arena.get<uint8_t>((size_t)blocks * 64 * stride);
     uint32_t* pool = c->b_oppool.get<uint32_t>(kOnePassPool);
     // [0] ops used, [1] work queue, [2] retry slots taken (low 32 bits)
@@ -564,9 +581,9 @@ static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::
     hip_check(hipMemcpyAsync(dt, sorted.data() + base, n * sizeof(DpTask), hipMemcpyHostToDevice, c->st),
               "copy tasks");
     timed_launch(c, 14, [&] {
-      hip_check(launch_dp_onepass((int)blocks, dt, n, c->d_mat, c->P.gap_open, c->P.gap_extend, arena, kOnePassFlat,
-                                  kOnePassRows, kOnePassOps, pool, pcount, kOnePassPool, dr, pcount + 1, big, ctr,
-                                  c->st),
+      hip_check(launch_dp_onepass((int)blocks, dt, n, c->d_mat, c->P.gap_open, c->P.gap_extend, arena, flat_cap,
+                                  rows_cap, ops_cap, pool, pcount, kOnePassPool, dr, pcount + 1, big, ctr,
+                                  wide, c->st),
                 "k_dp_onepass");
     });
     std::vector<OnePassResult> r(n);

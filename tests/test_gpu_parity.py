@@ -98,10 +98,11 @@ def test_gappy_divergent_and_microsatellites_match_oracle(tmp_path):
 
 
 def test_traceback_two_pass_fallback_matches_oracle(tmp_path):
-    """Halves whose band outgrows the traceback kernel's LDS ring are rerun in
-    the same launch with a retry slot; with no retry slots they take the
+    """At X-drop 400 the traceback kernel takes its 256-cell LDS rings.
+    Halves whose band outgrows a ring are rerun in the same launch with a
+    retry slot; with the 64-cell rings and no retry slots they take the
     two-pass path (score pass, then a traceback pass into exactly sized
-    storage).  Both must give the oracle's rows."""
+    storage).  All must give the oracle's rows."""
     rng = np.random.default_rng(11)
     fams = synth.make_families(rng, 3, [3000, 4000, 2500])
     g0, _ = synth.insert_copies(rng, 60_000, fams, 20, (400, 2500), (0.20, 0.35), (0.04, 0.08))
@@ -121,8 +122,8 @@ def test_traceback_two_pass_fallback_matches_oracle(tmp_path):
             "--complexity-adjust", "--dust", "no", "--outfmt", RM_FIELDS]
     o = run([ORACLE_BIN] + args)
     assert o.returncode == 0, o.stderr
-    for slots in ("32", "0"):
-        g = run([CLI] + args, env={"RMB_TB_RETRY_SLOTS": slots, "RMB_STAGE_TIMES": "1"})
+    for slots, wide in (("32", "1"), ("32", "0"), ("0", "0")):
+        g = run([CLI] + args, env={"RMB_TB_RETRY_SLOTS": slots, "RMB_TB_WIDE": wide, "RMB_STAGE_TIMES": "1"})
         assert g.returncode == 0, g.stderr
         assert g.stdout == o.stdout and g.stdout.count("\n") > 10
         if slots == "0":  # the two-pass path ran
