@@ -1390,8 +1390,11 @@ __device__ __forceinline__ void prelim_pivot(const UngHit& u, uint32_t qa_len, u
 // records (P.rec_ids); the task count may come from device memory (P.n_dev),
 // so the passes queue behind the seeding kernel without a host round trip.
 static constexpr uint32_t kPrelimThreads = 128, kPrelimRingLog2 = 4, kPrelimRun = 1;
+// wider rings (72 KB of LDS per workgroup, two per CU), for X-drops whose
+// bands outgrow 16 cells
+static constexpr uint32_t kPrelimWideLog2 = 6;
 
-template <bool LDS_RING>
+template <bool LDS_RING, uint32_t RLOG2>
 __global__ __launch_bounds__(kPrelimThreads) void k_prelim_fused(
     PrelimParams P, const UngHit* __restrict__ hits, const uint32_t* __restrict__ idx, uint64_t n_tasks,
     const RecDesc* __restrict__ recs, const SubjDesc* __restrict__ subjs, const uint8_t* __restrict__ qfwd,
@@ -1400,15 +1403,15 @@ __global__ __launch_bounds__(kPrelimThreads) void k_prelim_fused(
     uint32_t* __restrict__ redo, UngHit* __restrict__ redo_hits, unsigned long long* __restrict__ redo_count,
     unsigned long long* __restrict__ counters, uint32_t* __restrict__ err) {
   __shared__ int mat[256];
-  __shared__ int2 lring[LDS_RING ? (kPrelimThreads << kPrelimRingLog2) : 1];
-  __shared__ uint8_t lbring[LDS_RING ? (kPrelimThreads << kPrelimRingLog2) : 1];
+  __shared__ int2 lring[LDS_RING ? (kPrelimThreads << RLOG2) : 1];
+  __shared__ uint8_t lbring[LDS_RING ? (kPrelimThreads << RLOG2) : 1];
   for (int i = threadIdx.x; i < 256; i += blockDim.x) mat[i] = gmat[i];
   __syncthreads();
   const uint64_t gt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (uint64_t)gridDim.x * blockDim.x;
   constexpr uint32_t RS = LDS_RING ? kPrelimThreads : 1;
   int2* my = LDS_RING ? lring + threadIdx.x : gring + gt * ring_stride(wlog2);
   uint8_t* myb = LDS_RING ? lbring + threadIdx.x : nullptr;
-  const uint32_t wmask = LDS_RING ? (1u << kPrelimRingLog2) - 1 : (1u << wlog2) - 1;
+  const uint32_t wmask = LDS_RING ? (1u << RLOG2) - 1 : (1u << wlog2) - 1;
   unsigned long long cells = 0, dps = 0;
   if (P.n_dev) n_tasks = *P.n_dev < n_tasks ? *P.n_dev : n_tasks;
   // waves take runs of kPrelimRun x 64 hits from the launch's queue, so
@@ -1487,15 +1490,19 @@ hipError_t launch_prelim_fused(int blocks, bool lds_ring, PrelimParams P, const 
                                const uint8_t* qrc, const uint8_t* splus, const int32_t* gmat, int2* gring,
                                uint32_t wlog2, PreHit* out, unsigned long long* out_count, uint32_t* redo,
                                UngHit* redo_hits, unsigned long long* redo_count, unsigned long long* counters,
-                               uint32_t* err, hipStream_t st) {
-  if (lds_ring)
-    hipLaunchKernelGGL(k_prelim_fused<true>, dim3(blocks), dim3(kPrelimThreads), 0, st, P, hits, idx, n_tasks, recs,
-                       subjs, qfwd, qrc, splus, gmat, gring, wlog2, out, out_count, redo, redo_hits, redo_count,
-                       counters, err);
+                               uint32_t* err, bool wide, hipStream_t st) {
+  if (lds_ring && wide)
+    hipLaunchKernelGGL((k_prelim_fused<true, kPrelimWideLog2>), dim3(blocks), dim3(kPrelimThreads), 0, st, P, hits,
+                       idx, n_tasks, recs, subjs, qfwd, qrc, splus, gmat, gring, wlog2, out, out_count, redo,
+                       redo_hits, redo_count, counters, err);
+  else if (lds_ring)
+    hipLaunchKernelGGL((k_prelim_fused<true, kPrelimRingLog2>), dim3(blocks), dim3(kPrelimThreads), 0, st, P, hits,
+                       idx, n_tasks, recs, subjs, qfwd, qrc, splus, gmat, gring, wlog2, out, out_count, redo,
+                       redo_hits, redo_count, counters, err);
   else
-    hipLaunchKernelGGL(k_prelim_fused<false>, dim3(blocks), dim3(kPrelimThreads), 0, st, P, hits, idx, n_tasks, recs,
-                       subjs, qfwd, qrc, splus, gmat, gring, wlog2, out, out_count, redo, redo_hits, redo_count,
-                       counters, err);
+    hipLaunchKernelGGL((k_prelim_fused<false, kPrelimRingLog2>), dim3(blocks), dim3(kPrelimThreads), 0, st, P, hits,
+                       idx, n_tasks, recs, subjs, qfwd, qrc, splus, gmat, gring, wlog2, out, out_count, redo,
+                       redo_hits, redo_count, counters, err);
   return hipGetLastError();
 }
 

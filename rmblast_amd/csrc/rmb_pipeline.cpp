@@ -57,7 +57,7 @@ hipError_t launch_dp_onepass(int, const DpTask*, uint32_t, const int32_t*, int32
 hipError_t launch_prelim_fused(int, bool, PrelimParams, const UngHit*, const uint32_t*, uint64_t, const RecDesc*,
                                const SubjDesc*, const uint8_t*, const uint8_t*, const uint8_t*, const int32_t*, int2*,
                                uint32_t, PreHit*, unsigned long long*, uint32_t*, UngHit*, unsigned long long*,
-                               unsigned long long*, uint32_t*, hipStream_t);
+                               unsigned long long*, uint32_t*, bool, hipStream_t);
 
 struct Failure : std::runtime_error {
   int code;
@@ -806,9 +806,14 @@ static void enqueue_batch(rmb_ctx* c, P1Batch& b, const uint8_t* d_qfwd, const u
     PP.redo_cap = g.hits_cap;
     PP.next = cnt + 4 + 2 * gi;
     hip_check(hipEventRecord(S.ev[6 * gi + 2], st), "event");
-    hip_check(launch_prelim_fused(c->cu_count * 8, true, PP, dh, nullptr, g.hits_cap, drecs,
+    // past 12 band cells ((X - open) / extend to either side), 64-cell rings
+    // with two workgroups per CU (RMB_PRELIM_WIDE=0 keeps the 16-cell rings)
+    const char* pw_env = std::getenv("RMB_PRELIM_WIDE");
+    const bool pwide = !(pw_env && std::atoi(pw_env) == 0) &&
+                       2 * std::max(0, (int)c->P.xdrop_gap - (int)c->P.gap_open) / std::max(1, (int)c->P.gap_extend) + 1 > 12;
+    hip_check(launch_prelim_fused(c->cu_count * (pwide ? 2 : 8), true, PP, dh, nullptr, g.hits_cap, drecs,
                                   (const SubjDesc*)c->d_sdesc.p, d_qfwd, d_qrc, (const uint8_t*)c->d_splus.p,
-                                  c->d_mat, nullptr, 0, dpre, cnt, dredo, nullptr, hitcnt + 2 * gi + 1, ctr, err, st),
+                                  c->d_mat, nullptr, 0, dpre, cnt, dredo, nullptr, hitcnt + 2 * gi + 1, ctr, err, pwide, st),
               "k_prelim_fused");
     hip_check(hipEventRecord(S.ev[6 * gi + 3], st), "event");
     // pass 2: the hits whose band outgrew the LDS ring, with per-lane rings of
@@ -821,7 +826,7 @@ static void enqueue_batch(rmb_ctx* c, P1Batch& b, const uint8_t* d_qfwd, const u
     hip_check(hipEventRecord(S.ev[6 * gi + 4], st), "event");
     hip_check(launch_prelim_fused((int)(lanes / 128), false, PP, dh, dredo, g.hits_cap, drecs,
                                   (const SubjDesc*)c->d_sdesc.p, d_qfwd, d_qrc, (const uint8_t*)c->d_splus.p,
-                                  c->d_mat, ring, kRingLog2, dpre, cnt, nullptr, dredoh, cnt + 2, ctr, err, st),
+                                  c->d_mat, ring, kRingLog2, dpre, cnt, nullptr, dredoh, cnt + 2, ctr, err, false, st),
               "k_prelim_fused wide");
     hip_check(hipEventRecord(S.ev[6 * gi + 5], st), "event");
   }
