@@ -60,9 +60,20 @@ typedef struct rmb_params {
   int32_t xdrop_gap;
   int32_t xdrop_gap_final;
   int32_t min_raw_gapped_score;
-  int32_t ungapped_cutoff; /* < 0: NCBI cutoff_score_max (options.rs:80-82); else the KA-derived value */
+  /* Ungapped keep threshold.  >= 0: this value for every query record.
+   * < 0: resolved per query record as rmblastn resolves it before each
+   * search (rmblastn/src/main.rs:1154-1170 calling ungapped_cutoff,
+   * ka_cutoff.rs:126-161): from ka_lambda / ka_k below when both are > 0,
+   * with the record's full length (a split query's whole length) and the
+   * loaded library's average length; else NCBI's cutoff_score_max
+   * (options.rs:80-82). */
+  int32_t ungapped_cutoff;
   int32_t complexity_adjust;
   int32_t dust;
+  /* Gapped Karlin-Altschul lambda and K of the matrix + gap costs, as
+   * rmb_ka_lookup returns them; 0 when the combination has none. */
+  double ka_lambda;
+  double ka_k;
 } rmb_params;
 
 /* Substitution matrix in BLASTNA order, as ScoreMatrix holds it after parsing
@@ -97,6 +108,30 @@ typedef struct rmb_ctx rmb_ctx;
  * sentinel row/column, and estimate lambda.  Replaces ScoreMatrix::from_file
  * (matrix.rs:76-216, lambda :306-354).  err receives a message on failure. */
 int rmb_matrix_load(const char* path, rmb_matrix* out, char* err, size_t err_len);
+
+/* Gapped Karlin-Altschul (lambda, K) of a RepeatMasker matrix at its gap
+ * costs, from NCBI's ALP table (blast_stat.c rmblast_*_values, as restated in
+ * rmblast-lib/src/search/ka_cutoff.rs:40-87).  The matrix file's base name
+ * (the part after the last '/') is compared case-insensitively and both gap
+ * costs must match the entry.  Replaces lookup_ka (ka_cutoff.rs:105-114).
+ * *found = 1 and out = {lambda, K} on a match, else *found = 0. */
+int rmb_ka_lookup(const char* matrix_name, int32_t gap_open, int32_t gap_extend, double out[2], int32_t* found);
+
+/* The ungapped pre-filter cutoff for one query (ka_cutoff.rs:126-161):
+ * es = ceil(ln(K * min(avg_subj, 2 * query_length) * avg_subj / 0.05) / lambda)
+ * clamped to [1, cutoff_score_max] when rmb_ka_lookup matches and
+ * avg_subj_length > 0, else cutoff_score_max (min_raw_gapped_score / 2, or
+ * 1 when that is 0).  query_length is the whole query record's length.
+ * info (may be NULL) receives {found, es, lambda, K, searchsp} for the run log
+ * rmblastn prints (main.rs:1098-1131). */
+typedef struct rmb_ka_cutoff_info {
+  int32_t found;
+  int32_t es;          /* the KA score before the cap */
+  double lambda, k;
+  uint64_t searchsp;
+} rmb_ka_cutoff_info;
+int rmb_ungapped_cutoff(const char* matrix_name, int32_t gap_open, int32_t gap_extend, int32_t min_raw_gapped_score,
+                        uint64_t query_length, uint64_t avg_subj_length, int32_t* cutoff, rmb_ka_cutoff_info* info);
 
 /* Create a context on HIP device `device`.  Replaces the per-search state the
  * reference builds in rmblastn main (main.rs:516-700): parameters + matrix. */

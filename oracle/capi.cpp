@@ -184,3 +184,16 @@ extern "C" int orc_merge_chunk_prelims(const uint32_t* acc, int n_acc, const uin
   }
   return (int)a.size();
 }
+
+extern "C" {
+// ka_cutoff.rs:126-153; out = {found, es}, out_sp = searchsp; returns the cutoff
+int orc_ungapped_cutoff(const char* name, int go, int ge, int min_raw, uint64_t qlen, uint64_t avg, int* out,
+                        uint64_t* out_sp) {
+  KaCutoffInfo info{};
+  const int c = ungapped_cutoff(name, go, ge, min_raw, qlen, avg, &info);
+  out[0] = info.found ? 1 : 0;
+  out[1] = info.es;
+  *out_sp = info.searchsp;
+  return c;
+}
+}

@@ -66,8 +66,16 @@ int main(int argc, char** argv) {
   std::vector<PreparedSubject> subjects;
   std::vector<std::string> names;
   for (auto& r : db) { subjects.push_back(prepare_subject(r.seq)); names.push_back(r.id); }
+  // average subject length for the KA ungapped cutoff (main.rs:603-610)
+  uint64_t db_letters = 0;
+  for (auto& r : db) db_letters += r.seq.size() - 2;
+  const uint64_t avg_subj = db.empty() ? 0 : db_letters / db.size();
+  const std::string matrix_name = get("--matrix", "");
   auto t0 = std::chrono::steady_clock::now();
   for (auto& q : qs) {
+    // the cutoff of this query, from its whole length (main.rs:1154-1170)
+    p.ungapped_cutoff = ungapped_cutoff(matrix_name, p.gap_open, p.gap_extend, p.min_raw_gapped_score,
+                                        q.seq.size() - 2, avg_subj, nullptr);
     if (!dump.empty()) {  // stage dumps: seeds | ungapped | prelims, per subject
       QueryLookup ql = build_query_lookup(q.seq, p);
       std::vector<uint8_t> qrc = revcomp(q.seq);

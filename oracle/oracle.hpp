@@ -210,6 +210,13 @@ void improve_seed(const uint8_t* query, size_t qlen, const uint8_t* subject, siz
                   uint32_t s_seed, uint32_t pqs, uint32_t pqe, uint32_t pss, uint32_t pse, bool minus, uint32_t& oq,
                   uint32_t& os);
 
+// ---- Karlin-Altschul ungapped cutoff (rmblast-lib/src/search/ka_cutoff.rs) ----
+struct KaCutoffInfo { bool found; double lambda, k; uint64_t searchsp; int es; };
+bool ka_lookup(const std::string& matrix_name, int gap_open, int gap_extend, double& lambda, double& k);  // :105-114
+// :126-153; query_length = the whole query record, avg_subj_length = db residues / sequences
+int ungapped_cutoff(const std::string& matrix_name, int gap_open, int gap_extend, int min_raw_gapped_score,
+                    uint64_t query_length, uint64_t avg_subj_length, KaCutoffInfo* info);
+
 // unit-level entry points used by tests
 void purge_prelim_common_endpoints(std::vector<PrelimHsp>& v, uint32_t sa_len);
 int purge_hsps_with_common_endpoints(std::vector<AlignResult*>& arr, bool purge);

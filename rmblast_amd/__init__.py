@@ -21,14 +21,20 @@ CLI_PATH = os.path.join(_HERE, "_build", "rmblastn")
 EXPORTED = [
     "rmb_matrix_load", "rmb_ctx_create", "rmb_ctx_destroy", "rmb_last_error", "rmb_library_load",
     "rmb_search", "rmb_search_pairs", "rmb_dust_intervals", "rmb_results_size", "rmb_results_copy", "rmb_stats", "rmb_tap_ungapped",
-    "rmb_tap_align_score_only", "rmb_tap_align_ex",
+    "rmb_tap_align_score_only", "rmb_tap_align_ex", "rmb_ka_lookup", "rmb_ungapped_cutoff",
 ]
 
 
 class RmbParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in (
         "gap_open", "gap_extend", "word_size", "xdrop_ungap", "xdrop_gap", "xdrop_gap_final",
-        "min_raw_gapped_score", "ungapped_cutoff", "complexity_adjust", "dust")]
+        "min_raw_gapped_score", "ungapped_cutoff", "complexity_adjust", "dust")] + [
+        ("ka_lambda", ctypes.c_double), ("ka_k", ctypes.c_double)]
+
+
+class RmbKaCutoffInfo(ctypes.Structure):
+    _fields_ = [("found", ctypes.c_int32), ("es", ctypes.c_int32), ("lambda_", ctypes.c_double),
+                ("k", ctypes.c_double), ("searchsp", ctypes.c_uint64)]
 
 
 class RmbMatrix(ctypes.Structure):
@@ -68,6 +74,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rmb_results_size.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), P(ctypes.c_uint64)]
     lib.rmb_results_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     lib.rmb_stats.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), ctypes.c_int]
+    lib.rmb_ka_lookup.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, P(ctypes.c_double),
+                                  P(ctypes.c_int32)]
+    lib.rmb_ungapped_cutoff.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.c_uint64, ctypes.c_uint64, P(ctypes.c_int32), P(RmbKaCutoffInfo)]
     lib.rmb_dust_intervals.argtypes = [u8p, ctypes.c_uint64, P(ctypes.c_uint64), P(ctypes.c_uint64)]
     lib.rmb_tap_ungapped.argtypes = [ctypes.c_void_p, u8p, ctypes.c_uint32, ctypes.c_uint32,
                                      P(ctypes.c_uint32), P(ctypes.c_uint64)]
@@ -110,6 +120,30 @@ def dust_intervals(bases) -> list:
     return [(int(out[2 * k]), int(out[2 * k + 1])) for k in range(n.value)]
 
 
+def ka_lookup(matrix_name: str, gap_open: int, gap_extend: int):
+    """(lambda, K) of a RepeatMasker matrix at its gap costs from NCBI's ALP
+    table (ka_cutoff.rs:105-114), or None."""
+    out = (ctypes.c_double * 2)()
+    found = ctypes.c_int32(0)
+    if lib().rmb_ka_lookup(matrix_name.encode(), gap_open, gap_extend, out, ctypes.byref(found)):
+        raise RmbError("rmb_ka_lookup failed")
+    return (out[0], out[1]) if found.value else None
+
+
+def ungapped_cutoff(matrix_name: str, gap_open: int, gap_extend: int, min_raw_gapped_score: int,
+                    query_length: int, avg_subj_length: int):
+    """(cutoff, info or None) as ka_cutoff.rs:126-161 computes them; info is
+    a dict {lambda, k, searchsp, es} when the KA branch applied."""
+    c = ctypes.c_int32(0)
+    info = RmbKaCutoffInfo()
+    if lib().rmb_ungapped_cutoff(matrix_name.encode(), gap_open, gap_extend, min_raw_gapped_score, query_length,
+                                 avg_subj_length, ctypes.byref(c), ctypes.byref(info)):
+        raise RmbError("rmb_ungapped_cutoff failed")
+    if not info.found:
+        return c.value, None
+    return c.value, {"lambda": info.lambda_, "k": info.k, "searchsp": info.searchsp, "es": info.es}
+
+
 def _u8ptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
 
@@ -123,10 +157,15 @@ class Searcher:
 
     def __init__(self, matrix: RmbMatrix, *, gap_open=20, gap_extend=5, word_size=11, xdrop_ungap=20,
                  xdrop_gap=30, xdrop_gap_final=100, min_raw_gapped_score=0, ungapped_cutoff=-1,
-                 complexity_adjust=False, device=0):
+                 complexity_adjust=False, device=0, matrix_name=None):
+        """matrix_name: the -matrix path; when it names a matrix of NCBI's
+        ALP table at these gap costs, the ungapped cutoff is resolved per
+        query record as rmblastn does (ungapped_cutoff = -1 only)."""
         self._lib = lib()
+        ka = ka_lookup(matrix_name, gap_open, gap_extend) if matrix_name else None
         p = RmbParams(gap_open, gap_extend, word_size, xdrop_ungap, xdrop_gap, xdrop_gap_final,
-                      min_raw_gapped_score, ungapped_cutoff, int(bool(complexity_adjust)), 0)
+                      min_raw_gapped_score, ungapped_cutoff, int(bool(complexity_adjust)), 0,
+                      ka[0] if ka else 0.0, ka[1] if ka else 0.0)
         self.params = p
         self.matrix = matrix
         ctx = ctypes.c_void_p()

@@ -20,7 +20,7 @@ struct RecDesc {        // one query record (a chunk of the genome)
   uint64_t q_base;      // offset of its sentinel-wrapped arrays in the query pools
   uint64_t cell_base;   // offset of its (4^lw + 1) cell starts in the cell pool
   uint32_t q_len;       // bases, without sentinels
-  uint32_t pad;
+  int32_t cutoff;       // ungapped keep threshold of the record's query (ka_cutoff.rs:126-161)
 };
 
 struct SubjDesc {       // one library sequence
@@ -30,7 +30,7 @@ struct SubjDesc {       // one library sequence
 };
 
 struct P1Params {
-  int32_t word_size, lut_width, scan_step, xdrop_ungap, cutoff;
+  int32_t word_size, lut_width, scan_step, xdrop_ungap;
   uint32_t n_subj;
   uint64_t n_items;          // records x subjects, or the length of item_list
   uint64_t out_cap;

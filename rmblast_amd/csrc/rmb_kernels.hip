@@ -531,7 +531,7 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
           UngOut u;
           // without the fast tier every seed takes the full extension below
           if (P.known_word && ungapped_first(qpool, qa, splus, sa, eq, minus, mat, P.xdrop_ungap, u)) {
-            const bool hit = u.score >= P.cutoff;
+            const bool hit = u.score >= R.cutoff;
             if (hit) my_hitres[i] = make_uint4(u.q_beg, u.q_end, (uint32_t)u.score, (uint32_t)eq);
             level = hit ? (uint32_t)es + (u.q_end - (uint32_t)eq) : chk + (uint32_t)ws;
             steps = u.steps;
@@ -573,7 +573,7 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
           const uint8_t* qpool = minus ? qrc : qfwd;
           const UngOut u = ungapped_chunked<kLongChunk>(qpool, (int64_t)R.q_base + 1 + eq, splus, (int64_t)S.s_base + 1 + es,
                                             eq, mat, P.xdrop_ungap);
-          const bool hit = u.score >= P.cutoff;
+          const bool hit = u.score >= R.cutoff;
           if (hit) my_hitres[i] = make_uint4(u.q_beg, u.q_end, (uint32_t)u.score, (uint32_t)eq);
           const uint32_t level = hit ? (uint32_t)es + (u.q_end - (uint32_t)eq) : e.y + (uint32_t)ws;
           const uint32_t steps = u.steps < 65535u ? u.steps : 65535u;

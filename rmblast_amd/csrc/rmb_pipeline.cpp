@@ -31,6 +31,7 @@
 #include "../../include/rmblast_hip.h"
 #include "rmb_device.h"
 #include "rmb_dust.h"
+#include "rmb_ka.h"
 
 #include <map>
 #include <unordered_map>
@@ -250,6 +251,7 @@ struct rmb_ctx {
   // library
   std::vector<Subject> subj;
   std::vector<SubjDesc> sdesc;
+  uint64_t avg_subj_len = 0;  // total residues / sequences, truncated (BlastSeqSrcGetAvgSeqLen)
   DevBuf d_splus, d_src, d_salign, d_sdesc;
   // results of the last search
   std::vector<rmb_hsp> hsps;
@@ -704,8 +706,6 @@ static void launch_group_phase1(rmb_ctx* c, hipStream_t st, P1Group& g, const ui
   P.lut_width = lw;
   P.scan_step = ws - lw + 1;
   P.xdrop_ungap = c->P.xdrop_ungap;
-  P.cutoff = c->P.ungapped_cutoff >= 0 ? c->P.ungapped_cutoff
-                                      : (c->P.min_raw_gapped_score > 0 ? c->P.min_raw_gapped_score / 2 : 1);
   P.n_subj = nsub;
   P.n_items = (uint64_t)R * nsub;
   P.item_list = nullptr;
@@ -1290,6 +1290,16 @@ static void emit_results(rmb_ctx* c, const std::vector<ItemState>& items) {
 }
 
 // pairs: null for every (query, subject) pair, else the item ids q * nsub + s to search
+// Ungapped keep threshold of a query record of full_len bases: the caller's
+// fixed value, else the KA-derived cutoff rmblastn resolves per query before
+// its search (main.rs:1154-1170, ka_cutoff.rs:126-161), else cutoff_score_max.
+static int32_t record_cutoff(const rmb_ctx* c, uint64_t full_len) {
+  if (c->P.ungapped_cutoff >= 0) return c->P.ungapped_cutoff;
+  const int32_t ka = ka_ungapped_cutoff(c->P.ka_lambda, c->P.ka_k, c->P.min_raw_gapped_score, full_len,
+                                        c->avg_subj_len, nullptr, nullptr);
+  return ka >= 0 ? ka : cutoff_score_max(c->P.min_raw_gapped_score);
+}
+
 static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const uint32_t* lens,
                         const std::vector<uint64_t>* pairs = nullptr) {
   Timer total;
@@ -1314,7 +1324,7 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
     sr.first[i] = (uint32_t)sr.recs.size();
     if (split[i].count <= 1) {
       sr.whole_base[i] = sr.pool;
-      sr.recs.push_back(RecDesc{sr.pool, 0, lens[i], 0});
+      sr.recs.push_back(RecDesc{sr.pool, 0, lens[i], record_cutoff(c, lens[i])});
       sr.query.push_back(i);
       sr.offset.push_back(0);
       sr.pool += lens[i] + 2;
@@ -1325,7 +1335,7 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
       if (cs >= lens[i]) break;
       const uint32_t ce = std::min<uint64_t>((uint64_t)cs + split[i].size, lens[i]);
       if (ce - cs < (uint32_t)ws) break;  // main.rs:1256-1258
-      sr.recs.push_back(RecDesc{sr.pool, 0, ce - cs, 0});
+      sr.recs.push_back(RecDesc{sr.pool, 0, ce - cs, record_cutoff(c, lens[i])});
       sr.query.push_back(i);
       sr.offset.push_back(cs);
       sr.pool += ce - cs + 2;
@@ -1579,6 +1589,8 @@ int rmb_ctx_create(int device, const rmb_params* params, const rmb_matrix* matri
     if (params->dust != 0 && params->dust != 1) throw Failure(RMB_ERR_ARG, "dust must be 0 or 1");
     if (params->word_size < 4) throw Failure(RMB_ERR_ARG, "word_size below 4");
     if (params->gap_extend < 1 || params->gap_open < 0) throw Failure(RMB_ERR_ARG, "bad gap costs");
+    if (!(params->ka_lambda >= 0.0) || !(params->ka_k >= 0.0))
+      throw Failure(RMB_ERR_ARG, "ka_lambda and ka_k must be 0 or positive");
     int n = 0;
     hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
     if (device < 0 || device >= n) throw Failure(RMB_ERR_ARG, "no such HIP device");
@@ -1678,6 +1690,7 @@ int rmb_library_load(rmb_ctx* c, uint32_t n, const uint8_t* const* seqs, const u
       c->sdesc[i].s_len = lens[i];
       off += lens[i] + 2;
     }
+    c->avg_subj_len = (off - 2ull * n) / n;
     std::vector<uint8_t> pp(off), pr(off), pa(off);
     for (uint32_t i = 0; i < n; ++i) {
       std::memcpy(pp.data() + c->sdesc[i].s_base, c->subj[i].plus.data(), c->subj[i].plus.size());
@@ -1720,6 +1733,30 @@ int rmb_search_pairs(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const ui
     hip_check(hipSetDevice(c->device), "hipSetDevice");
     search_impl(c, nq, qs, lens, &items);
   });
+}
+
+int rmb_ka_lookup(const char* matrix_name, int32_t gap_open, int32_t gap_extend, double out[2], int32_t* found) {
+  if (!matrix_name || !out || !found) return RMB_ERR_ARG;
+  double lambda = 0, k = 0;
+  *found = ka_lookup(matrix_name, gap_open, gap_extend, lambda, k) ? 1 : 0;
+  out[0] = lambda;
+  out[1] = k;
+  return RMB_OK;
+}
+
+int rmb_ungapped_cutoff(const char* matrix_name, int32_t gap_open, int32_t gap_extend, int32_t min_raw_gapped_score,
+                        uint64_t query_length, uint64_t avg_subj_length, int32_t* cutoff, rmb_ka_cutoff_info* info) {
+  if (!matrix_name || !cutoff) return RMB_ERR_ARG;
+  double lambda = 0, k = 0;
+  const bool hit = ka_lookup(matrix_name, gap_open, gap_extend, lambda, k);
+  int32_t es = 0;
+  uint64_t searchsp = 0;
+  const int32_t ka = hit ? ka_ungapped_cutoff(lambda, k, min_raw_gapped_score, query_length, avg_subj_length, &es,
+                                              &searchsp)
+                         : -1;
+  *cutoff = ka >= 0 ? ka : cutoff_score_max(min_raw_gapped_score);
+  if (info) *info = rmb_ka_cutoff_info{ka >= 0 ? 1 : 0, es, lambda, k, searchsp};
+  return RMB_OK;
 }
 
 int rmb_dust_intervals(const uint8_t* bases, uint64_t n, uint64_t* out, uint64_t* n_out) {
@@ -1773,6 +1810,7 @@ int rmb_tap_ungapped(rmb_ctx* c, const uint8_t* query, uint32_t q_len, uint32_t 
     std::vector<RecDesc> recs(1);
     recs[0].q_base = 0;
     recs[0].q_len = q_len;
+    recs[0].cutoff = record_cutoff(c, q_len);
     unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
     hip_check(hipMemset(ctr, 0, C_NCOUNTERS * sizeof(unsigned long long)), "memset counters");
     const int lw = lut_width_for(c->P.word_size, 2ull * q_len);

@@ -423,8 +423,15 @@ int main(int argc, char** argv) {
   P.xdrop_gap = std::stoi(opt("--xdrop-gap", "30"));
   P.xdrop_gap_final = std::stoi(opt("--xdrop-gap-final", "100"));
   P.min_raw_gapped_score = std::stoi(opt("--min-raw-gapped-score", "0"));
-  P.ungapped_cutoff = -1;
+  P.ungapped_cutoff = -1;  // per query record (main.rs:1154-1170)
   P.complexity_adjust = complexity;
+  const std::string matrix_path = opt("--matrix", "");
+  {
+    double ka[2] = {0, 0};
+    int32_t found = 0;
+    rmb_ka_lookup(matrix_path.c_str(), P.gap_open, P.gap_extend, ka, &found);
+    if (found) { P.ka_lambda = ka[0]; P.ka_k = ka[1]; }
+  }
   // --dust yes|true|no|false (rmblastn/src/main.rs:330-342); NCBI's
   // 'level window linker' form is rejected
   const std::string dust = opt("--dust", "yes");
@@ -483,7 +490,7 @@ int main(int argc, char** argv) {
   rmb_matrix M;
   {
     char mbuf[256] = {0};
-    if (rmb_matrix_load(opt("--matrix", "").c_str(), &M, mbuf, sizeof mbuf)) { fprintf(stderr, "error: matrix: %s\n", mbuf); return 1; }
+    if (rmb_matrix_load(matrix_path.c_str(), &M, mbuf, sizeof mbuf)) { fprintf(stderr, "error: matrix: %s\n", mbuf); return 1; }
   }
   std::vector<Record> db, qs;
   if (!load_db(kv["--db"], db, err) || !load_fasta(kv["--query"], qs, err)) {
@@ -536,6 +543,27 @@ int main(int argc, char** argv) {
     std::vector<const uint8_t*> qp;
     std::vector<uint32_t> ql;
     for (auto& r : qs) { qp.push_back(r.bases.data()); ql.push_back((uint32_t)r.bases.size()); }
+    // The run-log note of a KA-derived ungapped cutoff, once, with the first
+    // query's numbers (main.rs:1098-1131)
+    if (P.ka_lambda > 0 && P.ka_k > 0) {
+      uint64_t letters = 0;
+      for (auto& r : db) letters += r.bases.size();
+      int32_t cut = 0;
+      rmb_ka_cutoff_info info{};
+      rmb_ungapped_cutoff(matrix_path.c_str(), P.gap_open, P.gap_extend, P.min_raw_gapped_score, ql[0],
+                          letters / db.size(), &cut, &info);
+      if (info.found) {
+        const size_t slash = matrix_path.rfind('/');
+        const std::string base = slash == std::string::npos ? matrix_path : matrix_path.substr(slash + 1);
+        fprintf(stderr, "# rmblastn: Karlin-Altschul ungapped cutoff IN EFFECT (matrix=%s gapopen=%d gapextend=%d)\n",
+                base.c_str(), P.gap_open, P.gap_extend);
+        fprintf(stderr, "#   lambda=%.10f K=%.10f E=0.05 avg_db_seq_len=%llu searchsp=%llu -> KA score=%d\n",
+                info.lambda, info.k, (unsigned long long)(letters / db.size()), (unsigned long long)info.searchsp,
+                info.es);
+        fprintf(stderr, "#   ungapped_cutoff = %d (overrides default min_raw_gapped_score/2 = %d)\n", cut,
+                P.min_raw_gapped_score / 2);
+      }
+    }
     if (rmb_search(ctx, (uint32_t)qs.size(), qp.data(), ql.data())) return fail("search");
     rmb_results_size(ctx, &nh, &no);
     H.resize(nh);
