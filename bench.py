@@ -4,18 +4,26 @@ Workload (the exact-path-at-scale configuration): a 100 Mbp synthetic genome
 split into 1,667 records of 60 kbp, searched on both strands against 1,000
 synthetic families (log-normal lengths, 100-15,000 bp), word size 7, gap
 costs 20/5, X-drop 20/30/100, min_raw_gapped_score 200, complexity adjustment,
-no DUST.  One step = one rmb_search over all records (lookup build, seeding,
-ungapped, preliminary and traceback DP, ordered replay), with the library
-already resident on the GPU.  Metric: query Mbp x families per second.
+mask level 80, no DUST.  One step = one rmb_search over all records (lookup
+build, seeding, ungapped, preliminary and traceback DP, ordered replay) plus
+the host tail into final outfmt-6 rows in memory (mask level, output order,
+alignment statistics, the RepeatMasker field list), with the library already
+resident on the GPU.  Metric: query Mbp x families per second.
 
 --workload gappy measures the divergent/gappy stress configuration instead:
 one 10 Mbp record (the split query path) against 300 families of 5-15 kbp,
 copies 25-40% diverged with 5-8% indels, 5% microsatellite tracts, X-drop
 gapped/final 120/400; its CPU baseline sample is the record's first 1.5 Mbp.
 
+The CPU baseline is the oracle (a C++ restatement of the reference's search
+path) built for this host (-march=native) at run time, on all the host cores
+this process may use and on one core, over a bounded sample of the same
+records; the GPU's rows for those records are compared with the oracle's.
+
 Prints one JSON line.  --gpus N with N > 1 is refused: this build runs on one GPU.
 """
 import argparse
+import hashlib
 import json
 import os
 import subprocess
@@ -32,6 +40,9 @@ HBM_PEAK_GBS = 8000.0
 INT32_VALU_PEAK = 256 * 64 * 2.4e9   # 256 CUs x 64 lanes x 2.4 GHz, int32 ops/s
 OPS_PER_UNGAPPED_STEP = 6
 OPS_PER_DP_CELL = 12
+RM_FIELDS = ("score perc_sub perc_query_gap perc_db_gap qseqid qstart qend qlen sstrand sseqid sstart send slen "
+             "kdiv cpg_kdiv transi transv cpg_sites qseq sseq")
+MASK_LEVEL = 80
 
 
 def parse():
@@ -43,7 +54,8 @@ def parse():
     ap.add_argument("--records", type=int, default=1667, help="records of the 100 Mbp genome to search")
     ap.add_argument("--families", type=int, default=1000)
     ap.add_argument("--cpu-records", type=int, default=300,
-                    help="records in the CPU baseline sample (about 15 s of 16-core work)")
+                    help="records in the all-cores CPU baseline sample (about 15 s of 16-core work)")
+    ap.add_argument("--cpu1-records", type=int, default=20, help="records in the one-core CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB three times (the reference "
@@ -63,6 +75,20 @@ def copy_calibration(mib):
     torch.cuda.synchronize()
     print(f"[bench] copied {n} bytes x 3", file=sys.stderr)
     return 0
+
+
+def host_cores():
+    """Cores this process may use: its CPU affinity, within the job's CPU share
+    where the environment states one (OMP_NUM_THREADS)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    return max(1, n)
+
+
+def kernel_source_sha():
+    return hashlib.sha256(open(os.path.join(ROOT, "rmblast_amd", "csrc", "rmb_kernels.hip"), "rb").read()).hexdigest()[:16]
 
 
 def main():
@@ -86,6 +112,11 @@ def main():
     print(f"[bench] workload generated in {gen_s:.1f}s: {len(w.records)} records x {len(w.families)} families",
           file=sys.stderr, flush=True)
     tmp = tempfile.mkdtemp(prefix="rmb_bench_")
+    oracle_build = None
+    if not a.no_cpu:  # the CPU baseline's oracle, built for this host while the GPU runs
+        oracle_build = subprocess.Popen(
+            ["make", "-s", "-C", os.path.join(ROOT, "oracle"), "-j8", "MARCH=native", "OUT=" + tmp + "/oracle",
+             tmp + "/oracle/rmblastn_oracle"], stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True)
     mpath = os.path.join(tmp, "synth.matrix")
     with open(mpath, "w") as f:
         f.write(synth.MATRIX_TEXT)
@@ -96,47 +127,74 @@ def main():
     s = rmblast_amd.Searcher(m, **params)
     s.load_library([f[1] for f in w.families])
     queries = [r[1] for r in w.records]
+    qids = [r[0] for r in w.records]
+    sids = [f[0] for f in w.families]
     mbp = sum(len(q) for q in queries) / 1e6
     nfam = len(w.families)
 
-    for i in range(a.warmup):
-        s.search(queries)
-        print(f"[bench] warmup {i + 1}/{a.warmup} done", file=sys.stderr, flush=True)
-    times, stats = [], []
-    for i in range(a.steps):
+    def step():
         t = time.perf_counter()
-        hsps, ops = s.search(queries)
-        times.append(time.perf_counter() - t)
+        hsps, _ = s.search(queries)
+        t_search = time.perf_counter() - t
+        rows = s.report_tabular(RM_FIELDS, qids, sids, MASK_LEVEL)
+        return time.perf_counter() - t, t_search, len(hsps), rows
+
+    for i in range(a.warmup):
+        step()
+        print(f"[bench] warmup {i + 1}/{a.warmup} done", file=sys.stderr, flush=True)
+    times, search_times, stats = [], [], []
+    rows = b""
+    for i in range(a.steps):
+        dt, ds, nh, rows = step()
+        times.append(dt)
+        search_times.append(ds)
         stats.append(s.stats())
-        print(f"[bench] step {i + 1}/{a.steps}: {times[-1]:.3f}s, {len(hsps)} HSPs", file=sys.stderr, flush=True)
-    total = sum(times)
-    step_s = total / a.steps
+        nrows = rows.count(b"\n")
+        print(f"[bench] step {i + 1}/{a.steps}: {dt:.3f}s ({ds:.3f}s search), {nh} HSPs, {nrows} rows",
+              file=sys.stderr, flush=True)
+    step_s = sum(times) / a.steps
+    search_s = sum(search_times) / a.steps
     st = stats[-1]
     value = mbp * nfam / step_s
 
-    # roofline of the dominant kernel (the seeding kernel): algorithmic scan
-    # bytes = ceil(L/4) of the streamed sequence per (streamed sequence, lookup
-    # table) pair; here every family is streamed against each record's table,
-    # so records x sum_f ceil(L_f/4); achieved = bytes / HIP-event kernel time.
+    # Roofline of the dominant kernel, by measured time: the seeding kernel
+    # (6 int32 ops per ungapped extension step) or the preliminary DPs (12
+    # int32 ops per DP cell), against the int32 VALU peak; each kernel's time
+    # is its HIP-event time on the stream it is launched on, summed over the
+    # search's launches.
+    p1_ns = max(st["t_phase1_kernel_ns"], 1)
+    pre_ns = max(st["t_prelim_kernel_ns"], 1)
+    prelim_cells = st["dp_cells"] - st["tb_cells"]
+    p1_ops = OPS_PER_UNGAPPED_STEP * st["ungapped_steps"]
+    pre_ops = OPS_PER_DP_CELL * prelim_cells
+    launches = max(st["phase1_launches"], 1)
     fam_len = np.array([len(f[1]) for f in w.families], dtype=np.float64)
-    algo_bytes = float(len(queries)) * float(np.ceil(fam_len / 4).sum())
-    k_ns = max(st["t_phase1_kernel_ns"], 1)
-    achieved = algo_bytes / (k_ns * 1e-9) / 1e9
-    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on the
-    # full workload, scaled by a plain copy's readings (profiles/)
+    scan_bytes = float(len(queries)) * float(np.ceil(fam_len / 4).sum())
+    hbm = {"kernel": "k_phase1", "algorithmic_bytes": scan_bytes,
+           "achieved": round(scan_bytes / (p1_ns * 1e-9) / 1e9, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    hbm["frac"] = round(hbm["achieved"] / HBM_PEAK_GBS, 6)
+    if p1_ns >= pre_ns:
+        kern, ops, ns = "k_phase1", p1_ops, p1_ns
+        per = f"{OPS_PER_UNGAPPED_STEP} int32 ops per ungapped extension step x {st['ungapped_steps']} steps"
+    else:
+        kern, ops, ns = "k_prelim_fused", pre_ops, pre_ns
+        per = f"{OPS_PER_DP_CELL} int32 ops per DP cell x {prelim_cells} preliminary DP cells"
+    achieved = ops / (ns * 1e-9) / 1e12
+    # HBM bytes per launch of the seeding kernel, measured with rocprofv3
+    # FETCH_SIZE / WRITE_SIZE on this workload and scaled by a plain copy's
+    # readings; used only while the kernel source is the one measured.
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "phase1_traffic.json")
-    if os.path.exists(tpath) and len(queries) == 1667 and nfam == 1000:
-        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch_measured")
-    valu_ops = OPS_PER_UNGAPPED_STEP * st["ungapped_steps"]
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "kernel": "k_phase1", "kernel_ms": round(k_ns / 1e6, 3),
-                "int32_valu": {"ops": valu_ops, "achieved_tops": round(valu_ops / (k_ns * 1e-9) / 1e12, 3),
-                               "peak_tops": round(INT32_VALU_PEAK / 1e12, 1),
-                               "frac": round(valu_ops / (k_ns * 1e-9) / INT32_VALU_PEAK, 5)}}
-    # gapped DP throughput: score-pass cells of the preliminary and traceback
-    # DPs over the time inside the gapped kernels (HIP events on their stream)
+    if kern == "k_phase1" and os.path.exists(tpath) and len(queries) == 1667 and nfam == 1000 and not gappy:
+        tj = json.load(open(tpath))
+        if tj.get("kernel_source_sha16") == kernel_source_sha():
+            traffic = tj.get("hbm_bytes_per_launch_measured")
+    roofline = {"bound": "int32_valu", "achieved": round(achieved, 4), "peak": round(INT32_VALU_PEAK / 1e12, 2),
+                "unit": "Tops/s", "frac": round(achieved / (INT32_VALU_PEAK / 1e12), 5), "traffic": traffic,
+                "kernel": kern, "kernel_ms": round(ns / 1e6, 3), "launches": launches if kern == "k_phase1" else None,
+                "algorithmic_ops": ops, "ops_definition": per, "hbm": hbm}
+    # gapped DP throughput: DP cells of the preliminary and traceback DPs (the
+    # reference's TOTAL_DP_CELLS unit) over the time inside the gapped kernels
     g_ns = st["t_gapped_kernel_ns"]
     gcups = st["dp_cells"] / g_ns if g_ns > 0 else 0.0
     ticks = [st[k] for k in ("p1_ticks_scan", "p1_ticks_reserve", "p1_ticks_extend", "p1_ticks_bucket",
@@ -147,17 +205,24 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        if gappy:
-            sample = [(w.records[0][0], w.records[0][1][:1_500_000])]
-            cpu = cpu_baseline(sample, w.families, tmp, mpath, xdrop, "the record's first 1.5 Mbp")
+        err = oracle_build.communicate()[1]
+        exe = os.path.join(tmp, "oracle", "rmblastn_oracle")
+        if oracle_build.returncode != 0 or not os.path.exists(exe):
+            print(f"[bench] native oracle build failed: {err[-500:]}", file=sys.stderr)
         else:
-            cpu = cpu_baseline(w.records[:a.cpu_records], w.families, tmp, mpath, xdrop,
-                               f"first {a.cpu_records} records")
+            if gappy:
+                sample = [(w.records[0][0], w.records[0][1][:1_500_000])]
+                cpu = cpu_baseline(exe, sample, sample, w.families, tmp, mpath, xdrop, "the record's first 1.5 Mbp")
+            else:
+                cpu = cpu_baseline(exe, w.records[:a.cpu_records], w.records[:a.cpu1_records], w.families, tmp,
+                                   mpath, xdrop, f"first {a.cpu_records} records",
+                                   gpu_rows=rows, sample_ids=set(qids[:a.cpu_records]))
     workload = ("divergent/gappy stress: 1 x 10 Mbp record x 300 families of 5-15 kbp, 25-40% divergence, "
                 "5-8% indels, X-drop 4x, ws 7" if gappy else
-                "exact path at scale: 1,667 x 60 kbp records x 1,000 families, ws 7")
+                "exact path at scale: 1,667 x 60 kbp records x 1,000 families, ws 7, search + host tail to "
+                "outfmt-6 rows")
     line = {
-        "metric": "query Mbp x families per second (exact search path)",
+        "metric": "query Mbp x families per second (exact search path, final outfmt-6 rows in memory)",
         "value": round(value, 2),
         "unit": "Mbp*families/s",
         "n_gpus": 1,
@@ -171,7 +236,9 @@ def main():
         "data": "synthetic (seeded generator, rmblast_amd/synth.py)",
         "config": {"workload": workload,
                    "records": len(queries), "families": nfam, "query_mbp": round(mbp, 3), "word_size": 7,
-                   "min_raw_gapped_score": 200, "xdrop": xdrop, "complexity_adjust": True, "dust": False},
+                   "min_raw_gapped_score": 200, "xdrop": xdrop, "complexity_adjust": True, "dust": False,
+                   "mask_level": MASK_LEVEL, "outfmt": "6 " + RM_FIELDS},
+        "search_only": {"ms_per_step": round(search_s * 1e3, 2), "value": round(mbp * nfam / search_s, 2)},
         "roofline": roofline,
         "gapped_gcups": round(gcups, 3),
         "gapped_kernel_ms": round(g_ns / 1e6, 3),
@@ -180,38 +247,66 @@ def main():
         "phase1_stage_frac": p1_stages,
         "counters": {k: st[k] for k in ("seeds_staged", "seed_evals", "ungapped_hits", "prelim_dps",
                                          "traceback_dps", "dp_cells", "tb_cells", "hsps", "ungapped_steps")},
+        "rows": rows.count(b"\n"),
         "cpu_baseline": cpu,
     }
+    if cpu and "parity" in cpu:
+        line["parity"] = cpu["parity"]
     print(json.dumps(line))
     return 0
 
 
-def cpu_baseline(records, families, tmp, mpath, xdrop, what):
-    """The CPU oracle (a restatement of the reference's search path) on the
-    sample `records` x all families, threads over subjects per record like
-    rmblastn's default thread mode."""
+def run_oracle(exe, records, families, tmp, mpath, xdrop, threads, tag):
     from rmblast_amd import synth
-    exe = os.path.join(ROOT, "oracle", "_build", "rmblastn_oracle")
-    if not os.path.exists(exe):
-        return None
-    cores = min(16, os.cpu_count() or 1)
-    q = os.path.join(tmp, "cpu_q.fa")
+    q = os.path.join(tmp, f"cpu_q_{tag}.fa")
     db = os.path.join(tmp, "cpu_db.fa")
     synth.to_fasta(records, q)
-    synth.to_fasta(families, db)
+    if not os.path.exists(db):
+        synth.to_fasta(families, db)
     args = [exe, "--db", db, "--query", q, "--matrix", mpath, "--gapopen", "20", "--gapextend", "5", "--word-size",
             "7", "--min-raw-gapped-score", "200", "--xdrop-ungap", str(xdrop[0]), "--xdrop-gap", str(xdrop[1]),
-            "--xdrop-gap-final", str(xdrop[2]), "--complexity-adjust", "--dust", "no", "--num-threads", str(cores),
-            "--outfmt", "6 score qseqid qstart qend sstrand sseqid sstart send"]
-    r = subprocess.run(args, capture_output=True, text=True, env=dict(os.environ, RMB_ORACLE_TIME="1"),
-                       timeout=900)
+            "--xdrop-gap-final", str(xdrop[2]), "--complexity-adjust", "--dust", "no", "--mask-level",
+            str(MASK_LEVEL), "--num-threads", str(threads), "--outfmt", "6 " + RM_FIELDS]
+    r = subprocess.run(args, capture_output=True, env=dict(os.environ, RMB_ORACLE_TIME="1"), timeout=900)
     if r.returncode != 0:
+        print(f"[bench] oracle failed: {r.stderr[-500:]}", file=sys.stderr)
+        return None, None
+    secs = float([l for l in r.stderr.decode().splitlines() if l.startswith("oracle_search_seconds")][0].split()[1])
+    return secs, r.stdout
+
+
+def cpu_baseline(exe, records, records1, families, tmp, mpath, xdrop, what, gpu_rows=None, sample_ids=None):
+    """The CPU oracle (a restatement of the reference's search path, built
+    -march=native on this host) on `records` x all families with threads over
+    subjects per record like rmblastn's default thread mode, on every core this
+    process may use, and on `records1` with one thread.  Its timed region runs
+    from encoded inputs to the final outfmt-6 rows in memory."""
+    cores = host_cores()
+    secs, out = run_oracle(exe, records, families, tmp, mpath, xdrop, cores, "n")
+    if secs is None:
         return None
-    secs = float([l for l in r.stderr.splitlines() if l.startswith("oracle_search_seconds")][0].split()[1])
+    print(f"[bench] CPU baseline ({cores} threads): {secs:.2f}s", file=sys.stderr, flush=True)
     mbp = sum(len(x[1]) for x in records) / 1e6
-    return {"value": round(mbp * len(families) / secs, 2), "unit": "Mbp*families/s", "cores": cores,
-            "kind": "port", "seconds": round(secs, 3),
-            "sample": f"{what} ({mbp:.2f} Mbp) x {len(families)} families, same parameters"}
+    res = {"value": round(mbp * len(families) / secs, 2), "unit": "Mbp*families/s", "cores": cores,
+           "kind": "port", "seconds": round(secs, 3),
+           "sample": f"{what} ({mbp:.2f} Mbp) x {len(families)} families, same parameters and fields, "
+                     f"oracle built -march=native on this host"}
+    secs1, _ = run_oracle(exe, records1, families, tmp, mpath, xdrop, 1, "1")
+    if secs1 is not None:
+        mbp1 = sum(len(x[1]) for x in records1) / 1e6
+        res["single_core"] = {"value": round(mbp1 * len(families) / secs1, 2), "seconds": round(secs1, 3),
+                              "sample": f"first {len(records1)} records ({mbp1:.2f} Mbp)"}
+        print(f"[bench] CPU baseline (1 thread): {secs1:.2f}s", file=sys.stderr, flush=True)
+    if gpu_rows is not None:
+        # the GPU's rows of the sample's records, in its order, against the oracle's
+        g = [l for l in gpu_rows.split(b"\n") if l and l.split(b"\t")[4].decode() in sample_ids]
+        o = [l for l in out.split(b"\n") if l]
+        same = g == o
+        res["parity"] = {"records": len(records), "gpu_rows": len(g), "oracle_rows": len(o),
+                         "identical": same}
+        print(f"[bench] parity on the CPU sample: {'identical' if same else 'DIFFERENT'} "
+              f"({len(g)} GPU rows, {len(o)} oracle rows)", file=sys.stderr, flush=True)
+    return res
 
 
 if __name__ == "__main__":

@@ -171,6 +171,53 @@ int rmb_dust_intervals(const uint8_t* bases, uint64_t n, uint64_t* out, uint64_t
 int rmb_results_size(const rmb_ctx* ctx, uint64_t* n_hsps, uint64_t* n_ops);
 int rmb_results_copy(const rmb_ctx* ctx, rmb_hsp* hsps, rmb_op* ops);
 
+/* ---- the host tail after a search (rmblastn/src/main.rs:1224-1234) ------- */
+
+/* Alignment statistics of one HSP (rmblast-lib/src/stats.rs:52-243,
+ * compute_align_stats) from its aligned IUPAC strings of length n ('-' for a
+ * gap), the subject taken as the ancestral sequence. */
+typedef struct rmb_align_stats {
+  uint32_t identities, mismatches, query_gaps, subject_gaps;
+  uint32_t transitions, transversions, cpg_sites, pad;
+  double kdiv, cpg_kdiv;
+  float perc_sub, perc_query_gap, perc_db_gap, pad2;
+} rmb_align_stats;
+int rmb_align_stats_of(const char* query_aligned, const char* subject_aligned, uint64_t n, rmb_align_stats* out);
+
+/* Mask level over one query's HSPs from all subjects (apply_mask_level in
+ * its cross-subject form, engine.rs:863-974): sorted by score, then subject
+ * index (the OID) descending, then ScoreCompareHSPs (hits.rs:208-216), an HSP
+ * is dropped when an accepted HSP of at least its score covers
+ * (int)(100 * overlap / span) >= mask_level percent of its query range
+ * (minus-strand ranges shifted by one; at level 0 touching counts).  q_len is
+ * the query's length; keep[i] receives 1 for a survivor.  Host only. */
+int rmb_mask_level(const rmb_hsp* hsps, uint64_t n, uint32_t q_len, int32_t mask_level, uint8_t* keep);
+
+/* The HSPs of the last search that rmblastn reports, in its output order:
+ * per query, mask level across all subjects (apply_mask_level,
+ * engine.rs:863-974; mask_level < 0 or >= 100 keeps every HSP), then NCBI's
+ * order (sort_hit_list_order_by_oid, engine.rs:998-1017, after main.rs:
+ * 1230-1234).  rows receives indices into the results of rmb_results_copy,
+ * queries in batch order.  Two-call: rows = NULL gives the count in *n. */
+int rmb_report_order(rmb_ctx* ctx, int32_t mask_level, uint64_t* rows, uint64_t* n);
+
+/* outfmt 6 of the last search (output.rs:294-363): the rows of
+ * rmb_report_order with the tab-separated fields, each line ending in '\n'.
+ * Fields: score perc_sub perc_query_gap perc_db_gap qseqid qstart qend qlen
+ * sstrand sseqid sstart send slen kdiv cpg_kdiv transi transv cpg_sites qseq
+ * sseq (with NCBI's column quirks, output.rs:338-340).  Queries are formatted
+ * on the library's host threads.  The text is held by the context until the
+ * next search or report; rmb_report_copy copies it out. */
+typedef struct rmb_report_spec {
+  int32_t mask_level;
+  int32_t pad;
+  const char* fields;             /* field names separated by spaces */
+  const char* const* query_ids;   /* qseqid of each query of the last search */
+  const char* const* subject_ids; /* sseqid of each loaded subject */
+} rmb_report_spec;
+int rmb_report_tabular(rmb_ctx* ctx, const rmb_report_spec* spec, uint64_t* n_bytes);
+int rmb_report_copy(const rmb_ctx* ctx, char* out);
+
 /* Stage counters of the last search (the reference's diagnostics counters,
  * engine.rs:28-37 and gapped.rs:25-36): [0] seeds that reached the diagonal
  * hash, [1] ungapped hits kept, [2] preliminary DPs run, [3] traceback DPs run,

@@ -121,6 +121,24 @@ int orc_mask_level(const uint32_t* h, int n, uint32_t q_len, int mask_level, uin
   return (int)r.size();
 }
 
+// engine.rs:863-974, cross-subject form: 7 u32 per HSP {score, q_start, q_end,
+// s_start, s_end, strand, subject}; subject k is the k-th of n_subjects names
+int orc_mask_level_xs(const uint32_t* h, int n, uint32_t q_len, int n_subjects, int mask_level, uint8_t* keep) {
+  std::vector<AlignResult> r(n);
+  std::vector<std::string> names;
+  for (int k = 0; k < n_subjects; ++k) names.push_back("s" + std::to_string(k));
+  for (int i = 0; i < n; ++i) {
+    const uint32_t* x = h + 7 * i;
+    r[i].hsp = Hsp{(int32_t)x[0], x[1], x[2], q_len, x[3], x[4], 0, x[5] ? MINUS : PLUS, {}, {}, {}};
+    r[i].subject_id = names[x[6]];
+    r[i].query_id = std::to_string(i);
+  }
+  apply_mask_level(r, mask_level, names);
+  for (int i = 0; i < n; ++i) keep[i] = 0;
+  for (auto& x : r) keep[std::stoi(x.query_id)] = 1;
+  return (int)r.size();
+}
+
 // engine.rs:1969-2083; prelims: 6 u32 {score, q_start, q_end, s_start, s_end, strand}
 int orc_purge_prelims(const uint32_t* h, int n, uint32_t sa_len, uint8_t* keep) {
   std::vector<PrelimHsp> v(n);

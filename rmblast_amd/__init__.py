@@ -21,7 +21,8 @@ CLI_PATH = os.path.join(_HERE, "_build", "rmblastn")
 EXPORTED = [
     "rmb_matrix_load", "rmb_ctx_create", "rmb_ctx_destroy", "rmb_last_error", "rmb_library_load",
     "rmb_search", "rmb_search_pairs", "rmb_dust_intervals", "rmb_results_size", "rmb_results_copy", "rmb_stats", "rmb_tap_ungapped",
-    "rmb_tap_align_score_only", "rmb_tap_align_ex", "rmb_ka_lookup", "rmb_ungapped_cutoff",
+    "rmb_tap_align_score_only", "rmb_tap_align_ex", "rmb_ka_lookup", "rmb_ungapped_cutoff", "rmb_mask_level",
+    "rmb_report_order", "rmb_report_tabular", "rmb_report_copy", "rmb_align_stats_of",
 ]
 
 
@@ -30,6 +31,11 @@ class RmbParams(ctypes.Structure):
         "gap_open", "gap_extend", "word_size", "xdrop_ungap", "xdrop_gap", "xdrop_gap_final",
         "min_raw_gapped_score", "ungapped_cutoff", "complexity_adjust", "dust")] + [
         ("ka_lambda", ctypes.c_double), ("ka_k", ctypes.c_double)]
+
+
+class RmbReportSpec(ctypes.Structure):
+    _fields_ = [("mask_level", ctypes.c_int32), ("pad", ctypes.c_int32), ("fields", ctypes.c_char_p),
+                ("query_ids", ctypes.POINTER(ctypes.c_char_p)), ("subject_ids", ctypes.POINTER(ctypes.c_char_p))]
 
 
 class RmbKaCutoffInfo(ctypes.Structure):
@@ -78,6 +84,11 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                   P(ctypes.c_int32)]
     lib.rmb_ungapped_cutoff.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                         ctypes.c_uint64, ctypes.c_uint64, P(ctypes.c_int32), P(RmbKaCutoffInfo)]
+    lib.rmb_mask_level.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int32,
+                                   P(ctypes.c_uint8)]
+    lib.rmb_report_order.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_uint64), P(ctypes.c_uint64)]
+    lib.rmb_report_tabular.argtypes = [ctypes.c_void_p, P(RmbReportSpec), P(ctypes.c_uint64)]
+    lib.rmb_report_copy.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
     lib.rmb_dust_intervals.argtypes = [u8p, ctypes.c_uint64, P(ctypes.c_uint64), P(ctypes.c_uint64)]
     lib.rmb_tap_ungapped.argtypes = [ctypes.c_void_p, u8p, ctypes.c_uint32, ctypes.c_uint32,
                                      P(ctypes.c_uint32), P(ctypes.c_uint64)]
@@ -142,6 +153,17 @@ def ungapped_cutoff(matrix_name: str, gap_open: int, gap_extend: int, min_raw_ga
     if not info.found:
         return c.value, None
     return c.value, {"lambda": info.lambda_, "k": info.k, "searchsp": info.searchsp, "es": info.es}
+
+
+def mask_level(hsps: np.ndarray, q_len: int, level: int) -> np.ndarray:
+    """apply_mask_level (engine.rs:863-974, cross-subject form) over one
+    query's HSPs (an HSP_DTYPE array): a bool array of the survivors."""
+    h = np.ascontiguousarray(hsps, dtype=HSP_DTYPE)
+    keep = np.zeros(len(h), dtype=np.uint8)
+    if lib().rmb_mask_level(h.ctypes.data_as(ctypes.c_void_p), len(h), q_len, level,
+                            keep.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))):
+        raise RmbError("rmb_mask_level failed")
+    return keep.astype(bool)
 
 
 def _u8ptr(a: np.ndarray):
@@ -228,6 +250,29 @@ class Searcher:
         self._lib.rmb_results_copy(self._ctx, hsps.ctypes.data_as(ctypes.c_void_p),
                                    ops.ctypes.data_as(ctypes.c_void_p))
         return hsps, ops
+
+    def report_order(self, mask_level: int) -> np.ndarray:
+        """Indices of the last search's HSPs that rmblastn reports, in its order."""
+        n = ctypes.c_uint64(0)
+        self._check(self._lib.rmb_report_order(self._ctx, mask_level, None, ctypes.byref(n)), "rmb_report_order")
+        rows = np.zeros(n.value, dtype=np.uint64)
+        self._check(self._lib.rmb_report_order(self._ctx, mask_level,
+                                               rows.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                               ctypes.byref(n)), "rmb_report_order")
+        return rows
+
+    def report_tabular(self, fields: str, query_ids, subject_ids, mask_level: int = 101) -> bytes:
+        """outfmt 6 text of the last search (mask level, order, statistics and
+        rows on the library's host threads)."""
+        qids = (ctypes.c_char_p * len(query_ids))(*[q.encode() for q in query_ids])
+        sids = (ctypes.c_char_p * len(subject_ids))(*[x.encode() for x in subject_ids])
+        spec = RmbReportSpec(mask_level, 0, fields.encode(), qids, sids)
+        n = ctypes.c_uint64(0)
+        self._check(self._lib.rmb_report_tabular(self._ctx, ctypes.byref(spec), ctypes.byref(n)),
+                    "rmb_report_tabular")
+        buf = ctypes.create_string_buffer(max(n.value, 1))
+        self._check(self._lib.rmb_report_copy(self._ctx, buf), "rmb_report_copy")
+        return buf.raw[:n.value]
 
     def stats(self):
         out = (ctypes.c_uint64 * len(STAT_NAMES))()

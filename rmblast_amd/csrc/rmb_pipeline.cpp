@@ -36,6 +36,7 @@
 #include <map>
 #include <unordered_map>
 #include "rmb_replay.h"
+#include "rmb_report.h"
 
 namespace rmb {
 hipError_t launch_lut_keys(const RecDesc*, uint32_t, const uint64_t*, const uint8_t*, int, int, int, uint64_t*,
@@ -261,6 +262,9 @@ struct rmb_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // scratch
   HostBuf h_qfwd, h_qrc;  // host query pools of the last search
+  std::vector<const uint8_t*> last_q;  // the last search's whole queries (wrapped) in h_qfwd, for the report
+  std::vector<uint32_t> last_qlen;
+  std::string report;                  // text of the last rmb_report_tabular
   DevBuf b_qfwd, b_qrc, b_recs, b_elem, b_keys, b_keys2, b_vals, b_vals2, b_counts, b_cells, b_sort_tmp;
   DevBuf b_hits, b_hitcount, b_spill, b_hitres, b_queue, b_counters, b_err;
   DevBuf b_tasks, b_dpres, b_ring, b_ring_big, b_tbtasks, b_tbres, b_flat, b_rows, b_ops;
@@ -1308,6 +1312,9 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
   c->host_prelim_dps = 0;
   c->hsps.clear();
   c->ops.clear();
+  c->last_q.clear();
+  c->last_qlen.clear();
+  c->report.clear();
   const int ws = c->P.word_size;
   const uint32_t nsub = (uint32_t)c->subj.size();
   // ---- query pools.  A query of 1,999,800 bases or more is searched
@@ -1367,6 +1374,9 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
           put(recs[k].q_base, qs[i] + sr.offset[k], recs[k].q_len);
     });
     for (uint32_t i = 0; i < nq; ++i) qf[i] = SeqView{pf + sr.whole_base[i], (size_t)lens[i] + 2};
+    c->last_q.resize(nq);
+    c->last_qlen.assign(lens, lens + nq);
+    for (uint32_t i = 0; i < nq; ++i) c->last_q[i] = qf[i].data();
     hip_check(hipMemcpyAsync(d_qfwd, pf, sr.pool, hipMemcpyHostToDevice, c->st1), "copy qfwd");
     hip_check(hipMemcpyAsync(d_qrc, pr, sr.pool, hipMemcpyHostToDevice, c->st1), "copy qrc");
   }
@@ -1657,6 +1667,10 @@ int rmb_library_load(rmb_ctx* c, uint32_t n, const uint8_t* const* seqs, const u
   return guarded(c, [&]() {
     if (n == 0 || !seqs || !lens) throw Failure(RMB_ERR_ARG, "empty library");
     hip_check(hipSetDevice(c->device), "hipSetDevice");
+    c->hsps.clear();  // results of an earlier search name the old library's subjects
+    c->ops.clear();
+    c->last_q.clear();
+    c->last_qlen.clear();
     c->subj.assign(n, Subject{});
     c->sdesc.assign(n, SubjDesc{});
     uint64_t off = 0;
@@ -1733,6 +1747,111 @@ int rmb_search_pairs(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const ui
     hip_check(hipSetDevice(c->device), "hipSetDevice");
     search_impl(c, nq, qs, lens, &items);
   });
+}
+
+// The last search's HSPs and sequences for the host tail.
+static ReportData report_data(const rmb_ctx* c, std::vector<const uint8_t*>& sp, std::vector<const uint8_t*>& sr,
+                              std::vector<uint32_t>& sl) {
+  for (const Subject& S : c->subj) {
+    sp.push_back(S.align.data());
+    sr.push_back(S.align_rc.data());
+    sl.push_back((uint32_t)S.align.size() - 2);
+  }
+  ReportData d;
+  d.hsps = c->hsps.data();
+  d.n_hsps = c->hsps.size();
+  d.ops = c->ops.data();
+  d.n_queries = (uint32_t)c->last_q.size();
+  d.queries = c->last_q.data();
+  d.query_lens = c->last_qlen.data();
+  d.n_subjects = (uint32_t)c->subj.size();
+  d.subjects = sp.data();
+  d.subjects_rc = sr.data();
+  d.subject_lens = sl.data();
+  return d;
+}
+
+int rmb_align_stats_of(const char* q, const char* s, uint64_t n, rmb_align_stats* out) {
+  if ((!q || !s) && n) return RMB_ERR_ARG;
+  if (!out) return RMB_ERR_ARG;
+  *out = align_stats(q, s, n);
+  return RMB_OK;
+}
+
+int rmb_mask_level(const rmb_hsp* hsps, uint64_t n, uint32_t q_len, int32_t mask_level, uint8_t* keep) {
+  if ((!hsps || !keep) && n) return RMB_ERR_ARG;
+  try {
+    ReportData d;
+    d.hsps = hsps;
+    d.n_hsps = n;
+    std::vector<uint64_t> rows(n);
+    for (uint64_t i = 0; i < n; ++i) rows[i] = i;
+    if (mask_level >= 0 && mask_level < 100) mask_level_filter(d, rows, mask_level, q_len);
+    std::fill(keep, keep + n, (uint8_t)0);
+    for (uint64_t r : rows) keep[r] = 1;
+    return RMB_OK;
+  } catch (...) {
+    return RMB_ERR_DEVICE;
+  }
+}
+
+int rmb_report_order(rmb_ctx* c, int32_t mask_level, uint64_t* rows, uint64_t* n) {
+  if (!c || !n) return RMB_ERR_ARG;
+  return guarded(c, [&]() {
+    std::vector<const uint8_t*> sp, sr;
+    std::vector<uint32_t> sl;
+    const ReportData d = report_data(c, sp, sr, sl);
+    const auto order = report_order(d, mask_level, host_threads());
+    uint64_t k = 0;
+    for (const auto& q : order) {
+      if (rows)
+        for (uint64_t r : q) rows[k++] = r;
+      else
+        k += q.size();
+    }
+    *n = k;
+  });
+}
+
+int rmb_report_tabular(rmb_ctx* c, const rmb_report_spec* spec, uint64_t* n_bytes) {
+  if (!c || !spec || !n_bytes) return RMB_ERR_ARG;
+  return guarded(c, [&]() {
+    if (!spec->fields) throw Failure(RMB_ERR_ARG, "no outfmt fields");
+    if ((!spec->query_ids && !c->last_q.empty()) || (!spec->subject_ids && !c->subj.empty()))
+      throw Failure(RMB_ERR_ARG, "query and subject ids are required");
+    std::vector<std::string> fields;
+    {
+      std::string w;
+      for (const char* p = spec->fields;; ++p) {
+        if (*p == ' ' || *p == '\t' || *p == 0) {
+          if (!w.empty()) fields.push_back(w);
+          w.clear();
+          if (!*p) break;
+        } else {
+          w += *p;
+        }
+      }
+    }
+    if (fields.empty()) throw Failure(RMB_ERR_ARG, "no outfmt fields");
+    std::vector<const uint8_t*> sp, sr;
+    std::vector<uint32_t> sl;
+    const ReportData d = report_data(c, sp, sr, sl);
+    const unsigned th = host_threads();
+    c->report.clear();
+    try {
+      c->report = report_tabular(d, report_order(d, spec->mask_level, th), fields, spec->query_ids,
+                                 spec->subject_ids, th);
+    } catch (const std::invalid_argument& e) {
+      throw Failure(RMB_ERR_ARG, e.what());
+    }
+    *n_bytes = c->report.size();
+  });
+}
+
+int rmb_report_copy(const rmb_ctx* c, char* out) {
+  if (!c || (!out && !c->report.empty())) return RMB_ERR_ARG;
+  if (!c->report.empty()) std::memcpy(out, c->report.data(), c->report.size());
+  return RMB_OK;
 }
 
 int rmb_ka_lookup(const char* matrix_name, int32_t gap_open, int32_t gap_extend, double out[2], int32_t* found) {

@@ -1,0 +1,51 @@
+// Host tail of librmblast_hip: what rmblastn does with a query's HSPs after
+// the search (rmblastn/src/main.rs:1224-1234): mask level across subjects
+// (rmblast-lib/src/search/engine.rs:863-974), NCBI's output order
+// (engine.rs:998-1017), alignment statistics (rmblast-lib/src/stats.rs:52-243)
+// and the outfmt 6 writer (rmblast-lib/src/output.rs:294-363).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/rmblast_hip.h"
+
+namespace rmb {
+
+// The HSPs of a search and the sequences they refer to (all host memory).
+struct ReportData {
+  const rmb_hsp* hsps = nullptr;
+  uint64_t n_hsps = 0;
+  const rmb_op* ops = nullptr;
+  uint32_t n_queries = 0;
+  const uint8_t* const* queries = nullptr;  // sentinel-wrapped BLASTNA, n_queries
+  const uint32_t* query_lens = nullptr;     // bases, without sentinels
+  uint32_t n_subjects = 0;
+  const uint8_t* const* subjects = nullptr;     // sentinel-wrapped, original ambiguity codes
+  const uint8_t* const* subjects_rc = nullptr;  // their reverse complements
+  const uint32_t* subject_lens = nullptr;
+};
+
+// Per query, the indices of its HSPs that survive mask level, in output
+// order.  mask_level >= 100 (or < 0) keeps every HSP.  Queries run on
+// `threads` host threads.
+std::vector<std::vector<uint64_t>> report_order(const ReportData& d, int32_t mask_level, unsigned threads);
+
+// Mask level over one query's HSPs (indices into d.hsps), in place.
+void mask_level_filter(const ReportData& d, std::vector<uint64_t>& rows, int32_t level, uint32_t q_len);
+
+// The outfmt 6 rows of `order` (one line per HSP, tab-separated `fields`).
+// Throws std::invalid_argument naming an unknown field.
+std::string report_tabular(const ReportData& d, const std::vector<std::vector<uint64_t>>& order,
+                           const std::vector<std::string>& fields, const char* const* query_ids,
+                           const char* const* subject_ids, unsigned threads);
+
+// The aligned IUPAC strings of one HSP (forward query frame; a minus-strand
+// subject reads reverse-complemented).
+void aligned_strings(const ReportData& d, const rmb_hsp& h, std::string& qa, std::string& sa);
+
+// stats.rs:52-243 on aligned IUPAC strings, the subject as the ancestral
+// sequence.
+rmb_align_stats align_stats(const char* q, const char* s, uint64_t n);
+
+}  // namespace rmb

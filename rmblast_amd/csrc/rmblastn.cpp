@@ -151,70 +151,6 @@ bool load_db(const std::string& path, std::vector<Record>& out, std::string& err
   return load_fasta(path, out, err);
 }
 
-struct Stats { uint32_t ident = 0, mism = 0, qgap = 0, sgap = 0, ti = 0, tv = 0, cpg = 0; double kdiv = 0, cpgk = 0; float ps = 0, pq = 0, pd = 0; };
-
-// stats.rs:52-243 with the subject as the ancestral sequence.
-Stats align_stats(const std::string& q, const std::string& s) {
-  auto pure = [](char c) { return c == 'A' || c == 'C' || c == 'G' || c == 'T'; };
-  auto bits = [](char c) -> int {
-    switch (c) {
-      case 'A': return 1; case 'C': return 2; case 'G': return 4; case 'T': return 8; case 'R': return 5;
-      case 'Y': return 10; case 'M': return 3; case 'K': return 12; case 'W': return 9; case 'S': return 6;
-      case 'B': return 14; case 'D': return 13; case 'H': return 11; case 'V': return 7; default: return 15;
-    }
-  };
-  enum { ID, TI, TV, IUBM, IUBS, GAP, NONE };
-  auto kind = [&](char a, char d) {
-    if (a == '-' || d == '-') return (int)GAP;
-    if (a == d) return (int)ID;
-    if (!pure(a) || !pure(d)) return ((pure(a) != pure(d)) && (bits(a) & bits(d))) ? (int)IUBM : (int)IUBS;
-    return ((a == 'A' || a == 'G') == (d == 'A' || d == 'G')) ? (int)TI : (int)TV;
-  };
-  Stats st;
-  uint32_t pairs = 0, qlen = 0, dlen = 0;
-  double cti = 0;
-  char prev = ' ';
-  int prevk = NONE;
-  for (size_t i = 0; i < s.size(); ++i) {
-    const char a = s[i], d = q[i];
-    if (d != '-') ++qlen;
-    if (a != '-') ++dlen;
-    if (prev == 'C' && a == 'G') {
-      ++st.cpg;
-      const int k = kind(a, d);
-      if (prevk == TI && k == TI) cti -= 1.0;
-      else if (prevk == TI || k == TI) cti -= 0.9;
-    }
-    if (a == '-' || d == '-') {
-      if (d == '-') ++st.qgap; else ++st.sgap;
-      if (a != '-') { prev = a; prevk = GAP; }
-      continue;
-    }
-    const int k = kind(a, d);
-    const bool both = pure(a) && pure(d);
-    if (k == ID) { ++st.ident; if (both) ++pairs; }
-    else if (k == TI) { ++st.mism; if (both) { ++pairs; ++st.ti; cti += 1.0; } }
-    else if (k == TV) { ++st.mism; if (both) { ++pairs; ++st.tv; } }
-    else if (k == IUBS) ++st.mism;
-    prev = a;
-    prevk = k;
-  }
-  auto kimura = [](double p, double qq) {
-    const double o = (1.0 - 2.0 * p - qq) * std::sqrt(1.0 - 2.0 * qq);
-    return o > 0.0 ? std::fabs(-0.5 * std::log(o)) * 100.0 : 100.0;
-  };
-  if (pairs < 1) { st.kdiv = st.cpgk = 100.0; }
-  else {
-    const double n = pairs;
-    st.kdiv = kimura(st.ti / n, st.tv / n);
-    st.cpgk = kimura(cti / n, st.tv / n);
-  }
-  st.ps = qlen ? ((float)st.mism / (float)qlen) * 100.0f : 0.0f;
-  st.pq = qlen ? ((float)st.qgap / (float)qlen) * 100.0f : 0.0f;
-  st.pd = dlen ? ((float)st.sgap / (float)dlen) * 100.0f : 0.0f;
-  return st;
-}
-
 struct Row {
   rmb_hsp h;
   uint32_t q_len, s_len;
@@ -347,17 +283,18 @@ std::string footer_block(const std::string& db_name, size_t n_seqs, uint64_t let
 // One HSP (output.rs:510-649): the statistics block, then 60-column rows of
 // query, match line and subject.  Query bases inside the DUST intervals
 // `mask` ([first, last] pairs, sorted) print in lower case.
-void hsp_block(std::string& o, const Row& r, const Stats& st, const std::vector<uint64_t>& mask) {
+void hsp_block(std::string& o, const Row& r, const rmb_align_stats& st, const std::vector<uint64_t>& mask) {
   const rmb_hsp& h = r.h;
-  const uint32_t n = (uint32_t)r.qa.size(), gaps = st.qgap + st.sgap;
+  const uint32_t n = (uint32_t)r.qa.size(), gaps = st.query_gaps + st.subject_gaps;
   char b[256];
   snprintf(b, sizeof b,
            " Score = %d\n Substitutions = %.2f%%, Query Gaps = %.2f%%, DB Gaps = %.2f%%\n"
            " TransI/TransV = %u/%u,  CpG_sites = %u\n Kimura Div = %.2f%%, Kimura Div CpG Adjusted = %.2f%%\n",
-           h.score, (double)st.ps, (double)st.pq, (double)st.pd, st.ti, st.tv, st.cpg, st.kdiv, st.cpgk);
+           h.score, (double)st.perc_sub, (double)st.perc_query_gap, (double)st.perc_db_gap, st.transitions,
+           st.transversions, st.cpg_sites, st.kdiv, st.cpg_kdiv);
   o += b;
-  snprintf(b, sizeof b, " Identities = %u/%u (%u%%), Gaps = %u/%u (%u%%)\n Strand=%s\n\n", st.ident, n,
-           percent_match(st.ident, n), gaps, n, percent_match(gaps, n), h.strand ? "Plus/Minus" : "Plus/Plus");
+  snprintf(b, sizeof b, " Identities = %u/%u (%u%%), Gaps = %u/%u (%u%%)\n Strand=%s\n\n", st.identities, n,
+           percent_match(st.identities, n), gaps, n, percent_match(gaps, n), h.strand ? "Plus/Minus" : "Plus/Plus");
   o += b;
   std::string qa = r.qa;
   if (!mask.empty()) {
@@ -570,98 +507,64 @@ int main(int argc, char** argv) {
     O.resize(no);
     rmb_results_copy(ctx, H.data(), O.data());
   }
-  std::vector<std::vector<Row>> per_query(qs.size());
-  for (const rmb_hsp& h : H) {
-    const Record& Q = qs[h.query];
-    const Record& S = db[h.subject];
-    Row r;
-    r.h = h;
-    r.q_len = (uint32_t)Q.bases.size();
-    r.s_len = (uint32_t)S.bases.size();
-    // aligned columns in the forward query frame; the minus subject is read reverse-complemented
-    size_t qi = h.q_start, si = h.strand ? S.bases.size() - h.s_end : h.s_start;
-    for (uint32_t k = 0; k < h.ops_count; ++k) {
-      const uint32_t kindv = O[h.ops_offset + k] & 3, n = O[h.ops_offset + k] >> 2;
-      for (uint32_t j = 0; j < n; ++j) {
-        if (kindv != 1) r.qa += kIupac[Q.bases[qi++] & 15]; else r.qa += '-';
-        if (kindv != 2) {
-          const uint8_t b = h.strand ? kComp[S.bases[S.bases.size() - 1 - si] & 15] : (S.bases[si] & 15);
-          ++si;
-          r.sa += kIupac[b];
-        } else {
-          r.sa += '-';
-        }
-      }
-    }
-    per_query[h.query].push_back(std::move(r));
-  }
   std::string out;
   uint64_t db_letters = 0;
   for (const Record& r : db) db_letters += r.bases.size();
   const std::string& db_name = kv["--db"];
-  if (pairwise) out += header_block(db_name, db.size(), db_letters);
-  for (size_t qi = 0; qi < qs.size(); ++qi) {
-    std::vector<Row>& rows = per_query[qi];
-    auto qoff = [](const Row& r) { return r.h.strand ? r.q_len - r.h.q_end : r.h.q_start; };
-    auto qend = [](const Row& r) { return r.h.strand ? r.q_len - r.h.q_start : r.h.q_end; };
-    auto better = [&](const Row& a, const Row& b) {  // ScoreCompareHSPs (hits.rs:208-216); <0 if a first
-      if (a.h.score != b.h.score) return a.h.score > b.h.score ? -1 : 1;
-      if (a.h.s_start != b.h.s_start) return a.h.s_start < b.h.s_start ? -1 : 1;
-      if (a.h.s_end != b.h.s_end) return a.h.s_end > b.h.s_end ? -1 : 1;
-      if (qoff(a) != qoff(b)) return qoff(a) < qoff(b) ? -1 : 1;
-      if (qend(a) != qend(b)) return qend(a) > qend(b) ? -1 : 1;
-      return 0;
-    };
-    if (mask_level < 100) {  // engine.rs:863-974, cross-subject form
-      std::sort(rows.begin(), rows.end(), [&](const Row& a, const Row& b) {
-        if (a.h.score != b.h.score) return a.h.score > b.h.score;
-        if (a.h.subject != b.h.subject) return a.h.subject > b.h.subject;
-        return better(a, b) < 0;
-      });
-      struct Span { uint32_t s, e; int32_t sc; };
-      std::vector<Span> kept;  // by start
-      uint32_t widest = 0;
-      std::vector<Row> survivors;
-      for (Row& r : rows) {
-        uint32_t s = r.h.q_start, e = r.h.q_end;
-        if (r.h.strand) { ++s; ++e; }
-        const int64_t span = (int64_t)e - s;
-        if (span == 0) continue;
-        const bool touch = mask_level == 0;
-        const uint32_t hi = touch ? e : e - 1, lo = s > widest ? s - widest : 0;
-        bool masked = false;
-        auto it = std::lower_bound(kept.begin(), kept.end(), lo, [](const Span& x, uint32_t v) { return x.s < v; });
-        for (; it != kept.end() && it->s <= hi && !masked; ++it) {
-          if (it->e < s || (it->e == s && !touch) || it->sc < r.h.score) continue;
-          const int64_t ov = (int64_t)std::min(it->e, e) - (int64_t)std::max(it->s, s);
-          if ((int64_t)(100.0 * ((double)ov / (double)span)) >= mask_level) masked = true;
+  std::vector<const char*> qids, sids;
+  for (const Record& r : qs) qids.push_back(r.id.c_str());
+  for (const Record& r : db) sids.push_back(r.id.c_str());
+  if (!pairwise && nh) {
+    // mask level, output order, statistics and the rows: the library's host tail
+    std::string fl;
+    for (const std::string& f : fields) fl += (fl.empty() ? "" : " ") + f;
+    rmb_report_spec spec{};
+    spec.mask_level = mask_level;
+    spec.fields = fl.c_str();
+    spec.query_ids = qids.data();
+    spec.subject_ids = sids.data();
+    uint64_t nb = 0;
+    if (rmb_report_tabular(ctx, &spec, &nb)) return fail("report");
+    out.resize(nb);
+    if (rmb_report_copy(ctx, out.data())) return fail("report");
+  }
+  if (pairwise) {
+    std::vector<uint64_t> order;
+    uint64_t n_rows = 0;
+    if (nh) {
+      if (rmb_report_order(ctx, mask_level, nullptr, &n_rows)) return fail("report");
+      order.resize(n_rows);
+      if (rmb_report_order(ctx, mask_level, order.data(), &n_rows)) return fail("report");
+    }
+    std::vector<std::vector<Row>> per_query(qs.size());
+    for (uint64_t k : order) {
+      const rmb_hsp& h = H[k];
+      const Record& Q = qs[h.query];
+      const Record& S = db[h.subject];
+      Row r;
+      r.h = h;
+      r.q_len = (uint32_t)Q.bases.size();
+      r.s_len = (uint32_t)S.bases.size();
+      // aligned columns in the forward query frame; the minus subject is read reverse-complemented
+      size_t qi = h.q_start, si = h.strand ? S.bases.size() - h.s_end : h.s_start;
+      for (uint32_t j = 0; j < h.ops_count; ++j) {
+        const uint32_t kindv = O[h.ops_offset + j] & 3, n = O[h.ops_offset + j] >> 2;
+        for (uint32_t t = 0; t < n; ++t) {
+          if (kindv != 1) r.qa += kIupac[Q.bases[qi++] & 15]; else r.qa += '-';
+          if (kindv != 2) {
+            const uint8_t b = h.strand ? kComp[S.bases[S.bases.size() - 1 - si] & 15] : (S.bases[si] & 15);
+            ++si;
+            r.sa += kIupac[b];
+          } else {
+            r.sa += '-';
+          }
         }
-        if (masked) continue;
-        kept.insert(std::upper_bound(kept.begin(), kept.end(), s, [](uint32_t v, const Span& x) { return v < x.s; }),
-                    Span{s, e, r.h.score});
-        widest = std::max(widest, e - s);
-        survivors.push_back(std::move(r));
       }
-      rows.swap(survivors);
+      per_query[h.query].push_back(std::move(r));
     }
-    std::sort(rows.begin(), rows.end(), [&](const Row& a, const Row& b) {  // main.rs:1230-1234
-      const std::string &ia = db[a.h.subject].id, &ib = db[b.h.subject].id;
-      if (ia != ib) return ia < ib;
-      if (a.h.strand != b.h.strand) return a.h.strand < b.h.strand;
-      return a.h.q_start < b.h.q_start;
-    });
-    std::map<uint32_t, int32_t> top;  // engine.rs:998-1017
-    for (auto& r : rows) {
-      auto it = top.emplace(r.h.subject, INT32_MIN).first;
-      it->second = std::max(it->second, r.h.score);
-    }
-    std::stable_sort(rows.begin(), rows.end(), [&](const Row& a, const Row& b) {
-      const int32_t ta = top[a.h.subject], tb = top[b.h.subject];
-      if (ta != tb) return ta > tb;
-      if (a.h.subject != b.h.subject) return a.h.subject > b.h.subject;
-      return better(a, b) < 0;
-    });
-    if (pairwise) {  // output.rs:685-720; rows are grouped by subject already
+    out += header_block(db_name, db.size(), db_letters);
+    for (size_t qi = 0; qi < qs.size(); ++qi) {  // output.rs:685-720; rows are grouped by subject already
+      const std::vector<Row>& rows = per_query[qi];
       std::string ql = qs[qi].defline;
       while (!ql.empty() && ascii_space(ql.back())) ql.pop_back();
       out += "Query= " + ql + "\n\nLength=" + std::to_string(qs[qi].bases.size()) + "\n\n";
@@ -687,41 +590,11 @@ int main(int argc, char** argv) {
             title = S.defline.substr(S.id.size() + 1);
           out += ">" + S.id + " " + wrap_title(title) + "\nLength=" + std::to_string(r.s_len) + "\n\n";
         }
-        hsp_block(out, r, align_stats(r.qa, r.sa), mask);
+        rmb_align_stats st{};
+        rmb_align_stats_of(r.qa.data(), r.sa.data(), r.qa.size(), &st);
+        hsp_block(out, r, st, mask);
       }
       out += "\n\n";
-      continue;
-    }
-    char buf[64];
-    for (const Row& r : rows) {
-      const Stats st = align_stats(r.qa, r.sa);
-      for (size_t k = 0; k < fields.size(); ++k) {
-        if (k) out += '\t';
-        const std::string& f = fields[k];
-        const rmb_hsp& h = r.h;
-        if (f == "score") out += std::to_string(h.score);
-        else if (f == "perc_sub") { snprintf(buf, sizeof buf, "%.2f", (double)st.ps); out += buf; }
-        else if (f == "perc_query_gap") { snprintf(buf, sizeof buf, "%.2f", (double)st.pq); out += buf; }
-        else if (f == "perc_db_gap") { snprintf(buf, sizeof buf, "%.2f", (double)st.pd); out += buf; }
-        else if (f == "qseqid") out += qs[qi].id;
-        else if (f == "qstart") out += std::to_string(h.q_start + 1);
-        else if (f == "qend") out += std::to_string(h.q_end);
-        else if (f == "qlen") out += std::to_string(r.q_len);
-        else if (f == "sstrand") out += h.strand ? "minus" : "plus";
-        else if (f == "sseqid") out += db[h.subject].id;
-        else if (f == "sstart") out += std::to_string(h.strand ? h.s_end : h.s_start + 1);
-        else if (f == "send") out += std::to_string(h.strand ? h.s_start + 1 : h.s_end);
-        else if (f == "slen") out += std::to_string(r.s_len);
-        else if (f == "kdiv") { snprintf(buf, sizeof buf, "%.2f", st.kdiv); out += buf; }
-        else if (f == "cpg_kdiv") { snprintf(buf, sizeof buf, "%.2f", st.cpgk); out += buf; }
-        else if (f == "transi") out += std::to_string(st.tv);     // NCBI column quirks (output.rs:338-340)
-        else if (f == "transv") out += std::to_string(st.cpg);
-        else if (f == "cpg_sites") out += std::to_string(st.ti);
-        else if (f == "qseq") out += r.qa;
-        else if (f == "sseq") out += r.sa;
-        else { fprintf(stderr, "error: unknown -outfmt field '%s'\n", f.c_str()); return 1; }
-      }
-      out += '\n';
     }
   }
   if (pairwise) {

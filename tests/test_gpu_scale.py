@@ -1,0 +1,96 @@
+"""HIP path parity at the configurations' real shapes.
+
+* The exact-path-at-scale workload (the bench's rows): 60 of its 1,667 60-kbp
+  records, spread over the genome, against all 1,000 families, with the C1
+  scoring line (mask level 80, complexity adjustment) and the RepeatMasker
+  outfmt 6 fields, GPU front end against the oracle.
+* The 1 Mbp x 50-family parity configuration at its full size.
+* The divergent/gappy stress configuration as one 4.2 Mbp record, so the
+  split-query path (five chunks, seams folded) runs at X-drop 120/400.
+* The reference's chr22 ungapped-stage golden (content and order) on the two
+  chunks of chr22 held by the committed 3 Mbp fixture, through the ungapped
+  tap (rmblast-lib/tests/ungapped_regression.rs:74-88).
+"""
+import gzip
+import os
+
+import numpy as np
+import pytest
+
+import rmblast_amd
+from rmblast_amd import synth
+from conftest import CLI, CHR22_3MB, ORACLE_BIN, RM_FIELDS, golden, run
+from test_oracle_chr22 import read_2bit
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_and_oracle(args, threads=16, tmo=900):
+    g = run([CLI] + args, timeout=tmo)
+    assert g.returncode == 0, g.stderr
+    o = run([ORACLE_BIN] + args + ["--num-threads", str(threads)], timeout=tmo)
+    assert o.returncode == 0, o.stderr
+    return g.stdout, o.stdout
+
+
+def synth_args(tmp_path, xdrop=(20, 30, 100)):
+    (tmp_path / "m.matrix").write_text(synth.MATRIX_TEXT)
+    return ["--db", str(tmp_path / "db.fa"), "--query", str(tmp_path / "q.fa"), "--matrix",
+            str(tmp_path / "m.matrix"), "--word-size", "7", "--gapopen", "20", "--gapextend", "5",
+            "--min-raw-gapped-score", "200", "--xdrop-ungap", str(xdrop[0]), "--xdrop-gap", str(xdrop[1]),
+            "--xdrop-gap-final", str(xdrop[2]), "--mask-level", "80", "--complexity-adjust", "--dust", "no",
+            "--outfmt", RM_FIELDS]
+
+
+def test_scale_workload_records_match_oracle(tmp_path):
+    w = synth.scale_config()
+    recs = w.records[::28]
+    assert len(recs) == 60 and len(w.families) == 1000
+    synth.to_fasta(recs, tmp_path / "q.fa")
+    synth.to_fasta(w.families, tmp_path / "db.fa")
+    g, o = gpu_and_oracle(synth_args(tmp_path))
+    assert g == o
+    assert g.count("\n") > 3000 and len({l.split("\t")[4] for l in g.splitlines()}) == 60
+
+
+def test_parity_config_full_size_matches_oracle(tmp_path):
+    w = synth.parity_config()
+    assert len(w.records[0][1]) == 1_000_000 and len(w.families) == 50
+    synth.to_fasta(w.records, tmp_path / "q.fa")
+    synth.to_fasta(w.families, tmp_path / "db.fa")
+    g, o = gpu_and_oracle(synth_args(tmp_path))
+    assert g == o and g.count("\n") > 1000
+
+
+def test_gappy_config_split_query_matches_oracle(tmp_path):
+    w = synth.gappy_config(genome_len=4_200_000, n_families=60, n_copies=1700)
+    synth.to_fasta(w.records, tmp_path / "q.fa")
+    synth.to_fasta(w.families, tmp_path / "db.fa")
+    g, o = gpu_and_oracle(synth_args(tmp_path, (20, 120, 400)))
+    assert g == o and g.count("\n") > 200
+
+
+def test_chr22_ungapped_stage_golden_on_gpu():
+    lines = gzip.open(golden("chr22_stage_chunks30_31.txt.gz"), "rt").read().splitlines()
+    want = [l for l in lines if not l.startswith("#")]
+    seq = "".join(l.strip() for l in gzip.open(golden(CHR22_3MB + ".gz"), "rt") if not l.startswith(">"))
+    # chunks 30 and 31 of chr22's NCBI query chunks, in the fixture's
+    # coordinates (it starts at chr22 position 30,000,000)
+    chunk_bounds = [(491_040, 1_507_508), (1_507_408, 2_523_876)]
+    m = rmblast_amd.load_matrix(golden("comparison.matrix"))
+    s = rmblast_amd.Searcher(m, gap_open=20, gap_extend=5, word_size=14, xdrop_ungap=400, xdrop_gap=100,
+                             xdrop_gap_final=200, min_raw_gapped_score=200, complexity_adjust=True)
+    alu = read_2bit(golden("alu.2bit"))
+    s.load_library([rmblast_amd.encode(x) for _, x in alu])
+    got = []
+    for a, b in chunk_bounds:
+        for subj, (_, sseq) in enumerate(alu):
+            for q_seed, s_seed, qs, qe, ss, se, score, minus in s.tap_ungapped(rmblast_amd.encode(seq[a:b]), subj):
+                if minus:  # record frame (reverse-complement subject) -> forward subject
+                    ss, se = len(sseq) - se, len(sseq) - ss
+                got.append(f"UNGAP strand={'-' if minus else '+'} qs={qs} qe={qe} ss={ss} se={se} "
+                           f"score={np.int32(score)}")
+    assert len(want) == 20860
+    first = next((i for i, (x, y) in enumerate(zip(got, want)) if x != y), None)
+    assert first is None, (first, got[first], want[first])
+    assert len(got) == len(want)
