@@ -259,6 +259,30 @@ int rmb_tap_align_score_only(rmb_ctx* ctx, const uint8_t* a, uint32_t m, const u
 int rmb_tap_align_ex(rmb_ctx* ctx, const uint8_t* a, uint32_t m, const uint8_t* b, uint32_t n, int32_t xdrop,
                      int32_t reverse, int64_t out[4], rmb_op* ops, uint32_t max_ops);
 
+/* ---- score-only X-drop DPs that allocate nothing ------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): the score-only half alignments of rmb_tap_align_score_only for
+ * n_halves halves whose sequences, descriptors and outputs are already in
+ * device memory.  halves[i].a / .b are device pointers to BLASTNA codes (no
+ * sentinels); out receives 3 int32 per half {score, a_len, b_len}.  The
+ * library allocates nothing in this call: it works in one fixed scratch area
+ * of about 19 MB per GPU, held until the process ends.  Limits:
+ * n_halves <= RMB_NOALLOC_MAX_HALVES (else RMB_ERR_ARG); a band wider than
+ * 1,024 cells gives RMB_ERR_CAPACITY (rmb_tap_align_score_only takes bands up
+ * to 2^17 cells).  The call runs on the context's stream and returns when the
+ * results are in `out`.  Calls that overlap on one GPU (from two threads or
+ * two contexts) race on the scratch area. */
+#define RMB_NOALLOC_MAX_HALVES 65536u
+typedef struct rmb_dp_half {
+  const uint8_t* a;   /* outer sequence (rows), device memory */
+  const uint8_t* b;   /* inner sequence (columns), device memory */
+  uint32_t m, n;      /* their lengths */
+  int32_t xdrop;
+  uint32_t reverse;   /* 1: align from the sequences' ends backwards */
+} rmb_dp_half;
+int rmb_align_score_only_noalloc(rmb_ctx* ctx, const rmb_dp_half* halves, uint32_t n_halves, int32_t* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1199,7 +1199,8 @@ __device__ __forceinline__ uint32_t tb_walk(const uint8_t* __restrict__ flat, co
 template <bool LDS_RING>
 __global__ __launch_bounds__(LDS_RING ? kDpLdsThreads : 256) void k_dp_score(
     const DpTask* __restrict__ tasks, uint32_t n_tasks, const int32_t* __restrict__ gmat, int32_t go, int32_t ge,
-    int2* __restrict__ ring, uint32_t wlog2, DpResult* __restrict__ out, unsigned long long* counters) {
+    int2* __restrict__ ring, uint32_t wlog2, DpResult* __restrict__ out, unsigned long long* counters,
+    uint32_t only_wide) {
   __shared__ int mat[256];
   __shared__ int2 lring[LDS_RING ? (kDpLdsThreads << kDpLdsLog2) : 1];
   __shared__ uint8_t lbring[LDS_RING ? (kDpLdsThreads << kDpLdsLog2) : 1];
@@ -1212,6 +1213,8 @@ __global__ __launch_bounds__(LDS_RING ? kDpLdsThreads : 256) void k_dp_score(
   const uint32_t wmask = LDS_RING ? (1u << kDpLdsLog2) - 1 : (1u << wlog2) - 1;
   unsigned long long cells = 0;
   for (uint32_t i = gt; i < n_tasks; i += nt) {
+    // only_wide: rerun just the tasks an earlier pass flagged for their band
+    if (only_wide && !(out[i].flags & E_BAND_WIDE)) continue;
     DpResult r;
     dp_half<false, RS>(tasks[i], mat, go, ge, my, myb, wmask, r, nullptr, nullptr, 0);
     out[i] = r;
@@ -1540,11 +1543,81 @@ hipError_t launch_dp_score(int blocks, const DpTask* tasks, uint32_t n, const in
   // ring == nullptr: LDS rings, `blocks` workgroups of kDpLdsThreads lanes
   if (!ring)
     hipLaunchKernelGGL(k_dp_score<true>, dim3(blocks), dim3(kDpLdsThreads), 0, st, tasks, n, gmat, go, ge, ring,
-                       wlog2, out, counters);
+                       wlog2, out, counters, 0u);
   else
     hipLaunchKernelGGL(k_dp_score<false>, dim3(blocks), dim3(256), 0, st, tasks, n, gmat, go, ge, ring, wlog2, out,
-                       counters);
+                       counters, 0u);
   return hipGetLastError();
+}
+
+// ---- score-only X-drop DPs without device allocation ------------------------
+//
+// One fixed scratch array per GPU (a __device__ array: each device holds its
+// own copy until the process ends) holds everything these DPs need beyond the
+// caller's inputs and outputs: per-half results, the counters, an error word
+// and the per-lane band rings of the rerun of halves whose band outgrows the
+// 64-cell LDS ring.  Host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device.
+constexpr uint32_t kNoAllocMaxHalves = 65536;
+constexpr uint32_t kNoAllocRerunBlocks = 8;  // x 256 lanes, each with a global ring
+constexpr uint32_t kNoAllocRingLog2 = 10;
+constexpr size_t kNoAllocCountersOff = (size_t)kNoAllocMaxHalves * sizeof(DpResult);
+constexpr size_t kNoAllocErrOff = kNoAllocCountersOff + C_NCOUNTERS * sizeof(unsigned long long);
+constexpr size_t kNoAllocRingOff = (kNoAllocErrOff + sizeof(uint32_t) + 255) & ~(size_t)255;
+constexpr size_t kNoAllocBytes =
+    kNoAllocRingOff + (size_t)kNoAllocRerunBlocks * 256 * ring_stride(kNoAllocRingLog2) * sizeof(int2);
+static_assert(kNoAllocBytes < (24u << 20), "no-alloc scratch stays modest");
+
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+__global__ void k_noalloc_clear(unsigned long long* __restrict__ counters, uint32_t* __restrict__ err) {
+  if (threadIdx.x < C_NCOUNTERS) counters[threadIdx.x] = 0;
+  if (threadIdx.x == 0) *err = 0;
+}
+
+__global__ void k_noalloc_finish(const DpResult* __restrict__ res, uint32_t n, int32_t* __restrict__ out,
+                                 uint32_t* __restrict__ err) {
+  uint32_t flags = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const DpResult r = res[i];
+    out[3 * i] = r.score;
+    out[3 * i + 1] = (int32_t)r.a_len;
+    out[3 * i + 2] = (int32_t)r.b_len;
+    flags |= r.flags;
+  }
+  if (flags) atomicOr(err, flags);
+}
+
+size_t noalloc_max_halves() { return kNoAllocMaxHalves; }
+
+// The scratch array's address on the current device.
+hipError_t noalloc_scratch_address(void** ptr) { return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch)); }
+
+// Score-only DPs of n halves (n <= kNoAllocMaxHalves) with `base`, the
+// scratch address of the current device: LDS-ring pass, rerun of the halves
+// flagged for their band with 2^10-cell rings, results to out (3 int32 per
+// half), then the OR of every half's flags into *err_host (a synchronous
+// read on st).
+hipError_t launch_noalloc_score(void* base, int cu_count, const DpTask* tasks, uint32_t n, const int32_t* gmat,
+                                int32_t go, int32_t ge, int32_t* out, uint32_t* err_host, hipStream_t st) {
+  uint8_t* b = static_cast<uint8_t*>(base);
+  DpResult* res = reinterpret_cast<DpResult*>(b);
+  unsigned long long* counters = reinterpret_cast<unsigned long long*>(b + kNoAllocCountersOff);
+  uint32_t* err = reinterpret_cast<uint32_t*>(b + kNoAllocErrOff);
+  int2* ring = reinterpret_cast<int2*>(b + kNoAllocRingOff);
+  hipLaunchKernelGGL(k_noalloc_clear, dim3(1), dim3(64), 0, st, counters, err);
+  const uint32_t blocks = std::min<uint32_t>((uint32_t)cu_count * 4, (n + kDpLdsThreads - 1) / kDpLdsThreads);
+  hipLaunchKernelGGL(k_dp_score<true>, dim3(blocks), dim3(kDpLdsThreads), 0, st, tasks, n, gmat, go, ge, nullptr,
+                     0u, res, counters, 0u);
+  hipLaunchKernelGGL(k_dp_score<false>, dim3(kNoAllocRerunBlocks), dim3(256), 0, st, tasks, n, gmat, go, ge, ring,
+                     kNoAllocRingLog2, res, counters, 1u);
+  hipLaunchKernelGGL(k_noalloc_finish, dim3(std::min<uint32_t>(1024, (n + 255) / 256)), dim3(256), 0, st, res, n, out,
+                     err);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = hipMemcpyAsync(err_host, err, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(st);
 }
 
 hipError_t launch_dp_onepass(int blocks, const DpTask* tasks, uint32_t n, const int32_t* gmat, int32_t go,

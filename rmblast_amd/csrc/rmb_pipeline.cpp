@@ -19,6 +19,7 @@
 #include <atomic>
 #include <chrono>
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -48,6 +49,10 @@ hipError_t launch_phase1(int, P1Params, const RecDesc*, const SubjDesc*, const u
                          const uint8_t*, const uint8_t*, const uint8_t*, const uint32_t*, const uint32_t*,
                          const int32_t*, UngHit*, unsigned long long*, uint2*, uint4*, unsigned long long*,
                          unsigned long long*, uint32_t*, hipStream_t);
+hipError_t noalloc_scratch_address(void**);
+size_t noalloc_max_halves();
+hipError_t launch_noalloc_score(void*, int, const DpTask*, uint32_t, const int32_t*, int32_t, int32_t, int32_t*,
+                                uint32_t*, hipStream_t);
 hipError_t launch_pack_ops(const PackTask*, uint32_t, const uint32_t*, uint32_t*, hipStream_t);
 hipError_t launch_dp_score(int, const DpTask*, uint32_t, const int32_t*, int32_t, int32_t, int2*, uint32_t,
                            DpResult*, unsigned long long*, hipStream_t);
@@ -249,6 +254,7 @@ struct rmb_ctx {
   std::string err;
   int32_t* d_mat = nullptr;
   int cu_count = 256;
+  void* noalloc_scratch = nullptr;  // this device's copy of the no-alloc DPs' scratch array
   // library
   std::vector<Subject> subj;
   std::vector<SubjDesc> sdesc;
@@ -1617,6 +1623,7 @@ int rmb_ctx_create(int device, const rmb_params* params, const rmb_matrix* matri
     std::memcpy(c->M.s, matrix->scores, sizeof c->M.s);
     std::memcpy(c->M.freqs, matrix->freqs, sizeof c->M.freqs);
     c->M.lambda = matrix->lambda;
+    hip_check(noalloc_scratch_address(&c->noalloc_scratch), "hipGetSymbolAddress (no-alloc scratch)");
     hip_check(hipMalloc(&c->d_mat, 256 * sizeof(int32_t)), "hipMalloc matrix");
     hip_check(hipMemcpy(c->d_mat, matrix->scores, 256 * sizeof(int32_t), hipMemcpyHostToDevice), "copy matrix");
   });
@@ -2033,6 +2040,28 @@ int rmb_tap_align_ex(rmb_ctx* c, const uint8_t* a, uint32_t m, const uint8_t* b,
     out[2] = r[0].b_len;
     out[3] = (int64_t)sc.size();
     for (size_t i = 0; i < sc.size() && i < max_ops; ++i) ops[i] = sc[i].kind | (sc[i].len << 2);
+  });
+}
+
+static_assert(sizeof(rmb_dp_half) == sizeof(DpTask) && offsetof(rmb_dp_half, m) == offsetof(DpTask, m) &&
+                  offsetof(rmb_dp_half, xdrop) == offsetof(DpTask, xdrop) &&
+                  offsetof(rmb_dp_half, reverse) == offsetof(DpTask, reverse),
+              "rmb_dp_half is the DP kernels' task record");
+
+int rmb_align_score_only_noalloc(rmb_ctx* c, const rmb_dp_half* halves, uint32_t n_halves, int32_t* out) {
+  if (!c) return RMB_ERR_ARG;
+  return guarded(c, [&]() {
+    if (n_halves == 0) return;
+    if (!halves || !out) throw Failure(RMB_ERR_ARG, "halves and out must be device pointers");
+    if (n_halves > noalloc_max_halves())
+      throw Failure(RMB_ERR_ARG, "n_halves exceeds RMB_NOALLOC_MAX_HALVES (65,536)");
+    if (!c->noalloc_scratch) throw Failure(RMB_ERR_STATE, "context has no scratch address");
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    uint32_t err = 0;
+    hip_check(launch_noalloc_score(c->noalloc_scratch, c->cu_count, reinterpret_cast<const DpTask*>(halves),
+                                   n_halves, c->d_mat, c->P.gap_open, c->P.gap_extend, out, &err, c->st),
+              "no-alloc score DPs");
+    if (err & E_BAND_WIDE) throw Failure(RMB_ERR_CAPACITY, "an X-drop band outgrew 1,024 cells");
   });
 }
 
