@@ -22,7 +22,7 @@
  *    the library resolves them for seeding exactly as FastaDb does
  *    (seq/fasta_db.rs:86-105) and keeps the originals for traceback.
  *  - Coordinates are 0-based, half-open; subject coordinates of minus-strand
- *    HSPs are on the plus strand (hits.rs:165-192).
+ *    HSPs are on the plus strand (hits.rs:62-98).
  *  - Every function returns RMB_OK (0) or an error code; the message of the
  *    last error is available from rmb_last_error().  No exception or abort
  *    crosses this boundary (the convention of the reference's only C ABI,
@@ -85,7 +85,7 @@ typedef struct rmb_matrix {
   double lambda;
 } rmb_matrix;
 
-/* One final HSP (hits.rs:165-192 Hsp without the aligned strings). */
+/* One final HSP (hits.rs:62-98 Hsp without the aligned strings). */
 typedef struct rmb_hsp {
   uint32_t query;     /* index into the query batch */
   uint32_t subject;   /* index into the loaded library */
@@ -97,7 +97,7 @@ typedef struct rmb_hsp {
   uint32_t ops_count;
 } rmb_hsp;
 
-/* Edit op: low 2 bits = kind (0 Sub, 1 GapInQuery, 2 GapInSubject; hits.rs:123-134),
+/* Edit op: low 2 bits = kind (0 Sub, 1 GapInQuery, 2 GapInSubject; hits.rs:21-30),
  * upper 30 bits = run length. Ops are in forward query order. */
 typedef uint32_t rmb_op;
 
@@ -186,7 +186,7 @@ int rmb_align_stats_of(const char* query_aligned, const char* subject_aligned, u
 
 /* Mask level over one query's HSPs from all subjects (apply_mask_level in
  * its cross-subject form, engine.rs:863-974): sorted by score, then subject
- * index (the OID) descending, then ScoreCompareHSPs (hits.rs:208-216), an HSP
+ * index (the OID) descending, then ScoreCompareHSPs (hits.rs:104-112), an HSP
  * is dropped when an accepted HSP of at least its score covers
  * (int)(100 * overlap / span) >= mask_level percent of its query range
  * (minus-strand ranges shifted by one; at level 0 touching counts).  q_len is

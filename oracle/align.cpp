@@ -1,5 +1,5 @@
 // CPU ORACLE (test infrastructure): extension and alignment kernels.
-// Restates rmblast-lib/src/search/ungapped.rs:223-306, diag_hash.rs:30-111,
+// Restates rmblast-lib/src/search/ungapped.rs:38-121, diag_hash.rs:30-111,
 // gapped.rs:124-820, engine.rs:2953-3039 (bidirectional traceback alignment),
 // complexity_adjust.rs:19-132 and stats.rs:52-251.
 #include "oracle.hpp"
@@ -105,7 +105,7 @@ void DiagHash::insert(int32_t diag, uint32_t s_end, uint32_t s_off, int32_t wind
 }
 
 // ----------------------------------------------------------- edit ops --------
-void push_op(EditOps& e, Op op, uint32_t n) {  // hits.rs:147-156
+void push_op(EditOps& e, Op op, uint32_t n) {  // hits.rs:43-54
   if (n == 0) return;
   if (!e.empty() && e.back().first == op) { e.back().second += n; return; }
   e.emplace_back(op, n);

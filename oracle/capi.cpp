@@ -13,7 +13,7 @@ static Matrix mat_from(const int32_t* s) {
 
 extern "C" {
 
-// ungapped.rs:223-306; out = {score, q_start, q_end, s_start, s_end}
+// ungapped.rs:38-121; out = {score, q_start, q_end, s_start, s_end}
 void orc_extend_ungapped(const uint8_t* q, size_t qn, const uint8_t* s, size_t sn, uint32_t q_off, uint32_t s_off,
                          const int32_t* matrix, int xdrop, int32_t* out) {
   Matrix m = mat_from(matrix);

@@ -957,7 +957,7 @@ void apply_mask_level(std::vector<AlignResult>& r, int mask_level, const std::ve
   r.swap(out);
 }
 
-static int score_compare(const Hsp& a, const Hsp& b) {  // hits.rs:208-216, <0 if a first
+static int score_compare(const Hsp& a, const Hsp& b) {  // hits.rs:104-112, <0 if a first
   uint32_t qoa = a.strand == PLUS ? a.q_start : a.q_len - a.q_end, qob = b.strand == PLUS ? b.q_start : b.q_len - b.q_end;
   uint32_t qea = a.strand == PLUS ? a.q_end : a.q_len - a.q_start, qeb = b.strand == PLUS ? b.q_end : b.q_len - b.q_start;
   if (a.score != b.score) return a.score > b.score ? -1 : 1;

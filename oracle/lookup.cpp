@@ -1,10 +1,10 @@
 // CPU ORACLE (test infrastructure): query lookup tables.
 // Restates rmblast-lib/src/blast/nalookup.rs:128-335 (small table, entries in
-// ascending query offset), blast/mblookup.rs:367-496 (megablast table, LIFO
+// ascending query offset), blast/mblookup.rs:106-245 (megablast table, LIFO
 // chains = descending query offset) and engine.rs:630-690 (combined two-context
 // buffer [fwd | N | rc]).  Both tables index every lut_width window of a
 // location that contains no code >= 4; a location shorter than word_length is
-// skipped (nalookup.rs:368-371, mblookup.rs:389-392).
+// skipped (nalookup.rs:134-141, mblookup.rs:137-147).
 #include "../rmblast_amd/csrc/rmb_dust.h"
 #include "oracle.hpp"
 
@@ -12,7 +12,7 @@
 
 namespace orc {
 
-int choose_lut_width(int ws, size_t e) {  // nalookup.rs:483-520
+int choose_lut_width(int ws, size_t e) {  // nalookup.rs:254-291
   switch (ws) {
     case 4: case 5: case 6: return ws;
     case 7: return e < 250 ? 6 : 7;
@@ -58,7 +58,7 @@ static Lookup build_table(const std::vector<uint8_t>& seq, const std::vector<std
   for (size_t i = 0; i < cells; ++i) {
     uint32_t a = L.cell_start[i], b = L.cell_start[i + 1];
     longest = std::max<int>(longest, (int)(b - a));
-    // megablast chains are LIFO (mblookup.rs:431-432 prepends): descending offsets
+    // megablast chains are LIFO (mblookup.rs:170 prepends): descending offsets
     if (mega) std::reverse(L.entries.begin() + a, L.entries.begin() + b);
   }
   L.longest_chain = longest;

@@ -101,7 +101,7 @@ QueryLookup build_query_lookup(const std::vector<uint8_t>& query, const Params& 
 struct Ungapped { int32_t score; uint32_t q_start, q_end, s_start, s_end; };
 Ungapped extend_ungapped(const uint8_t* query, size_t qlen_with_sent, const uint8_t* subject,
                          size_t slen_with_sent, uint32_t q_off, uint32_t s_off,
-                         const Matrix& m, int xdrop);          // ungapped.rs:223-306
+                         const Matrix& m, int xdrop);          // ungapped.rs:38-121
 
 struct DiagHash {                                              // diag_hash.rs:15-111
   std::vector<uint32_t> backbone;

@@ -2,7 +2,7 @@
 //
 //  k_lut_keys      lookup-table build, stage 1: one (record, word, offset)
 //                  key per window of the two-strand query buffer
-//                  (blast/nalookup.rs:128-181, blast/mblookup.rs:367-438)
+//                  (blast/nalookup.rs:128-181, blast/mblookup.rs:106-245)
 //  k_phase1        seeding + exact diagonal-hash dedup + ungapped X-drop for
 //                  (record, subject) work items pulled from a device queue
 //                  (search/engine.rs:1063-1277, diag_hash.rs, ungapped.rs)
@@ -76,7 +76,7 @@ static constexpr int kLongChunk = 8;  // steps per chunk of the full extension
 static constexpr int kFlankSteps = 5;  // flank steps per side of the fast first tier
 
 // Ungapped X-drop from (q_off, s_off) over sentinel-wrapped sequences: left
-// side fixed X, right side adaptive X (ungapped.rs:223-306).  qa / sa are the
+// side fixed X, right side adaptive X (ungapped.rs:38-121).  qa / sa are the
 // pool offsets of the start position.  Each side runs in chunks of 16 steps:
 // the chunk's 16 bytes of both sequences come in with four aligned loads, its
 // 16 matrix lookups are issued together, and the X-drop state advances
@@ -240,7 +240,7 @@ __device__ __forceinline__ bool ungapped_first(const uint8_t* __restrict__ qpool
 // (engine.rs:630-649).  Valid windows lie inside one strand, hold no code >= 4
 // and belong to a strand of at least word_size bases; invalid ones get the
 // sort-last key.  Small tables list offsets ascending per cell, megablast
-// tables descending (mblookup.rs:431-432): the radix sort is stable, so the
+// tables descending (mblookup.rs:170): the radix sort is stable, so the
 // value order fed in decides it.
 __global__ void k_lut_keys(const RecDesc* __restrict__ recs, uint32_t n_recs, const uint64_t* __restrict__ elem_base,
                            const uint8_t* __restrict__ qext, int lw, int word_size, int descending,

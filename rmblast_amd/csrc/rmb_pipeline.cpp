@@ -1300,6 +1300,14 @@ static void emit_results(rmb_ctx* c, const std::vector<ItemState>& items) {
 }
 
 // pairs: null for every (query, subject) pair, else the item ids q * nsub + s to search
+// Failure injection for tests of the error path (RMB_FORCE_ERROR=<stage>):
+// the host fails the search at that stage, after the stage's kernels ended
+// normally.
+static void fail_point(const char* stage) {
+  const char* e = std::getenv("RMB_FORCE_ERROR");
+  if (e && std::strcmp(e, stage) == 0) throw Failure(RMB_ERR_DEVICE, std::string("forced failure after ") + stage);
+}
+
 // Ungapped keep threshold of a query record of full_len bases: the caller's
 // fixed value, else the KA-derived cutoff rmblastn resolves per query before
 // its search (main.rs:1154-1170, ka_cutoff.rs:126-161), else cutoff_score_max.
@@ -1443,10 +1451,13 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
     if (b + 2 < batches.size()) enqueue_batch(c, batches[b + 2], d_qfwd, d_qrc, d_qext);
     t_seed += t1.us();
     Timer t2;
+    fail_point("phase1");
     std::vector<ItemState> items = phase2a(c, sr, survivors);
+    fail_point("phase2a");
     t_2a += t2.us();
     Timer t3;
     phase2b(c, sr, qf, lens, items, d_qfwd, d_qrc, tb_ms);
+    fail_point("phase2b");
     emit_results(c, items);
     t_2b += t3.us();
   }
@@ -1568,6 +1579,21 @@ static bool parse_matrix(const std::string& path, rmb_matrix& m, std::string& er
 }  // namespace rmb
 
 // ============================================================== C ABI =====
+// A search that fails part way drains the work it queued before the error
+// propagates, and leaves no partial results.
+template <class F>
+static void search_or_drain(rmb_ctx* c, F&& fn) {
+  try {
+    fn();
+  } catch (...) {
+    (void)hipStreamSynchronize(c->st1);
+    (void)hipStreamSynchronize(c->st);
+    c->hsps.clear();
+    c->ops.clear();
+    throw;
+  }
+}
+
 template <class F>
 static int guarded(rmb_ctx* c, F&& fn) {
   try {
@@ -1731,7 +1757,7 @@ int rmb_search(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const uint32_t
   return guarded(c, [&]() {
     if (nq == 0 || !qs || !lens) throw Failure(RMB_ERR_ARG, "empty query batch");
     hip_check(hipSetDevice(c->device), "hipSetDevice");
-    search_impl(c, nq, qs, lens);
+    search_or_drain(c, [&] { search_impl(c, nq, qs, lens); });
   });
 }
 
@@ -1752,7 +1778,7 @@ int rmb_search_pairs(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const ui
     std::sort(items.begin(), items.end());
     items.erase(std::unique(items.begin(), items.end()), items.end());
     hip_check(hipSetDevice(c->device), "hipSetDevice");
-    search_impl(c, nq, qs, lens, &items);
+    search_or_drain(c, [&] { search_impl(c, nq, qs, lens, &items); });
   });
 }
 

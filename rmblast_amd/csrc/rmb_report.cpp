@@ -31,12 +31,12 @@ void for_each_query(uint32_t n, unsigned threads, F&& fn) {
   for (auto& th : pool) th.join();
 }
 
-// Query offset and end in NCBI's strand frame (hits.rs:208-216): a minus HSP
+// Query offset and end in NCBI's strand frame (hits.rs:104-112): a minus HSP
 // counts from the query's far end.
 inline uint32_t q_off(const rmb_hsp& h, uint32_t q_len) { return h.strand ? q_len - h.q_end : h.q_start; }
 inline uint32_t q_fin(const rmb_hsp& h, uint32_t q_len) { return h.strand ? q_len - h.q_start : h.q_end; }
 
-// ScoreCompareHSPs (hits.rs:208-216): < 0 when a comes first
+// ScoreCompareHSPs (hits.rs:104-112): < 0 when a comes first
 int score_compare(const rmb_hsp& a, const rmb_hsp& b, uint32_t q_len) {
   if (a.score != b.score) return a.score > b.score ? -1 : 1;
   if (a.s_start != b.s_start) return a.s_start < b.s_start ? -1 : 1;
