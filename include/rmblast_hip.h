@@ -217,6 +217,9 @@ typedef struct rmb_report_spec {
 } rmb_report_spec;
 int rmb_report_tabular(rmb_ctx* ctx, const rmb_report_spec* spec, uint64_t* n_bytes);
 int rmb_report_copy(const rmb_ctx* ctx, char* out);
+/* The report text in place (not NUL-terminated), valid until the context's
+ * next search, report or destruction. */
+int rmb_report_text(const rmb_ctx* ctx, const char** text, uint64_t* n_bytes);
 
 /* Stage counters of the last search (the reference's diagnostics counters,
  * engine.rs:28-37 and gapped.rs:25-36): [0] seeds that reached the diagonal

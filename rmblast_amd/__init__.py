@@ -22,7 +22,7 @@ EXPORTED = [
     "rmb_matrix_load", "rmb_ctx_create", "rmb_ctx_destroy", "rmb_last_error", "rmb_library_load",
     "rmb_search", "rmb_search_pairs", "rmb_dust_intervals", "rmb_results_size", "rmb_results_copy", "rmb_stats", "rmb_tap_ungapped",
     "rmb_tap_align_score_only", "rmb_tap_align_ex", "rmb_ka_lookup", "rmb_ungapped_cutoff", "rmb_mask_level",
-    "rmb_report_order", "rmb_report_tabular", "rmb_report_copy", "rmb_align_stats_of",
+    "rmb_report_order", "rmb_report_tabular", "rmb_report_copy", "rmb_report_text", "rmb_align_stats_of",
 ]
 
 
@@ -89,6 +89,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rmb_report_order.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_uint64), P(ctypes.c_uint64)]
     lib.rmb_report_tabular.argtypes = [ctypes.c_void_p, P(RmbReportSpec), P(ctypes.c_uint64)]
     lib.rmb_report_copy.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+    lib.rmb_report_text.argtypes = [ctypes.c_void_p, P(ctypes.c_void_p), P(ctypes.c_uint64)]
     lib.rmb_dust_intervals.argtypes = [u8p, ctypes.c_uint64, P(ctypes.c_uint64), P(ctypes.c_uint64)]
     lib.rmb_tap_ungapped.argtypes = [ctypes.c_void_p, u8p, ctypes.c_uint32, ctypes.c_uint32,
                                      P(ctypes.c_uint32), P(ctypes.c_uint64)]
@@ -261,18 +262,24 @@ class Searcher:
                                                ctypes.byref(n)), "rmb_report_order")
         return rows
 
-    def report_tabular(self, fields: str, query_ids, subject_ids, mask_level: int = 101) -> bytes:
+    def report_tabular(self, fields: str, query_ids, subject_ids, mask_level: int = 101, copy: bool = True):
         """outfmt 6 text of the last search (mask level, order, statistics and
-        rows on the library's host threads)."""
+        rows on the library's host threads): bytes, or with copy=False a
+        read-only memoryview of the library's buffer, valid until the next
+        search or report."""
         qids = (ctypes.c_char_p * len(query_ids))(*[q.encode() for q in query_ids])
         sids = (ctypes.c_char_p * len(subject_ids))(*[x.encode() for x in subject_ids])
         spec = RmbReportSpec(mask_level, 0, fields.encode(), qids, sids)
         n = ctypes.c_uint64(0)
         self._check(self._lib.rmb_report_tabular(self._ctx, ctypes.byref(spec), ctypes.byref(n)),
                     "rmb_report_tabular")
-        buf = ctypes.create_string_buffer(max(n.value, 1))
-        self._check(self._lib.rmb_report_copy(self._ctx, buf), "rmb_report_copy")
-        return buf.raw[:n.value]
+        ptr, size = ctypes.c_void_p(), ctypes.c_uint64()
+        self._check(self._lib.rmb_report_text(self._ctx, ctypes.byref(ptr), ctypes.byref(size)), "rmb_report_text")
+        if size.value == 0:
+            return b"" if copy else memoryview(b"")
+        if copy:
+            return ctypes.string_at(ptr.value, size.value)
+        return memoryview((ctypes.c_char * size.value).from_address(ptr.value)).toreadonly()
 
     def stats(self):
         out = (ctypes.c_uint64 * len(STAT_NAMES))()

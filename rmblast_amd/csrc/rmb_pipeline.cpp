@@ -270,7 +270,7 @@ struct rmb_ctx {
   HostBuf h_qfwd, h_qrc;  // host query pools of the last search
   std::vector<const uint8_t*> last_q;  // the last search's whole queries (wrapped) in h_qfwd, for the report
   std::vector<uint32_t> last_qlen;
-  std::string report;                  // text of the last rmb_report_tabular
+  ReportText report;                   // text of the last rmb_report_tabular
   DevBuf b_qfwd, b_qrc, b_recs, b_elem, b_keys, b_keys2, b_vals, b_vals2, b_counts, b_cells, b_sort_tmp;
   DevBuf b_hits, b_hitcount, b_spill, b_hitres, b_queue, b_counters, b_err;
   DevBuf b_tasks, b_dpres, b_ring, b_ring_big, b_tbtasks, b_tbres, b_flat, b_rows, b_ops;
@@ -1328,7 +1328,7 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
   c->ops.clear();
   c->last_q.clear();
   c->last_qlen.clear();
-  c->report.clear();
+  c->report = ReportText{};
   const int ws = c->P.word_size;
   const uint32_t nsub = (uint32_t)c->subj.size();
   // ---- query pools.  A query of 1,999,800 bases or more is searched
@@ -1870,20 +1870,27 @@ int rmb_report_tabular(rmb_ctx* c, const rmb_report_spec* spec, uint64_t* n_byte
     std::vector<uint32_t> sl;
     const ReportData d = report_data(c, sp, sr, sl);
     const unsigned th = host_threads();
-    c->report.clear();
+    c->report = ReportText{};
     try {
       c->report = report_tabular(d, report_order(d, spec->mask_level, th), fields, spec->query_ids,
                                  spec->subject_ids, th);
     } catch (const std::invalid_argument& e) {
       throw Failure(RMB_ERR_ARG, e.what());
     }
-    *n_bytes = c->report.size();
+    *n_bytes = c->report.size;
   });
 }
 
 int rmb_report_copy(const rmb_ctx* c, char* out) {
-  if (!c || (!out && !c->report.empty())) return RMB_ERR_ARG;
-  if (!c->report.empty()) std::memcpy(out, c->report.data(), c->report.size());
+  if (!c || (!out && c->report.size)) return RMB_ERR_ARG;
+  if (c->report.size) std::memcpy(out, c->report.data.get(), c->report.size);
+  return RMB_OK;
+}
+
+int rmb_report_text(const rmb_ctx* c, const char** text, uint64_t* n_bytes) {
+  if (!c || !text || !n_bytes) return RMB_ERR_ARG;
+  *text = c->report.size ? c->report.data.get() : "";
+  *n_bytes = c->report.size;
   return RMB_OK;
 }
 

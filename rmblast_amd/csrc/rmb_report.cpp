@@ -6,6 +6,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <map>
 #include <stdexcept>
 #include <thread>
@@ -240,7 +241,7 @@ std::vector<std::vector<uint64_t>> report_order(const ReportData& d, int32_t mas
   return rows;
 }
 
-std::string report_tabular(const ReportData& d, const std::vector<std::vector<uint64_t>>& order,
+ReportText report_tabular(const ReportData& d, const std::vector<std::vector<uint64_t>>& order,
                            const std::vector<std::string>& fields, const char* const* query_ids,
                            const char* const* subject_ids, unsigned threads) {
   enum F {
@@ -298,11 +299,16 @@ std::string report_tabular(const ReportData& d, const std::vector<std::vector<ui
       o += '\n';
     }
   });
-  size_t total = 0;
-  for (const std::string& t : text) total += t.size();
-  std::string out;
-  out.reserve(total);
-  for (const std::string& t : text) out += t;
+  // one buffer, the queries' texts copied in parallel (not zero-filled first)
+  std::vector<uint64_t> at(d.n_queries + 1, 0);
+  for (uint32_t q = 0; q < d.n_queries; ++q) at[q + 1] = at[q] + text[q].size();
+  ReportText out;
+  out.size = at[d.n_queries];
+  out.data.reset(new char[out.size ? out.size : 1]);
+  for_each_query(d.n_queries, threads, [&](uint32_t q) {
+    if (!text[q].empty()) std::memcpy(out.data.get() + at[q], text[q].data(), text[q].size());
+    std::string().swap(text[q]);
+  });
   return out;
 }
 

@@ -5,6 +5,7 @@
 // and the outfmt 6 writer (rmblast-lib/src/output.rs:294-363).
 #pragma once
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -34,9 +35,15 @@ std::vector<std::vector<uint64_t>> report_order(const ReportData& d, int32_t mas
 // Mask level over one query's HSPs (indices into d.hsps), in place.
 void mask_level_filter(const ReportData& d, std::vector<uint64_t>& rows, int32_t level, uint32_t q_len);
 
+// A report's text in one buffer.
+struct ReportText {
+  std::unique_ptr<char[]> data;
+  uint64_t size = 0;
+};
+
 // The outfmt 6 rows of `order` (one line per HSP, tab-separated `fields`).
 // Throws std::invalid_argument naming an unknown field.
-std::string report_tabular(const ReportData& d, const std::vector<std::vector<uint64_t>>& order,
+ReportText report_tabular(const ReportData& d, const std::vector<std::vector<uint64_t>>& order,
                            const std::vector<std::string>& fields, const char* const* query_ids,
                            const char* const* subject_ids, unsigned threads);
 
