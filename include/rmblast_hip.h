@@ -133,6 +133,45 @@ typedef struct rmb_ka_cutoff_info {
 int rmb_ungapped_cutoff(const char* matrix_name, int32_t gap_open, int32_t gap_extend, int32_t min_raw_gapped_score,
                         uint64_t query_length, uint64_t avg_subj_length, int32_t* cutoff, rmb_ka_cutoff_info* info);
 
+/* ---- E-value statistics (host only) ---------------------------------------
+ * The gapped Karlin-Altschul block behind the evalue / bitscore fields and
+ * the --evalue reap, resolved as rmblastn resolves it (rmblast-lib/src/
+ * ka_stats.rs:153-300): the table of RepeatMasker matrices by file base name
+ * and gap costs (rmblast-stats/rmstats/src/tables_data.rs), then the
+ * -matrix_lambda/_k/_alpha overrides given together (with -matrix_beta), then
+ * a "# KARLIN" comment in the matrix file, else none: E-values print 1.0 and
+ * bit scores 0.0, and nothing is reaped.  ncbi_compat = 1 takes NCBI 2.17.1's
+ * sources only (no comparison.matrix entry, its 30p53g placeholder, no
+ * comment).  The reference's optional ALP fit of other matrices is not part
+ * of this library. */
+#define RMB_KA_SENTINEL 0
+#define RMB_KA_TABLE 1
+#define RMB_KA_CLI 2
+#define RMB_KA_COMMENT 3
+typedef struct rmb_ka_stats_options {
+  double matrix_lambda, matrix_k, matrix_alpha, matrix_beta;  /* 0 when not given */
+  int32_t ncbi_compat;
+  int32_t pad;
+} rmb_ka_stats_options;
+typedef struct rmb_ka_stats {
+  int32_t source;              /* RMB_KA_* */
+  int32_t pad;
+  double lambda, k, log_k, h;  /* -1 each without statistics */
+  double alpha, beta;          /* the length adjustment's slope (alpha / lambda) and intercept */
+} rmb_ka_stats;
+/* Resolve the statistics for a matrix file at its gap costs; a "# KARLIN"
+ * line that does not parse is RMB_ERR_ARG with its message in err. */
+int rmb_ka_stats_resolve(const char* matrix_path, int32_t gap_open, int32_t gap_extend,
+                         const rmb_ka_stats_options* opt, rmb_ka_stats* out, char* err, size_t err_len);
+/* Effective search space and length adjustment of a query of query_length
+ * bases against db_num_seqs sequences of db_length bases in all
+ * (BLAST_CalcEffLengths, rmstats/src/eff_lengths.rs:52-128, with
+ * BLAST_ComputeLengthAdjustment, length_adjust.rs:7-93); 0 without statistics. */
+int rmb_ka_search_space(const rmb_ka_stats* ka, int32_t query_length, int64_t db_length, int32_t db_num_seqs,
+                        int64_t* eff_searchsp, int32_t* length_adjustment);
+/* E-value and bit score of a raw score (rmstats/src/evalue.rs:96-126). */
+int rmb_ka_evalue(const rmb_ka_stats* ka, int64_t eff_searchsp, int32_t score, double* evalue, double* bit_score);
+
 /* Create a context on HIP device `device`.  Replaces the per-search state the
  * reference builds in rmblastn main (main.rs:516-700): parameters + matrix. */
 int rmb_ctx_create(int device, const rmb_params* params, const rmb_matrix* matrix, rmb_ctx** out);
@@ -193,28 +232,39 @@ int rmb_align_stats_of(const char* query_aligned, const char* subject_aligned, u
  * the query's length; keep[i] receives 1 for a survivor.  Host only. */
 int rmb_mask_level(const rmb_hsp* hsps, uint64_t n, uint32_t q_len, int32_t mask_level, uint8_t* keep);
 
-/* The HSPs of the last search that rmblastn reports, in its output order:
- * per query, mask level across all subjects (apply_mask_level,
- * engine.rs:863-974; mask_level < 0 or >= 100 keeps every HSP), then NCBI's
- * order (sort_hit_list_order_by_oid, engine.rs:998-1017, after main.rs:
- * 1230-1234).  rows receives indices into the results of rmb_results_copy,
- * queries in batch order.  Two-call: rows = NULL gives the count in *n. */
-int rmb_report_order(rmb_ctx* ctx, int32_t mask_level, uint64_t* rows, uint64_t* n);
-
-/* outfmt 6 of the last search (output.rs:294-363): the rows of
- * rmb_report_order with the tab-separated fields, each line ending in '\n'.
- * Fields: score perc_sub perc_query_gap perc_db_gap qseqid qstart qend qlen
- * sstrand sseqid sstart send slen kdiv cpg_kdiv transi transv cpg_sites qseq
- * sseq (with NCBI's column quirks, output.rs:338-340).  Queries are formatted
- * on the library's host threads.  The text is held by the context until the
- * next search or report; rmb_report_copy copies it out. */
+/* What rmblastn reports of the last search.  mask_level < 0 or >= 100 keeps
+ * every HSP.  ka (may be NULL: no statistics) gives the evalue / bitscore
+ * fields, per query from its whole length and the loaded library's residues
+ * and sequences; with evalue_max > 0 and statistics, HSPs whose E-value
+ * exceeds it are dropped first (the --evalue reap, main.rs:1224-1226 and
+ * 1418-1420, before mask level).  fields, query_ids and subject_ids are used
+ * by rmb_report_tabular only. */
 typedef struct rmb_report_spec {
   int32_t mask_level;
   int32_t pad;
   const char* fields;             /* field names separated by spaces */
   const char* const* query_ids;   /* qseqid of each query of the last search */
   const char* const* subject_ids; /* sseqid of each loaded subject */
+  const rmb_ka_stats* ka;
+  double evalue_max;
 } rmb_report_spec;
+
+/* The HSPs of the last search that rmblastn reports, in its output order:
+ * per query, the reap, then mask level across all subjects (apply_mask_level,
+ * engine.rs:863-974), then NCBI's order (sort_hit_list_order_by_oid,
+ * engine.rs:998-1017, after main.rs:1230-1234).  rows receives indices into
+ * the results of rmb_results_copy, queries in batch order.  Two-call:
+ * rows = NULL gives the count in *n. */
+int rmb_report_order(rmb_ctx* ctx, const rmb_report_spec* spec, uint64_t* rows, uint64_t* n);
+
+/* outfmt 6 of the last search (output.rs:294-363): the rows of
+ * rmb_report_order with the tab-separated fields, each line ending in '\n'.
+ * Fields: score perc_sub perc_query_gap perc_db_gap qseqid qstart qend qlen
+ * sstrand sseqid sstart send slen kdiv cpg_kdiv transi transv cpg_sites qseq
+ * sseq evalue bitscore (with NCBI's column quirks, output.rs:338-340, and its
+ * number formats, output.rs:256-288).  Queries are formatted on the
+ * library's host threads.  The text is held by the context until the next
+ * search or report; rmb_report_copy copies it out. */
 int rmb_report_tabular(rmb_ctx* ctx, const rmb_report_spec* spec, uint64_t* n_bytes);
 int rmb_report_copy(const rmb_ctx* ctx, char* out);
 /* The report text in place (not NUL-terminated), valid until the context's

@@ -215,3 +215,27 @@ int orc_ungapped_cutoff(const char* name, int go, int ge, int min_raw, uint64_t 
   return c;
 }
 }
+
+extern "C" {
+// ka_stats.rs resolve: opt = {lambda, k, alpha, beta}; out = {lambda, k, log_k, h, alpha, beta}; returns the
+// source (0 sentinel, 1 table, 2 CLI, 3 comment) or -1 on a bad KARLIN line
+int orc_ka_stats(const char* path, int go, int ge, const double* opt, int ncbi_compat, double* out) {
+  KaOptions o;
+  o.lambda = opt[0]; o.k = opt[1]; o.alpha = opt[2]; o.beta = opt[3]; o.ncbi_compat = ncbi_compat != 0;
+  KaStats s{};
+  std::string err;
+  if (!resolve_ka_stats(path, go, ge, o, s, err)) return -1;
+  const double v[6] = {s.lambda, s.k, s.log_k, s.h, s.alpha, s.beta};
+  for (int i = 0; i < 6; ++i) out[i] = v[i];
+  return s.source;
+}
+int orc_length_adjustment(double K, double logK, double a_d_l, double beta, int qlen, int64_t dblen, int nseq) {
+  return compute_length_adjustment(K, logK, a_d_l, beta, qlen, dblen, nseq);
+}
+// writes the E-value and bit-score strings (NUL-terminated) into e and b (64 bytes each)
+void orc_format_scores(double evalue, double bits, char* e, char* b) {
+  const std::string x = fmt_evalue(evalue), y = fmt_bits(bits);
+  std::snprintf(e, 64, "%s", x.c_str());
+  std::snprintf(b, 64, "%s", y.c_str());
+}
+}

@@ -1153,6 +1153,12 @@ std::vector<AlignResult> search_query(const std::vector<uint8_t>& query, const s
   std::vector<AlignResult> res;
   for (auto& v : per)
     for (auto& x : v) res.push_back(std::move(x));
+  if (p.reap_ka) {  // main.rs:1224-1226: the E-value reap comes before mask level
+    std::vector<AlignResult> kept;
+    for (auto& r : res)
+      if (evalue_of(*p.reap_ka, p.reap_searchsp, r.hsp.score) <= p.reap_evalue) kept.push_back(std::move(r));
+    res.swap(kept);
+  }
   apply_mask_level(res, p.mask_level, names);
   std::sort(res.begin(), res.end(), [](const AlignResult& a, const AlignResult& b) {
     if (a.subject_id != b.subject_id) return a.subject_id < b.subject_id;
@@ -1162,7 +1168,8 @@ std::vector<AlignResult> search_query(const std::vector<uint8_t>& query, const s
   return res;
 }
 
-std::string format_tabular(const AlignResult& r, const std::vector<std::string>& fields) {  // output.rs:294-363
+std::string format_tabular(const AlignResult& r, const std::vector<std::string>& fields, const KaStats* ka,
+                           int64_t searchsp) {  // output.rs:294-363
   const Hsp& h = r.hsp;
   const AlignStats& st = r.stats;
   uint32_t ss = h.strand == PLUS ? h.s_start + 1 : h.s_end;
@@ -1192,8 +1199,8 @@ std::string format_tabular(const AlignResult& r, const std::vector<std::string>&
     else if (f == "cpg_sites") out += std::to_string(st.transitions);  // output.rs:340 quirk
     else if (f == "qseq") out += to_iupac(h.q_seq);
     else if (f == "sseq") out += to_iupac(h.s_seq);
-    else if (f == "evalue") out += "1.0";
-    else if (f == "bitscore") out += " 0.0";
+    else if (f == "evalue") out += fmt_evalue(ka ? evalue_of(*ka, searchsp, h.score) : 1.0);
+    else if (f == "bitscore") out += fmt_bits(ka ? bit_score_of(*ka, h.score) : 0.0);
   }
   return out;
 }

@@ -20,10 +20,11 @@ static std::string norm_flag(std::string a) {
 
 int main(int argc, char** argv) {
   std::map<std::string, std::string> kv;
-  bool complexity = false;
+  bool complexity = false, ncbi_compat = false;
   for (int i = 1; i < argc; ++i) {
     std::string f = norm_flag(argv[i]);
     if (f == "--complexity-adjust") { complexity = true; continue; }
+    if (f == "--ncbi-compat") { ncbi_compat = true; continue; }
     if (i + 1 >= argc) { fprintf(stderr, "missing value for %s\n", argv[i]); return 2; }
     kv[f] = argv[++i];
   }
@@ -71,6 +72,25 @@ int main(int argc, char** argv) {
   for (auto& r : db) db_letters += r.seq.size() - 2;
   const uint64_t avg_subj = db.empty() ? 0 : db_letters / db.size();
   const std::string matrix_name = get("--matrix", "");
+  // E-value statistics when the fields print them or a reap threshold is set
+  // (main.rs:612-686): --evalue, or NCBI's 10 under --ncbi-compat
+  KaOptions kopt;
+  kopt.lambda = std::stod(get("--matrix-lambda", "0"));
+  kopt.k = std::stod(get("--matrix-k", "0"));
+  kopt.alpha = std::stod(get("--matrix-alpha", "0"));
+  kopt.beta = std::stod(get("--matrix-beta", "0"));
+  kopt.ncbi_compat = ncbi_compat;
+  const bool evalue_given = kv.count("--evalue") > 0;
+  const double threshold = evalue_given ? std::stod(kv["--evalue"]) : (ncbi_compat ? 10.0 : 0.0);
+  bool stats_fields = false;
+  for (auto& f : fields) stats_fields |= f == "evalue" || f == "bitscore";
+  KaStats ka{};
+  const bool have_ka = stats_fields || threshold > 0;
+  if (have_ka && !resolve_ka_stats(matrix_name, p.gap_open, p.gap_extend, kopt, ka, err)) {
+    fprintf(stderr, "matrix: %s\n", err.c_str());
+    return 1;
+  }
+  const bool reap = threshold > 0 && ka.lambda > 0;
   auto t0 = std::chrono::steady_clock::now();
   for (auto& q : qs) {
     // the cutoff of this query, from its whole length (main.rs:1154-1170)
@@ -99,10 +119,16 @@ int main(int argc, char** argv) {
       }
       continue;
     }
+    const int64_t searchsp = have_ka ? effective_search_space(ka, (int)(q.seq.size() - 2), (int64_t)db_letters,
+                                                               (int)db.size(), nullptr)
+                                     : 0;
+    p.reap_ka = reap ? &ka : nullptr;
+    p.reap_searchsp = searchsp;
+    p.reap_evalue = threshold;
     std::vector<AlignResult> res = search_query(q.seq, q.id, subjects, names, p, m, threads);
     sort_hit_list_order(res, names);
     std::string out;
-    for (auto& r : res) { out += format_tabular(r, fields); out += '\n'; }
+    for (auto& r : res) { out += format_tabular(r, fields, have_ka ? &ka : nullptr, searchsp); out += '\n'; }
     fwrite(out.data(), 1, out.size(), stdout);
   }
   double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

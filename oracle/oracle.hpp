@@ -35,6 +35,7 @@ bool matrix_from_text(const std::string& name, const std::string& text, Matrix& 
 bool matrix_from_file(const std::string& path, Matrix& m, std::string& err);
 
 // ---- parameters (rmblast-lib/src/options.rs:12-104) -------------------------
+struct KaStats;
 struct Params {
   int gap_open = 20, gap_extend = 5;
   int word_size = 11;
@@ -44,6 +45,11 @@ struct Params {
   bool complexity_adjust = false;
   bool dust = false;
   int mask_level = 101;
+  // --evalue reap before mask level (main.rs:1224-1226): statistics, the
+  // query's effective search space and the threshold; null = off
+  const KaStats* reap_ka = nullptr;
+  int64_t reap_searchsp = 0;
+  double reap_evalue = 0;
   int gapped_cutoff() const { return min_raw_gapped_score > 1 ? min_raw_gapped_score : 1; }
   int cutoff_score_max() const { return min_raw_gapped_score > 0 ? min_raw_gapped_score / 2 : 1; }
   int ungapped_keep() const { return ungapped_cutoff >= 0 ? ungapped_cutoff : cutoff_score_max(); }
@@ -203,7 +209,8 @@ std::vector<AlignResult> search_query(const std::vector<uint8_t>& query, const s
                                       const std::vector<std::string>& subject_names, const Params& p,
                                       const Matrix& m, int threads);
 // output.rs:294-363 outfmt 6
-std::string format_tabular(const AlignResult& r, const std::vector<std::string>& fields);
+std::string format_tabular(const AlignResult& r, const std::vector<std::string>& fields, const KaStats* ka = nullptr,
+                           int64_t searchsp = 0);
 
 // engine.rs:383-572 (query/subject without sentinels)
 void improve_seed(const uint8_t* query, size_t qlen, const uint8_t* subject, size_t slen, uint32_t q_seed,
@@ -216,6 +223,20 @@ bool ka_lookup(const std::string& matrix_name, int gap_open, int gap_extend, dou
 // :126-153; query_length = the whole query record, avg_subj_length = db residues / sequences
 int ungapped_cutoff(const std::string& matrix_name, int gap_open, int gap_extend, int min_raw_gapped_score,
                     uint64_t query_length, uint64_t avg_subj_length, KaCutoffInfo* info);
+
+// ---- E-value statistics (rmblast-lib/src/ka_stats.rs:153-300, rmstats) ----------
+enum { KA_SENTINEL = 0, KA_TABLE = 1, KA_CLI = 2, KA_COMMENT = 3 };
+struct KaStats { int source; double lambda, k, log_k, h, alpha, beta; };
+struct KaOptions { double lambda = 0, k = 0, alpha = 0, beta = 0; bool ncbi_compat = false; };
+bool karlin_comment(const std::string& path, bool& found, double kv[5], std::string& err);
+bool resolve_ka_stats(const std::string& matrix_path, int go, int ge, const KaOptions& o, KaStats& out,
+                      std::string& err);
+int compute_length_adjustment(double K, double logK, double a_d_l, double beta, int qlen, int64_t dblen, int nseq);
+int64_t effective_search_space(const KaStats& s, int qlen, int64_t dblen, int nseq, int* la_out);
+double evalue_of(const KaStats& s, int64_t searchsp, int score);
+double bit_score_of(const KaStats& s, int score);
+std::string fmt_evalue(double e);
+std::string fmt_bits(double v);
 
 // unit-level entry points used by tests
 void purge_prelim_common_endpoints(std::vector<PrelimHsp>& v, uint32_t sa_len);

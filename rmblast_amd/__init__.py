@@ -23,6 +23,7 @@ EXPORTED = [
     "rmb_search", "rmb_search_pairs", "rmb_dust_intervals", "rmb_results_size", "rmb_results_copy", "rmb_stats", "rmb_tap_ungapped",
     "rmb_tap_align_score_only", "rmb_tap_align_ex", "rmb_ka_lookup", "rmb_ungapped_cutoff", "rmb_mask_level",
     "rmb_report_order", "rmb_report_tabular", "rmb_report_copy", "rmb_report_text", "rmb_align_stats_of",
+    "rmb_ka_stats_resolve", "rmb_ka_search_space", "rmb_ka_evalue",
 ]
 
 
@@ -33,9 +34,24 @@ class RmbParams(ctypes.Structure):
         ("ka_lambda", ctypes.c_double), ("ka_k", ctypes.c_double)]
 
 
+class RmbKaStatsOptions(ctypes.Structure):
+    _fields_ = [("matrix_lambda", ctypes.c_double), ("matrix_k", ctypes.c_double), ("matrix_alpha", ctypes.c_double),
+                ("matrix_beta", ctypes.c_double), ("ncbi_compat", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+class RmbKaStats(ctypes.Structure):
+    _fields_ = [("source", ctypes.c_int32), ("pad", ctypes.c_int32), ("lambda_", ctypes.c_double),
+                ("k", ctypes.c_double), ("log_k", ctypes.c_double), ("h", ctypes.c_double),
+                ("alpha", ctypes.c_double), ("beta", ctypes.c_double)]
+
+
+KA_SOURCES = ("sentinel", "table", "cli", "comment")
+
+
 class RmbReportSpec(ctypes.Structure):
     _fields_ = [("mask_level", ctypes.c_int32), ("pad", ctypes.c_int32), ("fields", ctypes.c_char_p),
-                ("query_ids", ctypes.POINTER(ctypes.c_char_p)), ("subject_ids", ctypes.POINTER(ctypes.c_char_p))]
+                ("query_ids", ctypes.POINTER(ctypes.c_char_p)), ("subject_ids", ctypes.POINTER(ctypes.c_char_p)),
+                ("ka", ctypes.POINTER(RmbKaStats)), ("evalue_max", ctypes.c_double)]
 
 
 class RmbKaCutoffInfo(ctypes.Structure):
@@ -86,7 +102,13 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                         ctypes.c_uint64, ctypes.c_uint64, P(ctypes.c_int32), P(RmbKaCutoffInfo)]
     lib.rmb_mask_level.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int32,
                                    P(ctypes.c_uint8)]
-    lib.rmb_report_order.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_uint64), P(ctypes.c_uint64)]
+    lib.rmb_report_order.argtypes = [ctypes.c_void_p, P(RmbReportSpec), P(ctypes.c_uint64), P(ctypes.c_uint64)]
+    lib.rmb_ka_stats_resolve.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, P(RmbKaStatsOptions),
+                                         P(RmbKaStats), ctypes.c_char_p, ctypes.c_size_t]
+    lib.rmb_ka_search_space.argtypes = [P(RmbKaStats), ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
+                                        P(ctypes.c_int64), P(ctypes.c_int32)]
+    lib.rmb_ka_evalue.argtypes = [P(RmbKaStats), ctypes.c_int64, ctypes.c_int32, P(ctypes.c_double),
+                                  P(ctypes.c_double)]
     lib.rmb_report_tabular.argtypes = [ctypes.c_void_p, P(RmbReportSpec), P(ctypes.c_uint64)]
     lib.rmb_report_copy.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
     lib.rmb_report_text.argtypes = [ctypes.c_void_p, P(ctypes.c_void_p), P(ctypes.c_uint64)]
@@ -154,6 +176,33 @@ def ungapped_cutoff(matrix_name: str, gap_open: int, gap_extend: int, min_raw_ga
     if not info.found:
         return c.value, None
     return c.value, {"lambda": info.lambda_, "k": info.k, "searchsp": info.searchsp, "es": info.es}
+
+
+def ka_stats(matrix_path: str, gap_open: int, gap_extend: int, matrix_lambda=0.0, matrix_k=0.0,
+             matrix_alpha=0.0, matrix_beta=0.0, ncbi_compat=False) -> RmbKaStats:
+    """E-value statistics of a matrix file at its gap costs (ka_stats.rs:153-300)."""
+    opt = RmbKaStatsOptions(matrix_lambda, matrix_k, matrix_alpha, matrix_beta, int(bool(ncbi_compat)), 0)
+    out = RmbKaStats()
+    err = ctypes.create_string_buffer(256)
+    if lib().rmb_ka_stats_resolve(matrix_path.encode(), gap_open, gap_extend, ctypes.byref(opt), ctypes.byref(out),
+                                  err, 256):
+        raise RmbError(err.value.decode())
+    return out
+
+
+def ka_search_space(ka: RmbKaStats, query_length: int, db_length: int, db_num_seqs: int):
+    """(effective search space, length adjustment) of one query."""
+    sp, la = ctypes.c_int64(0), ctypes.c_int32(0)
+    lib().rmb_ka_search_space(ctypes.byref(ka), query_length, db_length, db_num_seqs, ctypes.byref(sp),
+                              ctypes.byref(la))
+    return sp.value, la.value
+
+
+def ka_evalue(ka: RmbKaStats, eff_searchsp: int, score: int):
+    """(E-value, bit score) of a raw score."""
+    e, b = ctypes.c_double(0), ctypes.c_double(0)
+    lib().rmb_ka_evalue(ctypes.byref(ka), eff_searchsp, score, ctypes.byref(e), ctypes.byref(b))
+    return e.value, b.value
 
 
 def mask_level(hsps: np.ndarray, q_len: int, level: int) -> np.ndarray:
@@ -252,24 +301,29 @@ class Searcher:
                                    ops.ctypes.data_as(ctypes.c_void_p))
         return hsps, ops
 
-    def report_order(self, mask_level: int) -> np.ndarray:
+    def report_order(self, mask_level: int, ka: RmbKaStats = None, evalue_max: float = 0.0) -> np.ndarray:
         """Indices of the last search's HSPs that rmblastn reports, in its order."""
+        spec = RmbReportSpec(mask_level, 0, None, None, None, ctypes.pointer(ka) if ka is not None else None,
+                             evalue_max)
         n = ctypes.c_uint64(0)
-        self._check(self._lib.rmb_report_order(self._ctx, mask_level, None, ctypes.byref(n)), "rmb_report_order")
+        self._check(self._lib.rmb_report_order(self._ctx, ctypes.byref(spec), None, ctypes.byref(n)),
+                    "rmb_report_order")
         rows = np.zeros(n.value, dtype=np.uint64)
-        self._check(self._lib.rmb_report_order(self._ctx, mask_level,
+        self._check(self._lib.rmb_report_order(self._ctx, ctypes.byref(spec),
                                                rows.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
                                                ctypes.byref(n)), "rmb_report_order")
         return rows
 
-    def report_tabular(self, fields: str, query_ids, subject_ids, mask_level: int = 101, copy: bool = True):
+    def report_tabular(self, fields: str, query_ids, subject_ids, mask_level: int = 101, copy: bool = True,
+                       ka: RmbKaStats = None, evalue_max: float = 0.0):
         """outfmt 6 text of the last search (mask level, order, statistics and
         rows on the library's host threads): bytes, or with copy=False a
         read-only memoryview of the library's buffer, valid until the next
         search or report."""
         qids = (ctypes.c_char_p * len(query_ids))(*[q.encode() for q in query_ids])
         sids = (ctypes.c_char_p * len(subject_ids))(*[x.encode() for x in subject_ids])
-        spec = RmbReportSpec(mask_level, 0, fields.encode(), qids, sids)
+        spec = RmbReportSpec(mask_level, 0, fields.encode(), qids, sids,
+                             ctypes.pointer(ka) if ka is not None else None, evalue_max)
         n = ctypes.c_uint64(0)
         self._check(self._lib.rmb_report_tabular(self._ctx, ctypes.byref(spec), ctypes.byref(n)),
                     "rmb_report_tabular")

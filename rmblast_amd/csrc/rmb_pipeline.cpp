@@ -259,6 +259,7 @@ struct rmb_ctx {
   std::vector<Subject> subj;
   std::vector<SubjDesc> sdesc;
   uint64_t avg_subj_len = 0;  // total residues / sequences, truncated (BlastSeqSrcGetAvgSeqLen)
+  uint64_t db_letters = 0;    // total residues of the loaded subjects
   DevBuf d_splus, d_src, d_salign, d_sdesc;
   // results of the last search
   std::vector<rmb_hsp> hsps;
@@ -1616,7 +1617,10 @@ extern "C" {
 int rmb_matrix_load(const char* path, rmb_matrix* out, char* err, size_t err_len) {
   if (!path || !out) return RMB_ERR_ARG;
   std::string e;
-  if (!rmb::parse_matrix(path, *out, e)) {
+  double karlin[5];
+  bool has_karlin = false;
+  // a "# KARLIN" comment line must parse too (matrix.rs:100-104,248-266)
+  if (!rmb::parse_matrix(path, *out, e) || !rmb::read_karlin_comment(path, karlin, has_karlin, e)) {
     if (err && err_len) { std::strncpy(err, e.c_str(), err_len - 1); err[err_len - 1] = 0; }
     return RMB_ERR_ARG;
   }
@@ -1737,7 +1741,8 @@ int rmb_library_load(rmb_ctx* c, uint32_t n, const uint8_t* const* seqs, const u
       c->sdesc[i].s_len = lens[i];
       off += lens[i] + 2;
     }
-    c->avg_subj_len = (off - 2ull * n) / n;
+    c->db_letters = off - 2ull * n;
+    c->avg_subj_len = c->db_letters / n;
     std::vector<uint8_t> pp(off), pr(off), pa(off);
     for (uint32_t i = 0; i < n; ++i) {
       std::memcpy(pp.data() + c->sdesc[i].s_base, c->subj[i].plus.data(), c->subj[i].plus.size());
@@ -1783,8 +1788,8 @@ int rmb_search_pairs(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const ui
 }
 
 // The last search's HSPs and sequences for the host tail.
-static ReportData report_data(const rmb_ctx* c, std::vector<const uint8_t*>& sp, std::vector<const uint8_t*>& sr,
-                              std::vector<uint32_t>& sl) {
+static ReportData report_data(const rmb_ctx* c, const rmb_report_spec* spec, std::vector<const uint8_t*>& sp,
+                              std::vector<const uint8_t*>& sr, std::vector<uint32_t>& sl) {
   for (const Subject& S : c->subj) {
     sp.push_back(S.align.data());
     sr.push_back(S.align_rc.data());
@@ -1801,6 +1806,12 @@ static ReportData report_data(const rmb_ctx* c, std::vector<const uint8_t*>& sp,
   d.subjects = sp.data();
   d.subjects_rc = sr.data();
   d.subject_lens = sl.data();
+  if (spec) {
+    d.ka = spec->ka;
+    d.evalue_max = spec->evalue_max;
+  }
+  d.db_length = (int64_t)c->db_letters;
+  d.db_num_seqs = (int32_t)c->subj.size();
   return d;
 }
 
@@ -1828,13 +1839,13 @@ int rmb_mask_level(const rmb_hsp* hsps, uint64_t n, uint32_t q_len, int32_t mask
   }
 }
 
-int rmb_report_order(rmb_ctx* c, int32_t mask_level, uint64_t* rows, uint64_t* n) {
-  if (!c || !n) return RMB_ERR_ARG;
+int rmb_report_order(rmb_ctx* c, const rmb_report_spec* spec, uint64_t* rows, uint64_t* n) {
+  if (!c || !spec || !n) return RMB_ERR_ARG;
   return guarded(c, [&]() {
     std::vector<const uint8_t*> sp, sr;
     std::vector<uint32_t> sl;
-    const ReportData d = report_data(c, sp, sr, sl);
-    const auto order = report_order(d, mask_level, host_threads());
+    const ReportData d = report_data(c, spec, sp, sr, sl);
+    const auto order = report_order(d, spec->mask_level, host_threads());
     uint64_t k = 0;
     for (const auto& q : order) {
       if (rows)
@@ -1868,7 +1879,7 @@ int rmb_report_tabular(rmb_ctx* c, const rmb_report_spec* spec, uint64_t* n_byte
     if (fields.empty()) throw Failure(RMB_ERR_ARG, "no outfmt fields");
     std::vector<const uint8_t*> sp, sr;
     std::vector<uint32_t> sl;
-    const ReportData d = report_data(c, sp, sr, sl);
+    const ReportData d = report_data(c, spec, sp, sr, sl);
     const unsigned th = host_threads();
     c->report = ReportText{};
     try {
@@ -1891,6 +1902,34 @@ int rmb_report_text(const rmb_ctx* c, const char** text, uint64_t* n_bytes) {
   if (!c || !text || !n_bytes) return RMB_ERR_ARG;
   *text = c->report.size ? c->report.data.get() : "";
   *n_bytes = c->report.size;
+  return RMB_OK;
+}
+
+int rmb_ka_stats_resolve(const char* matrix_path, int32_t gap_open, int32_t gap_extend,
+                         const rmb_ka_stats_options* opt, rmb_ka_stats* out, char* err, size_t err_len) {
+  if (!matrix_path || !opt || !out) return RMB_ERR_ARG;
+  std::string msg;
+  if (!ka_stats_resolve(matrix_path, gap_open, gap_extend, *opt, *out, msg)) {
+    if (err && err_len) {
+      std::strncpy(err, msg.c_str(), err_len - 1);
+      err[err_len - 1] = 0;
+    }
+    return RMB_ERR_ARG;
+  }
+  return RMB_OK;
+}
+
+int rmb_ka_search_space(const rmb_ka_stats* ka, int32_t query_length, int64_t db_length, int32_t db_num_seqs,
+                        int64_t* eff_searchsp, int32_t* la) {
+  if (!ka || !eff_searchsp || !la) return RMB_ERR_ARG;
+  ka_search_space(*ka, query_length, db_length, db_num_seqs, *eff_searchsp, *la);
+  return RMB_OK;
+}
+
+int rmb_ka_evalue(const rmb_ka_stats* ka, int64_t eff_searchsp, int32_t score, double* evalue, double* bit_score) {
+  if (!ka) return RMB_ERR_ARG;
+  if (evalue) *evalue = ka_evalue(*ka, eff_searchsp, score);
+  if (bit_score) *bit_score = ka_bit_score(*ka, score);
   return RMB_OK;
 }
 

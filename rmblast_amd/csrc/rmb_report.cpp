@@ -2,6 +2,8 @@
 // and the outfmt 6 writer, per query on the host threads.
 #include "rmb_report.h"
 
+#include "rmb_ka.h"
+
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -230,11 +232,48 @@ void aligned_strings(const ReportData& d, const rmb_hsp& h, std::string& qa, std
   }
 }
 
+// The effective search space of query q (0 without statistics).
+static int64_t query_searchsp(const ReportData& d, uint32_t q) {
+  if (!d.ka) return 0;
+  int64_t sp = 0;
+  int32_t la = 0;
+  ka_search_space(*d.ka, (int32_t)d.query_lens[q], d.db_length, d.db_num_seqs, sp, la);
+  return sp;
+}
+
+std::string format_evalue(double e) {
+  char b[64];
+  if (e < 1.0e-180) return "0.0";
+  if (e < 0.0009) std::snprintf(b, sizeof b, "%.2e", e);
+  else if (e < 0.1) std::snprintf(b, sizeof b, "%.3f", e);
+  else if (e < 1.0) std::snprintf(b, sizeof b, "%.2f", e);
+  else if (e < 10.0) std::snprintf(b, sizeof b, "%.1f", e);
+  else std::snprintf(b, sizeof b, "%.0f", e);
+  return b;
+}
+
+std::string format_bit_score(double bits) {
+  char b[64];
+  if (bits > 99999.0) std::snprintf(b, sizeof b, "%.3e", bits);
+  else if (bits > 99.9) std::snprintf(b, sizeof b, "%3lld", (long long)bits);  // truncated toward zero
+  else std::snprintf(b, sizeof b, "%4.1f", bits);
+  return b;
+}
+
 std::vector<std::vector<uint64_t>> report_order(const ReportData& d, int32_t mask_level, unsigned threads) {
   std::vector<std::vector<uint64_t>> rows(d.n_queries);
   for (uint64_t i = 0; i < d.n_hsps; ++i) rows[d.hsps[i].query].push_back(i);
   const bool masking = mask_level >= 0 && mask_level < 100;
+  const bool reap = d.ka && d.ka->lambda > 0.0 && d.evalue_max > 0.0;
   for_each_query(d.n_queries, threads, [&](uint32_t q) {
+    if (reap) {  // the --evalue reap, before mask level
+      const int64_t sp = query_searchsp(d, q);
+      std::vector<uint64_t> kept;
+      kept.reserve(rows[q].size());
+      for (uint64_t r : rows[q])
+        if (ka_evalue(*d.ka, sp, d.hsps[r].score) <= d.evalue_max) kept.push_back(r);
+      rows[q].swap(kept);
+    }
     if (masking) mask_level_filter(d, rows[q], mask_level, d.query_lens[q]);
     output_order(d, rows[q], d.query_lens[q]);
   });
@@ -246,11 +285,11 @@ ReportText report_tabular(const ReportData& d, const std::vector<std::vector<uin
                            const char* const* subject_ids, unsigned threads) {
   enum F {
     SCORE, PERC_SUB, PERC_QGAP, PERC_DGAP, QSEQID, QSTART, QEND, QLEN, SSTRAND, SSEQID, SSTART, SEND, SLEN, KDIV,
-    CPG_KDIV, TRANSI, TRANSV, CPG_SITES, QSEQ, SSEQ
+    CPG_KDIV, TRANSI, TRANSV, CPG_SITES, QSEQ, SSEQ, EVALUE, BITSCORE
   };
   static const char* kNames[] = {"score", "perc_sub", "perc_query_gap", "perc_db_gap", "qseqid", "qstart", "qend",
                                  "qlen", "sstrand", "sseqid", "sstart", "send", "slen", "kdiv", "cpg_kdiv", "transi",
-                                 "transv", "cpg_sites", "qseq", "sseq"};
+                                 "transv", "cpg_sites", "qseq", "sseq", "evalue", "bitscore"};
   std::vector<int> f;
   bool need_aln = false, need_stats = false;
   for (const std::string& name : fields) {
@@ -265,6 +304,7 @@ ReportText report_tabular(const ReportData& d, const std::vector<std::vector<uin
   for_each_query(d.n_queries, threads, [&](uint32_t q) {
     std::string& o = text[q];
     std::string qa, sa;
+    const int64_t sp = query_searchsp(d, q);
     for (uint64_t r : order[q]) {
       const rmb_hsp& h = d.hsps[r];
       if (need_aln) aligned_strings(d, h, qa, sa);
@@ -294,6 +334,8 @@ ReportText report_tabular(const ReportData& d, const std::vector<std::vector<uin
           case CPG_SITES: o += std::to_string(st.transitions); break;
           case QSEQ: o += qa; break;
           case SSEQ: o += sa; break;
+          case EVALUE: o += format_evalue(d.ka ? ka_evalue(*d.ka, sp, h.score) : 1.0); break;
+          case BITSCORE: o += format_bit_score(d.ka ? ka_bit_score(*d.ka, h.score) : 0.0); break;
         }
       }
       o += '\n';

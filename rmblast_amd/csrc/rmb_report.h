@@ -25,7 +25,17 @@ struct ReportData {
   const uint8_t* const* subjects = nullptr;     // sentinel-wrapped, original ambiguity codes
   const uint8_t* const* subjects_rc = nullptr;  // their reverse complements
   const uint32_t* subject_lens = nullptr;
+  // E-value statistics (null: none) with the database's residues and
+  // sequences, and the reap threshold (<= 0: no reap)
+  const rmb_ka_stats* ka = nullptr;
+  int64_t db_length = 0;
+  int32_t db_num_seqs = 0;
+  double evalue_max = 0;
 };
+
+// NCBI's tabular number formats (output.rs:256-288).
+std::string format_evalue(double evalue);
+std::string format_bit_score(double bits);
 
 // Per query, the indices of its HSPs that survive mask level, in output
 // order.  mask_level >= 100 (or < 0) keeps every HSP.  Queries run on

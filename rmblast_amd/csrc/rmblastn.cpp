@@ -342,10 +342,11 @@ void hsp_block(std::string& o, const Row& r, const rmb_align_stats& st, const st
 
 int main(int argc, char** argv) {
   std::map<std::string, std::string> kv;
-  bool complexity = false;
+  bool complexity = false, ncbi_compat = false;
   for (int i = 1; i < argc; ++i) {
     const std::string f = norm(argv[i]);
     if (f == "--complexity-adjust") { complexity = true; continue; }
+    if (f == "--ncbi-compat") { ncbi_compat = true; continue; }
     if (i + 1 >= argc) { fprintf(stderr, "error: %s needs a value\n", argv[i]); return 2; }
     kv[f] = argv[++i];
   }
@@ -410,17 +411,45 @@ int main(int argc, char** argv) {
                 names.c_str());
         return 1;
       }
-      if (w == "evalue" || w == "bitscore") {  // ka_stats.rs: E-values need the statistics fit
-        fprintf(stderr, "error: --outfmt '%s': field '%s' needs Karlin-Altschul statistics, which this build does "
-                "not compute\n", spec.c_str(), w.c_str());
-        return 1;
-      }
       fields.push_back(w);
     }
     if (!pairwise && fields.empty()) {
       fprintf(stderr, "error: --outfmt '%s': -outfmt 6 needs a field list; NCBI's default columns are not supported. "
               "Fields: %s\n", spec.c_str(), names.c_str());
       return 1;
+    }
+  }
+  // E-value statistics options (main.rs:243-281): --evalue > 0, the -matrix_*
+  // overrides finite (lambda, k and alpha not negative), --ncbi-compat
+  rmb_ka_stats_options kopt{};
+  kopt.ncbi_compat = ncbi_compat ? 1 : 0;
+  double evalue = 0;
+  bool evalue_given = false;
+  {
+    auto num = [&](const char* flag, double& v, bool nonneg) {
+      if (!kv.count(flag)) return true;
+      const std::string t = kv[flag];
+      char* end = nullptr;
+      v = std::strtod(t.c_str(), &end);
+      if (t.empty() || *end || !std::isfinite(v)) {
+        fprintf(stderr, "error: invalid value '%s' for '%s': not a finite number\n", t.c_str(), flag);
+        return false;
+      }
+      if (nonneg && v < 0) {
+        fprintf(stderr, "error: invalid value '%s' for '%s': '%s' must not be negative\n", t.c_str(), flag, t.c_str());
+        return false;
+      }
+      return true;
+    };
+    if (!num("--matrix-lambda", kopt.matrix_lambda, true) || !num("--matrix-k", kopt.matrix_k, true) ||
+        !num("--matrix-alpha", kopt.matrix_alpha, true) || !num("--matrix-beta", kopt.matrix_beta, false) ||
+        !num("--evalue", evalue, false))
+      return 2;
+    evalue_given = kv.count("--evalue") > 0;
+    if (evalue_given && !(evalue > 0)) {
+      fprintf(stderr, "error: invalid value '%s' for '--evalue': '%s': the expect value must be greater than zero\n",
+              kv["--evalue"].c_str(), kv["--evalue"].c_str());
+      return 2;
     }
   }
   std::string err;
@@ -511,6 +540,41 @@ int main(int argc, char** argv) {
   uint64_t db_letters = 0;
   for (const Record& r : db) db_letters += r.bases.size();
   const std::string& db_name = kv["--db"];
+  // Karlin-Altschul statistics, resolved when they are used: evalue/bitscore
+  // fields or a reap threshold (--evalue, or NCBI's 10 under --ncbi-compat)
+  // (main.rs:612-686)
+  const bool stats_requested =
+      !pairwise && std::any_of(fields.begin(), fields.end(), [](const std::string& f) { return f == "evalue" || f == "bitscore"; });
+  const double reap_threshold = ncbi_compat ? (evalue_given ? evalue : 10.0) : (evalue_given ? evalue : 0.0);
+  rmb_ka_stats ka{};
+  bool have_ka = false, reap_on = false;
+  if (stats_requested || reap_threshold > 0) {
+    char kbuf[256] = {0};
+    if (rmb_ka_stats_resolve(matrix_path.c_str(), P.gap_open, P.gap_extend, &kopt, &ka, kbuf, sizeof kbuf)) {
+      fprintf(stderr, "error: matrix: %s\n", kbuf);
+      return fail("statistics");
+    }
+    have_ka = true;
+    reap_on = reap_threshold > 0 && ka.source != RMB_KA_SENTINEL;
+    static const char* kDescribe[] = {"none (sentinel: evalue=1.0, bitscore=0.0)", "baked RMBlast table (NCBI Mode 1)",
+                                      "-matrix_lambda/k/alpha CLI overrides (NCBI Mode 2)",
+                                      "# KARLIN comment in matrix file"};
+    fprintf(stderr, "# rmblastn: E-value statistics%s: %s\n", ncbi_compat ? " (--ncbi-compat 2.17.1 emulation)" : "",
+            kDescribe[ka.source]);
+    if (ka.source == RMB_KA_TABLE) {
+      const size_t slash = matrix_path.rfind('/');
+      fprintf(stderr, "#   Karlin-Altschul parameters for '%s' at gap costs %d/%d taken from the built-in table by "
+              "matrix file name; the matrix contents are not checked\n",
+              (slash == std::string::npos ? matrix_path : matrix_path.substr(slash + 1)).c_str(), P.gap_open, P.gap_extend);
+    }
+    if (ka.lambda > 0) fprintf(stderr, "#   lambda=%.10f K=%.10f H=%.10f\n", ka.lambda, ka.k, ka.h);
+    const std::string t = reap_threshold > 0 ? (evalue_given ? kv["--evalue"] : std::string("10")) : "";
+    if (reap_on) fprintf(stderr, "#   -evalue cutoff ACTIVE: threshold %s\n", t.c_str());
+    else if (reap_threshold > 0)
+      fprintf(stderr, "#   -evalue cutoff requested (threshold %s) but no statistics available \xe2\x80\x94 nothing culled\n",
+              t.c_str());
+    else fprintf(stderr, "#   -evalue cutoff: off (no --evalue given)\n");
+  }
   std::vector<const char*> qids, sids;
   for (const Record& r : qs) qids.push_back(r.id.c_str());
   for (const Record& r : db) sids.push_back(r.id.c_str());
@@ -520,6 +584,8 @@ int main(int argc, char** argv) {
     for (const std::string& f : fields) fl += (fl.empty() ? "" : " ") + f;
     rmb_report_spec spec{};
     spec.mask_level = mask_level;
+    spec.ka = have_ka ? &ka : nullptr;
+    spec.evalue_max = reap_on ? reap_threshold : 0.0;
     spec.fields = fl.c_str();
     spec.query_ids = qids.data();
     spec.subject_ids = sids.data();
@@ -532,9 +598,13 @@ int main(int argc, char** argv) {
     std::vector<uint64_t> order;
     uint64_t n_rows = 0;
     if (nh) {
-      if (rmb_report_order(ctx, mask_level, nullptr, &n_rows)) return fail("report");
+      rmb_report_spec spec{};
+      spec.mask_level = mask_level;
+      spec.ka = have_ka ? &ka : nullptr;
+      spec.evalue_max = reap_on ? reap_threshold : 0.0;
+      if (rmb_report_order(ctx, &spec, nullptr, &n_rows)) return fail("report");
       order.resize(n_rows);
-      if (rmb_report_order(ctx, mask_level, order.data(), &n_rows)) return fail("report");
+      if (rmb_report_order(ctx, &spec, order.data(), &n_rows)) return fail("report");
     }
     std::vector<std::vector<Row>> per_query(qs.size());
     for (uint64_t k : order) {
