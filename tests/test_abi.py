@@ -74,8 +74,6 @@ def test_missing_library_fails_loudly(tmp_path):
     ("6 score pident qseqid", "unknown -outfmt field 'pident'"),
     ("6 std", "unknown -outfmt field 'std'"),
     ("0 score", "takes no fields"),
-    ("6 score evalue", "needs Karlin-Altschul statistics"),
-    ("6 bitscore qseqid", "needs Karlin-Altschul statistics"),
 ])
 def test_cli_rejects_unsupported_outfmt(fmt, needle):
     """The front end refuses these before it touches a device, naming the
