@@ -14,6 +14,11 @@ resident on the GPU.  Metric: query Mbp x families per second.
 one 10 Mbp record (the split query path) against 300 families of 5-15 kbp,
 copies 25-40% diverged with 5-8% indels, 5% microsatellite tracts, X-drop
 gapped/final 120/400; its CPU baseline sample is the record's first 1.5 Mbp.
+--workload uneven measures the uneven-batching configuration on its first 400
+records (log-uniform 1-500 kbp) against all 5,000 families (log-uniform
+50-20,000 bp).  The kernel schedule is the autotuner's entry for the matrix,
+word size and X-drops when rmblast_amd/tuning.tsv has one (--tuning default
+ignores it).
 
 The CPU baseline is the oracle (a C++ restatement of the reference's search
 path) built for this host (-march=native) at run time, on all the host cores
@@ -50,12 +55,15 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=("scale", "gappy"), default="scale")
-    ap.add_argument("--records", type=int, default=1667, help="records of the 100 Mbp genome to search")
-    ap.add_argument("--families", type=int, default=1000)
-    ap.add_argument("--cpu-records", type=int, default=300,
-                    help="records in the all-cores CPU baseline sample (about 15 s of 16-core work)")
-    ap.add_argument("--cpu1-records", type=int, default=20, help="records in the one-core CPU baseline sample")
+    ap.add_argument("--workload", choices=("scale", "gappy", "uneven"), default="scale")
+    ap.add_argument("--records", type=int, default=None,
+                    help="records to search (scale: of the 1,667; uneven: of the 20,000; default 1,667 / 400)")
+    ap.add_argument("--families", type=int, default=None, help="families (default 1,000 / 5,000)")
+    ap.add_argument("--cpu-records", type=int, default=None,
+                    help="records in the all-cores CPU baseline sample (about 15 s of 16-core work; default 300 / 12)")
+    ap.add_argument("--cpu1-records", type=int, default=None,
+                    help="records in the one-core CPU baseline sample (default 20 / 1)")
+    ap.add_argument("--tuning", default="auto", help="'auto' (the autotuner's table entry, if any) or 'default'")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB three times (the reference "
@@ -103,11 +111,16 @@ def main():
 
     t0 = time.time()
     gappy = a.workload == "gappy"
+    uneven = a.workload == "uneven"
     if gappy:
         w = synth.gappy_config()
+    elif uneven:
+        w = synth.uneven_config(n_records=a.records or 400, n_families=a.families or 5000)
     else:
         w = synth.scale_config()
-        w = synth.subset(w, a.records, a.families)
+        w = synth.subset(w, a.records or 1667, a.families or 1000)
+    cpu_records = a.cpu_records or (12 if uneven else 300)
+    cpu1_records = a.cpu1_records or (1 if uneven else 20)
     gen_s = time.time() - t0
     print(f"[bench] workload generated in {gen_s:.1f}s: {len(w.records)} records x {len(w.families)} families",
           file=sys.stderr, flush=True)
@@ -124,7 +137,9 @@ def main():
     xdrop = [20, 120, 400] if gappy else [20, 30, 100]
     params = dict(gap_open=20, gap_extend=5, word_size=7, xdrop_ungap=xdrop[0], xdrop_gap=xdrop[1],
                   xdrop_gap_final=xdrop[2], min_raw_gapped_score=200, complexity_adjust=True)
-    s = rmblast_amd.Searcher(m, **params)
+    s = rmblast_amd.Searcher(m, **params, matrix_name=mpath, tuning="auto" if a.tuning == "auto" else None)
+    if s.tuning:
+        print(f"[bench] tuning from the autotuner's table: {s.tuning}", file=sys.stderr, flush=True)
     s.load_library([f[1] for f in w.families])
     queries = [r[1] for r in w.records]
     qids = [r[0] for r in w.records]
@@ -215,13 +230,18 @@ def main():
                 sample = [(w.records[0][0], w.records[0][1][:1_500_000])]
                 cpu = cpu_baseline(exe, sample, sample, w.families, tmp, mpath, xdrop, "the record's first 1.5 Mbp")
             else:
-                cpu = cpu_baseline(exe, w.records[:a.cpu_records], w.records[:a.cpu1_records], w.families, tmp,
-                                   mpath, xdrop, f"first {a.cpu_records} records",
-                                   gpu_rows=rows, sample_ids=set(qids[:a.cpu_records]))
-    workload = ("divergent/gappy stress: 1 x 10 Mbp record x 300 families of 5-15 kbp, 25-40% divergence, "
-                "5-8% indels, X-drop 4x, ws 7" if gappy else
-                "exact path at scale: 1,667 x 60 kbp records x 1,000 families, ws 7, search + host tail to "
-                "outfmt-6 rows")
+                cpu = cpu_baseline(exe, w.records[:cpu_records], w.records[:cpu1_records], w.families, tmp,
+                                   mpath, xdrop, f"first {cpu_records} records",
+                                   gpu_rows=rows, sample_ids=set(qids[:cpu_records]))
+    if gappy:
+        workload = ("divergent/gappy stress: 1 x 10 Mbp record x 300 families of 5-15 kbp, 25-40% divergence, "
+                    "5-8% indels, X-drop 4x, ws 7")
+    elif uneven:
+        workload = (f"uneven batching: the first {len(queries)} of 20,000 records of log-uniform 1-500 kbp "
+                    f"x {nfam} families of log-uniform 50-20,000 bp, ws 7, search + host tail to outfmt-6 rows")
+    else:
+        workload = ("exact path at scale: 1,667 x 60 kbp records x 1,000 families, ws 7, search + host tail to "
+                    "outfmt-6 rows")
     line = {
         "metric": "query Mbp x families per second (exact search path, final outfmt-6 rows in memory)",
         "value": round(value, 2),
@@ -246,6 +266,7 @@ def main():
         "prelim_kernel_ms": round(st["t_prelim_kernel_ns"] / 1e6, 3),
         "stage_ms": {k: round(st[k] / 1e3, 2) for k in ("t_seed_us", "t_prelim_us", "t_traceback_us", "t_total_us")},
         "phase1_stage_frac": p1_stages,
+        "tuning": s.tuning,
         "counters": {k: st[k] for k in ("seeds_staged", "seed_evals", "ungapped_hits", "prelim_dps",
                                          "traceback_dps", "dp_cells", "tb_cells", "hsps", "ungapped_steps")},
         "rows": rows.count(b"\n"),

@@ -177,6 +177,31 @@ int rmb_ka_evalue(const rmb_ka_stats* ka, int64_t eff_searchsp, int32_t score, d
 int rmb_ctx_create(int device, const rmb_params* params, const rmb_matrix* matrix, rmb_ctx** out);
 int rmb_ctx_destroy(rmb_ctx* ctx);
 
+/* Schedule of a context's kernels (no counterpart in the reference: the
+ * choices the autotuner in rmblast_amd/autotune.py sweeps and stores per
+ * matrix and word size).  0 in a field keeps the library's default.  Every
+ * setting gives the same results; only the time changes.
+ *  batches          pipeline batches a search is cut into (1..64; default 8)
+ *  prelim_ring      preliminary-DP LDS band ring, 16 or 64 cells (default by
+ *                   X-drop: 64 past 12 band cells); wider bands rerun with
+ *                   global rings either way
+ *  prelim_wg_per_cu preliminary-DP workgroups (128 lanes) per CU (1..16;
+ *                   default 8 with 16-cell rings, 2 with 64)
+ *  tb_ring          traceback LDS band ring, 64 or 256 cells (default by
+ *                   X-drop: 256 past 48 band cells)
+ *  tb_wg_per_cu     traceback workgroups (64 lanes) per CU (1..16, capped at
+ *                   8, or 2 with the 256-cell rings, by the lane arenas'
+ *                   memory; default 7, or 1) */
+typedef struct rmb_tuning {
+  int32_t batches;
+  int32_t prelim_ring;
+  int32_t prelim_wg_per_cu;
+  int32_t tb_ring;
+  int32_t tb_wg_per_cu;
+  int32_t pad;
+} rmb_tuning;
+int rmb_ctx_set_tuning(rmb_ctx* ctx, const rmb_tuning* tuning);
+
 /* Copy of the last error message (NUL-terminated, truncated to len). */
 int rmb_last_error(const rmb_ctx* ctx, char* buf, size_t len);
 

@@ -23,7 +23,7 @@ EXPORTED = [
     "rmb_search", "rmb_search_pairs", "rmb_dust_intervals", "rmb_results_size", "rmb_results_copy", "rmb_stats", "rmb_tap_ungapped",
     "rmb_tap_align_score_only", "rmb_tap_align_ex", "rmb_ka_lookup", "rmb_ungapped_cutoff", "rmb_mask_level",
     "rmb_report_order", "rmb_report_tabular", "rmb_report_copy", "rmb_report_text", "rmb_align_stats_of",
-    "rmb_ka_stats_resolve", "rmb_ka_search_space", "rmb_ka_evalue",
+    "rmb_ka_stats_resolve", "rmb_ka_search_space", "rmb_ka_evalue", "rmb_ctx_set_tuning",
 ]
 
 
@@ -32,6 +32,39 @@ class RmbParams(ctypes.Structure):
         "gap_open", "gap_extend", "word_size", "xdrop_ungap", "xdrop_gap", "xdrop_gap_final",
         "min_raw_gapped_score", "ungapped_cutoff", "complexity_adjust", "dust")] + [
         ("ka_lambda", ctypes.c_double), ("ka_k", ctypes.c_double)]
+
+
+class RmbTuning(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ("batches", "prelim_ring", "prelim_wg_per_cu", "tb_ring",
+                                              "tb_wg_per_cu", "pad")]
+
+
+TUNING_FIELDS = ("batches", "prelim_ring", "prelim_wg_per_cu", "tb_ring", "tb_wg_per_cu")
+TUNING_PATH = os.path.join(_HERE, "tuning.tsv")
+
+
+def tuning_key(matrix_name: str, word_size: int, xdrop_gap: int, xdrop_gap_final: int):
+    return (os.path.basename(matrix_name), int(word_size), int(xdrop_gap), int(xdrop_gap_final))
+
+
+def read_tuning(path: str = TUNING_PATH) -> dict:
+    """The autotuner's choices: {(matrix, ws, xdrop_gap, xdrop_gap_final): {field: value}}."""
+    out = {}
+    if not os.path.exists(path):
+        return out
+    with open(path) as f:
+        head = None
+        for line in f:
+            if line.startswith("#") or not line.strip():
+                continue
+            cols = line.rstrip("\n").split("\t")
+            if head is None:
+                head = cols
+                continue
+            row = dict(zip(head, cols))
+            key = (row["matrix"], int(row["word_size"]), int(row["xdrop_gap"]), int(row["xdrop_gap_final"]))
+            out[key] = {k: int(row[k]) for k in TUNING_FIELDS}
+    return out
 
 
 class RmbKaStatsOptions(ctypes.Structure):
@@ -88,6 +121,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rmb_matrix_load.argtypes = [ctypes.c_char_p, P(RmbMatrix), ctypes.c_char_p, ctypes.c_size_t]
     lib.rmb_ctx_create.argtypes = [ctypes.c_int, P(RmbParams), P(RmbMatrix), P(ctypes.c_void_p)]
     lib.rmb_ctx_destroy.argtypes = [ctypes.c_void_p]
+    lib.rmb_ctx_set_tuning.argtypes = [ctypes.c_void_p, P(RmbTuning)]
     lib.rmb_last_error.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
     lib.rmb_library_load.argtypes = [ctypes.c_void_p, ctypes.c_uint32, P(u8p), P(ctypes.c_uint32)]
     lib.rmb_search.argtypes = [ctypes.c_void_p, ctypes.c_uint32, P(u8p), P(ctypes.c_uint32)]
@@ -229,10 +263,12 @@ class Searcher:
 
     def __init__(self, matrix: RmbMatrix, *, gap_open=20, gap_extend=5, word_size=11, xdrop_ungap=20,
                  xdrop_gap=30, xdrop_gap_final=100, min_raw_gapped_score=0, ungapped_cutoff=-1,
-                 complexity_adjust=False, device=0, matrix_name=None):
+                 complexity_adjust=False, device=0, matrix_name=None, tuning=None):
         """matrix_name: the -matrix path; when it names a matrix of NCBI's
         ALP table at these gap costs, the ungapped cutoff is resolved per
-        query record as rmblastn does (ungapped_cutoff = -1 only)."""
+        query record as rmblastn does (ungapped_cutoff = -1 only).
+        tuning: a dict of rmb_tuning fields, or "auto" for the autotuner's
+        entry for (matrix_name, word_size, X-drops) in tuning.tsv if any."""
         self._lib = lib()
         ka = ka_lookup(matrix_name, gap_open, gap_extend) if matrix_name else None
         p = RmbParams(gap_open, gap_extend, word_size, xdrop_ungap, xdrop_gap, xdrop_gap_final,
@@ -245,6 +281,16 @@ class Searcher:
         self._ctx = ctx
         self._check(rc, "rmb_ctx_create")
         self._keep = []
+        self.tuning = None
+        if tuning == "auto":
+            tuning = read_tuning().get(tuning_key(matrix_name or "", word_size, xdrop_gap, xdrop_gap_final))
+        if tuning:
+            self.set_tuning(tuning)
+
+    def set_tuning(self, tuning: dict):
+        t = RmbTuning(*[int(tuning.get(k, 0)) for k in TUNING_FIELDS], 0)
+        self._check(self._lib.rmb_ctx_set_tuning(self._ctx, ctypes.byref(t)), "rmb_ctx_set_tuning")
+        self.tuning = dict(tuning)
 
     def _check(self, rc, what):
         if rc:

@@ -255,6 +255,7 @@ struct rmb_ctx {
   int32_t* d_mat = nullptr;
   int cu_count = 256;
   void* noalloc_scratch = nullptr;  // this device's copy of the no-alloc DPs' scratch array
+  rmb_tuning tune{};                 // schedule choices (0 fields: the defaults)
   // library
   std::vector<Subject> subj;
   std::vector<SubjDesc> sdesc;
@@ -534,6 +535,7 @@ constexpr uint32_t kOnePassWidePerCu = 1;  // with the 256-cell rings, ~81 KB: o
 // in all the one-pass kernel takes its wider rings (RMB_TB_WIDE=0 keeps the
 // 64-cell rings).
 static bool onepass_wide(const rmb_ctx* c) {
+  if (c->tune.tb_ring) return c->tune.tb_ring == 256;
   if (const char* e = std::getenv("RMB_TB_WIDE"))
     if (std::atoi(e) == 0) return false;
   const int ge = std::max(1, (int)c->P.gap_extend);
@@ -572,8 +574,10 @@ static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::
     DpTask* dt = c->b_tasks.get<DpTask>(n);
     OnePassResult* dr = c->b_opres.get<OnePassResult>(n);
     const bool wide = onepass_wide(c);
-    const uint32_t blocks = std::min<uint32_t>((uint32_t)c->cu_count * (wide ? kOnePassWidePerCu : kOnePassPerCu),
-                                               (n + 63) / 64);
+    // lane arenas are per workgroup: at most 8 (2 with the wide rings) per CU
+    const uint32_t per_cu = c->tune.tb_wg_per_cu ? std::min<uint32_t>((uint32_t)c->tune.tb_wg_per_cu, wide ? 2u : 8u)
+                                                 : (wide ? kOnePassWidePerCu : kOnePassPerCu);
+    const uint32_t blocks = std::min<uint32_t>((uint32_t)c->cu_count * per_cu, (n + 63) / 64);
     const uint32_t flat_cap = wide ? kOnePassWideFlat : kOnePassFlat, rows_cap = wide ? kOnePassWideRows : kOnePassRows,
                    ops_cap = wide ? kOnePassWideOps : kOnePassOps;
     const uint64_t stride = onepass_stride(flat_cap, rows_cap, ops_cap);
@@ -820,9 +824,12 @@ static void enqueue_batch(rmb_ctx* c, P1Batch& b, const uint8_t* d_qfwd, const u
     // past 12 band cells ((X - open) / extend to either side), 64-cell rings
     // with two workgroups per CU (RMB_PRELIM_WIDE=0 keeps the 16-cell rings)
     const char* pw_env = std::getenv("RMB_PRELIM_WIDE");
-    const bool pwide = !(pw_env && std::atoi(pw_env) == 0) &&
-                       2 * std::max(0, (int)c->P.xdrop_gap - (int)c->P.gap_open) / std::max(1, (int)c->P.gap_extend) + 1 > 12;
-    hip_check(launch_prelim_fused(c->cu_count * (pwide ? 2 : 8), true, PP, dh, nullptr, g.hits_cap, drecs,
+    const bool pwide = c->tune.prelim_ring ? c->tune.prelim_ring == 64
+                                           : !(pw_env && std::atoi(pw_env) == 0) &&
+                                                 2 * std::max(0, (int)c->P.xdrop_gap - (int)c->P.gap_open) /
+                                                         std::max(1, (int)c->P.gap_extend) + 1 > 12;
+    const int pre_per_cu = c->tune.prelim_wg_per_cu ? c->tune.prelim_wg_per_cu : (pwide ? 2 : 8);
+    hip_check(launch_prelim_fused(c->cu_count * pre_per_cu, true, PP, dh, nullptr, g.hits_cap, drecs,
                                   (const SubjDesc*)c->d_sdesc.p, d_qfwd, d_qrc, (const uint8_t*)c->d_splus.p,
                                   c->d_mat, nullptr, 0, dpre, cnt, dredo, nullptr, hitcnt + 2 * gi + 1, ctr, err, pwide, st),
               "k_prelim_fused");
@@ -1411,6 +1418,7 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
   for (uint32_t i = 0; i < nsr; ++i) total_bases += recs[i].q_len;
   int max_batches = kMaxBatches;
   if (const char* e = std::getenv("RMB_BATCHES")) max_batches = std::max(1, std::atoi(e));
+  if (c->tune.batches) max_batches = c->tune.batches;
   const double span = std::max<double>((double)kBatchMinBases,
                                        max_batches > 1 ? (double)total_bases / (max_batches - 0.75)
                                                        : (double)total_bases);
@@ -1690,6 +1698,19 @@ int rmb_ctx_destroy(rmb_ctx* c) {
   if (c->st1) (void)hipStreamDestroy(c->st1);
   delete c;
   return RMB_OK;
+}
+
+int rmb_ctx_set_tuning(rmb_ctx* c, const rmb_tuning* t) {
+  if (!c || !t) return RMB_ERR_ARG;
+  return guarded(c, [&]() {
+    auto bad = [](const char* what) { throw Failure(RMB_ERR_ARG, std::string("tuning: ") + what); };
+    if (t->batches < 0 || t->batches > 64) bad("batches must be 0 (default) or 1..64");
+    if (t->prelim_ring != 0 && t->prelim_ring != 16 && t->prelim_ring != 64) bad("prelim_ring must be 0, 16 or 64");
+    if (t->prelim_wg_per_cu < 0 || t->prelim_wg_per_cu > 16) bad("prelim_wg_per_cu must be 0..16");
+    if (t->tb_ring != 0 && t->tb_ring != 64 && t->tb_ring != 256) bad("tb_ring must be 0, 64 or 256");
+    if (t->tb_wg_per_cu < 0 || t->tb_wg_per_cu > 16) bad("tb_wg_per_cu must be 0..16");
+    c->tune = *t;
+  });
 }
 
 int rmb_last_error(const rmb_ctx* c, char* buf, size_t len) {

@@ -498,6 +498,32 @@ int main(int argc, char** argv) {
     return 1;
   };
   if (rc) return fail("device");
+  if (kv.count("--tuning-file")) {
+    // the autotuner's table (rmblast_amd/autotune.py): the row for this
+    // matrix file name, word size and X-drops, if any, sets the schedule
+    std::ifstream f(kv["--tuning-file"]);
+    if (!f) { fprintf(stderr, "error: cannot open tuning file '%s'\n", kv["--tuning-file"].c_str()); return fail("tuning"); }
+    const size_t slash = matrix_path.rfind('/');
+    const std::string base = slash == std::string::npos ? matrix_path : matrix_path.substr(slash + 1);
+    std::string line;
+    std::vector<std::string> head;
+    while (std::getline(f, line)) {
+      if (line.empty() || line[0] == '#') continue;
+      std::vector<std::string> cols;
+      std::stringstream ss(line);
+      for (std::string c; std::getline(ss, c, '\t');) cols.push_back(c);
+      if (head.empty()) { head = cols; continue; }
+      std::map<std::string, std::string> row;
+      for (size_t k = 0; k < head.size() && k < cols.size(); ++k) row[head[k]] = cols[k];
+      auto num = [&](const char* k) { return row.count(k) ? std::atoi(row[k].c_str()) : 0; };
+      if (row["matrix"] != base || num("word_size") != P.word_size || num("xdrop_gap") != P.xdrop_gap ||
+          num("xdrop_gap_final") != P.xdrop_gap_final)
+        continue;
+      rmb_tuning t{num("batches"), num("prelim_ring"), num("prelim_wg_per_cu"), num("tb_ring"), num("tb_wg_per_cu"), 0};
+      if (rmb_ctx_set_tuning(ctx, &t)) return fail("tuning");
+      break;
+    }
+  }
   uint64_t nh = 0, no = 0;
   std::vector<rmb_hsp> H;
   std::vector<rmb_op> O;

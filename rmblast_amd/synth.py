@@ -158,6 +158,33 @@ def gappy_config(seed=4, genome_len=10_000_000, n_families=300, n_copies=4000, f
     return Workload([("chr1", g)], fams, [(0,) + x for x in ins])
 
 
+def uneven_config(seed=3, n_records=20_000, n_families=5_000, rec_range=(1_000, 500_000),
+                  fam_range=(50, 20_000), copies_per_kbp=2.0, frag_range=(50, 1000)):
+    """The uneven-batching configuration: records of log-uniform lengths in
+    rec_range (1 kbp - 500 kbp) against families of log-uniform lengths in
+    fam_range (50 - 20,000 bp); each record carries about copies_per_kbp
+    copies per kbp (the density of the exact-path-at-scale genome), at
+    2-35% divergence with 0.5-3% indels.  Family i and record i each come from
+    their own generator, so the first k records (or families) of a smaller
+    configuration are those of the full one: a subset is generated without the
+    whole (20,000 records are about 1.6 Gbp)."""
+    def log_uniform(rng, lo, hi):
+        return int(round(float(np.exp(rng.uniform(np.log(lo), np.log(hi))))))
+    fams = []
+    for i in range(n_families):
+        rng = np.random.default_rng([seed, 1, i])
+        fams.append((f"fam{i:05d}", random_bases(rng, log_uniform(rng, *fam_range))))
+    recs, inserts = [], []
+    for r in range(n_records):
+        rng = np.random.default_rng([seed, 2, r])
+        n = log_uniform(rng, *rec_range)
+        g, ins = insert_copies(rng, n, fams, int(rng.poisson(copies_per_kbp * n / 1000.0)), frag_range,
+                               (0.02, 0.35), (0.005, 0.03))
+        recs.append((f"rec{r:05d}", g))
+        inserts += [(r,) + x for x in ins]
+    return Workload(recs, fams, inserts)
+
+
 def subset(w: Workload, n_records: int, n_families: int | None = None) -> Workload:
     fams = w.families if n_families is None else w.families[:n_families]
     return Workload(w.records[:n_records], fams, [x for x in w.inserts if x[0] < n_records])

@@ -94,3 +94,39 @@ def test_chr22_ungapped_stage_golden_on_gpu():
     first = next((i for i, (x, y) in enumerate(zip(got, want)) if x != y), None)
     assert first is None, (first, got[first], want[first])
     assert len(got) == len(want)
+
+
+def test_uneven_batching_subset_matches_oracle_under_every_schedule(tmp_path):
+    """300 of the uneven configuration's records (1-500 kbp) against 200 of
+    its families (50-20,000 bp), with the default schedule and two others
+    from the autotuner's space: the same rows as the oracle each time."""
+    w = synth.uneven_config(n_records=300, n_families=200)
+    synth.to_fasta(w.records, tmp_path / "q.fa")
+    synth.to_fasta(w.families, tmp_path / "db.fa")
+    args = synth_args(tmp_path)
+    o = run([ORACLE_BIN] + args + ["--num-threads", "16"], timeout=900)
+    assert o.returncode == 0, o.stderr
+    assert o.stdout.count("\n") > 10000
+    head = "matrix\tword_size\txdrop_gap\txdrop_gap_final\tbatches\tprelim_ring\tprelim_wg_per_cu\ttb_ring\ttb_wg_per_cu\n"
+    for row in (None, "m.matrix\t7\t30\t100\t3\t64\t2\t256\t2\n", "m.matrix\t7\t30\t100\t12\t16\t4\t64\t4\n"):
+        extra = []
+        if row:
+            (tmp_path / "t.tsv").write_text(head + row)
+            extra = ["--tuning-file", str(tmp_path / "t.tsv")]
+        g = run([CLI] + args + extra, timeout=900)
+        assert g.returncode == 0, g.stderr
+        assert g.stdout == o.stdout, row
+
+
+def test_autotuner_keeps_results_and_writes_its_table(tmp_path):
+    from rmblast_amd import autotune
+    w = synth.uneven_config(n_records=24, n_families=150)
+    (tmp_path / "m.matrix").write_text(synth.MATRIX_TEXT)
+    s = rmblast_amd.Searcher(rmblast_amd.load_matrix(str(tmp_path / "m.matrix")), gap_open=20, gap_extend=5,
+                             word_size=7, min_raw_gapped_score=200, complexity_adjust=True)
+    s.load_library([f[1] for f in w.families])
+    best, t, trials = autotune.tune(s, [r[1] for r in w.records], reps=1, log=lambda m: None)
+    assert len(trials) >= 10 and t == min(x[1] for x in trials)
+    path = tmp_path / "tuning.tsv"
+    autotune.write_entry(str(path), ("m.matrix", 7, 30, 100), best, t * 1e3, "test")
+    assert rmblast_amd.read_tuning(str(path))[("m.matrix", 7, 30, 100)] == {k: best[k] for k in best}
