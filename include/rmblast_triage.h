@@ -7,9 +7,12 @@
  * (2080 -> 512 -> 256 per tower, dot-plus-two-layer head: 256 -> 64 -> 1, plus
  * a learned weight on the cosine of the raw profiles),
  * BF16 weights and GEMM operands with FP32 accumulation, trained on the GPU
- * by the library's own forward and backward kernels.  Pairs it passes go
- * through the exact search (rmb_search_pairs in rmblast_hip.h), so every HSP
- * it reports is one the exact path computes.
+ * by the library's own forward and backward kernels.  The caller prunes
+ * whole (query record, family) pairs with its scores (a pair passes when one
+ * of the record's windows does) and searches the passed pairs with
+ * rmb_search_pairs (rmblast_hip.h), whose search of a pair is the plain
+ * search's for that pair: every reported HSP is one the plain search of those
+ * records reports, and pruning can only drop HSPs.
  *
  * Conventions as in rmblast_hip.h: int status (0 ok, 1 argument, 2 device,
  * 3 state), message from rmb_triage_last_error, nothing thrown across the

@@ -18,7 +18,7 @@
 namespace rmb {
 namespace tri {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // FP32 -> BF16, round to nearest even (NaN kept a NaN).
@@ -59,21 +59,21 @@ __global__ __launch_bounds__(256) void k_kmer_profile(const uint8_t* __restrict_
 }
 
 // --------------------------------------------------------------- GEMM ------
-// C[M x N] (+)= op(A)[M x K] * op(B)[K x N] with v_mfma_f32_32x32x8_bf16.
+// C[M x N] (+)= op(A)[M x K] * op(B)[K x N] with gfx950's v_mfma_f32_32x32x16_bf16.
 //   A: TA ? A[k * lda + m] : A[m * lda + k]
 //   B: TB ? B[n * ldb + k] : B[k * ldb + n]      (TB: weights stored [out][in])
 // 64 x 64 tile per workgroup, one 32 x 32 MFMA tile per wave, K in steps of 32
 // staged through LDS (zero-filled past the edges).  Optional BF16 copy of C.
-// Operand layout of the 32x32x8 MFMA: lane l holds A[l % 32][4 (l / 32) + j]
-// and B[4 (l / 32) + j][l % 32], j = 0..3; result register v of lane l is
-// C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].
+// Operand layout of the 32x32x16 MFMA: lane l holds A[l % 32][8 (l / 32) + j]
+// and B[8 (l / 32) + j][l % 32], j = 0..7 (one 16-byte LDS read each); result
+// register v of lane l is C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const uint16_t* __restrict__ A, int lda,
                                                    const uint16_t* __restrict__ B, int ldb, float* __restrict__ C,
                                                    int ldc, int accumulate, uint16_t* __restrict__ Cbf) {
-  constexpr int BM = 64, BN = 64, BK = 32, PAD = 4;
-  __shared__ uint16_t As[BM][BK + PAD];
-  __shared__ uint16_t Bs[BN][BK + PAD];
+  constexpr int BM = 64, BN = 64, BK = 32, PAD = 8;  // 80-byte rows keep the 16-byte reads aligned
+  __shared__ __align__(16) uint16_t As[BM][BK + PAD];
+  __shared__ __align__(16) uint16_t Bs[BN][BK + PAD];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
@@ -99,14 +99,11 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const ui
     }
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < BK; kk += 8) {
-      const int kq = kk + 4 * (lane >> 5);
-      const uint16_t* ap = &As[wm + (lane & 31)][kq];
-      const uint16_t* bp = &Bs[wn + (lane & 31)][kq];
-      bf16x4 a, b;
-      a[0] = (short)ap[0]; a[1] = (short)ap[1]; a[2] = (short)ap[2]; a[3] = (short)ap[3];
-      b[0] = (short)bp[0]; b[1] = (short)bp[1]; b[2] = (short)bp[2]; b[3] = (short)bp[3];
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    for (int kk = 0; kk < BK; kk += 16) {
+      const int kq = kk + 8 * (lane >> 5);
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&As[wm + (lane & 31)][kq]);
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[wn + (lane & 31)][kq]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
     }
     __syncthreads();
   }

@@ -2,10 +2,14 @@
 plumbing around it (windows, labels from the exact search, training loop,
 pruned search and its evaluation against the full exact search).
 
-The triage prunes (query window, family) pairs before seeding; the pairs it
-passes are searched by the exact path (Searcher.search_pairs), so every HSP
-it reports is computed by the same kernels as the full search.  Windows are
-2 kbp at a 1 kbp stride; features are log1p canonical 6-mer counts.
+The genome is searched as fixed records (RECORD bp); the triage scores every
+window of a record against every family and prunes whole (record, family)
+pairs, a pair passing when one of its windows does.  The pairs it passes are
+searched by the exact path (Searcher.search_pairs), whose search of a pair is
+the plain search's for that pair, so every row it reports is a row of the
+plain search of those records (precision 1) and pruning can only lose rows.
+Windows are 2 kbp at a 1 kbp stride inside each record; features are log1p
+canonical 6-mer counts.
 """
 from __future__ import annotations
 
@@ -171,32 +175,70 @@ def labels_from_hsps(hsps, starts, lens, n_families, query=0, min_overlap=0):
     return lab
 
 
+RECORD = 4_000  # the genome is searched as records of this many bp
+
+
+def records_of(length: int, size: int = RECORD):
+    """[(begin, end)] of the records a genome of `length` bp is searched as:
+    consecutive, non-overlapping, `size` bp each (the last may be shorter)."""
+    return [(b, min(length, b + size)) for b in range(0, max(length, 1), size)]
+
+
+def record_windows(recs):
+    """Windows of every record (record-local windows shifted to genome
+    coordinates, so no window crosses a record end): (starts, lens, record of
+    each window).  Windows are grouped by record, in record order."""
+    st, ln, wr = [], [], []
+    for r, (b, e) in enumerate(recs):
+        s, l = windows(e - b)
+        st.append(s + np.uint64(b))
+        ln.append(l)
+        wr.append(np.full(len(s), r, dtype=np.uint32))
+    return np.concatenate(st), np.concatenate(ln), np.concatenate(wr)
+
+
 @dataclass
 class Instance:
-    seq: np.ndarray            # the query record (one C1-style genome)
+    seq: np.ndarray            # one C1-style genome
     families: list             # family sequences
-    starts: np.ndarray
+    recs: list                 # [(begin, end)]: the records the genome is searched as
+    starts: np.ndarray         # windows (genome coordinates), grouped by record
     lens: np.ndarray
-    hsps: np.ndarray           # full exact-search HSPs
+    win_rec: np.ndarray        # record of each window
+    hsps: np.ndarray           # the plain exact search of every (record, family) pair
     ops: np.ndarray
     labels: np.ndarray         # windows x families: 1 if the window overlaps an HSP of the family
+    pair_labels: np.ndarray    # records x families: 1 if the pair has an HSP
     xw: np.ndarray = None      # BF16 profiles
     xf: np.ndarray = None
     train_labels: np.ndarray = None  # as labels, marginal overlaps marked 2 (ignored)
 
+    def record_seqs(self):
+        return [self.seq[b:e] for b, e in self.recs]
 
-def segments_for(passed_windows, starts, lens, q_len, margin):
-    """Merge the passed windows of one family into query segments, each
-    widened by `margin` on both sides; returns [(begin, end)]."""
-    iv = sorted((max(0, int(starts[i]) - margin), min(q_len, int(starts[i]) + int(lens[i]) + margin))
-                for i in passed_windows)
-    out = []
-    for b, e in iv:
-        if out and b <= out[-1][1]:
-            out[-1] = (out[-1][0], max(out[-1][1], e))
-        else:
-            out.append((b, e))
-    return out
+
+def window_labels(hsps, recs, starts, lens, win_rec, n_families, min_overlap=0):
+    """labels_from_hsps over records: an HSP of record r (record-local
+    coordinates) labels the windows of record r it overlaps."""
+    lab = np.zeros((len(starts), n_families), dtype=np.uint8)
+    first = np.searchsorted(win_rec, np.arange(len(recs) + 1))
+    if not len(hsps):
+        return lab
+    order = np.argsort(hsps["query"], kind="stable")
+    hq = hsps["query"][order]
+    for r in np.unique(hq):
+        a, b = int(first[r]), int(first[r + 1])
+        mine = hsps[order[np.searchsorted(hq, r):np.searchsorted(hq, r, side="right")]]
+        lab[a:b] = labels_from_hsps(mine, starts[a:b] - np.uint64(recs[r][0]), lens[a:b], n_families, query=r,
+                                    min_overlap=min_overlap)
+    return lab
+
+
+def pair_scores(scores: np.ndarray, win_rec: np.ndarray, n_records: int) -> np.ndarray:
+    """records x families: a pair's score is the best score among its
+    record's windows (windows are grouped by record)."""
+    first = np.searchsorted(win_rec, np.arange(n_records))
+    return np.maximum.reduceat(scores, first, axis=0)
 
 
 def hsp_key(h, ops, q_shift=0):
@@ -205,55 +247,24 @@ def hsp_key(h, ops, q_shift=0):
             int(h["s_start"]), int(h["s_end"]), int(h["strand"]), o.tobytes())
 
 
-CHUNK, CHUNK_OVERLAP = 16_000, 4_000  # segment pieces searched as separate records
+def row_keys(hsps, ops):
+    """One key per HSP: (record,) + hsp_key."""
+    return {(int(h["query"]),) + hsp_key(h, ops) for h in hsps}
 
 
-def chunks_of(b, e, chunk=CHUNK, overlap=CHUNK_OVERLAP):
-    """Cut [b, e) into pieces of at most `chunk` bp overlapping by `overlap`
-    bp, so every HSP shorter than the overlap lies whole in some piece."""
-    if e - b <= chunk:
-        return [(b, e)]
-    out, s = [], b
-    while True:
-        out.append((s, min(e, s + chunk)))
-        if s + chunk >= e:
-            return out
-        s += chunk - overlap
-
-
-def pruned_search(searcher, inst: Instance, passed: np.ndarray, margin: int):
-    """Exact search of the segments the passed (window, family) pairs select,
-    each cut into overlapping pieces searched as separate records; HSPs are
-    mapped back to record coordinates and deduplicated.  Returns (set of HSP
-    keys, number of piece-family pairs searched, seconds)."""
+def pruned_search(searcher, inst: Instance, passed: np.ndarray):
+    """Exact search of the passed (record, family) pairs through
+    Searcher.search_pairs.  A pair's search is the plain search's for that
+    pair (HSPs of one record against one family depend on nothing else), so
+    every row is a row of the plain search; a pruned pair loses its rows.
+    Returns (set of row keys, pairs searched, seconds)."""
     t0 = time.perf_counter()
-    q_len = len(inst.seq)
-    seg_seqs, seg_off, pair_q, pair_s = [], [], [], []
-    seg_index = {}
-    for j in range(passed.shape[1]):
-        idx = np.nonzero(passed[:, j])[0]
-        if not len(idx):
-            continue
-        for sb, se in segments_for(idx, inst.starts, inst.lens, q_len, margin):
-            for b, e in chunks_of(sb, se):
-                if (b, e) not in seg_index:
-                    seg_index[(b, e)] = len(seg_seqs)
-                    seg_seqs.append(inst.seq[b:e])
-                    seg_off.append(b)
-                pair_q.append(seg_index[(b, e)])
-                pair_s.append(j)
+    pq, ps = np.nonzero(passed)
     keys = set()
-    if pair_q:
-        hsps, ops = searcher.search_pairs(seg_seqs, pair_q, pair_s)
-        for h in hsps:
-            k = int(h["query"])
-            b, n = seg_off[k], len(seg_seqs[k])
-            # an HSP reaching a cut end of its piece is truncated there; its
-            # whole copy lies inside the overlapping neighbour piece
-            if (b > 0 and int(h["q_start"]) == 0) or (b + n < q_len and int(h["q_end"]) == n):
-                continue
-            keys.add(hsp_key(h, ops, b))
-    return keys, len(pair_q), time.perf_counter() - t0
+    if len(pq):
+        hsps, ops = searcher.search_pairs(inst.record_seqs(), pq.astype(np.uint32), ps.astype(np.uint32))
+        keys = row_keys(hsps, ops)
+    return keys, int(len(pq)), time.perf_counter() - t0
 
 
 # ------------------------------------------------- training and evaluation --
@@ -280,32 +291,35 @@ def family_profiles(model, families):
     return model.profiles(cat, starts, lens)
 
 
-def make_instance(searcher, model, seed: int) -> Instance:
-    """One parity-configuration genome (1 Mbp, 50 families, 2,000 copies):
-    full exact search, windows, labels and profiles."""
+def make_instance(searcher, model, seed: int, record: int = RECORD, genome=None) -> Instance:
+    """One parity-configuration genome (1 Mbp, 50 families, 2,000 copies) as
+    `record`-bp records: the plain exact search of every (record, family)
+    pair, windows, labels and profiles."""
     from . import synth
-    w = synth.parity_config(seed=seed)
+    w = genome if genome is not None else synth.parity_config(seed=seed)
     q = w.records[0][1]
     fams = [f[1] for f in w.families]
     searcher.load_library(fams)
-    hsps, ops = searcher.search([q])
-    st, ln = windows(len(q))
-    lab = labels_from_hsps(hsps, st, ln, len(fams))
-    return Instance(seq=q, families=fams, starts=st, lens=ln, hsps=hsps, ops=ops, labels=lab,
-                    xw=model.profiles(q, st, ln), xf=family_profiles(model, fams),
-                    train_labels=labels_from_hsps(hsps, st, ln, len(fams), min_overlap=TRAIN_MIN_OVERLAP))
+    recs = records_of(len(q), record)
+    st, ln, wr = record_windows(recs)
+    hsps, ops = searcher.search([q[b:e] for b, e in recs])
+    lab = window_labels(hsps, recs, st, ln, wr, len(fams))
+    pl = np.zeros((len(recs), len(fams)), dtype=np.uint8)
+    if len(hsps):
+        pl[hsps["query"].astype(np.int64), hsps["subject"].astype(np.int64)] = 1
+    return Instance(seq=q, families=fams, recs=recs, starts=st, lens=ln, win_rec=wr, hsps=hsps, ops=ops,
+                    labels=lab, pair_labels=pl, xw=model.profiles(q, st, ln) if model else None,
+                    xf=family_profiles(model, fams) if model else None,
+                    train_labels=window_labels(hsps, recs, st, ln, wr, len(fams), min_overlap=TRAIN_MIN_OVERLAP))
 
 
-def hsp_window_scores(inst: Instance, scores: np.ndarray) -> np.ndarray:
-    """Per full-search HSP, the best score among the windows it overlaps (for
-    its family): the HSP is searched iff that score passes the threshold."""
-    s = inst.starts.astype(np.int64)
-    e = s + inst.lens.astype(np.int64)
-    out = np.empty(len(inst.hsps), dtype=np.float64)
-    for k, h in enumerate(inst.hsps):
-        hit = (s < int(h["q_end"])) & (e > int(h["q_start"]))
-        out[k] = scores[hit, int(h["subject"])].max() if hit.any() else -np.inf
-    return out
+def hsp_pair_scores(inst: Instance, scores: np.ndarray) -> np.ndarray:
+    """Per plain-search HSP, the score of its (record, family) pair: the HSP
+    is reported iff that score passes the threshold."""
+    ps = pair_scores(scores, inst.win_rec, len(inst.recs))
+    if not len(inst.hsps):
+        return np.empty(0)
+    return ps[inst.hsps["query"].astype(np.int64), inst.hsps["subject"].astype(np.int64)].astype(np.float64)
 
 
 def _from_bf16(bits):
@@ -416,53 +430,80 @@ def train(model, instances, epochs, lr, pos_weight, seed=0, log=None, val=None):
     return losses, val_losses
 
 
-def evaluate(searcher, inst: Instance, passed: np.ndarray, margin: int, repeats: int = 3):
-    """Pruned search of `inst` with the given passed (window, family) pairs,
-    against the full exact search: HSP recall, score-weighted recall,
-    precision, searched pairs, and the timed search seconds (best of
-    `repeats`)."""
+def evaluate(searcher, inst: Instance, passed: np.ndarray, repeats: int = 3):
+    """Pruned search of `inst` with the given passed (record, family) pairs,
+    against the plain exact search of every pair: row recall, score-weighted
+    recall, precision (rows the plain search does not report lower it),
+    pairs searched and the timed search seconds (best of `repeats`)."""
     searcher.load_library(inst.families)
-    full_keys = {hsp_key(h, inst.ops) for h in inst.hsps}
-    best = None
+    full_keys = row_keys(inst.hsps, inst.ops)
+    best, keys, n_pairs = None, set(), 0
     for _ in range(repeats):
-        keys, n_pairs, secs = pruned_search(searcher, inst, passed, margin)
+        keys, n_pairs, secs = pruned_search(searcher, inst, passed)
         best = secs if best is None else min(best, secs)
     hit = full_keys & keys
-    score = {k: k[1] for k in full_keys}
+    score = {k: k[2] for k in full_keys}
     return {
         "hsp_recall": len(hit) / max(1, len(full_keys)),
         "score_weighted_recall": sum(score[k] for k in hit) / max(1, sum(score.values())),
         "precision": len(hit) / max(1, len(keys)),
+        "rows_not_in_plain_search": len(keys - full_keys),
         "passed_pair_fraction": float(passed.mean()),
-        "segment_pairs_searched": n_pairs,
+        "pairs_searched": n_pairs,
         "search_seconds": best,
     }
 
 
-def full_search_seconds(searcher, inst: Instance, repeats: int = 3):
+def plain_search_seconds(searcher, inst: Instance, repeats: int = 3, whole: bool = False):
+    """Best-of-`repeats` seconds of the plain exact search of every pair
+    (with `whole`, of the genome as one record instead)."""
     searcher.load_library(inst.families)
+    qs = [inst.seq] if whole else inst.record_seqs()
     best = None
     for _ in range(repeats):
         t = time.perf_counter()
-        searcher.search([inst.seq])
+        searcher.search(qs)
         secs = time.perf_counter() - t
         best = secs if best is None else min(best, secs)
     return best
 
 
+def record_sweep(searcher, seed: int, sizes=(2_000, 4_000, 8_000, 16_000, 60_000), log=print):
+    """Per record size, on one genome: the plain search's time and rows, and
+    a perfect triage's (the true pair labels) pair fraction and time, i.e.
+    the speedup bound of pruning at (record, family) granularity."""
+    from . import synth
+    g = synth.parity_config(seed=seed)
+    out = {}
+    for size in sizes:
+        inst = make_instance(searcher, None, seed, size, genome=g)
+        t_plain = plain_search_seconds(searcher, inst)
+        ev = evaluate(searcher, inst, inst.pair_labels > 0)
+        out[str(size)] = {"records": len(inst.recs), "rows": int(len(inst.hsps)), "plain_search_seconds": t_plain,
+                          "true_pair_fraction": float(inst.pair_labels.mean()),
+                          "perfect_triage_seconds": ev["search_seconds"], "perfect_recall": ev["hsp_recall"],
+                          "perfect_precision": ev["precision"], "bound": t_plain / ev["search_seconds"]}
+        log(f"record {size} bp: {out[str(size)]}")
+    return out
+
+
 def run_experiment(n_train=40, n_val=4, eval_seed=100_000, epochs=30, lr=1e-3, pos_weight=4.0, target=0.995,
-                   margin=2000, model_seed=1, log=print):
+                   record=RECORD, model_seed=1, sweep=True, log=print):
     """Train on parity-configuration genomes of seeds 1..n_train (labels from
-    the full exact search), pick the threshold on n_val validation genomes so
-    that `target` of their HSPs pass, and evaluate on the held-out genome
-    `eval_seed`: the learned scores and, for comparison, the same procedure on
-    the cosine similarity of the profiles (the method without the network)."""
+    the plain exact search of `record`-bp records), pick the threshold on
+    n_val validation genomes so that `target` of their rows pass, and
+    evaluate on the held-out genome `eval_seed`: the learned scores and, for
+    comparison, the same procedure on the cosine similarity of the profiles
+    (the method without the network)."""
     searcher = make_searcher()
     model = Triage(seed=model_seed)
+    report = {"record_bp": record}
+    if sweep:
+        report["record_sweep"] = record_sweep(searcher, eval_seed, log=log)
     t0 = time.perf_counter()
-    train_set = [make_instance(searcher, model, 1 + k) for k in range(n_train)]
-    val_set = [make_instance(searcher, model, 10_000 + k) for k in range(n_val)]
-    held = make_instance(searcher, model, eval_seed)
+    train_set = [make_instance(searcher, model, 1 + k, record) for k in range(n_train)]
+    val_set = [make_instance(searcher, model, 10_000 + k, record) for k in range(n_val)]
+    held = make_instance(searcher, model, eval_seed, record)
     log(f"instances built in {time.perf_counter() - t0:.1f}s")
     n_pairs = sum(i.labels.size for i in train_set)
     pos_rate = float(np.mean([i.labels.mean() for i in train_set]))
@@ -484,46 +525,44 @@ def run_experiment(n_train=40, n_val=4, eval_seed=100_000, epochs=30, lr=1e-3, p
                              for v, l in zip(val_set, val_lab)]))
     log(f"validation pair loss: network {net_val:.4f}, cosine-only logistic {cos_val:.4f}")
 
-    report = {"train_pairs": n_pairs, "train_positive_rate": pos_rate, "epochs": epochs, "train_seconds": train_s,
-              "loss_first_epoch": losses[0], "loss_last_epoch": losses[-1],
-              "validation_loss_by_epoch": val_losses,
-              "validation_pair_loss": {"network": net_val, "cosine_only_logistic": cos_val},
-              "best_epoch": int(np.argmin(val_losses)) + 1 if val_losses else epochs, "target_hsp_recall": target,
-              "margin_bp": margin, "held_out_seed": eval_seed, "held_out_hsps": int(len(held.hsps))}
-    t_full = full_search_seconds(searcher, held)
-    report["full_search_seconds"] = t_full
-    # the same chunked exact search with every pair passed: what chunking alone gives
-    every = evaluate(searcher, held, np.ones_like(held.labels, dtype=bool), margin)
-    every["speedup_vs_full_search"] = t_full / every["search_seconds"]
-    report["all_pairs_chunked"] = every
-    log(f"all pairs, chunked: {every}")
-    # true labels (a perfect triage) as the upper bound of pruning
-    perfect = evaluate(searcher, held, held.labels > 0, margin)
-    perfect["speedup_vs_full_search"] = t_full / perfect["search_seconds"]
+    report.update({"train_window_pairs": n_pairs, "train_positive_rate": pos_rate, "epochs": epochs,
+                   "train_seconds": train_s, "loss_first_epoch": losses[0], "loss_last_epoch": losses[-1],
+                   "validation_loss_by_epoch": val_losses,
+                   "validation_pair_loss": {"network": net_val, "cosine_only_logistic": cos_val},
+                   "best_epoch": int(np.argmin(val_losses)) + 1 if val_losses else epochs,
+                   "target_hsp_recall": target, "held_out_seed": eval_seed, "held_out_rows": int(len(held.hsps)),
+                   "held_out_records": len(held.recs)})
+    t_plain = plain_search_seconds(searcher, held)
+    report["plain_search_seconds"] = t_plain
+    report["one_record_search_seconds"] = plain_search_seconds(searcher, held, whole=True)
+    # true pair labels (a perfect triage): the bound of pruning at this granularity
+    perfect = evaluate(searcher, held, held.pair_labels > 0)
+    perfect["speedup_vs_plain_search"] = t_plain / perfect["search_seconds"]
     report["true_labels"] = perfect
-    log(f"true labels: {perfect}")
+    log(f"plain search {t_plain * 1e3:.1f} ms; true labels: {perfect}")
     targets = sorted(set([0.9, 0.95, 0.99, target]))
     for name, fn in (("learned", learned), ("cosine_profile", cosine_scores)):
-        val_scores = [hsp_window_scores(v, fn(v)) for v in val_set]
+        val_scores = [hsp_pair_scores(v, fn(v)) for v in val_set]
         # the whole triage path on the held-out genome: profiles, scores, pruned search
         t = time.perf_counter()
         xw = model.profiles(held.seq, held.starts, held.lens)
         xf = family_profiles(model, held.families)
         t_feat = time.perf_counter() - t
         t = time.perf_counter()
-        scores = fn(Instance(held.seq, held.families, held.starts, held.lens, held.hsps, held.ops, held.labels, xw,
-                             xf))
+        probe = Instance(held.seq, held.families, held.recs, held.starts, held.lens, held.win_rec, held.hsps,
+                         held.ops, held.labels, held.pair_labels, xw, xf)
+        scores = pair_scores(fn(probe), held.win_rec, len(held.recs))
         t_score = time.perf_counter() - t
         curve = {}
         for tg in targets:
             thr = threshold_for_recall(val_scores, tg)
             passed = scores >= thr
-            ev = evaluate(searcher, held, passed, margin)
+            ev = evaluate(searcher, held, passed)
             total = t_feat + t_score + ev["search_seconds"]
             ev.update({"threshold": thr, "profile_seconds": t_feat, "score_seconds": t_score,
-                       "triage_path_seconds": total, "speedup_vs_full_search": t_full / total,
-                       "speedup_vs_all_pairs_chunked": every["search_seconds"] / total,
-                       "pair_recall": float((passed & (held.labels > 0)).sum() / max(1, held.labels.sum()))})
+                       "triage_path_seconds": total, "speedup_vs_plain_search": t_plain / total,
+                       "pair_recall": float((passed & (held.pair_labels > 0)).sum()
+                                            / max(1, held.pair_labels.sum()))})
             curve[str(tg)] = ev
             log(f"{name} @ validation HSP recall {tg}: {ev}")
         report[name] = curve
@@ -541,12 +580,13 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--pos-weight", type=float, default=4.0)
     ap.add_argument("--target", type=float, default=0.995)
-    ap.add_argument("--margin", type=int, default=2000)
+    ap.add_argument("--record", type=int, default=RECORD)
+    ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--out", default="")
     a = ap.parse_args(argv)
     import sys
     rep = run_experiment(a.train, a.val, epochs=a.epochs, lr=a.lr, pos_weight=a.pos_weight, target=a.target,
-                         margin=a.margin, log=lambda s: print("[triage]", s, file=sys.stderr, flush=True))
+                         record=a.record, sweep=not a.no_sweep, log=lambda s: print("[triage]", s, file=sys.stderr, flush=True))
     line = json.dumps(rep)
     print(line)
     if a.out:

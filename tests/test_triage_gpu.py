@@ -137,37 +137,69 @@ def test_search_pairs_equals_full_search_on_those_pairs(small_instance):
     assert {(int(h["query"]),) + hsp_key(h, aops) for h in allp} == full_keys
 
 
-def test_pruned_search_with_true_labels_recovers_the_full_search(small_instance):
-    from rmblast_amd.triage import Instance, labels_from_hsps, pruned_search, windows, hsp_key
+def test_pruned_search_rows_are_plain_search_rows(small_instance):
+    """Pruning whole (record, family) pairs: with the true pair labels the
+    rows are the plain search's, and with any subset of pairs they are
+    exactly the plain search's rows of those pairs (nothing added)."""
+    from rmblast_amd.triage import make_instance, pruned_search, row_keys
     s, w = small_instance
-    q = w.records[0][1]
-    full, ops = s.search([q])
-    st, ln = windows(len(q))
-    lab = labels_from_hsps(full, st, ln, len(w.families))
-    inst = Instance(seq=q, families=[f[1] for f in w.families], starts=st, lens=ln, hsps=full, ops=ops,
-                    labels=lab)
-    keys, n_pairs, _ = pruned_search(s, inst, lab.astype(bool), margin=2000)
-    full_keys = {hsp_key(h, ops) for h in full}
-    recall = len(full_keys & keys) / max(1, len(full_keys))
-    assert recall >= 0.99, recall
-    assert n_pairs > 0
+    inst = make_instance(s, None, 0, record=4000, genome=w)
+    assert len(inst.recs) == 30 and len(inst.hsps) > 50
+    full = row_keys(inst.hsps, inst.ops)
+    keys, n, _ = pruned_search(s, inst, inst.pair_labels > 0)
+    assert keys == full and n == int(inst.pair_labels.sum())
+    rng = np.random.default_rng(4)
+    for frac in (0.5, 0.1):
+        passed = rng.random(inst.pair_labels.shape) < frac
+        keys, _, _ = pruned_search(s, inst, passed)
+        assert keys == {k for k in full if passed[k[0], k[1]]}
+    keys, n, _ = pruned_search(s, inst, np.zeros_like(passed))
+    assert keys == set() and n == 0
 
 
-def test_trained_network_gains_over_cosine_only():
-    """The method without the network (a logistic fit on the cosine of the
-    profiles) against the trained network warm-started from it: on held-out
-    genomes the network's pair loss is lower."""
+@pytest.fixture(scope="module")
+def trained():
+    """The network trained on 8 parity-configuration genomes (4 kbp records)
+    and warm-started from the cosine logistic; 2 validation genomes and the
+    held-out genome (seed 100000, 1 Mbp x 50 families x 2,000 copies)."""
     from rmblast_amd import triage as tr
     searcher = tr.make_searcher()
     model = tr.Triage(seed=5)
     train_set = [tr.make_instance(searcher, model, 500 + k) for k in range(8)]
     val_set = [tr.make_instance(searcher, model, 600 + k) for k in range(2)]
+    held = tr.make_instance(searcher, model, 100_000)
     th = tr.fit_cosine_logistic(train_set, 4.0)
     tr.warm_start_from_cosine(model, th)
     tr.train(model, train_set, epochs=10, lr=1e-3, pos_weight=4.0, val=val_set)
+    yield searcher, model, train_set, val_set, held, th
+    model.close()
+
+
+def test_trained_network_gains_over_cosine_only(trained):
+    """The method without the network (a logistic fit on the cosine of the
+    profiles) against the trained network warm-started from it: on held-out
+    genomes the network's pair loss is lower."""
+    from rmblast_amd import triage as tr
+    _, model, _, val_set, _, th = trained
     cos_val = np.mean([tr._weighted_logistic_loss(th[0] * tr.cosine_scores(v) + th[1], v.train_labels, 4.0)
                        for v in val_set])
     net_val = np.mean([tr._weighted_logistic_loss(model.logits(v.xw, v.xf).astype(np.float64), v.train_labels, 4.0)
                        for v in val_set])
-    model.close()
     assert net_val < cos_val, (net_val, cos_val)
+
+
+def test_triage_rows_on_held_out_genome_are_exact_with_recall(trained):
+    """The held-out genome through the whole triage path (profiles, logits,
+    pair scores, pruned search) at a threshold set on the validation genomes:
+    every row is a row of the plain search (precision 1) and at least 99.5%
+    of the plain search's rows are kept."""
+    from rmblast_amd import triage as tr
+    searcher, model, _, val_set, held, _ = trained
+    val_scores = [tr.hsp_pair_scores(v, model.logits(v.xw, v.xf)) for v in val_set]
+    thr = tr.threshold_for_recall(val_scores, 0.999)
+    xw = model.profiles(held.seq, held.starts, held.lens)
+    assert np.array_equal(xw, held.xw)
+    passed = tr.pair_scores(model.logits(xw, held.xf), held.win_rec, len(held.recs)) >= thr
+    ev = tr.evaluate(searcher, held, passed, repeats=1)
+    assert ev["rows_not_in_plain_search"] == 0 and ev["precision"] == 1.0, ev
+    assert ev["hsp_recall"] >= 0.995, ev
