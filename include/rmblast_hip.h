@@ -191,14 +191,20 @@ int rmb_ctx_destroy(rmb_ctx* ctx);
  *                   X-drop: 256 past 48 band cells)
  *  tb_wg_per_cu     traceback workgroups (64 lanes) per CU (1..16, capped at
  *                   8, or 2 with the 256-cell rings, by the lane arenas'
- *                   memory; default 7, or 1) */
+ *                   memory; default 7, or 1)
+ *  tb_wave_halves   a traceback launch of at most this many halves runs
+ *                   every half one per wave, a band row per vector step,
+ *                   instead of one per lane (1..2^30; -1 never; default 112
+ *                   per CU): with few halves the lanes' serial walks along
+ *                   the longest ones set the time, with many the lanes'
+ *                   throughput wins; only with the 64-cell rings */
 typedef struct rmb_tuning {
   int32_t batches;
   int32_t prelim_ring;
   int32_t prelim_wg_per_cu;
   int32_t tb_ring;
   int32_t tb_wg_per_cu;
-  int32_t pad;
+  int32_t tb_wave_halves;
 } rmb_tuning;
 int rmb_ctx_set_tuning(rmb_ctx* ctx, const rmb_tuning* tuning);
 

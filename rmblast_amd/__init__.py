@@ -36,10 +36,10 @@ class RmbParams(ctypes.Structure):
 
 class RmbTuning(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in ("batches", "prelim_ring", "prelim_wg_per_cu", "tb_ring",
-                                              "tb_wg_per_cu", "pad")]
+                                              "tb_wg_per_cu", "tb_wave_halves")]
 
 
-TUNING_FIELDS = ("batches", "prelim_ring", "prelim_wg_per_cu", "tb_ring", "tb_wg_per_cu")
+TUNING_FIELDS = ("batches", "prelim_ring", "prelim_wg_per_cu", "tb_ring", "tb_wg_per_cu", "tb_wave_halves")
 TUNING_PATH = os.path.join(_HERE, "tuning.tsv")
 
 
@@ -63,7 +63,7 @@ def read_tuning(path: str = TUNING_PATH) -> dict:
                 continue
             row = dict(zip(head, cols))
             key = (row["matrix"], int(row["word_size"]), int(row["xdrop_gap"]), int(row["xdrop_gap_final"]))
-            out[key] = {k: int(row[k]) for k in TUNING_FIELDS}
+            out[key] = {k: int(row.get(k, 0)) for k in TUNING_FIELDS}
     return out
 
 
@@ -288,7 +288,7 @@ class Searcher:
             self.set_tuning(tuning)
 
     def set_tuning(self, tuning: dict):
-        t = RmbTuning(*[int(tuning.get(k, 0)) for k in TUNING_FIELDS], 0)
+        t = RmbTuning(*[int(tuning.get(k, 0)) for k in TUNING_FIELDS])
         self._check(self._lib.rmb_ctx_set_tuning(self._ctx, ctypes.byref(t)), "rmb_ctx_set_tuning")
         self.tuning = dict(tuning)
 

@@ -1,7 +1,8 @@
 """Autotuner of the search's kernel schedule, per matrix and word size.
 
 Sweeps the rmb_tuning choices (pipeline batches, the preliminary and traceback
-DPs' LDS band rings and workgroups per CU) on a sample workload, by
+DPs' LDS band rings and workgroups per CU, lane- or wave-run traceback) on a
+sample workload, by
 coordinate descent from the library's defaults: each parameter in turn takes
 each of its values with the others held at the best so far, and a value is
 kept when its search is faster.  Every configuration must return the same
@@ -30,6 +31,7 @@ SPACE = {  # candidate values per field; 0 is the library's default
     "prelim_wg_per_cu": [0, 2, 4, 8],
     "tb_ring": [0, 64, 256],
     "tb_wg_per_cu": [0, 2, 4, 7],
+    "tb_wave_halves": [0, -1, 1 << 30],  # default regime, always lane-run, always wave-run
 }
 
 
@@ -84,7 +86,7 @@ def write_entry(path, key, cfg, ms, workload):
             lines = [l.rstrip("\n").split("\t") for l in f if l.strip() and not l.startswith("#")]
         for cols in lines[1:]:
             rows[tuple(cols[:4])] = cols
-    rows[tuple(str(x) for x in key)] = [str(x) for x in key] + [str(cfg[k]) for k in rmblast_amd.TUNING_FIELDS] + \
+    rows[tuple(str(x) for x in key)] = [str(x) for x in key] + [str(cfg.get(k, 0)) for k in rmblast_amd.TUNING_FIELDS] + \
         [f"{ms:.1f}", workload]
     with open(path, "w") as f:
         f.write("# written by rmblast_amd/autotune.py: the fastest rmb_tuning per (matrix, word size, X-drops)\n")

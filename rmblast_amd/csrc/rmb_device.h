@@ -53,7 +53,11 @@ enum Counter {
   C_SEED_EVALS = 0, C_UNG_HITS = 1, C_UNG_STEPS = 2, C_DP_CELLS = 3, C_PRELIM_CELLS = 4, C_PRELIM_DPS = 5,
   C_SEEDS = 6,          // seeds staged by Phase 1 (valid or not)
   C_P1_STAGE = 8,       // 6 counters: Phase-1 stage A..F time, 100 MHz ticks summed over workgroups
-  C_NCOUNTERS = 16
+  C_WAVE_ROWS = 14,     // rows of the wave-cooperative traceback DPs
+  C_WAVE_ROUNDS = 15,   // their scan rounds (one per row when every first guess holds)
+  C_TB_TICKS = 16,      // 4 maxima in 100 MHz ticks: a wave-run half's DP and walk, a lane-run wave of 64
+                        // halves' DP and walk (the traceback kernel's critical paths)
+  C_NCOUNTERS = 20
 };
 
 struct PreHit {         // an ungapped hit whose preliminary DP reached the gapped cutoff

@@ -11,6 +11,8 @@
 //  k_dp_onepass    Phase 2b: score, traceback bytes and traceback walk of a
 //                  half alignment in one lane-serial pass
 //                  (search/gapped.rs:450-757)
+//  k_dp_wave       the same for long halves, one per wave, a band row per
+//                  vector step (prefix scans resolve the row's dependencies)
 //  k_dp_score      lane-serial score-only X-drop half alignment
 //                  (search/gapped.rs:124-337)
 //  k_dp_traceback  the same recurrence with 1-byte traceback cells and the
@@ -1144,6 +1146,253 @@ __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, in
   res.tb_rows = rows_at_best;
 }
 
+// ------------------------------------------ wave-cooperative X-drop DP ------
+// One wave runs one half alignment, a row at a time: lane l holds band column
+// first_b + l (H and E of the row before, the column's b code), so a row of up
+// to 64 cells is one vector step instead of a lane's serial walk along it.
+// The row's serial dependencies are resolved exactly:
+//  * G_j = max(diagonal, E_j) needs only the row before;
+//  * the horizontal gap F obeys F_{j+1} = max(F_j - ge, G_j - oe) after a kept
+//    cell and F_{j+1} = F_j after a pruned one (dp_half leaves sgr alone
+//    there), an affine map in (max, +) composed by a prefix scan;
+//  * the running best before cell j is max(best0, cs_k for k < j) over every
+//    earlier cell, kept or not (a pruned cell lies below best - X), so it is
+//    a prefix max;
+//  * whether a cell is kept depends on the two scans, which depend on which
+//    earlier cells are kept.  The row is iterated from a guess: one round's
+//    decisions are exact up to the first cell whose guess was wrong, so the
+//    rounds stop, at the latest after one per cell, with every decision
+//    dp_half's.
+// The band bookkeeping (first_b over the leading pruned run, b_size cut to the
+// last kept cell or extended while F stays within X of best, the sentinel
+// cell) and the traceback bytes and row table are dp_half's, so the result,
+// the bytes and the walk are identical.  Bands wider than 64 cells come back
+// E_BAND_WIDE, as from the 64-cell LDS ring.
+
+// lane i takes lane i - 1's value of v; lane 0 takes `fill`
+__device__ __forceinline__ int32_t wave_shr1(int32_t v, int32_t fill) {
+  return __builtin_amdgcn_update_dpp(fill, v, 0x138, 0xF, 0xF, false);  // wave_shr:1
+}
+// v from the lane `s` rows of 16 lanes back within its row (row_shr:s), or id
+template <int CTRL, int ROWMASK = 0xF>
+__device__ __forceinline__ int32_t dpp_or(int32_t v, int32_t id) {
+  return __builtin_amdgcn_update_dpp(id, v, CTRL, ROWMASK, 0xF, false);
+}
+// inclusive prefix max over the wave's 64 lanes
+__device__ __forceinline__ int32_t wave_prefix_max(int32_t v, int32_t id) {
+  v = max(v, dpp_or<0x111>(v, id));  // row_shr:1
+  v = max(v, dpp_or<0x112>(v, id));  // row_shr:2
+  v = max(v, dpp_or<0x114>(v, id));  // row_shr:4
+  v = max(v, dpp_or<0x118>(v, id));  // row_shr:8
+  v = max(v, dpp_or<0x142, 0xA>(v, id));  // row_bcast:15 into rows 1 and 3
+  v = max(v, dpp_or<0x143, 0xC>(v, id));  // row_bcast:31 into rows 2 and 3
+  return v;
+}
+// inclusive prefix composition of the maps x -> max(x + A, B), earlier lanes
+// first: (A1, B1) then (A2, B2) is (A1 + A2, max(B1 + A2, B2)); (0, neg) is
+// the identity
+template <int CTRL, int ROWMASK = 0xF>
+__device__ __forceinline__ void affine_step(int32_t& A, int32_t& B, int32_t neg) {
+  const int32_t pa = dpp_or<CTRL, ROWMASK>(A, 0), pb = dpp_or<CTRL, ROWMASK>(B, neg);
+  B = max(pb + A, B);
+  A = pa + A;
+}
+__device__ __forceinline__ void wave_prefix_affine(int32_t& A, int32_t& B, int32_t neg) {
+  affine_step<0x111>(A, B, neg);
+  affine_step<0x112>(A, B, neg);
+  affine_step<0x114>(A, B, neg);
+  affine_step<0x118>(A, B, neg);
+  affine_step<0x142, 0xA>(A, B, neg);
+  affine_step<0x143, 0xC>(A, B, neg);
+}
+__device__ __forceinline__ uint64_t wave_ballot(bool p) { return __ballot(p); }
+
+// Byte `idx` of a sequence read as dp_half reads it (reversed when rev).
+__device__ __forceinline__ uint32_t seq_code(const uint8_t* p, uint32_t len, bool rev, uint32_t idx) {
+  return idx < len ? (uint32_t)(p[rev ? len - 1 - idx : idx] & 15u) : 0u;
+}
+
+struct WaveDpOut {
+  DpResult res;
+  uint32_t rounds;  // scan rounds over all rows (diagnostics)
+};
+
+// Requires a full wave in lockstep (all 64 lanes call it with the same task).
+// Row table and traceback bytes as dp_half<true>; results as dp_half.
+__device__ __forceinline__ void dp_half_wave(const DpTask& t, const int* __restrict__ mat, int32_t go, int32_t ge,
+                                          DpResult& res, uint8_t* __restrict__ flat, uint32_t* __restrict__ rows,
+                                          uint32_t flat_cap, uint32_t rows_cap, uint32_t& rounds) {
+  constexpr uint32_t wmask = 63;
+  constexpr int32_t kNeg = INT32_MIN + (1 << 27);  // below every reachable value, and safe to add to
+  const uint32_t lane = threadIdx.x & 63u;
+  res.score = 0; res.a_len = 0; res.b_len = 0; res.flags = 0; res.cells = 0; res.tb_bytes = 0; res.tb_rows = 0;
+  const uint32_t m = t.m, n = t.n;
+  if (m == 0 || n == 0) return;
+  const bool rev = t.reverse != 0;
+  const int32_t oe = go + ge;
+  const int32_t xdrop = t.xdrop > oe ? t.xdrop : oe;
+  // row 0: columns 0..J with H = -oe - (j - 1) ge while it stays within X
+  uint32_t J;
+  if (ge <= 0) J = n;
+  else J = min(n, (uint32_t)((xdrop - oe) / ge) + 1u);
+  if (J > wmask) { res.flags |= E_BAND_WIDE; return; }
+  // b codes of columns [wb, wb + 192) in three registers, a codes of rows
+  // [ab, ab + 64) in one: lanes hold one byte of each
+  uint32_t wb = 0;
+  uint32_t bw0 = seq_code(t.b, n, rev, lane), bw1 = seq_code(t.b, n, rev, 64 + lane),
+           bw2 = seq_code(t.b, n, rev, 128 + lane);
+  uint32_t ab = 0;
+  uint32_t aw = seq_code(t.a, m, rev, lane), awn = seq_code(t.a, m, rev, 64 + lane);
+  // col in [wb, wb + 128) (first_b - wb < 64 holds); bw2 is the next slide's
+  // prefetch.  Every lane calls it.
+  auto bcode_of = [&](uint32_t col) -> uint32_t {
+    const uint32_t o = col - wb;
+    const int src = (int)((o & 63u) << 2);
+    const uint32_t v0 = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)bw0);
+    const uint32_t v1 = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)bw1);
+    return o < 64 ? v0 : v1;
+  };
+  // the band: lane l holds column first_b + l while first_b + l < b_size
+  uint32_t first_b = 0, b_size = J + 1;
+  int32_t H = lane == 0 ? 0 : -oe - (int32_t)(lane - 1) * ge;
+  int32_t E = H - oe;
+  uint32_t bc = bcode_of(lane);
+  if (lane == 0) { rows[0] = 0; rows[1] = 0; }
+  if (lane <= J) flat[lane] = lane == 0 ? SCRIPT_GAP_IN_B : SCRIPT_GAP_IN_A;
+  uint32_t flat_len = J + 1;
+  int32_t best = 0;
+  uint32_t best_a = 0, best_b = 0;
+  uint64_t cells = 0;
+  uint32_t bytes_at_best = flat_len, rows_at_best = 1;
+  for (uint32_t ai = 0; ai < m; ++ai) {
+    if (ai + 2 > rows_cap || flat_len + 2 * wmask + 4 > flat_cap) { res.flags |= E_TB_SPACE; return; }
+    if (ai - ab == 64) {
+      ab += 64;
+      aw = awn;
+      awn = seq_code(t.a, m, rev, ab + 64 + lane);
+    }
+    const uint32_t acode = (uint32_t)__builtin_amdgcn_readlane((int)aw, (int)(ai - ab));
+    const uint32_t w = b_size - first_b;
+    const bool act = lane < w;
+    if (lane == 0) { rows[2 * (ai + 1)] = flat_len; rows[2 * (ai + 1) + 1] = first_b; }
+    cells += w;
+    const int32_t sub = mat[(acode << 4) | bc];
+    const int32_t D = wave_shr1(H + sub, kMinInt);  // sc: the cell before's H + its substitution
+    const int32_t V = E;
+    const int32_t G = max(D, V);
+    // iterate the keep decisions to the fixed point
+    // (kept iff cs >= best - X, i.e. not best - cs > X)
+    bool kept = act && G >= best - xdrop;
+    int32_t F, cs, f_after;
+    uint64_t kmask = wave_ballot(kept);
+    for (uint32_t round = 0;; ++round) {
+      // each round settles at least the first unsettled cell, so 65 rounds
+      // settle a row of 64; more would mean a broken scan: hand the half back
+      if (round > 64) { res.flags |= E_BAND_WIDE; return; }
+      ++rounds;
+      int32_t A = kept ? -ge : 0, B = kept ? G - oe : kNeg;
+      wave_prefix_affine(A, B, kNeg);
+      f_after = max(kMinInt + A, B);  // F after this lane's cell
+      F = wave_shr1(f_after, kMinInt);
+      cs = act ? max(G, F) : kNeg;
+      const int32_t bb = max(best, wave_shr1(wave_prefix_max(cs, kNeg), kNeg));
+      const bool k2 = act && cs >= bb - xdrop;
+      const uint64_t m2 = wave_ballot(k2);
+      if (m2 == kmask) break;
+      kept = k2;
+      kmask = m2;
+    }
+    // traceback byte of each cell (dp_half's choice and extension flags)
+    uint8_t op = SCRIPT_SUB;
+    {
+      int32_t c0 = D;
+      if (V > c0) { c0 = V; op = SCRIPT_GAP_IN_B; }
+      if (F > c0) op = SCRIPT_GAP_IN_A;
+      if (kept) {
+        if (V - ge >= cs - oe) op |= SCRIPT_EXTEND_GAP_B;
+        if (F - ge >= cs - oe) op |= SCRIPT_EXTEND_GAP_A;
+      }
+    }
+    if (act) flat[flat_len + lane] = op;
+    if (kmask == 0) break;  // every cell pruned: first_b reaches b_size
+    const uint32_t lead = (uint32_t)__builtin_ctzll(kmask);
+    const uint32_t last = 63u - (uint32_t)__builtin_clzll(kmask);
+    // the row's best: the first cell holding the row maximum, if it beats best
+    const int32_t rmax = __builtin_amdgcn_readlane(wave_prefix_max(cs, kNeg), (int)(w - 1));
+    bool best_here = false;
+    if (rmax > best) {
+      const uint64_t at = wave_ballot(act && cs == rmax);
+      best = rmax;
+      best_a = ai + 1;
+      best_b = first_b + (uint32_t)__builtin_ctzll(at);
+      best_here = true;
+    }
+    // the row's cells for the next row: kept ones take (cs, E'), pruned ones
+    // (not leading) (minimum, E unchanged)
+    int32_t Hn = kept ? cs : kMinInt;
+    int32_t En = kept ? max(V - ge, cs - oe) : V;
+    flat_len += w;
+    const uint32_t nfirst = first_b + lead;
+    uint32_t nsize = b_size;
+    uint32_t ext = 0;
+    int32_t f_end = 0;
+    if (last + 1 < w) {
+      nsize = first_b + last + 1;
+    } else {
+      // F after the last cell, then the run of GAP_IN_A cells while F - k ge
+      // stays within X of best
+      f_end = __builtin_amdgcn_readlane(f_after, (int)(w - 1));
+      const int32_t floor_v = best - xdrop;
+      if (f_end >= floor_v) {
+        const uint32_t room = n + 1 - b_size;  // columns b_size..n
+        uint32_t k = ge <= 0 ? room : (uint32_t)((f_end - floor_v) / ge) + 1u;
+        ext = min(k, room);
+      }
+      if (ext && (b_size + ext - 1) - nfirst > wmask) { res.flags |= E_BAND_WIDE; return; }
+      if (lane < ext) flat[flat_len + lane] = SCRIPT_GAP_IN_A;
+      flat_len += ext;
+      nsize = b_size + ext;
+    }
+    uint32_t sentinel = 0;
+    if (nsize <= n) {
+      if (nsize - nfirst > wmask) { res.flags |= E_BAND_WIDE; return; }
+      sentinel = 1;
+    }
+    // shift the band to start at nfirst: lane l takes column nfirst + l
+    {
+      const uint32_t col = nfirst + lane;
+      const int src = (int)(((lane + lead) & 63u) << 2);
+      const int32_t hs = __builtin_amdgcn_ds_bpermute(src, Hn);
+      const int32_t es = __builtin_amdgcn_ds_bpermute(src, En);
+      if (col < min(b_size, nsize)) { H = hs; E = es; }  // the row's cells still in the band
+      else if (col < nsize) {  // extension cells
+        const int32_t f = f_end - (int32_t)(col - b_size) * ge;
+        H = f;
+        E = f - oe;
+      } else {
+        H = kMinInt;
+        E = kMinInt;
+      }
+    }
+    first_b = nfirst;
+    b_size = nsize + sentinel;
+    if (first_b - wb >= 64) {  // slide the b window by 64 columns
+      wb += 64;
+      bw0 = bw1;
+      bw1 = bw2;
+      bw2 = seq_code(t.b, n, rev, wb + 128 + lane);
+    }
+    bc = bcode_of(first_b + lane);
+    if (best_here) { bytes_at_best = flat_len; rows_at_best = ai + 2; }
+  }
+  res.score = best;
+  res.a_len = best_a;
+  res.b_len = best_b;
+  res.cells = cells;
+  res.tb_bytes = bytes_at_best;
+  res.tb_rows = rows_at_best;
+}
+
 // Traceback walk (gapped.rs:692-757) from (a, b), the best row and column,
 // over the rows' traceback bytes: the ops, run-length encoded, collected
 // backwards into ops[0, cap); the count is returned, E_TB_SPACE set in flags
@@ -1304,6 +1553,7 @@ __global__ __launch_bounds__(kDpLdsThreads) void k_dp_onepass(
     if (base >= n_tasks) break;
     const uint32_t i = (uint32_t)base + (threadIdx.x & 63u);
     if (i >= n_tasks) continue;
+    const unsigned long long l0 = wall_clock64();
     DpResult r;
     dp_half_tb16<RLOG2>(tasks[i], mat, go, ge, my, myb, max_pos, r, flat, rows, flat_cap, rows_cap);
     uint8_t* tflat = flat;
@@ -1320,6 +1570,7 @@ __global__ __launch_bounds__(kDpLdsThreads) void k_dp_onepass(
                          (1u << big.ring_log2) - 1, r, tflat, trows, 0xFFFFFFFFu, big.flat_cap, big.rows_cap);
       }
     }
+    const unsigned long long l1 = wall_clock64();
     OnePassResult o{r.score, r.a_len, r.b_len, 0, r.flags, 0};
     if (!r.flags) {
       cells += r.cells;
@@ -1338,8 +1589,78 @@ __global__ __launch_bounds__(kDpLdsThreads) void k_dp_onepass(
       }
     }
     out[i] = o;
+    {  // the wave's DP and walk times (its lanes finish together)
+      const unsigned long long l2 = wall_clock64();
+      if ((threadIdx.x & 63u) == (uint32_t)__builtin_ctzll(__ballot(true))) {
+        atomicMax(&counters[C_TB_TICKS + 2], l1 - l0);
+        atomicMax(&counters[C_TB_TICKS + 3], l2 - l1);
+      }
+    }
   }
   atomicAdd(&counters[C_DP_CELLS], cells);
+}
+
+// The traceback halves that run one per wave (dp_half_wave), four waves per
+// workgroup with no LDS beyond the matrix, so a SIMD holds several waves and
+// each wave's row-to-row latency hides behind the others.  Each wave has its
+// own arena (row table, traceback bytes, ops); a half beyond it, or with a
+// band wider than 64 cells, comes back flagged for the two-pass path.
+static constexpr uint32_t kDpWaveThreads = 256;
+__global__ __launch_bounds__(kDpWaveThreads) void k_dp_wave(
+    const DpTask* __restrict__ tasks, uint32_t n_tasks, const int32_t* __restrict__ gmat, int32_t go, int32_t ge,
+    uint8_t* __restrict__ arena, uint32_t rows_cap, uint32_t flat_cap, uint32_t ops_cap,
+    uint32_t* __restrict__ ops_pool, unsigned long long* __restrict__ ops_count, unsigned long long ops_pool_cap,
+    OnePassResult* __restrict__ out, unsigned long long* next, unsigned long long* counters) {
+  __shared__ int mat[256];
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) mat[i] = gmat[i];
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  uint8_t* wbase = arena + wave * onepass_stride(flat_cap, rows_cap, ops_cap);
+  uint32_t* wrows = reinterpret_cast<uint32_t*>(wbase);
+  uint8_t* wflat = wbase + (size_t)rows_cap * 8;
+  uint32_t* wops = reinterpret_cast<uint32_t*>(wflat + ((flat_cap + 15u) & ~15u));
+  unsigned long long cells = 0, wrows_n = 0;
+  uint32_t rounds = 0;
+  for (;;) {
+    const uint64_t k = wave_take(next, 1);
+    if (k >= n_tasks) break;
+    const uint32_t i = (uint32_t)k;
+    DpResult r;
+    const unsigned long long t0 = wall_clock64();
+    dp_half_wave(tasks[i], mat, go, ge, r, wflat, wrows, flat_cap, rows_cap, rounds);
+    // the walk reads bytes other lanes of the wave stored
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    const unsigned long long t1 = wall_clock64();
+    if (lane == 0) {
+      atomicMax(&counters[C_TB_TICKS], t1 - t0);
+      wrows_n += r.tb_rows;
+      OnePassResult o{r.score, r.a_len, r.b_len, 0, r.flags, 0};
+      if (!r.flags) {
+        cells += r.cells;
+        if (r.a_len | r.b_len) {
+          const uint32_t nops = tb_walk(wflat, wrows, r.a_len, r.b_len, r.tb_bytes, wops, ops_cap, o.flags);
+          if (!o.flags) {
+            const unsigned long long at = atomicAdd(ops_count, (unsigned long long)nops);
+            if (at + nops <= ops_pool_cap) {
+              for (uint32_t q = 0; q < nops; ++q) ops_pool[at + q] = wops[nops - 1 - q];
+              o.n_ops = nops;
+              o.ops_at = at;
+            } else {
+              o.flags |= E_TB_SPACE;
+            }
+          }
+        }
+      }
+      out[i] = o;
+      atomicMax(&counters[C_TB_TICKS + 1], wall_clock64() - t1);
+    }
+  }
+  if (lane == 0) {
+    atomicAdd(&counters[C_DP_CELLS], cells);
+    atomicAdd(&counters[C_WAVE_ROWS], wrows_n);
+    atomicAdd(&counters[C_WAVE_ROUNDS], (unsigned long long)rounds);
+  }
 }
 
 // Gathers each traceback's ops from its worst-case slot of the op pool into one
@@ -1633,6 +1954,17 @@ hipError_t launch_dp_onepass(int blocks, const DpTask* tasks, uint32_t n, const 
     hipLaunchKernelGGL((k_dp_onepass<true, kDpLdsLog2>), dim3(blocks), dim3(kDpLdsThreads), 0, st, tasks, n, gmat, go,
                        ge, nullptr, 0, arena, flat_cap, rows_cap, ops_cap, ops_pool, ops_count, ops_pool_cap, out,
                        next, big, counters);
+  return hipGetLastError();
+}
+
+// `blocks` workgroups of four waves; `arena` holds blocks x 4 wave arenas.
+hipError_t launch_dp_wave(int blocks, const DpTask* tasks, uint32_t n, const int32_t* gmat, int32_t go, int32_t ge,
+                          uint8_t* arena, uint32_t rows_cap, uint32_t flat_cap, uint32_t ops_cap, uint32_t* ops_pool,
+                          unsigned long long* ops_count, unsigned long long ops_pool_cap, OnePassResult* out,
+                          unsigned long long* next, unsigned long long* counters, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dp_wave, dim3(blocks), dim3(kDpWaveThreads), 0, st, tasks, n, gmat, go, ge, arena, rows_cap,
+                     flat_cap, ops_cap, ops_pool, ops_count, ops_pool_cap, out, next, counters);
   return hipGetLastError();
 }
 

@@ -61,6 +61,9 @@ hipError_t launch_dp_traceback(int, const TbTask*, uint32_t, const int32_t*, int
 hipError_t launch_dp_onepass(int, const DpTask*, uint32_t, const int32_t*, int32_t, int32_t, uint8_t*, uint32_t,
                              uint32_t, uint32_t, uint32_t*, unsigned long long*, unsigned long long, OnePassResult*,
                              unsigned long long*, OnePassBig, unsigned long long*, bool, hipStream_t);
+hipError_t launch_dp_wave(int, const DpTask*, uint32_t, const int32_t*, int32_t, int32_t, uint8_t*, uint32_t, uint32_t,
+                          uint32_t, uint32_t*, unsigned long long*, unsigned long long, OnePassResult*,
+                          unsigned long long*, unsigned long long*, hipStream_t);
 hipError_t launch_prelim_fused(int, bool, PrelimParams, const UngHit*, const uint32_t*, uint64_t, const RecDesc*,
                                const SubjDesc*, const uint8_t*, const uint8_t*, const uint8_t*, const int32_t*, int2*,
                                uint32_t, PreHit*, unsigned long long*, uint32_t*, UngHit*, unsigned long long*,
@@ -277,7 +280,7 @@ struct rmb_ctx {
   DevBuf b_hits, b_hitcount, b_spill, b_hitres, b_queue, b_counters, b_err;
   DevBuf b_tasks, b_dpres, b_ring, b_ring_big, b_tbtasks, b_tbres, b_flat, b_rows, b_ops;
   DevBuf b_pre, b_redo, b_precount, b_packtasks, b_packed, b_items, b_qext, b_recids;
-  DevBuf b_arena, b_arena_big, b_oppool, b_opres, b_opcount;  // one-pass traceback
+  DevBuf b_arena, b_arena_big, b_arena_wave, b_oppool, b_opres, b_opcount;  // one-pass traceback
 };
 
 namespace rmb {
@@ -550,8 +553,24 @@ static uint32_t retry_slots() {
   return kBigSlots;
 }
 
+// A launch of at most kTbWaveHalvesPerCu x CUs halves runs every half one
+// per wave, more halves one per lane.  A lane's half costs about 35
+// instructions per band cell, and a lane-run wave lasts as long as the most
+// cells among its 64 halves; a wave-run half costs about 150 per band row
+// (2 scan rounds) whatever the band (up to 64 cells).  With few halves the
+// lane-run kernel is one row of waves whose slowest lanes set the time
+// (1 Mbp x 50 families at 4 kbp records, 14 k halves: 27 ms lane-run, 10 ms
+// wave-run); with many, the lanes' throughput wins (the 100 Mbp x 1,000
+// family workload, 117 k halves a batch: 0.99 s lane-run, 1.13 s wave-run).
+constexpr int32_t kTbWaveHalvesPerCu = 112;
+// wave arenas: row table, traceback bytes (at most two rings' worth per row)
+// and ops of one half
+constexpr uint32_t kWaveRows = 8192, kWaveFlat = kWaveRows * 128 + 256, kWaveOps = 4096;
+constexpr uint32_t kWavePerCu = 8;  // workgroups of four waves per CU
+
 // Traceback DPs: one pass (score, traceback bytes, walk) per half in a
-// per-lane arena, longest first; the halves it flags run in two passes.
+// per-lane arena, longest first (the longest one per wave, the rest one per
+// lane); the halves it flags run in two passes.
 static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::vector<HalfTb>& out,
                              std::vector<uint32_t>& op_pool, const std::vector<uint64_t>* work = nullptr) {
   out.assign(tasks.size(), HalfTb{0, 0, 0, 0, 0});
@@ -566,6 +585,10 @@ static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::
   }
   std::vector<DpTask> sorted(tasks.size());
   for (size_t k = 0; k < ord.size(); ++k) sorted[k] = tasks[ord[k]];
+  // few halves: every one runs one per wave
+  const int64_t wave_halves =
+      c->tune.tb_wave_halves ? c->tune.tb_wave_halves : (int64_t)kTbWaveHalvesPerCu * c->cu_count;
+  const size_t n_wave_all = wave_halves > 0 && !onepass_wide(c) && ord.size() <= (size_t)wave_halves ? ord.size() : 0;
   unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
   std::vector<uint32_t> fallback;
   const size_t chunk = 1u << 20;
@@ -577,15 +600,22 @@ static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::
     // lane arenas are per workgroup: at most 8 (2 with the wide rings) per CU
     const uint32_t per_cu = c->tune.tb_wg_per_cu ? std::min<uint32_t>((uint32_t)c->tune.tb_wg_per_cu, wide ? 2u : 8u)
                                                  : (wide ? kOnePassWidePerCu : kOnePassPerCu);
-    const uint32_t blocks = std::min<uint32_t>((uint32_t)c->cu_count * per_cu, (n + 63) / 64);
+    // the first n_wave halves run one per wave, the rest one per lane
+    const uint32_t n_wave = base < n_wave_all ? (uint32_t)std::min<size_t>(n, n_wave_all - base) : 0u;
+    const uint32_t n_lane = n - n_wave;
+    const uint32_t blocks = std::min<uint32_t>((uint32_t)c->cu_count * per_cu, (n_lane + 63) / 64);
     const uint32_t flat_cap = wide ? kOnePassWideFlat : kOnePassFlat, rows_cap = wide ? kOnePassWideRows : kOnePassRows,
                    ops_cap = wide ? kOnePassWideOps : kOnePassOps;
     const uint64_t stride = onepass_stride(flat_cap, rows_cap, ops_cap);
-    uint8_t* arena = c->b_arena.get<uint8_t>((size_t)blocks * 64 * stride);
+    uint8_t* arena = n_lane ? c->b_arena.get<uint8_t>((size_t)blocks * 64 * stride) : nullptr;
+    const uint32_t wblocks = std::min<uint32_t>((uint32_t)c->cu_count * kWavePerCu, (n_wave + 3) / 4);
+    uint8_t* warena = n_wave ? c->b_arena_wave.get<uint8_t>((size_t)wblocks * 4 *
+                                                             onepass_stride(kWaveFlat, kWaveRows, kWaveOps))
+                             : nullptr;
     uint32_t* pool = c->b_oppool.get<uint32_t>(kOnePassPool);
-    // [0] ops used, [1] work queue, [2] retry slots taken (low 32 bits)
-    unsigned long long* pcount = c->b_opcount.get<unsigned long long>(3);
-    hip_check(hipMemsetAsync(pcount, 0, 3 * sizeof(unsigned long long), c->st), "clear op count");
+    // [0] ops used, [1] work queue, [2] retry slots taken (low 32 bits), [3] wave queue
+    unsigned long long* pcount = c->b_opcount.get<unsigned long long>(4);
+    hip_check(hipMemsetAsync(pcount, 0, 4 * sizeof(unsigned long long), c->st), "clear op count");
     OnePassBig big{};
     big.ring = c->b_ring_big.get<int2>((size_t)kBigSlots * ring_stride(kRingBigLog2));
     big.arena = c->b_arena_big.get<uint8_t>((size_t)kBigSlots * onepass_stride(kBigFlat, kBigRows, kBigOps));
@@ -598,10 +628,14 @@ static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::
     hip_check(hipMemcpyAsync(dt, sorted.data() + base, n * sizeof(DpTask), hipMemcpyHostToDevice, c->st),
               "copy tasks");
     timed_launch(c, 14, [&] {
-      hip_check(launch_dp_onepass((int)blocks, dt, n, c->d_mat, c->P.gap_open, c->P.gap_extend, arena, flat_cap,
-                                  rows_cap, ops_cap, pool, pcount, kOnePassPool, dr, pcount + 1, big, ctr,
-                                  wide, c->st),
-                "k_dp_onepass");
+      hip_check(launch_dp_wave((int)wblocks, dt, n_wave, c->d_mat, c->P.gap_open, c->P.gap_extend, warena, kWaveRows,
+                               kWaveFlat, kWaveOps, pool, pcount, kOnePassPool, dr, pcount + 3, ctr, c->st),
+                "k_dp_wave");
+      if (n_lane)
+        hip_check(launch_dp_onepass((int)blocks, dt + n_wave, n_lane, c->d_mat, c->P.gap_open, c->P.gap_extend, arena,
+                                    flat_cap, rows_cap, ops_cap, pool, pcount, kOnePassPool, dr + n_wave, pcount + 1,
+                                    big, ctr, wide, c->st),
+                  "k_dp_onepass");
     });
     std::vector<OnePassResult> r(n);
     unsigned long long used = 0;
@@ -1491,6 +1525,12 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
   c->stats[12] = total.us();
   c->stats[16] = p1_ctr[C_SEEDS];
   for (int k = 0; k < 6; ++k) c->stats[17 + k] = p1_ctr[C_P1_STAGE + k];
+  if (std::getenv("RMB_STAGE_TIMES"))
+    std::fprintf(stderr,
+                 "[rmb] wave-run traceback rows %llu, scan rounds %llu; longest (us): wave-run DP %.1f walk %.1f, "
+                 "lane-run DP %.1f walk %.1f\n",
+                 hc[C_WAVE_ROWS], hc[C_WAVE_ROUNDS], hc[C_TB_TICKS] / 100.0, hc[C_TB_TICKS + 1] / 100.0,
+                 hc[C_TB_TICKS + 2] / 100.0, hc[C_TB_TICKS + 3] / 100.0);
 }
 }  // namespace rmb
 
@@ -1682,7 +1722,7 @@ int rmb_ctx_destroy(rmb_ctx* c) {
                     &c->b_hits, &c->b_hitcount, &c->b_spill, &c->b_hitres, &c->b_queue, &c->b_counters, &c->b_err, &c->b_tasks,
                     &c->b_dpres, &c->b_ring, &c->b_ring_big, &c->b_tbtasks, &c->b_tbres, &c->b_flat, &c->b_rows,
                     &c->b_ops, &c->b_pre, &c->b_redo, &c->b_precount, &c->b_packtasks, &c->b_packed, &c->b_items, &c->b_qext,
-                    &c->b_recids, &c->b_arena, &c->b_arena_big, &c->b_oppool, &c->b_opres, &c->b_opcount})
+                    &c->b_recids, &c->b_arena, &c->b_arena_big, &c->b_arena_wave, &c->b_oppool, &c->b_opres, &c->b_opcount})
     b->release();
   c->h_qfwd.release();
   c->h_qrc.release();
@@ -1709,6 +1749,7 @@ int rmb_ctx_set_tuning(rmb_ctx* c, const rmb_tuning* t) {
     if (t->prelim_wg_per_cu < 0 || t->prelim_wg_per_cu > 16) bad("prelim_wg_per_cu must be 0..16");
     if (t->tb_ring != 0 && t->tb_ring != 64 && t->tb_ring != 256) bad("tb_ring must be 0, 64 or 256");
     if (t->tb_wg_per_cu < 0 || t->tb_wg_per_cu > 16) bad("tb_wg_per_cu must be 0..16");
+    if (t->tb_wave_halves < -1 || t->tb_wave_halves > (1 << 30)) bad("tb_wave_halves must be -1, 0 or 1..2^30");
     c->tune = *t;
   });
 }

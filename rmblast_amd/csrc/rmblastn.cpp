@@ -519,7 +519,8 @@ int main(int argc, char** argv) {
       if (row["matrix"] != base || num("word_size") != P.word_size || num("xdrop_gap") != P.xdrop_gap ||
           num("xdrop_gap_final") != P.xdrop_gap_final)
         continue;
-      rmb_tuning t{num("batches"), num("prelim_ring"), num("prelim_wg_per_cu"), num("tb_ring"), num("tb_wg_per_cu"), 0};
+      rmb_tuning t{num("batches"), num("prelim_ring"), num("prelim_wg_per_cu"), num("tb_ring"), num("tb_wg_per_cu"),
+                    num("tb_wave_halves")};
       if (rmb_ctx_set_tuning(ctx, &t)) return fail("tuning");
       break;
     }

@@ -234,6 +234,41 @@ def test_tap_dp_kernels_match_oracle(oracle_lib):
             assert gops == [(int(ops[2 * k]), int(ops[2 * k + 1])) for k in range(n)], (trial, rev)
 
 
+@pytest.mark.parametrize("wave_halves", [1 << 30, -1], ids=["wave_run", "lane_run"])
+def test_traceback_dp_kernels_match_oracle(oracle_lib, wave_halves):
+    """Every traceback half through the wave-cooperative DP (tb_wave_halves
+    2^30), then through the lane-serial one (-1), with 64-cell rings: score,
+    extents and ops identical to the oracle's, over
+    mutated copies, unrelated pairs, tandem repeats (bands past 64 cells,
+    which fall back to the two-pass path) and ambiguity codes, both
+    directions, four X-drops and three gap costs; and single-base and
+    empty-overlap edge cases."""
+    m = rmblast_amd.load_matrix(golden("comparison.matrix"))
+    mat = np.ascontiguousarray(np.array([[m.scores[i][j] for j in range(16)] for i in range(16)], dtype=np.int32))
+    rng = np.random.default_rng(2024)
+    gaps = [(20, 5), (4, 4), (8, 2)]
+    ctxs = {}
+    for g in gaps:
+        ctxs[g] = rmblast_amd.Searcher(m, gap_open=g[0], gap_extend=g[1])
+        ctxs[g].set_tuning({"tb_ring": 64, "tb_wave_halves": wave_halves})
+    pairs = [random_pair(rng) for _ in range(150)]
+    pairs += [(np.array([0], np.uint8), np.array([0], np.uint8)), (np.array([1], np.uint8), np.array([2], np.uint8)),
+              (np.array([0, 1, 2, 3], np.uint8), np.array([3], np.uint8)),
+              (rng.integers(0, 4, 5000).astype(np.uint8), rng.integers(0, 4, 3).astype(np.uint8))]
+    for trial, (a, b) in enumerate(pairs):
+        go, ge = gaps[trial % 3]
+        s = ctxs[(go, ge)]
+        x = [30, 100, 400, 10 ** 6][trial % 4]
+        for rev in (0, 1):
+            o = (ctypes.c_int64 * 4)()
+            ops = np.zeros(2 * 20000, dtype=np.uint32)
+            n = oracle_lib.orc_align_ex(ptr(a), ptr(b), len(a), len(b), go, ge, x, ptr(mat, ctypes.c_int32), rev, o,
+                                        ptr(ops, ctypes.c_uint32), 20000)
+            sc, al, bl, gops = s.tap_align_ex(a, b, x, rev)
+            assert (sc, al, bl) == tuple(o[:3]), (trial, rev, len(a), len(b))
+            assert gops == [(int(ops[2 * k]), int(ops[2 * k + 1])) for k in range(n)], (trial, rev)
+
+
 # the NCBI goldens that run with DUST on (cli.rs:147-299), through the product CLI
 GPU_DUST_GOLDENS = [
     ("rm_defaults", "query.fa", "db.fa", "comparison.matrix", ["--mask-level", "80", "--complexity-adjust"]),

@@ -62,6 +62,23 @@ def test_parity_config_full_size_matches_oracle(tmp_path):
     assert g == o and g.count("\n") > 1000
 
 
+def test_parity_config_with_every_traceback_half_lane_run(tmp_path):
+    """The 1 Mbp parity configuration with every traceback half through the
+    lane-serial DP (tb_wave_halves -1; its default at this size runs them one
+    per wave): the oracle's rows."""
+    w = synth.parity_config(seed=7)
+    synth.to_fasta(w.records, tmp_path / "q.fa")
+    synth.to_fasta(w.families, tmp_path / "db.fa")
+    args = synth_args(tmp_path)
+    (tmp_path / "t.tsv").write_text("matrix\tword_size\txdrop_gap\txdrop_gap_final\ttb_wave_halves\n"
+                                    "m.matrix\t7\t30\t100\t-1\n")
+    o = run([ORACLE_BIN] + args + ["--num-threads", "16"], timeout=900)
+    assert o.returncode == 0, o.stderr
+    g = run([CLI] + args + ["--tuning-file", str(tmp_path / "t.tsv")], timeout=900)
+    assert g.returncode == 0, g.stderr
+    assert g.stdout == o.stdout and g.stdout.count("\n") > 1000
+
+
 def test_gappy_config_split_query_matches_oracle(tmp_path):
     w = synth.gappy_config(genome_len=4_200_000, n_families=60, n_copies=1700)
     synth.to_fasta(w.records, tmp_path / "q.fa")

@@ -7,7 +7,7 @@ from rmblast_amd import autotune
 
 def test_tuning_table_round_trip(tmp_path):
     path = str(tmp_path / "tuning.tsv")
-    a = {"batches": 4, "prelim_ring": 16, "prelim_wg_per_cu": 8, "tb_ring": 64, "tb_wg_per_cu": 7}
+    a = {"batches": 4, "prelim_ring": 16, "prelim_wg_per_cu": 8, "tb_ring": 64, "tb_wg_per_cu": 7, "tb_wave_halves": -1}
     b = dict(a, batches=12)
     autotune.write_entry(path, ("synth.matrix", 7, 30, 100), a, 812.5, "uneven 120 x 1500")
     autotune.write_entry(path, ("18p43g.matrix", 7, 30, 100), a, 700.0, "x")
