@@ -333,6 +333,34 @@ int rmb_stats(const rmb_ctx* ctx, uint64_t* out, int n);
 int rmb_tap_ungapped(rmb_ctx* ctx, const uint8_t* query, uint32_t q_len, uint32_t subject, uint32_t* out,
                      uint64_t* n);
 
+/* The query's lookup table, as build_query_lookup builds it (engine.rs:606,
+ * blast/nalookup.rs:128-181, blast/mblookup.rs:106-245) for a one-query
+ * batch: *lut_width, then CSR cells[4^lut_width + 1] (cell w's offsets are
+ * entries[cells[w] .. cells[w + 1])) and entries (offsets into the two-strand
+ * query buffer: plus-strand word starts below q_len + 1, minus-strand ones
+ * from there, in the order the scan visits them).  Two-call: with cells or
+ * entries NULL, or *n_cells / *n_entries too small, only the sizes are set. */
+int rmb_tap_lookup(rmb_ctx* ctx, const uint8_t* query, uint32_t q_len, int32_t* lut_width, uint32_t* cells,
+                   uint64_t* n_cells, uint32_t* entries, uint64_t* n_entries);
+
+/* The seeds the diagonal hash lets through for the query against subject
+ * `subject`, as search_with_query_lookup_seeds collects them
+ * (engine.rs:796-816; pushed at :1165 and :1237): per seed {q_off, s_off,
+ * strand (0 plus, 1 minus)}, the reference's SeedRecord, in its scan order.
+ * Two-call as rmb_tap_ungapped. */
+int rmb_tap_seeds(rmb_ctx* ctx, const uint8_t* query, uint32_t q_len, uint32_t subject, uint32_t* out,
+                  uint64_t* n);
+
+/* Phase 2a's preliminary HSPs for the query against subject `subject` (the
+ * PrelimHsp list run_phase2a returns, engine.rs:1317-1411 with the common-
+ * endpoint purge :1969-2083; the reference's search_phase2a,
+ * engine.rs:1775-1784, per query and subject): per prelim {q_seed, s_seed
+ * (the traceback seed), q_start, q_end, s_start, s_end, score, strand}, record
+ * frame, in Phase 2a's order.  Runs the whole search of that pair.  Two-call
+ * as rmb_tap_ungapped. */
+int rmb_tap_prelims(rmb_ctx* ctx, const uint8_t* query, uint32_t q_len, uint32_t subject, uint32_t* out,
+                    uint64_t* n);
+
 /* Score-only X-drop half alignment (gapped.rs:124-337, align_ex_score_only):
  * out = {score, a_len, b_len}. */
 int rmb_tap_align_score_only(rmb_ctx* ctx, const uint8_t* a, uint32_t m, const uint8_t* b, uint32_t n, int32_t xdrop,

@@ -239,3 +239,89 @@ void orc_format_scores(double evalue, double bits, char* e, char* b) {
   std::snprintf(b, 64, "%s", y.c_str());
 }
 }
+
+extern "C" {
+
+static QueryLookup lookup_of(const uint8_t* q, size_t qn, const Params& p, std::vector<uint8_t>& qv) {
+  qv.assign(q, q + qn);
+  qv.insert(qv.begin(), 15);
+  qv.push_back(15);
+  return build_query_lookup(qv, p);
+}
+
+// engine.rs:606 build_query_lookup of a raw BLASTNA query (no sentinels), dust
+// off: *lw, CSR cells (4^lw + 1, written when cells_cap suffices) and entries
+// (written when ent_cap suffices); returns the entry count.
+int64_t orc_lookup(const uint8_t* q, size_t qn, int word_size, int* lw, uint32_t* cells, size_t cells_cap,
+                   uint32_t* entries, size_t ent_cap) {
+  Params p;
+  p.word_size = word_size;
+  std::vector<uint8_t> qv;
+  QueryLookup ql = lookup_of(q, qn, p, qv);
+  *lw = ql.lut.lut_width;
+  if (cells && cells_cap >= ql.lut.cell_start.size())
+    std::memcpy(cells, ql.lut.cell_start.data(), ql.lut.cell_start.size() * sizeof(uint32_t));
+  if (entries && ent_cap >= ql.lut.entries.size())
+    std::memcpy(entries, ql.lut.entries.data(), ql.lut.entries.size() * sizeof(uint32_t));
+  return (int64_t)ql.lut.entries.size();
+}
+
+// engine.rs:796-816 search_with_query_lookup_seeds for one pair: 3 u32 per seed
+// {q_off, s_off, strand} in push order; returns the count.
+int orc_seeds(const uint8_t* q, size_t qn, const uint8_t* s, size_t sn, int word_size, int xdrop_ungap, int min_raw,
+              const int32_t* matrix, uint32_t* out, int cap) {
+  Matrix m = mat_from(matrix);
+  Params p;
+  p.word_size = word_size;
+  p.xdrop_ungap = xdrop_ungap;
+  p.min_raw_gapped_score = min_raw;
+  std::vector<uint8_t> qv, sv(s, s + sn);
+  QueryLookup ql = lookup_of(q, qn, p, qv);
+  sv.insert(sv.begin(), 15);
+  sv.push_back(15);
+  PreparedSubject ps = prepare_subject(sv);
+  std::vector<SeedRecord> seeds;
+  Taps t;
+  t.seeds = &seeds;
+  search_one_subject(ql, qv, revcomp(qv), "q", ps, "s", p, m, &t);
+  for (int i = 0; i < (int)seeds.size() && i < cap; ++i) {
+    out[3 * i] = seeds[i].q_off;
+    out[3 * i + 1] = seeds[i].s_off;
+    out[3 * i + 2] = seeds[i].strand;
+  }
+  return (int)seeds.size();
+}
+
+// Phase 2a's prelims for one pair (engine.rs:1317-1411 + purge :1969-2083): 8 u32
+// per prelim {q_seed, s_seed, q_start, q_end, s_start, s_end, score, strand};
+// returns the count.
+int orc_prelims(const uint8_t* q, size_t qn, const uint8_t* s, size_t sn, int word_size, int xdrop_ungap,
+                int xdrop_gap, int gap_open, int gap_extend, int min_raw, const int32_t* matrix, uint32_t* out,
+                int cap) {
+  Matrix m = mat_from(matrix);
+  Params p;
+  p.word_size = word_size;
+  p.xdrop_ungap = xdrop_ungap;
+  p.xdrop_gap = xdrop_gap;
+  p.gap_open = gap_open;
+  p.gap_extend = gap_extend;
+  p.min_raw_gapped_score = min_raw;
+  std::vector<uint8_t> qv, sv(s, s + sn);
+  QueryLookup ql = lookup_of(q, qn, p, qv);
+  sv.insert(sv.begin(), 15);
+  sv.push_back(15);
+  PreparedSubject ps = prepare_subject(sv);
+  std::vector<PrelimHsp> pre;
+  Taps t;
+  t.prelims = &pre;
+  search_one_subject(ql, qv, revcomp(qv), "q", ps, "s", p, m, &t);
+  for (int i = 0; i < (int)pre.size() && i < cap; ++i) {
+    const PrelimHsp& h = pre[i];
+    uint32_t* o = out + 8 * i;
+    o[0] = h.q_seed; o[1] = h.s_seed; o[2] = h.q_start; o[3] = h.q_end; o[4] = h.s_start; o[5] = h.s_end;
+    o[6] = (uint32_t)h.score; o[7] = h.strand;
+  }
+  return (int)pre.size();
+}
+
+}  // extern "C"

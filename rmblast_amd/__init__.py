@@ -24,6 +24,7 @@ EXPORTED = [
     "rmb_tap_align_score_only", "rmb_tap_align_ex", "rmb_ka_lookup", "rmb_ungapped_cutoff", "rmb_mask_level",
     "rmb_report_order", "rmb_report_tabular", "rmb_report_copy", "rmb_report_text", "rmb_align_stats_of",
     "rmb_ka_stats_resolve", "rmb_ka_search_space", "rmb_ka_evalue", "rmb_ctx_set_tuning",
+    "rmb_tap_lookup", "rmb_tap_seeds", "rmb_tap_prelims",
 ]
 
 
@@ -149,6 +150,12 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rmb_dust_intervals.argtypes = [u8p, ctypes.c_uint64, P(ctypes.c_uint64), P(ctypes.c_uint64)]
     lib.rmb_tap_ungapped.argtypes = [ctypes.c_void_p, u8p, ctypes.c_uint32, ctypes.c_uint32,
                                      P(ctypes.c_uint32), P(ctypes.c_uint64)]
+    lib.rmb_tap_seeds.argtypes = [ctypes.c_void_p, u8p, ctypes.c_uint32, ctypes.c_uint32, P(ctypes.c_uint32),
+                                  P(ctypes.c_uint64)]
+    lib.rmb_tap_prelims.argtypes = [ctypes.c_void_p, u8p, ctypes.c_uint32, ctypes.c_uint32, P(ctypes.c_uint32),
+                                    P(ctypes.c_uint64)]
+    lib.rmb_tap_lookup.argtypes = [ctypes.c_void_p, u8p, ctypes.c_uint32, P(ctypes.c_int32), P(ctypes.c_uint32),
+                                   P(ctypes.c_uint64), P(ctypes.c_uint32), P(ctypes.c_uint64)]
     lib.rmb_tap_align_score_only.argtypes = [ctypes.c_void_p, u8p, ctypes.c_uint32, u8p, ctypes.c_uint32,
                                              ctypes.c_int32, ctypes.c_int32, P(ctypes.c_int64)]
     lib.rmb_tap_align_ex.argtypes = [ctypes.c_void_p, u8p, ctypes.c_uint32, u8p, ctypes.c_uint32,
@@ -396,6 +403,39 @@ class Searcher:
                                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
                                                ctypes.byref(n)), "rmb_tap_ungapped")
         return out[:n.value]
+
+    def _tap_rows(self, fn, query, subject, width, what):
+        q = np.ascontiguousarray(query, dtype=np.uint8)
+        n = ctypes.c_uint64(0)
+        self._check(fn(self._ctx, _u8ptr(q), len(q), subject, None, ctypes.byref(n)), what)
+        out = np.zeros((max(n.value, 1), width), dtype=np.uint32)
+        self._check(fn(self._ctx, _u8ptr(q), len(q), subject, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                       ctypes.byref(n)), what)
+        return out[:n.value]
+
+    def tap_seeds(self, query, subject: int):
+        """Post-dedup seeds {q_off, s_off, strand} in scan order (rmb_tap_seeds)."""
+        return self._tap_rows(self._lib.rmb_tap_seeds, query, subject, 3, "rmb_tap_seeds")
+
+    def tap_prelims(self, query, subject: int):
+        """Phase 2a's preliminary HSPs {q_seed, s_seed, q_start, q_end, s_start,
+        s_end, score, strand} (rmb_tap_prelims)."""
+        return self._tap_rows(self._lib.rmb_tap_prelims, query, subject, 8, "rmb_tap_prelims")
+
+    def tap_lookup(self, query):
+        """The query's lookup table: (lut_width, cells[4^lw + 1], entries)."""
+        q = np.ascontiguousarray(query, dtype=np.uint8)
+        lw = ctypes.c_int32(0)
+        nc, ne = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        self._check(self._lib.rmb_tap_lookup(self._ctx, _u8ptr(q), len(q), ctypes.byref(lw), None, ctypes.byref(nc),
+                                             None, ctypes.byref(ne)), "rmb_tap_lookup")
+        cells = np.zeros(nc.value, dtype=np.uint32)
+        entries = np.zeros(max(ne.value, 1), dtype=np.uint32)
+        u32 = ctypes.POINTER(ctypes.c_uint32)
+        self._check(self._lib.rmb_tap_lookup(self._ctx, _u8ptr(q), len(q), ctypes.byref(lw), cells.ctypes.data_as(u32),
+                                             ctypes.byref(nc), entries.ctypes.data_as(u32), ctypes.byref(ne)),
+                    "rmb_tap_lookup")
+        return lw.value, cells, entries[:ne.value]
 
     def tap_align_score_only(self, a, b, xdrop, reverse):
         a = np.ascontiguousarray(a, dtype=np.uint8)

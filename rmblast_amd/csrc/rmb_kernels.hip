@@ -303,6 +303,10 @@ __device__ __forceinline__ void sort8(uint32_t (&v)[8]) {
 enum SeedFlag : uint8_t { F_VALID = 1, F_HIT = 2, F_EMIT = 4, F_MINUS = 8, F_LONG = 16 };
 constexpr uint32_t kOutChunk = 256;  // output slots a workgroup reserves at a time
 
+// SEEDS (the seeds tap): every seed the diagonal hash lets through is output
+// instead of the hits, as {item, diagonal, hash check offset, strand, its
+// ordinal in the item's scan}; the hash evolves exactly as in the search.
+template <bool SEEDS>
 __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
     P1Params P, const RecDesc* __restrict__ recs, const SubjDesc* __restrict__ subjs,
     const uint8_t* __restrict__ qfwd, const uint8_t* __restrict__ qrc, const uint8_t* __restrict__ qext,
@@ -410,6 +414,7 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
     // seeds (subject offset ascending, then cell order) are taken kSeedCap
     // per pass, so each window is scanned once.
     uint32_t w0 = 0, wk = 0, wtotal = 0;
+    uint64_t wseen = 0;  // seeds of the item's earlier windows (the seeds tap's ordinals)
     bool scanned = false;
     for (;;) {
       if (!scanned) {
@@ -648,7 +653,7 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
             const bool apply = eval & !to_slow;
             evals += apply ? 1u : 0u;
             stepsum += apply ? (e.w & 0xFFFFu) : 0u;
-            emit_mask |= (apply & (((e.w >> 16) & F_HIT) != 0)) ? (1u << j) : 0u;
+            emit_mask |= (apply & (SEEDS || ((e.w >> 16) & F_HIT) != 0)) ? (1u << j) : 0u;
             const uint32_t L = e.z;
             const bool w2 = apply & (u2 | (grow & (cnt_b == 2)));
             const bool w1 = apply & (u1 | (grow & (cnt_b == 1)));
@@ -696,7 +701,7 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
             if (sc < lev) continue;
             ++n_evals;
             n_steps += e.w & 0xFFFFu;
-            if ((e.w >> 16) & F_HIT) {
+            if (SEEDS || ((e.w >> 16) & F_HIT)) {
               sd[i].w = e.w | ((uint32_t)F_EMIT << 16);
               ++nemit_local;
             }
@@ -760,6 +765,21 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
         for (uint32_t i = tid; i < take; i += kP1Threads) {
           const uint4 e = sd[i];
           if (!((e.w >> 16) & F_EMIT)) continue;
+          if constexpr (SEEDS) {
+            const uint64_t ord = wseen + wk + i;
+            UngHit h{};
+            h.item = (uint32_t)item;
+            h.q_seed = e.x;
+            h.s_seed = e.y;
+            h.q_start = (uint32_t)ord;
+            h.q_end = (uint32_t)(ord >> 32);
+            h.strand = ((e.w >> 16) & F_MINUS) ? 1u : 0u;
+            const uint32_t k = atomicAdd(&s_emit_i, 1u);
+            const unsigned long long pos = k < s_pass_an ? s_pass_a + k : s_pass_b + (k - s_pass_an);
+            if (pos < P.out_cap) out[pos] = h;
+            else atomicOr(err, (uint32_t)E_OUT_FULL);
+            continue;
+          }
           const uint4 hv = my_hitres[i];
           const int32_t eq = (int32_t)hv.w;
           const bool minus = ((e.w >> 16) & F_MINUS) != 0;
@@ -804,6 +824,7 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
       wk += take;
       if (wk == wtotal) {
         w0 += kP1Threads;
+        if constexpr (SEEDS) wseen += wtotal;
         scanned = false;
       }
     }
@@ -1853,9 +1874,13 @@ hipError_t launch_phase1(int blocks, P1Params P, const RecDesc* recs, const Subj
                          const uint8_t* qrc, const uint8_t* qext, const uint8_t* splus, const uint8_t* src,
                          const uint32_t* cells, const uint32_t* entries, const int32_t* gmat, UngHit* out,
                          unsigned long long* out_count, uint2* spill, uint4* hitres, unsigned long long* queue,
-                         unsigned long long* counters, uint32_t* err, hipStream_t st) {
-  hipLaunchKernelGGL(k_phase1, dim3(blocks), dim3(kP1Threads), 0, st, P, recs, subjs, qfwd, qrc, qext, splus, src,
-                     cells, entries, gmat, out, out_count, spill, hitres, queue, counters, err);
+                         unsigned long long* counters, uint32_t* err, bool seeds, hipStream_t st) {
+  if (seeds)
+    hipLaunchKernelGGL(k_phase1<true>, dim3(blocks), dim3(kP1Threads), 0, st, P, recs, subjs, qfwd, qrc, qext, splus,
+                       src, cells, entries, gmat, out, out_count, spill, hitres, queue, counters, err);
+  else
+    hipLaunchKernelGGL(k_phase1<false>, dim3(blocks), dim3(kP1Threads), 0, st, P, recs, subjs, qfwd, qrc, qext, splus,
+                       src, cells, entries, gmat, out, out_count, spill, hitres, queue, counters, err);
   return hipGetLastError();
 }
 

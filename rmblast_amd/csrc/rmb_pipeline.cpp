@@ -48,7 +48,7 @@ hipError_t exclusive_sum_u32(void*, size_t&, const uint32_t*, uint32_t*, uint64_
 hipError_t launch_phase1(int, P1Params, const RecDesc*, const SubjDesc*, const uint8_t*, const uint8_t*,
                          const uint8_t*, const uint8_t*, const uint8_t*, const uint32_t*, const uint32_t*,
                          const int32_t*, UngHit*, unsigned long long*, uint2*, uint4*, unsigned long long*,
-                         unsigned long long*, uint32_t*, hipStream_t);
+                         unsigned long long*, uint32_t*, bool, hipStream_t);
 hipError_t noalloc_scratch_address(void**);
 size_t noalloc_max_halves();
 hipError_t launch_noalloc_score(void*, int, const DpTask*, uint32_t, const int32_t*, int32_t, int32_t, int32_t*,
@@ -276,6 +276,8 @@ struct rmb_ctx {
   std::vector<const uint8_t*> last_q;  // the last search's whole queries (wrapped) in h_qfwd, for the report
   std::vector<uint32_t> last_qlen;
   ReportText report;                   // text of the last rmb_report_tabular
+  bool tap_pre_on = false;             // the prelims tap collects Phase 2a's output into tap_pre
+  std::vector<PreRec> tap_pre;
   DevBuf b_qfwd, b_qrc, b_recs, b_elem, b_keys, b_keys2, b_vals, b_vals2, b_counts, b_cells, b_sort_tmp;
   DevBuf b_hits, b_hitcount, b_spill, b_hitres, b_queue, b_counters, b_err;
   DevBuf b_tasks, b_dpres, b_ring, b_ring_big, b_tbtasks, b_tbres, b_flat, b_rows, b_ops;
@@ -716,11 +718,17 @@ struct P1Batch {
   uint64_t redo_cap = 0;
 };
 
-// Per-group seeding launch: lookup table for the group's records, then
-// k_phase1 writing up to `cap` hits to dh (count at *hitcount).
-static void launch_group_phase1(rmb_ctx* c, hipStream_t st, P1Group& g, const uint8_t* d_qfwd, const uint8_t* d_qrc,
-                                const uint8_t* d_qext, UngHit* dh, uint64_t cap, unsigned long long* hitcount,
-                                unsigned long long* ctr, uint32_t* err, hipEvent_t ev_a, hipEvent_t ev_b) {
+// Lookup tables of a group's records (blast/nalookup.rs, blast/mblookup.rs):
+// one (record, word, offset) key per window of the two-strand query buffer,
+// radix-sorted, then per (record, word) cell the start of its offsets:
+// cells [R x 4^lw + 1] and entries (query offsets in the two-strand buffer,
+// the cell's order), both device arrays of the context.
+struct GroupLut {
+  const uint32_t* cells;
+  const uint32_t* entries;
+  uint64_t n_entries;  // windows, valid or not (the valid ones come first)
+};
+static GroupLut build_group_lut(rmb_ctx* c, hipStream_t st, P1Group& g, const uint8_t* d_qext) {
   const int ws = c->P.word_size, lw = g.lw;
   const uint32_t R = (uint32_t)g.recs_idx.size();
   const uint64_t cells = 1ull << (2 * lw);
@@ -749,6 +757,20 @@ static void launch_group_phase1(rmb_ctx* c, hipStream_t st, P1Group& g, const ui
   void* tmp = c->b_sort_tmp.get<uint8_t>(std::max(tmp1, tmp2));
   hip_check(sort_pairs_u64(tmp, tmp1, keys, keys2, vals, vals2, E, end_bit, st), "radix sort");
   hip_check(exclusive_sum_u32(tmp, tmp2, counts, cstart, ncell, st), "scan");
+  return GroupLut{cstart, vals2, E};
+}
+
+// Per-group seeding launch: lookup table for the group's records, then
+// k_phase1 writing up to `cap` hits to dh (count at *hitcount); with `seeds`,
+// the seeds the diagonal hash lets through instead of the hits.
+static void launch_group_phase1(rmb_ctx* c, hipStream_t st, P1Group& g, const uint8_t* d_qfwd, const uint8_t* d_qrc,
+                                const uint8_t* d_qext, UngHit* dh, uint64_t cap, unsigned long long* hitcount,
+                                unsigned long long* ctr, uint32_t* err, hipEvent_t ev_a, hipEvent_t ev_b,
+                                bool seeds = false) {
+  const int ws = c->P.word_size, lw = g.lw;
+  const uint32_t R = (uint32_t)g.recs_idx.size();
+  const GroupLut lut = build_group_lut(c, st, g, d_qext);
+  RecDesc* drecs = c->b_recs.get<RecDesc>(R);
   const uint32_t nsub = (uint32_t)c->subj.size();
   P1Params P{};
   P.word_size = ws;
@@ -779,8 +801,8 @@ static void launch_group_phase1(rmb_ctx* c, hipStream_t st, P1Group& g, const ui
   hip_check(hipMemsetAsync(queue, 0, kXcds * sizeof(unsigned long long), st), "memset queue");
   if (ev_a) hip_check(hipEventRecord(ev_a, st), "event");
   hip_check(launch_phase1(blocks, P, drecs, (const SubjDesc*)c->d_sdesc.p, d_qfwd, d_qrc, d_qext,
-                          (const uint8_t*)c->d_splus.p, (const uint8_t*)c->d_src.p, cstart, vals2, c->d_mat, dh,
-                          hitcount, spill, hitres, queue, ctr, err, st),
+                          (const uint8_t*)c->d_splus.p, (const uint8_t*)c->d_src.p, lut.cells, lut.entries, c->d_mat,
+                          dh, hitcount, spill, hitres, queue, ctr, err, seeds, st),
             "k_phase1");
   if (ev_b) hip_check(hipEventRecord(ev_b, st), "event");
 }
@@ -1185,6 +1207,8 @@ static void phase2b(rmb_ctx* c, const SearchRecs& sr, const std::vector<SeqView>
   const int32_t cutoff = std::max(c->P.min_raw_gapped_score, 1);
   Timer t3;
   uint64_t tb_marks[4] = {0, 0, 0, 0};  // sort, seeds + tasks, DP on the GPU, replay (us since t3)
+  if (c->tap_pre_on)  // the prelims tap: Phase 2a's output, before the traceback order
+    for (const ItemState& s : items) c->tap_pre.insert(c->tap_pre.end(), s.pre.begin(), s.pre.end());
   {
     struct TbRef { uint32_t item; uint32_t pre; uint32_t tq, ts; };
     std::vector<size_t> first(items.size() + 1, 0);
@@ -2057,60 +2081,78 @@ int rmb_stats(const rmb_ctx* c, uint64_t* out, int n) {
   return RMB_OK;
 }
 
+// The query as one record of a one-record group, uploaded (both strands and,
+// with DUST, the masked copy) for the stage taps.
+struct TapQuery {
+  P1Group g;
+  const uint8_t *d_qfwd = nullptr, *d_qrc = nullptr, *d_qext = nullptr;
+};
+static TapQuery tap_query(rmb_ctx* c, const uint8_t* query, uint32_t q_len) {
+  TapQuery t;
+  std::vector<uint8_t> qf = wrap(query, q_len), qr = revcomp_wrapped(qf);
+  uint8_t* d_qfwd = c->b_qfwd.get<uint8_t>(qf.size());
+  uint8_t* d_qrc = c->b_qrc.get<uint8_t>(qr.size());
+  hip_check(hipMemcpy(d_qfwd, qf.data(), qf.size(), hipMemcpyHostToDevice), "copy q");
+  hip_check(hipMemcpy(d_qrc, qr.data(), qr.size(), hipMemcpyHostToDevice), "copy qrc");
+  std::vector<RecDesc> recs(1);
+  recs[0].q_base = 0;
+  recs[0].q_len = q_len;
+  recs[0].cutoff = record_cutoff(c, q_len);
+  SearchRecs one;  // the query as one record, whatever its length
+  one.recs = recs;
+  one.query = {0};
+  one.offset = {0};
+  one.first = {0, 1};
+  one.whole_base = {0};
+  one.pool = qf.size();
+  t.d_qfwd = d_qfwd;
+  t.d_qrc = d_qrc;
+  t.d_qext = c->P.dust ? upload_dusted(c, {SeqView{qf.data(), qf.size()}}, one) : d_qfwd;
+  t.g.lw = lut_width_for(c->P.word_size, 2ull * q_len);
+  t.g.recs_idx = {0};
+  t.g.recs = recs;
+  t.g.elem_base = {0, 2ull * q_len + 1};
+  return t;
+}
+
+// k_phase1 over the query against every subject: its hits, or with `seeds`
+// its seed records, in the kernel's output order.
+static std::vector<UngHit> tap_phase1(rmb_ctx* c, TapQuery& t, uint32_t q_len, bool seeds) {
+  std::vector<UngHit> hits;
+  if (q_len < (uint32_t)c->P.word_size) return hits;
+  unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
+  hip_check(hipMemset(ctr, 0, C_NCOUNTERS * sizeof(unsigned long long)), "memset counters");
+  unsigned long long* hitcount = c->b_hitcount.get<unsigned long long>(1);
+  uint32_t* err = c->b_err.get<uint32_t>(1);
+  for (uint64_t cap = 1u << 16;;) {
+    UngHit* dh = c->b_hits.get<UngHit>(cap);
+    hip_check(hipMemsetAsync(hitcount, 0, sizeof(unsigned long long), c->st1), "memset hitcount");
+    hip_check(hipMemsetAsync(err, 0, sizeof(uint32_t), c->st1), "memset err");
+    launch_group_phase1(c, c->st1, t.g, t.d_qfwd, t.d_qrc, t.d_qext, dh, cap, hitcount, ctr, err, nullptr, nullptr,
+                        seeds);
+    unsigned long long nh = 0;
+    uint32_t e = 0;
+    hip_check(hipMemcpyAsync(&nh, hitcount, sizeof nh, hipMemcpyDeviceToHost, c->st1), "copy hitcount");
+    hip_check(hipMemcpyAsync(&e, err, sizeof e, hipMemcpyDeviceToHost, c->st1), "copy err");
+    hip_check(hipStreamSynchronize(c->st1), "tap sync");
+    if (e & E_HASH_FULL) throw Failure(RMB_ERR_CAPACITY, "diagonal hash bucket exceeded 256 cells");
+    if (nh > cap) {
+      cap = nh + nh / 4;
+      continue;
+    }
+    hits.resize(nh);
+    hip_check(hipMemcpy(hits.data(), dh, nh * sizeof(UngHit), hipMemcpyDeviceToHost), "copy hits");
+    return hits;
+  }
+}
+
 int rmb_tap_ungapped(rmb_ctx* c, const uint8_t* query, uint32_t q_len, uint32_t subject, uint32_t* out, uint64_t* n) {
   if (!c || !query || !n) return RMB_ERR_ARG;
   return guarded(c, [&]() {
     if (subject >= c->subj.size()) throw Failure(RMB_ERR_ARG, "subject index out of range");
     hip_check(hipSetDevice(c->device), "hipSetDevice");
-    std::vector<uint8_t> qf = wrap(query, q_len), qr = revcomp_wrapped(qf);
-    uint8_t* d_qfwd = c->b_qfwd.get<uint8_t>(qf.size());
-    uint8_t* d_qrc = c->b_qrc.get<uint8_t>(qr.size());
-    hip_check(hipMemcpy(d_qfwd, qf.data(), qf.size(), hipMemcpyHostToDevice), "copy q");
-    hip_check(hipMemcpy(d_qrc, qr.data(), qr.size(), hipMemcpyHostToDevice), "copy qrc");
-    std::vector<RecDesc> recs(1);
-    recs[0].q_base = 0;
-    recs[0].q_len = q_len;
-    recs[0].cutoff = record_cutoff(c, q_len);
-    unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
-    hip_check(hipMemset(ctr, 0, C_NCOUNTERS * sizeof(unsigned long long)), "memset counters");
-    const int lw = lut_width_for(c->P.word_size, 2ull * q_len);
-    SearchRecs one;  // the query as one record, whatever its length
-    one.recs = recs;
-    one.query = {0};
-    one.offset = {0};
-    one.first = {0, 1};
-    one.whole_base = {0};
-    one.pool = qf.size();
-    const uint8_t* d_qext = c->P.dust ? upload_dusted(c, {SeqView{qf.data(), qf.size()}}, one) : d_qfwd;
-    std::vector<UngHit> hits;
-    if (q_len >= (uint32_t)c->P.word_size) {
-      P1Group g;
-      g.lw = lw;
-      g.recs_idx = {0};
-      g.recs = recs;
-      g.elem_base = {0, 2ull * q_len + 1};
-      unsigned long long* hitcount = c->b_hitcount.get<unsigned long long>(1);
-      uint32_t* err = c->b_err.get<uint32_t>(1);
-      for (uint64_t cap = 1u << 16;;) {
-        UngHit* dh = c->b_hits.get<UngHit>(cap);
-        hip_check(hipMemsetAsync(hitcount, 0, sizeof(unsigned long long), c->st1), "memset hitcount");
-        hip_check(hipMemsetAsync(err, 0, sizeof(uint32_t), c->st1), "memset err");
-        launch_group_phase1(c, c->st1, g, d_qfwd, d_qrc, d_qext, dh, cap, hitcount, ctr, err, nullptr, nullptr);
-        unsigned long long nh = 0;
-        uint32_t e = 0;
-        hip_check(hipMemcpyAsync(&nh, hitcount, sizeof nh, hipMemcpyDeviceToHost, c->st1), "copy hitcount");
-        hip_check(hipMemcpyAsync(&e, err, sizeof e, hipMemcpyDeviceToHost, c->st1), "copy err");
-        hip_check(hipStreamSynchronize(c->st1), "tap sync");
-        if (e & E_HASH_FULL) throw Failure(RMB_ERR_CAPACITY, "diagonal hash bucket exceeded 256 cells");
-        if (nh > cap) {
-          cap = nh + nh / 4;
-          continue;
-        }
-        hits.resize(nh);
-        hip_check(hipMemcpy(hits.data(), dh, nh * sizeof(UngHit), hipMemcpyDeviceToHost), "copy hits");
-        break;
-      }
-    }
+    TapQuery t = tap_query(c, query, q_len);
+    const std::vector<UngHit> hits = tap_phase1(c, t, q_len, false);
     std::vector<UngRec> v;
     for (auto& h : hits)
       if (h.item != kNoItem && h.item % c->subj.size() == subject)  // skip reserved slots left unused
@@ -2125,6 +2167,96 @@ int rmb_tap_ungapped(rmb_ctx* c, const uint8_t* query, uint32_t q_len, uint32_t 
       }
     }
     *n = v.size();
+  });
+}
+
+int rmb_tap_seeds(rmb_ctx* c, const uint8_t* query, uint32_t q_len, uint32_t subject, uint32_t* out, uint64_t* n) {
+  if (!c || !query || !n) return RMB_ERR_ARG;
+  return guarded(c, [&]() {
+    if (subject >= c->subj.size()) throw Failure(RMB_ERR_ARG, "subject index out of range");
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    TapQuery t = tap_query(c, query, q_len);
+    const std::vector<UngHit> recs = tap_phase1(c, t, q_len, true);
+    // {ordinal, diagonal, check offset, strand} of the subject's seeds, in scan order
+    struct S { uint64_t ord; int32_t diag; uint32_t chk, minus; };
+    std::vector<S> v;
+    for (auto& h : recs)
+      if (h.item != kNoItem && h.item % c->subj.size() == subject)
+        v.push_back(S{(uint64_t)h.q_start | ((uint64_t)h.q_end << 32), (int32_t)h.q_seed, h.s_seed, h.strand});
+    std::sort(v.begin(), v.end(), [](const S& a, const S& b) { return a.ord < b.ord; });
+    if (out) {
+      const uint64_t cap = *n;
+      const int64_t ws = c->P.word_size;
+      for (size_t i = 0; i < v.size() && i < cap; ++i) {
+        uint32_t* o = out + 3 * i;
+        // plus: the extended word's (q, s) on diagonal s - q; minus (engine.rs:1223-1236): the
+        // forward-query offset less the extension still needed, and the forward-subject offset
+        const int64_t q = v[i].minus ? 2 * (int64_t)q_len + 1 - ws + v[i].diag - (int64_t)v[i].chk
+                                     : (int64_t)(int32_t)(v[i].chk - (uint32_t)v[i].diag);
+        o[0] = (uint32_t)std::max<int64_t>(q, 0);
+        o[1] = v[i].chk;
+        o[2] = v[i].minus;
+      }
+    }
+    *n = v.size();
+  });
+}
+
+int rmb_tap_lookup(rmb_ctx* c, const uint8_t* query, uint32_t q_len, int32_t* lut_width, uint32_t* cells,
+                   uint64_t* n_cells, uint32_t* entries, uint64_t* n_entries) {
+  if (!c || !query || !lut_width || !n_cells || !n_entries) return RMB_ERR_ARG;
+  return guarded(c, [&]() {
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    TapQuery t = tap_query(c, query, q_len);
+    *lut_width = t.g.lw;
+    const uint64_t nc = (1ull << (2 * t.g.lw)) + 1;
+    if (q_len < (uint32_t)c->P.word_size) {  // no table (engine.rs:751-753): every cell empty
+      if (cells && *n_cells >= nc) std::fill(cells, cells + nc, 0u);
+      *n_cells = nc;
+      *n_entries = 0;
+      return;
+    }
+    const GroupLut lut = build_group_lut(c, c->st1, t.g, t.d_qext);
+    std::vector<uint32_t> hc(nc);
+    hip_check(hipMemcpyAsync(hc.data(), lut.cells, nc * sizeof(uint32_t), hipMemcpyDeviceToHost, c->st1),
+              "copy cells");
+    hip_check(hipStreamSynchronize(c->st1), "lookup sync");
+    const uint64_t ne = hc[nc - 1];  // valid windows
+    if (cells && *n_cells >= nc) std::copy(hc.begin(), hc.end(), cells);
+    if (entries && *n_entries >= ne && ne)
+      hip_check(hipMemcpy(entries, lut.entries, ne * sizeof(uint32_t), hipMemcpyDeviceToHost), "copy entries");
+    *n_cells = nc;
+    *n_entries = ne;
+  });
+}
+
+int rmb_tap_prelims(rmb_ctx* c, const uint8_t* query, uint32_t q_len, uint32_t subject, uint32_t* out, uint64_t* n) {
+  if (!c || !query || !n) return RMB_ERR_ARG;
+  return guarded(c, [&]() {
+    if (subject >= c->subj.size()) throw Failure(RMB_ERR_ARG, "subject index out of range");
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    const uint8_t* qs[1] = {query};
+    const uint32_t ls[1] = {q_len};
+    const std::vector<uint64_t> items{(uint64_t)subject};
+    c->tap_pre.clear();
+    c->tap_pre_on = true;
+    try {
+      search_or_drain(c, [&] { search_impl(c, 1, qs, ls, &items); });
+    } catch (...) {
+      c->tap_pre_on = false;
+      throw;
+    }
+    c->tap_pre_on = false;
+    if (out) {
+      const uint64_t cap = *n;
+      for (size_t i = 0; i < c->tap_pre.size() && i < cap; ++i) {
+        const PreRec& h = c->tap_pre[i];
+        uint32_t* o = out + 8 * i;
+        o[0] = h.q_seed; o[1] = h.s_seed; o[2] = h.q_start; o[3] = h.q_end;
+        o[4] = h.s_start; o[5] = h.s_end; o[6] = (uint32_t)h.score; o[7] = h.minus;
+      }
+    }
+    *n = c->tap_pre.size();
   });
 }
 

@@ -90,6 +90,14 @@ def oracle_lib():
     L.orc_merge_chunk_prelims.argtypes = [P(ctypes.c_uint32), ctypes.c_int, P(ctypes.c_uint32), ctypes.c_int,
                                           ctypes.c_uint32, P(ctypes.c_uint32)]
     L.orc_merge_chunk_prelims.restype = ctypes.c_int
+    L.orc_lookup.argtypes = [P(ctypes.c_uint8), ctypes.c_size_t, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_uint32),
+                             ctypes.c_size_t, P(ctypes.c_uint32), ctypes.c_size_t]
+    L.orc_lookup.restype = ctypes.c_int64
+    L.orc_seeds.argtypes = [P(ctypes.c_uint8), ctypes.c_size_t, P(ctypes.c_uint8), ctypes.c_size_t, ctypes.c_int,
+                            ctypes.c_int, ctypes.c_int, P(ctypes.c_int32), P(ctypes.c_uint32), ctypes.c_int]
+    L.orc_prelims.argtypes = [P(ctypes.c_uint8), ctypes.c_size_t, P(ctypes.c_uint8), ctypes.c_size_t, ctypes.c_int,
+                              ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_int32),
+                              P(ctypes.c_uint32), ctypes.c_int]
     return L
 
 
