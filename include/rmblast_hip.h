@@ -333,6 +333,14 @@ int rmb_stats(const rmb_ctx* ctx, uint64_t* out, int n);
 int rmb_tap_ungapped(rmb_ctx* ctx, const uint8_t* query, uint32_t q_len, uint32_t subject, uint32_t* out,
                      uint64_t* n);
 
+/* Subject `subject` as the search scans it, from the device: the plus strand
+ * with its ambiguity codes resolved by makeblastdb's generator (seeded with
+ * the length, one draw per ambiguous base 5' to 3'; seq/twobit.rs:50-134,
+ * seq/mod.rs:155-290 get_full_sequence_blastna) and its reverse complement,
+ * each *n bases (no sentinels).  Two-call: with plus and rc NULL, or *n too
+ * small, only the length is set. */
+int rmb_tap_subject(const rmb_ctx* ctx, uint32_t subject, uint8_t* plus, uint8_t* rc, uint32_t* n);
+
 /* The query's lookup table, as build_query_lookup builds it (engine.rs:606,
  * blast/nalookup.rs:128-181, blast/mblookup.rs:106-245) for a one-query
  * batch: *lut_width, then CSR cells[4^lut_width + 1] (cell w's offsets are

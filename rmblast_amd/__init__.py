@@ -24,7 +24,7 @@ EXPORTED = [
     "rmb_tap_align_score_only", "rmb_tap_align_ex", "rmb_ka_lookup", "rmb_ungapped_cutoff", "rmb_mask_level",
     "rmb_report_order", "rmb_report_tabular", "rmb_report_copy", "rmb_report_text", "rmb_align_stats_of",
     "rmb_ka_stats_resolve", "rmb_ka_search_space", "rmb_ka_evalue", "rmb_ctx_set_tuning",
-    "rmb_tap_lookup", "rmb_tap_seeds", "rmb_tap_prelims",
+    "rmb_tap_lookup", "rmb_tap_seeds", "rmb_tap_prelims", "rmb_tap_subject",
 ]
 
 
@@ -150,6 +150,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rmb_dust_intervals.argtypes = [u8p, ctypes.c_uint64, P(ctypes.c_uint64), P(ctypes.c_uint64)]
     lib.rmb_tap_ungapped.argtypes = [ctypes.c_void_p, u8p, ctypes.c_uint32, ctypes.c_uint32,
                                      P(ctypes.c_uint32), P(ctypes.c_uint64)]
+    lib.rmb_tap_subject.argtypes = [ctypes.c_void_p, ctypes.c_uint32, u8p, u8p, P(ctypes.c_uint32)]
     lib.rmb_tap_seeds.argtypes = [ctypes.c_void_p, u8p, ctypes.c_uint32, ctypes.c_uint32, P(ctypes.c_uint32),
                                   P(ctypes.c_uint64)]
     lib.rmb_tap_prelims.argtypes = [ctypes.c_void_p, u8p, ctypes.c_uint32, ctypes.c_uint32, P(ctypes.c_uint32),
@@ -412,6 +413,17 @@ class Searcher:
         self._check(fn(self._ctx, _u8ptr(q), len(q), subject, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
                        ctypes.byref(n)), what)
         return out[:n.value]
+
+    def tap_subject(self, subject: int):
+        """Subject `subject` as the search scans it: (resolved plus strand, its
+        reverse complement), from the device (rmb_tap_subject)."""
+        n = ctypes.c_uint32(0)
+        self._check(self._lib.rmb_tap_subject(self._ctx, subject, None, None, ctypes.byref(n)), "rmb_tap_subject")
+        plus = np.zeros(max(n.value, 1), dtype=np.uint8)
+        rc = np.zeros(max(n.value, 1), dtype=np.uint8)
+        self._check(self._lib.rmb_tap_subject(self._ctx, subject, _u8ptr(plus), _u8ptr(rc), ctypes.byref(n)),
+                    "rmb_tap_subject")
+        return plus[:n.value], rc[:n.value]
 
     def tap_seeds(self, query, subject: int):
         """Post-dedup seeds {q_off, s_off, strand} in scan order (rmb_tap_seeds)."""

@@ -18,7 +18,7 @@ COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + os.path.join(ROOT, "include"),
 ]
-LIB_SOURCES = ["rmb_kernels.hip", "rmb_pipeline.cpp", "rmb_replay.cpp", "rmb_dust.cpp", "rmb_ka.cpp", "rmb_report.cpp", "triage_kernels.hip",
+LIB_SOURCES = ["rmb_kernels.hip", "rmb_prep.hip", "rmb_pipeline.cpp", "rmb_replay.cpp", "rmb_dust.cpp", "rmb_ka.cpp", "rmb_report.cpp", "triage_kernels.hip",
                "triage.cpp"]
 
 

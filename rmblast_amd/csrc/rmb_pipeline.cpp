@@ -50,6 +50,10 @@ hipError_t launch_phase1(int, P1Params, const RecDesc*, const SubjDesc*, const u
                          const int32_t*, UngHit*, unsigned long long*, uint2*, uint4*, unsigned long long*,
                          unsigned long long*, uint32_t*, bool, hipStream_t);
 hipError_t noalloc_scratch_address(void**);
+hipError_t launch_resolve_ambiguity(uint8_t*, const SubjDesc*, uint32_t, hipStream_t);
+hipError_t launch_revcomp_segments(const uint8_t*, uint8_t*, const uint64_t*, const uint32_t*, const uint2*, uint32_t,
+                                   hipStream_t);
+uint32_t revcomp_chunk();
 size_t noalloc_max_halves();
 hipError_t launch_noalloc_score(void*, int, const DpTask*, uint32_t, const int32_t*, int32_t, int32_t, int32_t*,
                                 uint32_t*, hipStream_t);
@@ -184,40 +188,8 @@ static int lut_width_for(int ws, size_t e) {
   }
 }
 
-// makeblastdb's CRandom lagged-Fibonacci generator, seeded by the sequence
-// length (seq/twobit.rs:50-134): one draw per ambiguous base, 5' to 3'.
-class AmbiguityResolver {
- public:
-  explicit AmbiguityResolver(uint32_t seed) {
-    st_[0] = seed;
-    for (int i = 1; i < 33; ++i) st_[i] = 1103515245u * st_[i - 1] + 12345u;
-    for (int i = 0; i < 330; ++i) next();
-  }
-  uint8_t base_for(uint8_t code) {
-    static const uint8_t sets[16][4] = {{0}, {0}, {0}, {0}, {2, 0, 2}, {2, 1, 3}, {2, 0, 1}, {2, 2, 3},
-                                       {2, 0, 3}, {2, 1, 2}, {3, 1, 2, 3}, {3, 0, 2, 3}, {3, 0, 1, 3},
-                                       {3, 0, 1, 2}, {4}, {0}};
-    const uint32_t r = next() >> 1;
-    if (code == 14) return (uint8_t)(r & 3);
-    return sets[code][1 + r % sets[code][0]];
-  }
-
- private:
-  uint32_t st_[33];
-  int j_ = 12, k_ = 32;
-  uint32_t next() {
-    const uint32_t r = st_[k_] + st_[j_];
-    st_[k_] = r;
-    --k_;
-    --j_;
-    if (k_ < 0) k_ = 32;
-    else if (j_ < 0) j_ = 32;
-    return r;
-  }
-};
-
-struct Subject {
-  std::vector<uint8_t> plus, rc, align, align_rc;  // sentinel-wrapped
+struct Subject {  // host views; the resolved strands live on the device only
+  std::vector<uint8_t> align, align_rc;  // sentinel-wrapped original codes and their reverse complement
   std::vector<uint8_t> n_mask, n_mask_rc;
 };
 
@@ -272,7 +244,7 @@ struct rmb_ctx {
   uint64_t host_prelim_dps = 0;  // preliminary DPs of the last search run through run_dp_score
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // scratch
-  HostBuf h_qfwd, h_qrc;  // host query pools of the last search
+  HostBuf h_qfwd;  // host forward query pool of the last search
   std::vector<const uint8_t*> last_q;  // the last search's whole queries (wrapped) in h_qfwd, for the report
   std::vector<uint32_t> last_qlen;
   ReportText report;                   // text of the last rmb_report_tabular
@@ -283,6 +255,7 @@ struct rmb_ctx {
   DevBuf b_tasks, b_dpres, b_ring, b_ring_big, b_tbtasks, b_tbres, b_flat, b_rows, b_ops;
   DevBuf b_pre, b_redo, b_precount, b_packtasks, b_packed, b_items, b_qext, b_recids;
   DevBuf b_arena, b_arena_big, b_arena_wave, b_oppool, b_opres, b_opcount;  // one-pass traceback
+  DevBuf b_seg_base, b_seg_len, b_seg_work;  // reverse-complement segments of a pool
 };
 
 namespace rmb {
@@ -760,6 +733,26 @@ static GroupLut build_group_lut(rmb_ctx* c, hipStream_t st, P1Group& g, const ui
   return GroupLut{cstart, vals2, E};
 }
 
+// The reverse complement of every sentinel-wrapped segment (base, bases) of
+// the device pool `in` into `out`, on stream st (k_revcomp_segments).
+static void device_revcomp(rmb_ctx* c, hipStream_t st, const uint8_t* in, uint8_t* out,
+                           const std::vector<uint64_t>& base, const std::vector<uint32_t>& len) {
+  const uint32_t chunk = revcomp_chunk();
+  std::vector<uint2> work;
+  for (uint32_t s = 0; s < (uint32_t)base.size(); ++s)
+    for (uint64_t q = 0; q < (uint64_t)len[s] + 2; q += chunk) work.push_back(make_uint2(s, (uint32_t)q));
+  if (work.empty()) return;
+  uint64_t* db = c->b_seg_base.get<uint64_t>(base.size());
+  uint32_t* dl = c->b_seg_len.get<uint32_t>(len.size());
+  uint2* dw = c->b_seg_work.get<uint2>(work.size());
+  hip_check(hipMemcpyAsync(db, base.data(), base.size() * sizeof(uint64_t), hipMemcpyHostToDevice, st), "copy segs");
+  hip_check(hipMemcpyAsync(dl, len.data(), len.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st), "copy segs");
+  hip_check(hipMemcpyAsync(dw, work.data(), work.size() * sizeof(uint2), hipMemcpyHostToDevice, st), "copy work");
+  hip_check(launch_revcomp_segments(in, out, db, dl, dw, (uint32_t)work.size(), st), "k_revcomp_segments");
+  // the host vectors are read by the queued copies: wait for them
+  hip_check(hipStreamSynchronize(st), "revcomp sync");
+}
+
 // Per-group seeding launch: lookup table for the group's records, then
 // k_phase1 writing up to `cap` hits to dh (count at *hitcount); with `seeds`,
 // the seeds the diagonal hash lets through instead of the hits.
@@ -1139,7 +1132,7 @@ static std::vector<ItemState> phase2a(rmb_ctx* c, const SearchRecs& sr, std::vec
   }
   parallel_for(items.size(), [&](size_t k) {
     ItemState& s = items[k];
-    const uint32_t ql = recs[s.item / nsub].q_len, sl = (uint32_t)c->subj[s.item % nsub].plus.size() - 2;
+    const uint32_t ql = recs[s.item / nsub].q_len, sl = c->sdesc[s.item % nsub].s_len;
     std::vector<UngRec> u(s.surv.size());
     for (size_t i = 0; i < s.surv.size(); ++i) {
       const UngHit& h = survivors[s.surv[i]].u;
@@ -1436,16 +1429,14 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
   uint8_t* d_qfwd = c->b_qfwd.get<uint8_t>(sr.pool);
   uint8_t* d_qrc = c->b_qrc.get<uint8_t>(sr.pool);
   {
-    // pinned pools filled in parallel from the caller's buffers, uploaded on
-    // the seeding stream (its first kernels queue behind the copies)
+    // the forward pool filled in parallel from the caller's buffers (pinned),
+    // uploaded on the seeding stream; the reverse-complement pool is built
+    // from it on the device, and the first kernels queue behind both
     uint8_t* pf = c->h_qfwd.get(sr.pool);
-    uint8_t* pr = c->h_qrc.get(sr.pool);
     auto put = [&](uint64_t base, const uint8_t* bases, size_t n) {  // n bases between two sentinels
       uint8_t* f = pf + base;
-      uint8_t* r = pr + base;
-      f[0] = r[0] = f[n + 1] = r[n + 1] = 15;
+      f[0] = f[n + 1] = 15;
       for (size_t j = 0; j < n; ++j) f[1 + j] = bases[j] & 15;
-      for (size_t j = 0; j < n; ++j) r[1 + j] = kBlastnaComplement[bases[n - 1 - j] & 15];
     };
     parallel_for(nq, [&](size_t i) {
       put(sr.whole_base[i], qs[i], lens[i]);
@@ -1458,7 +1449,18 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
     c->last_qlen.assign(lens, lens + nq);
     for (uint32_t i = 0; i < nq; ++i) c->last_q[i] = qf[i].data();
     hip_check(hipMemcpyAsync(d_qfwd, pf, sr.pool, hipMemcpyHostToDevice, c->st1), "copy qfwd");
-    hip_check(hipMemcpyAsync(d_qrc, pr, sr.pool, hipMemcpyHostToDevice, c->st1), "copy qrc");
+    std::vector<uint64_t> sb;
+    std::vector<uint32_t> sl;
+    for (uint32_t i = 0; i < nq; ++i) {
+      sb.push_back(sr.whole_base[i]);
+      sl.push_back(lens[i]);
+      if (sr.first[i + 1] - sr.first[i] > 1)
+        for (uint32_t k = sr.first[i]; k < sr.first[i + 1]; ++k) {
+          sb.push_back(recs[k].q_base);
+          sl.push_back(recs[k].q_len);
+        }
+    }
+    device_revcomp(c, c->st1, d_qfwd, d_qrc, sb, sl);
   }
   // With DUST on, the lookup tables and the exact word extension read the
   // masked query; ungapped and gapped extension read the query itself
@@ -1749,7 +1751,6 @@ int rmb_ctx_destroy(rmb_ctx* c) {
                     &c->b_recids, &c->b_arena, &c->b_arena_big, &c->b_arena_wave, &c->b_oppool, &c->b_opres, &c->b_opcount})
     b->release();
   c->h_qfwd.release();
-  c->h_qrc.release();
   for (P1Slot& S : c->slot) {
     for (DevBuf* b : {&S.pre, &S.redo_hits, &S.cnt, &S.hitcnt, &S.ctr, &S.err}) b->release();
     for (hipEvent_t e : S.ev) (void)hipEventDestroy(e);
@@ -1799,23 +1800,24 @@ int rmb_library_load(rmb_ctx* c, uint32_t n, const uint8_t* const* seqs, const u
     uint64_t off = 0;
     for (uint32_t i = 0; i < n; ++i) {
       if (lens[i] > 0x7FFFFFFFu) throw Failure(RMB_ERR_ARG, "subject longer than 2^31-1 bases");
+      c->sdesc[i].s_base = off;
+      c->sdesc[i].s_len = lens[i];
+      off += lens[i] + 2;
+    }
+    // host views: the original codes, their reverse complement and the
+    // ambiguity masks (extract_aligned restores them)
+    std::vector<uint8_t> pa(off);
+    parallel_for(n, [&](size_t i) {
       Subject& S = c->subj[i];
-      std::vector<uint8_t> raw = wrap(seqs[i], lens[i]);
-      S.plus = raw;
-      AmbiguityResolver* rng = nullptr;
-      std::vector<AmbiguityResolver> holder;
+      S.align = wrap(seqs[i], lens[i]);
+      S.align_rc = revcomp_wrapped(S.align);
       for (uint32_t k = 0; k < lens[i]; ++k) {
-        const uint8_t code = raw[1 + k];
+        const uint8_t code = S.align[1 + k];
         if (code >= 4 && code <= 14) {
-          if (!rng) { holder.emplace_back(lens[i]); rng = &holder.back(); }
-          S.plus[1 + k] = rng->base_for(code);
           if (S.n_mask.empty()) S.n_mask.assign(lens[i], 0);
           S.n_mask[k] = code;
         }
       }
-      S.rc = revcomp_wrapped(S.plus);
-      S.align = raw;  // original codes (the plus strand with ambiguity restored)
-      S.align_rc = revcomp_wrapped(S.align);
       if (!S.n_mask.empty()) {
         S.n_mask_rc.resize(lens[i]);
         for (uint32_t k = 0; k < lens[i]; ++k) {
@@ -1823,23 +1825,27 @@ int rmb_library_load(rmb_ctx* c, uint32_t n, const uint8_t* const* seqs, const u
           S.n_mask_rc[k] = code ? kBlastnaComplement[code] : 0;
         }
       }
-      c->sdesc[i].s_base = off;
-      c->sdesc[i].s_len = lens[i];
-      off += lens[i] + 2;
-    }
+      std::memcpy(pa.data() + c->sdesc[i].s_base, S.align.data(), S.align.size());
+    });
     c->db_letters = off - 2ull * n;
     c->avg_subj_len = c->db_letters / n;
-    std::vector<uint8_t> pp(off), pr(off), pa(off);
-    for (uint32_t i = 0; i < n; ++i) {
-      std::memcpy(pp.data() + c->sdesc[i].s_base, c->subj[i].plus.data(), c->subj[i].plus.size());
-      std::memcpy(pr.data() + c->sdesc[i].s_base, c->subj[i].rc.data(), c->subj[i].rc.size());
-      std::memcpy(pa.data() + c->sdesc[i].s_base, c->subj[i].align.data(), c->subj[i].align.size());
-    }
-    hip_check(hipMemcpy(c->d_splus.get<uint8_t>(off), pp.data(), off, hipMemcpyHostToDevice), "copy subjects");
-    hip_check(hipMemcpy(c->d_src.get<uint8_t>(off), pr.data(), off, hipMemcpyHostToDevice), "copy subjects rc");
-    hip_check(hipMemcpy(c->d_salign.get<uint8_t>(off), pa.data(), off, hipMemcpyHostToDevice), "copy subjects align");
+    // the device builds the scanned strands from the one upload: the plus
+    // strand with its ambiguity codes resolved, and its reverse complement
+    uint8_t* dal = c->d_salign.get<uint8_t>(off);
+    uint8_t* dpl = c->d_splus.get<uint8_t>(off);
+    uint8_t* drc = c->d_src.get<uint8_t>(off);
+    hip_check(hipMemcpy(dal, pa.data(), off, hipMemcpyHostToDevice), "copy subjects");
     hip_check(hipMemcpy(c->d_sdesc.get<SubjDesc>(n), c->sdesc.data(), n * sizeof(SubjDesc), hipMemcpyHostToDevice),
               "copy subject table");
+    hip_check(hipMemcpyAsync(dpl, dal, off, hipMemcpyDeviceToDevice, c->st), "copy plus");
+    hip_check(launch_resolve_ambiguity(dpl, (const SubjDesc*)c->d_sdesc.p, n, c->st), "k_resolve_ambiguity");
+    std::vector<uint64_t> sb(n);
+    std::vector<uint32_t> sl(n);
+    for (uint32_t i = 0; i < n; ++i) {
+      sb[i] = c->sdesc[i].s_base;
+      sl[i] = lens[i];
+    }
+    device_revcomp(c, c->st, dpl, drc, sb, sl);
   });
 }
 
@@ -2157,7 +2163,7 @@ int rmb_tap_ungapped(rmb_ctx* c, const uint8_t* query, uint32_t q_len, uint32_t 
     for (auto& h : hits)
       if (h.item != kNoItem && h.item % c->subj.size() == subject)  // skip reserved slots left unused
         v.push_back(UngRec{h.q_seed, h.s_seed, h.q_start, h.q_end, h.s_start, h.s_end, h.score, (uint8_t)h.strand});
-    order_ungapped(v, q_len, (uint32_t)c->subj[subject].plus.size() - 2);
+    order_ungapped(v, q_len, c->sdesc[subject].s_len);
     if (out) {
       const uint64_t cap = *n;
       for (size_t i = 0; i < v.size() && i < cap; ++i) {
@@ -2199,6 +2205,20 @@ int rmb_tap_seeds(rmb_ctx* c, const uint8_t* query, uint32_t q_len, uint32_t sub
       }
     }
     *n = v.size();
+  });
+}
+
+int rmb_tap_subject(const rmb_ctx* c, uint32_t subject, uint8_t* plus, uint8_t* rc, uint32_t* n) {
+  if (!c || !n) return RMB_ERR_ARG;
+  return guarded(const_cast<rmb_ctx*>(c), [&]() {
+    if (subject >= c->subj.size()) throw Failure(RMB_ERR_ARG, "subject index out of range");
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    const SubjDesc& S = c->sdesc[subject];
+    if (plus && *n >= S.s_len && S.s_len)
+      hip_check(hipMemcpy(plus, (const uint8_t*)c->d_splus.p + S.s_base + 1, S.s_len, hipMemcpyDeviceToHost), "copy plus");
+    if (rc && *n >= S.s_len && S.s_len)
+      hip_check(hipMemcpy(rc, (const uint8_t*)c->d_src.p + S.s_base + 1, S.s_len, hipMemcpyDeviceToHost), "copy rc");
+    *n = S.s_len;
   });
 }
 

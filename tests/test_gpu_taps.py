@@ -4,7 +4,9 @@ megablast-width tables, ambiguity codes), the post-dedup seeds
 (rmb_tap_seeds: content and scan order against the oracle; the reference's
 chr22 dedup-seeds golden, rmblast-lib/tests/dedup_seeds_regression.rs, on
 the two chunks of the committed 3 Mbp fixture) and Phase 2a's preliminary
-HSPs (rmb_tap_prelims: content and order)."""
+HSPs (rmb_tap_prelims: content and order); the library as the device
+prepares it (rmb_tap_subject: ambiguity codes resolved by makeblastdb's
+generator, reverse complement) against the oracle's generator."""
 import ctypes
 import gzip
 
@@ -110,3 +112,44 @@ def test_tap_prelims_match_oracle_in_order(oracle_lib, ws, gaps):
         assert np.array_equal(got, out[:n]), (ws, f, len(got), n)
         seen += n
     assert seen > 20
+
+
+COMPLEMENT = np.array([3, 2, 1, 0, 5, 4, 7, 6, 8, 9, 13, 12, 11, 10, 14, 15], dtype=np.uint8)
+
+
+def test_device_library_prep_matches_oracle_generator(oracle_lib):
+    """Subjects with every ambiguity code (scattered, in runs, at the ends,
+    none at all, and one of 20 kbp mostly N) resolved on the device: the plus
+    strand equals the oracle's NcbiRandom seeded with the length, drawn once
+    per ambiguous base 5' to 3', and the minus strand is its reverse
+    complement."""
+    rng = np.random.default_rng(31)
+    subs = []
+    for k in range(40):
+        n = int(rng.integers(1, 3000))
+        x = rng.integers(0, 4, n).astype(np.uint8)
+        if k % 4:
+            x[rng.random(n) < 0.05] = rng.integers(4, 15, int((rng.random(n) < 0.05).sum()) or 1)[0]
+            amb = rng.random(n) < 0.03
+            x[amb] = rng.integers(4, 15, int(amb.sum()))
+        if k % 5 == 0 and n > 10:
+            x[:3] = 14
+            x[-3:] = 11
+        subs.append(x)
+    big = rng.integers(4, 15, 20000).astype(np.uint8)
+    big[::7] = 2
+    subs.append(big)
+    m, _ = comparison()
+    s = rmblast_amd.Searcher(m, word_size=11)
+    s.load_library(subs)
+    for i, x in enumerate(subs):
+        plus, rc = s.tap_subject(i)
+        want = x.copy()
+        amb = np.nonzero((x >= 4) & (x <= 14))[0]
+        if len(amb):
+            out = np.zeros(len(amb), dtype=np.uint8)
+            codes = np.ascontiguousarray(x[amb])
+            oracle_lib.orc_ncbi_random(len(x), ptr(codes), len(amb), ptr(out))
+            want[amb] = out
+        assert np.array_equal(plus, want), i
+        assert np.array_equal(rc, COMPLEMENT[want[::-1]]), i
