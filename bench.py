@@ -14,6 +14,11 @@ resident on the GPU.  Metric: query Mbp x families per second.
 one 10 Mbp record (the split query path) against 300 families of 5-15 kbp,
 copies 25-40% diverged with 5-8% indels, 5% microsatellite tracts, X-drop
 gapped/final 120/400; its CPU baseline sample is the record's first 1.5 Mbp.
+--workload parity measures the parity configuration: one 1 Mbp record
+against 50 families of 300-6,000 bp (2,000 copies, 5-25% divergence), a
+latency-bound search (the traceback DPs of its long copies set the time);
+its CPU baseline sample is the whole record on all cores and its first
+200 kbp on one.
 --workload uneven measures the uneven-batching configuration on its first 400
 records (log-uniform 1-500 kbp) against all 5,000 families (log-uniform
 50-20,000 bp).  The kernel schedule is the autotuner's entry for the matrix,
@@ -55,7 +60,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=("scale", "gappy", "uneven"), default="scale")
+    ap.add_argument("--workload", choices=("scale", "gappy", "uneven", "parity"), default="scale")
     ap.add_argument("--records", type=int, default=None,
                     help="records to search (scale: of the 1,667; uneven: of the 20,000; default 1,667 / 400)")
     ap.add_argument("--families", type=int, default=None, help="families (default 1,000 / 5,000)")
@@ -112,8 +117,11 @@ def main():
     t0 = time.time()
     gappy = a.workload == "gappy"
     uneven = a.workload == "uneven"
+    parity_wl = a.workload == "parity"
     if gappy:
         w = synth.gappy_config()
+    elif parity_wl:
+        w = synth.parity_config()
     elif uneven:
         w = synth.uneven_config(n_records=a.records or 400, n_families=a.families or 5000)
     else:
@@ -229,6 +237,11 @@ def main():
             if gappy:
                 sample = [(w.records[0][0], w.records[0][1][:1_500_000])]
                 cpu = cpu_baseline(exe, sample, sample, w.families, tmp, mpath, xdrop, "the record's first 1.5 Mbp")
+            elif parity_wl:
+                one = [(w.records[0][0], w.records[0][1][:200_000])]
+                cpu = cpu_baseline(exe, w.records, one, w.families, tmp, mpath, xdrop,
+                                   "the whole record (one core: its first 200 kbp)", gpu_rows=rows,
+                                   sample_ids=set(qids))
             else:
                 cpu = cpu_baseline(exe, w.records[:cpu_records], w.records[:cpu1_records], w.families, tmp,
                                    mpath, xdrop, f"first {cpu_records} records",
@@ -236,6 +249,9 @@ def main():
     if gappy:
         workload = ("divergent/gappy stress: 1 x 10 Mbp record x 300 families of 5-15 kbp, 25-40% divergence, "
                     "5-8% indels, X-drop 4x, ws 7")
+    elif parity_wl:
+        workload = ("parity configuration: 1 x 1 Mbp record x 50 families of 300-6,000 bp, 2,000 copies, ws 7, "
+                    "search + host tail to outfmt-6 rows")
     elif uneven:
         workload = (f"uneven batching: the first {len(queries)} of 20,000 records of log-uniform 1-500 kbp "
                     f"x {nfam} families of log-uniform 50-20,000 bp, ws 7, search + host tail to outfmt-6 rows")
