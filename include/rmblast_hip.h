@@ -11,9 +11,11 @@
  * records against the loaded library (both strands) and returns, per
  * (query, subject) pair, the final HSPs of the reference's traceback phase
  * (engine.rs:1438-1762: after complexity adjustment, the common-endpoint purge
- * and the final containment pass) with their edit scripts.  Mask level,
- * output ordering, alignment statistics and formatting stay with the caller
- * (engine.rs:863-1017, stats.rs, output.rs).
+ * and the final containment pass) with their edit scripts.  The host tail
+ * after it (E-value reap, mask level, output order, alignment statistics and
+ * the outfmt-6 text; main.rs:1224-1234, engine.rs:863-1017, stats.rs,
+ * output.rs) is offered by the rmb_report_* and rmb_mask_level entry points,
+ * which run on the library's host threads; a caller may keep its own.
  *
  * Conventions
  *  - Sequences are BLASTNA codes (A0 C1 G2 T3 R4 Y5 M6 K7 W8 S9 B10 D11 H12
