@@ -26,6 +26,8 @@
 #include <functional>
 #include <stdexcept>
 #include <string>
+#include <sched.h>
+
 #include <thread>
 #include <vector>
 
@@ -137,15 +139,18 @@ struct SeqView {
   const uint8_t* end() const { return p + n; }
 };
 
-// Host worker threads: RMB_THREADS if set, else the hardware threads, at most 16
-// (one GPU's share of the host on a multi-GPU node).
+// Host worker threads: RMB_THREADS if set, else every CPU the process may run
+// on (its affinity mask: a job's share of the host on a shared node).
 static unsigned host_threads() {
   static const unsigned n = [] {
     if (const char* e = std::getenv("RMB_THREADS")) {
       const long v = std::strtol(e, nullptr, 10);
       if (v > 0) return (unsigned)std::min<long>(v, 1024);
     }
-    return std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    cpu_set_t set;
+    CPU_ZERO(&set);
+    if (sched_getaffinity(0, sizeof set, &set) == 0 && CPU_COUNT(&set) > 0) return (unsigned)CPU_COUNT(&set);
+    return std::max(1u, std::thread::hardware_concurrency());
   }();
   return n;
 }
