@@ -306,7 +306,15 @@ constexpr uint32_t kOutChunk = 256;  // output slots a workgroup reserves at a t
 // SEEDS (the seeds tap): every seed the diagonal hash lets through is output
 // instead of the hits, as {item, diagonal, hash check offset, strand, its
 // ordinal in the item's scan}; the hash evolves exactly as in the search.
-template <bool SEEDS>
+// WORD: whole-word tables (word size = lookup width, scan step 1, positive
+// ACGT match scores; P.known_word).  A seed then needs no word extension and
+// its frame follows from its scan position and lookup entry alone, so a hit's
+// extension start and end are recomputed from the seed record at output time
+// and only its left end and score are kept, in an LDS table of the pass
+// (kHitLds entries, the rest in the workgroup's global area).
+constexpr uint32_t kHitLds = 128;
+constexpr uint32_t kHitSlotShift = 21;  // sd[].w: steps (16 bits) | flags << 16 (5 bits) | hit slot << 21
+template <bool SEEDS, bool WORD>
 __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
     P1Params P, const RecDesc* __restrict__ recs, const SubjDesc* __restrict__ subjs,
     const uint8_t* __restrict__ qfwd, const uint8_t* __restrict__ qrc, const uint8_t* __restrict__ qext,
@@ -318,11 +326,14 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
   __shared__ uint32_t s_cum[kP1Threads], s_cst[kP1Threads], wsum[4];
   __shared__ uint4 sd[kSeedCap];  // per seed: {diagonal, hash check offset, level, steps | flags << 16}
   __shared__ uint16_t blist[kSeedCap];
-  __shared__ uint16_t llist[kSeedCap];  // seeds whose extension outruns the fast first tier
+  // seeds whose extension outruns the fast first tier: written in B, read in
+  // C, so it shares blist's space (written from D on)
+  uint16_t* const llist = blist;
   __shared__ uint8_t posmap[kSeedCap];  // pass seed -> its scan position in the window
   __shared__ uint32_t bcnt[kBuckets], bst[kBuckets];
   __shared__ unsigned long long s_item, s_out_next, s_pass_a, s_pass_b;
-  __shared__ uint32_t s_out_left, s_pass_an, s_nemit, s_emit_i, s_nlong;
+  __shared__ uint32_t s_out_left, s_pass_an, s_nemit, s_emit_i, s_nlong, s_nhit;
+  __shared__ uint2 s_hit[WORD ? kHitLds : 1];  // WORD: {left end, score} of the pass's hits
 
   const int tid = threadIdx.x;
   const uint32_t xcd = blockIdx.x % kXcds;
@@ -331,6 +342,7 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
     s_out_next = 0;
     s_out_left = 0;
     s_nlong = 0;
+    s_nhit = 0;
   }
   uint2* my_spill = spill + (size_t)blockIdx.x * kBuckets * kHashGlobal;
   uint4* my_hitres = hitres + (size_t)blockIdx.x * kSeedCap;  // extension results of the pass's hits, by seed
@@ -352,6 +364,25 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
       s_tick[k] += t - s_tick[6];
       s_tick[6] = t;
     }
+  };
+  // WORD: a slot of the pass's hit table for each lane with `hit` (one LDS
+  // atomic per wave), its {left end, score} stored; returns the slot
+  auto put_hit = [&](bool hit, uint32_t q_beg, int32_t score) -> uint32_t {
+    const unsigned long long m = __ballot(hit);
+    uint32_t slot = 0;
+    if (m) {
+      const uint32_t lane = (uint32_t)tid & 63u;
+      const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(&s_nhit, (uint32_t)__popcll(m));
+      base = __shfl(base, (int)leader, 64);
+      slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+      if (hit) {
+        if (slot < kHitLds) s_hit[slot] = make_uint2(q_beg, (uint32_t)score);
+        else my_hitres[slot - kHitLds] = make_uint4(q_beg, (uint32_t)score, 0u, 0u);
+      }
+    }
+    return slot;
   };
 
   for (;;) {
@@ -471,7 +502,17 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
         // forward subject, anchored at the word's last base.
         bool ok;
         int32_t eq = 0, es = 0;
-        if (q0c < c1) {
+        if constexpr (WORD) {
+          // plus: forward query at q0c against the forward subject at s_fwd;
+          // minus: the reverse-complement query at the word's last base
+          // against the forward subject, the hash check offset s_fwd for both
+          const bool minus = q0c >= c1;
+          ok = true;
+          diag = (int32_t)((uint32_t)s_fwd - (uint32_t)q0c);
+          chk = (uint32_t)s_fwd;
+          eq = minus ? q0c - c1 + lw - 1 : q0c;
+          es = minus ? s_fwd + lw - 1 : s_fwd;
+        } else if (q0c < c1) {
           int32_t q_off = q0c, s_off = s_fwd;
           ok = true;
           if (ext_needed > 0) {  // engine.rs:1130-1153
@@ -537,11 +578,11 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
           const int64_t qa = (int64_t)R.q_base + 1 + eq, sa = (int64_t)S.s_base + 1 + es;
           UngOut u;
           // without the fast tier every seed takes the full extension below
-          if (P.known_word && ungapped_first(qpool, qa, splus, sa, eq, minus, mat, P.xdrop_ungap, u)) {
+          if (WORD && ungapped_first(qpool, qa, splus, sa, eq, minus, mat, P.xdrop_ungap, u)) {
             const bool hit = u.score >= R.cutoff;
-            if (hit) my_hitres[i] = make_uint4(u.q_beg, u.q_end, (uint32_t)u.score, (uint32_t)eq);
+            const uint32_t slot = put_hit(hit, u.q_beg, u.score);
             level = hit ? (uint32_t)es + (u.q_end - (uint32_t)eq) : chk + (uint32_t)ws;
-            steps = u.steps;
+            steps = (u.steps < 65535u ? u.steps : 65535u) | (hit ? slot << kHitSlotShift : 0u);
             flag = F_VALID | (hit ? F_HIT : 0) | (minus ? F_MINUS : 0);
           } else {  // the full extension runs below, over a compact list; level holds eq meanwhile
             longer = true;
@@ -553,8 +594,7 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
           diag = 0;
           chk = 0;
         }
-        sd[i] = make_uint4((uint32_t)diag, chk, level,
-                           (steps < 65535u ? steps : 65535u) | (flag << 16));
+        sd[i] = make_uint4((uint32_t)diag, chk, level, (WORD ? steps : (steps < 65535u ? steps : 65535u)) | (flag << 16));
         // the list of longer extensions: one LDS atomic per wave
         const unsigned long long m = __ballot(longer);
         if (m) {
@@ -581,9 +621,14 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
           const UngOut u = ungapped_chunked<kLongChunk>(qpool, (int64_t)R.q_base + 1 + eq, splus, (int64_t)S.s_base + 1 + es,
                                             eq, mat, P.xdrop_ungap);
           const bool hit = u.score >= R.cutoff;
-          if (hit) my_hitres[i] = make_uint4(u.q_beg, u.q_end, (uint32_t)u.score, (uint32_t)eq);
+          uint32_t steps = u.steps < 65535u ? u.steps : 65535u;
+          if constexpr (WORD) {
+            const uint32_t slot = put_hit(hit, u.q_beg, u.score);
+            steps |= hit ? slot << kHitSlotShift : 0u;
+          } else {
+            if (hit) my_hitres[i] = make_uint4(u.q_beg, u.q_end, (uint32_t)u.score, (uint32_t)eq);
+          }
           const uint32_t level = hit ? (uint32_t)es + (u.q_end - (uint32_t)eq) : e.y + (uint32_t)ws;
-          const uint32_t steps = u.steps < 65535u ? u.steps : 65535u;
           sd[i] = make_uint4(e.x, e.y, level, steps | ((flag | (hit ? F_HIT : 0)) << 16));
         }
         __syncthreads();
@@ -780,14 +825,32 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
             else atomicOr(err, (uint32_t)E_OUT_FULL);
             continue;
           }
-          const uint4 hv = my_hitres[i];
-          const int32_t eq = (int32_t)hv.w;
           const bool minus = ((e.w >> 16) & F_MINUS) != 0;
           const int32_t es = minus ? lw - 1 + (int32_t)e.y : (int32_t)e.y;
-          const uint32_t qb = hv.x, qe = hv.y;
+          int32_t eq;
+          uint32_t qb, qe;
+          int32_t score;
+          if constexpr (WORD) {
+            // the frame from the seed record (diagonal = s_fwd - q0c, check
+            // offset = s_fwd), the right end from the level a hit stores
+            const uint32_t slot = e.w >> kHitSlotShift;
+            const uint2 hv = slot < kHitLds ? s_hit[slot]
+                                            : make_uint2(my_hitres[slot - kHitLds].x, my_hitres[slot - kHitLds].y);
+            const int32_t q0c = (int32_t)(e.y - e.x);
+            eq = minus ? q0c - c1 + lw - 1 : q0c;
+            qb = hv.x;
+            qe = e.z - (uint32_t)es + (uint32_t)eq;
+            score = (int32_t)hv.y;
+          } else {
+            const uint4 hv = my_hitres[i];
+            eq = (int32_t)hv.w;
+            qb = hv.x;
+            qe = hv.y;
+            score = (int32_t)hv.z;
+          }
           UngHit h;
           h.item = (uint32_t)item;
-          h.score = (int32_t)hv.z;
+          h.score = score;
           if (!minus) {
             h.q_seed = (uint32_t)eq;
             h.s_seed = (uint32_t)es;
@@ -820,6 +883,7 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
       if (tid == 0) {  // read and updated only between this pass's E and F barriers
         s_nemit = 0;
         s_emit_i = 0;
+        s_nhit = 0;    // written in B and C, read in F
       }
       wk += take;
       if (wk == wtotal) {
@@ -1875,12 +1939,10 @@ hipError_t launch_phase1(int blocks, P1Params P, const RecDesc* recs, const Subj
                          const uint32_t* cells, const uint32_t* entries, const int32_t* gmat, UngHit* out,
                          unsigned long long* out_count, uint2* spill, uint4* hitres, unsigned long long* queue,
                          unsigned long long* counters, uint32_t* err, bool seeds, hipStream_t st) {
-  if (seeds)
-    hipLaunchKernelGGL(k_phase1<true>, dim3(blocks), dim3(kP1Threads), 0, st, P, recs, subjs, qfwd, qrc, qext, splus,
-                       src, cells, entries, gmat, out, out_count, spill, hitres, queue, counters, err);
-  else
-    hipLaunchKernelGGL(k_phase1<false>, dim3(blocks), dim3(kP1Threads), 0, st, P, recs, subjs, qfwd, qrc, qext, splus,
-                       src, cells, entries, gmat, out, out_count, spill, hitres, queue, counters, err);
+  auto kern = seeds ? (P.known_word ? k_phase1<true, true> : k_phase1<true, false>)
+                    : (P.known_word ? k_phase1<false, true> : k_phase1<false, false>);
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(kP1Threads), 0, st, P, recs, subjs, qfwd, qrc, qext, splus, src, cells,
+                     entries, gmat, out, out_count, spill, hitres, queue, counters, err);
   return hipGetLastError();
 }
 
