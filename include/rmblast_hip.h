@@ -299,6 +299,20 @@ int rmb_report_order(rmb_ctx* ctx, const rmb_report_spec* spec, uint64_t* rows, 
  * library's host threads.  The text is held by the context until the next
  * search or report; rmb_report_copy copies it out. */
 int rmb_report_tabular(rmb_ctx* ctx, const rmb_report_spec* spec, uint64_t* n_bytes);
+
+/* rmb_search, then rmb_report_tabular's text of that search, in one call:
+ * rmblastn's search then report (main.rs:1133-1234), with the report written
+ * while the search runs.  The search takes its queries in batches of whole
+ * queries; as soon as a batch's HSPs are final, its queries' rows (reap,
+ * mask level, order, statistics, outfmt 6 text) are written on a host worker
+ * thread while the GPU and the host go on with the next batches, so only the
+ * last batch's rows remain after the search.  The text is the bytes
+ * rmb_report_tabular gives with the same spec (in place through
+ * rmb_report_text, *n_bytes its length); the results are rmb_search's.
+ * spec, its strings and the ids must stay valid until the call returns.
+ * No counterpart call in the reference: the CLI's two steps in one. */
+int rmb_search_report(rmb_ctx* ctx, uint32_t n_queries, const uint8_t* const* queries, const uint32_t* lens,
+                      const rmb_report_spec* spec, uint64_t* n_bytes);
 int rmb_report_copy(const rmb_ctx* ctx, char* out);
 /* The report text in place (not NUL-terminated), valid until the context's
  * next search, report or destruction. */
@@ -318,10 +332,11 @@ int rmb_report_text(const rmb_ctx* ctx, const char** text, uint64_t* n_bytes);
  * kernel, [17..22] seeding-kernel stage time (scan, output-slot reservation,
  * seed fill + extension, bucket sort, hash replay, emit) in 100 MHz ticks
  * summed over workgroups,
- * [23] time inside the preliminary DP kernel in nanoseconds (part of [14]).
- * Entries past `n` are not written; RMB_NSTATS is the full
- * length. */
-#define RMB_NSTATS 24
+ * [23] time inside the preliminary DP kernel in nanoseconds (part of [14]),
+ * [24] (rmb_search_report) microseconds of report writing left after the
+ * last batch's HSPs were final.  Entries past `n` are not written;
+ * RMB_NSTATS is the full length. */
+#define RMB_NSTATS 25
 int rmb_stats(const rmb_ctx* ctx, uint64_t* out, int n);
 
 /* ---- parity taps (one stage at a time; same inputs as rmb_search) -------- */

@@ -106,7 +106,7 @@ STAT_NAMES = ["seed_evals", "ungapped_hits", "prelim_dps", "traceback_dps", "dp_
               "hsps", "ungapped_steps", "t_lookup_us", "t_seed_us", "t_prelim_us", "t_traceback_us",
               "t_total_us", "t_phase1_kernel_ns", "t_gapped_kernel_ns", "phase1_launches", "seeds_staged",
               "p1_ticks_scan", "p1_ticks_reserve", "p1_ticks_extend", "p1_ticks_bucket", "p1_ticks_replay",
-              "p1_ticks_emit", "t_prelim_kernel_ns"]
+              "p1_ticks_emit", "t_prelim_kernel_ns", "t_report_tail_us"]
 
 
 class RmbError(RuntimeError):
@@ -145,6 +145,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rmb_ka_evalue.argtypes = [P(RmbKaStats), ctypes.c_int64, ctypes.c_int32, P(ctypes.c_double),
                                   P(ctypes.c_double)]
     lib.rmb_report_tabular.argtypes = [ctypes.c_void_p, P(RmbReportSpec), P(ctypes.c_uint64)]
+    lib.rmb_search_report.argtypes = [ctypes.c_void_p, ctypes.c_uint32, P(u8p), P(ctypes.c_uint32), P(RmbReportSpec),
+                                      P(ctypes.c_uint64)]
     lib.rmb_report_copy.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
     lib.rmb_report_text.argtypes = [ctypes.c_void_p, P(ctypes.c_void_p), P(ctypes.c_uint64)]
     lib.rmb_dust_intervals.argtypes = [u8p, ctypes.c_uint64, P(ctypes.c_uint64), P(ctypes.c_uint64)]
@@ -368,6 +370,36 @@ class Searcher:
                                                ctypes.byref(n)), "rmb_report_order")
         return rows
 
+    def search_report(self, queries, fields: str, query_ids, subject_ids, mask_level: int = 101,
+                      copy: bool = True, ka: RmbKaStats = None, evalue_max: float = 0.0):
+        """rmb_search_report: search `queries` and write their outfmt 6 text
+        while the search runs (the bytes report_tabular gives after search).
+        Returns the text as report_tabular does; results() gives the HSPs."""
+        arrs = [np.ascontiguousarray(q, dtype=np.uint8) for q in queries]
+        ptrs = (ctypes.POINTER(ctypes.c_uint8) * len(arrs))(*[_u8ptr(a) for a in arrs])
+        lens = (ctypes.c_uint32 * len(arrs))(*[len(a) for a in arrs])
+        qids = (ctypes.c_char_p * len(query_ids))(*[q.encode() for q in query_ids])
+        sids = (ctypes.c_char_p * len(subject_ids))(*[x.encode() for x in subject_ids])
+        spec = RmbReportSpec(mask_level, 0, fields.encode(), qids, sids,
+                             ctypes.pointer(ka) if ka is not None else None, evalue_max)
+        n = ctypes.c_uint64(0)
+        self._check(self._lib.rmb_search_report(self._ctx, len(arrs), ptrs, lens, ctypes.byref(spec),
+                                                ctypes.byref(n)), "rmb_search_report")
+        return self._report_text(copy)
+
+    def results(self):
+        """The last search's HSPs and ops (HSP_DTYPE array, uint32 ops)."""
+        return self._results()
+
+    def _report_text(self, copy: bool):
+        ptr, size = ctypes.c_void_p(), ctypes.c_uint64()
+        self._check(self._lib.rmb_report_text(self._ctx, ctypes.byref(ptr), ctypes.byref(size)), "rmb_report_text")
+        if size.value == 0:
+            return b"" if copy else memoryview(b"")
+        if copy:
+            return ctypes.string_at(ptr.value, size.value)
+        return memoryview((ctypes.c_char * size.value).from_address(ptr.value)).toreadonly()
+
     def report_tabular(self, fields: str, query_ids, subject_ids, mask_level: int = 101, copy: bool = True,
                        ka: RmbKaStats = None, evalue_max: float = 0.0):
         """outfmt 6 text of the last search (mask level, order, statistics and
@@ -381,13 +413,7 @@ class Searcher:
         n = ctypes.c_uint64(0)
         self._check(self._lib.rmb_report_tabular(self._ctx, ctypes.byref(spec), ctypes.byref(n)),
                     "rmb_report_tabular")
-        ptr, size = ctypes.c_void_p(), ctypes.c_uint64()
-        self._check(self._lib.rmb_report_text(self._ctx, ctypes.byref(ptr), ctypes.byref(size)), "rmb_report_text")
-        if size.value == 0:
-            return b"" if copy else memoryview(b"")
-        if copy:
-            return ctypes.string_at(ptr.value, size.value)
-        return memoryview((ctypes.c_char * size.value).from_address(ptr.value)).toreadonly()
+        return self._report_text(copy)
 
     def stats(self):
         out = (ctypes.c_uint64 * len(STAT_NAMES))()

@@ -694,6 +694,7 @@ struct P1Batch {
   int slot = 0;
   std::vector<P1Group> groups;
   uint64_t redo_cap = 0;
+  uint32_t q0 = 0, q1 = 0;  // the whole queries of its records
 };
 
 // Lookup tables of a group's records (blast/nalookup.rs, blast/mblookup.rs):
@@ -1382,8 +1383,49 @@ static int32_t record_cutoff(const rmb_ctx* c, uint64_t full_len) {
   return ka >= 0 ? ka : cutoff_score_max(c->P.min_raw_gapped_score);
 }
 
+// Worker threads of the report written during a search: RMB_REPORT_THREADS
+// if set, else a quarter of the host threads (it runs beside the search's
+// own host work, which has the rest).
+static unsigned report_threads() {
+  static const unsigned n = [] {
+    if (const char* e = std::getenv("RMB_REPORT_THREADS")) {
+      const long v = std::strtol(e, nullptr, 10);
+      if (v > 0) return (unsigned)std::min<long>(v, 1024);
+    }
+    return std::max(1u, host_threads() / 4);
+  }();
+  return n;
+}
+
+// The report written while a search runs (rmb_search_report): after a
+// batch's HSPs are final, a worker thread writes its queries' rows while the
+// search goes on with the next batches; one batch's worker at a time, so the
+// text stays in query order.
+struct ReportPipe {
+  const rmb_report_spec* spec = nullptr;
+  ReportStream* stream = nullptr;
+  std::thread worker;
+  std::exception_ptr failure;
+  // the running batch's snapshot (its HSPs with ops rebased to its own ops)
+  std::vector<rmb_hsp> hsps;
+  std::vector<rmb_op> ops;
+  std::vector<const uint8_t*> sp, sr;
+  std::vector<uint32_t> sl;
+  void join() {
+    if (worker.joinable()) worker.join();
+    if (failure) {
+      std::exception_ptr f = failure;
+      failure = nullptr;
+      std::rethrow_exception(f);
+    }
+  }
+  ~ReportPipe() {
+    if (worker.joinable()) worker.join();
+  }
+};
+
 static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const uint32_t* lens,
-                        const std::vector<uint64_t>* pairs = nullptr) {
+                        const std::vector<uint64_t>* pairs = nullptr, ReportPipe* rp = nullptr) {
   Timer total;
   if (c->subj.empty()) throw Failure(RMB_ERR_STATE, "no library loaded");
   std::memset(c->stats, 0, sizeof c->stats);
@@ -1489,7 +1531,7 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
                                                        : (double)total_bases);
   std::vector<P1Batch> batches;
   {
-    uint32_t k0 = 0;
+    uint32_t k0 = 0, qb = 0;
     uint64_t acc = 0;
     for (uint32_t q = 0; q < nq; ++q) {
       for (uint32_t k = sr.first[q]; k < sr.first[q + 1]; ++k) acc += recs[k].q_len;
@@ -1500,10 +1542,23 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
           P1Batch b;
           b.slot = (int)(batches.size() % 2);
           b.groups = group_records(c, recs, sr.first, k0, k1, pairs);
+          b.q0 = qb;
+          b.q1 = q + 1;
           batches.push_back(std::move(b));
+          qb = q + 1;
         }
         k0 = k1;
       }
+    }
+  }
+  if (rp) {  // the subjects' views for the report, once per search
+    rp->sp.clear();
+    rp->sr.clear();
+    rp->sl.clear();
+    for (const Subject& S : c->subj) {
+      rp->sp.push_back(S.align.data());
+      rp->sr.push_back(S.align_rc.data());
+      rp->sl.push_back((uint32_t)S.align.size() - 2);
     }
   }
   uint64_t p1_ctr[C_NCOUNTERS] = {0};
@@ -1532,8 +1587,46 @@ static void search_impl(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const
     Timer t3;
     phase2b(c, sr, qf, lens, items, d_qfwd, d_qrc, tb_ms);
     fail_point("phase2b");
+    const size_t h0 = c->hsps.size(), o0 = c->ops.size();
     emit_results(c, items);
     t_2b += t3.us();
+    if (rp) {
+      // the batch's queries are complete: their rows are written on a worker
+      // thread over a copy of the batch's HSPs while the next batches run
+      rp->join();
+      rp->hsps.assign(c->hsps.begin() + (ptrdiff_t)h0, c->hsps.end());
+      rp->ops.assign(c->ops.begin() + (ptrdiff_t)o0, c->ops.end());
+      for (rmb_hsp& h : rp->hsps) h.ops_offset -= (uint32_t)o0;
+      const uint32_t q0 = batches[b].q0, q1 = batches[b].q1;
+      rp->worker = std::thread([c, rp, q0, q1]() {
+        try {
+          ReportData d;
+          d.hsps = rp->hsps.data();
+          d.n_hsps = rp->hsps.size();
+          d.ops = rp->ops.data();
+          d.n_queries = (uint32_t)c->last_q.size();
+          d.queries = c->last_q.data();
+          d.query_lens = c->last_qlen.data();
+          d.n_subjects = (uint32_t)c->subj.size();
+          d.subjects = rp->sp.data();
+          d.subjects_rc = rp->sr.data();
+          d.subject_lens = rp->sl.data();
+          d.ka = rp->spec->ka;
+          d.evalue_max = rp->spec->evalue_max;
+          d.db_length = (int64_t)c->db_letters;
+          d.db_num_seqs = (int32_t)c->subj.size();
+          rp->stream->add(d, q0, q1, report_threads());
+        } catch (...) {
+          rp->failure = std::current_exception();
+        }
+      });
+    }
+  }
+  if (rp) {
+    Timer tr;
+    rp->join();
+    c->report = rp->stream->finish();
+    c->stats[24] = tr.us();
   }
   c->stats[9] = t_seed;
   c->stats[10] = t_2a;
@@ -1690,6 +1783,8 @@ static int guarded(rmb_ctx* c, F&& fn) {
     return RMB_ERR_DEVICE;
   }
 }
+
+static std::vector<std::string> spec_fields(const rmb_report_spec* spec);
 
 extern "C" {
 
@@ -1863,6 +1958,24 @@ int rmb_search(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const uint32_t
   });
 }
 
+int rmb_search_report(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const uint32_t* lens,
+                      const rmb_report_spec* spec, uint64_t* n_bytes) {
+  if (!c || !spec || !n_bytes) return RMB_ERR_ARG;
+  return guarded(c, [&]() {
+    if (nq == 0 || !qs || !lens) throw Failure(RMB_ERR_ARG, "empty query batch");
+    const std::vector<std::string> fields = spec_fields(spec);
+    if (!spec->query_ids || (!spec->subject_ids && !c->subj.empty()))
+      throw Failure(RMB_ERR_ARG, "query and subject ids are required");
+    ReportStream stream(fields, spec->query_ids, spec->subject_ids, spec->mask_level);
+    ReportPipe rp;
+    rp.spec = spec;
+    rp.stream = &stream;
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    search_or_drain(c, [&] { search_impl(c, nq, qs, lens, nullptr, &rp); });
+    *n_bytes = c->report.size;
+  });
+}
+
 int rmb_search_pairs(rmb_ctx* c, uint32_t nq, const uint8_t* const* qs, const uint32_t* lens, uint64_t n_pairs,
                      const uint32_t* pair_query, const uint32_t* pair_subject) {
   if (!c) return RMB_ERR_ARG;
@@ -1954,26 +2067,39 @@ int rmb_report_order(rmb_ctx* c, const rmb_report_spec* spec, uint64_t* rows, ui
   });
 }
 
+}  // extern "C"
+
+// The outfmt 6 field names of a report spec, checked.
+static std::vector<std::string> spec_fields(const rmb_report_spec* spec) {
+  if (!spec->fields) throw Failure(RMB_ERR_ARG, "no outfmt fields");
+  std::vector<std::string> fields;
+  std::string w;
+  for (const char* p = spec->fields;; ++p) {
+    if (*p == ' ' || *p == '\t' || *p == 0) {
+      if (!w.empty()) fields.push_back(w);
+      w.clear();
+      if (!*p) break;
+    } else {
+      w += *p;
+    }
+  }
+  if (fields.empty()) throw Failure(RMB_ERR_ARG, "no outfmt fields");
+  try {
+    (void)parse_report_fields(fields);
+  } catch (const std::invalid_argument& e) {
+    throw Failure(RMB_ERR_ARG, e.what());
+  }
+  return fields;
+}
+
+extern "C" {
+
 int rmb_report_tabular(rmb_ctx* c, const rmb_report_spec* spec, uint64_t* n_bytes) {
   if (!c || !spec || !n_bytes) return RMB_ERR_ARG;
   return guarded(c, [&]() {
-    if (!spec->fields) throw Failure(RMB_ERR_ARG, "no outfmt fields");
+    const std::vector<std::string> fields = spec_fields(spec);
     if ((!spec->query_ids && !c->last_q.empty()) || (!spec->subject_ids && !c->subj.empty()))
       throw Failure(RMB_ERR_ARG, "query and subject ids are required");
-    std::vector<std::string> fields;
-    {
-      std::string w;
-      for (const char* p = spec->fields;; ++p) {
-        if (*p == ' ' || *p == '\t' || *p == 0) {
-          if (!w.empty()) fields.push_back(w);
-          w.clear();
-          if (!*p) break;
-        } else {
-          w += *p;
-        }
-      }
-    }
-    if (fields.empty()) throw Failure(RMB_ERR_ARG, "no outfmt fields");
     std::vector<const uint8_t*> sp, sr;
     std::vector<uint32_t> sl;
     const ReportData d = report_data(c, spec, sp, sr, sl);

@@ -13,6 +13,8 @@
 #include <stdexcept>
 #include <thread>
 
+#include <sys/mman.h>
+
 namespace rmb {
 namespace {
 
@@ -260,58 +262,60 @@ std::string format_bit_score(double bits) {
   return b;
 }
 
-std::vector<std::vector<uint64_t>> report_order(const ReportData& d, int32_t mask_level, unsigned threads) {
-  std::vector<std::vector<uint64_t>> rows(d.n_queries);
-  for (uint64_t i = 0; i < d.n_hsps; ++i) rows[d.hsps[i].query].push_back(i);
+std::vector<std::vector<uint64_t>> report_order(const ReportData& d, int32_t mask_level, unsigned threads,
+                                                uint32_t q0, uint32_t q1) {
+  q1 = std::min(q1, d.n_queries);
+  q0 = std::min(q0, q1);
+  std::vector<std::vector<uint64_t>> rows(q1 - q0);
+  for (uint64_t i = 0; i < d.n_hsps; ++i) {
+    const uint32_t q = d.hsps[i].query;
+    if (q >= q0 && q < q1) rows[q - q0].push_back(i);
+  }
   const bool masking = mask_level >= 0 && mask_level < 100;
   const bool reap = d.ka && d.ka->lambda > 0.0 && d.evalue_max > 0.0;
-  for_each_query(d.n_queries, threads, [&](uint32_t q) {
+  for_each_query(q1 - q0, threads, [&](uint32_t k) {
+    const uint32_t q = q0 + k;
+    std::vector<uint64_t>& rq = rows[k];
     if (reap) {  // the --evalue reap, before mask level
       const int64_t sp = query_searchsp(d, q);
       std::vector<uint64_t> kept;
-      kept.reserve(rows[q].size());
-      for (uint64_t r : rows[q])
+      kept.reserve(rq.size());
+      for (uint64_t r : rq)
         if (ka_evalue(*d.ka, sp, d.hsps[r].score) <= d.evalue_max) kept.push_back(r);
-      rows[q].swap(kept);
+      rq.swap(kept);
     }
-    if (masking) mask_level_filter(d, rows[q], mask_level, d.query_lens[q]);
-    output_order(d, rows[q], d.query_lens[q]);
+    if (masking) mask_level_filter(d, rq, mask_level, d.query_lens[q]);
+    output_order(d, rq, d.query_lens[q]);
   });
   return rows;
 }
 
-ReportText report_tabular(const ReportData& d, const std::vector<std::vector<uint64_t>>& order,
-                           const std::vector<std::string>& fields, const char* const* query_ids,
-                           const char* const* subject_ids, unsigned threads) {
-  enum F {
-    SCORE, PERC_SUB, PERC_QGAP, PERC_DGAP, QSEQID, QSTART, QEND, QLEN, SSTRAND, SSEQID, SSTART, SEND, SLEN, KDIV,
-    CPG_KDIV, TRANSI, TRANSV, CPG_SITES, QSEQ, SSEQ, EVALUE, BITSCORE
-  };
-  static const char* kNames[] = {"score", "perc_sub", "perc_query_gap", "perc_db_gap", "qseqid", "qstart", "qend",
-                                 "qlen", "sstrand", "sseqid", "sstart", "send", "slen", "kdiv", "cpg_kdiv", "transi",
-                                 "transv", "cpg_sites", "qseq", "sseq", "evalue", "bitscore"};
-  std::vector<int> f;
-  bool need_aln = false, need_stats = false;
-  for (const std::string& name : fields) {
-    const auto it = std::find_if(std::begin(kNames), std::end(kNames), [&](const char* k) { return name == k; });
-    if (it == std::end(kNames)) throw std::invalid_argument("unknown outfmt 6 field '" + name + "'");
-    const int k = (int)(it - std::begin(kNames));
-    f.push_back(k);
-    if (k == QSEQ || k == SSEQ) need_aln = true;
-    if (k == PERC_SUB || k == PERC_QGAP || k == PERC_DGAP || (k >= KDIV && k <= CPG_SITES)) need_aln = need_stats = true;
-  }
-  std::vector<std::string> text(d.n_queries);
-  for_each_query(d.n_queries, threads, [&](uint32_t q) {
-    std::string& o = text[q];
+namespace {
+enum FieldCode {
+  SCORE, PERC_SUB, PERC_QGAP, PERC_DGAP, QSEQID, QSTART, QEND, QLEN, SSTRAND, SSEQID, SSTART, SEND, SLEN, KDIV,
+  CPG_KDIV, TRANSI, TRANSV, CPG_SITES, QSEQ, SSEQ, EVALUE, BITSCORE
+};
+const char* const kFieldNames[] = {"score", "perc_sub", "perc_query_gap", "perc_db_gap", "qseqid", "qstart", "qend",
+                                   "qlen", "sstrand", "sseqid", "sstart", "send", "slen", "kdiv", "cpg_kdiv",
+                                   "transi", "transv", "cpg_sites", "qseq", "sseq", "evalue", "bitscore"};
+
+// The rows of queries [q0, q0 + order.size()) into text[k] (query q0 + k).
+void report_rows(const ReportData& d, const std::vector<std::vector<uint64_t>>& order, uint32_t q0,
+                 const ReportFields& rf, const char* const* query_ids, const char* const* subject_ids,
+                 unsigned threads, std::vector<std::string>& text) {
+  const std::vector<int>& f = rf.codes;
+  text.assign(order.size(), std::string());
+  for_each_query((uint32_t)order.size(), threads, [&](uint32_t k) {
+    std::string& o = text[k];
     std::string qa, sa;
-    const int64_t sp = query_searchsp(d, q);
-    for (uint64_t r : order[q]) {
+    const int64_t sp = query_searchsp(d, q0 + k);
+    for (uint64_t r : order[k]) {
       const rmb_hsp& h = d.hsps[r];
-      if (need_aln) aligned_strings(d, h, qa, sa);
-      const rmb_align_stats st = need_stats ? align_stats(qa.data(), sa.data(), qa.size()) : rmb_align_stats{};
-      for (size_t k = 0; k < f.size(); ++k) {
-        if (k) o += '\t';
-        switch (f[k]) {
+      if (rf.need_aln) aligned_strings(d, h, qa, sa);
+      const rmb_align_stats st = rf.need_stats ? align_stats(qa.data(), sa.data(), qa.size()) : rmb_align_stats{};
+      for (size_t j = 0; j < f.size(); ++j) {
+        if (j) o += '\t';
+        switch (f[j]) {
           case SCORE: o += std::to_string(h.score); break;
           case PERC_SUB: append_f2(o, (double)st.perc_sub); break;
           case PERC_QGAP: append_f2(o, (double)st.perc_query_gap); break;
@@ -341,16 +345,109 @@ ReportText report_tabular(const ReportData& d, const std::vector<std::vector<uin
       o += '\n';
     }
   });
+}
+}  // namespace
+
+ReportFields parse_report_fields(const std::vector<std::string>& fields) {
+  ReportFields rf;
+  for (const std::string& name : fields) {
+    const auto it = std::find_if(std::begin(kFieldNames), std::end(kFieldNames),
+                                 [&](const char* k) { return name == k; });
+    if (it == std::end(kFieldNames)) throw std::invalid_argument("unknown outfmt 6 field '" + name + "'");
+    const int k = (int)(it - std::begin(kFieldNames));
+    rf.codes.push_back(k);
+    if (k == QSEQ || k == SSEQ) rf.need_aln = true;
+    if (k == PERC_SUB || k == PERC_QGAP || k == PERC_DGAP || (k >= KDIV && k <= CPG_SITES))
+      rf.need_aln = rf.need_stats = true;
+  }
+  return rf;
+}
+
+ReportText report_tabular(const ReportData& d, const std::vector<std::vector<uint64_t>>& order,
+                           const std::vector<std::string>& fields, const char* const* query_ids,
+                           const char* const* subject_ids, unsigned threads) {
+  const ReportFields rf = parse_report_fields(fields);
+  std::vector<std::string> text;
+  report_rows(d, order, 0, rf, query_ids, subject_ids, threads, text);
   // one buffer, the queries' texts copied in parallel (not zero-filled first)
-  std::vector<uint64_t> at(d.n_queries + 1, 0);
-  for (uint32_t q = 0; q < d.n_queries; ++q) at[q + 1] = at[q] + text[q].size();
+  const uint32_t nq = (uint32_t)text.size();
+  std::vector<uint64_t> at(nq + 1, 0);
+  for (uint32_t q = 0; q < nq; ++q) at[q + 1] = at[q] + text[q].size();
   ReportText out;
-  out.size = at[d.n_queries];
-  out.data.reset(new char[out.size ? out.size : 1]);
-  for_each_query(d.n_queries, threads, [&](uint32_t q) {
+  out.size = at[nq];
+  out.data = std::shared_ptr<char>(new char[out.size ? out.size : 1], std::default_delete<char[]>());
+  for_each_query(nq, threads, [&](uint32_t q) {
     if (!text[q].empty()) std::memcpy(out.data.get() + at[q], text[q].data(), text[q].size());
     std::string().swap(text[q]);
   });
+  return out;
+}
+
+// The stream's buffer is an address range reserved once (pages are backed as
+// the text reaches them), so appends never move what is written; should the
+// text outgrow it, a larger buffer takes a copy.
+static constexpr uint64_t kStreamReserve = 1ull << 36;  // 64 GiB of address space
+
+ReportStream::ReportStream(const std::vector<std::string>& fields, const char* const* query_ids,
+                           const char* const* subject_ids, int32_t mask_level)
+    : f_(parse_report_fields(fields)), qids_(query_ids), sids_(subject_ids), mask_level_(mask_level) {
+  void* p = mmap(nullptr, kStreamReserve, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE, -1, 0);
+  if (p != MAP_FAILED) {
+    buf_ = static_cast<char*>(p);
+    cap_ = kStreamReserve;
+    mapped_ = true;
+  }
+}
+
+ReportStream::~ReportStream() {
+  if (!buf_) return;
+  if (mapped_) munmap(buf_, cap_);
+  else delete[] buf_;
+}
+
+void ReportStream::reserve(uint64_t need) {
+  if (need <= cap_) return;
+  const uint64_t cap = std::max<uint64_t>(need, 2 * cap_ + (1u << 20));
+  char* nb = new char[cap];
+  if (used_) std::memcpy(nb, buf_, used_);
+  if (buf_) {
+    if (mapped_) munmap(buf_, cap_);
+    else delete[] buf_;
+  }
+  buf_ = nb;
+  cap_ = cap;
+  mapped_ = false;
+}
+
+void ReportStream::add(const ReportData& d, uint32_t q0, uint32_t q1, unsigned threads) {
+  if (q1 <= q0) return;
+  const auto order = report_order(d, mask_level_, threads, q0, q1);
+  std::vector<std::string> text;
+  report_rows(d, order, q0, f_, qids_, sids_, threads, text);
+  std::vector<uint64_t> at(text.size() + 1, used_);
+  for (size_t k = 0; k < text.size(); ++k) at[k + 1] = at[k] + text[k].size();
+  reserve(at.back());
+  for_each_query((uint32_t)text.size(), threads, [&](uint32_t k) {
+    if (!text[k].empty()) std::memcpy(buf_ + at[k], text[k].data(), text[k].size());
+    std::string().swap(text[k]);
+  });
+  used_ = at.back();
+}
+
+ReportText ReportStream::finish() {
+  ReportText out;
+  out.size = used_;
+  char* b = buf_;
+  const uint64_t cap = cap_;
+  if (!b) {
+    out.data = std::shared_ptr<char>(new char[1], std::default_delete<char[]>());
+  } else if (mapped_) {
+    out.data = std::shared_ptr<char>(b, [cap](char* p) { munmap(p, cap); });
+  } else {
+    out.data = std::shared_ptr<char>(b, std::default_delete<char[]>());
+  }
+  buf_ = nullptr;
+  cap_ = used_ = 0;
   return out;
 }
 

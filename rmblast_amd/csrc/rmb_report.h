@@ -39,23 +39,61 @@ std::string format_bit_score(double bits);
 
 // Per query, the indices of its HSPs that survive mask level, in output
 // order.  mask_level >= 100 (or < 0) keeps every HSP.  Queries run on
-// `threads` host threads.
-std::vector<std::vector<uint64_t>> report_order(const ReportData& d, int32_t mask_level, unsigned threads);
+// `threads` host threads.  With a range [q0, q1), only those queries (entry
+// q - q0), whose HSPs are all in d.
+std::vector<std::vector<uint64_t>> report_order(const ReportData& d, int32_t mask_level, unsigned threads,
+                                                uint32_t q0 = 0, uint32_t q1 = UINT32_MAX);
 
 // Mask level over one query's HSPs (indices into d.hsps), in place.
 void mask_level_filter(const ReportData& d, std::vector<uint64_t>& rows, int32_t level, uint32_t q_len);
 
 // A report's text in one buffer.
 struct ReportText {
-  std::unique_ptr<char[]> data;
+  std::shared_ptr<char> data;
   uint64_t size = 0;
 };
+
+// The outfmt 6 field list: field codes, and whether the rows need the
+// aligned strings.  Throws std::invalid_argument naming an unknown field.
+struct ReportFields {
+  std::vector<int> codes;
+  bool need_aln = false, need_stats = false;
+};
+ReportFields parse_report_fields(const std::vector<std::string>& fields);
 
 // The outfmt 6 rows of `order` (one line per HSP, tab-separated `fields`).
 // Throws std::invalid_argument naming an unknown field.
 ReportText report_tabular(const ReportData& d, const std::vector<std::vector<uint64_t>>& order,
                            const std::vector<std::string>& fields, const char* const* query_ids,
                            const char* const* subject_ids, unsigned threads);
+
+// A report written while the search runs: the search hands over the final
+// HSPs of consecutive query ranges as it completes them (add), and each
+// range's rows (reap, mask level, order, statistics, outfmt 6 text) are
+// appended to one buffer in query order, reserved up front so that it never
+// moves; finish() gives the whole text, the same bytes report_tabular writes
+// over the finished search.
+class ReportStream {
+ public:
+  ReportStream(const std::vector<std::string>& fields, const char* const* query_ids, const char* const* subject_ids,
+               int32_t mask_level);
+  ~ReportStream();
+  ReportStream(const ReportStream&) = delete;
+  ReportStream& operator=(const ReportStream&) = delete;
+  // d holds every HSP of queries [q0, q1) (and no other query's)
+  void add(const ReportData& d, uint32_t q0, uint32_t q1, unsigned threads);
+  ReportText finish();
+
+ private:
+  void reserve(uint64_t need);
+  ReportFields f_;
+  const char* const* qids_;
+  const char* const* sids_;
+  int32_t mask_level_;
+  char* buf_ = nullptr;
+  uint64_t cap_ = 0, used_ = 0;
+  bool mapped_ = false;
+};
 
 // The aligned IUPAC strings of one HSP (forward query frame; a minus-strand
 // subject reads reverse-complemented).

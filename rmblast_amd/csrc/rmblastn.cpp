@@ -525,44 +525,6 @@ int main(int argc, char** argv) {
       break;
     }
   }
-  uint64_t nh = 0, no = 0;
-  std::vector<rmb_hsp> H;
-  std::vector<rmb_op> O;
-  if (!db.empty() && !qs.empty()) {  // a gilist may select no subject: a search with no hits
-    std::vector<const uint8_t*> ptr;
-    std::vector<uint32_t> len;
-    for (auto& r : db) { ptr.push_back(r.bases.data()); len.push_back((uint32_t)r.bases.size()); }
-    if (rmb_library_load(ctx, (uint32_t)db.size(), ptr.data(), len.data())) return fail("library");
-    std::vector<const uint8_t*> qp;
-    std::vector<uint32_t> ql;
-    for (auto& r : qs) { qp.push_back(r.bases.data()); ql.push_back((uint32_t)r.bases.size()); }
-    // The run-log note of a KA-derived ungapped cutoff, once, with the first
-    // query's numbers (main.rs:1098-1131)
-    if (P.ka_lambda > 0 && P.ka_k > 0) {
-      uint64_t letters = 0;
-      for (auto& r : db) letters += r.bases.size();
-      int32_t cut = 0;
-      rmb_ka_cutoff_info info{};
-      rmb_ungapped_cutoff(matrix_path.c_str(), P.gap_open, P.gap_extend, P.min_raw_gapped_score, ql[0],
-                          letters / db.size(), &cut, &info);
-      if (info.found) {
-        const size_t slash = matrix_path.rfind('/');
-        const std::string base = slash == std::string::npos ? matrix_path : matrix_path.substr(slash + 1);
-        fprintf(stderr, "# rmblastn: Karlin-Altschul ungapped cutoff IN EFFECT (matrix=%s gapopen=%d gapextend=%d)\n",
-                base.c_str(), P.gap_open, P.gap_extend);
-        fprintf(stderr, "#   lambda=%.10f K=%.10f E=0.05 avg_db_seq_len=%llu searchsp=%llu -> KA score=%d\n",
-                info.lambda, info.k, (unsigned long long)(letters / db.size()), (unsigned long long)info.searchsp,
-                info.es);
-        fprintf(stderr, "#   ungapped_cutoff = %d (overrides default min_raw_gapped_score/2 = %d)\n", cut,
-                P.min_raw_gapped_score / 2);
-      }
-    }
-    if (rmb_search(ctx, (uint32_t)qs.size(), qp.data(), ql.data())) return fail("search");
-    rmb_results_size(ctx, &nh, &no);
-    H.resize(nh);
-    O.resize(no);
-    rmb_results_copy(ctx, H.data(), O.data());
-  }
   std::string out;
   uint64_t db_letters = 0;
   for (const Record& r : db) db_letters += r.bases.size();
@@ -605,21 +567,62 @@ int main(int argc, char** argv) {
   std::vector<const char*> qids, sids;
   for (const Record& r : qs) qids.push_back(r.id.c_str());
   for (const Record& r : db) sids.push_back(r.id.c_str());
-  if (!pairwise && nh) {
-    // mask level, output order, statistics and the rows: the library's host tail
-    std::string fl;
-    for (const std::string& f : fields) fl += (fl.empty() ? "" : " ") + f;
-    rmb_report_spec spec{};
-    spec.mask_level = mask_level;
-    spec.ka = have_ka ? &ka : nullptr;
-    spec.evalue_max = reap_on ? reap_threshold : 0.0;
-    spec.fields = fl.c_str();
-    spec.query_ids = qids.data();
-    spec.subject_ids = sids.data();
-    uint64_t nb = 0;
-    if (rmb_report_tabular(ctx, &spec, &nb)) return fail("report");
-    out.resize(nb);
-    if (rmb_report_copy(ctx, out.data())) return fail("report");
+  // outfmt 6: the library writes the rows while the search runs (mask level,
+  // output order, statistics, text: its host tail)
+  std::string fl;
+  for (const std::string& f : fields) fl += (fl.empty() ? "" : " ") + f;
+  rmb_report_spec spec{};
+  spec.mask_level = mask_level;
+  spec.ka = have_ka ? &ka : nullptr;
+  spec.evalue_max = reap_on ? reap_threshold : 0.0;
+  spec.fields = fl.c_str();
+  spec.query_ids = qids.data();
+  spec.subject_ids = sids.data();
+  uint64_t nh = 0, no = 0;
+  std::vector<rmb_hsp> H;
+  std::vector<rmb_op> O;
+  if (!db.empty() && !qs.empty()) {  // a gilist may select no subject: a search with no hits
+    std::vector<const uint8_t*> ptr;
+    std::vector<uint32_t> len;
+    for (auto& r : db) { ptr.push_back(r.bases.data()); len.push_back((uint32_t)r.bases.size()); }
+    if (rmb_library_load(ctx, (uint32_t)db.size(), ptr.data(), len.data())) return fail("library");
+    std::vector<const uint8_t*> qp;
+    std::vector<uint32_t> ql;
+    for (auto& r : qs) { qp.push_back(r.bases.data()); ql.push_back((uint32_t)r.bases.size()); }
+    // The run-log note of a KA-derived ungapped cutoff, once, with the first
+    // query's numbers (main.rs:1098-1131)
+    if (P.ka_lambda > 0 && P.ka_k > 0) {
+      uint64_t letters = 0;
+      for (auto& r : db) letters += r.bases.size();
+      int32_t cut = 0;
+      rmb_ka_cutoff_info info{};
+      rmb_ungapped_cutoff(matrix_path.c_str(), P.gap_open, P.gap_extend, P.min_raw_gapped_score, ql[0],
+                          letters / db.size(), &cut, &info);
+      if (info.found) {
+        const size_t slash = matrix_path.rfind('/');
+        const std::string base = slash == std::string::npos ? matrix_path : matrix_path.substr(slash + 1);
+        fprintf(stderr, "# rmblastn: Karlin-Altschul ungapped cutoff IN EFFECT (matrix=%s gapopen=%d gapextend=%d)\n",
+                base.c_str(), P.gap_open, P.gap_extend);
+        fprintf(stderr, "#   lambda=%.10f K=%.10f E=0.05 avg_db_seq_len=%llu searchsp=%llu -> KA score=%d\n",
+                info.lambda, info.k, (unsigned long long)(letters / db.size()), (unsigned long long)info.searchsp,
+                info.es);
+        fprintf(stderr, "#   ungapped_cutoff = %d (overrides default min_raw_gapped_score/2 = %d)\n", cut,
+                P.min_raw_gapped_score / 2);
+      }
+    }
+    if (!pairwise) {
+      uint64_t nb = 0;
+      if (rmb_search_report(ctx, (uint32_t)qs.size(), qp.data(), ql.data(), &spec, &nb)) return fail("search");
+      out.resize(nb);
+      if (rmb_report_copy(ctx, out.data())) return fail("report");
+      rmb_results_size(ctx, &nh, &no);
+    } else {
+      if (rmb_search(ctx, (uint32_t)qs.size(), qp.data(), ql.data())) return fail("search");
+      rmb_results_size(ctx, &nh, &no);
+      H.resize(nh);
+      O.resize(no);
+      rmb_results_copy(ctx, H.data(), O.data());
+    }
   }
   if (pairwise) {
     std::vector<uint64_t> order;
