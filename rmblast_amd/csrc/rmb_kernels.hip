@@ -1131,11 +1131,20 @@ __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, in
   as.init(t.a, m, rev);
   bs.init(t.b, n, rev);
   auto bcode = [&](uint32_t col) -> uint32_t { return col < n ? (bs.at(col) & 15u) : 0u; };
-  uint32_t flat_len = 0;
+  // traceback bytes go out four at a time: a lane's stores then fill each
+  // cache line in 16 word writes instead of 64 byte writes (the word after
+  // the last byte is flushed whole; its bytes past the end are never read)
+  uint32_t flat_len = 0, acc = 0;
+  auto put = [&](uint32_t v) {
+    const uint32_t sh = (flat_len & 3u) * 8u;
+    acc = (sh ? acc : 0u) | (v << sh);
+    if (sh == 24u) *reinterpret_cast<uint32_t*>(flat + flat_len - 3) = acc;
+    ++flat_len;
+  };
   ring[0] = enc(0, -oe);
   bring[0] = (uint8_t)bcode(0);
   rows[0] = 0; rows[1] = 0;
-  flat[flat_len++] = SCRIPT_GAP_IN_B;
+  put(SCRIPT_GAP_IN_B);
   int32_t score = -oe;
   uint32_t b_size = 1;
   for (uint32_t j = 1; j <= n; ++j) {
@@ -1143,7 +1152,7 @@ __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, in
     if (j > wmask) { res.flags |= E_BAND_WIDE; return; }
     ring[(j & wmask) * RS] = enc(score, score - oe);
     bring[(j & wmask) * RS] = (uint8_t)bcode(j);
-    flat[flat_len++] = SCRIPT_GAP_IN_A;
+    put(SCRIPT_GAP_IN_A);
     score -= ge;
     b_size = j + 1;
   }
@@ -1197,7 +1206,7 @@ __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, in
         else op |= SCRIPT_EXTEND_GAP_A;
       }
       sc = next;
-      flat[flat_len++] = op;
+      put(op);
     }
     if (first_b >= b_size) {
       if (best_here) { bytes_at_best = flat_len; rows_at_best = ai + 2; }
@@ -1210,7 +1219,7 @@ __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, in
         if (b_size - first_b > wmask) { res.flags |= E_BAND_WIDE; return; }
         ring[(b_size & wmask) * RS] = enc(sgr - cbase, sgr - oe - cbase);
         bring[(b_size & wmask) * RS] = (uint8_t)bcode(b_size);
-        flat[flat_len++] = SCRIPT_GAP_IN_A;
+        put(SCRIPT_GAP_IN_A);
         sgr -= ge;
         ++b_size;
       }
@@ -1223,6 +1232,7 @@ __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, in
     }
     if (best_here) { bytes_at_best = flat_len; rows_at_best = ai + 2; }
   }
+  if (flat_len & 3u) *reinterpret_cast<uint32_t*>(flat + (flat_len & ~3u)) = acc;
   res.score = best;
   res.a_len = best_a;
   res.b_len = best_b;
