@@ -1109,13 +1109,15 @@ static constexpr uint32_t kDpLdsWideLog2 = 8;
 // never read by the walk.  Every other value is exact, so every decision is
 // dp_half's.  Kept cells lie in [base - X - open cost, base + ring cells x
 // largest score]; a task for which that could leave the 16-bit range comes
-// back E_BAND_WIDE, for the retry with dp_half.
-template <uint32_t RLOG2>
+// back E_BAND_WIDE, for the retry with dp_half.  Without TB it is the
+// score-only DP (the preliminary DPs' wide rings), rings interleaved across
+// RS lanes.
+template <uint32_t RLOG2, bool TB = true, uint32_t RS = kDpLdsThreads>
 __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, int32_t go, int32_t ge,
                                              uint32_t* __restrict__ ring, uint8_t* __restrict__ bring,
                                              int32_t max_pos, DpResult& res, uint8_t* __restrict__ flat,
                                              uint32_t* __restrict__ rows, uint32_t flat_cap, uint32_t rows_cap) {
-  constexpr uint32_t RS = kDpLdsThreads, wmask = (1u << RLOG2) - 1;
+  constexpr uint32_t wmask = (1u << RLOG2) - 1;
   res.score = 0; res.a_len = 0; res.b_len = 0; res.flags = 0; res.cells = 0; res.tb_bytes = 0; res.tb_rows = 0;
   const uint32_t m = t.m, n = t.n;
   if (m == 0 || n == 0) return;
@@ -1136,14 +1138,16 @@ __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, in
   // the last byte is flushed whole; its bytes past the end are never read)
   uint32_t flat_len = 0, acc = 0;
   auto put = [&](uint32_t v) {
-    const uint32_t sh = (flat_len & 3u) * 8u;
-    acc = (sh ? acc : 0u) | (v << sh);
-    if (sh == 24u) *reinterpret_cast<uint32_t*>(flat + flat_len - 3) = acc;
-    ++flat_len;
+    if constexpr (TB) {
+      const uint32_t sh = (flat_len & 3u) * 8u;
+      acc = (sh ? acc : 0u) | (v << sh);
+      if (sh == 24u) *reinterpret_cast<uint32_t*>(flat + flat_len - 3) = acc;
+      ++flat_len;
+    }
   };
   ring[0] = enc(0, -oe);
   bring[0] = (uint8_t)bcode(0);
-  rows[0] = 0; rows[1] = 0;
+  if (TB) { rows[0] = 0; rows[1] = 0; }
   put(SCRIPT_GAP_IN_B);
   int32_t score = -oe;
   uint32_t b_size = 1;
@@ -1161,12 +1165,14 @@ __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, in
   uint64_t cells = 0;
   uint32_t bytes_at_best = flat_len, rows_at_best = 1;
   for (uint32_t ai = 0; ai < m; ++ai) {
-    if (ai + 2 > rows_cap || flat_len + 2 * wmask + 4 > flat_cap) { res.flags |= E_TB_SPACE; return; }
+    if (TB && (ai + 2 > rows_cap || flat_len + 2 * wmask + 4 > flat_cap)) { res.flags |= E_TB_SPACE; return; }
     const int32_t pbase = cbase;  // the cells read in this row were written with it
     cbase = best;
     const int* row = mat + ((as.at(ai) & 15u) << 4);
-    rows[2 * (ai + 1)] = flat_len;
-    rows[2 * (ai + 1) + 1] = first_b;
+    if (TB) {
+      rows[2 * (ai + 1)] = flat_len;
+      rows[2 * (ai + 1) + 1] = first_b;
+    }
     int32_t sc = kMinInt, sgr = kMinInt;
     uint32_t last_b = first_b;
     bool best_here = false;
@@ -1232,7 +1238,7 @@ __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, in
     }
     if (best_here) { bytes_at_best = flat_len; rows_at_best = ai + 2; }
   }
-  if (flat_len & 3u) *reinterpret_cast<uint32_t*>(flat + (flat_len & ~3u)) = acc;
+  if (TB && (flat_len & 3u)) *reinterpret_cast<uint32_t*>(flat + (flat_len & ~3u)) = acc;
   res.score = best;
   res.a_len = best_a;
   res.b_len = best_b;
@@ -1809,11 +1815,12 @@ __device__ __forceinline__ void prelim_pivot(const UngHit& u, uint32_t qa_len, u
 // records (P.rec_ids); the task count may come from device memory (P.n_dev),
 // so the passes queue behind the seeding kernel without a host round trip.
 static constexpr uint32_t kPrelimThreads = 128, kPrelimRingLog2 = 4, kPrelimRun = 1;
-// wider rings (72 KB of LDS per workgroup, two per CU), for X-drops whose
-// bands outgrow 16 cells
-static constexpr uint32_t kPrelimWideLog2 = 6;
+// wider rings for X-drops whose bands outgrow 16 cells: 64 cells of 4 bytes
+// (H and E as 16-bit offsets, dp_half_tb16's score-only form) and a code
+// byte per lane, in 64-lane workgroups of 21 KB of LDS, seven per CU
+static constexpr uint32_t kPrelimWideLog2 = 6, kPrelimPackThreads = 64;
 
-template <bool LDS_RING, uint32_t RLOG2>
+template <bool LDS_RING, uint32_t RLOG2, bool PACK = false>
 __global__ __launch_bounds__(kPrelimThreads) void k_prelim_fused(
     PrelimParams P, const UngHit* __restrict__ hits, const uint32_t* __restrict__ idx, uint64_t n_tasks,
     const RecDesc* __restrict__ recs, const SubjDesc* __restrict__ subjs, const uint8_t* __restrict__ qfwd,
@@ -1821,13 +1828,27 @@ __global__ __launch_bounds__(kPrelimThreads) void k_prelim_fused(
     int2* __restrict__ gring, uint32_t wlog2, PreHit* __restrict__ out, unsigned long long* __restrict__ out_count,
     uint32_t* __restrict__ redo, UngHit* __restrict__ redo_hits, unsigned long long* __restrict__ redo_count,
     unsigned long long* __restrict__ counters, uint32_t* __restrict__ err) {
+  constexpr uint32_t T = PACK ? kPrelimPackThreads : kPrelimThreads;  // the workgroup's lanes
   __shared__ int mat[256];
-  __shared__ int2 lring[LDS_RING ? (kPrelimThreads << RLOG2) : 1];
-  __shared__ uint8_t lbring[LDS_RING ? (kPrelimThreads << RLOG2) : 1];
-  for (int i = threadIdx.x; i < 256; i += blockDim.x) mat[i] = gmat[i];
+  __shared__ int2 lring[LDS_RING && !PACK ? (T << RLOG2) : 1];
+  __shared__ uint32_t lring16[PACK ? (T << RLOG2) : 1];
+  __shared__ uint8_t lbring[LDS_RING ? (T << RLOG2) : 1];
+  __shared__ int s_max_pos;
+  static_assert(!PACK || LDS_RING, "packed rings live in LDS");
+  if (threadIdx.x == 0) s_max_pos = 0;
   __syncthreads();
+  int my_max = 0;  // largest score among codes 0..14 (0 if none is positive)
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) {
+    const int v = gmat[i];
+    mat[i] = v;
+    if ((i >> 4) < 15 && (i & 15) < 15) my_max = max(my_max, v);
+  }
+  if (PACK) atomicMax(&s_max_pos, my_max);
+  __syncthreads();
+  const int32_t max_pos = s_max_pos;
+  uint32_t* my16 = lring16 + threadIdx.x;
   const uint64_t gt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (uint64_t)gridDim.x * blockDim.x;
-  constexpr uint32_t RS = LDS_RING ? kPrelimThreads : 1;
+  constexpr uint32_t RS = LDS_RING ? T : 1;
   int2* my = LDS_RING ? lring + threadIdx.x : gring + gt * ring_stride(wlog2);
   uint8_t* myb = LDS_RING ? lbring + threadIdx.x : nullptr;
   const uint32_t wmask = LDS_RING ? (1u << RLOG2) - 1 : (1u << wlog2) - 1;
@@ -1867,8 +1888,15 @@ __global__ __launch_bounds__(kPrelimThreads) void k_prelim_fused(
     DpTask left{sa, qa, ps, pq, xd, 1};
     DpTask right{sa + ps, qa + pq, sl > ps ? sl - ps : 0, ql > pq ? ql - pq : 0, xd, 0};
     DpResult L, Rr;
-    dp_half<false, RS>(left, mat, P.gap_open, P.gap_extend, my, myb, wmask, L, nullptr, nullptr, 0);
-    dp_half<false, RS>(right, mat, P.gap_open, P.gap_extend, my, myb, wmask, Rr, nullptr, nullptr, 0);
+    if constexpr (PACK) {
+      dp_half_tb16<RLOG2, false, RS>(left, mat, P.gap_open, P.gap_extend, my16, myb, max_pos, L, nullptr, nullptr,
+                                     0u, 0u);
+      dp_half_tb16<RLOG2, false, RS>(right, mat, P.gap_open, P.gap_extend, my16, myb, max_pos, Rr, nullptr, nullptr,
+                                     0u, 0u);
+    } else {
+      dp_half<false, RS>(left, mat, P.gap_open, P.gap_extend, my, myb, wmask, L, nullptr, nullptr, 0);
+      dp_half<false, RS>(right, mat, P.gap_open, P.gap_extend, my, myb, wmask, Rr, nullptr, nullptr, 0);
+    }
     if ((L.flags | Rr.flags) & E_BAND_WIDE) {
       const unsigned long long kk = atomicAdd(redo_count, 1ull);
       if (kk < P.redo_cap) {
@@ -1911,7 +1939,7 @@ hipError_t launch_prelim_fused(int blocks, bool lds_ring, PrelimParams P, const 
                                UngHit* redo_hits, unsigned long long* redo_count, unsigned long long* counters,
                                uint32_t* err, bool wide, hipStream_t st) {
   if (lds_ring && wide)
-    hipLaunchKernelGGL((k_prelim_fused<true, kPrelimWideLog2>), dim3(blocks), dim3(kPrelimThreads), 0, st, P, hits,
+    hipLaunchKernelGGL((k_prelim_fused<true, kPrelimWideLog2, true>), dim3(blocks), dim3(kPrelimPackThreads), 0, st, P, hits,
                        idx, n_tasks, recs, subjs, qfwd, qrc, splus, gmat, gring, wlog2, out, out_count, redo,
                        redo_hits, redo_count, counters, err);
   else if (lds_ring)

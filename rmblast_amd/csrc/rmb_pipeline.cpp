@@ -877,13 +877,14 @@ static void enqueue_batch(rmb_ctx* c, P1Batch& b, const uint8_t* d_qfwd, const u
     PP.next = cnt + 4 + 2 * gi;
     hip_check(hipEventRecord(S.ev[6 * gi + 2], st), "event");
     // past 12 band cells ((X - open) / extend to either side), 64-cell rings
-    // with two workgroups per CU (RMB_PRELIM_WIDE=0 keeps the 16-cell rings)
+    // of 4-byte cells in 64-lane workgroups, seven per CU (RMB_PRELIM_WIDE=0
+    // keeps the 16-cell rings)
     const char* pw_env = std::getenv("RMB_PRELIM_WIDE");
     const bool pwide = c->tune.prelim_ring ? c->tune.prelim_ring == 64
                                            : !(pw_env && std::atoi(pw_env) == 0) &&
                                                  2 * std::max(0, (int)c->P.xdrop_gap - (int)c->P.gap_open) /
                                                          std::max(1, (int)c->P.gap_extend) + 1 > 12;
-    const int pre_per_cu = c->tune.prelim_wg_per_cu ? c->tune.prelim_wg_per_cu : (pwide ? 2 : 8);
+    const int pre_per_cu = c->tune.prelim_wg_per_cu ? c->tune.prelim_wg_per_cu : (pwide ? 7 : 8);
     hip_check(launch_prelim_fused(c->cu_count * pre_per_cu, true, PP, dh, nullptr, g.hits_cap, drecs,
                                   (const SubjDesc*)c->d_sdesc.p, d_qfwd, d_qrc, (const uint8_t*)c->d_splus.p,
                                   c->d_mat, nullptr, 0, dpre, cnt, dredo, nullptr, hitcnt + 2 * gi + 1, ctr, err, pwide, st),
