@@ -36,6 +36,8 @@ struct P1Params {
   uint64_t out_cap;
   const uint32_t* item_list; // null: every (record, subject) item; else these item ids
   int32_t known_word;        // whole-word table (ws >= 7, scan step 1) and positive ACGT match scores
+  uint32_t part_log2;        // each item is searched by 2^part_log2 workgroups, each owning as many
+                             // contiguous ranges of the 512 diagonal-hash buckets (0: one per item)
 };
 
 constexpr uint32_t kNoItem = 0xFFFFFFFFu;  // UngHit::item of an output slot reserved but left unused

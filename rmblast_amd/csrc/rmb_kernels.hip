@@ -292,6 +292,21 @@ __device__ __forceinline__ void sort8(uint32_t (&v)[8]) {
   }
 }
 
+// Bit position of the n-th (from 0) set bit of m (n < popcount(m)).
+__device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t n) {
+  uint32_t pos = 0;
+#pragma unroll
+  for (uint32_t w = 32; w >= 1; w >>= 1) {
+    const uint64_t low = m & ((1ull << w) - 1ull);
+    const uint32_t c = (uint32_t)__popcll(low);
+    const bool up = n >= c;
+    n -= up ? c : 0u;
+    m = up ? (m >> w) : low;
+    pos += up ? w : 0u;
+  }
+  return pos;
+}
+
 // ------------------------------------------------------------ Phase 1 -------
 // One workgroup per (record, subject) item at a time, items from a device
 // queue.  Per pass it stages up to kSeedCap seeds of the reference scan order
@@ -314,7 +329,12 @@ constexpr uint32_t kOutChunk = 256;  // output slots a workgroup reserves at a t
 // (kHitLds entries, the rest in the workgroup's global area).
 constexpr uint32_t kHitLds = 128;
 constexpr uint32_t kHitSlotShift = 21;  // sd[].w: steps (16 bits) | flags << 16 (5 bits) | hit slot << 21
-template <bool SEEDS, bool WORD>
+// SPLIT: items split over bucket ranges (P.part_log2 > 0).  With WORD, a
+// pass's seeds are first materialised (B1) and each wave then extends only
+// those on its part's diagonals (B2), handed to its lanes in scan order, so
+// the fast tier's lanes stay full; otherwise the other parts' seeds are
+// dropped where they are met.
+template <bool SEEDS, bool WORD, bool SPLIT = false>
 __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
     P1Params P, const RecDesc* __restrict__ recs, const SubjDesc* __restrict__ subjs,
     const uint8_t* __restrict__ qfwd, const uint8_t* __restrict__ qrc, const uint8_t* __restrict__ qext,
@@ -332,6 +352,7 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
   __shared__ uint8_t posmap[kSeedCap];  // pass seed -> its scan position in the window
   __shared__ uint32_t bcnt[kBuckets], bst[kBuckets];
   __shared__ unsigned long long s_item, s_out_next, s_pass_a, s_pass_b;
+  __shared__ uint32_t s_part;
   __shared__ uint32_t s_out_left, s_pass_an, s_nemit, s_emit_i, s_nlong, s_nhit;
   __shared__ uint2 s_hit[WORD ? kHitLds : 1];  // WORD: {left end, score} of the pass's hits
 
@@ -391,14 +412,21 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
       // XCD-aware queues: queue x hands out the items of records r = x (mod 8),
       // record-major, so one XCD's workgroups share a record's lookup table in
       // their L2; an XCD whose queue is empty takes work from the others.
+      // An item split over 2^part_log2 workgroups is 2^part_log2 consecutive
+      // work units, one per part.
       unsigned long long it = ~0ull;
+      uint32_t part = 0;
+      const uint32_t pl = SPLIT ? P.part_log2 : 0u;
       if (P.item_list) {  // listed items: the list is cut into kXcds contiguous slices
         for (int k = 0; k < kXcds && it == ~0ull; ++k) {
           const uint32_t x = (xcd + k) % kXcds;
           const unsigned long long lo = P.n_items * x / kXcds, hi = P.n_items * (x + 1) / kXcds;
           if (hi == lo) continue;
           const unsigned long long t = atomicAdd(&queue[x], 1ull);
-          if (t < hi - lo) it = P.item_list[lo + t];
+          if (t < ((hi - lo) << pl)) {
+            it = P.item_list[lo + (t >> pl)];
+            part = (uint32_t)t & ((1u << pl) - 1u);
+          }
         }
       } else {
         const uint32_t n_recs = (uint32_t)(P.n_items / P.n_subj);
@@ -407,11 +435,16 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
           const unsigned long long n_x =
               (unsigned long long)(n_recs > x ? (n_recs - x + kXcds - 1) / kXcds : 0) * P.n_subj;
           if (n_x == 0) continue;
-          const unsigned long long t = atomicAdd(&queue[x], 1ull);
-          if (t < n_x) it = (unsigned long long)(x + kXcds * (t / P.n_subj)) * P.n_subj + t % P.n_subj;
+          const unsigned long long tu = atomicAdd(&queue[x], 1ull);
+          if (tu < (n_x << pl)) {
+            const unsigned long long t = tu >> pl;
+            it = (unsigned long long)(x + kXcds * (t / P.n_subj)) * P.n_subj + t % P.n_subj;
+            part = (uint32_t)tu & ((1u << pl) - 1u);
+          }
         }
       }
       s_item = it;
+      s_part = part;
       s_nemit = 0;
       s_emit_i = 0;
     }
@@ -419,6 +452,8 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
     __syncthreads();
     const unsigned long long item = s_item;
     if (item == ~0ull) break;
+    // the buckets this workgroup owns: b >> own_shift == part (all when unsplit)
+    const uint32_t part = SPLIT ? s_part : 0u, own_shift = 9u - (SPLIT ? P.part_log2 : 0u);
     const uint32_t r = (uint32_t)(item / P.n_subj), f = (uint32_t)(item % P.n_subj);
     const RecDesc R = recs[r];
     const SubjDesc S = subjs[f];
@@ -476,7 +511,7 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
         }
       }
       const uint32_t take = (wtotal - wk) < (uint32_t)kSeedCap ? (wtotal - wk) : (uint32_t)kSeedCap;
-      if (tid == 0) s_nseeds += take;
+      if (tid == 0 && part == 0) s_nseeds += take;  // staged once per item
       // the pass's seed -> scan position map: each position's thread writes
       // the entries of its own seeds
       {
@@ -487,6 +522,81 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
       __syncthreads();
       // ---- B+C: materialise each seed, exact word extension, diagonal,
       // speculative ungapped extension (the lane that fills a seed extends it)
+      if constexpr (SPLIT && WORD) {
+        // B1: every seed of the pass materialised; the wave's seeds on this
+        // part's diagonals recorded per iteration as ballot masks
+        constexpr int kIters = (kSeedCap + kP1Threads - 1) / kP1Threads;
+        uint64_t own[kIters];
+        uint32_t n_own = 0;
+#pragma unroll
+        for (int k = 0; k < kIters; ++k) {
+          const uint32_t i = (uint32_t)tid + (uint32_t)(kP1Threads * k);
+          bool owned = false;
+          if (i < take) {
+            const uint32_t g = wk + i;
+            const int lo = posmap[i];
+            const uint32_t prev = lo ? s_cum[lo - 1] : 0;
+            const int32_t s_fwd = (int32_t)((w0 + (uint32_t)lo) * (uint32_t)step);
+            const int32_t q0c = (int32_t)entries[s_cst[lo] + (g - prev)];
+            const int32_t diag = (int32_t)((uint32_t)s_fwd - (uint32_t)q0c);
+            owned = (((uint32_t)diag & (uint32_t)(kBuckets - 1)) >> own_shift) == part;
+            const uint32_t fl = owned ? (F_VALID | (q0c >= c1 ? F_MINUS : 0)) : 0u;
+            sd[i] = make_uint4(owned ? (uint32_t)diag : 0u, owned ? (uint32_t)s_fwd : 0u, 0u, fl << 16);
+          }
+          own[k] = __ballot(owned);
+          n_own += (uint32_t)__popcll(own[k]);
+        }
+        // B2: the wave's owned seeds, its lanes taking them in scan order
+        const uint32_t lane = (uint32_t)tid & 63u, wbase = (uint32_t)tid & ~63u;
+        for (uint32_t j0 = 0; j0 < n_own; j0 += 64) {
+          const uint32_t j = j0 + lane;
+          const bool act = j < n_own;
+          uint32_t i = 0, r = j;
+          bool found = false;
+#pragma unroll
+          for (int k = 0; k < kIters; ++k) {
+            const uint32_t ck = (uint32_t)__popcll(own[k]);
+            const bool here = !found && r < ck;
+            if (here) i = (uint32_t)(kP1Threads * k) + wbase + nth_set_bit(own[k], r);
+            r -= (!found && !here) ? ck : 0u;
+            found = found || here;
+          }
+          bool longer = false;
+          if (act) {
+            const uint4 e0 = sd[i];
+            const bool minus = ((e0.w >> 16) & F_MINUS) != 0;
+            const int32_t s_fwd = (int32_t)e0.y, diag = (int32_t)e0.x;
+            const int32_t q0c = s_fwd - diag;
+            const int32_t eq = minus ? q0c - c1 + lw - 1 : q0c, es = minus ? s_fwd + lw - 1 : s_fwd;
+            const uint32_t chk = (uint32_t)s_fwd;
+            const uint8_t* qpool = minus ? qrc : qfwd;
+            const int64_t qa = (int64_t)R.q_base + 1 + eq, sa = (int64_t)S.s_base + 1 + es;
+            uint32_t level, steps = 0, flag;
+            UngOut u;
+            if (ungapped_first(qpool, qa, splus, sa, eq, minus, mat, P.xdrop_ungap, u)) {
+              const bool hit = u.score >= R.cutoff;
+              const uint32_t slot = put_hit(hit, u.q_beg, u.score);
+              level = hit ? (uint32_t)es + (u.q_end - (uint32_t)eq) : chk + (uint32_t)ws;
+              steps = (u.steps < 65535u ? u.steps : 65535u) | (hit ? slot << kHitSlotShift : 0u);
+              flag = F_VALID | (hit ? F_HIT : 0) | (minus ? F_MINUS : 0);
+            } else {  // the full extension runs below; level holds eq meanwhile
+              longer = true;
+              level = (uint32_t)eq;
+              flag = F_VALID | F_LONG | (minus ? F_MINUS : 0);
+            }
+            atomicAdd(&bcnt[(uint32_t)diag & (kBuckets - 1)], 1u);
+            sd[i] = make_uint4((uint32_t)diag, chk, level, steps | (flag << 16));
+          }
+          const unsigned long long m = __ballot(longer);
+          if (m) {
+            const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&s_nlong, (uint32_t)__popcll(m));
+            base = __shfl(base, (int)leader, 64);
+            if (longer) llist[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)i;
+          }
+        }
+      } else
       for (uint32_t i = tid; i < take; i += kP1Threads) {
         const uint32_t g = wk + i;  // seed index within the window
         const int lo = posmap[i];
@@ -570,6 +680,8 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
           es = s_len - 1 - s_off;
         }
         bool longer = false;
+        // a seed on a diagonal of another part's buckets is that part's
+        if (SPLIT) ok = ok && (((uint32_t)diag & (uint32_t)(kBuckets - 1)) >> own_shift) == part;
         if (ok) {
           // one extension call for both strands, so a wave holding seeds of
           // both runs it once
@@ -1978,7 +2090,8 @@ hipError_t launch_phase1(int blocks, P1Params P, const RecDesc* recs, const Subj
                          unsigned long long* out_count, uint2* spill, uint4* hitres, unsigned long long* queue,
                          unsigned long long* counters, uint32_t* err, bool seeds, hipStream_t st) {
   auto kern = seeds ? (P.known_word ? k_phase1<true, true> : k_phase1<true, false>)
-                    : (P.known_word ? k_phase1<false, true> : k_phase1<false, false>);
+                    : P.part_log2 ? (P.known_word ? k_phase1<false, true, true> : k_phase1<false, false, true>)
+                                  : (P.known_word ? k_phase1<false, true> : k_phase1<false, false>);
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(kP1Threads), 0, st, P, recs, subjs, qfwd, qrc, qext, splus, src, cells,
                      entries, gmat, out, out_count, spill, hitres, queue, counters, err);
   return hipGetLastError();

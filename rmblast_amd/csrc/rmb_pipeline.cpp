@@ -790,10 +790,18 @@ static void launch_group_phase1(rmb_ctx* c, hipStream_t st, P1Group& g, const ui
     P.item_list = dl;
     P.n_items = g.items.size();
   }
-  // ~31 KB of LDS per workgroup: kP1PerCu resident per CU.  Workgroups are
+  // A search of few items would leave most of the GPU idle, each item's seeds
+  // replayed through the diagonal hash by one workgroup: its items are then
+  // split over up to 16 workgroups, each owning an equal range of the 512
+  // hash buckets (the buckets are independent, so the split is exact), as
+  // far as the resident workgroups allow.
+  const uint64_t slots = (uint64_t)c->cu_count * kP1PerCu;
+  while (!seeds && P.part_log2 < 4 && (P.n_items << (P.part_log2 + 1)) <= slots) ++P.part_log2;
+  if (const char* e = std::getenv("RMB_P1_PARTS_LOG2")) P.part_log2 = seeds ? 0u : (uint32_t)std::min(4, std::max(0, std::atoi(e)));
+  // ~30 KB of LDS per workgroup: kP1PerCu resident per CU.  Workgroups are
   // dealt round-robin over the XCDs, so blockIdx % kXcds names the XCD.
-  const int blocks =
-      std::max(1, std::min<int>(c->cu_count * kP1PerCu, (int)std::min<uint64_t>(P.n_items, 1u << 20)));
+  const int blocks = std::max(
+      1, std::min<int>(c->cu_count * kP1PerCu, (int)std::min<uint64_t>(P.n_items << P.part_log2, 1u << 20)));
   uint2* spill = c->b_spill.get<uint2>((size_t)blocks * kBuckets * kHashGlobal);
   uint4* hitres = c->b_hitres.get<uint4>((size_t)blocks * kSeedCap);
   unsigned long long* queue = c->b_queue.get<unsigned long long>(kXcds);

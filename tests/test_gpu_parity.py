@@ -75,6 +75,27 @@ def test_synthetic_parity_config_matches_oracle(tmp_path, seed):
     assert g == o and g.count("\n") > 100
 
 
+@pytest.mark.parametrize("parts_log2", ["0", "1", "3", "4"])
+@pytest.mark.parametrize("ws", [7, 11])
+def test_items_split_over_bucket_ranges_match_oracle(tmp_path, parts_log2, ws):
+    """A search of few (record, family) items splits each item over up to 16
+    workgroups, each owning a range of the diagonal hash's 512 buckets; the
+    rows are the oracle's whatever the split (RMB_P1_PARTS_LOG2 forces it)."""
+    w = synth.parity_config(seed=13, genome_len=300_000, n_families=20, n_copies=600)
+    synth.to_fasta(w.records, tmp_path / "q.fa")
+    synth.to_fasta(w.families, tmp_path / "db.fa")
+    (tmp_path / "m.matrix").write_text(synth.MATRIX_TEXT)
+    args = ["--db", str(tmp_path / "db.fa"), "--query", str(tmp_path / "q.fa"), "--matrix",
+            str(tmp_path / "m.matrix"), "--word-size", str(ws), "--gapopen", "20", "--gapextend", "5",
+            "--min-raw-gapped-score", "200", "--mask-level", "80", "--complexity-adjust", "--dust", "no",
+            "--outfmt", RM_FIELDS]
+    g = run([CLI] + args, env={"RMB_P1_PARTS_LOG2": parts_log2})
+    assert g.returncode == 0, g.stderr
+    o = run([ORACLE_BIN] + args)
+    assert o.returncode == 0, o.stderr
+    assert g.stdout == o.stdout and g.stdout.count("\n") > 100
+
+
 def test_gappy_divergent_and_microsatellites_match_oracle(tmp_path):
     rng = np.random.default_rng(5)
     fams = synth.make_families(rng, 6, [5000, 8000, 6000, 7000, 9000, 5500])

@@ -143,7 +143,10 @@ def test_autotuner_keeps_results_and_writes_its_table(tmp_path):
                              word_size=7, min_raw_gapped_score=200, complexity_adjust=True)
     s.load_library([f[1] for f in w.families])
     best, t, trials = autotune.tune(s, [r[1] for r in w.records], reps=1, log=lambda m: None)
-    assert len(trials) >= 10 and t == min(x[1] for x in trials)
+    # a value is kept only when it beats the best so far by the margin, so no
+    # trial is more than the margin faster than the schedule kept
+    assert len(trials) >= 10 and min(x[1] for x in trials) >= t * (1.0 - autotune.MARGIN)
+    assert any(x[1] == t for x in trials)
     path = tmp_path / "tuning.tsv"
     autotune.write_entry(str(path), ("m.matrix", 7, 30, 100), best, t * 1e3, "test")
     assert rmblast_amd.read_tuning(str(path))[("m.matrix", 7, 30, 100)] == {k: best[k] for k in best}
