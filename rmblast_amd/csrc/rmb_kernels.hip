@@ -1209,6 +1209,16 @@ static constexpr uint32_t kDpLdsThreads = 64, kDpLdsLog2 = 6;
 // wider one-pass rings (81 KB of LDS per workgroup, one per CU), for
 // X-drops whose bands outgrow 64 cells
 static constexpr uint32_t kDpLdsWideLog2 = 8;
+// words of traceback bytes a lane gathers before writing them out at once
+static constexpr uint32_t kTbChunkWords = 8;
+// Row trailers of the traceback stream: a row holds at most twice the ring's
+// cells, so one byte per field serves rings of up to 64 cells, two bytes wider
+// ones.
+__host__ __device__ constexpr uint32_t tb_trailer_bytes(uint32_t rlog2) { return rlog2 <= 6 ? 2u : 4u; }
+__host__ __device__ constexpr uint32_t tb_trailer_max(uint32_t tr) { return tr == 2 ? 255u : 65535u; }
+struct TbRowStart {  // where the best cell's row starts in the stream
+  uint32_t off, fb, dfb;  // first byte, first column, first column's move from the row below
+};
 
 // The traceback DP of k_dp_onepass with 4-byte LDS ring cells, so twice as
 // many lanes fit a CU: H and E as 16-bit offsets from the best score at the
@@ -1228,8 +1238,9 @@ template <uint32_t RLOG2, bool TB = true, uint32_t RS = kDpLdsThreads>
 __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, int32_t go, int32_t ge,
                                              uint32_t* __restrict__ ring, uint8_t* __restrict__ bring,
                                              int32_t max_pos, DpResult& res, uint8_t* __restrict__ flat,
-                                             uint32_t* __restrict__ rows, uint32_t flat_cap, uint32_t rows_cap) {
+                                             uint32_t flat_cap, TbRowStart& at_best) {
   constexpr uint32_t wmask = (1u << RLOG2) - 1;
+  constexpr uint32_t TR = tb_trailer_bytes(RLOG2);
   res.score = 0; res.a_len = 0; res.b_len = 0; res.flags = 0; res.cells = 0; res.tb_bytes = 0; res.tb_rows = 0;
   const uint32_t m = t.m, n = t.n;
   if (m == 0 || n == 0) return;
@@ -1245,21 +1256,56 @@ __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, in
   as.init(t.a, m, rev);
   bs.init(t.b, n, rev);
   auto bcode = [&](uint32_t col) -> uint32_t { return col < n ? (bs.at(col) & 15u) : 0u; };
-  // traceback bytes go out four at a time: a lane's stores then fill each
-  // cache line in 16 word writes instead of 64 byte writes (the word after
-  // the last byte is flushed whole; its bytes past the end are never read)
+  // The traceback stream: each row's bytes, then its trailer (the row's byte
+  // count and how far its first column moved from the row before, TR/2 bytes
+  // each), so the walk steps down a row from the trailer below the row it is
+  // in, and no row table is written.  The stream leaves the lane in whole
+  // aligned chunks of kTbChunkWords words, each written once: a lane's arena
+  // lines are then never written back part-filled and written again.  Bytes
+  // gather in `acc`, completed words shift through `q`; the last chunk is
+  // padded, and its bytes past the end are never read.
   uint32_t flat_len = 0, acc = 0;
+  uint32_t q[kTbChunkWords];
+#pragma unroll
+  for (uint32_t k = 0; k < kTbChunkWords; ++k) q[k] = 0;
+  auto shift_in = [&](uint32_t w) {
+#pragma unroll
+    for (uint32_t k = 0; k + 1 < kTbChunkWords; ++k) q[k] = q[k + 1];
+    q[kTbChunkWords - 1] = w;
+  };
+  auto store_chunk = [&](uint32_t at) {
+#pragma unroll
+    for (uint32_t k = 0; k < kTbChunkWords; k += 4)
+      *reinterpret_cast<uint4*>(flat + at + 4 * k) = make_uint4(q[k], q[k + 1], q[k + 2], q[k + 3]);
+  };
   auto put = [&](uint32_t v) {
     if constexpr (TB) {
       const uint32_t sh = (flat_len & 3u) * 8u;
       acc = (sh ? acc : 0u) | (v << sh);
-      if (sh == 24u) *reinterpret_cast<uint32_t*>(flat + flat_len - 3) = acc;
+      if (sh == 24u) {
+        shift_in(acc);
+        if (((flat_len + 1) & (4 * kTbChunkWords - 1)) == 0) store_chunk(flat_len + 1 - 4 * kTbChunkWords);
+      }
       ++flat_len;
     }
   };
+  uint32_t row_off = 0, row_fb = 0, row_dfb = 0;  // the current row's start, first column, its move
+  auto put_trailer = [&]() -> bool {
+    const uint32_t len = flat_len - row_off;
+    if (len > tb_trailer_max(TR) || row_dfb > tb_trailer_max(TR)) return false;
+    if constexpr (TR == 2) {
+      put(len);
+      put(row_dfb);
+    } else {
+      put(len & 255u);
+      put(len >> 8);
+      put(row_dfb & 255u);
+      put(row_dfb >> 8);
+    }
+    return true;
+  };
   ring[0] = enc(0, -oe);
   bring[0] = (uint8_t)bcode(0);
-  if (TB) { rows[0] = 0; rows[1] = 0; }
   put(SCRIPT_GAP_IN_B);
   int32_t score = -oe;
   uint32_t b_size = 1;
@@ -1276,15 +1322,22 @@ __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, in
   uint32_t best_a = 0, best_b = 0, first_b = 0;
   uint64_t cells = 0;
   uint32_t bytes_at_best = flat_len, rows_at_best = 1;
+  if (TB) at_best = TbRowStart{0, 0, 0};
   for (uint32_t ai = 0; ai < m; ++ai) {
-    if (TB && (ai + 2 > rows_cap || flat_len + 2 * wmask + 4 > flat_cap)) { res.flags |= E_TB_SPACE; return; }
+    if (TB) {
+      // a row's bytes stay within twice the ring (its first column moves
+      // right inside the row by at most what the ring held at the start)
+      if (flat_len + 2 * (wmask + 1) + 2 * TR + 4 * kTbChunkWords > flat_cap || !put_trailer()) {
+        res.flags |= E_TB_SPACE;
+        return;
+      }
+      row_off = flat_len;
+      row_dfb = first_b - row_fb;
+      row_fb = first_b;
+    }
     const int32_t pbase = cbase;  // the cells read in this row were written with it
     cbase = best;
     const int* row = mat + ((as.at(ai) & 15u) << 4);
-    if (TB) {
-      rows[2 * (ai + 1)] = flat_len;
-      rows[2 * (ai + 1) + 1] = first_b;
-    }
     int32_t sc = kMinInt, sgr = kMinInt;
     uint32_t last_b = first_b;
     bool best_here = false;
@@ -1327,7 +1380,11 @@ __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, in
       put(op);
     }
     if (first_b >= b_size) {
-      if (best_here) { bytes_at_best = flat_len; rows_at_best = ai + 2; }
+      if (best_here) {
+        bytes_at_best = flat_len;
+        rows_at_best = ai + 2;
+        if (TB) at_best = TbRowStart{row_off, row_fb, row_dfb};
+      }
       break;
     }
     if (last_b + 1 < b_size) {
@@ -1348,9 +1405,20 @@ __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, in
       bring[(b_size & wmask) * RS] = (uint8_t)bcode(b_size);
       ++b_size;
     }
-    if (best_here) { bytes_at_best = flat_len; rows_at_best = ai + 2; }
+    if (best_here) {
+      bytes_at_best = flat_len;
+      rows_at_best = ai + 2;
+      if (TB) at_best = TbRowStart{row_off, row_fb, row_dfb};
+    }
   }
-  if (TB && (flat_len & 3u)) *reinterpret_cast<uint32_t*>(flat + (flat_len & ~3u)) = acc;
+  if (TB) {
+    if (flat_len & (4 * kTbChunkWords - 1)) {
+      uint32_t words = (flat_len >> 2) & (kTbChunkWords - 1);
+      if (flat_len & 3u) { shift_in(acc); ++words; }
+      for (; words < kTbChunkWords; ++words) shift_in(0);
+      store_chunk(flat_len & ~(4 * kTbChunkWords - 1));
+    }
+  }
   res.score = best;
   res.a_len = best_a;
   res.b_len = best_b;
@@ -1606,23 +1674,61 @@ __device__ __forceinline__ void dp_half_wave(const DpTask& t, const int* __restr
   res.tb_rows = rows_at_best;
 }
 
+// Where a row's traceback bytes lie, for the walk: from a row table (8 B per
+// row: first byte, first column; a row ends where the next begins) ...
+struct TbRowTable {
+  const uint32_t* rows;
+  uint32_t off, fb, end;
+  __device__ TbRowTable(const uint32_t* r, uint32_t a, uint32_t flat_total)
+      : rows(r), off(r[2 * a]), fb(r[2 * a + 1]), end(flat_total) {}
+  __device__ void down(const uint8_t*, uint32_t a) {  // to row a from row a + 1
+    end = off;
+    off = rows[2 * a];
+    fb = rows[2 * a + 1];
+  }
+};
+
+// ... or from the stream's row trailers (dp_half_tb16): starting from the best
+// row's start, each step down reads the trailer just below the current row.
+template <uint32_t TR>
+struct TbRowStream {
+  uint32_t off, fb, end, dfb;
+  __device__ TbRowStream(const TbRowStart& s, uint32_t flat_total) : off(s.off), fb(s.fb), end(flat_total), dfb(s.dfb) {}
+  __device__ void down(const uint8_t* flat, uint32_t) {
+    // every row below the best one was closed by a trailer, so off >= TR,
+    // len <= end and dfb <= fb hold; the clamps only keep every read inside
+    // [0, flat_total) whatever the bytes say
+    fb = fb >= dfb ? fb - dfb : 0;
+    end = off >= TR ? off - TR : 0;
+    uint32_t len = 0, d = 0;
+    if (off >= TR) {
+      if constexpr (TR == 2) {
+        len = flat[end];
+        d = flat[end + 1];
+      } else {
+        len = flat[end] | ((uint32_t)flat[end + 1] << 8);
+        d = flat[end + 2] | ((uint32_t)flat[end + 3] << 8);
+      }
+    }
+    off = end - (len <= end ? len : end);
+    dfb = d;
+  }
+};
+
 // Traceback walk (gapped.rs:692-757) from (a, b), the best row and column,
 // over the rows' traceback bytes: the ops, run-length encoded, collected
 // backwards into ops[0, cap); the count is returned, E_TB_SPACE set in flags
 // when they do not fit or the walk does not reach the origin.
-__device__ __forceinline__ uint32_t tb_walk(const uint8_t* __restrict__ flat, const uint32_t* __restrict__ rows,
-                                            uint32_t a, uint32_t b, uint32_t flat_total, uint32_t* __restrict__ ops,
-                                            uint32_t cap, uint32_t& flags) {
+template <class Rows>
+__device__ __forceinline__ uint32_t tb_walk_rows(const uint8_t* __restrict__ flat, Rows rs, uint32_t a, uint32_t b,
+                                                 uint32_t* __restrict__ ops, uint32_t cap, uint32_t& flags) {
   uint32_t nops = 0;
-  const uint32_t n_rows = a + 1;
   uint8_t prev = SCRIPT_SUB;
   uint32_t last_kind = 3, run = 0;
   const uint32_t budget = a + b;
   for (uint32_t stepi = 0; stepi < budget && (a | b); ++stepi) {
-    const uint32_t off = rows[2 * a], fb = rows[2 * a + 1];
-    const uint32_t row_end = a + 1 < n_rows ? rows[2 * (a + 1)] : flat_total;
     uint8_t raw = SCRIPT_GAP_IN_A;
-    if (b >= fb && off + (b - fb) < row_end) raw = flat[off + (b - fb)];
+    if (b >= rs.fb && rs.off + (b - rs.fb) < rs.end) raw = flat[rs.off + (b - rs.fb)];
     uint8_t op;
     if ((prev & SCRIPT_OP_MASK) == SCRIPT_GAP_IN_A)
       op = (raw & SCRIPT_EXTEND_GAP_A) ? SCRIPT_GAP_IN_A : (raw & SCRIPT_OP_MASK);
@@ -1632,9 +1738,14 @@ __device__ __forceinline__ uint32_t tb_walk(const uint8_t* __restrict__ flat, co
       op = raw & SCRIPT_OP_MASK;
     prev = op;
     uint32_t kind;
-    if (op == SCRIPT_GAP_IN_A) { kind = 1; if (b) --b; }
-    else if (op == SCRIPT_GAP_IN_B) { kind = 2; if (a) --a; }
-    else { kind = 0; if (a) --a; if (b) --b; }
+    if (op == SCRIPT_GAP_IN_A) {
+      kind = 1;
+      if (b) --b;
+    } else {
+      kind = op == SCRIPT_GAP_IN_B ? 2 : 0;
+      if (a) rs.down(flat, --a);
+      if (kind == 0 && b) --b;
+    }
     if (kind == last_kind) {
       ++run;
     } else {
@@ -1652,6 +1763,12 @@ __device__ __forceinline__ uint32_t tb_walk(const uint8_t* __restrict__ flat, co
   }
   if (a | b) flags |= E_TB_SPACE;
   return nops;
+}
+
+__device__ __forceinline__ uint32_t tb_walk(const uint8_t* __restrict__ flat, const uint32_t* __restrict__ rows,
+                                            uint32_t a, uint32_t b, uint32_t flat_total, uint32_t* __restrict__ ops,
+                                            uint32_t cap, uint32_t& flags) {
+  return tb_walk_rows(flat, TbRowTable(rows, a, flat_total), a, b, ops, cap, flags);
 }
 
 // Traceback-phase bands (X-drop 100 by default) stay within 64 cells: the
@@ -1768,12 +1885,15 @@ __global__ __launch_bounds__(kDpLdsThreads) void k_dp_onepass(
     if (i >= n_tasks) continue;
     const unsigned long long l0 = wall_clock64();
     DpResult r;
-    dp_half_tb16<RLOG2>(tasks[i], mat, go, ge, my, myb, max_pos, r, flat, rows, flat_cap, rows_cap);
+    TbRowStart best_row;
+    dp_half_tb16<RLOG2>(tasks[i], mat, go, ge, my, myb, max_pos, r, flat, flat_cap, best_row);
     uint8_t* tflat = flat;
     uint32_t *trows = rows, *tops = ops, tops_cap = ops_cap;
+    bool stream = true;  // the traceback bytes with row trailers, else dp_half's row table
     if (r.flags) {  // band or traceback beyond the lane's storage: once more with a retry slot
       const uint32_t slot = atomicAdd(big.next, 1u);
       if (slot < big.slots) {
+        stream = false;
         uint8_t* bb = big.arena + (size_t)slot * onepass_stride(big.flat_cap, big.rows_cap, big.ops_cap);
         trows = reinterpret_cast<uint32_t*>(bb);
         tflat = bb + (size_t)big.rows_cap * 8;
@@ -1788,7 +1908,10 @@ __global__ __launch_bounds__(kDpLdsThreads) void k_dp_onepass(
     if (!r.flags) {
       cells += r.cells;
       if (r.a_len | r.b_len) {
-        const uint32_t n = tb_walk(tflat, trows, r.a_len, r.b_len, r.tb_bytes, tops, tops_cap, o.flags);
+        const uint32_t n =
+            stream ? tb_walk_rows(tflat, TbRowStream<tb_trailer_bytes(RLOG2)>(best_row, r.tb_bytes), r.a_len,
+                                  r.b_len, tops, tops_cap, o.flags)
+                   : tb_walk(tflat, trows, r.a_len, r.b_len, r.tb_bytes, tops, tops_cap, o.flags);
         if (!o.flags) {
           const unsigned long long at = atomicAdd(ops_count, (unsigned long long)n);
           if (at + n <= ops_pool_cap) {
@@ -2001,10 +2124,10 @@ __global__ __launch_bounds__(kPrelimThreads) void k_prelim_fused(
     DpTask right{sa + ps, qa + pq, sl > ps ? sl - ps : 0, ql > pq ? ql - pq : 0, xd, 0};
     DpResult L, Rr;
     if constexpr (PACK) {
-      dp_half_tb16<RLOG2, false, RS>(left, mat, P.gap_open, P.gap_extend, my16, myb, max_pos, L, nullptr, nullptr,
-                                     0u, 0u);
-      dp_half_tb16<RLOG2, false, RS>(right, mat, P.gap_open, P.gap_extend, my16, myb, max_pos, Rr, nullptr, nullptr,
-                                     0u, 0u);
+      TbRowStart unused;
+      dp_half_tb16<RLOG2, false, RS>(left, mat, P.gap_open, P.gap_extend, my16, myb, max_pos, L, nullptr, 0u, unused);
+      dp_half_tb16<RLOG2, false, RS>(right, mat, P.gap_open, P.gap_extend, my16, myb, max_pos, Rr, nullptr, 0u,
+                                     unused);
     } else {
       dp_half<false, RS>(left, mat, P.gap_open, P.gap_extend, my, myb, wmask, L, nullptr, nullptr, 0);
       dp_half<false, RS>(right, mat, P.gap_open, P.gap_extend, my, myb, wmask, Rr, nullptr, nullptr, 0);

@@ -430,7 +430,7 @@ def train(model, instances, epochs, lr, pos_weight, seed=0, log=None, val=None):
     return losses, val_losses
 
 
-def evaluate(searcher, inst: Instance, passed: np.ndarray, repeats: int = 3):
+def evaluate(searcher, inst: Instance, passed: np.ndarray, repeats: int = 9):
     """Pruned search of `inst` with the given passed (record, family) pairs,
     against the plain exact search of every pair: row recall, score-weighted
     recall, precision (rows the plain search does not report lower it),
@@ -454,7 +454,7 @@ def evaluate(searcher, inst: Instance, passed: np.ndarray, repeats: int = 3):
     }
 
 
-def plain_search_seconds(searcher, inst: Instance, repeats: int = 3, whole: bool = False):
+def plain_search_seconds(searcher, inst: Instance, repeats: int = 9, whole: bool = False):
     """Best-of-`repeats` seconds of the plain exact search of every pair
     (with `whole`, of the genome as one record instead)."""
     searcher.load_library(inst.families)
