@@ -187,13 +187,13 @@ int rmb_ctx_destroy(rmb_ctx* ctx);
  *  prelim_ring      preliminary-DP LDS band ring, 16 or 64 cells (default by
  *                   X-drop: 64 past 12 band cells); wider bands rerun with
  *                   global rings either way
- *  prelim_wg_per_cu preliminary-DP workgroups (128 lanes) per CU (1..16;
- *                   default 8 with 16-cell rings, 2 with 64)
+ *  prelim_wg_per_cu preliminary-DP workgroups per CU (1..16; default 8 of
+ *                   128 lanes with 16-cell rings, 9 of 64 lanes with 64)
  *  tb_ring          traceback LDS band ring, 64 or 256 cells (default by
  *                   X-drop: 256 past 48 band cells)
  *  tb_wg_per_cu     traceback workgroups (64 lanes) per CU (1..16, capped at
- *                   8, or 2 with the 256-cell rings, by the lane arenas'
- *                   memory; default 7, or 1)
+ *                   9, or 2 with the 256-cell rings, by LDS and the lane
+ *                   arenas' memory; default 9, or 2)
  *  tb_wave_halves   a traceback launch of at most this many halves runs
  *                   every half one per wave, a band row per vector step,
  *                   instead of one per lane (1..2^30; -1 never; default 112

@@ -35,7 +35,7 @@ MARGIN = 0.005  # a value is kept when it is this much faster than the best so f
 SPACE = {  # candidate values per field; 0 is the library's default
     "batches": [0, 4, 12],
     "prelim_ring": [0, 16, 64],
-    "prelim_wg_per_cu": [0, 2, 4, 7, 8],
+    "prelim_wg_per_cu": [0, 2, 4, 7, 8, 9],
     "tb_ring": [0, 64, 256],
     "tb_wg_per_cu": [0, 2, 4, 7],
     "tb_wave_halves": [0, -1, 1 << 30],  # default regime, always lane-run, always wave-run

@@ -1221,20 +1221,19 @@ struct TbRowStart {  // where the best cell's row starts in the stream
 };
 
 // The traceback DP of k_dp_onepass with 4-byte LDS ring cells, so twice as
-// many lanes fit a CU: H and E as 16-bit offsets from the best score at the
-// start of the row that wrote the cell (every band cell is rewritten in each
-// row it stays in the band, a pruned cell with its E re-based, so one base per
-// row serves).  H of a pruned cell is the sentinel -32768, and E is held at
-// -32767 at the lowest: such an E lies more than 32,000 below the best score,
-// X-drop being far smaller, so it loses every maximum it meets and never keeps
-// a cell; the traceback code of a cell whose every candidate is that low is
-// never read by the walk.  Every other value is exact, so every decision is
-// dp_half's.  Kept cells lie in [base - X - open cost, base + ring cells x
-// largest score]; a task for which that could leave the 16-bit range comes
-// back E_BAND_WIDE, for the retry with dp_half.  Without TB it is the
-// score-only DP (the preliminary DPs' wide rings), rings interleaved across
-// RS lanes.
-template <uint32_t RLOG2, bool TB = true, uint32_t RS = kDpLdsThreads>
+// many lanes fit a CU: H and E as 16-bit (or, with C12, 12-bit) offsets from
+// the best score at the start of the row that wrote the cell (every band cell
+// is rewritten in each row it stays in the band, a pruned cell with its E
+// re-based, so one base per row serves).  H of a pruned cell is the sentinel
+// (the format's lowest value), and E is held one above it at the lowest: such
+// an E lies more than 2,000 below the best score, beyond X-drop plus the open
+// cost, so it loses every maximum it meets and never keeps a cell; the
+// traceback code of a cell whose every candidate is that low is never read by
+// the walk.  Every other value is exact, so every decision is dp_half's.  A
+// task for which a kept value could leave the format's range comes back
+// E_BAND_WIDE, for the retry with dp_half.  Without TB it is the score-only
+// DP (the preliminary DPs' wide rings), rings interleaved across RS lanes.
+template <uint32_t RLOG2, bool TB = true, uint32_t RS = kDpLdsThreads, bool C12 = false>
 __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, int32_t go, int32_t ge,
                                              uint32_t* __restrict__ ring, uint8_t* __restrict__ bring,
                                              int32_t max_pos, DpResult& res, uint8_t* __restrict__ flat,
@@ -1247,10 +1246,30 @@ __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, in
   const bool rev = t.reverse != 0;
   const int32_t oe = go + ge;
   const int32_t xdrop = t.xdrop > oe ? t.xdrop : oe;
-  if ((int64_t)max_pos * (wmask + 1) + xdrop + oe >= 30000) { res.flags |= E_BAND_WIDE; return; }
-  constexpr int32_t kNegH = -32768, kLowE = -32767;
+  // C12: H and E as 12-bit offsets and the column's base code in bits 24-27
+  // of the same cell, so no base-code ring is kept.  A cell of a row lies at
+  // most the largest score above the row's base (its candidates come from
+  // cells no higher than the best score before the row, plus one score), and
+  // a kept cell at most X below it, so 12 bits hold every kept value while
+  // X + open cost + the largest score stays below 2,000.
+  constexpr uint32_t kBits = C12 ? 12u : 16u, kMask = (1u << kBits) - 1;
+  if (C12 ? (int64_t)max_pos + xdrop + oe >= 2000 : (int64_t)max_pos * (wmask + 1) + xdrop + oe >= 30000) {
+    res.flags |= E_BAND_WIDE;
+    return;
+  }
+  constexpr int32_t kNegH = -(1 << (kBits - 1)), kLowE = kNegH + 1;
   auto enc = [](int32_t h, int32_t e) -> uint32_t {
-    return ((uint32_t)h & 0xFFFFu) | ((uint32_t)(e < kLowE ? kLowE : e) << 16);
+    return ((uint32_t)h & kMask) | (((uint32_t)(e < kLowE ? kLowE : e) & kMask) << kBits);
+  };
+  auto dec_h = [](uint32_t c) -> int32_t { return (int32_t)(c << (32 - kBits)) >> (32 - kBits); };
+  auto dec_e = [](uint32_t c) -> int32_t { return (int32_t)(c << (32 - 2 * kBits)) >> (32 - kBits); };
+  // a new column's cell: its scores and, with C12, its base code
+  auto put_col = [&](uint32_t col, uint32_t cell, uint32_t code) {
+    if constexpr (C12) ring[(col & wmask) * RS] = cell | (code << 24);
+    else {
+      ring[(col & wmask) * RS] = cell;
+      bring[(col & wmask) * RS] = (uint8_t)code;
+    }
   };
   SeqStream as, bs;
   as.init(t.a, m, rev);
@@ -1304,16 +1323,14 @@ __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, in
     }
     return true;
   };
-  ring[0] = enc(0, -oe);
-  bring[0] = (uint8_t)bcode(0);
+  put_col(0, enc(0, -oe), bcode(0));
   put(SCRIPT_GAP_IN_B);
   int32_t score = -oe;
   uint32_t b_size = 1;
   for (uint32_t j = 1; j <= n; ++j) {
     if (score < -xdrop) break;
     if (j > wmask) { res.flags |= E_BAND_WIDE; return; }
-    ring[(j & wmask) * RS] = enc(score, score - oe);
-    bring[(j & wmask) * RS] = (uint8_t)bcode(j);
+    put_col(j, enc(score, score - oe), bcode(j));
     put(SCRIPT_GAP_IN_A);
     score -= ge;
     b_size = j + 1;
@@ -1343,18 +1360,19 @@ __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, in
     bool best_here = false;
     cells += b_size - first_b;
     uint32_t cn = ring[(first_b & wmask) * RS];
-    uint32_t bn = bring[(first_b & wmask) * RS];
+    uint32_t bn = C12 ? (cn >> 24) & 15u : bring[(first_b & wmask) * RS];
     for (uint32_t bi = first_b; bi < b_size; ++bi) {
       const uint32_t cv = cn;
       const int32_t sub = row[bn];
       {  // the next cell's inputs, loaded while this one is computed
         const uint32_t ns = ((bi + 1) & wmask) * RS;
         cn = ring[ns];
-        bn = bring[ns];
+        bn = C12 ? (cn >> 24) & 15u : bring[ns];
       }
-      const int32_t hr = (int32_t)(int16_t)(uint16_t)(cv & 0xFFFFu);
+      const int32_t hr = dec_h(cv);
       const int32_t cx = hr == kNegH ? kMinInt : hr + pbase;
-      int32_t sgc = (int32_t)(int16_t)(uint16_t)(cv >> 16) + pbase;
+      int32_t sgc = dec_e(cv) + pbase;
+      const uint32_t code_bits = C12 ? cv & 0x0F000000u : 0u;  // kept with the column's cell
       const int32_t next = cx + sub;
       uint8_t op = SCRIPT_SUB;
       int32_t cs = sc;
@@ -1362,7 +1380,7 @@ __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, in
       if (sgr > cs) { cs = sgr; op = SCRIPT_GAP_IN_A; }
       if (best - cs > xdrop) {
         if (bi == first_b) ++first_b;
-        else ring[(bi & wmask) * RS] = enc(kNegH, sgc - cbase);
+        else ring[(bi & wmask) * RS] = enc(kNegH, sgc - cbase) | code_bits;
       } else {
         last_b = bi;
         if (cs > best) { best = cs; best_a = ai + 1; best_b = bi; best_here = true; }
@@ -1372,7 +1390,7 @@ __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, in
         int32_t cy;
         if (sgc < open) cy = open;
         else { cy = sgc; op |= SCRIPT_EXTEND_GAP_B; }
-        ring[(bi & wmask) * RS] = enc(cs - cbase, cy - cbase);
+        ring[(bi & wmask) * RS] = enc(cs - cbase, cy - cbase) | code_bits;
         if (sgr < open) sgr = open;
         else op |= SCRIPT_EXTEND_GAP_A;
       }
@@ -1392,8 +1410,7 @@ __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, in
     } else {
       while (sgr >= best - xdrop && b_size <= n) {
         if (b_size - first_b > wmask) { res.flags |= E_BAND_WIDE; return; }
-        ring[(b_size & wmask) * RS] = enc(sgr - cbase, sgr - oe - cbase);
-        bring[(b_size & wmask) * RS] = (uint8_t)bcode(b_size);
+        put_col(b_size, enc(sgr - cbase, sgr - oe - cbase), bcode(b_size));
         put(SCRIPT_GAP_IN_A);
         sgr -= ge;
         ++b_size;
@@ -1401,8 +1418,7 @@ __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, in
     }
     if (b_size <= n) {
       if (b_size - first_b > wmask) { res.flags |= E_BAND_WIDE; return; }
-      ring[(b_size & wmask) * RS] = enc(kNegH, kLowE);
-      bring[(b_size & wmask) * RS] = (uint8_t)bcode(b_size);
+      put_col(b_size, enc(kNegH, kLowE), bcode(b_size));
       ++b_size;
     }
     if (best_here) {
@@ -1850,8 +1866,7 @@ __global__ __launch_bounds__(kDpLdsThreads) void k_dp_onepass(
     unsigned long long ops_pool_cap, OnePassResult* __restrict__ out, unsigned long long* next, OnePassBig big,
     unsigned long long* counters) {
   __shared__ int mat[256];
-  __shared__ uint32_t lring[kDpLdsThreads << RLOG2];
-  __shared__ uint8_t lbring[kDpLdsThreads << RLOG2];
+  __shared__ uint32_t lring[kDpLdsThreads << RLOG2];  // 12-bit H and E and the column's base code
   __shared__ int s_max_pos;
   if (threadIdx.x == 0) s_max_pos = 0;
   __syncthreads();
@@ -1869,7 +1884,6 @@ __global__ __launch_bounds__(kDpLdsThreads) void k_dp_onepass(
   (void)ring;
   (void)wlog2;
   uint32_t* my = lring + threadIdx.x;
-  uint8_t* myb = lbring + threadIdx.x;
   uint8_t* base = arena + (size_t)gt * onepass_stride(flat_cap, rows_cap, ops_cap);
   uint32_t* rows = reinterpret_cast<uint32_t*>(base);
   uint8_t* flat = base + (size_t)rows_cap * 8;
@@ -1886,7 +1900,8 @@ __global__ __launch_bounds__(kDpLdsThreads) void k_dp_onepass(
     const unsigned long long l0 = wall_clock64();
     DpResult r;
     TbRowStart best_row;
-    dp_half_tb16<RLOG2>(tasks[i], mat, go, ge, my, myb, max_pos, r, flat, flat_cap, best_row);
+    dp_half_tb16<RLOG2, true, kDpLdsThreads, true>(tasks[i], mat, go, ge, my, nullptr, max_pos, r, flat, flat_cap,
+                                                   best_row);
     uint8_t* tflat = flat;
     uint32_t *trows = rows, *tops = ops, tops_cap = ops_cap;
     bool stream = true;  // the traceback bytes with row trailers, else dp_half's row table
@@ -2067,7 +2082,7 @@ __global__ __launch_bounds__(kPrelimThreads) void k_prelim_fused(
   __shared__ int mat[256];
   __shared__ int2 lring[LDS_RING && !PACK ? (T << RLOG2) : 1];
   __shared__ uint32_t lring16[PACK ? (T << RLOG2) : 1];
-  __shared__ uint8_t lbring[LDS_RING ? (T << RLOG2) : 1];
+  __shared__ uint8_t lbring[LDS_RING && !PACK ? (T << RLOG2) : 1];
   __shared__ int s_max_pos;
   static_assert(!PACK || LDS_RING, "packed rings live in LDS");
   if (threadIdx.x == 0) s_max_pos = 0;
@@ -2125,9 +2140,10 @@ __global__ __launch_bounds__(kPrelimThreads) void k_prelim_fused(
     DpResult L, Rr;
     if constexpr (PACK) {
       TbRowStart unused;
-      dp_half_tb16<RLOG2, false, RS>(left, mat, P.gap_open, P.gap_extend, my16, myb, max_pos, L, nullptr, 0u, unused);
-      dp_half_tb16<RLOG2, false, RS>(right, mat, P.gap_open, P.gap_extend, my16, myb, max_pos, Rr, nullptr, 0u,
-                                     unused);
+      dp_half_tb16<RLOG2, false, RS, true>(left, mat, P.gap_open, P.gap_extend, my16, nullptr, max_pos, L, nullptr,
+                                           0u, unused);
+      dp_half_tb16<RLOG2, false, RS, true>(right, mat, P.gap_open, P.gap_extend, my16, nullptr, max_pos, Rr, nullptr,
+                                           0u, unused);
     } else {
       dp_half<false, RS>(left, mat, P.gap_open, P.gap_extend, my, myb, wmask, L, nullptr, nullptr, 0);
       dp_half<false, RS>(right, mat, P.gap_open, P.gap_extend, my, myb, wmask, Rr, nullptr, nullptr, 0);

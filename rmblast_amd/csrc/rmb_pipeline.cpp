@@ -504,15 +504,15 @@ static void run_dp_traceback_2pass(rmb_ctx* c, const std::vector<DpTask>& tasks,
                  t_score / 1e3, t_tb / 1e3, t_unpack / 1e3, tt.us() / 1e3);
 }
 
-// Per-lane arena of the one-pass traceback: rows, traceback bytes and ops of
-// one half alignment.  Halves beyond it (and bands wider than the LDS ring)
-// take the two-pass path.
-constexpr uint32_t kOnePassRows = 8192, kOnePassFlat = 256u << 10, kOnePassOps = 4096;
-// lane arenas of the wide-ring launches (one workgroup per CU, so 16,384 lanes)
-constexpr uint32_t kOnePassWideRows = 32768, kOnePassWideFlat = 1u << 20, kOnePassWideOps = 16384;
+// Per-lane arena of the one-pass traceback: the traceback stream (each row's
+// bytes and its trailer) and the ops of one half alignment.  Halves beyond it
+// (and bands wider than the LDS ring) take a retry slot or the two-pass path.
+constexpr uint32_t kOnePassFlat = 256u << 10, kOnePassOps = 4096;
+// lane arenas of the wide-ring launches (two workgroups per CU, so 32,768 lanes)
+constexpr uint32_t kOnePassWideFlat = 1u << 20, kOnePassWideOps = 16384;
 constexpr unsigned long long kOnePassPool = 64ull << 20;  // packed ops per launch
-constexpr uint32_t kOnePassPerCu = 7;  // 64-lane workgroups with ~21 KB of LDS: seven fit a CU
-constexpr uint32_t kOnePassWidePerCu = 1;  // with the 256-cell rings, ~81 KB: one fits a CU
+constexpr uint32_t kOnePassPerCu = 9;  // 64-lane workgroups with ~17 KB of LDS: nine fit a CU
+constexpr uint32_t kOnePassWidePerCu = 2;  // with the 256-cell rings, ~65 KB: two fit a CU
 // A gap of g bases costs open + g x extend, so a band reaches about
 // (X - open) / extend cells to either side of its best cell: past 48 cells
 // in all the one-pass kernel takes its wider rings (RMB_TB_WIDE=0 keeps the
@@ -577,14 +577,16 @@ static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::
     DpTask* dt = c->b_tasks.get<DpTask>(n);
     OnePassResult* dr = c->b_opres.get<OnePassResult>(n);
     const bool wide = onepass_wide(c);
-    // lane arenas are per workgroup: at most 8 (2 with the wide rings) per CU
-    const uint32_t per_cu = c->tune.tb_wg_per_cu ? std::min<uint32_t>((uint32_t)c->tune.tb_wg_per_cu, wide ? 2u : 8u)
+    // lane arenas are per workgroup: at most 9 (2 with the wide rings) per CU
+    const uint32_t per_cu = c->tune.tb_wg_per_cu ? std::min<uint32_t>((uint32_t)c->tune.tb_wg_per_cu, wide ? 2u : 9u)
                                                  : (wide ? kOnePassWidePerCu : kOnePassPerCu);
     // the first n_wave halves run one per wave, the rest one per lane
     const uint32_t n_wave = base < n_wave_all ? (uint32_t)std::min<size_t>(n, n_wave_all - base) : 0u;
     const uint32_t n_lane = n - n_wave;
     const uint32_t blocks = std::min<uint32_t>((uint32_t)c->cu_count * per_cu, (n_lane + 63) / 64);
-    const uint32_t flat_cap = wide ? kOnePassWideFlat : kOnePassFlat, rows_cap = wide ? kOnePassWideRows : kOnePassRows,
+    // the lane arenas hold no row table: the traceback stream carries the
+    // rows' trailers (the retry slots keep theirs)
+    const uint32_t flat_cap = wide ? kOnePassWideFlat : kOnePassFlat, rows_cap = 0,
                    ops_cap = wide ? kOnePassWideOps : kOnePassOps;
     const uint64_t stride = onepass_stride(flat_cap, rows_cap, ops_cap);
     uint8_t* arena = n_lane ? c->b_arena.get<uint8_t>((size_t)blocks * 64 * stride) : nullptr;
@@ -885,14 +887,14 @@ static void enqueue_batch(rmb_ctx* c, P1Batch& b, const uint8_t* d_qfwd, const u
     PP.next = cnt + 4 + 2 * gi;
     hip_check(hipEventRecord(S.ev[6 * gi + 2], st), "event");
     // past 12 band cells ((X - open) / extend to either side), 64-cell rings
-    // of 4-byte cells in 64-lane workgroups, seven per CU (RMB_PRELIM_WIDE=0
-    // keeps the 16-cell rings)
+    // of 4-byte cells (12-bit H and E, the column's base code) in 64-lane
+    // workgroups, nine per CU (RMB_PRELIM_WIDE=0 keeps the 16-cell rings)
     const char* pw_env = std::getenv("RMB_PRELIM_WIDE");
     const bool pwide = c->tune.prelim_ring ? c->tune.prelim_ring == 64
                                            : !(pw_env && std::atoi(pw_env) == 0) &&
                                                  2 * std::max(0, (int)c->P.xdrop_gap - (int)c->P.gap_open) /
                                                          std::max(1, (int)c->P.gap_extend) + 1 > 12;
-    const int pre_per_cu = c->tune.prelim_wg_per_cu ? c->tune.prelim_wg_per_cu : (pwide ? 7 : 8);
+    const int pre_per_cu = c->tune.prelim_wg_per_cu ? c->tune.prelim_wg_per_cu : (pwide ? 9 : 8);
     hip_check(launch_prelim_fused(c->cu_count * pre_per_cu, true, PP, dh, nullptr, g.hits_cap, drecs,
                                   (const SubjDesc*)c->d_sdesc.p, d_qfwd, d_qrc, (const uint8_t*)c->d_splus.p,
                                   c->d_mat, nullptr, 0, dpre, cnt, dredo, nullptr, hitcnt + 2 * gi + 1, ctr, err, pwide, st),
