@@ -1690,6 +1690,240 @@ __device__ __forceinline__ void dp_half_wave(const DpTask& t, const int* __restr
   res.tb_rows = rows_at_best;
 }
 
+// dp_half_wave for bands of up to 256 cells: lane l holds band positions
+// 4l..4l+3 (columns first_b + 4l + j), so each scan runs over the lane's four
+// cells and then across the lanes, and the band moves by permutes whose
+// source lane and register are the same for the whole wave.  The base codes
+// of columns [wb, wb + 512) sit in the wave's LDS window `win`, reloaded when
+// the band's start passes wb + 256.  Row table, traceback bytes and results as
+// dp_half_wave.
+__device__ __forceinline__ void dp_half_wave4(const DpTask& t, const int* __restrict__ mat, int32_t go, int32_t ge,
+                                              DpResult& res, uint8_t* __restrict__ flat, uint32_t* __restrict__ rows,
+                                              uint32_t flat_cap, uint32_t rows_cap, uint32_t& rounds,
+                                              uint8_t* __restrict__ win) {
+  constexpr uint32_t K = 4, wmask = 64 * K - 1;
+  constexpr int32_t kNeg = INT32_MIN + (1 << 27);  // below every reachable value, and safe to add to
+  const uint32_t lane = threadIdx.x & 63u;
+  res.score = 0; res.a_len = 0; res.b_len = 0; res.flags = 0; res.cells = 0; res.tb_bytes = 0; res.tb_rows = 0;
+  const uint32_t m = t.m, n = t.n;
+  if (m == 0 || n == 0) return;
+  const bool rev = t.reverse != 0;
+  const int32_t oe = go + ge;
+  const int32_t xdrop = t.xdrop > oe ? t.xdrop : oe;
+  uint32_t J;
+  if (ge <= 0) J = n;
+  else J = min(n, (uint32_t)((xdrop - oe) / ge) + 1u);
+  if (J > wmask) { res.flags |= E_BAND_WIDE; return; }
+  uint32_t wb = 0;
+  auto load_win = [&]() {
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) win[lane + 64 * k] = (uint8_t)seq_code(t.b, n, rev, wb + lane + 64 * k);
+  };
+  load_win();
+  uint32_t ab = 0;
+  uint32_t aw = seq_code(t.a, m, rev, lane), awn = seq_code(t.a, m, rev, 64 + lane);
+  uint32_t first_b = 0, b_size = J + 1;
+  int32_t H[K], E[K];
+  uint32_t bc[K];
+#pragma unroll
+  for (uint32_t j = 0; j < K; ++j) {
+    const uint32_t p = K * lane + j;
+    H[j] = p == 0 ? 0 : -oe - (int32_t)(p - 1) * ge;
+    E[j] = H[j] - oe;
+    bc[j] = win[p];
+    if (p <= J) flat[p] = p == 0 ? SCRIPT_GAP_IN_B : SCRIPT_GAP_IN_A;
+  }
+  if (lane == 0) { rows[0] = 0; rows[1] = 0; }
+  uint32_t flat_len = J + 1;
+  int32_t best = 0;
+  uint32_t best_a = 0, best_b = 0;
+  uint64_t cells = 0;
+  uint32_t bytes_at_best = flat_len, rows_at_best = 1;
+  for (uint32_t ai = 0; ai < m; ++ai) {
+    if (ai + 2 > rows_cap || flat_len + 2 * wmask + 4 > flat_cap) { res.flags |= E_TB_SPACE; return; }
+    if (ai - ab == 64) {
+      ab += 64;
+      aw = awn;
+      awn = seq_code(t.a, m, rev, ab + 64 + lane);
+    }
+    const uint32_t acode = (uint32_t)__builtin_amdgcn_readlane((int)aw, (int)(ai - ab));
+    const uint32_t w = b_size - first_b;
+    if (lane == 0) { rows[2 * (ai + 1)] = flat_len; rows[2 * (ai + 1) + 1] = first_b; }
+    cells += w;
+    bool act[K];
+    int32_t D[K], V[K], G[K];
+    {
+      int32_t hs[K];
+#pragma unroll
+      for (uint32_t j = 0; j < K; ++j) {
+        act[j] = K * lane + j < w;
+        hs[j] = H[j] + mat[(acode << 4) | bc[j]];
+      }
+      D[0] = wave_shr1(hs[K - 1], kMinInt);  // the cell before's H + its substitution
+#pragma unroll
+      for (uint32_t j = 1; j < K; ++j) D[j] = hs[j - 1];
+    }
+    bool kept[K];
+    uint64_t km[K];
+#pragma unroll
+    for (uint32_t j = 0; j < K; ++j) {
+      V[j] = E[j];
+      G[j] = max(D[j], V[j]);
+      kept[j] = act[j] && G[j] >= best - xdrop;
+      km[j] = wave_ballot(kept[j]);
+    }
+    // iterate the keep decisions to the fixed point, as dp_half_wave
+    int32_t F[K], cs[K], fa[K], rowmax;
+    for (uint32_t round = 0;; ++round) {
+      if (round > K * 64) { res.flags |= E_BAND_WIDE; return; }
+      ++rounds;
+      int32_t A[K], B[K];
+#pragma unroll
+      for (uint32_t j = 0; j < K; ++j) {
+        A[j] = kept[j] ? -ge : 0;
+        B[j] = kept[j] ? G[j] - oe : kNeg;
+      }
+#pragma unroll
+      for (uint32_t j = 1; j < K; ++j) {  // the lane's own cells, in order
+        B[j] = max(B[j - 1] + A[j], B[j]);
+        A[j] = A[j - 1] + A[j];
+      }
+      int32_t LA = A[K - 1], LB = B[K - 1];
+      wave_prefix_affine(LA, LB, kNeg);
+      const int32_t EA = wave_shr1(LA, 0), EB = wave_shr1(LB, kNeg);  // the lanes before
+#pragma unroll
+      for (uint32_t j = 0; j < K; ++j) fa[j] = max(kMinInt + (EA + A[j]), max(EB + A[j], B[j]));
+      F[0] = wave_shr1(fa[K - 1], kMinInt);
+#pragma unroll
+      for (uint32_t j = 1; j < K; ++j) F[j] = fa[j - 1];
+      int32_t pm[K];
+#pragma unroll
+      for (uint32_t j = 0; j < K; ++j) {
+        cs[j] = act[j] ? max(G[j], F[j]) : kNeg;
+        pm[j] = j ? max(pm[j - 1], cs[j]) : cs[j];
+      }
+      const int32_t incl = wave_prefix_max(pm[K - 1], kNeg);
+      rowmax = __builtin_amdgcn_readlane(incl, 63);
+      const int32_t ex = wave_shr1(incl, kNeg);
+      bool same = true;
+#pragma unroll
+      for (uint32_t j = 0; j < K; ++j) {
+        const int32_t bb = max(best, j ? max(ex, pm[j - 1]) : ex);
+        const bool k2 = act[j] && cs[j] >= bb - xdrop;
+        const uint64_t m2 = wave_ballot(k2);
+        same = same && m2 == km[j];
+        kept[j] = k2;
+        km[j] = m2;
+      }
+      if (same) break;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < K; ++j) {  // traceback byte of each cell
+      uint8_t op = SCRIPT_SUB;
+      int32_t c0 = D[j];
+      if (V[j] > c0) { c0 = V[j]; op = SCRIPT_GAP_IN_B; }
+      if (F[j] > c0) op = SCRIPT_GAP_IN_A;
+      if (kept[j]) {
+        if (V[j] - ge >= cs[j] - oe) op |= SCRIPT_EXTEND_GAP_B;
+        if (F[j] - ge >= cs[j] - oe) op |= SCRIPT_EXTEND_GAP_A;
+      }
+      if (act[j]) flat[flat_len + K * lane + j] = op;
+    }
+    uint32_t lead = 0xFFFFFFFFu, last = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < K; ++j)
+      if (km[j]) {
+        lead = min(lead, K * (uint32_t)__builtin_ctzll(km[j]) + j);
+        last = max(last, K * (63u - (uint32_t)__builtin_clzll(km[j])) + j);
+      }
+    if (lead == 0xFFFFFFFFu) break;  // every cell pruned: first_b reaches b_size
+    bool best_here = false;
+    if (rowmax > best) {  // the first cell holding the row maximum
+      uint32_t at = 0xFFFFFFFFu;
+#pragma unroll
+      for (uint32_t j = 0; j < K; ++j) {
+        const uint64_t bm = wave_ballot(act[j] && cs[j] == rowmax);
+        if (bm) at = min(at, K * (uint32_t)__builtin_ctzll(bm) + j);
+      }
+      best = rowmax;
+      best_a = ai + 1;
+      best_b = first_b + at;
+      best_here = true;
+    }
+    int32_t Hn[K], En[K];
+#pragma unroll
+    for (uint32_t j = 0; j < K; ++j) {
+      Hn[j] = kept[j] ? cs[j] : kMinInt;
+      En[j] = kept[j] ? max(V[j] - ge, cs[j] - oe) : V[j];
+    }
+    flat_len += w;
+    const uint32_t nfirst = first_b + lead;
+    uint32_t nsize = b_size;
+    uint32_t ext = 0;
+    int32_t f_end = 0;
+    if (last + 1 < w) {
+      nsize = first_b + last + 1;
+    } else {
+      const uint32_t pj = (w - 1) & 3u;
+      const int32_t fv = pj == 0 ? fa[0] : pj == 1 ? fa[1] : pj == 2 ? fa[2] : fa[3];
+      f_end = __builtin_amdgcn_readlane(fv, (int)((w - 1) >> 2));
+      const int32_t floor_v = best - xdrop;
+      if (f_end >= floor_v) {
+        const uint32_t room = n + 1 - b_size;  // columns b_size..n
+        uint32_t k = ge <= 0 ? room : (uint32_t)((f_end - floor_v) / ge) + 1u;
+        ext = min(k, room);
+      }
+      if (ext && (b_size + ext - 1) - nfirst > wmask) { res.flags |= E_BAND_WIDE; return; }
+#pragma unroll
+      for (uint32_t j = 0; j < K; ++j)
+        if (K * lane + j < ext) flat[flat_len + K * lane + j] = SCRIPT_GAP_IN_A;
+      flat_len += ext;
+      nsize = b_size + ext;
+    }
+    uint32_t sentinel = 0;
+    if (nsize <= n) {
+      if (nsize - nfirst > wmask) { res.flags |= E_BAND_WIDE; return; }
+      sentinel = 1;
+    }
+    // shift the band to start at nfirst: position p takes old position
+    // p + lead, i.e. lane + (j + lead) / 4, register (j + lead) % 4
+#pragma unroll
+    for (uint32_t j = 0; j < K; ++j) {
+      const uint32_t sj = (j + lead) & 3u, qj = (j + lead) >> 2;
+      const int32_t hsrc = sj == 0 ? Hn[0] : sj == 1 ? Hn[1] : sj == 2 ? Hn[2] : Hn[3];
+      const int32_t esrc = sj == 0 ? En[0] : sj == 1 ? En[1] : sj == 2 ? En[2] : En[3];
+      const int src = (int)(((lane + qj) & 63u) << 2);
+      const int32_t hs = __builtin_amdgcn_ds_bpermute(src, hsrc);
+      const int32_t es = __builtin_amdgcn_ds_bpermute(src, esrc);
+      const uint32_t col = nfirst + K * lane + j;
+      if (col < min(b_size, nsize)) { H[j] = hs; E[j] = es; }  // the row's cells still in the band
+      else if (col < nsize) {  // extension cells
+        const int32_t f = f_end - (int32_t)(col - b_size) * ge;
+        H[j] = f;
+        E[j] = f - oe;
+      } else {
+        H[j] = kMinInt;
+        E[j] = kMinInt;
+      }
+    }
+    first_b = nfirst;
+    b_size = nsize + sentinel;
+    while (first_b - wb >= 256) {  // the band's columns stay inside [wb, wb + 512)
+      wb += 256;
+      load_win();
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < K; ++j) bc[j] = win[first_b - wb + K * lane + j];
+    if (best_here) { bytes_at_best = flat_len; rows_at_best = ai + 2; }
+  }
+  res.score = best;
+  res.a_len = best_a;
+  res.b_len = best_b;
+  res.cells = cells;
+  res.tb_bytes = bytes_at_best;
+  res.tb_rows = rows_at_best;
+}
+
 // Where a row's traceback bytes lie, for the walk: from a row table (8 B per
 // row: first byte, first column; a row ends where the next begins) ...
 struct TbRowTable {
@@ -1951,21 +2185,26 @@ __global__ __launch_bounds__(kDpLdsThreads) void k_dp_onepass(
   atomicAdd(&counters[C_DP_CELLS], cells);
 }
 
-// The traceback halves that run one per wave (dp_half_wave), four waves per
-// workgroup with no LDS beyond the matrix, so a SIMD holds several waves and
-// each wave's row-to-row latency hides behind the others.  Each wave has its
-// own arena (row table, traceback bytes, ops); a half beyond it, or with a
-// band wider than 64 cells, comes back flagged for the two-pass path.
+// The traceback halves that run one per wave (dp_half_wave, or with WIDE
+// dp_half_wave4 and its base-code windows), four waves per workgroup with
+// little LDS, so a SIMD holds several waves and each wave's row-to-row latency
+// hides behind the others.  Each wave has its own arena (row table, traceback
+// bytes, ops); a half beyond it, or with a band wider than 64 (WIDE: 256)
+// cells, comes back flagged for the two-pass path.
 static constexpr uint32_t kDpWaveThreads = 256;
+template <bool WIDE>
 __global__ __launch_bounds__(kDpWaveThreads) void k_dp_wave(
     const DpTask* __restrict__ tasks, uint32_t n_tasks, const int32_t* __restrict__ gmat, int32_t go, int32_t ge,
     uint8_t* __restrict__ arena, uint32_t rows_cap, uint32_t flat_cap, uint32_t ops_cap,
     uint32_t* __restrict__ ops_pool, unsigned long long* __restrict__ ops_count, unsigned long long ops_pool_cap,
     OnePassResult* __restrict__ out, unsigned long long* next, unsigned long long* counters) {
   __shared__ int mat[256];
+  __shared__ uint8_t bwin[WIDE ? (kDpWaveThreads / 64) * 512 : 1];
   for (int i = threadIdx.x; i < 256; i += blockDim.x) mat[i] = gmat[i];
   __syncthreads();
   const uint32_t lane = threadIdx.x & 63u;
+  uint8_t* win = bwin + (WIDE ? (threadIdx.x >> 6) * 512 : 0);
+  (void)win;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   uint8_t* wbase = arena + wave * onepass_stride(flat_cap, rows_cap, ops_cap);
   uint32_t* wrows = reinterpret_cast<uint32_t*>(wbase);
@@ -1979,7 +2218,8 @@ __global__ __launch_bounds__(kDpWaveThreads) void k_dp_wave(
     const uint32_t i = (uint32_t)k;
     DpResult r;
     const unsigned long long t0 = wall_clock64();
-    dp_half_wave(tasks[i], mat, go, ge, r, wflat, wrows, flat_cap, rows_cap, rounds);
+    if constexpr (WIDE) dp_half_wave4(tasks[i], mat, go, ge, r, wflat, wrows, flat_cap, rows_cap, rounds, win);
+    else dp_half_wave(tasks[i], mat, go, ge, r, wflat, wrows, flat_cap, rows_cap, rounds);
     // the walk reads bytes other lanes of the wave stored
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     const unsigned long long t1 = wall_clock64();
@@ -2338,10 +2578,14 @@ hipError_t launch_dp_onepass(int blocks, const DpTask* tasks, uint32_t n, const 
 hipError_t launch_dp_wave(int blocks, const DpTask* tasks, uint32_t n, const int32_t* gmat, int32_t go, int32_t ge,
                           uint8_t* arena, uint32_t rows_cap, uint32_t flat_cap, uint32_t ops_cap, uint32_t* ops_pool,
                           unsigned long long* ops_count, unsigned long long ops_pool_cap, OnePassResult* out,
-                          unsigned long long* next, unsigned long long* counters, hipStream_t st) {
+                          unsigned long long* next, unsigned long long* counters, bool wide, hipStream_t st) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_dp_wave, dim3(blocks), dim3(kDpWaveThreads), 0, st, tasks, n, gmat, go, ge, arena, rows_cap,
-                     flat_cap, ops_cap, ops_pool, ops_count, ops_pool_cap, out, next, counters);
+  if (wide)
+    hipLaunchKernelGGL(k_dp_wave<true>, dim3(blocks), dim3(kDpWaveThreads), 0, st, tasks, n, gmat, go, ge, arena,
+                       rows_cap, flat_cap, ops_cap, ops_pool, ops_count, ops_pool_cap, out, next, counters);
+  else
+    hipLaunchKernelGGL(k_dp_wave<false>, dim3(blocks), dim3(kDpWaveThreads), 0, st, tasks, n, gmat, go, ge, arena,
+                       rows_cap, flat_cap, ops_cap, ops_pool, ops_count, ops_pool_cap, out, next, counters);
   return hipGetLastError();
 }
 

@@ -69,7 +69,7 @@ hipError_t launch_dp_onepass(int, const DpTask*, uint32_t, const int32_t*, int32
                              unsigned long long*, OnePassBig, unsigned long long*, bool, hipStream_t);
 hipError_t launch_dp_wave(int, const DpTask*, uint32_t, const int32_t*, int32_t, int32_t, uint8_t*, uint32_t, uint32_t,
                           uint32_t, uint32_t*, unsigned long long*, unsigned long long, OnePassResult*,
-                          unsigned long long*, unsigned long long*, hipStream_t);
+                          unsigned long long*, unsigned long long*, bool, hipStream_t);
 hipError_t launch_prelim_fused(int, bool, PrelimParams, const UngHit*, const uint32_t*, uint64_t, const RecDesc*,
                                const SubjDesc*, const uint8_t*, const uint8_t*, const uint8_t*, const int32_t*, int2*,
                                uint32_t, PreHit*, unsigned long long*, uint32_t*, UngHit*, unsigned long long*,
@@ -537,7 +537,8 @@ static uint32_t retry_slots() {
 // per wave, more halves one per lane.  A lane's half costs about 35
 // instructions per band cell, and a lane-run wave lasts as long as the most
 // cells among its 64 halves; a wave-run half costs about 150 per band row
-// (2 scan rounds) whatever the band (up to 64 cells).  With few halves the
+// (2 scan rounds) whatever the band (up to 64 cells; four cells a lane up
+// to 256, about 300 per row).  With few halves the
 // lane-run kernel is one row of waves whose slowest lanes set the time
 // (1 Mbp x 50 families at 4 kbp records, 14 k halves: 27 ms lane-run, 10 ms
 // wave-run); with many, the lanes' throughput wins (the 100 Mbp x 1,000
@@ -547,6 +548,10 @@ constexpr int32_t kTbWaveHalvesPerCu = 112;
 // and ops of one half
 constexpr uint32_t kWaveRows = 8192, kWaveFlat = kWaveRows * 128 + 256, kWaveOps = 4096;
 constexpr uint32_t kWavePerCu = 8;  // workgroups of four waves per CU
+// with bands of up to 256 cells (four per lane): rows of up to 512 bytes, the
+// arena holding 2 MB of them, four workgroups per CU (8.6 GB of arenas)
+constexpr uint32_t kWaveWideRows = 8192, kWaveWideFlat = 2u << 20, kWaveWideOps = 4096;
+constexpr uint32_t kWaveWidePerCu = 4;
 
 // Traceback DPs: one pass (score, traceback bytes, walk) per half in a
 // per-lane arena, longest first (the longest one per wave, the rest one per
@@ -565,10 +570,11 @@ static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::
   }
   std::vector<DpTask> sorted(tasks.size());
   for (size_t k = 0; k < ord.size(); ++k) sorted[k] = tasks[ord[k]];
-  // few halves: every one runs one per wave
-  const int64_t wave_halves =
-      c->tune.tb_wave_halves ? c->tune.tb_wave_halves : (int64_t)kTbWaveHalvesPerCu * c->cu_count;
-  const size_t n_wave_all = wave_halves > 0 && !onepass_wide(c) && ord.size() <= (size_t)wave_halves ? ord.size() : 0;
+  // few halves: every one runs one per wave (RMB_TB_WAVE_HALVES sets the
+  // limit; 0 runs every half one per lane)
+  int64_t wave_halves = c->tune.tb_wave_halves ? c->tune.tb_wave_halves : (int64_t)kTbWaveHalvesPerCu * c->cu_count;
+  if (const char* e = std::getenv("RMB_TB_WAVE_HALVES")) wave_halves = std::atoll(e);
+  const size_t n_wave_all = wave_halves > 0 && ord.size() <= (size_t)wave_halves ? ord.size() : 0;
   unsigned long long* ctr = c->b_counters.get<unsigned long long>(C_NCOUNTERS);
   std::vector<uint32_t> fallback;
   const size_t chunk = 1u << 20;
@@ -590,10 +596,12 @@ static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::
                    ops_cap = wide ? kOnePassWideOps : kOnePassOps;
     const uint64_t stride = onepass_stride(flat_cap, rows_cap, ops_cap);
     uint8_t* arena = n_lane ? c->b_arena.get<uint8_t>((size_t)blocks * 64 * stride) : nullptr;
-    const uint32_t wblocks = std::min<uint32_t>((uint32_t)c->cu_count * kWavePerCu, (n_wave + 3) / 4);
-    uint8_t* warena = n_wave ? c->b_arena_wave.get<uint8_t>((size_t)wblocks * 4 *
-                                                             onepass_stride(kWaveFlat, kWaveRows, kWaveOps))
-                             : nullptr;
+    const uint32_t wrows = wide ? kWaveWideRows : kWaveRows, wflat = wide ? kWaveWideFlat : kWaveFlat,
+                   wops = wide ? kWaveWideOps : kWaveOps;
+    const uint32_t wblocks =
+        std::min<uint32_t>((uint32_t)c->cu_count * (wide ? kWaveWidePerCu : kWavePerCu), (n_wave + 3) / 4);
+    uint8_t* warena =
+        n_wave ? c->b_arena_wave.get<uint8_t>((size_t)wblocks * 4 * onepass_stride(wflat, wrows, wops)) : nullptr;
     uint32_t* pool = c->b_oppool.get<uint32_t>(kOnePassPool);
     // [0] ops used, [1] work queue, [2] retry slots taken (low 32 bits), [3] wave queue
     unsigned long long* pcount = c->b_opcount.get<unsigned long long>(4);
@@ -610,8 +618,8 @@ static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::
     hip_check(hipMemcpyAsync(dt, sorted.data() + base, n * sizeof(DpTask), hipMemcpyHostToDevice, c->st),
               "copy tasks");
     timed_launch(c, 14, [&] {
-      hip_check(launch_dp_wave((int)wblocks, dt, n_wave, c->d_mat, c->P.gap_open, c->P.gap_extend, warena, kWaveRows,
-                               kWaveFlat, kWaveOps, pool, pcount, kOnePassPool, dr, pcount + 3, ctr, c->st),
+      hip_check(launch_dp_wave((int)wblocks, dt, n_wave, c->d_mat, c->P.gap_open, c->P.gap_extend, warena, wrows,
+                               wflat, wops, pool, pcount, kOnePassPool, dr, pcount + 3, ctr, wide, c->st),
                 "k_dp_wave");
       if (n_lane)
         hip_check(launch_dp_onepass((int)blocks, dt + n_wave, n_lane, c->d_mat, c->P.gap_open, c->P.gap_extend, arena,
@@ -661,8 +669,10 @@ static void run_dp_traceback(rmb_ctx* c, const std::vector<DpTask>& tasks, std::
     }
   }
   if (std::getenv("RMB_STAGE_TIMES"))
-    std::fprintf(stderr, "[rmb] traceback DPs ms: one pass %.1f (%zu halves)  two-pass fallback %.1f (%zu halves)\n",
-                 t_one / 1e3, tasks.size(), (tt.us() - t_one) / 1e3, fallback.size());
+    std::fprintf(stderr,
+                 "[rmb] traceback DPs ms: one pass %.1f (%zu halves, %zu wave-run)  two-pass fallback %.1f (%zu "
+                 "halves)\n",
+                 t_one / 1e3, tasks.size(), n_wave_all, (tt.us() - t_one) / 1e3, fallback.size());
 }
 
 struct ItemState {

@@ -3,6 +3,7 @@ rmblastn front end on top of it) against the NCBI golden and the CPU oracle.
 Integer work throughout, so every comparison is byte/bit exact."""
 import ctypes
 import os
+import re
 import shutil
 import subprocess
 
@@ -143,13 +144,21 @@ def test_traceback_two_pass_fallback_matches_oracle(tmp_path):
             "--complexity-adjust", "--dust", "no", "--outfmt", RM_FIELDS]
     o = run([ORACLE_BIN] + args)
     assert o.returncode == 0, o.stderr
-    for slots, wide in (("32", "1"), ("32", "0"), ("0", "0")):
-        g = run([CLI] + args, env={"RMB_TB_RETRY_SLOTS": slots, "RMB_TB_WIDE": wide, "RMB_STAGE_TIMES": "1"})
+    # wave "0": every half one per lane (the one-pass kernels and their retry
+    # slots); "": the few halves here run one per wave (256-cell bands with the
+    # wide rings, 64-cell ones without)
+    for slots, wide, wave in (("32", "1", "0"), ("32", "0", "0"), ("0", "0", "0"), ("32", "1", ""), ("0", "0", "")):
+        env = {"RMB_TB_RETRY_SLOTS": slots, "RMB_TB_WIDE": wide, "RMB_STAGE_TIMES": "1"}
+        if wave:
+            env["RMB_TB_WAVE_HALVES"] = wave
+        g = run([CLI] + args, env=env)
         assert g.returncode == 0, g.stderr
         assert g.stdout == o.stdout and g.stdout.count("\n") > 10
+        tb = [l for l in g.stderr.splitlines() if "two-pass fallback" in l]
+        assert tb, g.stderr[-2000:]
+        assert all((re.search(r"(\d+) wave-run", l).group(1) == "0") == (wave == "0") for l in tb), tb
         if slots == "0":  # the two-pass path ran
-            assert any("two-pass fallback" in l and "(0 halves)" not in l.split("fallback")[1]
-                       for l in g.stderr.splitlines()), g.stderr[-2000:]
+            assert any("(0 halves)" not in l.split("fallback")[1] for l in tb), tb
 
 
 def test_edge_cases_short_and_empty_records(tmp_path):
