@@ -1696,7 +1696,8 @@ __device__ __forceinline__ void dp_half_wave(const DpTask& t, const int* __restr
 // source lane and register are the same for the whole wave.  The base codes
 // of columns [wb, wb + 512) sit in the wave's LDS window `win`, reloaded when
 // the band's start passes wb + 256.  Row table, traceback bytes and results as
-// dp_half_wave.
+// dp_half_wave; without TB, results only (flat and rows unused).
+template <bool TB = true>
 __device__ __forceinline__ void dp_half_wave4(const DpTask& t, const int* __restrict__ mat, int32_t go, int32_t ge,
                                               DpResult& res, uint8_t* __restrict__ flat, uint32_t* __restrict__ rows,
                                               uint32_t flat_cap, uint32_t rows_cap, uint32_t& rounds,
@@ -1731,16 +1732,16 @@ __device__ __forceinline__ void dp_half_wave4(const DpTask& t, const int* __rest
     H[j] = p == 0 ? 0 : -oe - (int32_t)(p - 1) * ge;
     E[j] = H[j] - oe;
     bc[j] = win[p];
-    if (p <= J) flat[p] = p == 0 ? SCRIPT_GAP_IN_B : SCRIPT_GAP_IN_A;
+    if (TB && p <= J) flat[p] = p == 0 ? SCRIPT_GAP_IN_B : SCRIPT_GAP_IN_A;
   }
-  if (lane == 0) { rows[0] = 0; rows[1] = 0; }
+  if (TB && lane == 0) { rows[0] = 0; rows[1] = 0; }
   uint32_t flat_len = J + 1;
   int32_t best = 0;
   uint32_t best_a = 0, best_b = 0;
   uint64_t cells = 0;
   uint32_t bytes_at_best = flat_len, rows_at_best = 1;
   for (uint32_t ai = 0; ai < m; ++ai) {
-    if (ai + 2 > rows_cap || flat_len + 2 * wmask + 4 > flat_cap) { res.flags |= E_TB_SPACE; return; }
+    if (TB && (ai + 2 > rows_cap || flat_len + 2 * wmask + 4 > flat_cap)) { res.flags |= E_TB_SPACE; return; }
     if (ai - ab == 64) {
       ab += 64;
       aw = awn;
@@ -1748,7 +1749,7 @@ __device__ __forceinline__ void dp_half_wave4(const DpTask& t, const int* __rest
     }
     const uint32_t acode = (uint32_t)__builtin_amdgcn_readlane((int)aw, (int)(ai - ab));
     const uint32_t w = b_size - first_b;
-    if (lane == 0) { rows[2 * (ai + 1)] = flat_len; rows[2 * (ai + 1) + 1] = first_b; }
+    if (TB && lane == 0) { rows[2 * (ai + 1)] = flat_len; rows[2 * (ai + 1) + 1] = first_b; }
     cells += w;
     bool act[K];
     int32_t D[K], V[K], G[K];
@@ -1827,7 +1828,7 @@ __device__ __forceinline__ void dp_half_wave4(const DpTask& t, const int* __rest
         if (V[j] - ge >= cs[j] - oe) op |= SCRIPT_EXTEND_GAP_B;
         if (F[j] - ge >= cs[j] - oe) op |= SCRIPT_EXTEND_GAP_A;
       }
-      if (act[j]) flat[flat_len + K * lane + j] = op;
+      if (TB && act[j]) flat[flat_len + K * lane + j] = op;
     }
     uint32_t lead = 0xFFFFFFFFu, last = 0;
 #pragma unroll
@@ -1876,7 +1877,7 @@ __device__ __forceinline__ void dp_half_wave4(const DpTask& t, const int* __rest
       if (ext && (b_size + ext - 1) - nfirst > wmask) { res.flags |= E_BAND_WIDE; return; }
 #pragma unroll
       for (uint32_t j = 0; j < K; ++j)
-        if (K * lane + j < ext) flat[flat_len + K * lane + j] = SCRIPT_GAP_IN_A;
+        if (TB && K * lane + j < ext) flat[flat_len + K * lane + j] = SCRIPT_GAP_IN_A;
       flat_len += ext;
       nsize = b_size + ext;
     }
@@ -2310,6 +2311,53 @@ static constexpr uint32_t kPrelimThreads = 128, kPrelimRingLog2 = 4, kPrelimRun 
 // byte per lane, in 64-lane workgroups of 21 KB of LDS, seven per CU
 static constexpr uint32_t kPrelimWideLog2 = 6, kPrelimPackThreads = 64;
 
+// A preliminary hit's two halves (gapped.rs's preliminary extension from the
+// hit's pivot) and its item's record and subject.
+struct PrelimHalves {
+  uint32_t r, f, ql, sl, pq, ps, tq, ts;
+  DpTask left, right;
+};
+__device__ __forceinline__ PrelimHalves prelim_halves(const PrelimParams& P, const UngHit& u,
+                                                      const RecDesc* __restrict__ recs,
+                                                      const SubjDesc* __restrict__ subjs,
+                                                      const uint8_t* __restrict__ qfwd,
+                                                      const uint8_t* __restrict__ qrc,
+                                                      const uint8_t* __restrict__ splus) {
+  PrelimHalves h;
+  h.r = u.item / P.n_subj;
+  h.f = u.item % P.n_subj;
+  const RecDesc R = recs[h.r];
+  const SubjDesc S = subjs[h.f];
+  h.ql = R.q_len;
+  h.sl = S.s_len;
+  prelim_pivot(u, h.ql, h.sl, (uint32_t)P.lut_width, h.pq, h.ps, h.tq, h.ts);
+  const uint8_t* qa = (u.strand ? qrc : qfwd) + R.q_base + 1;
+  const uint8_t* sa = splus + S.s_base + 1;
+  const int32_t xd = P.xdrop_gap < u.score ? P.xdrop_gap : u.score;
+  h.left = DpTask{sa, qa, h.ps, h.pq, xd, 1};
+  h.right = DpTask{sa + h.ps, qa + h.pq, h.sl > h.ps ? h.sl - h.ps : 0, h.ql > h.pq ? h.ql - h.pq : 0, xd, 0};
+  return h;
+}
+// The hit's preliminary HSP, kept when its score reaches the cutoff.
+__device__ __forceinline__ void prelim_emit(const PrelimParams& P, const UngHit& u, const PrelimHalves& hv,
+                                            const DpResult& L, const DpResult& Rr, PreHit* __restrict__ out,
+                                            unsigned long long* __restrict__ out_count) {
+  const int32_t sc = L.score + Rr.score;
+  if (sc < 0 || sc < P.cutoff) return;
+  const uint32_t q0 = hv.pq - L.b_len, q1 = hv.pq + Rr.b_len, s0 = hv.ps - L.a_len, s1 = hv.ps + Rr.a_len;
+  const uint32_t ql = hv.ql, sl = hv.sl;
+  PreHit h;
+  h.u = u;
+  h.u.item = P.rec_ids[hv.r] * P.n_subj + hv.f;
+  h.score = sc;
+  if (u.strand) { h.q_start = ql - q1; h.q_end = ql - q0; h.s_start = sl - s1; h.s_end = sl - s0; }
+  else { h.q_start = q0; h.q_end = q1; h.s_start = s0; h.s_end = s1; }
+  h.tq = hv.tq;
+  h.ts = hv.ts;
+  const unsigned long long kk = atomicAdd(out_count, 1ull);
+  if (kk < P.out_cap) out[kk] = h;
+}
+
 template <bool LDS_RING, uint32_t RLOG2, bool PACK = false>
 __global__ __launch_bounds__(kPrelimThreads) void k_prelim_fused(
     PrelimParams P, const UngHit* __restrict__ hits, const uint32_t* __restrict__ idx, uint64_t n_tasks,
@@ -2366,17 +2414,9 @@ __global__ __launch_bounds__(kPrelimThreads) void k_prelim_fused(
       u_next = hits[i_next];
     }
     if (u.item == kNoItem) continue;  // reserved slot the seeding kernel left unused
-    const uint32_t r = u.item / P.n_subj, f = u.item % P.n_subj;
-    const RecDesc R = recs[r];
-    const SubjDesc S = subjs[f];
-    const uint32_t ql = R.q_len, sl = S.s_len;
-    uint32_t pq, ps, tq, ts;
-    prelim_pivot(u, ql, sl, (uint32_t)P.lut_width, pq, ps, tq, ts);
-    const uint8_t* qa = (u.strand ? qrc : qfwd) + R.q_base + 1;
-    const uint8_t* sa = splus + S.s_base + 1;
-    const int32_t xd = P.xdrop_gap < u.score ? P.xdrop_gap : u.score;
-    DpTask left{sa, qa, ps, pq, xd, 1};
-    DpTask right{sa + ps, qa + pq, sl > ps ? sl - ps : 0, ql > pq ? ql - pq : 0, xd, 0};
+    const PrelimHalves hv = prelim_halves(P, u, recs, subjs, qfwd, qrc, splus);
+    const uint32_t r = hv.r, f = hv.f;
+    const DpTask &left = hv.left, &right = hv.right;
     DpResult L, Rr;
     if constexpr (PACK) {
       TbRowStart unused;
@@ -2403,23 +2443,65 @@ __global__ __launch_bounds__(kPrelimThreads) void k_prelim_fused(
     }
     ++dps;
     cells += L.cells + Rr.cells;
-    const int32_t sc = L.score + Rr.score;
-    if (sc < 0 || sc < P.cutoff) continue;
-    const uint32_t q0 = pq - L.b_len, q1 = pq + Rr.b_len, s0 = ps - L.a_len, s1 = ps + Rr.a_len;
-    PreHit h;
-    h.u = u;
-    h.u.item = P.rec_ids[r] * P.n_subj + f;
-    h.score = sc;
-    if (u.strand) { h.q_start = ql - q1; h.q_end = ql - q0; h.s_start = sl - s1; h.s_end = sl - s0; }
-    else { h.q_start = q0; h.q_end = q1; h.s_start = s0; h.s_end = s1; }
-    h.tq = tq;
-    h.ts = ts;
-    const unsigned long long kk = atomicAdd(out_count, 1ull);
-    if (kk < P.out_cap) out[kk] = h;
+    prelim_emit(P, u, hv, L, Rr, out, out_count);
    }
   }
   atomicAdd(&counters[C_PRELIM_CELLS], cells);
   atomicAdd(&counters[C_PRELIM_DPS], dps);
+}
+
+// Preliminary DPs of the hits pass 1 handed on (bands past its LDS ring), one
+// hit per wave with dp_half_wave4: when pass 1 handed on at most wave_max
+// hits, so that their longest DPs, not the lanes' throughput, set the time.
+// A hit whose band outgrows 256 cells goes on to the global-ring pass (its
+// index appended to `redo`); with more than wave_max hits the whole list goes
+// on to it (copied to `redo`).
+__global__ __launch_bounds__(kDpWaveThreads) void k_prelim_wave(
+    PrelimParams P, const UngHit* __restrict__ hits, const uint32_t* __restrict__ idx, uint64_t wave_max,
+    const RecDesc* __restrict__ recs, const SubjDesc* __restrict__ subjs, const uint8_t* __restrict__ qfwd,
+    const uint8_t* __restrict__ qrc, const uint8_t* __restrict__ splus, const int32_t* __restrict__ gmat,
+    PreHit* __restrict__ out, unsigned long long* __restrict__ out_count, uint32_t* __restrict__ redo,
+    unsigned long long* __restrict__ redo_count, unsigned long long* __restrict__ counters) {
+  __shared__ int mat[256];
+  __shared__ uint8_t bwin[(kDpWaveThreads / 64) * 512];
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) mat[i] = gmat[i];
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & 63u;
+  uint8_t* win = bwin + (threadIdx.x >> 6) * 512;
+  uint64_t n = P.n_hits;
+  if (P.n_dev) n = *P.n_dev < n ? *P.n_dev : n;
+  if (n > wave_max) {  // too many for one per wave: all of them to the lane pass
+    const uint64_t gt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = gt; k < n && k < P.redo_cap; k += nt) redo[k] = idx[k];
+    if (gt == 0) *redo_count = n;
+    return;
+  }
+  unsigned long long cells = 0, dps = 0;
+  uint32_t rounds = 0;
+  for (;;) {
+    const uint64_t k = wave_take(P.next, 1);
+    if (k >= n) break;
+    const uint32_t i = idx[k];
+    const UngHit u = hits[i];
+    const PrelimHalves hv = prelim_halves(P, u, recs, subjs, qfwd, qrc, splus);
+    DpResult L, Rr;
+    dp_half_wave4<false>(hv.left, mat, P.gap_open, P.gap_extend, L, nullptr, nullptr, 0u, 0u, rounds, win);
+    dp_half_wave4<false>(hv.right, mat, P.gap_open, P.gap_extend, Rr, nullptr, nullptr, 0u, 0u, rounds, win);
+    if (lane == 0) {
+      if ((L.flags | Rr.flags) & E_BAND_WIDE) {
+        const unsigned long long kk = atomicAdd(redo_count, 1ull);
+        if (kk < P.redo_cap) redo[kk] = i;
+      } else {
+        ++dps;
+        cells += L.cells + Rr.cells;
+        prelim_emit(P, u, hv, L, Rr, out, out_count);
+      }
+    }
+  }
+  if (lane == 0) {
+    atomicAdd(&counters[C_PRELIM_CELLS], cells);
+    atomicAdd(&counters[C_PRELIM_DPS], dps);
+  }
 }
 
 // ---------------------------------------------------------- launchers --------
@@ -2441,6 +2523,16 @@ hipError_t launch_prelim_fused(int blocks, bool lds_ring, PrelimParams P, const 
     hipLaunchKernelGGL((k_prelim_fused<false, kPrelimRingLog2>), dim3(blocks), dim3(kPrelimThreads), 0, st, P, hits,
                        idx, n_tasks, recs, subjs, qfwd, qrc, splus, gmat, gring, wlog2, out, out_count, redo,
                        redo_hits, redo_count, counters, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_prelim_wave(int blocks, PrelimParams P, const UngHit* hits, const uint32_t* idx, uint64_t wave_max,
+                              const RecDesc* recs, const SubjDesc* subjs, const uint8_t* qfwd, const uint8_t* qrc,
+                              const uint8_t* splus, const int32_t* gmat, PreHit* out, unsigned long long* out_count,
+                              uint32_t* redo, unsigned long long* redo_count, unsigned long long* counters,
+                              hipStream_t st) {
+  hipLaunchKernelGGL(k_prelim_wave, dim3(blocks), dim3(kDpWaveThreads), 0, st, P, hits, idx, wave_max, recs, subjs,
+                     qfwd, qrc, splus, gmat, out, out_count, redo, redo_count, counters);
   return hipGetLastError();
 }
 

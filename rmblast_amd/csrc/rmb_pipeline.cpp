@@ -67,6 +67,9 @@ hipError_t launch_dp_traceback(int, const TbTask*, uint32_t, const int32_t*, int
 hipError_t launch_dp_onepass(int, const DpTask*, uint32_t, const int32_t*, int32_t, int32_t, uint8_t*, uint32_t,
                              uint32_t, uint32_t, uint32_t*, unsigned long long*, unsigned long long, OnePassResult*,
                              unsigned long long*, OnePassBig, unsigned long long*, bool, hipStream_t);
+hipError_t launch_prelim_wave(int, PrelimParams, const UngHit*, const uint32_t*, uint64_t, const RecDesc*,
+                              const SubjDesc*, const uint8_t*, const uint8_t*, const uint8_t*, const int32_t*, PreHit*,
+                              unsigned long long*, uint32_t*, unsigned long long*, unsigned long long*, hipStream_t);
 hipError_t launch_dp_wave(int, const DpTask*, uint32_t, const int32_t*, int32_t, int32_t, uint8_t*, uint32_t, uint32_t,
                           uint32_t, uint32_t*, unsigned long long*, unsigned long long, OnePassResult*,
                           unsigned long long*, unsigned long long*, bool, hipStream_t);
@@ -258,7 +261,7 @@ struct rmb_ctx {
   DevBuf b_qfwd, b_qrc, b_recs, b_elem, b_keys, b_keys2, b_vals, b_vals2, b_counts, b_cells, b_sort_tmp;
   DevBuf b_hits, b_hitcount, b_spill, b_hitres, b_queue, b_counters, b_err;
   DevBuf b_tasks, b_dpres, b_ring, b_ring_big, b_tbtasks, b_tbres, b_flat, b_rows, b_ops;
-  DevBuf b_pre, b_redo, b_precount, b_packtasks, b_packed, b_items, b_qext, b_recids;
+  DevBuf b_pre, b_redo, b_redo2, b_precount, b_packtasks, b_packed, b_items, b_qext, b_recids;
   DevBuf b_arena, b_arena_big, b_arena_wave, b_oppool, b_opres, b_opcount;  // one-pass traceback
   DevBuf b_seg_base, b_seg_len, b_seg_work;  // reverse-complement segments of a pool
 };
@@ -552,6 +555,15 @@ constexpr uint32_t kWavePerCu = 8;  // workgroups of four waves per CU
 // arena holding 2 MB of them, four workgroups per CU (8.6 GB of arenas)
 constexpr uint32_t kWaveWideRows = 8192, kWaveWideFlat = 2u << 20, kWaveWideOps = 4096;
 constexpr uint32_t kWaveWidePerCu = 4;
+// Preliminary hits past the first pass's LDS ring: up to this many per CU run
+// one per wave (RMB_PRELIM_WAVE_HITS sets the total; 0 sends every one to the
+// global-ring lane pass), in workgroups of four waves, four per CU.
+constexpr uint64_t kPrelimWaveHitsPerCu = 128;
+constexpr uint32_t kPrelimWaveWgPerCu = 4;
+static uint64_t prelim_wave_hits(const rmb_ctx* c) {
+  if (const char* e = std::getenv("RMB_PRELIM_WAVE_HITS")) return (uint64_t)std::max(0LL, std::atoll(e));
+  return kPrelimWaveHitsPerCu * (uint64_t)c->cu_count;
+}
 
 // Traceback DPs: one pass (score, traceback bytes, walk) per half in a
 // per-lane arena, longest first (the longest one per wave, the rest one per
@@ -857,19 +869,21 @@ static void enqueue_batch(rmb_ctx* c, P1Batch& b, const uint8_t* d_qfwd, const u
   if (max_cap > 0xFFFFFFFFull) throw Failure(RMB_ERR_CAPACITY, "more than 2^32 ungapped hits in one group");
   PreHit* dpre = S.pre.get<PreHit>(pre_cap);
   UngHit* dredoh = S.redo_hits.get<UngHit>(b.redo_cap);
-  // [0] prelims, [2] wide hits, [4 + 2g] and [5 + 2g] group g's two
-  // preliminary passes' work queues
-  unsigned long long* cnt = S.cnt.get<unsigned long long>(4 + 2 * ng);
+  // [0] prelims, [2] wide hits, [4 + 2g] and [5 + 2g] group g's first and
+  // global-ring preliminary passes' work queues, [4 + 2ng + 2g] its wave-run
+  // pass's queue and [5 + 2ng + 2g] the hits that pass hands on
+  unsigned long long* cnt = S.cnt.get<unsigned long long>(4 + 4 * ng);
   unsigned long long* hitcnt = S.hitcnt.get<unsigned long long>(2 * ng + 2);  // per group: hits, wide (pass 1)
   unsigned long long* ctr = S.ctr.get<unsigned long long>(C_NCOUNTERS);
   uint32_t* err = S.err.get<uint32_t>(1);
-  hip_check(hipMemsetAsync(cnt, 0, (4 + 2 * ng) * sizeof(unsigned long long), st), "memset counts");
+  hip_check(hipMemsetAsync(cnt, 0, (4 + 4 * ng) * sizeof(unsigned long long), st), "memset counts");
   hip_check(hipMemsetAsync(hitcnt, 0, (2 * ng + 2) * sizeof(unsigned long long), st), "memset hit counts");
   hip_check(hipMemsetAsync(ctr, 0, C_NCOUNTERS * sizeof(unsigned long long), st), "memset counters");
   hip_check(hipMemsetAsync(err, 0, sizeof(uint32_t), st), "memset err");
   ensure_events(S.ev, 6 * ng);
   UngHit* dh = c->b_hits.get<UngHit>(max_cap);
   uint32_t* dredo = c->b_redo.get<uint32_t>(max_cap);
+  uint32_t* dredo2 = c->b_redo2.get<uint32_t>(max_cap);
   const uint32_t nsub = (uint32_t)c->subj.size();
   for (size_t gi = 0; gi < ng; ++gi) {
     P1Group& g = b.groups[gi];
@@ -910,15 +924,27 @@ static void enqueue_batch(rmb_ctx* c, P1Batch& b, const uint8_t* d_qfwd, const u
                                   c->d_mat, nullptr, 0, dpre, cnt, dredo, nullptr, hitcnt + 2 * gi + 1, ctr, err, pwide, st),
               "k_prelim_fused");
     hip_check(hipEventRecord(S.ev[6 * gi + 3], st), "event");
-    // pass 2: the hits whose band outgrew the LDS ring, with per-lane rings of
-    // 2^kRingLog2 cells in global memory; wider still go back as records
+    // pass 2: the hits whose band outgrew the LDS ring.  Few of them run one
+    // per wave (bands of up to 256 cells); the rest, and wider bands, with
+    // per-lane rings of 2^kRingLog2 cells in global memory; wider still go
+    // back as records
+    hip_check(hipEventRecord(S.ev[6 * gi + 4], st), "event");
+    {
+      PrelimParams PW = PP;
+      PW.n_dev = hitcnt + 2 * gi + 1;
+      PW.redo_cap = g.hits_cap;
+      PW.next = cnt + 4 + 2 * ng + 2 * gi;
+      hip_check(launch_prelim_wave((int)c->cu_count * kPrelimWaveWgPerCu, PW, dh, dredo, prelim_wave_hits(c), drecs,
+                                   (const SubjDesc*)c->d_sdesc.p, d_qfwd, d_qrc, (const uint8_t*)c->d_splus.p,
+                                   c->d_mat, dpre, cnt, dredo2, cnt + 5 + 2 * ng + 2 * gi, ctr, st),
+                "k_prelim_wave");
+    }
     const uint64_t lanes = (uint64_t)c->cu_count * 4 * 128;
     int2* ring = c->b_ring.get<int2>((size_t)lanes * ring_stride(kRingLog2));
-    PP.n_dev = hitcnt + 2 * gi + 1;
+    PP.n_dev = cnt + 5 + 2 * ng + 2 * gi;
     PP.redo_cap = b.redo_cap;
     PP.next = cnt + 5 + 2 * gi;
-    hip_check(hipEventRecord(S.ev[6 * gi + 4], st), "event");
-    hip_check(launch_prelim_fused((int)(lanes / 128), false, PP, dh, dredo, g.hits_cap, drecs,
+    hip_check(launch_prelim_fused((int)(lanes / 128), false, PP, dh, dredo2, g.hits_cap, drecs,
                                   (const SubjDesc*)c->d_sdesc.p, d_qfwd, d_qrc, (const uint8_t*)c->d_splus.p,
                                   c->d_mat, ring, kRingLog2, dpre, cnt, nullptr, dredoh, cnt + 2, ctr, err, false, st),
               "k_prelim_fused wide");
@@ -981,11 +1007,16 @@ static void complete_batch(rmb_ctx* c, P1Batch& b, const std::vector<RecDesc>& r
       hip_check(hipEventElapsedTime(&ms, S.ev[6 * gi], S.ev[6 * gi + 1]), "event time");
       c->stats[13] += (uint64_t)((double)ms * 1e6);
       c->stats[15] += 1;
+      float pms[2] = {0, 0};
       for (int p = 0; p < 2; ++p) {
         hip_check(hipEventElapsedTime(&ms, S.ev[6 * gi + 2 + 2 * p], S.ev[6 * gi + 3 + 2 * p]), "event time");
         c->stats[14] += (uint64_t)((double)ms * 1e6);
         c->stats[23] += (uint64_t)((double)ms * 1e6);
+        pms[p] = ms;
       }
+      if (std::getenv("RMB_STAGE_TIMES"))
+        std::fprintf(stderr, "[rmb] preliminary DPs ms: %.1f (%llu hits), wider bands %.1f (%llu hits)\n", pms[0],
+                     hc[2 * gi], pms[1], hc[2 * gi + 1]);
     }
     const size_t base = out.size();
     out.resize(base + cnt[0]);
@@ -1868,7 +1899,7 @@ int rmb_ctx_destroy(rmb_ctx* c) {
                     &c->b_keys, &c->b_keys2, &c->b_vals, &c->b_vals2, &c->b_counts, &c->b_cells, &c->b_sort_tmp,
                     &c->b_hits, &c->b_hitcount, &c->b_spill, &c->b_hitres, &c->b_queue, &c->b_counters, &c->b_err, &c->b_tasks,
                     &c->b_dpres, &c->b_ring, &c->b_ring_big, &c->b_tbtasks, &c->b_tbres, &c->b_flat, &c->b_rows,
-                    &c->b_ops, &c->b_pre, &c->b_redo, &c->b_precount, &c->b_packtasks, &c->b_packed, &c->b_items, &c->b_qext,
+                    &c->b_ops, &c->b_pre, &c->b_redo, &c->b_redo2, &c->b_precount, &c->b_packtasks, &c->b_packed, &c->b_items, &c->b_qext,
                     &c->b_recids, &c->b_arena, &c->b_arena_big, &c->b_arena_wave, &c->b_oppool, &c->b_opres, &c->b_opcount})
     b->release();
   c->h_qfwd.release();

@@ -112,11 +112,20 @@ def test_gappy_divergent_and_microsatellites_match_oracle(tmp_path):
     synth.to_fasta([("q", g0)], tmp_path / "q.fa")
     synth.to_fasta(fams, tmp_path / "db.fa")
     (tmp_path / "m.matrix").write_text(synth.MATRIX_TEXT)
-    g, o = both(["--db", str(tmp_path / "db.fa"), "--query", str(tmp_path / "q.fa"), "--matrix",
-                 str(tmp_path / "m.matrix"), "--word-size", "7", "--gapopen", "20", "--gapextend", "5",
-                 "--min-raw-gapped-score", "200", "--xdrop-gap", "120", "--xdrop-gap-final", "400",
-                 "--complexity-adjust", "--dust", "no", "--outfmt", RM_FIELDS])
+    args = ["--db", str(tmp_path / "db.fa"), "--query", str(tmp_path / "q.fa"), "--matrix",
+            str(tmp_path / "m.matrix"), "--word-size", "7", "--gapopen", "20", "--gapextend", "5",
+            "--min-raw-gapped-score", "200", "--xdrop-gap", "120", "--xdrop-gap-final", "400",
+            "--complexity-adjust", "--dust", "no", "--outfmt", RM_FIELDS]
+    g, o = both(args)
     assert g == o and g.count("\n") > 20
+    # the preliminary hits past the LDS ring: one per wave by default, here
+    # every one on the global-ring lane pass instead
+    for env in ({"RMB_PRELIM_WAVE_HITS": "0", "RMB_STAGE_TIMES": "1"}, {"RMB_STAGE_TIMES": "1"}):
+        r = run([CLI] + args, env=env)
+        assert r.returncode == 0, r.stderr
+        assert r.stdout == o
+        wide = [l for l in r.stderr.splitlines() if "wider bands" in l]
+        assert wide and any(" (0 hits)" not in l.split("wider bands")[1] for l in wide), wide
 
 
 def test_traceback_two_pass_fallback_matches_oracle(tmp_path):
