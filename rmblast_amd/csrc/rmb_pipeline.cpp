@@ -818,9 +818,13 @@ static void launch_group_phase1(rmb_ctx* c, hipStream_t st, P1Group& g, const ui
   // replayed through the diagonal hash by one workgroup: its items are then
   // split over up to 16 workgroups, each owning an equal range of the 512
   // hash buckets (the buckets are independent, so the split is exact), as
-  // far as the resident workgroups allow.
+  // far as the resident workgroups allow; items fewer than four times the
+  // resident workgroups are split in two, so the last round of items is not
+  // a few workgroups' (one 10 Mbp record as ten chunks x 300 families, 3,000
+  // items: 3% off the step; four or eight parts cost more than they balance).
   const uint64_t slots = (uint64_t)c->cu_count * kP1PerCu;
   while (!seeds && P.part_log2 < 4 && (P.n_items << (P.part_log2 + 1)) <= slots) ++P.part_log2;
+  if (!seeds && P.part_log2 == 0 && P.n_items < 4 * slots) P.part_log2 = 1;
   if (const char* e = std::getenv("RMB_P1_PARTS_LOG2")) P.part_log2 = seeds ? 0u : (uint32_t)std::min(4, std::max(0, std::atoi(e)));
   // ~30 KB of LDS per workgroup: kP1PerCu resident per CU.  Workgroups are
   // dealt round-robin over the XCDs, so blockIdx % kXcds names the XCD.
