@@ -137,8 +137,9 @@ struct OnePassBig {
   uint32_t ring_log2, slots, flat_cap, rows_cap, ops_cap;
 };
 
-// Per-lane arena of the one-pass traceback: row table (8 B per row), then the
-// traceback bytes (16-byte aligned), then the ops (4 B each).
+// Per-lane arena of the one-pass traceback: row table (8 B per row; none for
+// the traceback stream with row trailers), then the traceback bytes (16-byte
+// aligned), then the ops (4 B each).
 inline constexpr uint64_t onepass_stride(uint32_t flat_cap, uint32_t rows_cap, uint32_t ops_cap) {
   return (uint64_t)rows_cap * 8 + ((flat_cap + 15u) & ~15u) + (((uint64_t)ops_cap * 4 + 15u) & ~15ull);
 }

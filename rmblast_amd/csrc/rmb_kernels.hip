@@ -2084,15 +2084,18 @@ __global__ __launch_bounds__(LDS_RING ? kDpLdsThreads : 256) void k_dp_traceback
         ops[y] = tmp;
       }
       o.n_ops = nops;
-    }    out[i] = o;
+    }
+    out[i] = o;
   }
 }
 
-// Score and traceback in one pass: each lane keeps its half alignment's row
-// table, traceback bytes (for every row: the best row is known only at the
-// end) and ops in its own arena, walks back from the best cell, and appends
-// the ops to one packed pool.  A half whose band, rows, bytes or ops outgrow
-// the LDS ring or the arena comes back flagged for the two-pass path.
+// Score and traceback in one pass: each lane keeps its half alignment's
+// traceback stream (every row's bytes and trailer: the best row is known only
+// at the end) and ops in its own arena (the launch passes rows_cap 0: no row
+// table), walks back from the best cell, and appends the ops to one packed
+// pool.  A half whose band, bytes or ops outgrow the LDS ring or the arena is
+// rerun in a retry slot (dp_half, with a row table), else comes back flagged
+// for the two-pass path.
 template <bool LDS_RING, uint32_t RLOG2>
 __global__ __launch_bounds__(kDpLdsThreads) void k_dp_onepass(
     const DpTask* __restrict__ tasks, uint32_t n_tasks, const int32_t* __restrict__ gmat, int32_t go, int32_t ge,
