@@ -1148,6 +1148,26 @@ __device__ __forceinline__ void dp_half(const DpTask& t, const int* mat, int32_t
       }
       int32_t sgc = c.y;
       const int32_t next = c.x + sub;
+      if constexpr (!TB) {
+        // score only: both outcomes computed and selected, no branch per cell.
+        // A leading pruned cell leaves the band, so rewriting its slot (the
+        // column's next use writes it first) changes nothing.
+        const int32_t cs = max(sc, max(sgc, sgr));
+        const bool prune = best - cs > xdrop;
+        const int32_t open = cs - oe;
+        ring[(bi & wmask) * RS] = prune ? make_int2(kMinInt, sgc) : make_int2(cs, max(sgc - ge, open));
+        first_b += (prune && bi == first_b) ? 1u : 0u;
+        last_b = prune ? last_b : bi;
+        const bool nb = cs > best;  // never a pruned cell
+        best_a = nb ? ai + 1 : best_a;
+        best_b = nb ? bi : best_b;
+        best_here = best_here || nb;
+        best = nb ? cs : best;
+        sgr = prune ? sgr : max(sgr - ge, open);
+        sc = next;
+        ++flat_len;
+        continue;
+      }
       uint8_t op = SCRIPT_SUB;
       int32_t cs = sc;
       if (sgc > cs) { cs = sgc; op = SCRIPT_GAP_IN_B; }
@@ -1374,6 +1394,24 @@ __device__ __forceinline__ void dp_half_tb16(const DpTask& t, const int* mat, in
       int32_t sgc = dec_e(cv) + pbase;
       const uint32_t code_bits = C12 ? cv & 0x0F000000u : 0u;  // kept with the column's cell
       const int32_t next = cx + sub;
+      if constexpr (!TB) {
+        // score only: both outcomes computed and selected, as in dp_half
+        const int32_t cs = max(sc, max(sgc, sgr));
+        const bool prune = best - cs > xdrop;
+        const int32_t open = cs - oe;
+        ring[(bi & wmask) * RS] =
+            (prune ? enc(kNegH, sgc - cbase) : enc(cs - cbase, max(sgc - ge, open) - cbase)) | code_bits;
+        first_b += (prune && bi == first_b) ? 1u : 0u;
+        last_b = prune ? last_b : bi;
+        const bool nb = cs > best;  // never a pruned cell
+        best_a = nb ? ai + 1 : best_a;
+        best_b = nb ? bi : best_b;
+        best_here = best_here || nb;
+        best = nb ? cs : best;
+        sgr = prune ? sgr : max(sgr - ge, open);
+        sc = next;
+        continue;
+      }
       uint8_t op = SCRIPT_SUB;
       int32_t cs = sc;
       if (sgc > cs) { cs = sgc; op = SCRIPT_GAP_IN_B; }
