@@ -845,12 +845,24 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
             const uint4 e = sd[i];
             const int32_t d = (int32_t)e.x;
             const uint32_t sc = e.y;
-            // get(): newest cell first
+            // get(): newest cell first.  One pass over the spill cells, four
+            // loads at a time, finds both the level (the newest cell on d) and
+            // insert()'s target (the newest cell on d or stale): nothing is
+            // written between the two, so they are the scans' answers.
             uint32_t lev = 0;
             bool found = false;
-            for (int32_t k = (int32_t)cnt_b - 1; k >= kHashLds; --k) {
-              const uint2 gv = GS[k - kHashLds];
-              if ((int32_t)gv.x == d) { lev = gv.y; found = true; break; }
+            int32_t ins = -1;
+            for (int32_t k = (int32_t)cnt_b - 1; k >= kHashLds && !found; k -= 4) {
+              uint2 gq[4];
+#pragma unroll
+              for (int q = 0; q < 4; ++q) gq[q] = GS[max(k - q, kHashLds) - kHashLds];
+#pragma unroll
+              for (int q = 0; q < 4; ++q) {
+                if (k - q < kHashLds || found) continue;
+                const bool same = (int32_t)gq[q].x == d;
+                if (ins < 0 && (same || (int32_t)(sc - gq[q].y) > window)) ins = k - q;
+                if (same) { lev = gq[q].y; found = true; }
+              }
             }
             if (!found && cnt_b > 2 && d2 == d) { lev = l2; found = true; }
             if (!found && cnt_b > 1 && d1 == d) { lev = l1; found = true; }
@@ -862,15 +874,13 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
               sd[i].w = e.w | ((uint32_t)F_EMIT << 16);
               ++nemit_local;
             }
-            // insert(): same diagonal, else first stale cell, else a new head cell
+            // insert(): same diagonal, else first stale cell, else a new head
+            // cell; among the spill cells the scan above found the target
             const uint32_t L = e.z;
             bool done = false;
-            for (int32_t k = (int32_t)cnt_b - 1; k >= kHashLds && !done; --k) {
-              const uint2 gv = GS[k - kHashLds];
-              if ((int32_t)gv.x == d || (int32_t)(sc - gv.y) > window) {
-                GS[k - kHashLds] = make_uint2((uint32_t)d, L);
-                done = true;
-              }
+            if (ins >= 0) {
+              GS[ins - kHashLds] = make_uint2((uint32_t)d, L);
+              done = true;
             }
             if (!done && cnt_b > 2 && (d2 == d || (int32_t)(sc - l2) > window)) { d2 = d; l2 = L; done = true; }
             if (!done && cnt_b > 1 && (d1 == d || (int32_t)(sc - l1) > window)) { d1 = d; l1 = L; done = true; }
