@@ -845,19 +845,20 @@ __global__ __launch_bounds__(kP1Threads, kP1PerCu) void k_phase1(
             const uint4 e = sd[i];
             const int32_t d = (int32_t)e.x;
             const uint32_t sc = e.y;
-            // get(): newest cell first.  One pass over the spill cells, four
-            // loads at a time, finds both the level (the newest cell on d) and
-            // insert()'s target (the newest cell on d or stale): nothing is
-            // written between the two, so they are the scans' answers.
+            // get(): newest cell first.  One pass over the spill cells, two
+            // loads at a time (four cost the kernel register spills), finds
+            // both the level (the newest cell on d) and insert()'s target (the
+            // newest cell on d or stale): nothing is written between the two,
+            // so they are the scans' answers.
             uint32_t lev = 0;
             bool found = false;
             int32_t ins = -1;
-            for (int32_t k = (int32_t)cnt_b - 1; k >= kHashLds && !found; k -= 4) {
-              uint2 gq[4];
+            for (int32_t k = (int32_t)cnt_b - 1; k >= kHashLds && !found; k -= 2) {
+              uint2 gq[2];
 #pragma unroll
-              for (int q = 0; q < 4; ++q) gq[q] = GS[max(k - q, kHashLds) - kHashLds];
+              for (int q = 0; q < 2; ++q) gq[q] = GS[max(k - q, kHashLds) - kHashLds];
 #pragma unroll
-              for (int q = 0; q < 4; ++q) {
+              for (int q = 0; q < 2; ++q) {
                 if (k - q < kHashLds || found) continue;
                 const bool same = (int32_t)gq[q].x == d;
                 if (ins < 0 && (same || (int32_t)(sc - gq[q].y) > window)) ins = k - q;
