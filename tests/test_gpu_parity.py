@@ -170,12 +170,14 @@ def test_traceback_two_pass_fallback_matches_oracle(tmp_path):
             assert any("(0 halves)" not in l.split("fallback")[1] for l in tb), tb
 
 
-def test_very_large_xdrops_take_the_wide_paths_with_the_oracles_rows(tmp_path):
-    """X-drop 300/2,500: past the 12-bit ring cells' range (X-drop + open +
-    largest score < 2,000) and past 256-cell bands, so the traceback halves
-    go from the one-pass and wave-run kernels to the retry slots or the
-    two-pass path, and the preliminary hits past the first pass's ring to
-    the later passes; the rows stay the oracle's."""
+@pytest.mark.parametrize("xg,xf", [("300", "2500"), ("1200", "1500")])
+def test_very_large_xdrops_take_the_wide_paths_with_the_oracles_rows(tmp_path, xg, xf):
+    """X-drops past the 12-bit ring cells' range (X-drop + open + largest
+    score < 2,000) or past 256-cell bands: the traceback halves go from the
+    one-pass and wave-run kernels to the retry slots or the two-pass path,
+    and the preliminary hits past the first pass's ring to the wave-run pass
+    and (bands past 256 cells, X-drop 1,200) on to the global-ring pass; the
+    rows stay the oracle's."""
     rng = np.random.default_rng(13)
     fams = synth.make_families(rng, 3, [2500, 3000, 2000])
     g0, _ = synth.insert_copies(rng, 40_000, fams, 16, (400, 2000), (0.10, 0.30), (0.02, 0.06))
@@ -184,7 +186,7 @@ def test_very_large_xdrops_take_the_wide_paths_with_the_oracles_rows(tmp_path):
     (tmp_path / "m.matrix").write_text(synth.MATRIX_TEXT)
     args = ["--db", str(tmp_path / "db.fa"), "--query", str(tmp_path / "q.fa"), "--matrix",
             str(tmp_path / "m.matrix"), "--word-size", "7", "--gapopen", "20", "--gapextend", "5",
-            "--min-raw-gapped-score", "200", "--xdrop-gap", "300", "--xdrop-gap-final", "2500",
+            "--min-raw-gapped-score", "200", "--xdrop-gap", xg, "--xdrop-gap-final", xf,
             "--complexity-adjust", "--dust", "no", "--outfmt", RM_FIELDS]
     g, o = both(args)
     assert g == o and g.count("\n") > 5
